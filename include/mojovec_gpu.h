@@ -1,0 +1,181 @@
+/*
+ * mojovec_gpu.h — C ABI of the MI355X (gfx950) batched vector-scan library
+ * (libmojovec_gpu.so).
+ *
+ * The entry points replace, on one GPU, the native calls that MojoVec's host
+ * code makes for exact Flat / SQ8 search, IVF coarse assignment, PQ
+ * LUT + inverted-list scan and the Lloyd k-means that trains IVF centroids and
+ * PQ codebooks. Each declaration cites the reference interface it replaces
+ * (paths relative to the MojoVec v0.8.2 source tree).
+ *
+ * Conventions
+ *   - Plain pointers and sizes only. Host pointers are borrowed for the call;
+ *     the library owns all device memory of an index handle.
+ *   - Vectors are row-major float32 [n x d]; results are row-major [nq x k],
+ *     ascending distance, exact ties broken by the lower id.
+ *   - Metrics: 0 = L2 (squared), 1 = inner product (core/types.mojo:5-7);
+ *     collections also accept "cosine".
+ *   - Every function returning int returns MVG_OK (0) or an MVG_ERR_* code;
+ *     mvg_last_error() then holds the message (thread-local).
+ *   - A handle may be searched from several threads; mutations need external
+ *     exclusion, as in the reference (README "Thread safety").
+ */
+#ifndef MOJOVEC_GPU_H
+#define MOJOVEC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MVG_OK 0
+#define MVG_ERR_INVALID 1 /* bad argument (the reference raises Error) */
+#define MVG_ERR_DEVICE 2  /* HIP runtime failure */
+#define MVG_ERR_ALLOC 3   /* device or host allocation failure */
+#define MVG_ERR_STATE 4   /* call not valid in the handle's state */
+
+#define MVG_METRIC_L2 0
+#define MVG_METRIC_IP 1
+
+typedef struct mvg_flat mvg_flat;
+typedef struct mvg_sq8 mvg_sq8;
+typedef struct mvg_ivfpq mvg_ivfpq;
+typedef struct mvg_collection mvg_collection;
+
+/* ---- library ---------------------------------------------------------- */
+const char *mvg_last_error(void);
+const char *mvg_version(void);
+/* Name and compute-unit count of the current HIP device. */
+int mvg_device_info(char *name, int name_len, int *compute_units);
+
+/* ---- exact Flat: IndexFlat (mojovec/index/index_flat.mojo:117-330) ---- */
+/* IndexFlat.__init__ (:131-151) */
+int mvg_flat_create(int d, int metric, mvg_flat **out);
+/* IndexFlat.add (:185-210) */
+int mvg_flat_add(mvg_flat *h, const float *x, int64_t n);
+/* IndexFlat.search (:229-258, _search_impl :260-325). exclude[j] > 0 skips
+ * internal id j (the `filter` span of core/index.mojo:21-31); may be NULL.
+ * Slots past the number of candidates hold (0.0, -1). */
+int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_t *exclude,
+                    int64_t exclude_len, float *distances, int64_t *labels);
+int64_t mvg_flat_ntotal(const mvg_flat *h);
+void mvg_flat_destroy(mvg_flat *h);
+
+/* ---- SQ8 Flat: IndexFlatSQ8 (mojovec/index/index_flat_sq8.mojo:192-582) */
+/* IndexFlatSQ8.__init__ (:209-237) */
+int mvg_sq8_create(int d, int metric, mvg_sq8 **out);
+/* IndexFlatSQ8.add (:286-397): global min/max calibration, re-encode on
+ * range growth, FP32 rows kept for the rerank. */
+int mvg_sq8_add(mvg_sq8 *h, const float *x, int64_t n);
+/* IndexFlatSQ8.search (:399-526). k_factor > 1 reranks the k*k_factor code
+ * candidates with exact FP32 distances, as the SQ8 rerank of
+ * index_hnsw.mojo:433-496 (fixed at 2 there). Pad (0.0, -1). */
+int mvg_sq8_search(mvg_sq8 *h, const float *q, int64_t nq, int k, int k_factor,
+                   const uint8_t *exclude, int64_t exclude_len, float *distances,
+                   int64_t *labels);
+/* Calibration state: global_min, global_max, scale (:203-206). */
+int mvg_sq8_params(const mvg_sq8 *h, float *gmin, float *gmax, float *scale, int64_t *ntotal);
+/* Codes [ntotal x d] and code norms [ntotal] (codes_u8 / norms_u32). */
+int mvg_sq8_get_codes(mvg_sq8 *h, uint8_t *codes, uint32_t *norms);
+void mvg_sq8_destroy(mvg_sq8 *h);
+
+/* ---- k-means: KMeans (mojovec/clustering/kmeans.mojo:8-220) ------------- */
+/* KMeans(d, k, niter).train(x). init == NULL selects the seeded pick of
+ * kmeans.mojo:67-82 (seed 0x4D4F4A4F564543 there); otherwise init is
+ * [k x d] and sample_offset drives the empty-cluster reseed (:211-220).
+ * assign (nullable) receives the last iteration's assignment. */
+int mvg_kmeans_train(const float *x, int64_t n, int d, int k, int niter, const float *init,
+                     int64_t sample_offset, uint64_t seed, float *centroids, int32_t *assign);
+
+/* ---- PQ: ProductQuantizer (mojovec/quantization/pq.mojo:80-292) -------- */
+/* ProductQuantizer.train (:125-176): codebooks [M x 256 x d/M]. */
+int mvg_pq_train(const float *x, int64_t n, int d, int M, int niter, uint64_t seed,
+                 float *codebooks);
+/* ProductQuantizer.compute_codes (:178-219): codes [n x M]. */
+int mvg_pq_encode(const float *x, int64_t n, int d, int M, const float *codebooks,
+                  uint8_t *codes);
+/* ProductQuantizer.compute_distance_table (:252-292) for nq vectors:
+ * lut [nq x M x 256]. */
+int mvg_pq_lut(const float *q, int64_t nq, int d, int M, const float *codebooks, int metric,
+               float *lut);
+
+/* ---- IVF-PQ: IndexIVFPQ (mojovec/index/index_ivf_pq.mojo:12-360) ------- */
+/* IndexIVFPQ.__init__ (:35-71); nprobe starts at 1 (:63). */
+int mvg_ivfpq_create(int d, int nlist, int M, int metric, mvg_ivfpq **out);
+/* IndexIVFPQ.train (:87-136). The reference fixes both k-means iteration
+ * counts at 15 (:98, pq.mojo:64); they are explicit here. No-op once trained. */
+int mvg_ivfpq_train(mvg_ivfpq *h, const float *x, int64_t n, int niter_coarse, int niter_pq,
+                    uint64_t seed);
+/* Installs trained codebooks (coarse [nlist x d], pq [M x 256 x d/M]) and
+ * marks the index trained: the state IndexIVFPQ.train leaves behind. */
+int mvg_ivfpq_set_codebooks(mvg_ivfpq *h, const float *coarse, const float *pq);
+int mvg_ivfpq_get_codebooks(mvg_ivfpq *h, float *coarse, float *pq);
+/* IndexIVFPQ.add (:138-148): internal ids ntotal..ntotal+n-1. Silently does
+ * nothing while untrained (:161). FP32 rows are kept on the device for the
+ * k_factor rerank unless disabled with mvg_ivfpq_set_keep_vectors(h, 0)
+ * before the first add. */
+int mvg_ivfpq_add(mvg_ivfpq *h, const float *x, int64_t n);
+int mvg_ivfpq_set_keep_vectors(mvg_ivfpq *h, int keep);
+/* The public `nprobe` field (:22). Values above nlist are clamped at search
+ * time (:253-254). */
+int mvg_ivfpq_set_nprobe(mvg_ivfpq *h, int nprobe);
+/* IndexIVFPQ.search (:215-360). nprobe_per_query (nullable, [nq]) overrides
+ * nprobe per query; k_factor > 1 reranks k*k_factor ADC candidates with exact
+ * FP32 distances; exclude[id] != 0 skips internal id (:328-334). Pad
+ * (1e38, -1); inner-product results are un-negated (:357-360). */
+int mvg_ivfpq_search(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                     const int32_t *nprobe_per_query, int k_factor, const uint8_t *exclude,
+                     int64_t exclude_len, float *distances, int64_t *labels);
+/* Same search with every pointer in device memory, enqueued on `stream`
+ * (a hipStream_t; NULL = the handle's stream) without synchronising. */
+int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                            const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                            const uint8_t *exclude, int64_t exclude_len, float *distances,
+                            int64_t *labels, void *stream);
+/* Stage timing of the next searches (no reference counterpart; used by the
+ * benchmark): when on, each search records HIP events on its stream around
+ * the coarse, scan and merge/rerank stages and counts the list entries the
+ * scan visits. read_profile returns the last search's stage_ms[3] and that
+ * count. */
+int mvg_ivfpq_set_profiling(mvg_ivfpq *h, int on);
+int mvg_ivfpq_read_profile(mvg_ivfpq *h, float *stage_ms, int64_t *scanned_entries);
+/* State: ntotal, trained flag, nprobe. */
+int mvg_ivfpq_info(mvg_ivfpq *h, int64_t *ntotal, int *trained, int *nprobe);
+/* Inverted-list sizes [nlist] and one list's ids/codes in append order
+ * (ArrayInvertedLists.list_size/get_ids/get_codes, inverted_lists.mojo). */
+int mvg_ivfpq_list_sizes(mvg_ivfpq *h, int64_t *sizes);
+int mvg_ivfpq_get_list(mvg_ivfpq *h, int list, int64_t *ids, uint8_t *codes);
+void mvg_ivfpq_destroy(mvg_ivfpq *h);
+
+/* ---- managed collection: CollectionIVFPQ ------------------------------- */
+/* (mojovec/api/collection_ivfpq.mojo:52-333). Messages match the reference.
+ * CollectionIVFPQ.__init__ (:68-100); nprobe 0 selects min(10, nlist). */
+int mvg_coll_create(int d, int nlist, int M, int nprobe, const char *name, const char *metric,
+                    mvg_collection **out);
+/* CollectionIVFPQ.train (:169-195); `count` values (a multiple of d). */
+int mvg_coll_train(mvg_collection *h, const float *x, int64_t count);
+/* CollectionIVFPQ.add (:206-242): unique user ids, auto-train on the first
+ * batch. */
+int mvg_coll_add(mvg_collection *h, const int64_t *ids, int64_t n, const float *x, int64_t count);
+/* CollectionIVFPQ.query (:244-333) plus k_factor (1 = reference behaviour):
+ * ids/distances [nq x n_results]; cosine and ip distances are 1 - sim.
+ * *nq_out receives count / d. */
+int mvg_coll_query(mvg_collection *h, const float *q, int64_t count, int n_results, int k_factor,
+                   int64_t *ids, float *distances, int64_t *nq_out);
+/* set_nprobe (:152-155). */
+int mvg_coll_set_nprobe(mvg_collection *h, int nprobe);
+/* k-means iteration counts used by train (15 / 15 in the reference). */
+int mvg_coll_set_train_params(mvg_collection *h, int niter_coarse, int niter_pq, uint64_t seed);
+/* name(), dimension(), metric(), count(), nlist(), pq_subvectors(),
+ * nprobe(), is_trained() (:124-150). metric: 0 l2, 1 ip, 2 cosine. */
+int mvg_coll_info(mvg_collection *h, int *dimension, int *metric, int64_t *count, int *nlist,
+                  int *M, int *nprobe, int *trained);
+int mvg_coll_name(mvg_collection *h, char *buf, int buf_len);
+mvg_ivfpq *mvg_coll_index(mvg_collection *h);
+void mvg_coll_destroy(mvg_collection *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

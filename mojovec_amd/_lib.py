@@ -1,0 +1,111 @@
+"""ctypes binding of libmojovec_gpu.so (declarations in include/mojovec_gpu.h).
+
+The library is the only compute path: importing it fails loudly when the
+shared object has not been built, and every call that reaches the GPU raises
+when no HIP device is present. There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmojovec_gpu.so")
+
+MVG_OK = 0
+MVG_ERR_INVALID = 1
+MVG_ERR_DEVICE = 2
+MVG_ERR_ALLOC = 3
+MVG_ERR_STATE = 4
+
+
+class MojoVecError(Exception):
+    """Raised with the library's message when a native call fails."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
+_f = ctypes.c_float
+
+# name -> (restype, argtypes); pointers travel as c_void_p.
+SIGNATURES = {
+    "mvg_last_error": (ctypes.c_char_p, []),
+    "mvg_version": (ctypes.c_char_p, []),
+    "mvg_device_info": (_i, [ctypes.c_char_p, _i, ctypes.POINTER(_i)]),
+    "mvg_flat_create": (_i, [_i, _i, ctypes.POINTER(_p)]),
+    "mvg_flat_add": (_i, [_p, _p, _i64]),
+    "mvg_flat_search": (_i, [_p, _p, _i64, _i, _p, _i64, _p, _p]),
+    "mvg_flat_ntotal": (_i64, [_p]),
+    "mvg_flat_destroy": (None, [_p]),
+    "mvg_sq8_create": (_i, [_i, _i, ctypes.POINTER(_p)]),
+    "mvg_sq8_add": (_i, [_p, _p, _i64]),
+    "mvg_sq8_search": (_i, [_p, _p, _i64, _i, _i, _p, _i64, _p, _p]),
+    "mvg_sq8_params": (_i, [_p, ctypes.POINTER(_f), ctypes.POINTER(_f), ctypes.POINTER(_f),
+                            ctypes.POINTER(_i64)]),
+    "mvg_sq8_get_codes": (_i, [_p, _p, _p]),
+    "mvg_sq8_destroy": (None, [_p]),
+    "mvg_kmeans_train": (_i, [_p, _i64, _i, _i, _i, _p, _i64, _u64, _p, _p]),
+    "mvg_pq_train": (_i, [_p, _i64, _i, _i, _i, _u64, _p]),
+    "mvg_pq_encode": (_i, [_p, _i64, _i, _i, _p, _p]),
+    "mvg_pq_lut": (_i, [_p, _i64, _i, _i, _p, _i, _p]),
+    "mvg_ivfpq_create": (_i, [_i, _i, _i, _i, ctypes.POINTER(_p)]),
+    "mvg_ivfpq_train": (_i, [_p, _p, _i64, _i, _i, _u64]),
+    "mvg_ivfpq_set_codebooks": (_i, [_p, _p, _p]),
+    "mvg_ivfpq_get_codebooks": (_i, [_p, _p, _p]),
+    "mvg_ivfpq_add": (_i, [_p, _p, _i64]),
+    "mvg_ivfpq_set_keep_vectors": (_i, [_p, _i]),
+    "mvg_ivfpq_set_nprobe": (_i, [_p, _i]),
+    "mvg_ivfpq_search": (_i, [_p, _p, _i64, _i, _p, _i, _p, _i64, _p, _p]),
+    "mvg_ivfpq_search_device": (_i, [_p, _p, _i64, _i, _p, _i, _i, _p, _i64, _p, _p, _p]),
+    "mvg_ivfpq_set_profiling": (_i, [_p, _i]),
+    "mvg_ivfpq_read_profile": (_i, [_p, _p, ctypes.POINTER(_i64)]),
+    "mvg_ivfpq_info": (_i, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "mvg_ivfpq_list_sizes": (_i, [_p, _p]),
+    "mvg_ivfpq_get_list": (_i, [_p, _i, _p, _p]),
+    "mvg_ivfpq_destroy": (None, [_p]),
+    "mvg_coll_create": (_i, [_i, _i, _i, _i, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_p)]),
+    "mvg_coll_train": (_i, [_p, _p, _i64]),
+    "mvg_coll_add": (_i, [_p, _p, _i64, _p, _i64]),
+    "mvg_coll_query": (_i, [_p, _p, _i64, _i, _i, _p, _p, ctypes.POINTER(_i64)]),
+    "mvg_coll_set_nprobe": (_i, [_p, _i]),
+    "mvg_coll_set_train_params": (_i, [_p, _i, _i, _u64]),
+    "mvg_coll_info": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i64),
+                           ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                           ctypes.POINTER(_i)]),
+    "mvg_coll_name": (_i, [_p, ctypes.c_char_p, _i]),
+    "mvg_coll_index": (_p, [_p]),
+    "mvg_coll_destroy": (None, [_p]),
+}
+
+_lib = None
+
+
+def load():
+    """Returns the loaded library; raises ImportError if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m mojovec_amd.build`"
+            )
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != MVG_OK:
+        msg = load().mvg_last_error().decode("utf-8", "replace")
+        if rc == MVG_ERR_INVALID:
+            raise MojoVecError(rc, msg)
+        raise MojoVecError(rc, msg)

@@ -1,0 +1,685 @@
+// C ABI (include/mojovec_gpu.h): handles, validation, host<->device staging.
+#include "../../include/mojovec_gpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+
+#include "kernels.h"
+
+using mvg::DevBuf;
+using mvg::Error;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int record(int code, const char *msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define API_BEGIN try {
+#define API_END                                                 \
+  }                                                             \
+  catch (const mvg::Error &e) {                                 \
+    return record(e.code, e.what());                            \
+  }                                                             \
+  catch (const std::bad_alloc &) {                              \
+    return record(mvg::kAlloc, "host allocation failed");       \
+  }                                                             \
+  catch (const std::exception &e) {                             \
+    return record(mvg::kDevice, e.what());                      \
+  }                                                             \
+  return MVG_OK;
+
+void require(bool ok, const char *msg) {
+  if (!ok) throw Error(mvg::kInvalid, msg);
+}
+
+hipStream_t new_stream() {
+  hipStream_t s;
+  MVG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  return s;
+}
+
+template <typename T>
+T *upload(DevBuf &b, const T *src, size_t count, hipStream_t s) {
+  b.reserve(sizeof(T) * std::max<size_t>(count, 1));
+  if (count) MVG_HIP(hipMemcpyAsync(b.ptr, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+  return b.as<T>();
+}
+
+template <typename T>
+void download(T *dst, const void *src, size_t count, hipStream_t s) {
+  if (count) MVG_HIP(hipMemcpyAsync(dst, src, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+}
+
+// Philox-4x32-10 (Salmon et al., SC'11) keyed by the 64-bit seed, counter
+// (lo, hi, 0, 0): the generator behind the seeded initial centroid picks of
+// kmeans.mojo:67-82.
+void philox(uint64_t seed, uint64_t counter, uint32_t out[4]) {
+  uint32_t c0 = (uint32_t)counter, c1 = (uint32_t)(counter >> 32), c2 = 0, c3 = 0;
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Row indices of the initial centroids and the reseed offset
+// (kmeans.mojo:70-82).
+int64_t kmeans_pick(int64_t n, int k, uint64_t seed, std::vector<int64_t> &rows) {
+  uint32_t w[4];
+  uint64_t ctr = 0;
+  philox(seed, ctr++, w);
+  const int64_t offset = (int64_t)(w[0] % (uint32_t)n);
+  rows.resize(k);
+  for (int i = 0; i < k; i++) {
+    if (i != 0 && i % 4 == 0) philox(seed, ctr++, w);
+    rows[i] = (int64_t)(w[i % 4] % (uint32_t)n);
+  }
+  return offset;
+}
+
+constexpr uint64_t kDefaultSeed = 0x4D4F4A4F564543ull;  // kmeans.mojo:70
+
+// Device-side PQ training on rows already in HBM ([n x d]).
+void pq_train_device(const float *dx, int64_t n, int d, int M, int niter, uint64_t seed,
+                     float *dcb, mvg::KmeansWork &kw, hipStream_t s) {
+  const int dsub = d / M;
+  std::vector<int64_t> rows;
+  const int64_t off = kmeans_pick(n, mvg::kKsub, seed, rows);
+  std::vector<float> picked((size_t)mvg::kKsub * d), cb((size_t)mvg::kKsub * d);
+  for (int c = 0; c < mvg::kKsub; c++)
+    MVG_HIP(hipMemcpyAsync(picked.data() + (size_t)c * d, dx + rows[c] * d, sizeof(float) * d,
+                           hipMemcpyDeviceToHost, s));
+  MVG_HIP(hipStreamSynchronize(s));
+  for (int m = 0; m < M; m++)
+    for (int c = 0; c < mvg::kKsub; c++)
+      for (int j = 0; j < dsub; j++)
+        cb[((size_t)m * mvg::kKsub + c) * dsub + j] = picked[(size_t)c * d + m * dsub + j];
+  MVG_HIP(hipMemcpyAsync(dcb, cb.data(), sizeof(float) * cb.size(), hipMemcpyHostToDevice, s));
+  mvg::kmeans_fit(dx, n, M, dsub, mvg::kKsub, niter, off, dcb, kw, s);
+}
+
+}  // namespace
+
+void mvg::set_last_error(const std::string &msg) { g_last_error = msg; }
+
+// ===========================================================================
+// Handles
+
+struct mvg_flat {
+  int d = 0, metric = 0;
+  int64_t ntotal = 0;
+  DevBuf base, part, q, D, I, excl;
+  hipStream_t s = nullptr;
+};
+
+struct mvg_sq8 {
+  int d = 0, metric = 0;
+  int64_t ntotal = 0;
+  float gmin = 3.40282347e38f, gmax = -3.40282347e38f, scale = 1.0f;  // index_flat_sq8.mojo:234-236
+  DevBuf f32, codes, norms, tmp, q, D, I, excl;
+  mvg::Sq8Work work;
+  hipStream_t s = nullptr;
+};
+
+struct mvg_ivfpq {
+  int d = 0, nlist = 0, M = 0, metric = 0, nprobe = 1;
+  bool trained = false, keep_vectors = true, dirty = true;
+  int64_t ntotal = 0;
+  std::vector<std::vector<int32_t>> lids;
+  std::vector<std::vector<uint8_t>> lcodes;
+  DevBuf coarse, pq, vectors, list_off, codes, ids;
+  DevBuf tx, tlab, tcodes, tres, q, D, I, excl, npq;
+  mvg::IvfWork work;
+  mvg::KmeansWork kw;
+  hipStream_t s = nullptr;
+};
+
+namespace {
+
+void ivf_sync_lists(mvg_ivfpq *h) {
+  if (!h->dirty) return;
+  std::vector<int64_t> off(h->nlist + 1, 0);
+  for (int l = 0; l < h->nlist; l++) off[l + 1] = off[l] + (int64_t)h->lids[l].size();
+  const int64_t total = off[h->nlist];
+  std::vector<int32_t> ids((size_t)std::max<int64_t>(total, 1));
+  std::vector<uint8_t> codes((size_t)std::max<int64_t>(total * h->M, 1));
+  for (int l = 0; l < h->nlist; l++) {
+    std::copy(h->lids[l].begin(), h->lids[l].end(), ids.begin() + off[l]);
+    std::copy(h->lcodes[l].begin(), h->lcodes[l].end(), codes.begin() + off[l] * h->M);
+  }
+  upload(h->list_off, off.data(), off.size(), h->s);
+  // +16 bytes so 16-byte code loads of the last entry stay inside the buffer.
+  h->codes.reserve(codes.size() + 16);
+  MVG_HIP(hipMemcpyAsync(h->codes.ptr, codes.data(), codes.size(), hipMemcpyHostToDevice, h->s));
+  upload(h->ids, ids.data(), ids.size(), h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  h->dirty = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mvg_last_error(void) { return g_last_error.c_str(); }
+
+const char *mvg_version(void) { return "mojovec-gpu 0.1.0 (gfx950)"; }
+
+int mvg_device_info(char *name, int name_len, int *compute_units) {
+  API_BEGIN
+  int dev = 0;
+  MVG_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  MVG_HIP(hipGetDeviceProperties(&p, dev));
+  if (name && name_len > 0) {
+    std::strncpy(name, p.gcnArchName, (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (compute_units) *compute_units = p.multiProcessorCount;
+  API_END
+}
+
+// ---- Flat ------------------------------------------------------------------
+
+int mvg_flat_create(int d, int metric, mvg_flat **out) {
+  API_BEGIN
+  require(d > 0 && d <= 65536, "dimension must be between 1 and 65536.");
+  require(metric == MVG_METRIC_L2 || metric == MVG_METRIC_IP, "metric must be 0 (L2) or 1 (IP).");
+  auto h = std::make_unique<mvg_flat>();
+  h->d = d;
+  h->metric = metric;
+  h->s = new_stream();
+  *out = h.release();
+  API_END
+}
+
+int mvg_flat_add(mvg_flat *h, const float *x, int64_t n) {
+  API_BEGIN
+  require(h && n >= 0, "invalid arguments");
+  if (n == 0) return MVG_OK;
+  const size_t keep = sizeof(float) * (size_t)h->ntotal * h->d;
+  const size_t need = keep + sizeof(float) * (size_t)n * h->d;
+  h->base.grow_keep(std::max(need, h->base.bytes * 2), keep, h->s);
+  MVG_HIP(hipMemcpyAsync((char *)h->base.ptr + keep, x, need - keep, hipMemcpyHostToDevice, h->s));
+  MVG_HIP(hipStreamSynchronize(h->s));
+  h->ntotal += n;
+  API_END
+}
+
+int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_t *exclude,
+                    int64_t exclude_len, float *distances, int64_t *labels) {
+  API_BEGIN
+  require(h && nq >= 0 && k >= 1, "invalid arguments");
+  if (nq == 0) return MVG_OK;
+  const float *dq = upload(h->q, q, (size_t)nq * h->d, h->s);
+  const uint8_t *dx = exclude && exclude_len > 0 ? upload(h->excl, exclude, (size_t)exclude_len, h->s) : nullptr;
+  h->D.reserve(sizeof(float) * nq * k);
+  h->I.reserve(sizeof(int64_t) * nq * k);
+  mvg::flat_topk(h->base.as<float>(), h->ntotal, h->d, h->metric, dq, nq, k, dx,
+                 dx ? exclude_len : 0, h->part, 0.0f, h->D.as<float>(), h->I.as<int64_t>(),
+                 nullptr, nullptr, h->s);
+  download(distances, h->D.ptr, (size_t)nq * k, h->s);
+  download(labels, h->I.ptr, (size_t)nq * k, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int64_t mvg_flat_ntotal(const mvg_flat *h) { return h ? h->ntotal : 0; }
+
+void mvg_flat_destroy(mvg_flat *h) {
+  if (!h) return;
+  if (h->s) (void)hipStreamDestroy(h->s);
+  delete h;
+}
+
+// ---- SQ8 ---------------------------------------------------------------------
+
+int mvg_sq8_create(int d, int metric, mvg_sq8 **out) {
+  API_BEGIN
+  require(d > 0 && d <= 65536, "dimension must be between 1 and 65536.");
+  require(metric == MVG_METRIC_L2 || metric == MVG_METRIC_IP, "metric must be 0 (L2) or 1 (IP).");
+  auto h = std::make_unique<mvg_sq8>();
+  h->d = d;
+  h->metric = metric;
+  h->s = new_stream();
+  *out = h.release();
+  API_END
+}
+
+int mvg_sq8_add(mvg_sq8 *h, const float *x, int64_t n) {
+  API_BEGIN
+  require(h && n >= 0, "invalid arguments");
+  if (n == 0) return MVG_OK;
+  const int d = h->d;
+  const size_t keep = sizeof(float) * (size_t)h->ntotal * d;
+  const size_t need = keep + sizeof(float) * (size_t)n * d;
+  const size_t cap_rows = std::max<size_t>(need / (sizeof(float) * d), h->f32.bytes / (sizeof(float) * d) * 2);
+  h->f32.grow_keep(cap_rows * d * sizeof(float), keep, h->s);
+  h->codes.grow_keep(cap_rows * d + 16, (size_t)h->ntotal * d, h->s);
+  h->norms.grow_keep(cap_rows * sizeof(uint32_t), (size_t)h->ntotal * sizeof(uint32_t), h->s);
+  float *dst = (float *)((char *)h->f32.ptr + keep);
+  MVG_HIP(hipMemcpyAsync(dst, x, need - keep, hipMemcpyHostToDevice, h->s));
+  float mm[2];  // index_flat_sq8.mojo:317-331
+  mvg::sq8_minmax(dst, n * d, mm, h->tmp, h->s);
+  bool requant = false;
+  if (mm[0] < h->gmin) { h->gmin = mm[0]; requant = true; }
+  if (mm[1] > h->gmax) { h->gmax = mm[1]; requant = true; }
+  if (h->metric == MVG_METRIC_L2) {  // :333-342
+    h->scale = h->gmax == h->gmin ? 1.0f : (h->gmax - h->gmin) / 255.0f;
+  } else {
+    const float mx = std::max(std::fabs(h->gmin), std::fabs(h->gmax));
+    h->scale = mx > 0.0f ? mx / 127.0f : 1.0f;
+  }
+  const float inv = 1.0f / h->scale;
+  const int sym = h->metric != MVG_METRIC_L2;
+  const float mn = sym ? 0.0f : h->gmin;
+  if (requant && h->ntotal > 0)  // :346-367
+    mvg::sq8_encode(h->f32.as<float>(), h->ntotal, d, sym, mn, inv, h->codes.as<uint8_t>(),
+                    h->norms.as<uint32_t>(), h->s);
+  mvg::sq8_encode(dst, n, d, sym, mn, inv, h->codes.as<uint8_t>() + (size_t)h->ntotal * d,
+                  h->norms.as<uint32_t>() + h->ntotal, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  h->ntotal += n;
+  API_END
+}
+
+int mvg_sq8_search(mvg_sq8 *h, const float *q, int64_t nq, int k, int k_factor,
+                   const uint8_t *exclude, int64_t exclude_len, float *distances,
+                   int64_t *labels) {
+  API_BEGIN
+  require(h && nq >= 0 && k >= 1 && k_factor >= 1, "invalid arguments");
+  if (nq == 0) return MVG_OK;
+  mvg::Sq8SearchArgs a{};
+  a.q = upload(h->q, q, (size_t)nq * h->d, h->s);
+  a.nq = nq;
+  a.d = h->d;
+  a.metric = h->metric;
+  a.n = h->ntotal;
+  a.codes = h->codes.as<uint8_t>();
+  a.norms = h->norms.as<uint32_t>();
+  a.vectors = h->f32.as<float>();
+  a.gmin = h->gmin;
+  a.scale = h->scale;
+  a.k = k;
+  a.k_factor = k_factor;
+  a.excl = exclude && exclude_len > 0 ? upload(h->excl, exclude, (size_t)exclude_len, h->s) : nullptr;
+  a.excl_len = a.excl ? exclude_len : 0;
+  h->D.reserve(sizeof(float) * nq * k);
+  h->I.reserve(sizeof(int64_t) * nq * k);
+  a.D = h->D.as<float>();
+  a.I = h->I.as<int64_t>();
+  mvg::sq8_search(a, h->work, h->s);
+  download(distances, h->D.ptr, (size_t)nq * k, h->s);
+  download(labels, h->I.ptr, (size_t)nq * k, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_sq8_params(const mvg_sq8 *h, float *gmin, float *gmax, float *scale, int64_t *ntotal) {
+  API_BEGIN
+  require(h, "invalid handle");
+  if (gmin) *gmin = h->gmin;
+  if (gmax) *gmax = h->gmax;
+  if (scale) *scale = h->scale;
+  if (ntotal) *ntotal = h->ntotal;
+  API_END
+}
+
+int mvg_sq8_get_codes(mvg_sq8 *h, uint8_t *codes, uint32_t *norms) {
+  API_BEGIN
+  require(h, "invalid handle");
+  if (codes) download(codes, h->codes.ptr, (size_t)h->ntotal * h->d, h->s);
+  if (norms) download(norms, h->norms.ptr, (size_t)h->ntotal, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+void mvg_sq8_destroy(mvg_sq8 *h) {
+  if (!h) return;
+  if (h->s) (void)hipStreamDestroy(h->s);
+  delete h;
+}
+
+// ---- k-means / PQ ------------------------------------------------------------
+
+int mvg_kmeans_train(const float *x, int64_t n, int d, int k, int niter, const float *init,
+                     int64_t sample_offset, uint64_t seed, float *centroids, int32_t *assign) {
+  API_BEGIN
+  require(n > 0 && d > 0 && k > 0 && niter >= 0, "invalid k-means arguments");
+  hipStream_t s = new_stream();
+  struct Guard { hipStream_t s; ~Guard() { (void)hipStreamDestroy(s); } } g{s};
+  DevBuf bx, bc;
+  mvg::KmeansWork kw;
+  const float *dx = upload(bx, x, (size_t)n * d, s);
+  std::vector<float> c0((size_t)k * d);
+  int64_t off = sample_offset;
+  if (init) {
+    std::copy(init, init + (size_t)k * d, c0.begin());
+  } else {
+    std::vector<int64_t> rows;
+    off = kmeans_pick(n, k, seed, rows);
+    for (int c = 0; c < k; c++) std::copy(x + rows[c] * d, x + rows[c] * d + d, c0.begin() + (size_t)c * d);
+  }
+  float *dc = upload(bc, c0.data(), c0.size(), s);
+  mvg::kmeans_fit(dx, n, 1, d, k, niter, off, dc, kw, s);
+  download(centroids, dc, (size_t)k * d, s);
+  if (assign && niter > 0) download(assign, kw.assign.ptr, (size_t)n, s);
+  MVG_HIP(hipStreamSynchronize(s));
+  API_END
+}
+
+int mvg_pq_train(const float *x, int64_t n, int d, int M, int niter, uint64_t seed,
+                 float *codebooks) {
+  API_BEGIN
+  require(d > 0, "PQ dimension must be positive.");
+  require(M > 0 && M <= d && d % M == 0, "PQ M must divide dimension and be between 1 and dimension.");
+  require(n > 0, "PQ training needs at least one vector");
+  hipStream_t s = new_stream();
+  struct Guard { hipStream_t s; ~Guard() { (void)hipStreamDestroy(s); } } g{s};
+  DevBuf bx, bcb;
+  mvg::KmeansWork kw;
+  const float *dx = upload(bx, x, (size_t)n * d, s);
+  bcb.reserve(sizeof(float) * (size_t)d * mvg::kKsub);
+  pq_train_device(dx, n, d, M, niter, seed, bcb.as<float>(), kw, s);
+  download(codebooks, bcb.ptr, (size_t)d * mvg::kKsub, s);
+  MVG_HIP(hipStreamSynchronize(s));
+  API_END
+}
+
+int mvg_pq_encode(const float *x, int64_t n, int d, int M, const float *codebooks, uint8_t *codes) {
+  API_BEGIN
+  require(M > 0 && M <= d && d % M == 0, "PQ M must divide dimension and be between 1 and dimension.");
+  hipStream_t s = new_stream();
+  struct Guard { hipStream_t s; ~Guard() { (void)hipStreamDestroy(s); } } g{s};
+  DevBuf bx, bcb, bcodes;
+  const float *dx = upload(bx, x, (size_t)n * d, s);
+  const float *dcb = upload(bcb, codebooks, (size_t)d * mvg::kKsub, s);
+  bcodes.reserve((size_t)std::max<int64_t>(n * M, 1));
+  mvg::pq_assign(dx, n, d, M, dcb, nullptr, nullptr, bcodes.as<uint8_t>(), nullptr, s);
+  download(codes, bcodes.ptr, (size_t)n * M, s);
+  MVG_HIP(hipStreamSynchronize(s));
+  API_END
+}
+
+int mvg_pq_lut(const float *q, int64_t nq, int d, int M, const float *codebooks, int metric,
+               float *lut) {
+  API_BEGIN
+  require(M > 0 && M <= d && d % M == 0, "PQ M must divide dimension and be between 1 and dimension.");
+  hipStream_t s = new_stream();
+  struct Guard { hipStream_t s; ~Guard() { (void)hipStreamDestroy(s); } } g{s};
+  DevBuf bq, bcb, bl;
+  const float *dq = upload(bq, q, (size_t)nq * d, s);
+  const float *dcb = upload(bcb, codebooks, (size_t)d * mvg::kKsub, s);
+  bl.reserve(sizeof(float) * (size_t)std::max<int64_t>(nq, 1) * M * mvg::kKsub);
+  mvg::pq_lut(dq, nq, d, M, dcb, metric, bl.as<float>(), s);
+  download(lut, bl.ptr, (size_t)nq * M * mvg::kKsub, s);
+  MVG_HIP(hipStreamSynchronize(s));
+  API_END
+}
+
+// ---- IVF-PQ ------------------------------------------------------------------
+
+int mvg_ivfpq_create(int d, int nlist, int M, int metric, mvg_ivfpq **out) {
+  API_BEGIN
+  require(d > 0, "IVF-PQ dimension must be positive.");  // index_ivf_pq.mojo:52-59
+  require(nlist > 0 && nlist <= 1000000, "IVF-PQ nlist must be between 1 and 1000000.");
+  require(M > 0 && M <= d && d % M == 0,
+          "IVF-PQ M must divide dimension and be between 1 and dimension.");
+  require(metric == MVG_METRIC_L2 || metric == MVG_METRIC_IP, "metric must be 0 (L2) or 1 (IP).");
+  require(d / M <= 64, "PQ subvector dimension above 64 is not supported");
+  auto h = std::make_unique<mvg_ivfpq>();
+  h->d = d;
+  h->nlist = nlist;
+  h->M = M;
+  h->metric = metric;
+  h->lids.resize(nlist);
+  h->lcodes.resize(nlist);
+  h->s = new_stream();
+  h->coarse.reserve(sizeof(float) * (size_t)nlist * d);
+  h->pq.reserve(sizeof(float) * (size_t)d * mvg::kKsub);
+  *out = h.release();
+  API_END
+}
+
+int mvg_ivfpq_set_keep_vectors(mvg_ivfpq *h, int keep) {
+  API_BEGIN
+  require(h, "invalid handle");
+  if (h->ntotal > 0 && (keep != 0) != h->keep_vectors)
+    throw Error(mvg::kState, "keep_vectors can only change before the first add");
+  h->keep_vectors = keep != 0;
+  API_END
+}
+
+int mvg_ivfpq_train(mvg_ivfpq *h, const float *x, int64_t n, int niter_coarse, int niter_pq,
+                    uint64_t seed) {
+  API_BEGIN
+  require(h && n >= 0, "invalid arguments");
+  if (h->trained || n == 0) return MVG_OK;  // index_ivf_pq.mojo:93
+  const int d = h->d;
+  float *dx = upload(h->tx, x, (size_t)n * d, h->s);
+  // 1. coarse k-means (:97-103)
+  std::vector<int64_t> rows;
+  const int64_t off = kmeans_pick(n, h->nlist, seed, rows);
+  std::vector<float> c0((size_t)h->nlist * d);
+  for (int c = 0; c < h->nlist; c++) std::copy(x + rows[c] * d, x + rows[c] * d + d, c0.begin() + (size_t)c * d);
+  float *dc = h->coarse.as<float>();
+  MVG_HIP(hipMemcpyAsync(dc, c0.data(), sizeof(float) * c0.size(), hipMemcpyHostToDevice, h->s));
+  mvg::kmeans_fit(dx, n, 1, d, h->nlist, niter_coarse, off, dc, h->kw, h->s);
+  // 2. residuals against quantizer.search(x, 1) (:106-128), PQ (:130-132)
+  h->tlab.reserve(sizeof(int32_t) * n);
+  mvg::flat_argmin(dc, h->nlist, d, h->metric, false, dx, n, h->tlab.as<int32_t>(), nullptr, h->s);
+  h->tres.reserve(sizeof(float) * (size_t)n * d);
+  mvg::residuals(dx, n, d, dc, h->tlab.as<int32_t>(), h->tres.as<float>(), h->s);
+  pq_train_device(h->tres.as<float>(), n, d, h->M, niter_pq, seed, h->pq.as<float>(), h->kw, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  h->trained = true;
+  API_END
+}
+
+int mvg_ivfpq_set_codebooks(mvg_ivfpq *h, const float *coarse, const float *pq) {
+  API_BEGIN
+  require(h && coarse && pq, "invalid arguments");
+  MVG_HIP(hipMemcpyAsync(h->coarse.ptr, coarse, sizeof(float) * (size_t)h->nlist * h->d,
+                         hipMemcpyHostToDevice, h->s));
+  MVG_HIP(hipMemcpyAsync(h->pq.ptr, pq, sizeof(float) * (size_t)h->d * mvg::kKsub,
+                         hipMemcpyHostToDevice, h->s));
+  MVG_HIP(hipStreamSynchronize(h->s));
+  h->trained = true;
+  API_END
+}
+
+int mvg_ivfpq_get_codebooks(mvg_ivfpq *h, float *coarse, float *pq) {
+  API_BEGIN
+  require(h, "invalid handle");
+  if (coarse) download(coarse, h->coarse.ptr, (size_t)h->nlist * h->d, h->s);
+  if (pq) download(pq, h->pq.ptr, (size_t)h->d * mvg::kKsub, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_ivfpq_add(mvg_ivfpq *h, const float *x, int64_t n) {
+  API_BEGIN
+  require(h && n >= 0, "invalid arguments");
+  if (!h->trained || n == 0) return MVG_OK;  // index_ivf_pq.mojo:161
+  if (h->ntotal + n > INT32_MAX) throw Error(mvg::kInvalid, "IVF-PQ holds at most 2^31-1 vectors");
+  const int d = h->d, M = h->M;
+  const float *dx = upload(h->tx, x, (size_t)n * d, h->s);
+  h->tlab.reserve(sizeof(int32_t) * n);
+  h->tcodes.reserve((size_t)n * M);
+  int32_t *dl = h->tlab.as<int32_t>();
+  mvg::flat_argmin(h->coarse.as<float>(), h->nlist, d, h->metric, false, dx, n, dl, nullptr, h->s);
+  mvg::pq_assign(dx, n, d, M, h->pq.as<float>(), h->coarse.as<float>(), dl,
+                 h->tcodes.as<uint8_t>(), nullptr, h->s);
+  std::vector<int32_t> lab(n);
+  std::vector<uint8_t> codes((size_t)n * M);
+  download(lab.data(), dl, (size_t)n, h->s);
+  download(codes.data(), h->tcodes.ptr, (size_t)n * M, h->s);
+  if (h->keep_vectors) {
+    const size_t keep = sizeof(float) * (size_t)h->ntotal * d;
+    const size_t need = keep + sizeof(float) * (size_t)n * d;
+    h->vectors.grow_keep(std::max(need, h->vectors.bytes + h->vectors.bytes / 2), keep, h->s);
+    MVG_HIP(hipMemcpyAsync((char *)h->vectors.ptr + keep, dx, need - keep,
+                           hipMemcpyDeviceToDevice, h->s));
+  }
+  MVG_HIP(hipStreamSynchronize(h->s));
+  for (int64_t i = 0; i < n; i++) {  // :199-211, append order
+    const int l = lab[i];
+    if (l < 0 || l >= h->nlist) continue;
+    h->lids[l].push_back((int32_t)(h->ntotal + i));
+    h->lcodes[l].insert(h->lcodes[l].end(), codes.begin() + i * M, codes.begin() + (i + 1) * M);
+  }
+  h->ntotal += n;
+  h->dirty = true;
+  API_END
+}
+
+int mvg_ivfpq_set_nprobe(mvg_ivfpq *h, int nprobe) {
+  API_BEGIN
+  require(h && nprobe >= 1, "nprobe must be at least 1");
+  h->nprobe = nprobe;
+  API_END
+}
+
+int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                            const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                            const uint8_t *exclude, int64_t exclude_len, float *distances,
+                            int64_t *labels, void *stream) {
+  API_BEGIN
+  require(h && nq >= 0 && k >= 1 && k_factor >= 1, "invalid arguments");
+  require(h->trained && h->ntotal > 0, "search_device needs a trained, non-empty index");
+  hipStream_t s = stream ? (hipStream_t)stream : h->s;
+  ivf_sync_lists(h);
+  mvg::IvfSearchArgs a{};
+  a.q = q;
+  a.nq = nq;
+  a.d = h->d;
+  a.M = h->M;
+  a.nlist = h->nlist;
+  a.metric = h->metric;
+  a.coarse = h->coarse.as<float>();
+  a.pq = h->pq.as<float>();
+  a.list_off = h->list_off.as<int64_t>();
+  a.codes = h->codes.as<uint8_t>();
+  a.ids = h->ids.as<int32_t>();
+  a.vectors = h->keep_vectors ? h->vectors.as<float>() : nullptr;
+  a.k = k;
+  a.k_factor = k_factor;
+  a.nprobe = std::min(nprobe_per_query ? max_nprobe : h->nprobe, h->nlist);
+  require(a.nprobe >= 1, "max_nprobe must be at least 1");
+  a.nprobe_per_query = nprobe_per_query;
+  a.excl = exclude_len > 0 ? exclude : nullptr;
+  a.excl_len = exclude_len;
+  a.D = distances;
+  a.I = labels;
+  mvg::ivfpq_search(a, h->work, s);
+  API_END
+}
+
+int mvg_ivfpq_search(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                     const int32_t *nprobe_per_query, int k_factor, const uint8_t *exclude,
+                     int64_t exclude_len, float *distances, int64_t *labels) {
+  API_BEGIN
+  require(h && nq >= 0 && k >= 1 && k_factor >= 1, "invalid arguments");
+  if (nq == 0) return MVG_OK;
+  if (!h->trained || h->ntotal == 0) {  // index_ivf_pq.mojo:247-251
+    for (int64_t i = 0; i < nq * k; i++) {
+      distances[i] = mvg::kIvfPadDist;
+      labels[i] = -1;
+    }
+    return MVG_OK;
+  }
+  const float *dq = upload(h->q, q, (size_t)nq * h->d, h->s);
+  const int32_t *dnp = nullptr;
+  int maxp = h->nprobe;
+  if (nprobe_per_query) {
+    std::vector<int32_t> npq(nprobe_per_query, nprobe_per_query + nq);
+    maxp = 1;
+    for (auto &v : npq) {
+      require(v >= 1, "nprobe must be at least 1");
+      v = std::min(v, h->nlist);
+      maxp = std::max(maxp, (int)v);
+    }
+    dnp = upload(h->npq, npq.data(), (size_t)nq, h->s);
+  }
+  const uint8_t *dx = exclude && exclude_len > 0 ? upload(h->excl, exclude, (size_t)exclude_len, h->s) : nullptr;
+  h->D.reserve(sizeof(float) * nq * k);
+  h->I.reserve(sizeof(int64_t) * nq * k);
+  int rc = mvg_ivfpq_search_device(h, dq, nq, k, dnp, maxp, k_factor, dx, dx ? exclude_len : 0,
+                                   h->D.as<float>(), h->I.as<int64_t>(), nullptr);
+  if (rc != MVG_OK) return rc;
+  download(distances, h->D.ptr, (size_t)nq * k, h->s);
+  download(labels, h->I.ptr, (size_t)nq * k, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_ivfpq_set_profiling(mvg_ivfpq *h, int on) {
+  API_BEGIN
+  require(h, "invalid handle");
+  h->work.prof.on = on != 0;
+  h->work.prof.recorded = false;
+  API_END
+}
+
+int mvg_ivfpq_read_profile(mvg_ivfpq *h, float *stage_ms, int64_t *scanned_entries) {
+  API_BEGIN
+  require(h, "invalid handle");
+  auto &pf = h->work.prof;
+  if (!pf.recorded) throw Error(mvg::kState, "no profiled search has run");
+  MVG_HIP(hipEventSynchronize(pf.ev[3]));
+  for (int i = 0; i < 3; i++)
+    if (stage_ms) MVG_HIP(hipEventElapsedTime(&stage_ms[i], pf.ev[i], pf.ev[i + 1]));
+  if (scanned_entries) {
+    unsigned long long v = 0;
+    MVG_HIP(hipMemcpy(&v, pf.counter.ptr, sizeof(v), hipMemcpyDeviceToHost));
+    *scanned_entries = (int64_t)v;
+  }
+  API_END
+}
+
+int mvg_ivfpq_info(mvg_ivfpq *h, int64_t *ntotal, int *trained, int *nprobe) {
+  API_BEGIN
+  require(h, "invalid handle");
+  if (ntotal) *ntotal = h->ntotal;
+  if (trained) *trained = h->trained ? 1 : 0;
+  if (nprobe) *nprobe = h->nprobe;
+  API_END
+}
+
+int mvg_ivfpq_list_sizes(mvg_ivfpq *h, int64_t *sizes) {
+  API_BEGIN
+  require(h, "invalid handle");
+  for (int l = 0; l < h->nlist; l++) sizes[l] = (int64_t)h->lids[l].size();
+  API_END
+}
+
+int mvg_ivfpq_get_list(mvg_ivfpq *h, int list, int64_t *ids, uint8_t *codes) {
+  API_BEGIN
+  require(h && list >= 0 && list < h->nlist, "list index out of range");
+  const auto &li = h->lids[list];
+  for (size_t i = 0; i < li.size(); i++) ids[i] = li[i];
+  std::copy(h->lcodes[list].begin(), h->lcodes[list].end(), codes);
+  API_END
+}
+
+void mvg_ivfpq_destroy(mvg_ivfpq *h) {
+  if (!h) return;
+  if (h->s) (void)hipStreamDestroy(h->s);
+  delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+// Exact-order FP32 distance routines.
+//
+// The plain (FP32) form of every kernel reproduces the reference's operation
+// order so that distances, argmins and top-k ids agree bit for bit with the
+// CPU restatement:
+//   l2_distance_simd[W] / inner_product_simd[W]  mojovec/utils/distances.mojo:43-228
+//   l2_distance_short4 / inner_product_short4    mojovec/utils/distances.mojo:9-39
+// Structure: four W-lane FMA accumulators over 4W blocks, lane-wise
+// ((a0 + a1) + a2) + a3, W-blocks, then the lanes are folded by the halving
+// tree v[j] += v[j + W/2] ... (SIMD.reduce_add), then narrower tails.
+// Everything here is compiled with -ffp-contract=off: only __builtin_fmaf
+// fuses.
+#pragma once
+
+#include "common.h"
+
+namespace mvg {
+
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <int W>
+__device__ __forceinline__ float tree_sum(float *v) {
+#pragma unroll
+  for (int h = W / 2; h >= 1; h >>= 1) {
+#pragma unroll
+    for (int j = 0; j < h; j++) v[j] = v[j] + v[j + h];
+  }
+  return v[0];
+}
+
+// l2_distance_simd[8] for a runtime dimension (flat / coarse / k-means).
+template <typename PX, typename PY>
+__device__ float l2_w8(PX x, PY y, int d) {
+  if (d < 8) {
+    float r = 0.0f;
+    for (int i = 0; i < d; i++) {
+      float df = x[i] - y[i];
+      r = r + df * df;
+    }
+    return r;
+  }
+  float a0[8], a1[8], a2[8], a3[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) a0[j] = a1[j] = a2[j] = a3[j] = 0.0f;
+  int i = 0;
+  for (; i <= d - 32; i += 32) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      float d0 = x[i + j] - y[i + j];
+      float d1 = x[i + 8 + j] - y[i + 8 + j];
+      float d2 = x[i + 16 + j] - y[i + 16 + j];
+      float d3 = x[i + 24 + j] - y[i + 24 + j];
+      a0[j] = fma_(d0, d0, a0[j]);
+      a1[j] = fma_(d1, d1, a1[j]);
+      a2[j] = fma_(d2, d2, a2[j]);
+      a3[j] = fma_(d3, d3, a3[j]);
+    }
+  }
+  float v[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) v[j] = ((a0[j] + a1[j]) + a2[j]) + a3[j];
+  for (; i <= d - 8; i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      float df = x[i + j] - y[i + j];
+      v[j] = fma_(df, df, v[j]);
+    }
+  }
+  float res = tree_sum<8>(v);
+  for (; i <= d - 4; i += 4) {
+    float r[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      float df = x[i + j] - y[i + j];
+      r[j] = df * df;
+    }
+    res += tree_sum<4>(r);
+  }
+  for (; i < d; i++) {
+    float df = x[i] - y[i];
+    res += df * df;
+  }
+  return res;
+}
+
+// inner_product_simd[W] for W in {4, 8} and a runtime dimension.
+template <int W, typename PX, typename PY>
+__device__ float ip_wn(PX x, PY y, int d) {
+  if (d < W) {
+    float r = 0.0f;
+    for (int i = 0; i < d; i++) r = r + x[i] * y[i];
+    return r;
+  }
+  float p0[W], p1[W], p2[W], p3[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) p0[j] = p1[j] = p2[j] = p3[j] = 0.0f;
+  int i = 0;
+  for (; i <= d - 4 * W; i += 4 * W) {
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      p0[j] = fma_(x[i + j], y[i + j], p0[j]);
+      p1[j] = fma_(x[i + W + j], y[i + W + j], p1[j]);
+      p2[j] = fma_(x[i + 2 * W + j], y[i + 2 * W + j], p2[j]);
+      p3[j] = fma_(x[i + 3 * W + j], y[i + 3 * W + j], p3[j]);
+    }
+  }
+  float v[W];
+#pragma unroll
+  for (int j = 0; j < W; j++) v[j] = ((p0[j] + p1[j]) + p2[j]) + p3[j];
+  for (; i <= d - W; i += W) {
+#pragma unroll
+    for (int j = 0; j < W; j++) v[j] = fma_(x[i + j], y[i + j], v[j]);
+  }
+  float res = tree_sum<W>(v);
+  if (W >= 8) {
+    for (; i <= d - 4; i += 4) {
+      float r[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) r[j] = fma_(x[i + j], y[i + j], 0.0f);
+      res += tree_sum<4>(r);
+    }
+  }
+  for (; i < d; i++) res += x[i] * y[i];
+  return res;
+}
+
+// l2_distance_short4 / inner_product_short4 (d <= 4).
+template <typename PX, typename PY>
+__device__ __forceinline__ float l2_short4(PX x, PY y, int d) {
+  if (d == 4) {
+    float r[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      float df = x[j] - y[j];
+      r[j] = df * df;
+    }
+    return tree_sum<4>(r);
+  }
+  float res = 0.0f;
+  for (int i = 0; i < d; i++) {
+    float df = x[i] - y[i];
+    res += df * df;
+  }
+  return res;
+}
+
+template <typename PX, typename PY>
+__device__ __forceinline__ float ip_short4(PX x, PY y, int d) {
+  if (d == 4) {
+    float r[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) r[j] = x[j] * y[j];
+    return tree_sum<4>(r);
+  }
+  float res = 0.0f;
+  for (int i = 0; i < d; i++) res += x[i] * y[i];
+  return res;
+}
+
+// K-means / PQ distance selector (kmeans.mojo:131-138, pq.mojo:206-213).
+template <typename PX, typename PY>
+__device__ __forceinline__ float l2_kmeans(PX x, PY y, int d) {
+  return d <= 4 ? l2_short4(x, y, d) : l2_w8(x, y, d);
+}
+
+// PQ inner-product LUT entry (pq.mojo:285-292): -ip with W=4 or short4.
+template <typename PX, typename PY>
+__device__ __forceinline__ float ip_pq(PX x, PY y, int d) {
+  return d <= 4 ? ip_short4(x, y, d) : ip_wn<4>(x, y, d);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront forms of the W = 64 routines used by the exact rerank
+// (index_hnsw.mojo:456-465): lane j owns SIMD lane j; the halving tree is a
+// shuffle-down ladder. Every lane returns the full result.
+
+__device__ __forceinline__ float wave_tree64(float v) {
+#pragma unroll
+  for (int h = 32; h >= 1; h >>= 1) v = v + __shfl_down(v, h, 64);
+  return __shfl(v, 0, 64);
+}
+
+template <int N>
+__device__ __forceinline__ float wave_tree_n(float v) {
+#pragma unroll
+  for (int h = N / 2; h >= 1; h >>= 1) v = v + __shfl_down(v, h, 64);
+  return __shfl(v, 0, 64);
+}
+
+template <bool IP>
+__device__ float wave_dist_w64(const float *q, const float *x, int d, int lane) {
+  if (d < 64) {
+    float r = 0.0f;
+    for (int i = 0; i < d; i++) {
+      if (IP) {
+        r = r + q[i] * x[i];
+      } else {
+        float df = q[i] - x[i];
+        r = r + df * df;
+      }
+    }
+    return r;
+  }
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+  int i = 0;
+  for (; i <= d - 256; i += 256) {
+    if (IP) {
+      a0 = fma_(q[i + lane], x[i + lane], a0);
+      a1 = fma_(q[i + 64 + lane], x[i + 64 + lane], a1);
+      a2 = fma_(q[i + 128 + lane], x[i + 128 + lane], a2);
+      a3 = fma_(q[i + 192 + lane], x[i + 192 + lane], a3);
+    } else {
+      float d0 = q[i + lane] - x[i + lane];
+      float d1 = q[i + 64 + lane] - x[i + 64 + lane];
+      float d2 = q[i + 128 + lane] - x[i + 128 + lane];
+      float d3 = q[i + 192 + lane] - x[i + 192 + lane];
+      a0 = fma_(d0, d0, a0);
+      a1 = fma_(d1, d1, a1);
+      a2 = fma_(d2, d2, a2);
+      a3 = fma_(d3, d3, a3);
+    }
+  }
+  float v = ((a0 + a1) + a2) + a3;
+  for (; i <= d - 64; i += 64) {
+    if (IP) {
+      v = fma_(q[i + lane], x[i + lane], v);
+    } else {
+      float df = q[i + lane] - x[i + lane];
+      v = fma_(df, df, v);
+    }
+  }
+  float res = wave_tree64(v);
+  for (; i <= d - 16; i += 16) {
+    float r = 0.0f;
+    if (lane < 16) {
+      if (IP) {
+        r = fma_(q[i + lane], x[i + lane], 0.0f);
+      } else {
+        float df = q[i + lane] - x[i + lane];
+        r = df * df;
+      }
+    }
+    res += wave_tree_n<16>(r);
+  }
+  for (; i <= d - 8; i += 8) {
+    float r = 0.0f;
+    if (lane < 8) {
+      if (IP) {
+        r = fma_(q[i + lane], x[i + lane], 0.0f);
+      } else {
+        float df = q[i + lane] - x[i + lane];
+        r = df * df;
+      }
+    }
+    res += wave_tree_n<8>(r);
+  }
+  for (; i <= d - 4; i += 4) {
+    float r = 0.0f;
+    if (lane < 4) {
+      if (IP) {
+        r = fma_(q[i + lane], x[i + lane], 0.0f);
+      } else {
+        float df = q[i + lane] - x[i + lane];
+        r = df * df;
+      }
+    }
+    res += wave_tree_n<4>(r);
+  }
+  for (; i < d; i++) {
+    if (IP) {
+      res += q[i] * x[i];
+    } else {
+      float df = q[i] - x[i];
+      res += df * df;
+    }
+  }
+  return res;
+}
+
+}  // namespace mvg

@@ -1,0 +1,292 @@
+// IVF-PQ search: coarse top-nprobe, per-(query, list) LUT build in LDS, ADC
+// scan of the list's codes, per-list top-(k*k_factor), per-query merge and the
+// optional exact FP32 rerank.
+//
+// Plain form of IndexIVFPQ.search (mojovec/index/index_ivf_pq.mojo:225-360):
+//   L2: LUT of the residual q - c_l (pq.mojo:252-292), dist = sum_m LUT[m][code_m]
+//       summed sequentially in m starting from 0.0 (:336-341).
+//   IP: LUT of q, dist = -<q, c_l> + sum_m LUT[m][code_m] (:316-324).
+// Ids inside one list are ascending with position (append order of internal
+// ids), so the list-local (key, position) order is the canonical (dist, id)
+// order.
+#include "dist.h"
+#include "kernels.h"
+#include "select.h"
+
+namespace mvg {
+
+namespace {
+
+constexpr int kScanChunk = 4096;  // list positions scored per select round
+
+template <int MF>
+__device__ __forceinline__ float adc_sum(const uint8_t *__restrict__ code, const float *lut, int M,
+                                         float init) {
+  float dist = init;
+  if constexpr (MF == 16) {
+    const uint4 c = *reinterpret_cast<const uint4 *>(code);
+    const uint32_t w[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+    for (int m = 0; m < 16; m++) dist += lut[m * kKsub + ((w[m >> 2] >> (8 * (m & 3))) & 0xFF)];
+  } else if constexpr (MF == 12 || MF == 8) {
+    uint32_t w[MF / 4];
+#pragma unroll
+    for (int t = 0; t < MF / 4; t++) w[t] = reinterpret_cast<const uint32_t *>(code)[t];
+#pragma unroll
+    for (int m = 0; m < MF; m++) dist += lut[m * kKsub + ((w[m >> 2] >> (8 * (m & 3))) & 0xFF)];
+  } else {
+    for (int m = 0; m < M; m++) dist += lut[m * kKsub + code[m]];
+  }
+  return dist;
+}
+
+// One workgroup per (query, probe rank).
+template <int MF, bool IP>
+__global__ __launch_bounds__(kNT) void ivfpq_scan_kernel(
+    const float *__restrict__ q, int d, int M, const float *__restrict__ coarse,
+    const float *__restrict__ pq, const int64_t *__restrict__ list_off,
+    const uint8_t *__restrict__ codes, const int32_t *__restrict__ ids,
+    const int32_t *__restrict__ probes, int maxp, const int32_t *__restrict__ nprobe_q,
+    int nprobe_all, int kk, const uint8_t *__restrict__ excl, int64_t excl_len,
+    uint64_t *__restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float *lut = reinterpret_cast<float *>(smem);                    // M * 256
+  float *rv = lut + M * kKsub;                                     // d (+ 1 for coarse term)
+  uint32_t *keys = reinterpret_cast<uint32_t *>(rv + ((d + 4) & ~3));  // kk + kScanChunk
+  uint32_t *sel = keys + kk + kScanChunk;                          // kk
+  uint32_t *prev_pos = sel + kk;                                   // kk
+  SelectScratch *scr = reinterpret_cast<SelectScratch *>(prev_pos + ((kk + 3) & ~3));
+
+  const int tid = threadIdx.x;
+  const int64_t w = blockIdx.x;
+  const int64_t qi = w / maxp;
+  const int p = (int)(w - qi * maxp);
+  uint64_t *dst = part + w * kk;
+  int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
+  np = min(max(np, 1), maxp);
+  const int l = p < np ? probes[qi * maxp + p] : -1;
+  if (l < 0) {
+    for (int t = tid; t < kk; t += kNT) dst[t] = kSentinel64;
+    return;
+  }
+  const int64_t off = list_off[l];
+  const int64_t L = list_off[l + 1] - off;
+  if (L == 0) {
+    for (int t = tid; t < kk; t += kNT) dst[t] = kSentinel64;
+    return;
+  }
+  const int dsub = d / M;
+  const float *qq = q + qi * d;
+  const float *cl = coarse + (int64_t)l * d;
+  for (int j = tid; j < d; j += kNT) rv[j] = IP ? qq[j] : qq[j] - cl[j];  // :306-310
+  __syncthreads();
+  for (int e = tid; e < M * kKsub; e += kNT) {
+    const int m = e >> 8;
+    const float *c = pq + (int64_t)e * dsub;
+    lut[e] = IP ? -ip_pq(rv + m * dsub, c, dsub) : l2_kmeans(rv + m * dsub, c, dsub);
+  }
+  if (IP && tid == 0) rv[d] = -ip_wn<8>(qq, cl, d);  // :322-324
+  __syncthreads();
+  const float coarse_d = IP ? rv[d] : 0.0f;
+
+  int nprev = 0;
+  for (int64_t cb = 0; cb < L; cb += kScanChunk) {
+    const int len = (int)min<int64_t>(kScanChunk, L - cb);
+    for (int t = tid; t < len; t += kNT) {
+      const int64_t pos = off + cb + t;
+      uint32_t key = kSentinel;
+      bool skip = false;
+      if (excl) {
+        const int32_t id = ids[pos];
+        skip = id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
+      }
+      if (!skip) key = f2key(adc_sum<MF>(codes + pos * M, lut, M, coarse_d));
+      keys[nprev + t] = key;
+    }
+    __syncthreads();
+    const int cnt = select_topk_u32(keys, nprev + len, kk, *scr, sel);
+    // Compact the winners to the front, mapping indices to list positions.
+    uint32_t kv[(kMaxCandidates + kNT - 1) / kNT];
+    uint32_t pv[(kMaxCandidates + kNT - 1) / kNT];
+    int nm = 0;
+    for (int t = tid; t < cnt; t += kNT, nm++) {
+      const uint32_t idx = sel[t];
+      kv[nm] = keys[idx];
+      pv[nm] = idx < (uint32_t)nprev ? prev_pos[idx] : (uint32_t)(cb + idx - nprev);
+    }
+    __syncthreads();
+    nm = 0;
+    for (int t = tid; t < cnt; t += kNT, nm++) {
+      keys[t] = kv[nm];
+      prev_pos[t] = pv[nm];
+    }
+    nprev = cnt;
+    __syncthreads();
+  }
+  for (int t = tid; t < kk; t += kNT) {
+    uint64_t v = kSentinel64;
+    if (t < nprev) v = ((uint64_t)keys[t] << 32) | (uint32_t)ids[off + prev_pos[t]];
+    dst[t] = v;
+  }
+}
+
+template <bool IP>
+__global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q, int d,
+                                                     const float *__restrict__ vectors,
+                                                     const uint64_t *__restrict__ cand, int kk,
+                                                     int k, int buf_n, float pad_dist,
+                                                     float *__restrict__ D,
+                                                     int64_t *__restrict__ I) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float *qs = reinterpret_cast<float *>(smem);
+  uint64_t *buf = reinterpret_cast<uint64_t *>(qs + ((d + 1) & ~1));
+  uint64_t *top = buf + buf_n;
+  SelectScratch *scr = reinterpret_cast<SelectScratch *>(top + buf_n);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t qi = blockIdx.x;
+  for (int j = tid; j < d; j += kNT) qs[j] = q[qi * d + j];
+  for (int t = tid; t < buf_n; t += kNT) buf[t] = kSentinel64;
+  __syncthreads();
+  const uint64_t *cq = cand + qi * kk;
+  for (int c = wave; c < kk; c += kNW) {
+    const uint64_t key = cq[c];
+    if (key == kSentinel64) continue;
+    const uint32_t id = (uint32_t)(key & 0xFFFFFFFFu);
+    const float ex = wave_dist_w64<IP>(qs, vectors + (int64_t)id * d, d, lane);
+    if (lane == 0) buf[c] = pack_key(IP ? -ex : ex, id);
+  }
+  __syncthreads();
+  const int cnt = select_sort_u64(buf, kk, k, *scr, top);
+  for (int t = tid; t < k; t += kNT) {
+    float dv = pad_dist;
+    int64_t iv = -1;
+    if (t < cnt) {
+      dv = key2f((uint32_t)(top[t] >> 32));
+      if (IP) dv = -dv;
+      iv = (int64_t)(uint32_t)(top[t] & 0xFFFFFFFFu);
+    }
+    D[qi * k + t] = dv;
+    I[qi * k + t] = iv;
+  }
+}
+
+__global__ void scanned_entries_kernel(const int32_t *__restrict__ probes, int64_t nq, int maxp,
+                                       const int32_t *__restrict__ nprobe_q, int nprobe_all,
+                                       const int64_t *__restrict__ list_off,
+                                       unsigned long long *__restrict__ total) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  if (t < nq * maxp) {
+    const int64_t qi = t / maxp;
+    const int p = (int)(t - qi * maxp);
+    int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
+    np = min(max(np, 1), maxp);
+    const int l = probes[t];
+    if (p < np && l >= 0) v = (unsigned long long)(list_off[l + 1] - list_off[l]);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(total, v);
+}
+
+template <int MF, bool IP>
+void launch_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk, uint64_t *part,
+                 hipStream_t s) {
+  const size_t sm = sizeof(float) * a.M * kKsub + sizeof(float) * ((a.d + 4) & ~3) +
+                    sizeof(uint32_t) * (kk + kScanChunk) + sizeof(uint32_t) * kk +
+                    sizeof(uint32_t) * ((kk + 3) & ~3) + sizeof(SelectScratch);
+  if (sm > 150 * 1024) throw Error(kInvalid, "IVF-PQ scan workspace exceeds LDS");
+  if (sm > 64 * 1024)
+    MVG_HIP(hipFuncSetAttribute((const void *)ivfpq_scan_kernel<MF, IP>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  const int64_t nblocks = a.nq * maxp;
+  hipLaunchKernelGGL((ivfpq_scan_kernel<MF, IP>), dim3((unsigned)nblocks), dim3(kNT), sm, s, a.q,
+                     a.d, a.M, a.coarse, a.pq, a.list_off, a.codes, a.ids, probes, maxp,
+                     a.nprobe_per_query, a.nprobe, kk, a.excl, a.excl_len, part);
+  MVG_LAUNCH_CHECK("ivfpq_scan_kernel");
+}
+
+}  // namespace
+
+void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
+            const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
+            hipStream_t s) {
+  if (nq == 0) return;
+  int buf_n = 1;
+  while (buf_n < kk || buf_n < k) buf_n <<= 1;
+  const size_t sm = sizeof(float) * ((d + 1) & ~1) + 2 * sizeof(uint64_t) * buf_n + sizeof(SelectScratch);
+  if (sm > 150 * 1024) throw Error(kInvalid, "rerank workspace exceeds LDS");
+  const bool ip = metric == kMetricIP;
+  const void *fn = ip ? (const void *)rerank_kernel<true> : (const void *)rerank_kernel<false>;
+  if (sm > 64 * 1024)
+    MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  if (ip)
+    hipLaunchKernelGGL(rerank_kernel<true>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d, vectors,
+                       cand, kk, k, buf_n, pad_dist, D, I);
+  else
+    hipLaunchKernelGGL(rerank_kernel<false>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d, vectors,
+                       cand, kk, k, buf_n, pad_dist, D, I);
+  MVG_LAUNCH_CHECK("rerank_kernel");
+}
+
+void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
+  if (a.nq == 0) return;
+  const int kf = a.k_factor > 1 ? a.k_factor : 1;
+  const int64_t kk64 = (int64_t)a.k * kf;
+  if (kk64 > kMaxCandidates) throw Error(kInvalid, "k * k_factor exceeds the device limit of 4096");
+  const int kk = (int)kk64;
+  if (kf > 1 && !a.vectors) throw Error(kState, "k_factor rerank needs the stored vectors");
+  // Largest probe count across the batch (the host clamps to nlist).
+  const int maxp = min(a.nprobe, a.nlist);
+  IvfProfile &pf = w.prof;
+  if (pf.on) {
+    for (auto &e : pf.ev)
+      if (!e) MVG_HIP(hipEventCreate(&e));
+    MVG_HIP(hipEventRecord(pf.ev[0], s));
+  }
+  // 1. coarse top-maxp (IndexFlat.search over the centroids, :264).
+  w.probes.reserve(sizeof(int32_t) * (size_t)a.nq * maxp);
+  int32_t *probes = w.probes.as<int32_t>();
+  flat_topk(a.coarse, a.nlist, a.d, a.metric, a.q, a.nq, maxp, nullptr, 0, w.coarse_part, 0.0f,
+            nullptr, nullptr, probes, nullptr, s);
+  // 2. per-(query, probe) LUT + ADC scan + list-local top-kk.
+  w.part.reserve(sizeof(uint64_t) * (size_t)a.nq * maxp * kk);
+  uint64_t *part = w.part.as<uint64_t>();
+  const bool ip = a.metric == kMetricIP;
+  if (pf.on) MVG_HIP(hipEventRecord(pf.ev[1], s));
+#define MVG_SCAN(MFV)                                                          \
+  if (ip) launch_scan<MFV, true>(a, maxp, probes, kk, part, s);                \
+  else launch_scan<MFV, false>(a, maxp, probes, kk, part, s);
+  switch (a.M) {
+    case 8: MVG_SCAN(8) break;
+    case 12: MVG_SCAN(12) break;
+    case 16: MVG_SCAN(16) break;
+    default: MVG_SCAN(0) break;
+  }
+#undef MVG_SCAN
+  if (pf.on) MVG_HIP(hipEventRecord(pf.ev[2], s));
+  // 3. per-query merge (and rerank).
+  if (kf == 1) {
+    launch_merge(part, a.nq, (int64_t)maxp * kk, a.k, ip ? 1 : 0, kIvfPadDist, a.D, a.I, nullptr,
+                 nullptr, s);
+  } else {
+    w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
+    launch_merge(part, a.nq, (int64_t)maxp * kk, kk, 0, kIvfPadDist, nullptr, nullptr, nullptr,
+                 w.keys.as<uint64_t>(), s);
+    rerank(a.q, a.nq, a.d, a.metric, a.vectors, w.keys.as<uint64_t>(), kk, a.k, kIvfPadDist, a.D,
+           a.I, s);
+  }
+  if (pf.on) {
+    MVG_HIP(hipEventRecord(pf.ev[3], s));
+    pf.counter.reserve(sizeof(unsigned long long));
+    MVG_HIP(hipMemsetAsync(pf.counter.ptr, 0, sizeof(unsigned long long), s));
+    const int64_t nt = a.nq * maxp;
+    hipLaunchKernelGGL(scanned_entries_kernel, dim3(ceil_div(nt, 256)), dim3(256), 0, s, probes, a.nq,
+                       maxp, a.nprobe_per_query, a.nprobe, a.list_off,
+                       pf.counter.as<unsigned long long>());
+    MVG_LAUNCH_CHECK("scanned_entries_kernel");
+    pf.recorded = true;
+  }
+}
+
+}  // namespace mvg

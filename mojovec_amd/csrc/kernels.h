@@ -1,0 +1,104 @@
+// Host-side launchers of the HIP kernels (one translation unit per family).
+#pragma once
+
+#include "common.h"
+
+namespace mvg {
+
+// ---- flat.hip ----
+// Canonical top-k of q against base rows (IndexFlat search). Outputs are any
+// subset of: distances (IP un-negated), int64 ids, int32 ids, u64 keys.
+void flat_topk(const float *base, int64_t n, int d, int metric, const float *q, int64_t nq, int k,
+               const uint8_t *excl, int64_t excl_len, DevBuf &part, float pad_dist, float *D,
+               int64_t *I64, int32_t *I32, uint64_t *keys, hipStream_t s);
+// Canonical argmin (k = 1); kmeans_mode restates the k-means scan rules.
+void flat_argmin(const float *base, int64_t n, int d, int metric, bool kmeans_mode, const float *q,
+                 int64_t nq, int32_t *out_id, float *out_dist, hipStream_t s);
+// Reduce ncand candidate keys per query to the sorted top-k.
+void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int negate,
+                  float pad_dist, float *D, int64_t *I64, int32_t *I32, uint64_t *keys,
+                  hipStream_t s);
+
+// ---- kmeans.hip ----
+struct KmeansWork {
+  DevBuf sums, counts, assign;
+};
+// Lloyd iterations over S independent subspaces (S = 1: coarse quantizer;
+// S = M: PQ codebooks) of dsub columns each, starting from `cent`
+// ([S][k][dsub], updated in place). x is [n][S * dsub] row-major.
+void kmeans_fit(const float *x, int64_t n, int S, int dsub, int k, int niter,
+                int64_t sample_offset, float *cent, KmeansWork &w, hipStream_t s);
+// Per (row, subspace) argmin over the 256 PQ centroids. If coarse/labels are
+// given, the residual x - coarse[label] is encoded (index_ivf_pq.mojo:181-192).
+void pq_assign(const float *x, int64_t n, int d, int M, const float *cb, const float *coarse,
+               const int32_t *labels, uint8_t *codes, int32_t *assign, hipStream_t s);
+// LUT[q][m][256] (pq.mojo:252-292) for nq query (or residual) vectors.
+void pq_lut(const float *q, int64_t nq, int d, int M, const float *cb, int metric, float *lut,
+            hipStream_t s);
+// x - coarse[label] row-wise.
+void residuals(const float *x, int64_t n, int d, const float *coarse, const int32_t *labels,
+               float *out, hipStream_t s);
+
+// ---- ivfpq.hip ----
+struct IvfSearchArgs {
+  const float *q;
+  int64_t nq;
+  int d, M, nlist, metric;
+  const float *coarse, *pq;
+  const int64_t *list_off;
+  const uint8_t *codes;
+  const int32_t *ids;
+  const float *vectors;  // may be null when no rerank is requested
+  int k, k_factor, nprobe;
+  const int32_t *nprobe_per_query;  // device, nullable
+  const uint8_t *excl;
+  int64_t excl_len;
+  float *D;
+  int64_t *I;
+};
+// Opt-in stage timing: events recorded on the search stream around the
+// coarse, scan and merge/rerank stages, and the number of list entries the
+// scan visited (sum of probed list lengths).
+struct IvfProfile {
+  bool on = false;
+  bool recorded = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  DevBuf counter;
+};
+struct IvfWork {
+  DevBuf coarse_part, probes, part, keys;
+  IvfProfile prof;
+};
+void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
+
+// Exact FP32 rerank of kk sorted candidate keys per query to the top-k
+// (index_hnsw.mojo:449-496 generalised to k_factor).
+void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
+            const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
+            hipStream_t s);
+
+// ---- sq8.hip ----
+void sq8_minmax(const float *x, int64_t count, float *out2, DevBuf &tmp, hipStream_t s);
+void sq8_encode(const float *x, int64_t n, int d, int symmetric, float minimum, float inv_scale,
+                uint8_t *codes, uint32_t *norms, hipStream_t s);
+struct Sq8SearchArgs {
+  const float *q;
+  int64_t nq;
+  int d, metric;
+  int64_t n;
+  const uint8_t *codes;
+  const uint32_t *norms;
+  const float *vectors;
+  float gmin, scale;
+  int k, k_factor;
+  const uint8_t *excl;
+  int64_t excl_len;
+  float *D;
+  int64_t *I;
+};
+struct Sq8Work {
+  DevBuf qcodes, qnorms, part, keys;
+};
+void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s);
+
+}  // namespace mvg

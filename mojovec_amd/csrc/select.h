@@ -1,0 +1,229 @@
+// Workgroup-level top-k selection in LDS.
+//
+// Candidates are ordered canonically: by distance, exact ties broken by the
+// lower id (the reference's heap, mojovec/utils/heap.mojo, keeps the same
+// multiset of distances but is not tie-stable; the canonical order is the
+// build's documented tie rule). Distances become order-preserving u32 keys
+// (f2key); within one inverted list or one base chunk the position order is
+// the id order, so (key, position) is the canonical order there. Across
+// lists the ids travel with the key as a u64 (key << 32 | id).
+//
+// Selection is a most-significant-digit radix select: a few LDS histogram
+// passes find the k-th key, then an order-preserving compaction emits the
+// winners. Cost is linear in the candidate count; no sorting network runs on
+// the full candidate set.
+#pragma once
+
+#include "common.h"
+
+namespace mvg {
+
+constexpr int kNT = 256;  // threads per workgroup for every select user
+constexpr int kNW = kNT / 64;
+
+// Exclusive scan of one int per thread over the workgroup; `sh` needs kNW+1
+// ints. All threads must call it.
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kNW; w++) {
+    int t = sh[w];
+    if (w < wave) off += t;
+    tot += t;
+  }
+  __syncthreads();
+  total = tot;
+  return off + x - v;
+}
+
+__device__ __forceinline__ int block_sum(int v, int *sh) {
+  int total;
+  (void)block_excl_scan(v, sh, total);
+  return total;
+}
+
+// Scratch for the selects: a 2048-bin histogram and a few scalars.
+struct SelectScratch {
+  int hist[2048];
+  int sh[16];
+};
+
+// k-th smallest valid u32 key (1 <= k <= number of valid keys). Returns the
+// key T and how many keys equal to T belong to the k smallest.
+__device__ void radix_kth_u32(const uint32_t *keys, int n, int k, SelectScratch &s,
+                              uint32_t &T, int &need_eq) {
+  const int tid = threadIdx.x;
+  uint32_t prefix = 0, mask = 0;
+  int remaining = k;
+  const int shifts[3] = {21, 10, 0};
+  const int nbits[3] = {11, 11, 10};
+  for (int p = 0; p < 3; p++) {
+    const int nb = 1 << nbits[p];
+    const uint32_t dm = (uint32_t)nb - 1u;
+    for (int b = tid; b < nb; b += kNT) s.hist[b] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += kNT) {
+      uint32_t key = keys[i];
+      if (key != kSentinel && (key & mask) == prefix)
+        atomicAdd(&s.hist[(key >> shifts[p]) & dm], 1);
+    }
+    __syncthreads();
+    const int per = nb / kNT;
+    int local = 0;
+    for (int b = 0; b < per; b++) local += s.hist[tid * per + b];
+    int total;
+    int excl = block_excl_scan(local, s.sh, total);
+    if (excl < remaining && remaining <= excl + local) {
+      int c = excl;
+      for (int b = 0; b < per; b++) {
+        int h = s.hist[tid * per + b];
+        if (c + h >= remaining) {
+          s.sh[8] = tid * per + b;
+          s.sh[9] = c;
+          break;
+        }
+        c += h;
+      }
+    }
+    __syncthreads();
+    const int bin = s.sh[8];
+    remaining -= s.sh[9];
+    prefix |= (uint32_t)bin << shifts[p];
+    mask |= dm << shifts[p];
+    __syncthreads();
+  }
+  T = prefix;
+  need_eq = remaining;
+}
+
+// Selects the min(k, valid) smallest keys of keys[0..n) under (key, index)
+// order and writes their indices, in index order, to out[]. Returns the
+// number written. Sentinel keys are never selected.
+__device__ int select_topk_u32(const uint32_t *keys, int n, int k, SelectScratch &s,
+                               uint32_t *out) {
+  const int tid = threadIdx.x;
+  const int per = (n + kNT - 1) / kNT;
+  const int beg = min(tid * per, n), end = min(beg + per, n);
+  int valid = 0;
+  for (int i = beg; i < end; i++) valid += keys[i] != kSentinel;
+  const int nvalid = block_sum(valid, s.sh);
+  const bool take_all = nvalid <= k;
+  uint32_t T = 0;
+  int need_eq = 0;
+  if (!take_all) radix_kth_u32(keys, n, k, s, T, need_eq);
+  int lt = 0, eq = 0;
+  for (int i = beg; i < end; i++) {
+    uint32_t key = keys[i];
+    if (key == kSentinel) continue;
+    if (take_all || key < T) lt++;
+    else if (key == T) eq++;
+  }
+  int tot;
+  const int eq_before = block_excl_scan(eq, s.sh, tot);
+  const int my_eq = max(0, min(eq, need_eq - eq_before));
+  int total;
+  int off = block_excl_scan(lt + my_eq, s.sh, total);
+  int e = eq_before;
+  for (int i = beg; i < end; i++) {
+    uint32_t key = keys[i];
+    if (key == kSentinel) continue;
+    if (take_all || key < T) {
+      out[off++] = (uint32_t)i;
+    } else if (key == T) {
+      if (e < need_eq) out[off++] = (uint32_t)i;
+      e++;
+    }
+  }
+  __syncthreads();
+  return total;
+}
+
+// k-th smallest of unique u64 keys (1 <= k <= number of valid keys).
+__device__ uint64_t radix_kth_u64(const uint64_t *keys, int n, int k, SelectScratch &s) {
+  const int tid = threadIdx.x;
+  uint64_t prefix = 0, mask = 0;
+  int remaining = k;
+  for (int p = 0; p < 8; p++) {
+    const int shift = 56 - 8 * p;
+    s.hist[tid] = 0;  // 256 bins == kNT
+    __syncthreads();
+    for (int i = tid; i < n; i += kNT) {
+      uint64_t key = keys[i];
+      if (key != kSentinel64 && (key & mask) == prefix)
+        atomicAdd(&s.hist[(int)((key >> shift) & 0xFF)], 1);
+    }
+    __syncthreads();
+    const int local = s.hist[tid];
+    int total;
+    const int excl = block_excl_scan(local, s.sh, total);
+    if (excl < remaining && remaining <= excl + local) {
+      s.sh[8] = tid;
+      s.sh[9] = excl;
+    }
+    __syncthreads();
+    const int bin = s.sh[8];
+    remaining -= s.sh[9];
+    prefix |= (uint64_t)bin << shift;
+    mask |= (uint64_t)0xFF << shift;
+    __syncthreads();
+  }
+  return prefix;
+}
+
+// Ascending bitonic sort of keys[0..n) in LDS, n a power of two.
+__device__ void bitonic_sort_u64(uint64_t *keys, int n) {
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n; i += kNT) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          uint64_t a = keys[i], b = keys[ixj];
+          bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Writes the min(k, valid) smallest unique u64 keys of in[0..n) to out[] in
+// ascending order; out must hold next_pow2(k) entries and must not overlap
+// in. Returns the count.
+__device__ int select_sort_u64(const uint64_t *in, int n, int k, SelectScratch &s,
+                               uint64_t *out) {
+  const int tid = threadIdx.x;
+  int valid = 0;
+  for (int i = tid; i < n; i += kNT) valid += in[i] != kSentinel64;
+  const int nvalid = block_sum(valid, s.sh);
+  const int cnt = min(k, nvalid);
+  uint64_t T = kSentinel64 - 1;
+  if (nvalid > k) T = radix_kth_u64(in, n, k, s);
+  if (tid == 0) s.sh[10] = 0;
+  __syncthreads();
+  for (int i = tid; i < n; i += kNT) {
+    const uint64_t key = in[i];
+    if (key != kSentinel64 && key <= T) out[atomicAdd(&s.sh[10], 1)] = key;
+  }
+  __syncthreads();
+  int np2 = 1;
+  while (np2 < cnt) np2 <<= 1;
+  for (int i = cnt + tid; i < np2; i += kNT) out[i] = kSentinel64;
+  __syncthreads();
+  bitonic_sort_u64(out, np2);
+  return cnt;
+}
+
+}  // namespace mvg

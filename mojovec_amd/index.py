@@ -1,0 +1,271 @@
+"""Index-level wrappers over the C ABI (the reference's Mojo index layer).
+
+Mirrors of mojovec/index/index_flat.mojo (IndexFlat), index_flat_sq8.mojo
+(IndexFlatSQ8), index_ivf_pq.mojo (IndexIVFPQ), clustering/kmeans.mojo (KMeans)
+and quantization/pq.mojo (ProductQuantizer). Arrays are numpy; results come
+back as (distances float32 [nq, k], labels int64 [nq, k]).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+METRIC_L2 = 0
+METRIC_INNER_PRODUCT = 1
+KMEANS_SEED = 0x4D4F4A4F564543  # kmeans.mojo:70
+
+
+def _f32(x, d=None):
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    if d is not None:
+        a = a.reshape(-1, d)
+    return a
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _mask(exclude):
+    if exclude is None:
+        return None, 0
+    m = np.ascontiguousarray(exclude, dtype=np.uint8)
+    return m, m.size
+
+
+class IndexFlat:
+    """Exact brute-force index (IndexFlat, index_flat.mojo:117-330)."""
+
+    def __init__(self, d: int, metric: int = METRIC_L2):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.mvg_flat_create(d, metric, ctypes.byref(h)))
+        self._h, self.d, self.metric_type = h, d, metric
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.load().mvg_flat_destroy(self._h)
+            self._h = None
+
+    @property
+    def ntotal(self) -> int:
+        return int(_lib.load().mvg_flat_ntotal(self._h))
+
+    def add(self, x) -> None:
+        x = _f32(x, self.d)
+        _lib.check(_lib.load().mvg_flat_add(self._h, _ptr(x), x.shape[0]))
+
+    def search(self, x, k: int, exclude=None):
+        x = _f32(x, self.d)
+        nq = x.shape[0]
+        D = np.empty((nq, k), np.float32)
+        I = np.empty((nq, k), np.int64)
+        m, ml = _mask(exclude)
+        _lib.check(_lib.load().mvg_flat_search(self._h, _ptr(x), nq, k, _ptr(m), ml, _ptr(D), _ptr(I)))
+        return D, I
+
+
+class IndexFlatSQ8:
+    """Global-calibration SQ8 index (IndexFlatSQ8, index_flat_sq8.mojo:192-582)."""
+
+    def __init__(self, d: int, metric: int = METRIC_L2):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.mvg_sq8_create(d, metric, ctypes.byref(h)))
+        self._h, self.d, self.metric_type = h, d, metric
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.load().mvg_sq8_destroy(self._h)
+            self._h = None
+
+    def add(self, x) -> None:
+        x = _f32(x, self.d)
+        _lib.check(_lib.load().mvg_sq8_add(self._h, _ptr(x), x.shape[0]))
+
+    def params(self):
+        a, b, s = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        n = ctypes.c_int64()
+        _lib.check(_lib.load().mvg_sq8_params(self._h, ctypes.byref(a), ctypes.byref(b),
+                                               ctypes.byref(s), ctypes.byref(n)))
+        return a.value, b.value, s.value, n.value
+
+    @property
+    def ntotal(self) -> int:
+        return self.params()[3]
+
+    @property
+    def scale(self) -> float:
+        return self.params()[2]
+
+    def codes(self):
+        n = self.ntotal
+        c = np.empty((n, self.d), np.uint8)
+        nr = np.empty(n, np.uint32)
+        _lib.check(_lib.load().mvg_sq8_get_codes(self._h, _ptr(c), _ptr(nr)))
+        return c, nr
+
+    def search(self, x, k: int, k_factor: int = 1, exclude=None):
+        x = _f32(x, self.d)
+        nq = x.shape[0]
+        D = np.empty((nq, k), np.float32)
+        I = np.empty((nq, k), np.int64)
+        m, ml = _mask(exclude)
+        _lib.check(_lib.load().mvg_sq8_search(self._h, _ptr(x), nq, k, k_factor, _ptr(m), ml,
+                                               _ptr(D), _ptr(I)))
+        return D, I
+
+
+class KMeans:
+    """Lloyd k-means (KMeans, kmeans.mojo:8-220) on the GPU."""
+
+    def __init__(self, d: int, k: int, niter: int = 15, seed: int = KMEANS_SEED):
+        self.d, self.k, self.niter, self.seed = d, k, niter, seed
+        self.centroids = None
+        self.assignments = None
+
+    def train(self, x, init=None, sample_offset: int = 0):
+        x = _f32(x, self.d)
+        c = np.empty((self.k, self.d), np.float32)
+        a = np.empty(x.shape[0], np.int32)
+        ini = None if init is None else _f32(init, self.d)
+        _lib.check(_lib.load().mvg_kmeans_train(_ptr(x), x.shape[0], self.d, self.k, self.niter,
+                                                 _ptr(ini), sample_offset, self.seed, _ptr(c), _ptr(a)))
+        self.centroids, self.assignments = c, a
+        return c
+
+
+class ProductQuantizer:
+    """Product quantizer with 8-bit codes (ProductQuantizer, pq.mojo:80-292)."""
+
+    def __init__(self, d: int, M: int, niter: int = 15, seed: int = KMEANS_SEED):
+        if d <= 0:
+            raise _lib.MojoVecError(1, "PQ dimension must be positive.")
+        if M <= 0 or M > d or d % M != 0:
+            raise _lib.MojoVecError(1, "PQ M must divide dimension and be between 1 and dimension.")
+        self.d, self.M, self.ksub, self.dsub = d, M, 256, d // M
+        self.niter, self.seed = niter, seed
+        self.centroids = None
+        self.is_trained = False
+
+    def train(self, x):
+        x = _f32(x, self.d)
+        cb = np.empty((self.M, 256, self.dsub), np.float32)
+        _lib.check(_lib.load().mvg_pq_train(_ptr(x), x.shape[0], self.d, self.M, self.niter,
+                                             self.seed, _ptr(cb)))
+        self.centroids, self.is_trained = cb, True
+
+    def compute_codes(self, x):
+        x = _f32(x, self.d)
+        codes = np.empty((x.shape[0], self.M), np.uint8)
+        cb = np.ascontiguousarray(self.centroids, np.float32)
+        _lib.check(_lib.load().mvg_pq_encode(_ptr(x), x.shape[0], self.d, self.M, _ptr(cb), _ptr(codes)))
+        return codes
+
+    def decode(self, codes):
+        codes = np.asarray(codes, np.uint8).reshape(-1, self.M)
+        return np.concatenate([self.centroids[m][codes[:, m]] for m in range(self.M)], axis=1)
+
+    def compute_distance_table(self, q, metric: int = METRIC_L2):
+        q = _f32(q, self.d)
+        lut = np.empty((q.shape[0], self.M, 256), np.float32)
+        cb = np.ascontiguousarray(self.centroids, np.float32)
+        _lib.check(_lib.load().mvg_pq_lut(_ptr(q), q.shape[0], self.d, self.M, _ptr(cb), metric, _ptr(lut)))
+        return lut
+
+
+class IndexIVFPQ:
+    """IVF-PQ index with an HBM-resident CSR list layout (IndexIVFPQ,
+    index_ivf_pq.mojo:12-360)."""
+
+    def __init__(self, d: int, nlist: int, M: int, metric: int = METRIC_L2, keep_vectors: bool = True,
+                 _handle=None):
+        L = _lib.load()
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _lib.check(L.mvg_ivfpq_create(d, nlist, M, metric, ctypes.byref(h)))
+            _lib.check(L.mvg_ivfpq_set_keep_vectors(h, int(keep_vectors)))
+            self._owned = True
+        else:
+            h = ctypes.c_void_p(_handle)
+            self._owned = False
+        self._h, self.d, self.nlist, self.M, self.metric_type = h, d, nlist, M, metric
+
+    def __del__(self):
+        if getattr(self, "_h", None) and getattr(self, "_owned", False):
+            _lib.load().mvg_ivfpq_destroy(self._h)
+            self._h = None
+
+    def _info(self):
+        n, t, p = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().mvg_ivfpq_info(self._h, ctypes.byref(n), ctypes.byref(t), ctypes.byref(p)))
+        return n.value, bool(t.value), p.value
+
+    @property
+    def ntotal(self) -> int:
+        return self._info()[0]
+
+    @property
+    def is_trained(self) -> bool:
+        return self._info()[1]
+
+    @property
+    def nprobe(self) -> int:
+        return self._info()[2]
+
+    @nprobe.setter
+    def nprobe(self, v: int) -> None:
+        _lib.check(_lib.load().mvg_ivfpq_set_nprobe(self._h, int(v)))
+
+    def train(self, x, niter_coarse: int = 15, niter_pq: int = 15, seed: int = KMEANS_SEED) -> None:
+        x = _f32(x, self.d)
+        _lib.check(_lib.load().mvg_ivfpq_train(self._h, _ptr(x), x.shape[0], niter_coarse, niter_pq, seed))
+
+    def set_codebooks(self, coarse, pq) -> None:
+        c = _f32(coarse, self.d)
+        p = np.ascontiguousarray(pq, np.float32)
+        _lib.check(_lib.load().mvg_ivfpq_set_codebooks(self._h, _ptr(c), _ptr(p)))
+
+    def codebooks(self):
+        c = np.empty((self.nlist, self.d), np.float32)
+        p = np.empty((self.M, 256, self.d // self.M), np.float32)
+        _lib.check(_lib.load().mvg_ivfpq_get_codebooks(self._h, _ptr(c), _ptr(p)))
+        return c, p
+
+    def add(self, x) -> None:
+        x = _f32(x, self.d)
+        _lib.check(_lib.load().mvg_ivfpq_add(self._h, _ptr(x), x.shape[0]))
+
+    def list_sizes(self):
+        s = np.empty(self.nlist, np.int64)
+        _lib.check(_lib.load().mvg_ivfpq_list_sizes(self._h, _ptr(s)))
+        return s
+
+    def get_list(self, l: int):
+        n = int(self.list_sizes()[l])
+        ids = np.empty(n, np.int64)
+        codes = np.empty((n, self.M), np.uint8)
+        _lib.check(_lib.load().mvg_ivfpq_get_list(self._h, l, _ptr(ids), _ptr(codes)))
+        return ids, codes
+
+    def search(self, x, k: int, k_factor: int = 1, nprobe_per_query=None, exclude=None):
+        x = _f32(x, self.d)
+        nq = x.shape[0]
+        D = np.empty((nq, k), np.float32)
+        I = np.empty((nq, k), np.int64)
+        npq = None if nprobe_per_query is None else np.ascontiguousarray(nprobe_per_query, np.int32)
+        m, ml = _mask(exclude)
+        _lib.check(_lib.load().mvg_ivfpq_search(self._h, _ptr(x), nq, k, _ptr(npq), k_factor,
+                                                 _ptr(m), ml, _ptr(D), _ptr(I)))
+        return D, I
+
+    def search_device(self, q_ptr: int, nq: int, k: int, D_ptr: int, I_ptr: int, k_factor: int = 1,
+                      nprobe_ptr: int = 0, max_nprobe: int = 0, stream: int = 0) -> None:
+        """Search with every buffer already in device memory (raw pointers)."""
+        _lib.check(_lib.load().mvg_ivfpq_search_device(
+            self._h, ctypes.c_void_p(q_ptr), nq, k, ctypes.c_void_p(nprobe_ptr or None), max_nprobe,
+            k_factor, None, 0, ctypes.c_void_p(D_ptr), ctypes.c_void_p(I_ptr),
+            ctypes.c_void_p(stream or None)))

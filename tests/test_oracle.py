@@ -1,0 +1,439 @@
+"""Pins the CPU oracle (oracle/mvref.c) before it is trusted as the checker.
+
+Each check compares the oracle with something that does not share its code:
+exact integer arithmetic, float64 numpy, the reference tests' known answers
+(tests/golden, from MojoVec's tests/*.mojo) or invariants the mathematics
+fixes. Where the reference leaves an arithmetic detail to the (unvendored)
+Mojo stdlib -- reduce_add lane order, round() ties, the Philox stream -- the
+test says so and pins the restatement's documented choice only.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rint(rng, shape, lo=0, hi=256):
+    return rng.integers(lo, hi, size=shape).astype(np.float32)
+
+
+# ---------------------------------------------------------------- distances
+@pytest.mark.parametrize("W", [4, 8, 64])
+@pytest.mark.parametrize("d", [1, 3, 4, 7, 8, 13, 31, 32, 33, 96, 127, 128, 257, 768])
+def test_l2_ip_exact_on_integer_data(W, d):
+    # Every order of FP32 operations is exact here (d * 255^2 < 2^24 and
+    # |dot| < 2^24), so a dropped, doubled or mis-indexed term shows up.
+    rng = np.random.default_rng(d * 100 + W)
+    x = rint(rng, (20, d))
+    y = rint(rng, (20, d))
+    exact = ((x.astype(np.int64) - y.astype(np.int64)) ** 2).sum(1)
+    assert np.array_equal(o.l2_simd_batch(x, y, W), exact.astype(np.float32))
+    a = rint(rng, (20, d), -15, 16)
+    b = rint(rng, (20, d), -15, 16)
+    ip = (a.astype(np.int64) * b.astype(np.int64)).sum(1)
+    assert np.array_equal(o.ip_simd_batch(a, b, W), ip.astype(np.float32))
+
+
+@pytest.mark.parametrize("d", [1, 3, 7, 13, 15, 27, 31, 33, 127, 4, 8, 16, 32, 64])
+def test_distances_match_float64_like_reference_test(d):
+    # tests/test_distances_simd.mojo:25-65 (atol 1e-4 against a scalar loop).
+    rng = np.random.default_rng(d)
+    x = (rng.random(d) * 20 - 10).astype(np.float32)
+    y = (rng.random(d) * 20 - 10).astype(np.float32)
+    l2 = float(((x.astype(np.float64) - y) ** 2).sum())
+    ip = float((x.astype(np.float64) * y).sum())
+    assert abs(o.l2_simd(x, y, 4) - l2) <= 1e-4 * max(1.0, abs(l2))
+    assert abs(o.ip_simd(x, y, 4) - ip) <= 1e-4 * max(1.0, abs(ip))
+
+
+def test_sub_simd_accumulators_are_zero():
+    # tests/test_distances_simd.mojo:68-84
+    x = np.array([1, 2, 3, 4], np.float32)
+    y = np.arange(4, dtype=np.float32) * 0.25
+    assert abs(o.l2_simd(x, y, 8) - float(((x - y) ** 2).sum())) < 1e-6
+    assert abs(o.ip_simd(x, y, 8) - float((x * y).sum())) < 1e-6
+
+
+def _tree(v):
+    v = list(v)
+    h = len(v) // 2
+    while h >= 1:
+        v = [np.float32(v[j] + v[j + h]) for j in range(h)]
+        h //= 2
+    return v[0]
+
+
+def _l2_w_order(x, y, W):
+    # Independent numpy statement of the documented order (four W-lane FMA
+    # accumulators, lane-wise ((a0+a1)+a2)+a3, halving tree). This pins the
+    # C code to the documented order; the order itself is the restatement's
+    # choice for Mojo's SIMD.reduce_add (parity unpinned against Mojo).
+    x = x.astype(np.float32)
+    y = y.astype(np.float32)
+    d = x.size
+    acc = [np.zeros(W, np.float32) for _ in range(4)]
+    i = 0
+    while i <= d - 4 * W:
+        for t in range(4):
+            df = (x[i + t * W:i + (t + 1) * W] - y[i + t * W:i + (t + 1) * W]).astype(np.float64)
+            acc[t] = (df * df + acc[t].astype(np.float64)).astype(np.float32)  # exact fma
+        i += 4 * W
+    v = ((acc[0] + acc[1]) + acc[2]) + acc[3]
+    while i <= d - W:
+        df = (x[i:i + W] - y[i:i + W]).astype(np.float64)
+        v = (df * df + v.astype(np.float64)).astype(np.float32)
+        i += W
+    assert i == d
+    return _tree(v)
+
+
+@pytest.mark.parametrize("W,d", [(8, 128), (8, 96), (64, 768), (64, 128), (8, 8)])
+def test_documented_reduction_order(W, d):
+    rng = np.random.default_rng(W + d)
+    x = rng.standard_normal((8, d)).astype(np.float32)
+    y = rng.standard_normal((8, d)).astype(np.float32)
+    got = o.l2_simd_batch(x, y, W)
+    want = np.array([_l2_w_order(x[i], y[i], W) for i in range(8)], np.float32)
+    assert np.array_equal(got, want)
+
+
+def test_short4_tree():
+    x = np.array([1e8, 1.0, -1e8, 1.0], np.float32)
+    y = np.zeros(4, np.float32)
+    # (r0 + r2) + (r1 + r3) on squares: 1e16+1e16 then +2
+    r = (x.astype(np.float32) ** 2)
+    want = np.float32(np.float32(r[0] + r[2]) + np.float32(r[1] + r[3]))
+    assert o.l2_short4(x, y) == want
+
+
+def test_int8_and_uint8_dots_match_reference_patterns():
+    for name, fn in (("i8_dot.npz", o.dot_i8), ("u8_dot.npz", o.dot_u8)):
+        g = np.load(os.path.join(GOLDEN, name))
+        off = 0
+        for d, e in zip(g["dims"], g["expected"]):
+            assert fn(g["x"][off:off + d], g["y"][off:off + d]) == e
+            off += d
+
+
+def test_l2_from_dot_clamps_at_zero():
+    assert o.l2_from_dot(1, 1, 5) == 0
+    assert o.l2_from_dot(100, 50, 20) == 110
+
+
+def test_uint8_rebase_identity():
+    # sum u_q u_b = sum s_q s_b + 128 (sum s_q + sum s_b) + 16384 d, s = u - 128
+    rng = np.random.default_rng(3)
+    for d in (1, 5, 64, 768):
+        u = rng.integers(0, 256, (2, d)).astype(np.uint8)
+        s = (u.astype(np.int64) - 128).astype(np.int8)
+        lhs = o.dot_u8(u[0], u[1])
+        rhs = o.dot_i8(s[0], s[1]) + 128 * (int(s[0].astype(np.int64).sum()) + int(s[1].astype(np.int64).sum())) + 16384 * d
+        assert lhs == rhs
+
+
+# ---------------------------------------------------------------- top-k
+def test_reference_heap_is_not_tie_stable_but_canonical_is():
+    d, i = o.heap_topk_reference(np.array([5, 5, 1], np.float32), 2)
+    assert sorted(i.tolist()) == [1, 2]
+    d, i = o.topk_canonical(np.array([5, 5, 1], np.float32), 2)
+    assert i.tolist() == [2, 0]
+
+
+def test_canonical_topk_matches_lexsort_and_reference_distance_multiset():
+    rng = np.random.default_rng(1)
+    for trial in range(200):
+        n = int(rng.integers(1, 60))
+        k = int(rng.integers(1, 20))
+        dist = rng.integers(0, 7, n).astype(np.float32)
+        cd, ci = o.topk_canonical(dist, k)
+        order = np.lexsort((np.arange(n), dist))[:k]
+        assert ci.tolist() == order.tolist()
+        rd, ri = o.heap_topk_reference(dist, k)
+        assert np.array_equal(np.sort(rd), np.sort(cd))
+
+
+# ---------------------------------------------------------------- flat
+def test_flat_known_answers():
+    g = json.load(open(os.path.join(GOLDEN, "flat_known.json")))
+    for key, metric in (("l2", o.L2), ("ip", o.IP)):
+        c = g[key]
+        D, I = o.flat_search(np.array(c["base"], np.float32), np.array([c["query"]], np.float32),
+                             c["k"], metric)
+        assert I[0].tolist() == c["expected_ids"]
+        assert np.allclose(D[0], c["expected_dist"], atol=c["atol"])
+
+
+def test_flat_canonical_ties_and_exclusion():
+    g = np.load(os.path.join(GOLDEN, "ties.npz"))
+    k = int(g["k"])
+    D, I = o.flat_search(g["base"], g["q"], k)
+    assert np.array_equal(I, g["ids"])
+    assert np.array_equal(D, g["dist"])
+    ex = np.zeros(g["base"].shape[0], np.uint8)
+    ex[g["ids"][0, 0]] = 1
+    D2, I2 = o.flat_search(g["base"], g["q"][:1], k, exclude=ex)
+    assert g["ids"][0, 0] not in I2[0]
+    assert I2[0].tolist() == [i for i in g["ids"][0].tolist() if i != g["ids"][0, 0]][: k - 1] + [I2[0, -1]]
+
+
+def test_flat_pads_when_k_exceeds_n():
+    D, I = o.flat_search(np.ones((3, 4), np.float32), np.zeros((1, 4), np.float32), 5)
+    assert I[0, 3:].tolist() == [-1, -1] and D[0, 3:].tolist() == [0.0, 0.0]
+
+
+# ---------------------------------------------------------------- k-means
+def _lloyd_step64(x, c):
+    d2 = ((x[:, None, :].astype(np.float64) - c[None, :, :]) ** 2).sum(-1)
+    a = d2.argmin(1)
+    out = c.copy()
+    for j in range(c.shape[0]):
+        m = a == j
+        if m.any():
+            s = x[m].astype(np.float64).sum(0)
+            out[j] = (s.astype(np.float32) * np.float32(np.float32(1.0) / np.float32(m.sum())))
+    return a, out
+
+
+def test_kmeans_iteration_matches_lloyd_closed_form():
+    rng = np.random.default_rng(5)
+    x = rint(rng, (500, 12), 0, 50)
+    init = x[rng.choice(500, 7, replace=False)].copy()
+    for it in (1, 2, 3):
+        c, a = o.kmeans_train(x, 7, it, init=init, sample_offset=0)
+        ref_c = init.copy()
+        for _ in range(it):
+            ref_a, ref_c = _lloyd_step64(x, ref_c)
+        # integer data: exact distances and exact sums, so bitwise equal
+        assert np.array_equal(c, ref_c)
+
+
+def test_kmeans_fp64_and_reference_fp32_sums_agree_on_integer_data():
+    rng = np.random.default_rng(6)
+    x = rint(rng, (2000, 16))
+    init, off = o.kmeans_init(x, 20)
+    c64, a64 = o.kmeans_train(x, 20, 5, init=init, sample_offset=off, fp64_sums=True)
+    c32, a32 = o.kmeans_train(x, 20, 5, init=init, sample_offset=off, fp64_sums=False)
+    assert np.array_equal(c64, c32) and np.array_equal(a64, a32)
+
+
+def test_kmeans_empty_cluster_reseed_rule():
+    x = np.arange(40, dtype=np.float32).reshape(20, 2)
+    init = np.array([[0, 1], [1e6, 1e6]], np.float32)  # cluster 1 stays empty
+    c, a = o.kmeans_train(x, 2, 1, init=init, sample_offset=3)
+    src = (3 + 1 * 2 + 1) % 20
+    assert np.array_equal(c[1], x[src])
+    assert (a == 0).all()
+
+
+def test_kmeans_identical_points_k_gt_n_and_reproducible():
+    # tests/test_kmeans.mojo:22-72
+    c, _ = o.kmeans_train(np.ones((100, 16), np.float32), 5, 5)
+    assert not np.isnan(c).any()
+    c, _ = o.kmeans_train(np.arange(80, dtype=np.float32).reshape(5, 16), 10, 5)
+    assert c.shape == (10, 16)
+    i = np.arange(257 * 8)
+    x = (((i * 37 + i // 8 * 11) % 101) / 50.0 - 1.0).astype(np.float32).reshape(257, 8)
+    c1, _ = o.kmeans_train(x, 17, 5)
+    c2, _ = o.kmeans_train(x, 17, 5)
+    assert np.array_equal(c1, c2)
+
+
+def test_philox_known_answer():
+    # Philox-4x32-10 known-answer vectors (Random123 kat_vectors):
+    # counter 0, key 0 -> 6627e8d5 e169c58d bc57ac4c 9b00dbd8
+    out = np.zeros(4, np.uint32)
+    o.lib().mvref_philox4x32_10(0, 0, out)
+    assert [hex(v) for v in out] == ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+
+
+# ---------------------------------------------------------------- PQ
+def _uniform(seed, n, d):
+    return (np.random.default_rng(seed).random((n, d)) * 2 - 1).astype(np.float32)
+
+
+def test_pq_reconstruction_mse_and_adc_identity():
+    # tests/test_pq.mojo:8-114
+    x = _uniform(181, 1000, 16)
+    cb = o.pq_train(x, 4)
+    codes = o.pq_encode(x, cb)
+    dec = o.pq_decode(codes, cb)
+    mse = float(((x.astype(np.float64) - dec) ** 2).sum(1).mean())
+    assert mse < 1.0
+    lut = o.pq_lut(x[0], cb)
+    adc = np.float32(0)
+    for m in range(4):
+        adc = np.float32(adc + lut[m, codes[0, m]])
+    assert abs(float(adc) - float(((x[0].astype(np.float64) - dec[0]) ** 2).sum())) < 1e-5
+
+
+def test_pq_encode_is_bruteforce_argmin():
+    x = _uniform(3, 400, 32)
+    cb = o.pq_train(x, 4, niter=3)
+    codes = o.pq_encode(x, cb)
+    for m in range(4):
+        sub = x[:, m * 8:(m + 1) * 8].astype(np.float64)
+        d2 = ((sub[:, None, :] - cb[m][None, :, :]) ** 2).sum(-1)
+        best = d2.argmin(1)
+        srt = np.sort(d2, 1)
+        clear = (srt[:, 1] - srt[:, 0]) > 1e-4
+        assert np.array_equal(codes[clear, m], best[clear])
+
+
+def test_pq_lut_l2_and_ip():
+    x = _uniform(4, 300, 24)
+    cb = o.pq_train(x, 3, niter=2)
+    q = _uniform(5, 1, 24)[0]
+    l2 = o.pq_lut(q, cb, o.L2)
+    ip = o.pq_lut(q, cb, o.IP)
+    for m in range(3):
+        sub = q[m * 8:(m + 1) * 8].astype(np.float64)
+        assert np.allclose(l2[m], ((cb[m] - sub) ** 2).sum(1), rtol=1e-5, atol=1e-6)
+        assert np.allclose(ip[m], -(cb[m] @ sub), rtol=1e-5, atol=1e-6)
+
+
+# ---------------------------------------------------------------- SQ8
+def _round_half_away(v):
+    return np.sign(v) * np.floor(np.abs(v) + 0.5)
+
+
+def test_sq8_encode_formula_and_norms():
+    rng = np.random.default_rng(8)
+    x = (rng.standard_normal(777) * 3).astype(np.float32)
+    mn, inv = np.float32(x.min()), np.float32(1.0) / np.float32((x.max() - x.min()) / np.float32(255.0))
+    code, norm = o.sq8_encode(x, False, float(mn), float(inv))
+    want = np.clip(_round_half_away(((x - mn) * inv).astype(np.float32)), 0, 255).astype(np.uint8)
+    assert np.array_equal(code, want)
+    assert norm == int((want.astype(np.int64) ** 2).sum())
+    inv2 = np.float32(1.0) / np.float32(np.abs(x).max() / np.float32(127.0))
+    code, norm = o.sq8_encode(x, True, 0.0, float(inv2))
+    want = np.clip(_round_half_away((x * inv2).astype(np.float32)), -127, 127).astype(np.int8)
+    assert np.array_equal(code, want)
+
+
+def test_sq8_integer_data_codes_are_the_data_and_search_is_exact():
+    rng = np.random.default_rng(9)
+    x = rint(rng, (3000, 32))
+    x[0, 0], x[1, 0] = 0, 255
+    ix = o.SQ8Index(32)
+    ix.add(x)
+    gmin, gmax, scale, n = ix.params()
+    assert (gmin, gmax, scale, n) == (0.0, 255.0, 1.0, 3000)
+    codes, norms = ix.codes()
+    assert np.array_equal(codes, x.astype(np.uint8))
+    q = rint(rng, (30, 32))
+    D, I = ix.search(q, 10)
+    Df, If = o.flat_search(x, q, 10)
+    assert np.array_equal(I, If) and np.array_equal(D, Df)
+
+
+def test_sq8_ip_ordering_and_requantize_scale():
+    # tests/test_metrics.mojo:56-93
+    ix = o.SQ8Index(3, o.IP)
+    ix.add(np.array([[1, 0, 0], [-1, 0, 0], [0.5, 0, 0]], np.float32))
+    D, I = ix.search(np.array([[1, 0, 0]], np.float32), 3)
+    assert I[0].tolist() == [0, 2, 1] and D[0, 0] > D[0, 1] > D[0, 2]
+    ix = o.SQ8Index(2, o.IP)
+    ix.add(np.array([[1.0, 0.25]], np.float32))
+    ix.add(np.array([[-4.0, 0.0]], np.float32))
+    assert abs(ix.params()[2] - 4.0 / 127.0) < 1e-6
+
+
+def _uniform_dataset():
+    g = np.random.default_rng(42)
+    return (g.random((5000, 32)) * 2 - 1).astype(np.float32), (g.random((100, 32)) * 2 - 1).astype(np.float32)
+
+
+def _recall(gt, got):
+    return np.mean([len(set(a) & set(b)) / len(a) for a, b in zip(gt.tolist(), got.tolist())])
+
+
+def test_sq8_ip_recall_and_rerank():
+    # tests/test_accuracy.mojo:190-230 (recall >= 0.95 vs exact IP)
+    db, q = _uniform_dataset()
+    _, gt = o.flat_search(db, q, 10, o.IP)
+    ix = o.SQ8Index(32, o.IP)
+    ix.add(db)
+    _, I = ix.search(q, 10)
+    assert _recall(gt, I) >= 0.95
+    _, I4 = ix.search(q, 10, k_factor=4)
+    assert _recall(gt, I4) >= _recall(gt, I)
+
+
+# ---------------------------------------------------------------- IVF-PQ
+def test_ivfpq_accuracy_threshold():
+    # tests/test_accuracy.mojo:129-161: recall@10 >= 0.50 at nlist 64, M 8, nprobe 16
+    db, q = _uniform_dataset()
+    _, gt = o.flat_search(db, q, 10)
+    ix = o.IVFPQIndex(32, 64, 8)
+    ix.train(db)
+    ix.add(db)
+    _, I = ix.search(q, 10, 16)
+    assert _recall(gt, I) >= 0.50
+    _, Ir = ix.search(q, 10, 16, k_factor=8)
+    assert _recall(gt, Ir) > _recall(gt, I)
+
+
+def test_ivfpq_lifecycle_self_query():
+    # tests/test_ivf_pq.mojo:48-83: 256 x 8 closed form, nlist 8, M 2, nprobe 4
+    rows = np.load(os.path.join(GOLDEN, "ivfpq_rows.npz"))["rows"]
+    ix = o.IVFPQIndex(8, 8, 2)
+    ix.train(rows)
+    ix.add(rows)
+    D, I = ix.search(rows[:1], 5, 4)
+    assert I[0, 0] == 0 and D[0, 0] < 1e-4
+
+
+def test_ivfpq_exhaustive_equals_adc_bruteforce():
+    rng = np.random.default_rng(11)
+    x = rint(rng, (2000, 16), 0, 64)
+    ix = o.IVFPQIndex(16, 8, 4)
+    ix.train(x, niter_coarse=5, niter_pq=5)
+    ix.add(x, keep_vectors=True)
+    coarse, pq = ix.codebooks()
+    q = rint(rng, (10, 16), 0, 64)
+    D, I = ix.search(q, 20, 8)  # nprobe = nlist: every list
+    # reconstruct each vector as c_list + decode(code) from the stored lists
+    recon = np.zeros((2000, 16), np.float64)
+    for l in range(8):
+        ids, codes = ix.list(l)
+        assert np.all(np.diff(ids) > 0)  # append order == id order
+        recon[ids] = coarse[l] + o.pq_decode(codes, pq)
+    d2 = ((q[:, None, :].astype(np.float64) - recon[None]) ** 2).sum(-1)
+    for i in range(10):
+        got = d2[i, I[i]]
+        assert np.allclose(D[i], got, rtol=1e-4, atol=1e-3)
+        kth = np.sort(d2[i])[19]
+        assert (got <= kth + 1e-3).all()
+
+
+def test_ivfpq_padding_exclusion_and_untrained():
+    ix = o.IVFPQIndex(8, 4, 2)
+    D, I = ix.search(np.zeros((2, 8), np.float32), 3, 2)
+    assert (I == -1).all() and (D == np.float32(1e38)).all()
+    rows = np.load(os.path.join(GOLDEN, "ivfpq_rows.npz"))["rows"]
+    ix.train(rows)
+    ix.add(rows)
+    ex = np.zeros(256, np.uint8)
+    ex[0] = 1
+    _, I = ix.search(rows[:1], 1, 4, exclude=ex)
+    assert I[0, 0] != 0
+    D, I = ix.search(rows[:1], 300, 1)
+    n_found = (I[0] >= 0).sum()
+    assert n_found < 300 and (D[0, n_found:] == np.float32(1e38)).all()
+
+
+def test_ivfpq_rerank_with_every_candidate_is_exact_search():
+    rng = np.random.default_rng(12)
+    x = rint(rng, (600, 16), 0, 32)
+    ix = o.IVFPQIndex(16, 4, 4)
+    ix.train(x, niter_coarse=3, niter_pq=3)
+    ix.add(x)
+    q = rint(rng, (5, 16), 0, 32)
+    D, I = ix.search(q, 5, 4, k_factor=120)  # k * k_factor = every row
+    Df, If = o.flat_search(x, q, 5)
+    assert np.array_equal(I, If) and np.array_equal(D, Df)
