@@ -133,6 +133,13 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
                             const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
                             const uint8_t *exclude, int64_t exclude_len, float *distances,
                             int64_t *labels, void *stream);
+/* Search form (no reference counterpart). precision 0: plain FP32, the
+ * reference's residual-LUT arithmetic in its operation order (bit-exact ids);
+ * 1: mixed, L2 tables decomposed as |q - c_l|^2 + T[l] + Q[q] (precomputed
+ * per list / per query), candidates then reranked exactly when k_factor > 1.
+ * split_scan 1 selects one workgroup per (query, list) with a merge pass
+ * instead of one workgroup per query (same results). */
+int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan);
 /* Stage timing of the next searches (no reference counterpart; used by the
  * benchmark): when on, each search records HIP events on its stream around
  * the coarse, scan and merge/rerank stages and counts the list entries the

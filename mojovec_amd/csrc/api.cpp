@@ -138,7 +138,9 @@ struct mvg_sq8 {
 
 struct mvg_ivfpq {
   int d = 0, nlist = 0, M = 0, metric = 0, nprobe = 1;
-  bool trained = false, keep_vectors = true, dirty = true;
+  bool trained = false, keep_vectors = true, dirty = true, tables_dirty = true;
+  int precision = 0, split_scan = 0;
+  DevBuf ptab;
   int64_t ntotal = 0;
   std::vector<std::vector<int32_t>> lids;
   std::vector<std::vector<uint8_t>> lcodes;
@@ -487,6 +489,7 @@ int mvg_ivfpq_train(mvg_ivfpq *h, const float *x, int64_t n, int niter_coarse, i
   pq_train_device(h->tres.as<float>(), n, d, h->M, niter_pq, seed, h->pq.as<float>(), h->kw, h->s);
   MVG_HIP(hipStreamSynchronize(h->s));
   h->trained = true;
+  h->tables_dirty = true;
   API_END
 }
 
@@ -499,6 +502,7 @@ int mvg_ivfpq_set_codebooks(mvg_ivfpq *h, const float *coarse, const float *pq) 
                          hipMemcpyHostToDevice, h->s));
   MVG_HIP(hipStreamSynchronize(h->s));
   h->trained = true;
+  h->tables_dirty = true;
   API_END
 }
 
@@ -585,6 +589,18 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.excl_len = exclude_len;
   a.D = distances;
   a.I = labels;
+  a.precision = h->precision;
+  a.split_scan = h->split_scan;
+  if (h->precision != 0 && h->metric == MVG_METRIC_L2) {
+    if (h->tables_dirty) {
+      h->ptab.reserve(sizeof(float) * (size_t)h->nlist * h->M * mvg::kKsub);
+      mvg::precompute_ivf_tables(h->coarse.as<float>(), h->pq.as<float>(), h->d, h->M, h->nlist,
+                                 h->ptab.as<float>(), h->s);
+      MVG_HIP(hipStreamSynchronize(h->s));
+      h->tables_dirty = false;
+    }
+    a.ptab = h->ptab.as<float>();
+  }
   mvg::ivfpq_search(a, h->work, s);
   API_END
 }
@@ -624,6 +640,14 @@ int mvg_ivfpq_search(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   download(distances, h->D.ptr, (size_t)nq * k, h->s);
   download(labels, h->I.ptr, (size_t)nq * k, h->s);
   MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan) {
+  API_BEGIN
+  require(h && (precision == 0 || precision == 1), "precision must be 0 (plain) or 1 (mixed)");
+  h->precision = precision;
+  h->split_scan = split_scan != 0;
   API_END
 }
 

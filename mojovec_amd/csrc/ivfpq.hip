@@ -170,6 +170,220 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
   }
 }
 
+// ---------------------------------------------------------------------------
+// Per-query scan: one workgroup walks the query's probed lists in coarse
+// order and keeps a running canonical top-kk of u64 (key << 32 | id) in LDS.
+// A candidate enters the append buffer only if it beats the current kk-th
+// key, which can only shrink, so the filter never drops a final winner. The
+// buffer is merged into the top set by a rank merge (small sets) or the radix
+// select (large sets).
+//
+// LUT per (query, list):
+//   plain L2  : the exact residual table of pq.mojo:252-292 (q - c_l), as the
+//               reference computes it;
+//   mixed L2  : T[l][m][k] + Q[q][m][k] with T = |c_mk|^2 + 2<c_l,m, c_mk>
+//               (precomputed per list) and Q = -2<q_m, c_mk> (per query), on
+//               top of the exact FP32 coarse term |q - c_l|^2;
+//   IP (both) : -<q_m, c_mk> once per query plus -<q, c_l> per list
+//               (index_ivf_pq.mojo:288-293, 316-324).
+
+constexpr int kCandCap = 1024;  // append-buffer capacity (u64 keys)
+constexpr int kRound = 2 * kNT;  // list entries scored between buffer checks
+
+struct QueryScanShared {
+  int cnt;
+  uint64_t thr;
+  float base;
+};
+
+// Merges the append buffer cand[kk .. kk+cnt) into the sorted top set
+// cand[0 .. kk) (sentinel padded). Leaves cand[0..kk) sorted, cnt = 0.
+__device__ void flush_candidates(uint64_t *cand, uint64_t *tmp, int kk, QueryScanShared &st,
+                                 SelectScratch64 &scr) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+  const int n = kk + st.cnt;
+  if (n <= 2 * kNT) {
+    // rank merge: keys are unique, sentinels sort last and are dropped
+    for (int i = tid; i < kk; i += kNT) tmp[i] = kSentinel64;
+    __syncthreads();
+    for (int i = tid; i < n; i += kNT) {
+      const uint64_t key = cand[i];
+      if (key == kSentinel64) continue;
+      int r = 0;
+      for (int j = 0; j < n; j++) r += cand[j] < key;
+      if (r < kk) tmp[r] = key;
+    }
+    __syncthreads();
+  } else {
+    select_sort_u64(cand, n, kk, scr, tmp);  // tmp[0..np2) sorted, sentinel tail
+  }
+  for (int i = tid; i < kk; i += kNT) cand[i] = tmp[i];
+  __syncthreads();
+  if (tid == 0) {
+    st.cnt = 0;
+    st.thr = cand[kk - 1];  // sentinel until kk candidates exist
+  }
+  __syncthreads();
+}
+
+__host__ __device__ inline int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// LDS layout of the per-query scan (bytes), shared by host and device.
+struct QueryScanLayout {
+  int lut, qtab, rv, cand, tmp, scr, st, total;
+  __host__ __device__ QueryScanLayout(int M, int d, int kk, bool has_qtab) {
+    const int nlut = M * kKsub * 4;
+    lut = 0;
+    qtab = lut + nlut;
+    rv = qtab + (has_qtab ? nlut : 0);
+    cand = (rv + 4 * d + 15) & ~15;
+    tmp = cand + 8 * (kk + kCandCap);
+    scr = tmp + 8 * next_pow2(kk);
+    st = (scr + (int)sizeof(SelectScratch64) + 15) & ~15;
+    total = st + (int)sizeof(QueryScanShared);
+  }
+};
+
+template <int MF, bool IP, bool MIXED>
+__global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
+    const float *__restrict__ q, int d, int M, const float *__restrict__ coarse,
+    const float *__restrict__ pq, const float *__restrict__ ptab,
+    const int64_t *__restrict__ list_off, const uint8_t *__restrict__ codes,
+    const int32_t *__restrict__ ids, const int32_t *__restrict__ probes, int maxp,
+    const int32_t *__restrict__ nprobe_q, int nprobe_all, int kk, int k, const uint8_t *__restrict__ excl,
+    int64_t excl_len, uint64_t *__restrict__ out_keys, float *__restrict__ D,
+    int64_t *__restrict__ I, float pad_dist) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nlut = M * kKsub;
+  const QueryScanLayout lay(M, d, kk, MIXED && !IP);
+  float *lut = reinterpret_cast<float *>(smem + lay.lut);
+  float *qtab = reinterpret_cast<float *>(smem + lay.qtab);
+  float *rv = reinterpret_cast<float *>(smem + lay.rv);
+  uint64_t *cand = reinterpret_cast<uint64_t *>(smem + lay.cand);
+  uint64_t *tmp = reinterpret_cast<uint64_t *>(smem + lay.tmp);
+  SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(smem + lay.scr);
+  QueryScanShared *st = reinterpret_cast<QueryScanShared *>(smem + lay.st);
+
+  const int tid = threadIdx.x;
+  const int64_t qi = blockIdx.x;
+  const float *qq = q + qi * d;
+  const int dsub = d / M;
+  int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
+  np = min(max(np, 1), maxp);
+
+  for (int i = tid; i < kk; i += kNT) cand[i] = kSentinel64;
+  if (tid == 0) {
+    st->cnt = 0;
+    st->thr = kSentinel64;
+  }
+  // Per-query tables: IP LUT (pq.mojo:285-292) or the mixed -2<q_m, c_mk>.
+  if (IP || MIXED) {
+    for (int j = tid; j < d; j += kNT) rv[j] = qq[j];
+    __syncthreads();
+    for (int e = tid; e < nlut; e += kNT) {
+      const int m = e >> 8;
+      const float *c = pq + (int64_t)e * dsub;
+      if (IP) {
+        lut[e] = -ip_pq(rv + m * dsub, c, dsub);
+      } else {
+        float dot = 0.0f;
+        for (int j = 0; j < dsub; j++) dot = fma_(rv[m * dsub + j], c[j], dot);
+        qtab[e] = -2.0f * dot;
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int p = 0; p < np; p++) {
+    const int l = probes[qi * maxp + p];
+    if (l < 0) continue;
+    const int64_t off = list_off[l];
+    const int64_t L = list_off[l + 1] - off;
+    if (L == 0) continue;
+    const float *cl = coarse + (int64_t)l * d;
+    if (!IP) {
+      if (!MIXED) {
+        for (int j = tid; j < d; j += kNT) rv[j] = qq[j] - cl[j];  // :306-310
+        __syncthreads();
+        for (int e = tid; e < nlut; e += kNT) {
+          const int m = e >> 8;
+          lut[e] = l2_kmeans(rv + m * dsub, pq + (int64_t)e * dsub, dsub);
+        }
+      } else {
+        const float *tl = ptab + (int64_t)l * nlut;
+        for (int e = tid; e < nlut; e += kNT) lut[e] = tl[e] + qtab[e];
+        if (tid == 0) st->base = l2_w8(qq, cl, d);
+      }
+    } else if (tid == 0) {
+      st->base = -ip_wn<8>(qq, cl, d);  // :322-324
+    }
+    __syncthreads();
+    const float base = (IP || MIXED) ? st->base : 0.0f;
+    for (int64_t r0 = 0; r0 < L; r0 += kRound) {
+      const uint64_t thr = st->thr;
+#pragma unroll
+      for (int u = 0; u < kRound / kNT; u++) {
+        const int64_t pos = r0 + u * kNT + tid;
+        if (pos < L) {
+          const int32_t id = ids[off + pos];
+          const bool skip = excl && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
+          if (!skip) {
+            const float dist = adc_sum<MF>(codes + (off + pos) * M, lut, M, base);
+            const uint64_t key = pack_key(dist, (uint32_t)id);
+            if (key < thr) cand[kk + atomicAdd(&st->cnt, 1)] = key;
+          }
+        }
+      }
+      __syncthreads();
+      if (st->cnt > kCandCap - kRound) flush_candidates(cand, tmp, kk, *st, *scr);
+    }
+    if (st->cnt > 0) flush_candidates(cand, tmp, kk, *st, *scr);
+  }
+  __syncthreads();
+  // Output: sorted top-kk keys for the rerank, or final distances / ids.
+  if (out_keys) {
+    for (int t = tid; t < kk; t += kNT) out_keys[qi * kk + t] = cand[t];
+  } else {
+    for (int t = tid; t < k; t += kNT) {
+      const uint64_t kv = cand[t];
+      float dv = pad_dist;
+      int64_t iv = -1;
+      if (kv != kSentinel64) {
+        dv = key2f((uint32_t)(kv >> 32));
+        if (IP) dv = -dv;  // :357-360
+        iv = (int64_t)(uint32_t)(kv & 0xFFFFFFFFu);
+      }
+      D[qi * k + t] = dv;
+      I[qi * k + t] = iv;
+    }
+  }
+}
+
+// T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> (mixed L2 tables).
+__global__ void precompute_tables_kernel(const float *__restrict__ coarse, const float *__restrict__ pq,
+                                         int d, int M, int nlist, float *__restrict__ ptab) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nlut = (int64_t)M * kKsub;
+  if (t >= (int64_t)nlist * nlut) return;
+  const int l = (int)(t / nlut);
+  const int e = (int)(t - (int64_t)l * nlut);
+  const int m = e >> 8;
+  const int dsub = d / M;
+  const float *c = pq + (int64_t)e * dsub;
+  const float *cl = coarse + (int64_t)l * d + (int64_t)m * dsub;
+  float nn = 0.0f, dot = 0.0f;
+  for (int j = 0; j < dsub; j++) {
+    nn = fma_(c[j], c[j], nn);
+    dot = fma_(cl[j], c[j], dot);
+  }
+  ptab[t] = nn + 2.0f * dot;
+}
+
 __global__ void scanned_entries_kernel(const int32_t *__restrict__ probes, int64_t nq, int maxp,
                                        const int32_t *__restrict__ nprobe_q, int nprobe_all,
                                        const int64_t *__restrict__ list_off,
@@ -206,7 +420,31 @@ void launch_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk
   MVG_LAUNCH_CHECK("ivfpq_scan_kernel");
 }
 
+template <int MF, bool IP, bool MIXED>
+void launch_query_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                       uint64_t *out_keys, hipStream_t s) {
+  const QueryScanLayout lay(a.M, a.d, kk, MIXED && !IP);
+  const size_t sm = (size_t)lay.total;
+  if (sm > 150 * 1024) throw Error(kInvalid, "IVF-PQ scan workspace exceeds LDS");
+  if (sm > 64 * 1024)
+    MVG_HIP(hipFuncSetAttribute((const void *)ivfpq_query_scan_kernel<MF, IP, MIXED>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  hipLaunchKernelGGL((ivfpq_query_scan_kernel<MF, IP, MIXED>), dim3((unsigned)a.nq), dim3(kNT), sm, s,
+                     a.q, a.d, a.M, a.coarse, a.pq, a.ptab, a.list_off, a.codes, a.ids, probes, maxp,
+                     a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D, a.I,
+                     kIvfPadDist);
+  MVG_LAUNCH_CHECK("ivfpq_query_scan_kernel");
+}
+
 }  // namespace
+
+void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, int nlist,
+                           float *ptab, hipStream_t s) {
+  const int64_t total = (int64_t)nlist * M * kKsub;
+  hipLaunchKernelGGL(precompute_tables_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, s, coarse, pq,
+                     d, M, nlist, ptab);
+  MVG_LAUNCH_CHECK("precompute_tables_kernel");
+}
 
 void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
@@ -249,11 +487,36 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   int32_t *probes = w.probes.as<int32_t>();
   flat_topk(a.coarse, a.nlist, a.d, a.metric, a.q, a.nq, maxp, nullptr, 0, w.coarse_part, 0.0f,
             nullptr, nullptr, probes, nullptr, s);
+  const bool ip = a.metric == kMetricIP;
+  if (pf.on) MVG_HIP(hipEventRecord(pf.ev[1], s));
+  if (!a.split_scan) {
+    // 2'. one workgroup per query: LUT + ADC scan + running top-kk, then the
+    // optional rerank.
+    uint64_t *keys = nullptr;
+    if (kf > 1) {
+      w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
+      keys = w.keys.as<uint64_t>();
+    }
+    const bool mixed = a.precision != 0 && !ip;
+    if (mixed && !a.ptab) throw Error(kState, "mixed-precision tables are missing");
+#define MVG_QSCAN(MFV)                                                                   \
+  if (ip) launch_query_scan<MFV, true, false>(a, maxp, probes, kk, keys, s);            \
+  else if (mixed) launch_query_scan<MFV, false, true>(a, maxp, probes, kk, keys, s);    \
+  else launch_query_scan<MFV, false, false>(a, maxp, probes, kk, keys, s);
+    switch (a.M) {
+      case 8: MVG_QSCAN(8) break;
+      case 12: MVG_QSCAN(12) break;
+      case 16: MVG_QSCAN(16) break;
+      default: MVG_QSCAN(0) break;
+    }
+#undef MVG_QSCAN
+    if (pf.on) MVG_HIP(hipEventRecord(pf.ev[2], s));
+    if (kf > 1)
+      rerank(a.q, a.nq, a.d, a.metric, a.vectors, keys, kk, a.k, kIvfPadDist, a.D, a.I, s);
+  } else {
   // 2. per-(query, probe) LUT + ADC scan + list-local top-kk.
   w.part.reserve(sizeof(uint64_t) * (size_t)a.nq * maxp * kk);
   uint64_t *part = w.part.as<uint64_t>();
-  const bool ip = a.metric == kMetricIP;
-  if (pf.on) MVG_HIP(hipEventRecord(pf.ev[1], s));
 #define MVG_SCAN(MFV)                                                          \
   if (ip) launch_scan<MFV, true>(a, maxp, probes, kk, part, s);                \
   else launch_scan<MFV, false>(a, maxp, probes, kk, part, s);
@@ -275,6 +538,7 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
                  w.keys.as<uint64_t>(), s);
     rerank(a.q, a.nq, a.d, a.metric, a.vectors, w.keys.as<uint64_t>(), kk, a.k, kIvfPadDist, a.D,
            a.I, s);
+  }
   }
   if (pf.on) {
     MVG_HIP(hipEventRecord(pf.ev[3], s));

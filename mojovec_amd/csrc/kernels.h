@@ -55,7 +55,13 @@ struct IvfSearchArgs {
   int64_t excl_len;
   float *D;
   int64_t *I;
+  int precision;       // 0 plain FP32 (reference order), 1 mixed
+  const float *ptab;   // mixed L2 tables [nlist][M][256]
+  int split_scan;      // 1: one workgroup per (query, list) + merge pass
 };
+// T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> for the mixed L2 scan.
+void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, int nlist,
+                           float *ptab, hipStream_t s);
 // Opt-in stage timing: events recorded on the search stream around the
 // coarse, scan and merge/rerank stages, and the number of list entries the
 // scan visited (sum of probed list lengths).

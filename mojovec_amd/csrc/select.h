@@ -57,6 +57,12 @@ struct SelectScratch {
   int sh[16];
 };
 
+// Scratch for the u64 selects only (8-bit digits).
+struct SelectScratch64 {
+  int hist[256];
+  int sh[16];
+};
+
 // k-th smallest valid u32 key (1 <= k <= number of valid keys). Returns the
 // key T and how many keys equal to T belong to the k smallest.
 __device__ void radix_kth_u32(const uint32_t *keys, int n, int k, SelectScratch &s,
@@ -148,7 +154,8 @@ __device__ int select_topk_u32(const uint32_t *keys, int n, int k, SelectScratch
 }
 
 // k-th smallest of unique u64 keys (1 <= k <= number of valid keys).
-__device__ uint64_t radix_kth_u64(const uint64_t *keys, int n, int k, SelectScratch &s) {
+template <typename Scratch>
+__device__ uint64_t radix_kth_u64(const uint64_t *keys, int n, int k, Scratch &s) {
   const int tid = threadIdx.x;
   uint64_t prefix = 0, mask = 0;
   int remaining = k;
@@ -202,7 +209,8 @@ __device__ void bitonic_sort_u64(uint64_t *keys, int n) {
 // Writes the min(k, valid) smallest unique u64 keys of in[0..n) to out[] in
 // ascending order; out must hold next_pow2(k) entries and must not overlap
 // in. Returns the count.
-__device__ int select_sort_u64(const uint64_t *in, int n, int k, SelectScratch &s,
+template <typename Scratch>
+__device__ int select_sort_u64(const uint64_t *in, int n, int k, Scratch &s,
                                uint64_t *out) {
   const int tid = threadIdx.x;
   int valid = 0;

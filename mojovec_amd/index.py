@@ -220,6 +220,12 @@ class IndexIVFPQ:
     def nprobe(self, v: int) -> None:
         _lib.check(_lib.load().mvg_ivfpq_set_nprobe(self._h, int(v)))
 
+    def set_search_mode(self, precision: str = "plain", split_scan: bool = False) -> None:
+        """precision "plain" (reference FP32 order) or "mixed" (decomposed
+        tables; pair with k_factor > 1 for the exact rerank)."""
+        p = {"plain": 0, "mixed": 1}[precision]
+        _lib.check(_lib.load().mvg_ivfpq_set_search_mode(self._h, p, int(split_scan)))
+
     def train(self, x, niter_coarse: int = 15, niter_pq: int = 15, seed: int = KMEANS_SEED) -> None:
         x = _f32(x, self.d)
         _lib.check(_lib.load().mvg_ivfpq_train(self._h, _ptr(x), x.shape[0], niter_coarse, niter_pq, seed))

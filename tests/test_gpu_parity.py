@@ -149,13 +149,15 @@ def test_ivfpq_train_matches_oracle(gpu, metric):
     assert np.array_equal(pg, po)
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("metric", [0, 1])
 @pytest.mark.parametrize("M", [16, 8, 4])
-def test_ivfpq_add_and_search_match_oracle(gpu, metric, M):
+def test_ivfpq_add_and_search_match_oracle(gpu, metric, M, split):
     x = clustered(4, 8000, 64, nclusters=60)
     if metric == 1:
         x = x / 255.0
     oi, gi = _pair(gpu, x, 64, 40, M, metric)
+    gi.set_search_mode("plain", split_scan=split)
     oi.add(x[:5000])
     oi.add(x[5000:])
     gi.add(x[:5000])
@@ -211,6 +213,28 @@ def test_ivfpq_long_lists_cross_scan_chunks(gpu):
         Dg, Ig = gi.search(q, k)
         Do, Io = oi.search(q, k, 4)
         assert_same(Dg, Ig, Do, Io)
+
+
+def test_ivfpq_mixed_form_error_bound_and_rerank(gpu):
+    # Mixed L2 tables: |q - c_l|^2 + T[l] + Q[q] equals the residual ADC
+    # distance up to FP32 rounding; with every candidate reranked the result
+    # is the exact search.
+    x = clustered(9, 6000, 32, nclusters=30)
+    oi, gi = _pair(gpu, x, 32, 16, 8, 0)
+    oi.add(x)
+    gi.add(x)
+    q = clustered(10, 30, 32, nclusters=30)
+    gi.nprobe = 16
+    gi.set_search_mode("mixed")
+    Dm, Im = gi.search(q, 50)
+    Do, Io = oi.search(q, 50, 16)
+    assert np.allclose(Dm, Do, rtol=1e-4, atol=0.5)
+    overlap = np.mean([len(set(a) & set(b)) / 50 for a, b in zip(Im.tolist(), Io.tolist())])
+    assert overlap > 0.95
+    Dr, Ir = gi.search(q[:5], 5, k_factor=800)  # 4000 of 6000 candidates reranked
+    Df, If = o.flat_search(x, q[:5], 5)
+    assert np.array_equal(Ir, If) and np.array_equal(Dr, Df)
+    gi.set_search_mode("plain")
 
 
 def test_ivfpq_untrained_and_empty(gpu):
