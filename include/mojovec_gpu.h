@@ -59,6 +59,11 @@ int mvg_flat_add(mvg_flat *h, const float *x, int64_t n);
  * Slots past the number of candidates hold (0.0, -1). */
 int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_t *exclude,
                     int64_t exclude_len, float *distances, int64_t *labels);
+/* Search form (no reference counterpart). precision 0: plain FP32 in the
+ * reference's operation order; 1: mixed, candidates from a bf16 GEMM-form
+ * distance on MFMA (FP32 accumulate), reranked with exact FP32 distances when
+ * k_factor > 1. */
+int mvg_flat_set_search_mode(mvg_flat *h, int precision, int k_factor);
 int64_t mvg_flat_ntotal(const mvg_flat *h);
 void mvg_flat_destroy(mvg_flat *h);
 
@@ -78,6 +83,10 @@ int mvg_sq8_search(mvg_sq8 *h, const float *q, int64_t nq, int k, int k_factor,
 int mvg_sq8_params(const mvg_sq8 *h, float *gmin, float *gmax, float *scale, int64_t *ntotal);
 /* Codes [ntotal x d] and code norms [ntotal] (codes_u8 / norms_u32). */
 int mvg_sq8_get_codes(mvg_sq8 *h, uint8_t *codes, uint32_t *norms);
+/* Code-dot engine (no reference counterpart): 1 (default) contracts the
+ * codes on int8 MFMA (affine uint8 rebased to int8 with an exact correction),
+ * 0 uses dot4 VALU instructions. Both give bit-identical results. */
+int mvg_sq8_set_search_mode(mvg_sq8 *h, int use_mfma);
 void mvg_sq8_destroy(mvg_sq8 *h);
 
 /* ---- k-means: KMeans (mojovec/clustering/kmeans.mojo:8-220) ------------- */

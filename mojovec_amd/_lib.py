@@ -41,6 +41,7 @@ SIGNATURES = {
     "mvg_flat_create": (_i, [_i, _i, ctypes.POINTER(_p)]),
     "mvg_flat_add": (_i, [_p, _p, _i64]),
     "mvg_flat_search": (_i, [_p, _p, _i64, _i, _p, _i64, _p, _p]),
+    "mvg_flat_set_search_mode": (_i, [_p, _i, _i]),
     "mvg_flat_ntotal": (_i64, [_p]),
     "mvg_flat_destroy": (None, [_p]),
     "mvg_sq8_create": (_i, [_i, _i, ctypes.POINTER(_p)]),
@@ -49,6 +50,7 @@ SIGNATURES = {
     "mvg_sq8_params": (_i, [_p, ctypes.POINTER(_f), ctypes.POINTER(_f), ctypes.POINTER(_f),
                             ctypes.POINTER(_i64)]),
     "mvg_sq8_get_codes": (_i, [_p, _p, _p]),
+    "mvg_sq8_set_search_mode": (_i, [_p, _i]),
     "mvg_sq8_destroy": (None, [_p]),
     "mvg_kmeans_train": (_i, [_p, _i64, _i, _i, _i, _p, _i64, _u64, _p, _p]),
     "mvg_pq_train": (_i, [_p, _i64, _i, _i, _i, _u64, _p]),

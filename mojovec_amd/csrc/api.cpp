@@ -123,7 +123,9 @@ void mvg::set_last_error(const std::string &msg) { g_last_error = msg; }
 struct mvg_flat {
   int d = 0, metric = 0;
   int64_t ntotal = 0;
-  DevBuf base, part, q, D, I, excl;
+  int precision = 0, k_factor = 1;
+  bool bf16_dirty = true;
+  DevBuf base, part, q, D, I, excl, base_bf16, base_norm, qbf, qnorm, keys;
   hipStream_t s = nullptr;
 };
 
@@ -131,7 +133,8 @@ struct mvg_sq8 {
   int d = 0, metric = 0;
   int64_t ntotal = 0;
   float gmin = 3.40282347e38f, gmax = -3.40282347e38f, scale = 1.0f;  // index_flat_sq8.mojo:234-236
-  DevBuf f32, codes, norms, tmp, q, D, I, excl;
+  DevBuf f32, codes, norms, sums, tmp, q, D, I, excl;
+  int use_mfma = 1;
   mvg::Sq8Work work;
   hipStream_t s = nullptr;
 };
@@ -140,7 +143,7 @@ struct mvg_ivfpq {
   int d = 0, nlist = 0, M = 0, metric = 0, nprobe = 1;
   bool trained = false, keep_vectors = true, dirty = true, tables_dirty = true;
   int precision = 0, split_scan = 0;
-  DevBuf ptab;
+  DevBuf ptab, coarse_bf16, coarse_norm;
   int64_t ntotal = 0;
   std::vector<std::vector<int32_t>> lids;
   std::vector<std::vector<uint8_t>> lcodes;
@@ -219,6 +222,7 @@ int mvg_flat_add(mvg_flat *h, const float *x, int64_t n) {
   MVG_HIP(hipMemcpyAsync((char *)h->base.ptr + keep, x, need - keep, hipMemcpyHostToDevice, h->s));
   MVG_HIP(hipStreamSynchronize(h->s));
   h->ntotal += n;
+  h->bf16_dirty = true;
   API_END
 }
 
@@ -231,12 +235,54 @@ int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_
   const uint8_t *dx = exclude && exclude_len > 0 ? upload(h->excl, exclude, (size_t)exclude_len, h->s) : nullptr;
   h->D.reserve(sizeof(float) * nq * k);
   h->I.reserve(sizeof(int64_t) * nq * k);
-  mvg::flat_topk(h->base.as<float>(), h->ntotal, h->d, h->metric, dq, nq, k, dx,
-                 dx ? exclude_len : 0, h->part, 0.0f, h->D.as<float>(), h->I.as<int64_t>(),
-                 nullptr, nullptr, h->s);
+  if (h->precision == 0 || h->ntotal == 0) {
+    mvg::flat_topk(h->base.as<float>(), h->ntotal, h->d, h->metric, dq, nq, k, dx,
+                   dx ? exclude_len : 0, h->part, 0.0f, h->D.as<float>(), h->I.as<int64_t>(),
+                   nullptr, nullptr, h->s);
+  } else {
+    // mixed: bf16 GEMM-form candidates on MFMA, optional exact FP32 rerank
+    const int rb = mvg::gemm_row_bytes(mvg::kGemmBf16L2, h->d);
+    if (h->bf16_dirty) {
+      h->base_bf16.reserve((size_t)h->ntotal * rb);
+      h->base_norm.reserve(sizeof(float) * h->ntotal);
+      mvg::to_bf16(h->base.as<float>(), h->ntotal, h->d, h->base_bf16.as<uint16_t>(),
+                   h->base_norm.as<float>(), h->s);
+      h->bf16_dirty = false;
+    }
+    h->qbf.reserve((size_t)nq * rb);
+    h->qnorm.reserve(sizeof(float) * nq);
+    mvg::to_bf16(dq, nq, h->d, h->qbf.as<uint16_t>(), h->qnorm.as<float>(), h->s);
+    const int kf = h->k_factor;
+    const int kk = k * kf;
+    if (kk > mvg::kMaxCandidates) throw Error(mvg::kInvalid, "k * k_factor exceeds the device limit of 4096");
+    const bool ip = h->metric == MVG_METRIC_IP;
+    int nsplit = 0;
+    mvg::gemm_topk(ip ? mvg::kGemmBf16IP : mvg::kGemmBf16L2, h->qbf.as<uint8_t>(),
+                   h->base_bf16.as<uint8_t>(), nq, h->ntotal, h->d, h->qnorm.as<float>(),
+                   h->base_norm.as<float>(), nullptr, nullptr, nullptr, nullptr, 0.0f, kk, dx,
+                   dx ? exclude_len : 0, h->part, nsplit, h->s);
+    if (kf == 1) {
+      mvg::launch_merge(h->part.as<uint64_t>(), nq, (int64_t)nsplit * kk, k, ip ? 1 : 0, 0.0f,
+                        h->D.as<float>(), h->I.as<int64_t>(), nullptr, nullptr, h->s);
+    } else {
+      h->keys.reserve(sizeof(uint64_t) * nq * kk);
+      mvg::launch_merge(h->part.as<uint64_t>(), nq, (int64_t)nsplit * kk, kk, 0, 0.0f, nullptr,
+                        nullptr, nullptr, h->keys.as<uint64_t>(), h->s);
+      mvg::rerank(dq, nq, h->d, h->metric, h->base.as<float>(), h->keys.as<uint64_t>(), kk, k, 0.0f,
+                  h->D.as<float>(), h->I.as<int64_t>(), h->s);
+    }
+  }
   download(distances, h->D.ptr, (size_t)nq * k, h->s);
   download(labels, h->I.ptr, (size_t)nq * k, h->s);
   MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_flat_set_search_mode(mvg_flat *h, int precision, int k_factor) {
+  API_BEGIN
+  require(h && (precision == 0 || precision == 1) && k_factor >= 1, "invalid search mode");
+  h->precision = precision;
+  h->k_factor = k_factor;
   API_END
 }
 
@@ -270,9 +316,11 @@ int mvg_sq8_add(mvg_sq8 *h, const float *x, int64_t n) {
   const size_t keep = sizeof(float) * (size_t)h->ntotal * d;
   const size_t need = keep + sizeof(float) * (size_t)n * d;
   const size_t cap_rows = std::max<size_t>(need / (sizeof(float) * d), h->f32.bytes / (sizeof(float) * d) * 2);
+  const int ld = mvg::sq8_row_stride(d);
   h->f32.grow_keep(cap_rows * d * sizeof(float), keep, h->s);
-  h->codes.grow_keep(cap_rows * d + 16, (size_t)h->ntotal * d, h->s);
+  h->codes.grow_keep(cap_rows * ld + 16, (size_t)h->ntotal * ld, h->s);
   h->norms.grow_keep(cap_rows * sizeof(uint32_t), (size_t)h->ntotal * sizeof(uint32_t), h->s);
+  h->sums.grow_keep(cap_rows * sizeof(uint32_t), (size_t)h->ntotal * sizeof(uint32_t), h->s);
   float *dst = (float *)((char *)h->f32.ptr + keep);
   MVG_HIP(hipMemcpyAsync(dst, x, need - keep, hipMemcpyHostToDevice, h->s));
   float mm[2];  // index_flat_sq8.mojo:317-331
@@ -291,9 +339,9 @@ int mvg_sq8_add(mvg_sq8 *h, const float *x, int64_t n) {
   const float mn = sym ? 0.0f : h->gmin;
   if (requant && h->ntotal > 0)  // :346-367
     mvg::sq8_encode(h->f32.as<float>(), h->ntotal, d, sym, mn, inv, h->codes.as<uint8_t>(),
-                    h->norms.as<uint32_t>(), h->s);
-  mvg::sq8_encode(dst, n, d, sym, mn, inv, h->codes.as<uint8_t>() + (size_t)h->ntotal * d,
-                  h->norms.as<uint32_t>() + h->ntotal, h->s);
+                    h->norms.as<uint32_t>(), h->sums.as<uint32_t>(), h->s);
+  mvg::sq8_encode(dst, n, d, sym, mn, inv, h->codes.as<uint8_t>() + (size_t)h->ntotal * ld,
+                  h->norms.as<uint32_t>() + h->ntotal, h->sums.as<uint32_t>() + h->ntotal, h->s);
   MVG_HIP(hipStreamSynchronize(h->s));
   h->ntotal += n;
   API_END
@@ -313,6 +361,8 @@ int mvg_sq8_search(mvg_sq8 *h, const float *q, int64_t nq, int k, int k_factor,
   a.n = h->ntotal;
   a.codes = h->codes.as<uint8_t>();
   a.norms = h->norms.as<uint32_t>();
+  a.sums = h->sums.as<uint32_t>();
+  a.use_mfma = h->use_mfma;
   a.vectors = h->f32.as<float>();
   a.gmin = h->gmin;
   a.scale = h->scale;
@@ -344,9 +394,18 @@ int mvg_sq8_params(const mvg_sq8 *h, float *gmin, float *gmax, float *scale, int
 int mvg_sq8_get_codes(mvg_sq8 *h, uint8_t *codes, uint32_t *norms) {
   API_BEGIN
   require(h, "invalid handle");
-  if (codes) download(codes, h->codes.ptr, (size_t)h->ntotal * h->d, h->s);
+  const int ld = mvg::sq8_row_stride(h->d);
+  if (codes && h->ntotal)
+    MVG_HIP(hipMemcpy2DAsync(codes, h->d, h->codes.ptr, ld, h->d, h->ntotal, hipMemcpyDeviceToHost, h->s));
   if (norms) download(norms, h->norms.ptr, (size_t)h->ntotal, h->s);
   MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_sq8_set_search_mode(mvg_sq8 *h, int use_mfma) {
+  API_BEGIN
+  require(h, "invalid handle");
+  h->use_mfma = use_mfma != 0;
   API_END
 }
 
@@ -591,15 +650,23 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.I = labels;
   a.precision = h->precision;
   a.split_scan = h->split_scan;
-  if (h->precision != 0 && h->metric == MVG_METRIC_L2) {
+  if (h->precision != 0) {
     if (h->tables_dirty) {
-      h->ptab.reserve(sizeof(float) * (size_t)h->nlist * h->M * mvg::kKsub);
-      mvg::precompute_ivf_tables(h->coarse.as<float>(), h->pq.as<float>(), h->d, h->M, h->nlist,
-                                 h->ptab.as<float>(), h->s);
+      if (h->metric == MVG_METRIC_L2) {
+        h->ptab.reserve(sizeof(float) * (size_t)h->nlist * h->M * mvg::kKsub);
+        mvg::precompute_ivf_tables(h->coarse.as<float>(), h->pq.as<float>(), h->d, h->M, h->nlist,
+                                   h->ptab.as<float>(), h->s);
+      }
+      h->coarse_bf16.reserve((size_t)h->nlist * mvg::gemm_row_bytes(mvg::kGemmBf16L2, h->d));
+      h->coarse_norm.reserve(sizeof(float) * h->nlist);
+      mvg::to_bf16(h->coarse.as<float>(), h->nlist, h->d, h->coarse_bf16.as<uint16_t>(),
+                   h->coarse_norm.as<float>(), h->s);
       MVG_HIP(hipStreamSynchronize(h->s));
       h->tables_dirty = false;
     }
-    a.ptab = h->ptab.as<float>();
+    if (h->metric == MVG_METRIC_L2) a.ptab = h->ptab.as<float>();
+    a.coarse_bf16 = h->coarse_bf16.as<uint16_t>();
+    a.coarse_norm = h->coarse_norm.as<float>();
   }
   mvg::ivfpq_search(a, h->work, s);
   API_END

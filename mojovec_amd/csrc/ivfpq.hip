@@ -485,8 +485,23 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   // 1. coarse top-maxp (IndexFlat.search over the centroids, :264).
   w.probes.reserve(sizeof(int32_t) * (size_t)a.nq * maxp);
   int32_t *probes = w.probes.as<int32_t>();
-  flat_topk(a.coarse, a.nlist, a.d, a.metric, a.q, a.nq, maxp, nullptr, 0, w.coarse_part, 0.0f,
-            nullptr, nullptr, probes, nullptr, s);
+  if (a.precision != 0 && a.coarse_bf16) {
+    // mixed: bf16 GEMM-form coarse distances on MFMA, FP32 accumulate
+    const int rb = gemm_row_bytes(kGemmBf16L2, a.d);
+    w.qbf.reserve((size_t)a.nq * rb);
+    w.qnorm.reserve(sizeof(float) * a.nq);
+    to_bf16(a.q, a.nq, a.d, w.qbf.as<uint16_t>(), w.qnorm.as<float>(), s);
+    int nsplit = 0;
+    gemm_topk(a.metric == kMetricIP ? kGemmBf16IP : kGemmBf16L2, w.qbf.as<uint8_t>(),
+              reinterpret_cast<const uint8_t *>(a.coarse_bf16), a.nq, a.nlist, a.d, w.qnorm.as<float>(),
+              a.coarse_norm, nullptr, nullptr, nullptr, nullptr, 0.0f, maxp, nullptr, 0, w.coarse_part,
+              nsplit, s);
+    launch_merge(w.coarse_part.as<uint64_t>(), a.nq, (int64_t)nsplit * maxp, maxp, 0, 0.0f, nullptr,
+                 nullptr, probes, nullptr, s);
+  } else {
+    flat_topk(a.coarse, a.nlist, a.d, a.metric, a.q, a.nq, maxp, nullptr, 0, w.coarse_part, 0.0f,
+              nullptr, nullptr, probes, nullptr, s);
+  }
   const bool ip = a.metric == kMetricIP;
   if (pf.on) MVG_HIP(hipEventRecord(pf.ev[1], s));
   if (!a.split_scan) {

@@ -58,6 +58,8 @@ struct IvfSearchArgs {
   int precision;       // 0 plain FP32 (reference order), 1 mixed
   const float *ptab;   // mixed L2 tables [nlist][M][256]
   int split_scan;      // 1: one workgroup per (query, list) + merge pass
+  const uint16_t *coarse_bf16;  // mixed coarse: bf16 centroid rows (padded)
+  const float *coarse_norm;     // and their squared norms
 };
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> for the mixed L2 scan.
 void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, int nlist,
@@ -72,7 +74,7 @@ struct IvfProfile {
   DevBuf counter;
 };
 struct IvfWork {
-  DevBuf coarse_part, probes, part, keys;
+  DevBuf coarse_part, probes, part, keys, qbf, qnorm;
   IvfProfile prof;
 };
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
@@ -83,17 +85,38 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
             hipStream_t s);
 
+// ---- gemm.hip ----
+// Kinds of the MFMA GEMM-form distance: 0 bf16 L2, 1 bf16 IP, 2 int8 IP
+// (symmetric SQ8), 3 uint8 L2 rebased to int8 (affine SQ8).
+enum { kGemmBf16L2 = 0, kGemmBf16IP = 1, kGemmI8IP = 2, kGemmU8L2 = 3 };
+// Bytes per padded operand row (multiple of 64).
+int gemm_row_bytes(int kind, int d);
+// FP32 -> bf16 rows padded to gemm_row_bytes; norms (nullable) of the
+// rounded rows.
+void to_bf16(const float *x, int64_t n, int d, uint16_t *out, float *norms, hipStream_t s);
+// Per-(query, base split) top-kk keys part[nq][nsplit][kk] of the GEMM-form
+// distance; reduce with launch_merge over nsplit * kk candidates.
+void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t n, int d,
+               const float *qnorm, const float *xnorm, const uint32_t *qn_u, const uint32_t *qs_u,
+               const uint32_t *xn_u, const uint32_t *xs_u, float scale_sq, int kk,
+               const uint8_t *excl, int64_t excl_len, DevBuf &part, int &nsplit, hipStream_t s);
+
 // ---- sq8.hip ----
 void sq8_minmax(const float *x, int64_t count, float *out2, DevBuf &tmp, hipStream_t s);
+// Device code rows are zero-padded to this stride (a multiple of 64 bytes).
+int sq8_row_stride(int d);
+// Codes [n][stride] (padding zeroed), UInt32 norms and (nullable) UInt32 code
+// sums of n rows.
 void sq8_encode(const float *x, int64_t n, int d, int symmetric, float minimum, float inv_scale,
-                uint8_t *codes, uint32_t *norms, hipStream_t s);
+                uint8_t *codes, uint32_t *norms, uint32_t *sums, hipStream_t s);
 struct Sq8SearchArgs {
   const float *q;
   int64_t nq;
   int d, metric;
   int64_t n;
-  const uint8_t *codes;
+  const uint8_t *codes;   // [n][sq8_row_stride(d)]
   const uint32_t *norms;
+  const uint32_t *sums;
   const float *vectors;
   float gmin, scale;
   int k, k_factor;
@@ -101,9 +124,10 @@ struct Sq8SearchArgs {
   int64_t excl_len;
   float *D;
   int64_t *I;
+  int use_mfma;  // 1: int8 MFMA contraction; 0: dot4 VALU kernel (same results)
 };
 struct Sq8Work {
-  DevBuf qcodes, qnorms, part, keys;
+  DevBuf qcodes, qnorms, qsums, part, keys;
 };
 void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s);
 

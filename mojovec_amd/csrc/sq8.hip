@@ -44,16 +44,18 @@ __global__ void init_minmax_kernel(uint32_t *mm) {
   mm[1] = 0u;
 }
 
-// One wavefront per row: codes and the row's UInt32 code norm.
-__global__ void sq8_encode_kernel(const float *__restrict__ x, int64_t n, int d, int symmetric,
-                                  float minimum, float inv, uint8_t *__restrict__ codes,
-                                  uint32_t *__restrict__ norms) {
+// One wavefront per row: codes, the row's UInt32 code norm and (for the
+// rebased int8 contraction) the UInt32 sum of the uint8 codes.
+__global__ void sq8_encode_kernel(const float *__restrict__ x, int64_t n, int d, int ld,
+                                  int symmetric, float minimum, float inv,
+                                  uint8_t *__restrict__ codes, uint32_t *__restrict__ norms,
+                                  uint32_t *__restrict__ sums) {
   const int lane = threadIdx.x & 63;
   const int64_t row = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (row >= n) return;
   const float *xr = x + row * d;
-  uint8_t *cr = codes + row * d;
-  uint32_t norm = 0;
+  uint8_t *cr = codes + row * ld;
+  uint32_t norm = 0, sum = 0;
   for (int j = lane; j < d; j += 64) {
     if (symmetric) {
       float s = roundf(__fmul_rn(xr[j], inv));
@@ -67,11 +69,18 @@ __global__ void sq8_encode_kernel(const float *__restrict__ x, int64_t n, int d,
       const uint8_t c = (uint8_t)s;
       cr[j] = c;
       norm += (uint32_t)c * (uint32_t)c;
+      sum += c;
     }
   }
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) norm += (uint32_t)__shfl_xor((int)norm, o, 64);
-  if (lane == 0) norms[row] = norm;
+  for (int o = 32; o >= 1; o >>= 1) {
+    norm += (uint32_t)__shfl_xor((int)norm, o, 64);
+    sum += (uint32_t)__shfl_xor((int)sum, o, 64);
+  }
+  if (lane == 0) {
+    norms[row] = norm;
+    if (sums) sums[row] = sum;
+  }
 }
 
 // kTQ queries x one chunk of kCB rows: exact code dots, FP32 code distance,
@@ -83,7 +92,7 @@ __global__ __launch_bounds__(kNT) void sq8_chunk_topk_kernel(
     float scale_sq, int kk, const uint8_t *__restrict__ excl, int64_t excl_len,
     uint64_t *__restrict__ part, int nchunks) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int dw = d / 4;  // d % 4 == 0 on this path
+  const int dw = d / 4;  // d is the zero-padded code row stride (multiple of 64)
   uint32_t *qs = reinterpret_cast<uint32_t *>(smem);  // kTQ * dw
   uint32_t *keys = qs + kTQ * dw;                     // kTQ * kCB
   uint32_t *sel = keys + kTQ * kCB;                   // kk
@@ -176,11 +185,16 @@ void sq8_minmax(const float *x, int64_t count, float *out2, DevBuf &tmp, hipStre
   out2[1] = key2f(h[1]);
 }
 
+int sq8_row_stride(int d) { return (d + 63) & ~63; }
+
 void sq8_encode(const float *x, int64_t n, int d, int symmetric, float minimum, float inv_scale,
-                uint8_t *codes, uint32_t *norms, hipStream_t s) {
+                uint8_t *codes, uint32_t *norms, uint32_t *sums, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(sq8_encode_kernel, dim3(ceil_div(n, 4)), dim3(256), 0, s, x, n, d, symmetric,
-                     minimum, inv_scale, codes, norms);
+  const int ld = sq8_row_stride(d);
+  if (ld != d)  // zero padding: contributes nothing to uint8 or int8 dots
+    MVG_HIP(hipMemset2DAsync(codes + d, ld, 0, ld - d, n, s));
+  hipLaunchKernelGGL(sq8_encode_kernel, dim3(ceil_div(n, 4)), dim3(256), 0, s, x, n, d, ld,
+                     symmetric, minimum, inv_scale, codes, norms, sums);
   MVG_LAUNCH_CHECK("sq8_encode_kernel");
 }
 
@@ -189,19 +203,36 @@ void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s) {
   const int kf = a.k_factor > 1 ? a.k_factor : 1;
   const int64_t kk64 = (int64_t)a.k * kf;
   if (kk64 > kMaxCandidates) throw Error(kInvalid, "k * k_factor exceeds the device limit of 4096");
-  if (a.d % 4 != 0) throw Error(kInvalid, "SQ8 device search needs a dimension divisible by 4");
   const int kk = (int)kk64;
   const bool ip = a.metric == kMetricIP;
   const float inv = 1.0f / a.scale;
-  w.qcodes.reserve((size_t)a.nq * a.d);
+  const int ld = sq8_row_stride(a.d);
+  w.qcodes.reserve((size_t)a.nq * ld);
   w.qnorms.reserve(sizeof(uint32_t) * a.nq);
+  w.qsums.reserve(sizeof(uint32_t) * a.nq);
   sq8_encode(a.q, a.nq, a.d, ip ? 1 : 0, ip ? 0.0f : a.gmin, inv, w.qcodes.as<uint8_t>(),
-             w.qnorms.as<uint32_t>(), s);
-  const int nchunks = a.n > 0 ? ceil_div(a.n, kCB) : 0;
+             w.qnorms.as<uint32_t>(), w.qsums.as<uint32_t>(), s);
   const float ssq = a.scale * a.scale;
+  if (a.use_mfma && a.n > 0) {
+    int nsplit = 0;
+    gemm_topk(ip ? kGemmI8IP : kGemmU8L2, w.qcodes.as<uint8_t>(), a.codes, a.nq, a.n, ld, nullptr,
+              nullptr, w.qnorms.as<uint32_t>(), w.qsums.as<uint32_t>(), a.norms, a.sums, ssq, kk,
+              a.excl, a.excl_len, w.part, nsplit, s);
+    if (kf == 1) {
+      launch_merge(w.part.as<uint64_t>(), a.nq, (int64_t)nsplit * kk, a.k, ip ? 1 : 0, 0.0f, a.D, a.I,
+                   nullptr, nullptr, s);
+    } else {
+      w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
+      launch_merge(w.part.as<uint64_t>(), a.nq, (int64_t)nsplit * kk, kk, 0, 0.0f, nullptr, nullptr,
+                   nullptr, w.keys.as<uint64_t>(), s);
+      rerank(a.q, a.nq, a.d, a.metric, a.vectors, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s);
+    }
+    return;
+  }
+  const int nchunks = a.n > 0 ? ceil_div(a.n, kCB) : 0;
   if (nchunks > 0) {
     w.part.reserve(sizeof(uint64_t) * (size_t)a.nq * nchunks * kk);
-    const size_t sm = sizeof(uint32_t) * kTQ * (a.d / 4) + sizeof(uint32_t) * kTQ * kCB +
+    const size_t sm = sizeof(uint32_t) * kTQ * (ld / 4) + sizeof(uint32_t) * kTQ * kCB +
                       sizeof(uint32_t) * ((kk + 3) & ~3) + sizeof(SelectScratch);
     if (sm > 150 * 1024) throw Error(kInvalid, "SQ8 scan workspace exceeds LDS");
     const void *fn = ip ? (const void *)sq8_chunk_topk_kernel<true> : (const void *)sq8_chunk_topk_kernel<false>;
@@ -210,11 +241,11 @@ void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s) {
     dim3 grid(ceil_div(a.nq, kTQ), nchunks);
     if (ip)
       hipLaunchKernelGGL(sq8_chunk_topk_kernel<true>, grid, dim3(kNT), sm, s, a.codes, a.norms, a.n,
-                         a.d, w.qcodes.as<uint8_t>(), w.qnorms.as<uint32_t>(), a.nq, ssq, kk, a.excl,
+                         ld, w.qcodes.as<uint8_t>(), w.qnorms.as<uint32_t>(), a.nq, ssq, kk, a.excl,
                          a.excl_len, w.part.as<uint64_t>(), nchunks);
     else
       hipLaunchKernelGGL(sq8_chunk_topk_kernel<false>, grid, dim3(kNT), sm, s, a.codes, a.norms, a.n,
-                         a.d, w.qcodes.as<uint8_t>(), w.qnorms.as<uint32_t>(), a.nq, ssq, kk, a.excl,
+                         ld, w.qcodes.as<uint8_t>(), w.qnorms.as<uint32_t>(), a.nq, ssq, kk, a.excl,
                          a.excl_len, w.part.as<uint64_t>(), nchunks);
     MVG_LAUNCH_CHECK("sq8_chunk_topk_kernel");
   }

@@ -54,6 +54,12 @@ class IndexFlat:
     def ntotal(self) -> int:
         return int(_lib.load().mvg_flat_ntotal(self._h))
 
+    def set_search_mode(self, precision: str = "plain", k_factor: int = 1) -> None:
+        """"plain" FP32 (reference order) or "mixed" (bf16 MFMA candidates,
+        exact FP32 rerank of k * k_factor when k_factor > 1)."""
+        p = {"plain": 0, "mixed": 1}[precision]
+        _lib.check(_lib.load().mvg_flat_set_search_mode(self._h, p, int(k_factor)))
+
     def add(self, x) -> None:
         x = _f32(x, self.d)
         _lib.check(_lib.load().mvg_flat_add(self._h, _ptr(x), x.shape[0]))
@@ -96,6 +102,10 @@ class IndexFlatSQ8:
     @property
     def ntotal(self) -> int:
         return self.params()[3]
+
+    def set_search_mode(self, use_mfma: bool = True) -> None:
+        """int8 MFMA code contraction (default) or dot4 VALU; same results."""
+        _lib.check(_lib.load().mvg_sq8_set_search_mode(self._h, int(use_mfma)))
 
     @property
     def scale(self) -> float:

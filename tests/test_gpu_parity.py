@@ -61,6 +61,39 @@ def test_flat_integer_clustered_bit_exact(gpu):
     assert np.array_equal(Ig, Io) and np.array_equal(Dg, Do)
 
 
+@pytest.mark.parametrize("metric", [0, 1])
+def test_flat_mixed_bf16_mfma_is_exact_on_integer_data(gpu, metric):
+    # Integers up to 256 are exact in bf16 and every partial sum stays below
+    # 2^24, so the GEMM-form distance on MFMA equals the FP32 result.
+    base = clustered(11, 9000, 128)
+    q = clustered(12, 100, 128)
+    if metric == 1:
+        base, q = base - 128, q - 128  # signed values keep IP ranking non-trivial
+    ix = gpu.IndexFlat(128, metric)
+    ix.add(base)
+    Dp, Ip = ix.search(q, 10)
+    ix.set_search_mode("mixed")
+    Dm, Im = ix.search(q, 10)
+    assert np.array_equal(Im, Ip) and np.array_equal(Dm, Dp)
+    ix.set_search_mode("mixed", k_factor=4)
+    Dr, Ir = ix.search(q, 10)
+    assert np.array_equal(Ir, Ip) and np.array_equal(Dr, Dp)
+
+
+def test_flat_mixed_rerank_on_float_data(gpu):
+    rng = np.random.default_rng(13)
+    base = rng.standard_normal((20000, 96)).astype(np.float32)
+    q = rng.standard_normal((64, 96)).astype(np.float32)
+    ix = gpu.IndexFlat(96)
+    ix.add(base)
+    ix.set_search_mode("mixed", k_factor=8)
+    Dm, Im = ix.search(q, 10)
+    Do, Io = o.flat_search(base, q, 10)
+    assert np.mean(Im == Io) > 0.99
+    same = Im == Io
+    assert np.allclose(Dm[same], Do[same], rtol=1e-5)
+
+
 def test_flat_ties_exclusion_and_padding(gpu):
     g = np.load(os.path.join(GOLDEN, "ties.npz"))
     ix = gpu.IndexFlat(16)
@@ -246,12 +279,15 @@ def test_ivfpq_untrained_and_empty(gpu):
 
 
 # ------------------------------------------------------------------ SQ8
+@pytest.mark.parametrize("use_mfma", [True, False])
+@pytest.mark.parametrize("d", [128, 100, 3])
 @pytest.mark.parametrize("metric", [0, 1])
-def test_sq8_codes_and_search_match_oracle(gpu, metric):
-    rng = np.random.default_rng(30 + metric)
-    x = rng.standard_normal((6000, 128)).astype(np.float32)
-    gx = gpu.IndexFlatSQ8(128, metric)
-    ox = o.SQ8Index(128, metric)
+def test_sq8_codes_and_search_match_oracle(gpu, metric, d, use_mfma):
+    rng = np.random.default_rng(30 + metric + d)
+    x = rng.standard_normal((6000, d)).astype(np.float32)
+    gx = gpu.IndexFlatSQ8(d, metric)
+    gx.set_search_mode(use_mfma)
+    ox = o.SQ8Index(d, metric)
     for part in (x[:2000] * 0.5, x[2000:]):  # second batch widens the range
         gx.add(part)
         ox.add(part)
@@ -259,7 +295,7 @@ def test_sq8_codes_and_search_match_oracle(gpu, metric):
     cg, ng = gx.codes()
     co, no = ox.codes()
     assert np.array_equal(cg, co) and np.array_equal(ng, no)
-    q = rng.standard_normal((33, 128)).astype(np.float32)
+    q = rng.standard_normal((33, d)).astype(np.float32)
     for k, kf in [(1, 1), (10, 1), (10, 2), (10, 4), (100, 1)]:
         Dg, Ig = gx.search(q, k, k_factor=kf)
         Do, Io = ox.search(q, k, k_factor=kf)
