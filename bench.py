@@ -85,11 +85,18 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--workload", default="ivfpq", choices=["ivfpq", "flat", "sq8"],
+                    help="ivfpq: the headline 1M x 128 IVF-PQ line; flat / sq8: the Flat "
+                         "100k x 128 and SQ8 1M x 768 side measurements")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(every box of this build has a single MI355X)", file=sys.stderr)
         return 2
+    if args.workload == "flat":
+        return bench_flat(args)
+    if args.workload == "sq8":
+        return bench_sq8(args)
 
     import torch
 
@@ -267,6 +274,124 @@ def cpu_baseline(ix, x_h, q_h, nprobe, kf, budget_s):
     dt = time.perf_counter() - t0
     return {"value": round(n / dt, 1), "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"{n} of the 10,000 queries, same index, nprobe={nprobe}, k_factor={kf}"}
+
+
+def _timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_flat(args):
+    """Exact Flat L2: 100k x 128 integer-valued mixture (100 equal clusters,
+    sigma 12), 1,000 queries, k = 10. Plain FP32 (reference operation order)
+    and the bf16 MFMA GEMM form (exact on this integer data)."""
+    import torch
+
+    import mojovec_amd as mv
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    n, nq, d, k = 100_000, 1_000, 128, 10
+    centres = torch.rand((100, d), generator=g, device=dev) * 128.0
+
+    def draw(m):
+        lab = torch.randint(0, 100, (m,), generator=g, device=dev)
+        return (centres[lab] + torch.randn((m, d), generator=g, device=dev) * 12.0).clamp_(0, 255).round_()
+
+    x, q = draw(n).contiguous(), draw(nq).contiguous()
+    x_h, q_h = x.cpu().numpy(), q.cpu().numpy()
+    gt = ground_truth(torch, x, q, k)
+    ix = mv.IndexFlat(d)
+    ix.add(x_h)
+    out = {}
+    for form, peak, unit in (("plain", 157.3, "TFLOP/s"), ("mixed", 2500.0, "TFLOP/s")):
+        ix.set_search_mode(form)
+        res = {}
+
+        def run():
+            res["r"] = ix.search(q_h, k)
+
+        dt = _timed(torch, run, args.steps, args.warmup)
+        ids = torch.from_numpy(res["r"][1]).to(dev)
+        tflops = 2.0 * n * nq * d / dt / 1e12
+        out[form] = {"qps": round(nq / dt, 1), "ms": round(dt * 1e3, 3),
+                     "recall_at_10": round(recall_at(torch, gt, ids), 4),
+                     "achieved_tflops": round(tflops, 2), "peak": peak, "frac": round(tflops / peak, 4)}
+        log(f"flat {form}: {out[form]}")
+    print(json.dumps({"workload": "Flat L2 100,000 x 128, 1,000 queries, k=10 (host buffers, PCIe included)",
+                      "forms": out}), flush=True)
+    return 0
+
+
+def bench_sq8(args):
+    """SQ8 Flat with exact rerank: 1M x 768 unit-norm embeddings (2,000
+    directions, angular spread 0.3 rad), cosine (int8 codes) and L2 (uint8
+    codes rebased to int8) on int8 MFMA, k = 10, k_factor in {1, 2, 4};
+    10,000 queries; recall and max distance error against FP64."""
+    import torch
+
+    import mojovec_amd as mv
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(21)
+    n, nq, d, k = 1_000_000, 10_000, 768, 10
+    dirs = torch.nn.functional.normalize(torch.randn((2000, d), generator=g, device=dev), dim=1)
+
+    def draw(m):
+        lab = torch.randint(0, 2000, (m,), generator=g, device=dev)
+        v = dirs[lab] + (0.3 / d ** 0.5) * torch.randn((m, d), generator=g, device=dev)
+        return torch.nn.functional.normalize(v, dim=1).contiguous()
+
+    x, q = draw(n), draw(nq)
+    log("sq8 data ready")
+    x_h, q_h = x.cpu().numpy(), q.cpu().numpy()
+    out = {}
+    for metric, name in ((1, "cosine_int8"), (0, "l2_uint8")):
+        x64 = x.double()
+        gt_d, gt_i = [], []
+        for s in range(0, nq, 250):
+            qq = q[s:s + 250].double()
+            if metric == 1:
+                dd = -(qq @ x64.T)
+            else:
+                dd = (qq * qq).sum(1)[:, None] + (x64 * x64).sum(1)[None, :] - 2.0 * (qq @ x64.T)
+            v, i = torch.topk(dd, k, dim=1, largest=False)
+            gt_d.append(v)
+            gt_i.append(i)
+        del x64
+        gt_d, gt_i = torch.cat(gt_d), torch.cat(gt_i)
+        ix = mv.IndexFlatSQ8(d, metric)
+        ix.add(x_h)
+        log(f"sq8 {name} index ready")
+        for kf in (1, 2, 4):
+            res = {}
+
+            def run():
+                res["r"] = ix.search(q_h, k, k_factor=kf)
+
+            dt = _timed(torch, run, max(1, args.steps // 4), 1)
+            D = torch.from_numpy(res["r"][0]).to(dev).double()
+            ids = torch.from_numpy(res["r"][1]).to(dev)
+            exact = -gt_d if metric == 1 else gt_d
+            err = float((D[:, 0] - exact[:, 0]).abs().max()) if kf > 1 else None
+            tops = 2.0 * n * nq * d / dt / 1e12
+            out[f"{name}_kf{kf}"] = {"qps": round(nq / dt, 1), "ms": round(dt * 1e3, 2),
+                                     "recall_at_10": round(recall_at(torch, gt_i, ids), 4),
+                                     "max_top1_dist_err": err, "achieved_tops": round(tops, 1),
+                                     "peak_tops": 5000.0, "frac": round(tops / 5000.0, 4)}
+            log(f"sq8 {name} kf={kf}: {out[f'{name}_kf{kf}']}")
+        del ix
+    print(json.dumps({"workload": "SQ8 Flat 1,000,000 x 768 unit-norm, 10,000 queries, k=10 "
+                                  "(host buffers, PCIe included)", "forms": out}), flush=True)
+    return 0
 
 
 if __name__ == "__main__":

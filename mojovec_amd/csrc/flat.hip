@@ -20,103 +20,6 @@ constexpr int kCB = 1024;   // base vectors per chunk
 constexpr int kPerThread = kCB / kNT;
 constexpr int kMergeCap = 8192;
 
-template <int D>
-__device__ __forceinline__ float l2_w8_fixed(const float *xr, const float *q) {
-  float a0[8], a1[8], a2[8], a3[8];
-#pragma unroll
-  for (int j = 0; j < 8; j++) a0[j] = a1[j] = a2[j] = a3[j] = 0.0f;
-  constexpr int NB4 = D / 32;
-#pragma unroll
-  for (int b = 0; b < NB4; b++) {
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int i = b * 32;
-      float d0 = q[i + j] - xr[i + j];
-      float d1 = q[i + 8 + j] - xr[i + 8 + j];
-      float d2 = q[i + 16 + j] - xr[i + 16 + j];
-      float d3 = q[i + 24 + j] - xr[i + 24 + j];
-      a0[j] = fma_(d0, d0, a0[j]);
-      a1[j] = fma_(d1, d1, a1[j]);
-      a2[j] = fma_(d2, d2, a2[j]);
-      a3[j] = fma_(d3, d3, a3[j]);
-    }
-  }
-  float v[8];
-#pragma unroll
-  for (int j = 0; j < 8; j++) v[j] = ((a0[j] + a1[j]) + a2[j]) + a3[j];
-  constexpr int I1 = NB4 * 32;
-  constexpr int NW8 = (D - I1) / 8;
-#pragma unroll
-  for (int w = 0; w < NW8; w++) {
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      float df = q[I1 + w * 8 + j] - xr[I1 + w * 8 + j];
-      v[j] = fma_(df, df, v[j]);
-    }
-  }
-  float res = tree_sum<8>(v);
-  constexpr int I2 = I1 + NW8 * 8;
-  constexpr int NW4 = (D - I2) / 4;
-#pragma unroll
-  for (int w = 0; w < NW4; w++) {
-    float r[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      float df = q[I2 + w * 4 + j] - xr[I2 + w * 4 + j];
-      r[j] = df * df;
-    }
-    res += tree_sum<4>(r);
-  }
-#pragma unroll
-  for (int i = I2 + NW4 * 4; i < D; i++) {
-    float df = q[i] - xr[i];
-    res += df * df;
-  }
-  return res;
-}
-
-template <int D>
-__device__ __forceinline__ float ip_w8_fixed(const float *xr, const float *q) {
-  float p0[8], p1[8], p2[8], p3[8];
-#pragma unroll
-  for (int j = 0; j < 8; j++) p0[j] = p1[j] = p2[j] = p3[j] = 0.0f;
-  constexpr int NB4 = D / 32;
-#pragma unroll
-  for (int b = 0; b < NB4; b++) {
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int i = b * 32;
-      p0[j] = fma_(q[i + j], xr[i + j], p0[j]);
-      p1[j] = fma_(q[i + 8 + j], xr[i + 8 + j], p1[j]);
-      p2[j] = fma_(q[i + 16 + j], xr[i + 16 + j], p2[j]);
-      p3[j] = fma_(q[i + 24 + j], xr[i + 24 + j], p3[j]);
-    }
-  }
-  float v[8];
-#pragma unroll
-  for (int j = 0; j < 8; j++) v[j] = ((p0[j] + p1[j]) + p2[j]) + p3[j];
-  constexpr int I1 = NB4 * 32;
-  constexpr int NW8 = (D - I1) / 8;
-#pragma unroll
-  for (int w = 0; w < NW8; w++) {
-#pragma unroll
-    for (int j = 0; j < 8; j++) v[j] = fma_(q[I1 + w * 8 + j], xr[I1 + w * 8 + j], v[j]);
-  }
-  float res = tree_sum<8>(v);
-  constexpr int I2 = I1 + NW8 * 8;
-  constexpr int NW4 = (D - I2) / 4;
-#pragma unroll
-  for (int w = 0; w < NW4; w++) {
-    float r[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) r[j] = fma_(q[I2 + w * 4 + j], xr[I2 + w * 4 + j], 0.0f);
-    res += tree_sum<4>(r);
-  }
-#pragma unroll
-  for (int i = I2 + NW4 * 4; i < D; i++) res += q[i] * xr[i];
-  return res;
-}
-
 // Distance of LDS query `qs` to base row `xg` (global) or its register copy.
 template <int D, bool IP, bool KMEANS>
 __device__ __forceinline__ float pair_dist(const float *xr, const float *xg, const float *qs, int d) {

@@ -249,7 +249,10 @@ struct QueryScanLayout {
   }
 };
 
-template <int MF, bool IP, bool MIXED>
+// DF > 0 (with MF > 0): the plain L2 LUT keeps PQ centroid `tid` of every
+// subspace in registers (DF floats per thread), so each per-list LUT rebuild
+// reads only the LDS residual.
+template <int MF, bool IP, bool MIXED, int DF>
 __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
     const float *__restrict__ q, int d, int M, const float *__restrict__ coarse,
     const float *__restrict__ pq, const float *__restrict__ ptab,
@@ -281,6 +284,16 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
     st->cnt = 0;
     st->thr = kSentinel64;
   }
+  constexpr bool kRegCb = !IP && !MIXED && DF > 0 && MF > 0;
+  constexpr int kDsub = (DF > 0 && MF > 0) ? DF / MF : 1;
+  float cbr[kRegCb ? DF : 1];
+  if constexpr (kRegCb) {
+#pragma unroll
+    for (int m = 0; m < MF; m++) {
+#pragma unroll
+      for (int j = 0; j < kDsub; j++) cbr[m * kDsub + j] = pq[((int64_t)m * kKsub + tid) * kDsub + j];
+    }
+  }
   // Per-query tables: IP LUT (pq.mojo:285-292) or the mixed -2<q_m, c_mk>.
   if (IP || MIXED) {
     for (int j = tid; j < d; j += kNT) rv[j] = qq[j];
@@ -310,9 +323,15 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
       if (!MIXED) {
         for (int j = tid; j < d; j += kNT) rv[j] = qq[j] - cl[j];  // :306-310
         __syncthreads();
-        for (int e = tid; e < nlut; e += kNT) {
-          const int m = e >> 8;
-          lut[e] = l2_kmeans(rv + m * dsub, pq + (int64_t)e * dsub, dsub);
+        if constexpr (kRegCb) {
+#pragma unroll
+          for (int m = 0; m < MF; m++)
+            lut[m * kKsub + tid] = l2_w8_fixed<kDsub>(cbr + m * kDsub, rv + m * kDsub);
+        } else {
+          for (int e = tid; e < nlut; e += kNT) {
+            const int m = e >> 8;
+            lut[e] = l2_kmeans(rv + m * dsub, pq + (int64_t)e * dsub, dsub);
+          }
         }
       } else {
         const float *tl = ptab + (int64_t)l * nlut;
@@ -420,20 +439,39 @@ void launch_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk
   MVG_LAUNCH_CHECK("ivfpq_scan_kernel");
 }
 
-template <int MF, bool IP, bool MIXED>
-void launch_query_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
-                       uint64_t *out_keys, hipStream_t s) {
+template <int MF, bool IP, bool MIXED, int DF>
+void launch_query_scan_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                          uint64_t *out_keys, hipStream_t s) {
   const QueryScanLayout lay(a.M, a.d, kk, MIXED && !IP);
   const size_t sm = (size_t)lay.total;
   if (sm > 150 * 1024) throw Error(kInvalid, "IVF-PQ scan workspace exceeds LDS");
   if (sm > 64 * 1024)
-    MVG_HIP(hipFuncSetAttribute((const void *)ivfpq_query_scan_kernel<MF, IP, MIXED>,
+    MVG_HIP(hipFuncSetAttribute((const void *)ivfpq_query_scan_kernel<MF, IP, MIXED, DF>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-  hipLaunchKernelGGL((ivfpq_query_scan_kernel<MF, IP, MIXED>), dim3((unsigned)a.nq), dim3(kNT), sm, s,
+  hipLaunchKernelGGL((ivfpq_query_scan_kernel<MF, IP, MIXED, DF>), dim3((unsigned)a.nq), dim3(kNT), sm, s,
                      a.q, a.d, a.M, a.coarse, a.pq, a.ptab, a.list_off, a.codes, a.ids, probes, maxp,
                      a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D, a.I,
                      kIvfPadDist);
   MVG_LAUNCH_CHECK("ivfpq_query_scan_kernel");
+}
+
+// Register-resident codebooks for the plain L2 shapes (M, d) in
+// {(16, 128), (12, 96), (8, 64), (16, 64), (32, 128)}; other shapes read the
+// codebook through the cache.
+template <int MF, bool IP, bool MIXED>
+void launch_query_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                       uint64_t *out_keys, hipStream_t s) {
+  if constexpr (!IP && !MIXED && MF == 16) {
+    if (a.d == 128) return launch_query_scan_df<MF, IP, MIXED, 128>(a, maxp, probes, kk, out_keys, s);
+    if (a.d == 64) return launch_query_scan_df<MF, IP, MIXED, 64>(a, maxp, probes, kk, out_keys, s);
+  }
+  if constexpr (!IP && !MIXED && MF == 12) {
+    if (a.d == 96) return launch_query_scan_df<MF, IP, MIXED, 96>(a, maxp, probes, kk, out_keys, s);
+  }
+  if constexpr (!IP && !MIXED && MF == 8) {
+    if (a.d == 64) return launch_query_scan_df<MF, IP, MIXED, 64>(a, maxp, probes, kk, out_keys, s);
+  }
+  launch_query_scan_df<MF, IP, MIXED, 0>(a, maxp, probes, kk, out_keys, s);
 }
 
 }  // namespace
