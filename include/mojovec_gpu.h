@@ -149,6 +149,12 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
  * split_scan 1 selects one workgroup per (query, list) with a merge pass
  * instead of one workgroup per query (same results). */
 int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan);
+/* Skewed-list load balance (no reference counterpart): entries_per_item > 0
+ * runs the scan as persistent workgroups over a device-built queue of
+ * (query, entry-range) items of about that many list entries (at most 8 per
+ * query), merged per query afterwards; 0 (default) scans one query per
+ * workgroup. Results are identical. */
+int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item);
 /* Stage timing of the next searches (no reference counterpart; used by the
  * benchmark): when on, each search records HIP events on its stream around
  * the coarse, scan and merge/rerank stages and counts the list entries the

@@ -142,7 +142,7 @@ struct mvg_sq8 {
 struct mvg_ivfpq {
   int d = 0, nlist = 0, M = 0, metric = 0, nprobe = 1;
   bool trained = false, keep_vectors = true, dirty = true, tables_dirty = true;
-  int precision = 0, split_scan = 0;
+  int precision = 0, split_scan = 0, wq_entries = 0;
   DevBuf ptab, coarse_bf16, coarse_norm;
   int64_t ntotal = 0;
   std::vector<std::vector<int32_t>> lids;
@@ -650,6 +650,7 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.I = labels;
   a.precision = h->precision;
   a.split_scan = h->split_scan;
+  a.wq_entries = h->wq_entries;
   if (h->precision != 0) {
     if (h->tables_dirty) {
       if (h->metric == MVG_METRIC_L2) {
@@ -715,6 +716,13 @@ int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan) {
   require(h && (precision == 0 || precision == 1), "precision must be 0 (plain) or 1 (mixed)");
   h->precision = precision;
   h->split_scan = split_scan != 0;
+  API_END
+}
+
+int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item) {
+  API_BEGIN
+  require(h && entries_per_item >= 0, "entries_per_item must be >= 0");
+  h->wq_entries = entries_per_item;
   API_END
 }
 

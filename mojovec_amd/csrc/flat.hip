@@ -173,18 +173,21 @@ __global__ __launch_bounds__(kNT) void flat_argmin_kernel(
 __global__ __launch_bounds__(kNT) void merge_topk_kernel(
     const uint64_t *__restrict__ cand, int64_t ncand, int k, int negate, float pad_dist,
     float *__restrict__ D, int64_t *__restrict__ I64, int32_t *__restrict__ I32,
-    uint64_t *__restrict__ out_keys) {
+    uint64_t *__restrict__ out_keys, const int64_t *__restrict__ seg_off) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t *buf = reinterpret_cast<uint64_t *>(smem);  // kMergeCap
   uint64_t *top = buf + kMergeCap;                       // kMaxCandidates
-  SelectScratch &scr = *reinterpret_cast<SelectScratch *>(top + kMaxCandidates);
+  SelectScratch64 &scr = *reinterpret_cast<SelectScratch64 *>(top + kMaxCandidates);
   const int tid = threadIdx.x;
   const int64_t qi = blockIdx.x;
-  const uint64_t *src = cand + qi * ncand;
+  // Fixed-size candidate rows, or (seg_off) rows of seg_off[q+1]-seg_off[q]
+  // units of ncand keys.
+  const uint64_t *src = seg_off ? cand + seg_off[qi] * ncand : cand + qi * ncand;
+  const int64_t n = seg_off ? (seg_off[qi + 1] - seg_off[qi]) * ncand : ncand;
   int cur = 0;
   const int room = kMergeCap - k;
-  for (int64_t b = 0; b < ncand; b += room) {
-    const int len = (int)min<int64_t>(room, ncand - b);
+  for (int64_t b = 0; b < n; b += room) {
+    const int len = (int)min<int64_t>(room, n - b);
     for (int i = tid; i < cur; i += kNT) buf[i] = top[i];
     for (int i = tid; i < len; i += kNT) buf[cur + i] = src[b + i];
     __syncthreads();
@@ -210,10 +213,10 @@ __global__ __launch_bounds__(kNT) void merge_topk_kernel(
 
 void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int negate,
                   float pad_dist, float *D, int64_t *I64, int32_t *I32, uint64_t *keys,
-                  hipStream_t s) {
+                  hipStream_t s, const int64_t *seg_off) {
   if (nq == 0) return;
   if (k > kMaxCandidates) throw Error(kInvalid, "k exceeds the device top-k limit of 4096");
-  const size_t sm = sizeof(uint64_t) * (kMergeCap + kMaxCandidates) + sizeof(SelectScratch);
+  const size_t sm = sizeof(uint64_t) * (kMergeCap + kMaxCandidates) + sizeof(SelectScratch64);
   static bool attr_set = false;
   if (!attr_set) {
     MVG_HIP(hipFuncSetAttribute((const void *)merge_topk_kernel,
@@ -221,7 +224,7 @@ void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int ne
     attr_set = true;
   }
   hipLaunchKernelGGL(merge_topk_kernel, dim3((unsigned)nq), dim3(kNT), sm, s, cand, ncand, k,
-                     negate, pad_dist, D, I64, I32, keys);
+                     negate, pad_dist, D, I64, I32, keys, seg_off);
   MVG_LAUNCH_CHECK("merge_topk_kernel");
 }
 

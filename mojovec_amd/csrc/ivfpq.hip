@@ -194,7 +194,66 @@ struct QueryScanShared {
   int cnt;
   uint64_t thr;
   float base;
+  int64_t item;
 };
+
+// Publishes this workgroup's kk-th key to the query's shared threshold and
+// takes the tighter of the two (called right after a flush).
+__device__ void share_threshold(uint64_t *g, QueryScanShared &st) {
+  if (threadIdx.x == 0) {
+    const uint64_t loc = st.thr;
+    const uint64_t old = atomicMin(reinterpret_cast<unsigned long long *>(g), (unsigned long long)loc);
+    if (old < loc) st.thr = old;
+  }
+  __syncthreads();
+}
+
+// Work-queue setup: per query, the entry offsets of its probes in coarse
+// order, the item count ceil(total / R) clamped to [1, kMaxItemsPerQuery],
+// the per-item entry count and a fresh threshold.
+constexpr int kMaxItemsPerQuery = 8;
+__global__ void wq_setup_kernel(const int32_t *__restrict__ probes, int64_t nq, int maxp,
+                                const int32_t *__restrict__ nprobe_q, int nprobe_all,
+                                const int64_t *__restrict__ list_off, int64_t R,
+                                int64_t *__restrict__ poff, int *__restrict__ nitems,
+                                int64_t *__restrict__ rq, uint64_t *__restrict__ gthr) {
+  const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= nq) return;
+  int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
+  np = min(max(np, 1), maxp);
+  int64_t acc = 0;
+  int64_t *po = poff + qi * (int64_t)(maxp + 1);
+  for (int p = 0; p < np; p++) {
+    po[p] = acc;
+    const int l = probes[qi * maxp + p];
+    if (l >= 0) acc += list_off[l + 1] - list_off[l];
+  }
+  po[np] = acc;
+  const int64_t n_items = min<int64_t>(kMaxItemsPerQuery, max<int64_t>(1, (acc + R - 1) / R));
+  nitems[qi] = (int)n_items;
+  rq[qi] = max<int64_t>(1, (acc + n_items - 1) / n_items);
+  gthr[qi] = kSentinel64;
+}
+
+// Exclusive scan of the item counts (one workgroup) and queue reset.
+__global__ __launch_bounds__(kNT) void wq_offsets_kernel(const int *__restrict__ nitems, int64_t nq,
+                                                         int64_t *__restrict__ item_off,
+                                                         unsigned *__restrict__ counter) {
+  __shared__ int sh[kNW + 1];
+  int64_t carry = 0;
+  for (int64_t b = 0; b < nq; b += kNT) {
+    const int64_t i = b + threadIdx.x;
+    const int v = i < nq ? nitems[i] : 0;
+    int total;
+    const int ex = block_excl_scan(v, sh, total);
+    if (i < nq) item_off[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) {
+    item_off[nq] = carry;
+    *counter = 0u;
+  }
+}
 
 // Merges the append buffer cand[kk .. kk+cnt) into the sorted top set
 // cand[0 .. kk) (sentinel padded). Leaves cand[0..kk) sorted, cnt = 0.
@@ -252,7 +311,25 @@ struct QueryScanLayout {
 // DF > 0 (with MF > 0): the plain L2 LUT keeps PQ centroid `tid` of every
 // subspace in registers (DF floats per thread), so each per-list LUT rebuild
 // reads only the LDS residual.
-template <int MF, bool IP, bool MIXED, int DF>
+//
+// WQ (work queue): the workgroups are persistent and pull items from a
+// device counter. An item is a range of at most R consecutive entries of one
+// query's probed lists (in coarse order), so a skewed list is shared by
+// several workgroups. Each item writes its top-kk to part[item]; a segmented
+// merge reduces the items of a query. Items of one query share a threshold
+// (gthr[q], atomicMin of each item's kk-th key): a final winner is never
+// above it, since every id lives in exactly one item.
+struct WorkQueue {
+  int64_t nq;
+  const int64_t *item_off;  // [nq + 1] first item of each query
+  const int64_t *poff;      // [nq][maxp + 1] entry offsets of the probes
+  const int64_t *rq;        // [nq] entries per item of each query
+  unsigned *counter;        // next item
+  uint64_t *gthr;           // [nq] shared threshold
+  uint64_t *part;           // [items][kk]
+};
+
+template <int MF, bool IP, bool MIXED, int DF, bool WQ>
 __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
     const float *__restrict__ q, int d, int M, const float *__restrict__ coarse,
     const float *__restrict__ pq, const float *__restrict__ ptab,
@@ -260,7 +337,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
     const int32_t *__restrict__ ids, const int32_t *__restrict__ probes, int maxp,
     const int32_t *__restrict__ nprobe_q, int nprobe_all, int kk, int k, const uint8_t *__restrict__ excl,
     int64_t excl_len, uint64_t *__restrict__ out_keys, float *__restrict__ D,
-    int64_t *__restrict__ I, float pad_dist) {
+    int64_t *__restrict__ I, float pad_dist, WorkQueue wq) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int nlut = M * kKsub;
   const QueryScanLayout lay(M, d, kk, MIXED && !IP);
@@ -273,17 +350,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
   QueryScanShared *st = reinterpret_cast<QueryScanShared *>(smem + lay.st);
 
   const int tid = threadIdx.x;
-  const int64_t qi = blockIdx.x;
-  const float *qq = q + qi * d;
   const int dsub = d / M;
-  int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
-  np = min(max(np, 1), maxp);
-
-  for (int i = tid; i < kk; i += kNT) cand[i] = kSentinel64;
-  if (tid == 0) {
-    st->cnt = 0;
-    st->thr = kSentinel64;
-  }
   constexpr bool kRegCb = !IP && !MIXED && DF > 0 && MF > 0;
   constexpr int kDsub = (DF > 0 && MF > 0) ? DF / MF : 1;
   float cbr[kRegCb ? DF : 1];
@@ -293,6 +360,37 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
 #pragma unroll
       for (int j = 0; j < kDsub; j++) cbr[m * kDsub + j] = pq[((int64_t)m * kKsub + tid) * kDsub + j];
     }
+  }
+  for (;;) {
+  // ---- pick the work: a whole query, or one queue item
+  int64_t qi, item = 0, begin = 0, end = INT64_MAX;
+  if constexpr (WQ) {
+    __syncthreads();
+    if (tid == 0) st->item = (int64_t)atomicAdd(wq.counter, 1u);
+    __syncthreads();
+    item = st->item;
+    const int64_t nq_total = wq.nq;
+    if (item >= wq.item_off[nq_total]) break;
+    int64_t lo = 0, hi = nq_total;  // last query whose first item <= item
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (wq.item_off[mid] <= item) lo = mid; else hi = mid;
+    }
+    qi = lo;
+    begin = (item - wq.item_off[qi]) * wq.rq[qi];
+    end = begin + wq.rq[qi];
+  } else {
+    qi = blockIdx.x;
+  }
+  const float *qq = q + qi * d;
+  int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
+  np = min(max(np, 1), maxp);
+  const int64_t *poff = WQ ? wq.poff + qi * (int64_t)(maxp + 1) : nullptr;
+
+  for (int i = tid; i < kk; i += kNT) cand[i] = kSentinel64;
+  if (tid == 0) {
+    st->cnt = 0;
+    st->thr = WQ ? wq.gthr[qi] : kSentinel64;
   }
   // Per-query tables: IP LUT (pq.mojo:285-292) or the mixed -2<q_m, c_mk>.
   if (IP || MIXED) {
@@ -312,12 +410,22 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
   }
   __syncthreads();
 
-  for (int p = 0; p < np; p++) {
+  int p = 0;
+  if constexpr (WQ) {  // first probe whose entries reach `begin`
+    while (p + 1 < np && poff[p + 1] <= begin) p++;
+  }
+  for (; p < np; p++) {
     const int l = probes[qi * maxp + p];
     if (l < 0) continue;
     const int64_t off = list_off[l];
-    const int64_t L = list_off[l + 1] - off;
-    if (L == 0) continue;
+    int64_t L = list_off[l + 1] - off;
+    int64_t seg_lo = 0;
+    if constexpr (WQ) {
+      if (poff[p] >= end) break;
+      seg_lo = max<int64_t>(begin - poff[p], 0);
+      L = min<int64_t>(L, end - poff[p]);
+    }
+    if (L <= seg_lo) continue;
     const float *cl = coarse + (int64_t)l * d;
     if (!IP) {
       if (!MIXED) {
@@ -343,7 +451,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
     }
     __syncthreads();
     const float base = (IP || MIXED) ? st->base : 0.0f;
-    for (int64_t r0 = 0; r0 < L; r0 += kRound) {
+    for (int64_t r0 = seg_lo; r0 < L; r0 += kRound) {
       const uint64_t thr = st->thr;
 #pragma unroll
       for (int u = 0; u < kRound / kNT; u++) {
@@ -359,11 +467,22 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
         }
       }
       __syncthreads();
-      if (st->cnt > kCandCap - kRound) flush_candidates(cand, tmp, kk, *st, *scr);
+      if (st->cnt > kCandCap - kRound) {
+        flush_candidates(cand, tmp, kk, *st, *scr);
+        if (WQ) share_threshold(wq.gthr + qi, *st);
+      }
     }
-    if (st->cnt > 0) flush_candidates(cand, tmp, kk, *st, *scr);
+    if (st->cnt > 0) {
+      flush_candidates(cand, tmp, kk, *st, *scr);
+      if (WQ) share_threshold(wq.gthr + qi, *st);
+    }
   }
   __syncthreads();
+  if constexpr (WQ) {
+    uint64_t *dst = wq.part + item * kk;
+    for (int t = tid; t < kk; t += kNT) dst[t] = cand[t];
+    continue;
+  }
   // Output: sorted top-kk keys for the rerank, or final distances / ids.
   if (out_keys) {
     for (int t = tid; t < kk; t += kNT) out_keys[qi * kk + t] = cand[t];
@@ -381,6 +500,8 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
       I[qi * k + t] = iv;
     }
   }
+  break;
+  }  // for (;;)
 }
 
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> (mixed L2 tables).
@@ -439,20 +560,43 @@ void launch_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk
   MVG_LAUNCH_CHECK("ivfpq_scan_kernel");
 }
 
-template <int MF, bool IP, bool MIXED, int DF>
-void launch_query_scan_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
-                          uint64_t *out_keys, hipStream_t s) {
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    MVG_HIP(hipGetDevice(&dev));
+    MVG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  return cus;
+}
+
+template <int MF, bool IP, bool MIXED, int DF, bool WQ>
+void launch_query_scan_wq(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                          uint64_t *out_keys, const WorkQueue &wq, hipStream_t s) {
   const QueryScanLayout lay(a.M, a.d, kk, MIXED && !IP);
   const size_t sm = (size_t)lay.total;
   if (sm > 150 * 1024) throw Error(kInvalid, "IVF-PQ scan workspace exceeds LDS");
+  const void *fn = (const void *)ivfpq_query_scan_kernel<MF, IP, MIXED, DF, WQ>;
   if (sm > 64 * 1024)
-    MVG_HIP(hipFuncSetAttribute((const void *)ivfpq_query_scan_kernel<MF, IP, MIXED, DF>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-  hipLaunchKernelGGL((ivfpq_query_scan_kernel<MF, IP, MIXED, DF>), dim3((unsigned)a.nq), dim3(kNT), sm, s,
+    MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  unsigned grid = (unsigned)a.nq;
+  if (WQ) {  // persistent: enough workgroups to fill every CU
+    int per_cu = 0;
+    MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
+    grid = (unsigned)std::max(1, per_cu) * (unsigned)device_cus();
+  }
+  hipLaunchKernelGGL((ivfpq_query_scan_kernel<MF, IP, MIXED, DF, WQ>), dim3(grid), dim3(kNT), sm, s,
                      a.q, a.d, a.M, a.coarse, a.pq, a.ptab, a.list_off, a.codes, a.ids, probes, maxp,
                      a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D, a.I,
-                     kIvfPadDist);
+                     kIvfPadDist, wq);
   MVG_LAUNCH_CHECK("ivfpq_query_scan_kernel");
+}
+
+template <int MF, bool IP, bool MIXED, int DF>
+void launch_query_scan_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                          uint64_t *out_keys, const WorkQueue *wq, hipStream_t s) {
+  if (wq) launch_query_scan_wq<MF, IP, MIXED, DF, true>(a, maxp, probes, kk, out_keys, *wq, s);
+  else launch_query_scan_wq<MF, IP, MIXED, DF, false>(a, maxp, probes, kk, out_keys, WorkQueue{}, s);
 }
 
 // Register-resident codebooks for the plain L2 shapes (M, d) in
@@ -460,18 +604,18 @@ void launch_query_scan_df(const IvfSearchArgs &a, int maxp, const int32_t *probe
 // codebook through the cache.
 template <int MF, bool IP, bool MIXED>
 void launch_query_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
-                       uint64_t *out_keys, hipStream_t s) {
+                       uint64_t *out_keys, const WorkQueue *wq, hipStream_t s) {
   if constexpr (!IP && !MIXED && MF == 16) {
-    if (a.d == 128) return launch_query_scan_df<MF, IP, MIXED, 128>(a, maxp, probes, kk, out_keys, s);
-    if (a.d == 64) return launch_query_scan_df<MF, IP, MIXED, 64>(a, maxp, probes, kk, out_keys, s);
+    if (a.d == 128) return launch_query_scan_df<MF, IP, MIXED, 128>(a, maxp, probes, kk, out_keys, wq, s);
+    if (a.d == 64) return launch_query_scan_df<MF, IP, MIXED, 64>(a, maxp, probes, kk, out_keys, wq, s);
   }
   if constexpr (!IP && !MIXED && MF == 12) {
-    if (a.d == 96) return launch_query_scan_df<MF, IP, MIXED, 96>(a, maxp, probes, kk, out_keys, s);
+    if (a.d == 96) return launch_query_scan_df<MF, IP, MIXED, 96>(a, maxp, probes, kk, out_keys, wq, s);
   }
   if constexpr (!IP && !MIXED && MF == 8) {
-    if (a.d == 64) return launch_query_scan_df<MF, IP, MIXED, 64>(a, maxp, probes, kk, out_keys, s);
+    if (a.d == 64) return launch_query_scan_df<MF, IP, MIXED, 64>(a, maxp, probes, kk, out_keys, wq, s);
   }
-  launch_query_scan_df<MF, IP, MIXED, 0>(a, maxp, probes, kk, out_keys, s);
+  launch_query_scan_df<MF, IP, MIXED, 0>(a, maxp, probes, kk, out_keys, wq, s);
 }
 
 }  // namespace
@@ -552,10 +696,38 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     }
     const bool mixed = a.precision != 0 && !ip;
     if (mixed && !a.ptab) throw Error(kState, "mixed-precision tables are missing");
+    WorkQueue wq{};
+    const WorkQueue *wqp = nullptr;
+    if (a.wq_entries > 0) {
+      // (query, entry-range) work queue built on the device: no host sync.
+      w.wq_poff.reserve(sizeof(int64_t) * (size_t)a.nq * (maxp + 1));
+      w.wq_nitems.reserve(sizeof(int) * (size_t)a.nq);
+      w.wq_rq.reserve(sizeof(int64_t) * (size_t)a.nq);
+      w.wq_off.reserve(sizeof(int64_t) * (size_t)(a.nq + 1));
+      w.wq_gthr.reserve(sizeof(uint64_t) * (size_t)a.nq);
+      w.wq_counter.reserve(sizeof(unsigned));
+      w.part.reserve(sizeof(uint64_t) * (size_t)a.nq * kMaxItemsPerQuery * kk);
+      hipLaunchKernelGGL(wq_setup_kernel, dim3(ceil_div(a.nq, 256)), dim3(256), 0, s, probes, a.nq,
+                         maxp, a.nprobe_per_query, a.nprobe, a.list_off, (int64_t)a.wq_entries,
+                         w.wq_poff.as<int64_t>(), w.wq_nitems.as<int>(), w.wq_rq.as<int64_t>(),
+                         w.wq_gthr.as<uint64_t>());
+      MVG_LAUNCH_CHECK("wq_setup_kernel");
+      hipLaunchKernelGGL(wq_offsets_kernel, dim3(1), dim3(kNT), 0, s, w.wq_nitems.as<int>(), a.nq,
+                         w.wq_off.as<int64_t>(), w.wq_counter.as<unsigned>());
+      MVG_LAUNCH_CHECK("wq_offsets_kernel");
+      wq.nq = a.nq;
+      wq.item_off = w.wq_off.as<int64_t>();
+      wq.poff = w.wq_poff.as<int64_t>();
+      wq.rq = w.wq_rq.as<int64_t>();
+      wq.counter = w.wq_counter.as<unsigned>();
+      wq.gthr = w.wq_gthr.as<uint64_t>();
+      wq.part = w.part.as<uint64_t>();
+      wqp = &wq;
+    }
 #define MVG_QSCAN(MFV)                                                                   \
-  if (ip) launch_query_scan<MFV, true, false>(a, maxp, probes, kk, keys, s);            \
-  else if (mixed) launch_query_scan<MFV, false, true>(a, maxp, probes, kk, keys, s);    \
-  else launch_query_scan<MFV, false, false>(a, maxp, probes, kk, keys, s);
+  if (ip) launch_query_scan<MFV, true, false>(a, maxp, probes, kk, keys, wqp, s);       \
+  else if (mixed) launch_query_scan<MFV, false, true>(a, maxp, probes, kk, keys, wqp, s); \
+  else launch_query_scan<MFV, false, false>(a, maxp, probes, kk, keys, wqp, s);
     switch (a.M) {
       case 8: MVG_QSCAN(8) break;
       case 12: MVG_QSCAN(12) break;
@@ -563,6 +735,14 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
       default: MVG_QSCAN(0) break;
     }
 #undef MVG_QSCAN
+    if (wqp) {  // segmented merge of each query's items
+      if (kf == 1)
+        launch_merge(wq.part, a.nq, kk, a.k, ip ? 1 : 0, kIvfPadDist, a.D, a.I, nullptr, nullptr, s,
+                     wq.item_off);
+      else
+        launch_merge(wq.part, a.nq, kk, kk, 0, kIvfPadDist, nullptr, nullptr, nullptr, keys, s,
+                     wq.item_off);
+    }
     if (pf.on) MVG_HIP(hipEventRecord(pf.ev[2], s));
     if (kf > 1)
       rerank(a.q, a.nq, a.d, a.metric, a.vectors, keys, kk, a.k, kIvfPadDist, a.D, a.I, s);

@@ -14,10 +14,11 @@ void flat_topk(const float *base, int64_t n, int d, int metric, const float *q, 
 // Canonical argmin (k = 1); kmeans_mode restates the k-means scan rules.
 void flat_argmin(const float *base, int64_t n, int d, int metric, bool kmeans_mode, const float *q,
                  int64_t nq, int32_t *out_id, float *out_dist, hipStream_t s);
-// Reduce ncand candidate keys per query to the sorted top-k.
+// Reduce ncand candidate keys per query to the sorted top-k. With seg_off,
+// query q owns units [seg_off[q], seg_off[q+1]) of ncand keys each.
 void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int negate,
                   float pad_dist, float *D, int64_t *I64, int32_t *I32, uint64_t *keys,
-                  hipStream_t s);
+                  hipStream_t s, const int64_t *seg_off = nullptr);
 
 // ---- kmeans.hip ----
 struct KmeansWork {
@@ -60,6 +61,7 @@ struct IvfSearchArgs {
   int split_scan;      // 1: one workgroup per (query, list) + merge pass
   const uint16_t *coarse_bf16;  // mixed coarse: bf16 centroid rows (padded)
   const float *coarse_norm;     // and their squared norms
+  int wq_entries;  // > 0: persistent work queue, items of about this many entries
 };
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> for the mixed L2 scan.
 void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, int nlist,
@@ -75,6 +77,7 @@ struct IvfProfile {
 };
 struct IvfWork {
   DevBuf coarse_part, probes, part, keys, qbf, qnorm;
+  DevBuf wq_poff, wq_nitems, wq_rq, wq_off, wq_gthr, wq_counter;
   IvfProfile prof;
 };
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);

@@ -236,6 +236,10 @@ class IndexIVFPQ:
         p = {"plain": 0, "mixed": 1}[precision]
         _lib.check(_lib.load().mvg_ivfpq_set_search_mode(self._h, p, int(split_scan)))
 
+    def set_work_queue(self, entries_per_item: int = 0) -> None:
+        """Persistent (query, entry-range) work queue for skewed lists; 0 off."""
+        _lib.check(_lib.load().mvg_ivfpq_set_work_queue(self._h, int(entries_per_item)))
+
     def train(self, x, niter_coarse: int = 15, niter_pq: int = 15, seed: int = KMEANS_SEED) -> None:
         x = _f32(x, self.d)
         _lib.check(_lib.load().mvg_ivfpq_train(self._h, _ptr(x), x.shape[0], niter_coarse, niter_pq, seed))

@@ -211,9 +211,11 @@ def test_ivfpq_add_and_search_match_oracle(gpu, metric, M, split):
         assert_same(Dg, Ig, Do, Io)
 
 
-def test_ivfpq_per_query_nprobe_exclusion_padding(gpu):
+@pytest.mark.parametrize("wq", [0, 300, 4096])
+def test_ivfpq_per_query_nprobe_exclusion_padding(gpu, wq):
     x = clustered(6, 4000, 32, nclusters=30)
     oi, gi = _pair(gpu, x, 32, 64, 8, 0)
+    gi.set_work_queue(wq)
     oi.add(x)
     gi.add(x)
     q = clustered(7, 40, 32, nclusters=30)
@@ -231,20 +233,22 @@ def test_ivfpq_per_query_nprobe_exclusion_padding(gpu):
     assert (Ig[:, -1] == -1).all() and (Dg[:, -1] == np.float32(1e38)).all()
 
 
-def test_ivfpq_long_lists_cross_scan_chunks(gpu):
+@pytest.mark.parametrize("mode", ["query", "split", "wq"])
+def test_ivfpq_long_lists_cross_scan_chunks(gpu, mode):
     # one dominant cluster so a list exceeds the 4096-entry scan chunk
     rng = np.random.default_rng(8)
     x = np.rint(np.clip(rng.normal(100, 3, (20000, 16)), 0, 255)).astype(np.float32)
     oi, gi = _pair(gpu, x, 16, 4, 4, 0)
+    gi.set_search_mode("plain", split_scan=mode == "split")
+    gi.set_work_queue(2048 if mode == "wq" else 0)
     oi.add(x)
     gi.add(x)
     assert gi.list_sizes().max() > 4096
     q = np.rint(np.clip(rng.normal(100, 3, (20, 16)), 0, 255)).astype(np.float32)
-    for k in (10, 500):
-        Dg, Ig = gi.search(q, k, k_factor=1)
-        gi.nprobe = 4
-        Dg, Ig = gi.search(q, k)
-        Do, Io = oi.search(q, k, 4)
+    gi.nprobe = 4
+    for k, kf in ((10, 1), (500, 1), (10, 3)):
+        Dg, Ig = gi.search(q, k, k_factor=kf)
+        Do, Io = oi.search(q, k, 4, k_factor=kf)
         assert_same(Dg, Ig, Do, Io)
 
 
