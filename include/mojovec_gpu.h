@@ -170,6 +170,26 @@ int mvg_ivfpq_list_sizes(mvg_ivfpq *h, int64_t *sizes);
 int mvg_ivfpq_get_list(mvg_ivfpq *h, int list, int64_t *ids, uint8_t *codes);
 void mvg_ivfpq_destroy(mvg_ivfpq *h);
 
+/* ---- continuous-batching request loop (no reference counterpart) ------- */
+/* For serving callers of an HBM-resident IVF-PQ index. A dispatcher thread
+ * packs waiting requests with equal (k, k_factor) into batches of at most
+ * max_batch_queries queries, admits a batch when one of `slots` (1..16)
+ * streams is free and its device-memory estimate fits mem_budget_bytes, and
+ * overlaps the batches on the device. Each request's result equals the same
+ * request searched alone. The index must not be mutated while served. */
+typedef struct mvg_server mvg_server;
+int mvg_server_create(mvg_ivfpq *index, int slots, int max_batch_queries, int64_t mem_budget_bytes,
+                      mvg_server **out);
+/* Copies q [nq x d]; distances/labels [nq x k] must stay valid until
+ * mvg_server_wait(ticket) returns. */
+int mvg_server_submit(mvg_server *s, const float *q, int64_t nq, int k, int nprobe, int k_factor,
+                      float *distances, int64_t *labels, int64_t *ticket);
+/* Blocks until the request is done; latency_ms = submit to completion. */
+int mvg_server_wait(mvg_server *s, int64_t ticket, double *latency_ms);
+int mvg_server_stats(mvg_server *s, int64_t *completed_queries, int64_t *batches,
+                     int64_t *max_inflight_batches);
+void mvg_server_destroy(mvg_server *s);
+
 /* ---- managed collection: CollectionIVFPQ ------------------------------- */
 /* (mojovec/api/collection_ivfpq.mojo:52-333). Messages match the reference.
  * CollectionIVFPQ.__init__ (:68-100); nprobe 0 selects min(10, nlist). */

@@ -17,6 +17,8 @@ from .index import (
     ProductQuantizer,
 )
 
+from .serving import Server
+
 __version__ = "0.1.0"
 
 _load_native()
@@ -33,4 +35,5 @@ __all__ = [
     "METRIC_L2",
     "METRIC_INNER_PRODUCT",
     "MojoVecError",
+    "Server",
 ]

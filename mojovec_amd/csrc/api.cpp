@@ -9,6 +9,7 @@
 #include <unordered_map>
 #include <unordered_set>
 
+#include "handles.h"
 #include "kernels.h"
 
 using mvg::DevBuf;
@@ -136,21 +137,6 @@ struct mvg_sq8 {
   DevBuf f32, codes, norms, sums, tmp, q, D, I, excl;
   int use_mfma = 1;
   mvg::Sq8Work work;
-  hipStream_t s = nullptr;
-};
-
-struct mvg_ivfpq {
-  int d = 0, nlist = 0, M = 0, metric = 0, nprobe = 1;
-  bool trained = false, keep_vectors = true, dirty = true, tables_dirty = true;
-  int precision = 0, split_scan = 0, wq_entries = 0;
-  DevBuf ptab, coarse_bf16, coarse_norm;
-  int64_t ntotal = 0;
-  std::vector<std::vector<int32_t>> lids;
-  std::vector<std::vector<uint8_t>> lcodes;
-  DevBuf coarse, pq, vectors, list_off, codes, ids;
-  DevBuf tx, tlab, tcodes, tres, q, D, I, excl, npq;
-  mvg::IvfWork work;
-  mvg::KmeansWork kw;
   hipStream_t s = nullptr;
 };
 
@@ -625,7 +611,36 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   require(h && nq >= 0 && k >= 1 && k_factor >= 1, "invalid arguments");
   require(h->trained && h->ntotal > 0, "search_device needs a trained, non-empty index");
   hipStream_t s = stream ? (hipStream_t)stream : h->s;
+  std::lock_guard<std::mutex> lk(h->search_mu);
+  mvg::ivf_prepare(h);
+  mvg::ivf_search_with(h, q, nq, k, nprobe_per_query, max_nprobe, k_factor, exclude, exclude_len,
+                       distances, labels, h->work, s);
+  API_END
+}
+
+}  // extern "C"
+
+void mvg::ivf_prepare(mvg_ivfpq *h) {
   ivf_sync_lists(h);
+  if (h->precision != 0 && h->tables_dirty) {
+    if (h->metric == MVG_METRIC_L2) {
+      h->ptab.reserve(sizeof(float) * (size_t)h->nlist * h->M * mvg::kKsub);
+      mvg::precompute_ivf_tables(h->coarse.as<float>(), h->pq.as<float>(), h->d, h->M, h->nlist,
+                                 h->ptab.as<float>(), h->s);
+    }
+    h->coarse_bf16.reserve((size_t)h->nlist * mvg::gemm_row_bytes(mvg::kGemmBf16L2, h->d));
+    h->coarse_norm.reserve(sizeof(float) * h->nlist);
+    mvg::to_bf16(h->coarse.as<float>(), h->nlist, h->d, h->coarse_bf16.as<uint16_t>(),
+                 h->coarse_norm.as<float>(), h->s);
+    MVG_HIP(hipStreamSynchronize(h->s));
+    h->tables_dirty = false;
+  }
+}
+
+void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                          const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                          const uint8_t *exclude, int64_t exclude_len, float *distances,
+                          int64_t *labels, mvg::IvfWork &work, hipStream_t s) {
   mvg::IvfSearchArgs a{};
   a.q = q;
   a.nq = nq;
@@ -652,26 +667,15 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.split_scan = h->split_scan;
   a.wq_entries = h->wq_entries;
   if (h->precision != 0) {
-    if (h->tables_dirty) {
-      if (h->metric == MVG_METRIC_L2) {
-        h->ptab.reserve(sizeof(float) * (size_t)h->nlist * h->M * mvg::kKsub);
-        mvg::precompute_ivf_tables(h->coarse.as<float>(), h->pq.as<float>(), h->d, h->M, h->nlist,
-                                   h->ptab.as<float>(), h->s);
-      }
-      h->coarse_bf16.reserve((size_t)h->nlist * mvg::gemm_row_bytes(mvg::kGemmBf16L2, h->d));
-      h->coarse_norm.reserve(sizeof(float) * h->nlist);
-      mvg::to_bf16(h->coarse.as<float>(), h->nlist, h->d, h->coarse_bf16.as<uint16_t>(),
-                   h->coarse_norm.as<float>(), h->s);
-      MVG_HIP(hipStreamSynchronize(h->s));
-      h->tables_dirty = false;
-    }
+    if (h->tables_dirty) throw Error(mvg::kState, "mixed-precision tables are not prepared");
     if (h->metric == MVG_METRIC_L2) a.ptab = h->ptab.as<float>();
     a.coarse_bf16 = h->coarse_bf16.as<uint16_t>();
     a.coarse_norm = h->coarse_norm.as<float>();
   }
-  mvg::ivfpq_search(a, h->work, s);
-  API_END
+  mvg::ivfpq_search(a, work, s);
 }
+
+extern "C" {
 
 int mvg_ivfpq_search(mvg_ivfpq *h, const float *q, int64_t nq, int k,
                      const int32_t *nprobe_per_query, int k_factor, const uint8_t *exclude,
@@ -686,6 +690,7 @@ int mvg_ivfpq_search(mvg_ivfpq *h, const float *q, int64_t nq, int k,
     }
     return MVG_OK;
   }
+  std::lock_guard<std::mutex> lk(h->search_mu);
   const float *dq = upload(h->q, q, (size_t)nq * h->d, h->s);
   const int32_t *dnp = nullptr;
   int maxp = h->nprobe;
@@ -702,9 +707,9 @@ int mvg_ivfpq_search(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   const uint8_t *dx = exclude && exclude_len > 0 ? upload(h->excl, exclude, (size_t)exclude_len, h->s) : nullptr;
   h->D.reserve(sizeof(float) * nq * k);
   h->I.reserve(sizeof(int64_t) * nq * k);
-  int rc = mvg_ivfpq_search_device(h, dq, nq, k, dnp, maxp, k_factor, dx, dx ? exclude_len : 0,
-                                   h->D.as<float>(), h->I.as<int64_t>(), nullptr);
-  if (rc != MVG_OK) return rc;
+  mvg::ivf_prepare(h);
+  mvg::ivf_search_with(h, dq, nq, k, dnp, maxp, k_factor, dx, dx ? exclude_len : 0, h->D.as<float>(),
+                       h->I.as<int64_t>(), h->work, h->s);
   download(distances, h->D.ptr, (size_t)nq * k, h->s);
   download(labels, h->I.ptr, (size_t)nq * k, h->s);
   MVG_HIP(hipStreamSynchronize(h->s));

@@ -1,0 +1,40 @@
+// Internal layout of the IVF-PQ handle, shared by the C ABI (api.cpp) and the
+// request loop (server.cpp).
+#pragma once
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/mojovec_gpu.h"
+#include "kernels.h"
+
+struct mvg_ivfpq {
+  int d = 0, nlist = 0, M = 0, metric = 0, nprobe = 1;
+  bool trained = false, keep_vectors = true, dirty = true, tables_dirty = true;
+  int precision = 0, split_scan = 0, wq_entries = 0;
+  mvg::DevBuf ptab, coarse_bf16, coarse_norm;
+  int64_t ntotal = 0;
+  std::vector<std::vector<int32_t>> lids;   // inverted lists, append order
+  std::vector<std::vector<uint8_t>> lcodes;
+  mvg::DevBuf coarse, pq, vectors, list_off, codes, ids;
+  mvg::DevBuf tx, tlab, tcodes, tres, q, D, I, excl, npq;
+  mvg::IvfWork work;
+  mvg::KmeansWork kw;
+  hipStream_t s = nullptr;
+  std::mutex search_mu;  // serialises searches that share `work`
+};
+
+namespace mvg {
+
+// Brings the device CSR lists and the mixed-form tables up to date (host
+// synchronising; call before capture or concurrent searches).
+void ivf_prepare(mvg_ivfpq *h);
+
+// Enqueues one search on `s` with the caller's workspace; every pointer is
+// device memory. Throws mvg::Error.
+void ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                     const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                     const uint8_t *exclude, int64_t exclude_len, float *distances,
+                     int64_t *labels, IvfWork &work, hipStream_t s);
+
+}  // namespace mvg

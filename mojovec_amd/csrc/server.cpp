@@ -1,0 +1,355 @@
+// Continuous-batching request loop over an HBM-resident IVF-PQ index.
+//
+// Requests (a block of queries with its own k, nprobe and k_factor) queue on
+// the host. A dispatcher thread packs waiting requests that share (k,
+// k_factor) into a batch (per-query nprobe travels with each query), admits
+// the batch when a slot is free and the batch's device-memory estimate fits
+// the budget, and enqueues it on the slot's stream with the slot's own
+// workspace, so batches overlap on the device. Completed slots are retired as
+// soon as their event fires; a request's result depends only on its own
+// queries (every search kernel works per query), so it is the same alone and
+// in any mix.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "handles.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  int64_t ticket = 0;
+  std::vector<float> q;
+  int64_t nq = 0;
+  int k = 10, nprobe = 1, k_factor = 1;
+  float *D = nullptr;
+  int64_t *I = nullptr;
+  Clock::time_point t_submit, t_done;
+  bool done = false;
+  int status = MVG_OK;
+  std::string error;
+};
+
+template <typename T>
+struct PinnedBuf {
+  T *ptr = nullptr;
+  size_t count = 0;
+  ~PinnedBuf() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+  void reserve(size_t n) {
+    if (n <= count) return;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    MVG_HIP(hipHostMalloc((void **)&ptr, sizeof(T) * n, hipHostMallocDefault));
+    count = n;
+  }
+};
+
+struct Slot {
+  hipStream_t s = nullptr;
+  hipEvent_t done = nullptr;
+  mvg::IvfWork work;
+  mvg::DevBuf q, np, D, I;
+  PinnedBuf<float> hq, hD;
+  PinnedBuf<int32_t> hnp;
+  PinnedBuf<int64_t> hI;
+  std::vector<Request *> reqs;
+  int64_t nq = 0, bytes = 0;
+  int k = 0;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct mvg_server {
+  mvg_ivfpq *ix = nullptr;
+  int device = 0;
+  int max_batch = 4096;
+  int64_t budget = 0, in_use = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request *> pending;
+  std::unordered_map<int64_t, std::unique_ptr<Request>> all;
+  int64_t next_ticket = 1;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::thread th;
+  bool stop = false;
+  int64_t done_queries = 0, batches = 0, max_inflight = 0, inflight = 0;
+  std::string fatal;
+};
+
+namespace {
+
+int64_t batch_bytes(const mvg_ivfpq *ix, int64_t nq, int k, int kf, int maxp) {
+  const int64_t kk = (int64_t)k * kf;
+  // queries, probes, per-query candidates (work queue: up to 8 items), outputs
+  return nq * ((int64_t)ix->d * 6 + (int64_t)maxp * 12 + kk * 8 * 9 + (int64_t)k * 12) + (1 << 20);
+}
+
+void retire(mvg_server *sv, Slot &sl) {
+  // results: pinned staging -> each request's buffers
+  int64_t off = 0;
+  const auto now = Clock::now();
+  std::lock_guard<std::mutex> lk(sv->mu);
+  for (Request *r : sl.reqs) {
+    std::copy(sl.hD.ptr + off * sl.k, sl.hD.ptr + (off + r->nq) * sl.k, r->D);
+    std::copy(sl.hI.ptr + off * sl.k, sl.hI.ptr + (off + r->nq) * sl.k, r->I);
+    off += r->nq;
+    r->t_done = now;
+    r->done = true;
+  }
+  sv->done_queries += sl.nq;
+  sv->in_use -= sl.bytes;
+  sv->inflight--;
+  sl.reqs.clear();
+  sl.busy = false;
+  sv->cv_done.notify_all();
+}
+
+void fail_slot(mvg_server *sv, Slot &sl, const std::string &msg) {
+  std::lock_guard<std::mutex> lk(sv->mu);
+  for (Request *r : sl.reqs) {
+    r->status = MVG_ERR_DEVICE;
+    r->error = msg;
+    r->t_done = Clock::now();
+    r->done = true;
+  }
+  sv->in_use -= sl.bytes;
+  sv->inflight--;
+  sl.reqs.clear();
+  sl.busy = false;
+  sv->cv_done.notify_all();
+}
+
+// Takes a batch from the queue front under the lock; empty if none fits.
+std::vector<Request *> take_batch(mvg_server *sv, int64_t &bytes) {
+  std::vector<Request *> batch;
+  if (sv->pending.empty()) return batch;
+  Request *head = sv->pending.front();
+  const int k = head->k, kf = head->k_factor;
+  int64_t nq = 0;
+  int maxp = 1;
+  for (auto it = sv->pending.begin(); it != sv->pending.end() && batch.size() < 4096;) {
+    Request *r = *it;
+    if (r->k != k || r->k_factor != kf) {
+      ++it;
+      continue;
+    }
+    if (!batch.empty() && nq + r->nq > sv->max_batch) break;
+    const int mp = std::max(maxp, std::min(r->nprobe, sv->ix->nlist));
+    const int64_t b = batch_bytes(sv->ix, nq + r->nq, k, kf, mp);
+    // admission by memory: the first request of an idle server always runs
+    if (sv->in_use + b > sv->budget && !(batch.empty() && sv->inflight == 0)) break;
+    batch.push_back(r);
+    nq += r->nq;
+    maxp = mp;
+    bytes = b;
+    it = sv->pending.erase(it);
+  }
+  return batch;
+}
+
+void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t bytes) {
+  const int d = sv->ix->d;
+  int64_t nq = 0;
+  int maxp = 1;
+  for (Request *r : batch) nq += r->nq;
+  const int k = batch[0]->k, kf = batch[0]->k_factor;
+  sl.hq.reserve((size_t)nq * d);
+  sl.hnp.reserve((size_t)nq);
+  sl.hD.reserve((size_t)nq * k);
+  sl.hI.reserve((size_t)nq * k);
+  int64_t off = 0;
+  for (Request *r : batch) {
+    std::copy(r->q.begin(), r->q.end(), sl.hq.ptr + off * d);
+    const int np = std::min(r->nprobe, sv->ix->nlist);
+    maxp = std::max(maxp, np);
+    std::fill(sl.hnp.ptr + off, sl.hnp.ptr + off + r->nq, np);
+    off += r->nq;
+  }
+  sl.q.reserve(sizeof(float) * nq * d);
+  sl.np.reserve(sizeof(int32_t) * nq);
+  sl.D.reserve(sizeof(float) * nq * k);
+  sl.I.reserve(sizeof(int64_t) * nq * k);
+  MVG_HIP(hipMemcpyAsync(sl.q.ptr, sl.hq.ptr, sizeof(float) * nq * d, hipMemcpyHostToDevice, sl.s));
+  MVG_HIP(hipMemcpyAsync(sl.np.ptr, sl.hnp.ptr, sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.s));
+  mvg::ivf_search_with(sv->ix, sl.q.as<float>(), nq, k, sl.np.as<int32_t>(), maxp, kf, nullptr, 0,
+                       sl.D.as<float>(), sl.I.as<int64_t>(), sl.work, sl.s);
+  MVG_HIP(hipMemcpyAsync(sl.hD.ptr, sl.D.ptr, sizeof(float) * nq * k, hipMemcpyDeviceToHost, sl.s));
+  MVG_HIP(hipMemcpyAsync(sl.hI.ptr, sl.I.ptr, sizeof(int64_t) * nq * k, hipMemcpyDeviceToHost, sl.s));
+  MVG_HIP(hipEventRecord(sl.done, sl.s));
+  sl.reqs = batch;
+  sl.nq = nq;
+  sl.k = k;
+  sl.bytes = bytes;
+}
+
+void dispatcher(mvg_server *sv) {
+  (void)hipSetDevice(sv->device);
+  for (;;) {
+    bool progressed = false;
+    for (auto &up : sv->slots) {  // retire finished batches
+      Slot &sl = *up;
+      if (!sl.busy) continue;
+      const hipError_t e = hipEventQuery(sl.done);
+      if (e == hipSuccess) {
+        retire(sv, sl);
+        progressed = true;
+      } else if (e != hipErrorNotReady) {
+        fail_slot(sv, sl, hipGetErrorString(e));
+        progressed = true;
+      }
+    }
+    for (auto &up : sv->slots) {  // admit waiting requests into free slots
+      Slot &sl = *up;
+      if (sl.busy) continue;
+      std::vector<Request *> batch;
+      int64_t bytes = 0;
+      {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        batch = take_batch(sv, bytes);
+        if (batch.empty()) break;
+        sl.busy = true;
+        sl.reqs = batch;
+        sv->in_use += bytes;
+        sv->inflight++;
+        sv->batches++;
+        sv->max_inflight = std::max(sv->max_inflight, sv->inflight);
+      }
+      sl.bytes = bytes;
+      try {
+        launch(sv, sl, batch, bytes);
+      } catch (const std::exception &ex) {
+        fail_slot(sv, sl, ex.what());
+      }
+      progressed = true;
+    }
+    std::unique_lock<std::mutex> lk(sv->mu);
+    if (sv->stop && sv->pending.empty() && sv->inflight == 0) return;
+    if (!progressed) {
+      if (sv->inflight > 0)
+        sv->cv_work.wait_for(lk, std::chrono::microseconds(20));
+      else
+        sv->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mvg_server_create(mvg_ivfpq *ix, int slots, int max_batch_queries, int64_t mem_budget_bytes,
+                      mvg_server **out) {
+  try {
+    if (!ix || slots < 1 || slots > 16 || max_batch_queries < 1 || mem_budget_bytes <= 0)
+      throw mvg::Error(MVG_ERR_INVALID,
+                       "server needs an index, 1..16 slots, max_batch_queries >= 1 and a positive "
+                       "memory budget");
+    if (!ix->trained || ix->ntotal == 0)
+      throw mvg::Error(MVG_ERR_STATE, "server needs a trained, non-empty index");
+    mvg::ivf_prepare(ix);
+    auto sv = std::make_unique<mvg_server>();
+    sv->ix = ix;
+    MVG_HIP(hipGetDevice(&sv->device));
+    sv->max_batch = max_batch_queries;
+    sv->budget = mem_budget_bytes;
+    for (int i = 0; i < slots; i++) {
+      auto sl = std::make_unique<Slot>();
+      MVG_HIP(hipStreamCreateWithFlags(&sl->s, hipStreamNonBlocking));
+      MVG_HIP(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
+      sv->slots.push_back(std::move(sl));
+    }
+    mvg_server *raw = sv.get();
+    sv->th = std::thread(dispatcher, raw);
+    *out = sv.release();
+    return MVG_OK;
+  } catch (const mvg::Error &e) {
+    mvg::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    mvg::set_last_error(e.what());
+    return MVG_ERR_DEVICE;
+  }
+}
+
+int mvg_server_submit(mvg_server *sv, const float *q, int64_t nq, int k, int nprobe, int k_factor,
+                      float *distances, int64_t *labels, int64_t *ticket) {
+  if (!sv || nq < 1 || k < 1 || nprobe < 1 || k_factor < 1 || !distances || !labels) {
+    mvg::set_last_error("invalid request");
+    return MVG_ERR_INVALID;
+  }
+  if ((int64_t)k * k_factor > mvg::kMaxCandidates) {
+    mvg::set_last_error("k * k_factor exceeds the device limit of 4096");
+    return MVG_ERR_INVALID;
+  }
+  auto r = std::make_unique<Request>();
+  r->q.assign(q, q + nq * sv->ix->d);
+  r->nq = nq;
+  r->k = k;
+  r->nprobe = nprobe;
+  r->k_factor = k_factor;
+  r->D = distances;
+  r->I = labels;
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    r->ticket = sv->next_ticket++;
+    *ticket = r->ticket;
+    sv->pending.push_back(r.get());
+    sv->all[r->ticket] = std::move(r);
+  }
+  sv->cv_work.notify_one();
+  return MVG_OK;
+}
+
+int mvg_server_wait(mvg_server *sv, int64_t ticket, double *latency_ms) {
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->all.find(ticket);
+  if (it == sv->all.end()) {
+    mvg::set_last_error("unknown ticket");
+    return MVG_ERR_INVALID;
+  }
+  Request *r = it->second.get();
+  sv->cv_done.wait(lk, [&] { return r->done; });
+  if (latency_ms)
+    *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+  const int st = r->status;
+  if (st != MVG_OK) mvg::set_last_error(r->error);
+  sv->all.erase(it);
+  return st;
+}
+
+int mvg_server_stats(mvg_server *sv, int64_t *completed_queries, int64_t *batches,
+                     int64_t *max_inflight) {
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (completed_queries) *completed_queries = sv->done_queries;
+  if (batches) *batches = sv->batches;
+  if (max_inflight) *max_inflight = sv->max_inflight;
+  return MVG_OK;
+}
+
+void mvg_server_destroy(mvg_server *sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->stop = true;
+  }
+  sv->cv_work.notify_all();
+  if (sv->th.joinable()) sv->th.join();
+  for (auto &sl : sv->slots) {
+    (void)hipStreamSynchronize(sl->s);
+    (void)hipEventDestroy(sl->done);
+    (void)hipStreamDestroy(sl->s);
+  }
+  delete sv;
+}
+
+}  // extern "C"

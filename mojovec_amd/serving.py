@@ -1,0 +1,55 @@
+"""Continuous-batching server over an HBM-resident IVF-PQ index
+(mvg_server_* in include/mojovec_gpu.h).
+
+    srv = Server(index, slots=4, max_batch=8192, mem_budget=8 << 30)
+    t = srv.submit(queries, k=10, nprobe=32, k_factor=4)
+    D, I, latency_ms = srv.wait(t)
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class Server:
+    def __init__(self, index, slots: int = 4, max_batch: int = 8192, mem_budget: int = 8 << 30):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().mvg_server_create(index._h, slots, max_batch, mem_budget, ctypes.byref(h)))
+        self._h = h
+        self._index = index  # keeps the index alive
+        self._out = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().mvg_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, queries, k: int = 10, nprobe: int = 8, k_factor: int = 1) -> int:
+        q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, self._index.d)
+        nq = q.shape[0]
+        D = np.empty((nq, k), np.float32)
+        I = np.empty((nq, k), np.int64)
+        t = ctypes.c_int64()
+        _lib.check(_lib.load().mvg_server_submit(self._h, ctypes.c_void_p(q.ctypes.data), nq, k, nprobe,
+                                                 k_factor, ctypes.c_void_p(D.ctypes.data),
+                                                 ctypes.c_void_p(I.ctypes.data), ctypes.byref(t)))
+        self._out[t.value] = (D, I)
+        return t.value
+
+    def wait(self, ticket: int):
+        lat = ctypes.c_double()
+        rc = _lib.load().mvg_server_wait(self._h, ticket, ctypes.byref(lat))
+        D, I = self._out.pop(ticket)
+        _lib.check(rc)
+        return D, I, lat.value
+
+    def stats(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.load().mvg_server_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"completed_queries": a.value, "batches": b.value, "max_inflight_batches": c.value}
