@@ -85,6 +85,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--fixed", default=None,
+                    help="skip the sweep: 'plain:NPROBE:KF,mixed:NPROBE:KF' (profiling runs)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--workload", default="ivfpq", choices=["ivfpq", "flat", "sq8"],
                     help="ivfpq: the headline 1M x 128 IVF-PQ line; flat / sq8: the Flat "
                          "100k x 128 and SQ8 1M x 768 side measurements")
@@ -134,10 +137,15 @@ def main(argv=None):
         ix.search_device(q.data_ptr(), NQ, K, Dd.data_ptr(), Id.data_ptr(), k_factor=kf, stream=stream)
 
     sweeps, best = {}, {}
+    fixed = {}
+    if args.fixed:
+        for part in args.fixed.split(","):
+            f, npb, kfx = part.split(":")
+            fixed[f] = (int(npb), int(kfx))
     for form in ("plain", "mixed"):
         grid = []
-        for nprobe in (8, 16, 32, 64):
-            for kf in (1, 2, 4, 8, 16):
+        for nprobe in ((fixed[form][0],) if form in fixed else (8, 16, 32, 64)):
+            for kf in ((fixed[form][1],) if form in fixed else (1, 2, 4, 8, 16)):
                 search(form, nprobe, kf)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
@@ -194,8 +202,8 @@ def main(argv=None):
     plain_qps = forms["plain"]["qps"]
     ratio = forms["mixed"]["qps"] / plain_qps if forms["mixed"]["recall"] >= RECALL_BAR else None
 
-    cpu = cpu_baseline(ix, x_h, q.cpu().numpy(), best["plain"]["nprobe"], best["plain"]["k_factor"],
-                       args.cpu_seconds)
+    cpu = None if args.no_cpu else cpu_baseline(
+        ix, x_h, q.cpu().numpy(), best["plain"]["nprobe"], best["plain"]["k_factor"], args.cpu_seconds)
 
     line = {
         "metric": "QPS@recall10>=0.95, 1M x 128 IVF-PQ: mixed vs FP32 GPU; %HBM roofline of the list scan",
