@@ -170,6 +170,20 @@ int mvg_ivfpq_list_sizes(mvg_ivfpq *h, int64_t *sizes);
 int mvg_ivfpq_get_list(mvg_ivfpq *h, int list, int64_t *ids, uint8_t *codes);
 void mvg_ivfpq_destroy(mvg_ivfpq *h);
 
+/* ---- captured search batch (no reference counterpart) ------------------ */
+/* Records one whole device-pointer search (coarse, work-queue build, scan,
+ * merge, rerank) of a fixed batch shape into a HIP graph with its own
+ * workspace; mvg_graph_launch replays it on `stream` (a hipStream_t; NULL =
+ * the null stream) against the same query/output buffers, whose contents may
+ * change between replays. The index must not be mutated while a graph of it
+ * exists. */
+typedef struct mvg_graph mvg_graph;
+int mvg_ivfpq_capture(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                      const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                      float *distances, int64_t *labels, mvg_graph **out);
+int mvg_graph_launch(mvg_graph *g, void *stream);
+void mvg_graph_destroy(mvg_graph *g);
+
 /* ---- continuous-batching request loop (no reference counterpart) ------- */
 /* For serving callers of an HBM-resident IVF-PQ index. A dispatcher thread
  * packs waiting requests with equal (k, k_factor) into batches of at most

@@ -787,3 +787,63 @@ void mvg_ivfpq_destroy(mvg_ivfpq *h) {
 }
 
 }  // extern "C"
+
+// ---- captured batch ---------------------------------------------------------
+
+struct mvg_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  mvg::IvfWork work;  // the workspace the graph's kernels point into
+};
+
+extern "C" {
+
+int mvg_ivfpq_capture(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                      const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                      float *distances, int64_t *labels, mvg_graph **out) {
+  API_BEGIN
+  require(h && nq >= 1 && k >= 1 && k_factor >= 1 && out, "invalid arguments");
+  require(h->trained && h->ntotal > 0, "capture needs a trained, non-empty index");
+  std::lock_guard<std::mutex> lk(h->search_mu);
+  mvg::ivf_prepare(h);
+  auto g = std::make_unique<mvg_graph>();
+  hipStream_t s = new_stream();
+  struct Guard { hipStream_t s; ~Guard() { (void)hipStreamDestroy(s); } } guard{s};
+  const bool prof = h->work.prof.on;
+  g->work.prof.on = false;
+  // One eager run sizes every buffer, so the capture allocates nothing.
+  mvg::ivf_search_with(h, q, nq, k, nprobe_per_query, max_nprobe, k_factor, nullptr, 0, distances,
+                       labels, g->work, s);
+  MVG_HIP(hipStreamSynchronize(s));
+  MVG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  try {
+    mvg::ivf_search_with(h, q, nq, k, nprobe_per_query, max_nprobe, k_factor, nullptr, 0, distances,
+                         labels, g->work, s);
+  } catch (...) {
+    hipGraph_t dummy = nullptr;
+    (void)hipStreamEndCapture(s, &dummy);
+    if (dummy) (void)hipGraphDestroy(dummy);
+    throw;
+  }
+  MVG_HIP(hipStreamEndCapture(s, &g->graph));
+  MVG_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  (void)prof;
+  *out = g.release();
+  API_END
+}
+
+int mvg_graph_launch(mvg_graph *g, void *stream) {
+  API_BEGIN
+  require(g && g->exec, "invalid graph");
+  MVG_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  API_END
+}
+
+void mvg_graph_destroy(mvg_graph *g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
+
+}  // extern "C"

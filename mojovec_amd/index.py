@@ -282,6 +282,15 @@ class IndexIVFPQ:
                                                  _ptr(m), ml, _ptr(D), _ptr(I)))
         return D, I
 
+    def capture(self, q_ptr: int, nq: int, k: int, D_ptr: int, I_ptr: int, k_factor: int = 1,
+                nprobe_ptr: int = 0, max_nprobe: int = 0) -> "SearchGraph":
+        """Record one device-pointer search of this batch shape as a HIP graph."""
+        g = ctypes.c_void_p()
+        _lib.check(_lib.load().mvg_ivfpq_capture(
+            self._h, ctypes.c_void_p(q_ptr), nq, k, ctypes.c_void_p(nprobe_ptr or None), max_nprobe,
+            k_factor, ctypes.c_void_p(D_ptr), ctypes.c_void_p(I_ptr), ctypes.byref(g)))
+        return SearchGraph(g, self)
+
     def search_device(self, q_ptr: int, nq: int, k: int, D_ptr: int, I_ptr: int, k_factor: int = 1,
                       nprobe_ptr: int = 0, max_nprobe: int = 0, stream: int = 0) -> None:
         """Search with every buffer already in device memory (raw pointers)."""
@@ -289,3 +298,18 @@ class IndexIVFPQ:
             self._h, ctypes.c_void_p(q_ptr), nq, k, ctypes.c_void_p(nprobe_ptr or None), max_nprobe,
             k_factor, None, 0, ctypes.c_void_p(D_ptr), ctypes.c_void_p(I_ptr),
             ctypes.c_void_p(stream or None)))
+
+
+class SearchGraph:
+    """A captured IVF-PQ search batch (mvg_ivfpq_capture); replay with launch()."""
+
+    def __init__(self, handle, index):
+        self._h, self._index = handle, index
+
+    def launch(self, stream: int = 0) -> None:
+        _lib.check(_lib.load().mvg_graph_launch(self._h, ctypes.c_void_p(stream or None)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.load().mvg_graph_destroy(self._h)
+            self._h = None
