@@ -88,7 +88,7 @@ def main(argv=None):
     ap.add_argument("--fixed", default=None,
                     help="skip the sweep: 'plain:NPROBE:KF,mixed:NPROBE:KF' (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="ivfpq", choices=["ivfpq", "flat", "sq8"],
+    ap.add_argument("--workload", default="ivfpq", choices=["ivfpq", "flat", "sq8", "batch"],
                     help="ivfpq: the headline 1M x 128 IVF-PQ line; flat / sq8: the Flat "
                          "100k x 128 and SQ8 1M x 768 side measurements")
     args = ap.parse_args(argv)
@@ -100,6 +100,8 @@ def main(argv=None):
         return bench_flat(args)
     if args.workload == "sq8":
         return bench_sq8(args)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     import torch
 
@@ -399,6 +401,100 @@ def bench_sq8(args):
         del ix
     print(json.dumps({"workload": "SQ8 Flat 1,000,000 x 768 unit-norm, 10,000 queries, k=10 "
                                   "(host buffers, PCIe included)", "forms": out}), flush=True)
+    return 0
+
+
+def bench_batch(args):
+    """Skewed batch: IVF-PQ on 10M x 96 integer-valued mixture (20,000
+    clusters, Zipf 1.2), nlist 16,384 (k-means 25 iterations on 655,360
+    rows), PQ 12 x 8; 50,000 queries, each with nprobe ~ U{4..128} and
+    k ~ {1, 10, 100}. Queries are grouped by k; each group's schedule is
+    autotuned, then the group is captured in a HIP graph and replayed."""
+    import torch
+
+    import mojovec_amd as mv
+    from mojovec_amd import _lib
+
+    torch.backends.cuda.matmul.allow_tf32 = False
+    dev = torch.device("cuda:0")
+    n, d, nlist, m, nq = 10_000_000, 96, 16_384, 12, 50_000
+    g = torch.Generator(device=dev)
+    g.manual_seed(31)
+    centres = torch.rand((20_000, d), generator=g, device=dev) * 128.0
+    w = torch.arange(1, 20_001, device=dev, dtype=torch.float64) ** -1.2
+    w = (w / w.sum()).float()
+
+    def draw(cnt):
+        out = torch.empty((cnt, d), dtype=torch.float32, device=dev)
+        for s in range(0, cnt, 1_000_000):
+            e = min(cnt, s + 1_000_000)
+            lab = torch.multinomial(w, e - s, replacement=True, generator=g)
+            out[s:e] = (centres[lab] + torch.randn((e - s, d), generator=g, device=dev) * 12.0).clamp_(0, 255).round_()
+        return out
+
+    x = draw(n)
+    q = draw(nq)
+    sample = x[torch.randperm(n, generator=g, device=dev)[:40 * nlist]].contiguous()
+    log("batch data ready")
+    ix = mv.IndexIVFPQ(d, nlist, m)
+    t0 = time.perf_counter()
+    ix.train(sample.cpu().numpy(), niter_coarse=25, niter_pq=15)
+    t_train = time.perf_counter() - t0
+    log(f"trained in {t_train:.1f}s")
+    t0 = time.perf_counter()
+    for s in range(0, n, 2_000_000):
+        ix.add(x[s:s + 2_000_000].cpu().numpy())
+    t_add = time.perf_counter() - t0
+    log(f"added in {t_add:.1f}s")
+    sizes = ix.list_sizes()
+    rng = np.random.default_rng(32)
+    nprobe = rng.integers(4, 129, nq).astype(np.int32)
+    kq = rng.choice(np.array([1, 10, 100]), nq)
+    L = _lib.load()
+    total_s, groups = 0.0, {}
+    for k in (1, 10, 100):
+        sel = np.nonzero(kq == k)[0]
+        qg = q[torch.from_numpy(sel).to(dev)].contiguous()
+        npg = torch.from_numpy(nprobe[sel]).to(dev)
+        ng = len(sel)
+        D = torch.empty((ng, k), dtype=torch.float32, device=dev)
+        I = torch.empty((ng, k), dtype=torch.int64, device=dev)
+        sched, ms = ix.autotune(qg.data_ptr(), ng, k, D.data_ptr(), I.data_ptr(), k_factor=1,
+                                nprobe_ptr=npg.data_ptr(), max_nprobe=128)
+        graph = ix.capture(qg.data_ptr(), ng, k, D.data_ptr(), I.data_ptr(), k_factor=1,
+                           nprobe_ptr=npg.data_ptr(), max_nprobe=128)
+        dt = _timed(torch, graph.launch, max(2, args.steps // 4), 1)
+        total_s += dt
+        # recall@min(k,10) on 500 queries of the group against exact search
+        sub = qg[:500]
+        best = []
+        for s0 in range(0, sub.shape[0], 100):
+            qq = sub[s0:s0 + 100]
+            acc = None
+            for xs in range(0, n, 2_000_000):
+                xx = x[xs:xs + 2_000_000]
+                d2 = (qq * qq).sum(1)[:, None] + (xx * xx).sum(1)[None, :] - 2.0 * (qq @ xx.T)
+                v, i = torch.topk(d2, k, dim=1, largest=False)
+                i = i + xs
+                if acc is None:
+                    acc = (v, i)
+                else:
+                    vv = torch.cat([acc[0], v], 1)
+                    ii = torch.cat([acc[1], i], 1)
+                    o = torch.argsort(vv, 1)[:, :k]
+                    acc = (torch.gather(vv, 1, o), torch.gather(ii, 1, o))
+            best.append(acc[1])
+        gt = torch.cat(best)
+        rec = recall_at(torch, gt, I[:500])
+        groups[str(k)] = {"queries": ng, "ms": round(dt * 1e3, 3), "qps": round(ng / dt, 1),
+                          "schedule": sched, "recall_at_k": round(rec, 4)}
+        log(f"batch k={k}: {groups[str(k)]}")
+    print(json.dumps({"workload": "IVF-PQ 10M x 96, nlist 16384, PQ 12x8, 50,000 queries, nprobe "
+                                  "U{4..128}, k in {1,10,100} (graph replay per k group)",
+                      "qps": round(nq / total_s, 1), "total_ms": round(total_s * 1e3, 3),
+                      "train_s": round(t_train, 1), "add_s": round(t_add, 1),
+                      "list_len_max": int(sizes.max()), "list_len_mean": float(sizes.mean()),
+                      "groups": groups}), flush=True)
     return 0
 
 

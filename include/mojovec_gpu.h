@@ -182,6 +182,13 @@ int mvg_ivfpq_capture(mvg_ivfpq *h, const float *q, int64_t nq, int k,
                       const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
                       float *distances, int64_t *labels, mvg_graph **out);
 int mvg_graph_launch(mvg_graph *g, void *stream);
+/* Times the scan schedules (one workgroup per query; work queue with 1024,
+ * 4096 or 16384 entries per item; one workgroup per (query, list)) on this
+ * batch shape (device pointers) and keeps the fastest on the handle.
+ * choice: 0..4 in that order; best_ms: its time per search. */
+int mvg_ivfpq_autotune(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                       const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                       float *distances, int64_t *labels, int *choice, float *best_ms);
 void mvg_graph_destroy(mvg_graph *g);
 
 /* ---- continuous-batching request loop (no reference counterpart) ------- */

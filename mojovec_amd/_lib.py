@@ -75,6 +75,8 @@ SIGNATURES = {
     "mvg_ivfpq_destroy": (None, [_p]),
     "mvg_ivfpq_capture": (_i, [_p, _p, _i64, _i, _p, _i, _i, _p, _p, ctypes.POINTER(_p)]),
     "mvg_graph_launch": (_i, [_p, _p]),
+    "mvg_ivfpq_autotune": (_i, [_p, _p, _i64, _i, _p, _i, _i, _p, _p, ctypes.POINTER(_i),
+                                ctypes.POINTER(_f)]),
     "mvg_graph_destroy": (None, [_p]),
     "mvg_server_create": (_i, [_p, _i, _i, _i64, ctypes.POINTER(_p)]),
     "mvg_server_submit": (_i, [_p, _p, _i64, _i, _i, _i, _p, _p, ctypes.POINTER(_i64)]),

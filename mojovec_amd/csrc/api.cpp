@@ -832,6 +832,61 @@ int mvg_ivfpq_capture(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   API_END
 }
 
+int mvg_ivfpq_autotune(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                       const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                       float *distances, int64_t *labels, int *choice, float *best_ms) {
+  API_BEGIN
+  require(h && nq >= 1 && k >= 1 && k_factor >= 1, "invalid arguments");
+  require(h->trained && h->ntotal > 0, "autotune needs a trained, non-empty index");
+  std::lock_guard<std::mutex> lk(h->search_mu);
+  mvg::ivf_prepare(h);
+  // Candidate scan schedules: {split_scan, entries per work-queue item}.
+  const int cands[][2] = {{0, 0}, {0, 1024}, {0, 4096}, {0, 16384}, {1, 0}};
+  const int ncand = sizeof(cands) / sizeof(cands[0]);
+  hipStream_t s = new_stream();
+  struct Guard { hipStream_t s; ~Guard() { (void)hipStreamDestroy(s); } } guard{s};
+  hipEvent_t e0, e1;
+  MVG_HIP(hipEventCreate(&e0));
+  MVG_HIP(hipEventCreate(&e1));
+  struct EvGuard { hipEvent_t a, b; ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); } } eg{e0, e1};
+  const int keep_split = h->split_scan, keep_wq = h->wq_entries;
+  int best = -1;
+  float best_t = 0.0f;
+  for (int c = 0; c < ncand; c++) {
+    h->split_scan = cands[c][0];
+    h->wq_entries = cands[c][1];
+    try {
+      mvg::ivf_search_with(h, q, nq, k, nprobe_per_query, max_nprobe, k_factor, nullptr, 0,
+                           distances, labels, h->work, s);  // warm-up sizes the workspace
+      MVG_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < 3; r++)
+        mvg::ivf_search_with(h, q, nq, k, nprobe_per_query, max_nprobe, k_factor, nullptr, 0,
+                             distances, labels, h->work, s);
+      MVG_HIP(hipEventRecord(e1, s));
+      MVG_HIP(hipEventSynchronize(e1));
+    } catch (const Error &) {
+      continue;  // a schedule this shape cannot run (workspace limits)
+    }
+    float ms = 0.0f;
+    MVG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    ms /= 3.0f;
+    if (best < 0 || ms < best_t) {
+      best = c;
+      best_t = ms;
+    }
+  }
+  if (best < 0) {
+    h->split_scan = keep_split;
+    h->wq_entries = keep_wq;
+    throw Error(mvg::kState, "no scan schedule ran for this batch shape");
+  }
+  h->split_scan = cands[best][0];
+  h->wq_entries = cands[best][1];
+  if (choice) *choice = best;
+  if (best_ms) *best_ms = best_t;
+  API_END
+}
+
 int mvg_graph_launch(mvg_graph *g, void *stream) {
   API_BEGIN
   require(g && g->exec, "invalid graph");

@@ -282,6 +282,18 @@ class IndexIVFPQ:
                                                  _ptr(m), ml, _ptr(D), _ptr(I)))
         return D, I
 
+    SCHEDULES = ("per_query", "queue_1024", "queue_4096", "queue_16384", "per_list")
+
+    def autotune(self, q_ptr: int, nq: int, k: int, D_ptr: int, I_ptr: int, k_factor: int = 1,
+                 nprobe_ptr: int = 0, max_nprobe: int = 0):
+        """Time the scan schedules on this batch (device pointers), keep the
+        fastest; returns (schedule name, ms per search)."""
+        c, ms = ctypes.c_int(), ctypes.c_float()
+        _lib.check(_lib.load().mvg_ivfpq_autotune(
+            self._h, ctypes.c_void_p(q_ptr), nq, k, ctypes.c_void_p(nprobe_ptr or None), max_nprobe,
+            k_factor, ctypes.c_void_p(D_ptr), ctypes.c_void_p(I_ptr), ctypes.byref(c), ctypes.byref(ms)))
+        return self.SCHEDULES[c.value], ms.value
+
     def capture(self, q_ptr: int, nq: int, k: int, D_ptr: int, I_ptr: int, k_factor: int = 1,
                 nprobe_ptr: int = 0, max_nprobe: int = 0) -> "SearchGraph":
         """Record one device-pointer search of this batch shape as a HIP graph."""
