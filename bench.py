@@ -93,8 +93,9 @@ def main(argv=None):
                          "100k x 128 and SQ8 1M x 768 side measurements")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(every box of this build has a single MI355X)", file=sys.stderr)
+        print(f"bench.py measures a single MI355X; --gpus {args.gpus} is not supported: "
+              "the multi-GPU path (sharded lists, all-gather of per-GPU top-k) is not "
+              "built", file=sys.stderr)
         return 2
     if args.workload == "flat":
         return bench_flat(args)

@@ -164,6 +164,9 @@ int mvg_ivfpq_set_profiling(mvg_ivfpq *h, int on);
 int mvg_ivfpq_read_profile(mvg_ivfpq *h, float *stage_ms, int64_t *scanned_entries);
 /* State: ntotal, trained flag, nprobe. */
 int mvg_ivfpq_info(mvg_ivfpq *h, int64_t *ntotal, int *trained, int *nprobe);
+/* Construction parameters: d, nlist, M, metric (IndexIVFPQ fields d, nlist,
+ * M, metric_type, index_ivf_pq.mojo:19-33). */
+int mvg_ivfpq_shape(mvg_ivfpq *h, int *d, int *nlist, int *M, int *metric);
 /* Inverted-list sizes [nlist] and one list's ids/codes in append order
  * (ArrayInvertedLists.list_size/get_ids/get_codes, inverted_lists.mojo). */
 int mvg_ivfpq_list_sizes(mvg_ivfpq *h, int64_t *sizes);
@@ -237,6 +240,25 @@ int mvg_coll_info(mvg_collection *h, int *dimension, int *metric, int64_t *count
 int mvg_coll_name(mvg_collection *h, char *buf, int buf_len);
 mvg_ivfpq *mvg_coll_index(mvg_collection *h);
 void mvg_coll_destroy(mvg_collection *h);
+
+/* ---- snapshots in the reference's file format -------------------------- */
+/* CollectionIVFPQ.save (collection_ivfpq.mojo:335-370): collection header,
+ * user ids, the IndexIVFPQ stream and a checksum trailer, written to a
+ * temporary file beside `path` and renamed over it. */
+int mvg_coll_save(mvg_collection *h, const char *path);
+/* CollectionIVFPQ.load (collection_ivfpq.mojo:372-476): validates the
+ * trailer and every header with the reference's messages, then builds the
+ * HBM index. Snapshots carry no FP32 rows: a loaded collection answers
+ * k_factor 1 queries only. */
+int mvg_coll_load(const char *path, mvg_collection **out);
+/* write_index_ivf_pq / read_index_ivf_pq (mojovec/io/serialization.mojo:
+ * 1029-1090) as a whole file: the bare index stream, no trailer. Internal
+ * ids must fit in int32 on the device. */
+int mvg_ivfpq_save(mvg_ivfpq *h, const char *path);
+int mvg_ivfpq_load(const char *path, mvg_ivfpq **out);
+/* _checksum_prefix (mojovec/io/snapshot_file.mojo:19-59) of a payload held
+ * in host memory: 8 MiB chunks from offset 0. Host only; no device needed. */
+int mvg_snapshot_checksum(const uint8_t *data, int64_t len, uint64_t *out);
 
 #ifdef __cplusplus
 }

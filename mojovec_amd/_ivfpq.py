@@ -8,6 +8,7 @@ Same constructor, methods, defaults, result dictionaries and error messages;
 from __future__ import annotations
 
 import ctypes
+import os
 from collections.abc import Sequence
 from numbers import Real
 from typing import Any, TypedDict
@@ -95,6 +96,21 @@ class IVFPQCollection:
                                      metric.encode(), ctypes.byref(h)))
         self._h = h
         self._k_factor = int(k_factor)
+
+    @classmethod
+    def load(cls, path) -> "IVFPQCollection":
+        """Load an owned IVF-PQ snapshot (collection_ivfpq.mojo:372-476)."""
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().mvg_coll_load(os.fspath(path).encode(), ctypes.byref(h)))
+        collection = cls.__new__(cls)
+        collection._h = h
+        collection._k_factor = 1
+        return collection
+
+    def save(self, path) -> None:
+        """Atomically persist a checksummed owned IVF-PQ snapshot in the
+        reference's format (collection_ivfpq.mojo:335-370)."""
+        _lib.check(_lib.load().mvg_coll_save(self._h, os.fspath(path).encode()))
 
     def __del__(self):
         if getattr(self, "_h", None):

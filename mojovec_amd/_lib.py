@@ -70,6 +70,8 @@ SIGNATURES = {
     "mvg_ivfpq_set_profiling": (_i, [_p, _i]),
     "mvg_ivfpq_read_profile": (_i, [_p, _p, ctypes.POINTER(_i64)]),
     "mvg_ivfpq_info": (_i, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "mvg_ivfpq_shape": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                             ctypes.POINTER(_i)]),
     "mvg_ivfpq_list_sizes": (_i, [_p, _p]),
     "mvg_ivfpq_get_list": (_i, [_p, _i, _p, _p]),
     "mvg_ivfpq_destroy": (None, [_p]),
@@ -95,6 +97,11 @@ SIGNATURES = {
     "mvg_coll_name": (_i, [_p, ctypes.c_char_p, _i]),
     "mvg_coll_index": (_p, [_p]),
     "mvg_coll_destroy": (None, [_p]),
+    "mvg_coll_save": (_i, [_p, ctypes.c_char_p]),
+    "mvg_coll_load": (_i, [ctypes.c_char_p, ctypes.POINTER(_p)]),
+    "mvg_ivfpq_save": (_i, [_p, ctypes.c_char_p]),
+    "mvg_ivfpq_load": (_i, [ctypes.c_char_p, ctypes.POINTER(_p)]),
+    "mvg_snapshot_checksum": (_i, [_p, _i64, ctypes.POINTER(_u64)]),
 }
 
 _lib = None
@@ -106,7 +113,7 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(
-                f"{LIB_PATH} is missing: build it with `python -m mojovec_amd.build`"
+                f"{LIB_PATH} is missing: build it with `python mojovec_amd/build.py`"
             )
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
@@ -120,6 +127,4 @@ def load():
 def check(rc: int) -> None:
     if rc != MVG_OK:
         msg = load().mvg_last_error().decode("utf-8", "replace")
-        if rc == MVG_ERR_INVALID:
-            raise MojoVecError(rc, msg)
         raise MojoVecError(rc, msg)

@@ -1,6 +1,6 @@
 """Builds libmojovec_gpu.so (HIP, gfx950) in-tree.
 
-    python -m mojovec_amd.build            # or: python mojovec_amd/build.py
+    python mojovec_amd/build.py
 
 Every source under csrc/ is compiled with hipcc for gfx950 only, with FP
 contraction off (the plain kernels reproduce the reference's FP32 operation

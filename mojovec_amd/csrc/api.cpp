@@ -254,7 +254,7 @@ int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_
       h->keys.reserve(sizeof(uint64_t) * nq * kk);
       mvg::launch_merge(h->part.as<uint64_t>(), nq, (int64_t)nsplit * kk, kk, 0, 0.0f, nullptr,
                         nullptr, nullptr, h->keys.as<uint64_t>(), h->s);
-      mvg::rerank(dq, nq, h->d, h->metric, h->base.as<float>(), h->keys.as<uint64_t>(), kk, k, 0.0f,
+      mvg::rerank(dq, nq, h->d, h->metric, h->base.as<float>(), h->ntotal, h->keys.as<uint64_t>(), kk, k, 0.0f,
                   h->D.as<float>(), h->I.as<int64_t>(), h->s);
     }
   }
@@ -666,6 +666,7 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.precision = h->precision;
   a.split_scan = h->split_scan;
   a.wq_entries = h->wq_entries;
+  a.ntotal = h->ntotal;
   if (h->precision != 0) {
     if (h->tables_dirty) throw Error(mvg::kState, "mixed-precision tables are not prepared");
     if (h->metric == MVG_METRIC_L2) a.ptab = h->ptab.as<float>();
@@ -761,6 +762,16 @@ int mvg_ivfpq_info(mvg_ivfpq *h, int64_t *ntotal, int *trained, int *nprobe) {
   if (ntotal) *ntotal = h->ntotal;
   if (trained) *trained = h->trained ? 1 : 0;
   if (nprobe) *nprobe = h->nprobe;
+  API_END
+}
+
+int mvg_ivfpq_shape(mvg_ivfpq *h, int *d, int *nlist, int *M, int *metric) {
+  API_BEGIN
+  require(h, "invalid handle");
+  if (d) *d = h->d;
+  if (nlist) *nlist = h->nlist;
+  if (M) *M = h->M;
+  if (metric) *metric = h->metric;
   API_END
 }
 

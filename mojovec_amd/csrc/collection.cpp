@@ -9,26 +9,17 @@
 #include <vector>
 
 #include "../../include/mojovec_gpu.h"
+#include "collection.h"
 #include "common.h"
 
 namespace {
 
-constexpr int kMetricCosine = 2;  // collection_codec.mojo:21
-constexpr int kMaxNlist = 1000000;
-constexpr int kMaxNprobe = 65536;
+using mvg::check_index;
+using mvg::kMetricCosine;
 constexpr int kMaxQueryResults = 2048;
 constexpr int kPqCentroids = 256;
 
-struct Fail {
-  int code;
-  std::string msg;
-};
-
-[[noreturn]] void raise(const char *msg, int code = MVG_ERR_INVALID) { throw Fail{code, msg}; }
-
-void check_index(int rc) {
-  if (rc != MVG_OK) throw Fail{rc, mvg_last_error()};
-}
+[[noreturn]] void raise(const char *msg) { mvg::coll_raise(msg); }
 
 // _validate_finite_vectors (collection_vector_query.mojo:17-21).
 void validate_finite(const float *x, int64_t count) {
@@ -57,30 +48,24 @@ std::vector<float> normalize(const float *x, int64_t count, int d) {
 
 }  // namespace
 
-struct mvg_collection {
-  std::string name;
-  int d = 0, metric = 0, nlist = 0, M = 0;
-  mvg_ivfpq *ix = nullptr;
-  std::vector<int64_t> user_ids;
-  std::unordered_map<int64_t, int64_t> id_to_internal;
-  int niter_coarse = 15, niter_pq = 15;  // index_ivf_pq.mojo:98, pq.mojo:64
-  uint64_t seed = 0x4D4F4A4F564543ull;
-  std::vector<float> D;
-  std::vector<int64_t> I;
-};
+void mvg::validate_ivfpq_parameters(int64_t d, int64_t nlist, int64_t M, int64_t nprobe) {
+  if (d <= 0 || d > 65536) raise("dimension must be between 1 and 65536.");
+  if (nlist <= 0 || nlist > kCollMaxNlist) raise("nlist must be between 1 and 1000000.");
+  if (M <= 0 || M > d || d % M != 0) raise("M must divide dimension and be between 1 and dimension.");
+  if (nprobe <= 0 || nprobe > nlist || nprobe > kCollMaxNprobe)
+    raise("nprobe must be between 1 and nlist.");
+}
 
-#define COLL_BEGIN try {
-#define COLL_END                                              \
-  }                                                           \
-  catch (const Fail &f) {                                     \
-    mvg::set_last_error(f.msg);                               \
-    return f.code;                                            \
-  }                                                           \
-  catch (const std::bad_alloc &) {                            \
-    mvg::set_last_error("host allocation failed");            \
-    return MVG_ERR_ALLOC;                                     \
-  }                                                           \
-  return MVG_OK;
+int mvg::parse_metric(const std::string &m) {
+  if (m == "l2") return MVG_METRIC_L2;
+  if (m == "cosine") return kMetricCosine;
+  if (m == "ip") return MVG_METRIC_IP;
+  raise("metric must be \"l2\", \"cosine\", or \"ip\".");
+}
+
+const char *mvg::metric_name(int metric) {
+  return metric == MVG_METRIC_L2 ? "l2" : metric == kMetricCosine ? "cosine" : "ip";
+}
 
 namespace {
 
@@ -110,24 +95,15 @@ int mvg_coll_create(int d, int nlist, int M, int nprobe, const char *name, const
                     mvg_collection **out) {
   COLL_BEGIN
   const int eff_nprobe = nprobe == 0 ? std::min(10, nlist) : nprobe;  // :77
-  // _validate_ivfpq_parameters (:37-49), validation.mojo:11-14
-  if (d <= 0 || d > 65536) raise("dimension must be between 1 and 65536.");
-  if (nlist <= 0 || nlist > kMaxNlist) raise("nlist must be between 1 and 1000000.");
-  if (M <= 0 || M > d || d % M != 0) raise("M must divide dimension and be between 1 and dimension.");
-  if (eff_nprobe <= 0 || eff_nprobe > nlist || eff_nprobe > kMaxNprobe)
-    raise("nprobe must be between 1 and nlist.");
-  const std::string m = metric ? metric : "l2";
-  int mt;
-  if (m == "l2") mt = MVG_METRIC_L2;  // _parse_metric (collection_codec.mojo:24-31)
-  else if (m == "cosine") mt = kMetricCosine;
-  else if (m == "ip") mt = MVG_METRIC_IP;
-  else raise("metric must be \"l2\", \"cosine\", or \"ip\".");
+  mvg::validate_ivfpq_parameters(d, nlist, M, eff_nprobe);
+  const int mt = mvg::parse_metric(metric ? metric : "l2");
   auto h = std::make_unique<mvg_collection>();
   h->name = name ? name : "";
   h->d = d;
   h->metric = mt;
   h->nlist = nlist;
   h->M = M;
+  h->list_capacity.assign((size_t)nlist, 0);
   check_index(mvg_ivfpq_create(d, nlist, M, mt == MVG_METRIC_L2 ? MVG_METRIC_L2 : MVG_METRIC_IP, &h->ix));
   check_index(mvg_ivfpq_set_nprobe(h->ix, eff_nprobe));
   *out = h.release();
@@ -246,8 +222,6 @@ int mvg_coll_name(mvg_collection *h, char *buf, int buf_len) {
 mvg_ivfpq *mvg_coll_index(mvg_collection *h) { return h ? h->ix : nullptr; }
 
 void mvg_coll_destroy(mvg_collection *h) {
-  if (!h) return;
-  mvg_ivfpq_destroy(h->ix);
   delete h;
 }
 

@@ -133,6 +133,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_scan_kernel(
 template <bool IP>
 __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q, int d,
                                                      const float *__restrict__ vectors,
+                                                     int64_t nrows,
                                                      const uint64_t *__restrict__ cand, int kk,
                                                      int k, int buf_n, float pad_dist,
                                                      float *__restrict__ D,
@@ -150,8 +151,8 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
   const uint64_t *cq = cand + qi * kk;
   for (int c = wave; c < kk; c += kNW) {
     const uint64_t key = cq[c];
-    if (key == kSentinel64) continue;
     const uint32_t id = (uint32_t)(key & 0xFFFFFFFFu);
+    if (key == kSentinel64 || (int64_t)id >= nrows) continue;
     const float ex = wave_dist_w64<IP>(qs, vectors + (int64_t)id * d, d, lane);
     if (lane == 0) buf[c] = pack_key(IP ? -ex : ex, id);
   }
@@ -628,7 +629,7 @@ void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, i
   MVG_LAUNCH_CHECK("precompute_tables_kernel");
 }
 
-void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
+void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors, int64_t nrows,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
             hipStream_t s) {
   if (nq == 0) return;
@@ -642,10 +643,10 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
     MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   if (ip)
     hipLaunchKernelGGL(rerank_kernel<true>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d, vectors,
-                       cand, kk, k, buf_n, pad_dist, D, I);
+                       nrows, cand, kk, k, buf_n, pad_dist, D, I);
   else
     hipLaunchKernelGGL(rerank_kernel<false>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d, vectors,
-                       cand, kk, k, buf_n, pad_dist, D, I);
+                       nrows, cand, kk, k, buf_n, pad_dist, D, I);
   MVG_LAUNCH_CHECK("rerank_kernel");
 }
 
@@ -745,7 +746,7 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     }
     if (pf.on) MVG_HIP(hipEventRecord(pf.ev[2], s));
     if (kf > 1)
-      rerank(a.q, a.nq, a.d, a.metric, a.vectors, keys, kk, a.k, kIvfPadDist, a.D, a.I, s);
+      rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, keys, kk, a.k, kIvfPadDist, a.D, a.I, s);
   } else {
   // 2. per-(query, probe) LUT + ADC scan + list-local top-kk.
   w.part.reserve(sizeof(uint64_t) * (size_t)a.nq * maxp * kk);
@@ -769,7 +770,7 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
     launch_merge(part, a.nq, (int64_t)maxp * kk, kk, 0, kIvfPadDist, nullptr, nullptr, nullptr,
                  w.keys.as<uint64_t>(), s);
-    rerank(a.q, a.nq, a.d, a.metric, a.vectors, w.keys.as<uint64_t>(), kk, a.k, kIvfPadDist, a.D,
+    rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, w.keys.as<uint64_t>(), kk, a.k, kIvfPadDist, a.D,
            a.I, s);
   }
   }

@@ -62,6 +62,7 @@ struct IvfSearchArgs {
   const uint16_t *coarse_bf16;  // mixed coarse: bf16 centroid rows (padded)
   const float *coarse_norm;     // and their squared norms
   int wq_entries;  // > 0: persistent work queue, items of about this many entries
+  int64_t ntotal;  // rows of `vectors`
 };
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> for the mixed L2 scan.
 void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, int nlist,
@@ -84,7 +85,8 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
 
 // Exact FP32 rerank of kk sorted candidate keys per query to the top-k
 // (index_hnsw.mojo:449-496 generalised to k_factor).
-void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
+// Candidate ids at or above nrows (not rows of `vectors`) are dropped.
+void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors, int64_t nrows,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
             hipStream_t s);
 

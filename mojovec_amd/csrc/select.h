@@ -207,8 +207,8 @@ __device__ void bitonic_sort_u64(uint64_t *keys, int n) {
 }
 
 // Writes the min(k, valid) smallest unique u64 keys of in[0..n) to out[] in
-// ascending order; out must hold next_pow2(k) entries and must not overlap
-// in. Returns the count.
+// ascending order and sentinels to the rest of out[0..k); out must hold
+// next_pow2(k) entries and must not overlap in. Returns the count.
 template <typename Scratch>
 __device__ int select_sort_u64(const uint64_t *in, int n, int k, Scratch &s,
                                uint64_t *out) {
@@ -228,7 +228,9 @@ __device__ int select_sort_u64(const uint64_t *in, int n, int k, Scratch &s,
   __syncthreads();
   int np2 = 1;
   while (np2 < cnt) np2 <<= 1;
-  for (int i = cnt + tid; i < np2; i += kNT) out[i] = kSentinel64;
+  // Callers read out[0..k): every slot past the count holds a sentinel.
+  const int fill_to = max(np2, k);
+  for (int i = cnt + tid; i < fill_to; i += kNT) out[i] = kSentinel64;
   __syncthreads();
   bitonic_sort_u64(out, np2);
   return cnt;

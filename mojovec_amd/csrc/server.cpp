@@ -227,6 +227,8 @@ void dispatcher(mvg_server *sv) {
       try {
         launch(sv, sl, batch, bytes);
       } catch (const std::exception &ex) {
+        // work enqueued before the failure still uses the slot's buffers
+        (void)hipStreamSynchronize(sl.s);
         fail_slot(sv, sl, ex.what());
       }
       progressed = true;

@@ -225,7 +225,7 @@ void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s) {
       w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
       launch_merge(w.part.as<uint64_t>(), a.nq, (int64_t)nsplit * kk, kk, 0, 0.0f, nullptr, nullptr,
                    nullptr, w.keys.as<uint64_t>(), s);
-      rerank(a.q, a.nq, a.d, a.metric, a.vectors, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s);
+      rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.n, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s);
     }
     return;
   }
@@ -257,7 +257,7 @@ void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s) {
     w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
     launch_merge(part, a.nq, (int64_t)nchunks * kk, kk, 0, 0.0f, nullptr, nullptr, nullptr,
                  w.keys.as<uint64_t>(), s);
-    rerank(a.q, a.nq, a.d, a.metric, a.vectors, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s);
+    rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.n, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s);
   }
 }
 

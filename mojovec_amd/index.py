@@ -8,6 +8,7 @@ back as (distances float32 [nq, k], labels int64 [nq, k]).
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -258,6 +259,25 @@ class IndexIVFPQ:
     def add(self, x) -> None:
         x = _f32(x, self.d)
         _lib.check(_lib.load().mvg_ivfpq_add(self._h, _ptr(x), x.shape[0]))
+
+    def save(self, path) -> None:
+        """write_index_ivf_pq (serialization.mojo:1029-1044) into a file."""
+        _lib.check(_lib.load().mvg_ivfpq_save(self._h, os.fspath(path).encode()))
+
+    @classmethod
+    def load(cls, path) -> "IndexIVFPQ":
+        """read_index_ivf_pq (serialization.mojo:1046-1090) from a file; the
+        lists and codebooks go to HBM. Snapshots hold no FP32 rows, so a loaded
+        index searches with k_factor 1."""
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.mvg_ivfpq_load(os.fspath(path).encode(), ctypes.byref(h)))
+        vals = [ctypes.c_int() for _ in range(4)]
+        _lib.check(L.mvg_ivfpq_shape(h, *[ctypes.byref(v) for v in vals]))
+        ix = cls.__new__(cls)
+        ix._h, ix._owned = h, True
+        ix.d, ix.nlist, ix.M, ix.metric_type = (v.value for v in vals)
+        return ix
 
     def list_sizes(self):
         s = np.empty(self.nlist, np.int64)
