@@ -115,6 +115,14 @@ def load():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python mojovec_amd/build.py`"
             )
+        # One HIP runtime per process. PyTorch ships its own libamdhip64; when
+        # it is loaded first the library binds to it (same soname), whereas
+        # loading the library first starts a second runtime next to PyTorch's,
+        # and whichever initialises second finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
