@@ -155,6 +155,12 @@ int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan);
  * query), merged per query afterwards; 0 (default) scans one query per
  * workgroup. Results are identical. */
 int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item);
+/* Deep candidate pools (no reference counterpart): from k * k_factor >=
+ * min_candidates (0 = default 513) the scan stores every visited entry's
+ * distance key in HBM and selects the top k * k_factor once per query by
+ * radix select, instead of keeping a running top set. Results are
+ * identical. */
+int mvg_ivfpq_set_deep_select(mvg_ivfpq *h, int min_candidates);
 /* Stage timing of the next searches (no reference counterpart; used by the
  * benchmark): when on, each search records HIP events on its stream around
  * the coarse, scan and merge/rerank stages and counts the list entries the

@@ -2,6 +2,7 @@
 #include "../../include/mojovec_gpu.h"
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <memory>
 #include <new>
@@ -153,6 +154,11 @@ void ivf_sync_lists(mvg_ivfpq *h) {
     std::copy(h->lids[l].begin(), h->lids[l].end(), ids.begin() + off[l]);
     std::copy(h->lcodes[l].begin(), h->lcodes[l].end(), codes.begin() + off[l] * h->M);
   }
+  std::vector<int64_t> sizes(h->nlist);
+  for (int l = 0; l < h->nlist; l++) sizes[l] = off[l + 1] - off[l];
+  std::sort(sizes.begin(), sizes.end(), std::greater<int64_t>());
+  h->top_sum.assign(h->nlist + 1, 0);
+  for (int l = 0; l < h->nlist; l++) h->top_sum[l + 1] = h->top_sum[l] + sizes[l];
   upload(h->list_off, off.data(), off.size(), h->s);
   // +16 bytes so 16-byte code loads of the last entry stay inside the buffer.
   h->codes.reserve(codes.size() + 16);
@@ -667,6 +673,8 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.split_scan = h->split_scan;
   a.wq_entries = h->wq_entries;
   a.ntotal = h->ntotal;
+  a.deep_cap = h->top_sum.empty() ? 0 : h->top_sum[(size_t)a.nprobe];
+  a.deep_min = h->deep_min;
   if (h->precision != 0) {
     if (h->tables_dirty) throw Error(mvg::kState, "mixed-precision tables are not prepared");
     if (h->metric == MVG_METRIC_L2) a.ptab = h->ptab.as<float>();
@@ -729,6 +737,13 @@ int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item) {
   API_BEGIN
   require(h && entries_per_item >= 0, "entries_per_item must be >= 0");
   h->wq_entries = entries_per_item;
+  API_END
+}
+
+int mvg_ivfpq_set_deep_select(mvg_ivfpq *h, int min_candidates) {
+  API_BEGIN
+  require(h && min_candidates >= 0, "invalid arguments");
+  h->deep_min = min_candidates;
   API_END
 }
 

@@ -309,9 +309,97 @@ struct QueryScanLayout {
   }
 };
 
-// DF > 0 (with MF > 0): the plain L2 LUT keeps PQ centroid `tid` of every
+// ---- LUT construction shared by the per-query scan kernels --------------
+//
+// kRegCb shapes (plain L2, MF and DF > 0) keep PQ centroid `tid` of every
 // subspace in registers (DF floats per thread), so each per-list LUT rebuild
 // reads only the LDS residual.
+template <int MF, bool IP, bool MIXED, int DF>
+struct LutShape {
+  static constexpr bool kRegCb = !IP && !MIXED && DF > 0 && MF > 0;
+  static constexpr int kDsub = (DF > 0 && MF > 0) ? DF / MF : 1;
+  static constexpr int kRegs = kRegCb ? DF : 1;
+};
+
+template <int MF, bool IP, bool MIXED, int DF>
+__device__ __forceinline__ void load_reg_codebook(float *cbr, const float *__restrict__ pq) {
+  using S = LutShape<MF, IP, MIXED, DF>;
+  if constexpr (S::kRegCb) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < MF; m++) {
+#pragma unroll
+      for (int j = 0; j < S::kDsub; j++)
+        cbr[m * S::kDsub + j] = pq[((int64_t)m * kKsub + tid) * S::kDsub + j];
+    }
+  }
+}
+
+// Per-query tables: the IP LUT (pq.mojo:285-292) or the mixed -2<q_m, c_mk>.
+// Ends with a barrier.
+template <bool IP, bool MIXED>
+__device__ __forceinline__ void build_query_tables(const float *__restrict__ qq, int d, int M,
+                                                   const float *__restrict__ pq, float *rv,
+                                                   float *lut, float *qtab) {
+  const int tid = threadIdx.x, nlut = M * kKsub, dsub = d / M;
+  if (IP || MIXED) {
+    for (int j = tid; j < d; j += kNT) rv[j] = qq[j];
+    __syncthreads();
+    for (int e = tid; e < nlut; e += kNT) {
+      const int m = e >> 8;
+      const float *c = pq + (int64_t)e * dsub;
+      if (IP) {
+        lut[e] = -ip_pq(rv + m * dsub, c, dsub);
+      } else {
+        float dot = 0.0f;
+        for (int j = 0; j < dsub; j++) dot = fma_(rv[m * dsub + j], c[j], dot);
+        qtab[e] = -2.0f * dot;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// LUT of probed list l in LDS; returns the list's additive base (IP: -<q,
+// c_l>; mixed L2: |q - c_l|^2; plain L2: 0). The caller must have finished
+// every read of the previous list's LUT and base. Ends with a barrier.
+template <int MF, bool IP, bool MIXED, int DF>
+__device__ __forceinline__ float build_list_lut(const float *__restrict__ qq,
+                                                const float *__restrict__ coarse, int l, int d,
+                                                int M, const float *__restrict__ pq,
+                                                const float *__restrict__ ptab, float *rv,
+                                                float *lut, const float *qtab, const float *cbr,
+                                                float *base_slot) {
+  using S = LutShape<MF, IP, MIXED, DF>;
+  const int tid = threadIdx.x, nlut = M * kKsub, dsub = d / M;
+  const float *cl = coarse + (int64_t)l * d;
+  if (!IP) {
+    if (!MIXED) {
+      for (int j = tid; j < d; j += kNT) rv[j] = qq[j] - cl[j];  // index_ivf_pq.mojo:306-310
+      __syncthreads();
+      if constexpr (S::kRegCb) {
+#pragma unroll
+        for (int m = 0; m < MF; m++)
+          lut[m * kKsub + tid] = l2_w8_fixed<S::kDsub>(cbr + m * S::kDsub, rv + m * S::kDsub);
+      } else {
+        for (int e = tid; e < nlut; e += kNT) {
+          const int m = e >> 8;
+          lut[e] = l2_kmeans(rv + m * dsub, pq + (int64_t)e * dsub, dsub);
+        }
+      }
+    } else {
+      const float *tl = ptab + (int64_t)l * nlut;
+      for (int e = tid; e < nlut; e += kNT) lut[e] = tl[e] + qtab[e];
+      if (tid == 0) *base_slot = l2_w8(qq, cl, d);
+    }
+  } else if (tid == 0) {
+    *base_slot = -ip_wn<8>(qq, cl, d);  // index_ivf_pq.mojo:322-324
+  }
+  __syncthreads();
+  return (IP || MIXED) ? *base_slot : 0.0f;
+}
+
+// DF > 0 (with MF > 0): register-resident codebook, see LutShape.
 //
 // WQ (work queue): the workgroups are persistent and pull items from a
 // device counter. An item is a range of at most R consecutive entries of one
@@ -340,7 +428,6 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
     int64_t excl_len, uint64_t *__restrict__ out_keys, float *__restrict__ D,
     int64_t *__restrict__ I, float pad_dist, WorkQueue wq) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int nlut = M * kKsub;
   const QueryScanLayout lay(M, d, kk, MIXED && !IP);
   float *lut = reinterpret_cast<float *>(smem + lay.lut);
   float *qtab = reinterpret_cast<float *>(smem + lay.qtab);
@@ -351,17 +438,8 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
   QueryScanShared *st = reinterpret_cast<QueryScanShared *>(smem + lay.st);
 
   const int tid = threadIdx.x;
-  const int dsub = d / M;
-  constexpr bool kRegCb = !IP && !MIXED && DF > 0 && MF > 0;
-  constexpr int kDsub = (DF > 0 && MF > 0) ? DF / MF : 1;
-  float cbr[kRegCb ? DF : 1];
-  if constexpr (kRegCb) {
-#pragma unroll
-    for (int m = 0; m < MF; m++) {
-#pragma unroll
-      for (int j = 0; j < kDsub; j++) cbr[m * kDsub + j] = pq[((int64_t)m * kKsub + tid) * kDsub + j];
-    }
-  }
+  float cbr[LutShape<MF, IP, MIXED, DF>::kRegs];
+  load_reg_codebook<MF, IP, MIXED, DF>(cbr, pq);
   for (;;) {
   // ---- pick the work: a whole query, or one queue item
   int64_t qi, item = 0, begin = 0, end = INT64_MAX;
@@ -393,23 +471,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
     st->cnt = 0;
     st->thr = WQ ? wq.gthr[qi] : kSentinel64;
   }
-  // Per-query tables: IP LUT (pq.mojo:285-292) or the mixed -2<q_m, c_mk>.
-  if (IP || MIXED) {
-    for (int j = tid; j < d; j += kNT) rv[j] = qq[j];
-    __syncthreads();
-    for (int e = tid; e < nlut; e += kNT) {
-      const int m = e >> 8;
-      const float *c = pq + (int64_t)e * dsub;
-      if (IP) {
-        lut[e] = -ip_pq(rv + m * dsub, c, dsub);
-      } else {
-        float dot = 0.0f;
-        for (int j = 0; j < dsub; j++) dot = fma_(rv[m * dsub + j], c[j], dot);
-        qtab[e] = -2.0f * dot;
-      }
-    }
-  }
-  __syncthreads();
+  build_query_tables<IP, MIXED>(qq, d, M, pq, rv, lut, qtab);
 
   int p = 0;
   if constexpr (WQ) {  // first probe whose entries reach `begin`
@@ -427,31 +489,8 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
       L = min<int64_t>(L, end - poff[p]);
     }
     if (L <= seg_lo) continue;
-    const float *cl = coarse + (int64_t)l * d;
-    if (!IP) {
-      if (!MIXED) {
-        for (int j = tid; j < d; j += kNT) rv[j] = qq[j] - cl[j];  // :306-310
-        __syncthreads();
-        if constexpr (kRegCb) {
-#pragma unroll
-          for (int m = 0; m < MF; m++)
-            lut[m * kKsub + tid] = l2_w8_fixed<kDsub>(cbr + m * kDsub, rv + m * kDsub);
-        } else {
-          for (int e = tid; e < nlut; e += kNT) {
-            const int m = e >> 8;
-            lut[e] = l2_kmeans(rv + m * dsub, pq + (int64_t)e * dsub, dsub);
-          }
-        }
-      } else {
-        const float *tl = ptab + (int64_t)l * nlut;
-        for (int e = tid; e < nlut; e += kNT) lut[e] = tl[e] + qtab[e];
-        if (tid == 0) st->base = l2_w8(qq, cl, d);
-      }
-    } else if (tid == 0) {
-      st->base = -ip_wn<8>(qq, cl, d);  // :322-324
-    }
-    __syncthreads();
-    const float base = (IP || MIXED) ? st->base : 0.0f;
+    const float base = build_list_lut<MF, IP, MIXED, DF>(qq, coarse, l, d, M, pq, ptab, rv, lut,
+                                                         qtab, cbr, &st->base);
     for (int64_t r0 = seg_lo; r0 < L; r0 += kRound) {
       const uint64_t thr = st->thr;
 #pragma unroll
@@ -504,6 +543,264 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
   break;
   }  // for (;;)
 }
+
+// ---------------------------------------------------------------------------
+// Deep candidate pools (k * k_factor in the thousands): a running top-kk
+// would re-select and re-sort thousands of keys on every buffer flush, so
+// the selection is done once per query instead.
+//   1. scan: every entry's u32 distance key is written, in probe order, to
+//      the workgroup's private key segment in HBM, and the top 11-bit digit
+//      of each key is histogrammed in LDS;
+//   2. two more digit passes over the segment find the kk-th smallest key T
+//      and how many keys equal to T it takes (MSD radix select);
+//   3. if T is shared by more keys than are taken, three passes over the
+//      ids of those keys find the largest id taken (lower ids win ties);
+//   4. one walk over the lists emits (key, id) of the winners, which are
+//      sorted in LDS and written out.
+// The workgroups are persistent: each owns one segment of seg_cap keys,
+// where seg_cap is the sum of the max_nprobe largest lists (the most any
+// query can visit), and takes queries from a device counter.
+constexpr int kDeepMinCandidates = 513;  // default: the deep form for k * k_factor >= this
+
+struct DeepShared {
+  int sh[16];  // block_excl_scan / digit search scratch
+  int cnt, valid, need_eq, ties;
+  int64_t q;
+  float base;
+  uint32_t T, Tid;
+};
+
+struct DeepLayout {
+  int lut, qtab, rv, out, hist, st, total;
+  __host__ __device__ DeepLayout(int M, int d, int kk, bool has_qtab) {
+    const int nlut = M * kKsub * 4;
+    lut = 0;
+    qtab = lut + nlut;
+    rv = qtab + (has_qtab ? nlut : 0);
+    const int tables_end = rv + 4 * (d + 4);
+    out = 0;  // the winners reuse the table space once the scan is done
+    const int out_end = 8 * next_pow2(kk);
+    hist = ((tables_end > out_end ? tables_end : out_end) + 15) & ~15;
+    st = hist + 4 * 2048;
+    total = st + (int)sizeof(DeepShared);
+  }
+};
+
+// Given a digit histogram hist[0..nb) (nb a multiple of kNT), finds the bin
+// holding the remaining-th key: sets bin, the count before it and its size.
+__device__ void deep_find_bin(const int *hist, int nb, int remaining, DeepShared &ds, int &bin,
+                              int &before, int &count) {
+  const int tid = threadIdx.x;
+  const int per = nb / kNT;
+  int local = 0;
+  for (int b = 0; b < per; b++) local += hist[tid * per + b];
+  int total;
+  const int excl = block_excl_scan(local, ds.sh, total);
+  if (excl < remaining && remaining <= excl + local) {
+    int c = excl;
+    for (int b = 0; b < per; b++) {
+      const int h = hist[tid * per + b];
+      if (c + h >= remaining) {
+        ds.sh[8] = tid * per + b;
+        ds.sh[9] = c;
+        ds.sh[10] = h;
+        break;
+      }
+      c += h;
+    }
+  }
+  __syncthreads();
+  bin = ds.sh[8];
+  before = ds.sh[9];
+  count = ds.sh[10];
+  __syncthreads();
+}
+
+// One MSD digit pass over the segment: histogram of the digit at `shift`
+// (nbits wide) of the valid keys whose bits under `mask` equal `prefix`.
+__device__ void deep_key_pass(const uint32_t *__restrict__ seg, int64_t n, int shift, int nbits,
+                              uint32_t prefix, uint32_t mask, int *hist) {
+  const int tid = threadIdx.x, nb = 1 << nbits;
+  const uint32_t dm = (uint32_t)nb - 1u;
+  for (int b = tid; b < nb; b += kNT) hist[b] = 0;
+  __syncthreads();
+  for (int64_t i = tid; i < n; i += kNT) {
+    const uint32_t key = seg[i];
+    if (key != kSentinel && (key & mask) == prefix) atomicAdd(&hist[(key >> shift) & dm], 1);
+  }
+  __syncthreads();
+}
+
+template <int MF, bool IP, bool MIXED, int DF>
+__global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
+    const float *__restrict__ q, int64_t nq, int d, int M, const float *__restrict__ coarse,
+    const float *__restrict__ pq, const float *__restrict__ ptab,
+    const int64_t *__restrict__ list_off, const uint8_t *__restrict__ codes,
+    const int32_t *__restrict__ ids, const int32_t *__restrict__ probes, int maxp,
+    const int32_t *__restrict__ nprobe_q, int nprobe_all, int kk, int k,
+    const uint8_t *__restrict__ excl, int64_t excl_len, uint64_t *__restrict__ out_keys,
+    float *__restrict__ D, int64_t *__restrict__ I, float pad_dist, uint32_t *__restrict__ seg_base,
+    int64_t seg_cap, unsigned *__restrict__ counter) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const DeepLayout lay(M, d, kk, MIXED && !IP);
+  float *lut = reinterpret_cast<float *>(smem + lay.lut);
+  float *qtab = reinterpret_cast<float *>(smem + lay.qtab);
+  float *rv = reinterpret_cast<float *>(smem + lay.rv);
+  uint64_t *out = reinterpret_cast<uint64_t *>(smem + lay.out);
+  int *hist = reinterpret_cast<int *>(smem + lay.hist);
+  DeepShared &ds = *reinterpret_cast<DeepShared *>(smem + lay.st);
+  uint32_t *seg = seg_base + (int64_t)blockIdx.x * seg_cap;
+  const int tid = threadIdx.x;
+  const int np2 = next_pow2(kk);
+  float cbr[LutShape<MF, IP, MIXED, DF>::kRegs];
+  load_reg_codebook<MF, IP, MIXED, DF>(cbr, pq);
+
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) ds.q = (int64_t)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int64_t qi = ds.q;
+    if (qi >= nq) break;
+    const float *qq = q + qi * d;
+    int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
+    np = min(max(np, 1), maxp);
+    for (int b = tid; b < 2048; b += kNT) hist[b] = 0;
+    build_query_tables<IP, MIXED>(qq, d, M, pq, rv, lut, qtab);
+
+    // 1. scan: keys to the segment, top digit histogram
+    int64_t n = 0;
+    for (int p = 0; p < np; p++) {
+      const int l = probes[qi * maxp + p];
+      if (l < 0) continue;
+      const int64_t off = list_off[l];
+      const int64_t L = min<int64_t>(list_off[l + 1] - off, seg_cap - n);
+      if (L <= 0) continue;
+      const float base = build_list_lut<MF, IP, MIXED, DF>(qq, coarse, l, d, M, pq, ptab, rv, lut,
+                                                           qtab, cbr, &ds.base);
+      for (int64_t pos = tid; pos < L; pos += kNT) {
+        bool skip = false;
+        if (excl) {  // :328-334
+          const int32_t id = ids[off + pos];
+          skip = id >= 0 && id < excl_len && excl[id] != 0;
+        }
+        uint32_t key = kSentinel;
+        if (!skip) {
+          key = f2key(adc_sum<MF>(codes + (off + pos) * M, lut, M, base));
+          atomicAdd(&hist[key >> 21], 1);
+        }
+        seg[n + pos] = key;
+      }
+      n += L;
+      __syncthreads();  // LUT and base are rebuilt for the next list
+    }
+
+    // 2. kk-th smallest key T and the count of T-keys it takes
+    {
+      int local = 0;
+      for (int b = tid * 8; b < tid * 8 + 8; b++) local += hist[b];
+      const int valid = block_sum(local, ds.sh);
+      if (tid == 0) ds.valid = valid;
+    }
+    __syncthreads();
+    const bool take_all = ds.valid <= kk;
+    uint32_t T = 0u, Tid = 0xFFFFFFFFu;
+    if (!take_all) {
+      int bin, before, count, remaining = kk;
+      deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
+      remaining -= before;
+      uint32_t prefix = (uint32_t)bin << 21, mask = 0xFFE00000u;
+      deep_key_pass(seg, n, 10, 11, prefix, mask, hist);
+      deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
+      remaining -= before;
+      prefix |= (uint32_t)bin << 10;
+      mask |= 0x001FFC00u;
+      deep_key_pass(seg, n, 0, 10, prefix, mask, hist);
+      deep_find_bin(hist, 1024, remaining, ds, bin, before, count);
+      remaining -= before;
+      T = prefix | (uint32_t)bin;
+      // 3. ties at T: the `remaining` lowest ids among `count` keys equal to T
+      if (remaining < count) {
+        uint32_t iprefix = 0u, imask = 0u;
+        const int shifts[3] = {21, 10, 0};
+        const int nbits[3] = {11, 11, 10};
+        for (int pass = 0; pass < 3; pass++) {
+          const int nb = 1 << nbits[pass];
+          const uint32_t dm = (uint32_t)nb - 1u;
+          for (int b = tid; b < nb; b += kNT) hist[b] = 0;
+          __syncthreads();
+          int64_t base_i = 0;
+          for (int p = 0; p < np; p++) {
+            const int l = probes[qi * maxp + p];
+            if (l < 0) continue;
+            const int64_t off = list_off[l];
+            const int64_t L = min<int64_t>(list_off[l + 1] - off, seg_cap - base_i);
+            if (L <= 0) continue;
+            for (int64_t pos = tid; pos < L; pos += kNT) {
+              if (seg[base_i + pos] != T) continue;
+              const uint32_t id = (uint32_t)ids[off + pos];
+              if ((id & imask) == iprefix) atomicAdd(&hist[(id >> shifts[pass]) & dm], 1);
+            }
+            base_i += L;
+          }
+          __syncthreads();
+          int ibin, ibefore, icount;
+          deep_find_bin(hist, nb, remaining, ds, ibin, ibefore, icount);
+          remaining -= ibefore;
+          iprefix |= (uint32_t)ibin << shifts[pass];
+          imask |= dm << shifts[pass];
+        }
+        Tid = iprefix;
+      }
+    }
+
+    // 4. emit the winners, sort, write out
+    if (tid == 0) ds.cnt = 0;
+    __syncthreads();
+    {
+      int64_t base_i = 0;
+      for (int p = 0; p < np; p++) {
+        const int l = probes[qi * maxp + p];
+        if (l < 0) continue;
+        const int64_t off = list_off[l];
+        const int64_t L = min<int64_t>(list_off[l + 1] - off, seg_cap - base_i);
+        if (L <= 0) continue;
+        for (int64_t pos = tid; pos < L; pos += kNT) {
+          const uint32_t key = seg[base_i + pos];
+          if (key == kSentinel || (!take_all && key > T)) continue;
+          const uint32_t id = (uint32_t)ids[off + pos];
+          if (take_all || key < T || id <= Tid) {
+            const int slot = atomicAdd(&ds.cnt, 1);
+            if (slot < kk) out[slot] = ((uint64_t)key << 32) | id;
+          }
+        }
+        base_i += L;
+      }
+    }
+    __syncthreads();
+    const int cnt = min(ds.cnt, kk);
+    for (int i = cnt + tid; i < np2; i += kNT) out[i] = kSentinel64;
+    __syncthreads();
+    bitonic_sort_u64(out, np2);
+    if (out_keys) {
+      for (int t = tid; t < kk; t += kNT) out_keys[qi * kk + t] = out[t];
+    } else {
+      for (int t = tid; t < k; t += kNT) {
+        const uint64_t kv = out[t];
+        float dv = pad_dist;
+        int64_t iv = -1;
+        if (kv != kSentinel64) {
+          dv = key2f((uint32_t)(kv >> 32));
+          if (IP) dv = -dv;  // :357-360
+          iv = (int64_t)(uint32_t)(kv & 0xFFFFFFFFu);
+        }
+        D[qi * k + t] = dv;
+        I[qi * k + t] = iv;
+      }
+    }
+  }
+}
+
+__global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
 
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> (mixed L2 tables).
 __global__ void precompute_tables_kernel(const float *__restrict__ coarse, const float *__restrict__ pq,
@@ -619,6 +916,49 @@ void launch_query_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, 
   launch_query_scan_df<MF, IP, MIXED, 0>(a, maxp, probes, kk, out_keys, wq, s);
 }
 
+template <int MF, bool IP, bool MIXED, int DF>
+void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                    uint64_t *out_keys, IvfWork &w, hipStream_t s) {
+  const DeepLayout lay(a.M, a.d, kk, MIXED && !IP);
+  const size_t sm = (size_t)lay.total;
+  if (sm > 150 * 1024) throw Error(kInvalid, "IVF-PQ deep scan workspace exceeds LDS");
+  const void *fn = (const void *)ivfpq_deep_scan_kernel<MF, IP, MIXED, DF>;
+  if (sm > 64 * 1024)
+    MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  int per_cu = 0;
+  MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
+  const int64_t cap = std::max<int64_t>(a.deep_cap, 1);
+  int64_t grid = std::min<int64_t>((int64_t)std::max(1, per_cu) * device_cus(), a.nq);
+  // at most 4 GiB of key segments: fewer persistent workgroups beyond that
+  grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(4) << 30) / (4 * cap)));
+  w.deep_seg.reserve(sizeof(uint32_t) * (size_t)(grid * cap));
+  w.deep_counter.reserve(sizeof(unsigned));
+  unsigned *counter = w.deep_counter.as<unsigned>();
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, counter);
+  MVG_LAUNCH_CHECK("reset_counter_kernel");
+  hipLaunchKernelGGL((ivfpq_deep_scan_kernel<MF, IP, MIXED, DF>), dim3((unsigned)grid), dim3(kNT), sm,
+                     s, a.q, a.nq, a.d, a.M, a.coarse, a.pq, a.ptab, a.list_off, a.codes, a.ids,
+                     probes, maxp, a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len,
+                     out_keys, a.D, a.I, kIvfPadDist, w.deep_seg.as<uint32_t>(), cap, counter);
+  MVG_LAUNCH_CHECK("ivfpq_deep_scan_kernel");
+}
+
+template <int MF, bool IP, bool MIXED>
+void launch_deep(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk, uint64_t *out_keys,
+                 IvfWork &w, hipStream_t s) {
+  if constexpr (!IP && !MIXED && MF == 16) {
+    if (a.d == 128) return launch_deep_df<MF, IP, MIXED, 128>(a, maxp, probes, kk, out_keys, w, s);
+    if (a.d == 64) return launch_deep_df<MF, IP, MIXED, 64>(a, maxp, probes, kk, out_keys, w, s);
+  }
+  if constexpr (!IP && !MIXED && MF == 12) {
+    if (a.d == 96) return launch_deep_df<MF, IP, MIXED, 96>(a, maxp, probes, kk, out_keys, w, s);
+  }
+  if constexpr (!IP && !MIXED && MF == 8) {
+    if (a.d == 64) return launch_deep_df<MF, IP, MIXED, 64>(a, maxp, probes, kk, out_keys, w, s);
+  }
+  launch_deep_df<MF, IP, MIXED, 0>(a, maxp, probes, kk, out_keys, w, s);
+}
+
 }  // namespace
 
 void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, int nlist,
@@ -697,9 +1037,22 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     }
     const bool mixed = a.precision != 0 && !ip;
     if (mixed && !a.ptab) throw Error(kState, "mixed-precision tables are missing");
+    const bool deep = kk >= (a.deep_min > 0 ? a.deep_min : kDeepMinCandidates);
     WorkQueue wq{};
     const WorkQueue *wqp = nullptr;
-    if (a.wq_entries > 0) {
+    if (deep) {
+#define MVG_DEEP(MFV)                                                                \
+  if (ip) launch_deep<MFV, true, false>(a, maxp, probes, kk, keys, w, s);           \
+  else if (mixed) launch_deep<MFV, false, true>(a, maxp, probes, kk, keys, w, s);   \
+  else launch_deep<MFV, false, false>(a, maxp, probes, kk, keys, w, s);
+      switch (a.M) {
+        case 8: MVG_DEEP(8) break;
+        case 12: MVG_DEEP(12) break;
+        case 16: MVG_DEEP(16) break;
+        default: MVG_DEEP(0) break;
+      }
+#undef MVG_DEEP
+    } else if (a.wq_entries > 0) {
       // (query, entry-range) work queue built on the device: no host sync.
       w.wq_poff.reserve(sizeof(int64_t) * (size_t)a.nq * (maxp + 1));
       w.wq_nitems.reserve(sizeof(int) * (size_t)a.nq);
@@ -729,11 +1082,13 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   if (ip) launch_query_scan<MFV, true, false>(a, maxp, probes, kk, keys, wqp, s);       \
   else if (mixed) launch_query_scan<MFV, false, true>(a, maxp, probes, kk, keys, wqp, s); \
   else launch_query_scan<MFV, false, false>(a, maxp, probes, kk, keys, wqp, s);
-    switch (a.M) {
-      case 8: MVG_QSCAN(8) break;
-      case 12: MVG_QSCAN(12) break;
-      case 16: MVG_QSCAN(16) break;
-      default: MVG_QSCAN(0) break;
+    if (!deep) {
+      switch (a.M) {
+        case 8: MVG_QSCAN(8) break;
+        case 12: MVG_QSCAN(12) break;
+        case 16: MVG_QSCAN(16) break;
+        default: MVG_QSCAN(0) break;
+      }
     }
 #undef MVG_QSCAN
     if (wqp) {  // segmented merge of each query's items

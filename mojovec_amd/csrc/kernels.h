@@ -63,6 +63,8 @@ struct IvfSearchArgs {
   const float *coarse_norm;     // and their squared norms
   int wq_entries;  // > 0: persistent work queue, items of about this many entries
   int64_t ntotal;  // rows of `vectors`
+  int64_t deep_cap;  // sum of the nprobe largest lists: most entries one query visits
+  int deep_min;      // k * k_factor from which the deep-pool scan runs (0: default)
 };
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> for the mixed L2 scan.
 void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, int nlist,
@@ -79,6 +81,7 @@ struct IvfProfile {
 struct IvfWork {
   DevBuf coarse_part, probes, part, keys, qbf, qnorm;
   DevBuf wq_poff, wq_nitems, wq_rq, wq_off, wq_gthr, wq_counter;
+  DevBuf deep_seg, deep_counter;  // deep candidate pools: key segments, query counter
   IvfProfile prof;
 };
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);

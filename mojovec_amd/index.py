@@ -241,6 +241,11 @@ class IndexIVFPQ:
         """Persistent (query, entry-range) work queue for skewed lists; 0 off."""
         _lib.check(_lib.load().mvg_ivfpq_set_work_queue(self._h, int(entries_per_item)))
 
+    def set_deep_select(self, min_candidates: int = 0) -> None:
+        """k * k_factor from which the scan selects its candidates once per
+        query from keys stored in HBM (0: default 513). Same results."""
+        _lib.check(_lib.load().mvg_ivfpq_set_deep_select(self._h, int(min_candidates)))
+
     def train(self, x, niter_coarse: int = 15, niter_pq: int = 15, seed: int = KMEANS_SEED) -> None:
         x = _f32(x, self.d)
         _lib.check(_lib.load().mvg_ivfpq_train(self._h, _ptr(x), x.shape[0], niter_coarse, niter_pq, seed))

@@ -252,6 +252,67 @@ def test_ivfpq_long_lists_cross_scan_chunks(gpu, mode):
         assert_same(Dg, Ig, Do, Io)
 
 
+@pytest.mark.parametrize("d,M,metric", [(128, 16, 0), (64, 8, 0), (32, 8, 1), (48, 6, 0)])
+def test_ivfpq_deep_select_matches_oracle(gpu, d, M, metric):
+    # deep-pool scan (keys in HBM, one radix select per query) on every
+    # k * k_factor: same canonical top-k as the oracle
+    x = clustered(20 + d, 6000, d, nclusters=40)
+    if metric == 1:
+        x = x / 255.0
+    oi, gi = _pair(gpu, x, d, 32, M, metric)
+    oi.add(x)
+    gi.add(x)
+    gi.set_deep_select(1)
+    q = clustered(21 + d, 30, d, nclusters=40)
+    if metric == 1:
+        q = q / 255.0
+    for nprobe, k, kf in [(4, 10, 1), (8, 700, 1), (32, 10, 60), (32, 1000, 1), (32, 3, 1)]:
+        gi.nprobe = nprobe
+        Dg, Ig = gi.search(q, k, k_factor=kf)
+        Do, Io = oi.search(q, k, nprobe, k_factor=kf)
+        assert_same(Dg, Ig, Do, Io)
+    npq = np.random.default_rng(1).integers(1, 40, 30).astype(np.int32)
+    ex = np.zeros(6000, np.uint8)
+    ex[::5] = 1
+    Dg, Ig = gi.search(q, 800, nprobe_per_query=npq, exclude=ex)
+    Do, Io = oi.search(q, 800, 1, nprobe_per_query=npq, exclude=ex)
+    assert_same(Dg, Ig, Do, Io)
+    assert not np.isin(Ig[Ig >= 0], np.nonzero(ex)[0]).any()
+
+
+def test_ivfpq_deep_select_ties_break_by_lower_id(gpu):
+    # every row 12 times: runs of equal codes and equal ADC distances, so the
+    # k-th key is shared by keys on both sides of the cut
+    x = np.repeat(clustered(30, 400, 16, nclusters=8), 12, axis=0)
+    oi, gi = _pair(gpu, x, 16, 8, 4, 0)
+    oi.add(x)
+    gi.add(x)
+    q = clustered(31, 20, 16, nclusters=8)
+    gi.nprobe = 8
+    for k in (515, 1001, 7):
+        Do, Io = oi.search(q, k, 8)
+        for deep_min in (1, 100000):  # deep form, running top-k form
+            gi.set_deep_select(deep_min)
+            Dg, Ig = gi.search(q, k)
+            assert_same(Dg, Ig, Do, Io)
+
+
+def test_ivfpq_deep_select_mixed_equals_running_form(gpu):
+    x = clustered(32, 8000, 64, nclusters=40)
+    oi, gi = _pair(gpu, x, 64, 32, 16, 0)
+    gi.add(x)
+    gi.set_search_mode("mixed")
+    gi.nprobe = 12
+    q = clustered(33, 40, 64, nclusters=40)
+    for k, kf in ((600, 1), (10, 80)):
+        gi.set_deep_select(1)
+        Dd, Id = gi.search(q, k, k_factor=kf)
+        gi.set_deep_select(100000)
+        Dr, Ir = gi.search(q, k, k_factor=kf)
+        assert np.array_equal(Id, Ir) and np.array_equal(Dd, Dr)
+    gi.set_search_mode("plain")
+
+
 def test_ivfpq_mixed_form_error_bound_and_rerank(gpu):
     # Mixed L2 tables: |q - c_l|^2 + T[l] + Q[q] equals the residual ADC
     # distance up to FP32 rounding; with every candidate reranked the result

@@ -32,7 +32,7 @@ def test_server_results_match_alone(gpu):
     srv.close()
 
 
-@pytest.mark.parametrize("wq", [0, 1024])
+@pytest.mark.parametrize("wq", [0, 1024, "deep"])
 def test_captured_search_graph_replays_match_eager(gpu, wq):
     import torch
 
@@ -41,7 +41,10 @@ def test_captured_search_graph_replays_match_eager(gpu, wq):
     ix = gpu.IndexIVFPQ(32, 64, 8)
     ix.train(x, niter_coarse=4, niter_pq=4)
     ix.add(x)
-    ix.set_work_queue(wq)
+    if wq == "deep":  # every k * k_factor takes the deep-pool scan
+        ix.set_deep_select(1)
+    else:
+        ix.set_work_queue(wq)
     nq, k = 300, 10
     dev = torch.device("cuda:0")
     q = torch.zeros((nq, 32), dtype=torch.float32, device=dev)
