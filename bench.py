@@ -36,6 +36,7 @@ RECALL_BAR = 0.95
 # ~0.17 at nprobe 64): the 0.95 bar needs a deep exact-rerank pool, up to
 # k * k_factor = 4000 candidates (the device top-k limit is 4096).
 KF_SWEEP = (1, 4, 16, 64, 128, 256, 400)
+DEEP_MIN = 513  # k * k_factor from which the library runs the deep-pool scan kernel
 
 
 def log(msg):
@@ -173,42 +174,52 @@ def main(argv=None):
 
     L = _lib.load()
 
-    def timed(form, nprobe, kf, profile):
-        _lib.check(L.mvg_ivfpq_set_profiling(ix._h, int(profile)))
+    def timed(form, nprobe, kf):
         for _ in range(args.warmup):
             search(form, nprobe, kf)
         torch.cuda.synchronize()
-        stage = (ctypes.c_float * 3)()
-        scanned = ctypes.c_int64()
-        scan_ms, entries = [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             search(form, nprobe, kf)
-            if profile:
-                _lib.check(L.mvg_ivfpq_read_profile(ix._h, stage, ctypes.byref(scanned)))
-                scan_ms.append(stage[1])
-                entries.append(scanned.value)
         torch.cuda.synchronize()
-        elapsed = time.perf_counter() - t0
+        return time.perf_counter() - t0, recall_at(torch, gt, Id)
+
+    def profiled(form, nprobe, kf, reps=5):
+        """Stage times from HIP events on the search stream: coarse, scan,
+        merge + rerank; and the list entries the scan visits."""
+        _lib.check(L.mvg_ivfpq_set_profiling(ix._h, 1))
+        stage = (ctypes.c_float * 3)()
+        scanned = ctypes.c_int64()
+        rows = []
+        search(form, nprobe, kf)
+        for _ in range(reps):
+            search(form, nprobe, kf)
+            _lib.check(L.mvg_ivfpq_read_profile(ix._h, stage, ctypes.byref(scanned)))
+            rows.append((stage[0], stage[1], stage[2], scanned.value))
         _lib.check(L.mvg_ivfpq_set_profiling(ix._h, 0))
-        return elapsed, recall_at(torch, gt, Id), scan_ms, entries
+        r = np.mean(np.asarray(rows, np.float64), axis=0)
+        return {"coarse_ms": round(r[0], 4), "scan_ms": round(r[1], 4),
+                "merge_rerank_ms": round(r[2], 4)}, float(r[3])
 
     # headline form: mixed when it reaches the recall bar, else plain
     head = "mixed" if best["mixed"]["recall"] >= RECALL_BAR else "plain"
     other = "plain" if head == "mixed" else "mixed"
     nprobe, kf = best[head]["nprobe"], best[head]["k_factor"]
     log(f"timing {head} nprobe={nprobe} k_factor={kf}")
-    elapsed, rec, scan_ms, entries = timed(head, nprobe, kf, True)
+    elapsed, rec = timed(head, nprobe, kf)
     qps = NQ * args.steps / elapsed
-    o_elapsed, o_rec, _, _ = timed(other, best[other]["nprobe"], best[other]["k_factor"], False)
+    o_elapsed, o_rec = timed(other, best[other]["nprobe"], best[other]["k_factor"])
     o_qps = NQ * args.steps / o_elapsed
-    scan_s = float(np.mean(scan_ms)) / 1e3
-    bytes_per_launch = float(np.mean(entries)) * (M + 4)
+    stages, entries = profiled(head, nprobe, kf)
+    o_stages, _ = profiled(other, best[other]["nprobe"], best[other]["k_factor"])
+    scan_s = stages["scan_ms"] / 1e3
+    bytes_per_launch = entries * (M + 4)
     achieved = bytes_per_launch / scan_s / 1e9
-    log(f"{head} qps={qps:.0f} recall={rec:.4f} scan={scan_s * 1e3:.3f}ms {achieved:.0f} GB/s; "
-        f"{other} qps={o_qps:.0f} recall={o_rec:.4f}")
-    forms = {head: {**best[head], "qps": round(qps, 1), "recall": round(rec, 4)},
-             other: {**best[other], "qps": round(o_qps, 1), "recall": round(o_rec, 4)}}
+    log(f"{head} qps={qps:.0f} recall={rec:.4f} stages={stages} scan {achieved:.0f} GB/s; "
+        f"{other} qps={o_qps:.0f} recall={o_rec:.4f} stages={o_stages}")
+    forms = {head: {**best[head], "qps": round(qps, 1), "recall": round(rec, 4), "stages": stages},
+             other: {**best[other], "qps": round(o_qps, 1), "recall": round(o_rec, 4),
+                     "stages": o_stages}}
     plain_qps = forms["plain"]["qps"]
     ratio = forms["mixed"]["qps"] / plain_qps if forms["mixed"]["recall"] >= RECALL_BAR else None
 
@@ -245,7 +256,7 @@ def main(argv=None):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "ivfpq_query_scan_kernel",
+            "kernel": "ivfpq_deep_scan_kernel" if K * kf >= DEEP_MIN else "ivfpq_query_scan_kernel",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -253,7 +264,7 @@ def main(argv=None):
             "traffic": None,
             "scan_ms": round(scan_s * 1e3, 4),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
-            "scanned_entries_per_launch": int(np.mean(entries)),
+            "scanned_entries_per_launch": int(entries),
         },
         "cpu_baseline": cpu,
     }

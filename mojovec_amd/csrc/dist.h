@@ -288,6 +288,58 @@ __device__ __forceinline__ float wave_tree_n(float v) {
   return __shfl(v, 0, 64);
 }
 
+// The same W = 64 distance computed by a group of 16 lanes, each holding four
+// consecutive lanes of the 64-wide accumulators as a float4, for d a positive
+// multiple of 64 (no tails). The four-accumulator blocks, the W blocks and
+// every pairing of the halving tree are those of wave_dist_w64: virtual
+// strides 32, 16, 8 and 4 are lane strides 8, 4, 2 and 1, strides 2 and 1
+// fold inside the float4. The result is bit-identical and valid in glane 0.
+// q and x must be 16-byte aligned.
+template <bool IP>
+__device__ __forceinline__ void w64_step(float4 &acc, float4 qv, float4 xv) {
+  if (IP) {
+    acc.x = fma_(qv.x, xv.x, acc.x);
+    acc.y = fma_(qv.y, xv.y, acc.y);
+    acc.z = fma_(qv.z, xv.z, acc.z);
+    acc.w = fma_(qv.w, xv.w, acc.w);
+  } else {
+    const float dx = qv.x - xv.x, dy = qv.y - xv.y, dz = qv.z - xv.z, dw = qv.w - xv.w;
+    acc.x = fma_(dx, dx, acc.x);
+    acc.y = fma_(dy, dy, acc.y);
+    acc.z = fma_(dz, dz, acc.z);
+    acc.w = fma_(dw, dw, acc.w);
+  }
+}
+
+template <bool IP>
+__device__ float group16_dist_w64(const float *q, const float *x, int d, int glane) {
+  const float4 *q4 = reinterpret_cast<const float4 *>(q);
+  const float4 *x4 = reinterpret_cast<const float4 *>(x);
+  float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0, a3 = a0;
+  const int n4 = d >> 2;
+  int i = 0;  // float4 index; a 4W block is 64 float4, a W block 16
+  for (; i + 64 <= n4; i += 64) {
+    w64_step<IP>(a0, q4[i + glane], x4[i + glane]);
+    w64_step<IP>(a1, q4[i + 16 + glane], x4[i + 16 + glane]);
+    w64_step<IP>(a2, q4[i + 32 + glane], x4[i + 32 + glane]);
+    w64_step<IP>(a3, q4[i + 48 + glane], x4[i + 48 + glane]);
+  }
+  float4 v;
+  v.x = ((a0.x + a1.x) + a2.x) + a3.x;
+  v.y = ((a0.y + a1.y) + a2.y) + a3.y;
+  v.z = ((a0.z + a1.z) + a2.z) + a3.z;
+  v.w = ((a0.w + a1.w) + a2.w) + a3.w;
+  for (; i + 16 <= n4; i += 16) w64_step<IP>(v, q4[i + glane], x4[i + glane]);
+#pragma unroll
+  for (int h = 8; h >= 1; h >>= 1) {
+    v.x = v.x + __shfl_down(v.x, h, 16);
+    v.y = v.y + __shfl_down(v.y, h, 16);
+    v.z = v.z + __shfl_down(v.z, h, 16);
+    v.w = v.w + __shfl_down(v.w, h, 16);
+  }
+  return (v.x + v.z) + (v.y + v.w);
+}
+
 template <bool IP>
 __device__ float wave_dist_w64(const float *q, const float *x, int d, int lane) {
   if (d < 64) {

@@ -19,6 +19,12 @@ namespace {
 
 constexpr int kScanChunk = 4096;  // list positions scored per select round
 
+__host__ __device__ inline int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
 template <int MF>
 __device__ __forceinline__ float adc_sum(const uint8_t *__restrict__ code, const float *lut, int M,
                                          float init) {
@@ -171,6 +177,50 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
   }
 }
 
+// Rerank for d a multiple of 64: sixteen 16-lane groups per workgroup each
+// score one candidate row at a time (group16_dist_w64, bit-identical to the
+// wavefront form), so 16 rows per workgroup are in flight instead of 4.
+template <bool IP>
+__global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restrict__ q, int d,
+                                                           const float *__restrict__ vectors,
+                                                           int64_t nrows,
+                                                           const uint64_t *__restrict__ cand,
+                                                           int kk, int k, float pad_dist,
+                                                           float *__restrict__ D,
+                                                           int64_t *__restrict__ I) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float *qs = reinterpret_cast<float *>(smem);                      // d
+  uint64_t *buf = reinterpret_cast<uint64_t *>(qs + d);             // kk
+  uint64_t *top = buf + ((kk + 1) & ~1);                            // next_pow2(k)
+  SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(top + next_pow2(k));
+  const int tid = threadIdx.x, glane = tid & 15, group = tid >> 4;
+  const int64_t qi = blockIdx.x;
+  for (int j = tid; j < d; j += kNT) qs[j] = q[qi * d + j];
+  __syncthreads();
+  const uint64_t *cq = cand + qi * kk;
+  for (int c = group; c < kk; c += kNT / 16) {
+    const uint64_t key = cq[c];
+    const uint32_t id = (uint32_t)(key & 0xFFFFFFFFu);
+    const bool valid = key != kSentinel64 && (int64_t)id < nrows;
+    float ex = 0.0f;
+    if (valid) ex = group16_dist_w64<IP>(qs, vectors + (int64_t)id * d, d, glane);
+    if (glane == 0) buf[c] = valid ? pack_key(IP ? -ex : ex, id) : kSentinel64;
+  }
+  __syncthreads();
+  const int cnt = select_sort_u64(buf, kk, k, *scr, top);
+  for (int t = tid; t < k; t += kNT) {
+    float dv = pad_dist;
+    int64_t iv = -1;
+    if (t < cnt) {
+      dv = key2f((uint32_t)(top[t] >> 32));
+      if (IP) dv = -dv;
+      iv = (int64_t)(uint32_t)(top[t] & 0xFFFFFFFFu);
+    }
+    D[qi * k + t] = dv;
+    I[qi * k + t] = iv;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Per-query scan: one workgroup walks the query's probed lists in coarse
 // order and keeps a running canonical top-kk of u64 (key << 32 | id) in LDS.
@@ -285,12 +335,6 @@ __device__ void flush_candidates(uint64_t *cand, uint64_t *tmp, int kk, QuerySca
     st.thr = cand[kk - 1];  // sentinel until kk candidates exist
   }
   __syncthreads();
-}
-
-__host__ __device__ inline int next_pow2(int v) {
-  int p = 1;
-  while (p < v) p <<= 1;
-  return p;
 }
 
 // LDS layout of the per-query scan (bytes), shared by host and device.
@@ -973,11 +1017,28 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
             hipStream_t s) {
   if (nq == 0) return;
+  const bool ip = metric == kMetricIP;
+  if (d >= 64 && d % 64 == 0) {
+    const size_t sm = sizeof(float) * d + sizeof(uint64_t) * (size_t)(((kk + 1) & ~1) + next_pow2(k)) +
+                      sizeof(SelectScratch64);
+    if (sm <= 150 * 1024) {
+      const void *fn = ip ? (const void *)rerank_group_kernel<true> : (const void *)rerank_group_kernel<false>;
+      if (sm > 64 * 1024)
+        MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+      if (ip)
+        hipLaunchKernelGGL(rerank_group_kernel<true>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d,
+                           vectors, nrows, cand, kk, k, pad_dist, D, I);
+      else
+        hipLaunchKernelGGL(rerank_group_kernel<false>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d,
+                           vectors, nrows, cand, kk, k, pad_dist, D, I);
+      MVG_LAUNCH_CHECK("rerank_group_kernel");
+      return;
+    }
+  }
   int buf_n = 1;
   while (buf_n < kk || buf_n < k) buf_n <<= 1;
   const size_t sm = sizeof(float) * ((d + 1) & ~1) + 2 * sizeof(uint64_t) * buf_n + sizeof(SelectScratch);
   if (sm > 150 * 1024) throw Error(kInvalid, "rerank workspace exceeds LDS");
-  const bool ip = metric == kMetricIP;
   const void *fn = ip ? (const void *)rerank_kernel<true> : (const void *)rerank_kernel<false>;
   if (sm > 64 * 1024)
     MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
