@@ -166,6 +166,7 @@ void ivf_sync_lists(mvg_ivfpq *h) {
   upload(h->ids, ids.data(), ids.size(), h->s);
   MVG_HIP(hipStreamSynchronize(h->s));
   h->dirty = false;
+  h->pre_dirty = true;
 }
 
 }  // namespace
@@ -640,6 +641,14 @@ void mvg::ivf_prepare(mvg_ivfpq *h) {
                  h->coarse_norm.as<float>(), h->s);
     MVG_HIP(hipStreamSynchronize(h->s));
     h->tables_dirty = false;
+    h->pre_dirty = true;
+  }
+  if (h->precision != 0 && h->metric == MVG_METRIC_L2 && h->pre_dirty) {
+    h->entry_pre.reserve(sizeof(float) * (size_t)std::max<int64_t>(h->ntotal, 1));
+    mvg::precompute_entry_terms(h->list_off.as<int64_t>(), h->nlist, h->codes.as<uint8_t>(), h->M,
+                                h->ptab.as<float>(), h->ntotal, h->entry_pre.as<float>(), h->s);
+    MVG_HIP(hipStreamSynchronize(h->s));
+    h->pre_dirty = false;
   }
 }
 
@@ -677,7 +686,11 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.deep_min = h->deep_min;
   if (h->precision != 0) {
     if (h->tables_dirty) throw Error(mvg::kState, "mixed-precision tables are not prepared");
-    if (h->metric == MVG_METRIC_L2) a.ptab = h->ptab.as<float>();
+    if (h->metric == MVG_METRIC_L2) {
+      a.ptab = h->ptab.as<float>();
+      if (h->pre_dirty) throw Error(mvg::kState, "mixed-precision entry terms are not prepared");
+      a.pre = h->entry_pre.as<float>();
+    }
     a.coarse_bf16 = h->coarse_bf16.as<uint16_t>();
     a.coarse_norm = h->coarse_norm.as<float>();
   }

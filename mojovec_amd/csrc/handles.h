@@ -13,7 +13,8 @@ struct mvg_ivfpq {
   bool trained = false, keep_vectors = true, dirty = true, tables_dirty = true;
   int precision = 0, split_scan = 0, wq_entries = 0, deep_min = 0;
   std::vector<int64_t> top_sum;  // top_sum[p]: entries in the p largest lists
-  mvg::DevBuf ptab, coarse_bf16, coarse_norm;
+  mvg::DevBuf ptab, coarse_bf16, coarse_norm, entry_pre;
+  bool pre_dirty = true;  // entry_pre out of date with the lists or tables
   int64_t ntotal = 0;
   std::vector<std::vector<int32_t>> lids;   // inverted lists, append order
   std::vector<std::vector<uint8_t>> lcodes;

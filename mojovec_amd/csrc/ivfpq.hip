@@ -46,6 +46,40 @@ __device__ __forceinline__ float adc_sum(const uint8_t *__restrict__ code, const
   return dist;
 }
 
+// Code words of one entry for the fixed-M scans (MF = 8, 12 or 16), loaded
+// ahead of the LUT lookups so that a thread keeps several entries' loads in
+// flight; adc_words sums them in the same order as adc_sum.
+template <int MF>
+struct Code {
+  uint32_t w[MF > 0 ? MF / 4 : 1];
+};
+
+template <int MF>
+__device__ __forceinline__ Code<MF> load_code(const uint8_t *__restrict__ codes, int64_t entry) {
+  Code<MF> c;
+  if constexpr (MF == 16) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(codes + entry * 16);
+    c.w[0] = v.x; c.w[1] = v.y; c.w[2] = v.z; c.w[3] = v.w;
+  } else if constexpr (MF == 8) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(codes + entry * 8);
+    c.w[0] = v.x; c.w[1] = v.y;
+  } else if constexpr (MF == 12) {
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(codes + entry * 12);
+    c.w[0] = p[0]; c.w[1] = p[1]; c.w[2] = p[2];
+  } else {
+    c.w[0] = 0u;
+  }
+  return c;
+}
+
+template <int MF>
+__device__ __forceinline__ float adc_words(const Code<MF> &c, const float *lut, float init) {
+  float dist = init;
+#pragma unroll
+  for (int m = 0; m < MF; m++) dist += lut[m * kKsub + ((c.w[m >> 2] >> (8 * (m & 3))) & 0xFF)];
+  return dist;
+}
+
 // One workgroup per (query, probe rank).
 template <int MF, bool IP>
 __global__ __launch_bounds__(kNT) void ivfpq_scan_kernel(
@@ -238,8 +272,8 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
 //   IP (both) : -<q_m, c_mk> once per query plus -<q, c_l> per list
 //               (index_ivf_pq.mojo:288-293, 316-324).
 
-constexpr int kCandCap = 1024;  // append-buffer capacity (u64 keys)
-constexpr int kRound = 2 * kNT;  // list entries scored between buffer checks
+constexpr int kCandCap = 2048;  // append-buffer capacity (u64 keys)
+constexpr int kRound = 4 * kNT;  // list entries scored between buffer checks
 
 struct QueryScanShared {
   int cnt;
@@ -537,16 +571,44 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
                                                          qtab, cbr, &st->base);
     for (int64_t r0 = seg_lo; r0 < L; r0 += kRound) {
       const uint64_t thr = st->thr;
+      constexpr int U = kRound / kNT;
+      if constexpr (MF > 0) {
+        Code<MF> cw[U];
+        int32_t idv[U];
 #pragma unroll
-      for (int u = 0; u < kRound / kNT; u++) {
-        const int64_t pos = r0 + u * kNT + tid;
-        if (pos < L) {
-          const int32_t id = ids[off + pos];
-          const bool skip = excl && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
-          if (!skip) {
-            const float dist = adc_sum<MF>(codes + (off + pos) * M, lut, M, base);
-            const uint64_t key = pack_key(dist, (uint32_t)id);
-            if (key < thr) cand[kk + atomicAdd(&st->cnt, 1)] = key;
+        for (int u = 0; u < U; u++) {  // every load of the round in flight first
+          const int64_t pos = r0 + u * kNT + tid;
+          cw[u] = Code<MF>{};
+          idv[u] = -1;
+          if (pos < L) {
+            cw[u] = load_code<MF>(codes, off + pos);
+            idv[u] = ids[off + pos];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int64_t pos = r0 + u * kNT + tid;
+          if (pos < L) {
+            const int32_t id = idv[u];
+            const bool skip = excl && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
+            if (!skip) {
+              const uint64_t key = pack_key(adc_words<MF>(cw[u], lut, base), (uint32_t)id);
+              if (key < thr) cand[kk + atomicAdd(&st->cnt, 1)] = key;
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int64_t pos = r0 + u * kNT + tid;
+          if (pos < L) {
+            const int32_t id = ids[off + pos];
+            const bool skip = excl && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
+            if (!skip) {
+              const float dist = adc_sum<MF>(codes + (off + pos) * M, lut, M, base);
+              const uint64_t key = pack_key(dist, (uint32_t)id);
+              if (key < thr) cand[kk + atomicAdd(&st->cnt, 1)] = key;
+            }
           }
         }
       }
@@ -605,6 +667,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
 // where seg_cap is the sum of the max_nprobe largest lists (the most any
 // query can visit), and takes queries from a device counter.
 constexpr int kDeepMinCandidates = 513;  // default: the deep form for k * k_factor >= this
+constexpr int kDeepBatch = 4;            // list entries per thread with loads in flight
 
 struct DeepShared {
   int sh[16];  // block_excl_scan / digit search scratch
@@ -721,18 +784,47 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
       if (L <= 0) continue;
       const float base = build_list_lut<MF, IP, MIXED, DF>(qq, coarse, l, d, M, pq, ptab, rv, lut,
                                                            qtab, cbr, &ds.base);
-      for (int64_t pos = tid; pos < L; pos += kNT) {
-        bool skip = false;
-        if (excl) {  // :328-334
-          const int32_t id = ids[off + pos];
-          skip = id >= 0 && id < excl_len && excl[id] != 0;
+      if constexpr (MF > 0) {
+        constexpr int U = kDeepBatch;
+        for (int64_t p0 = tid; p0 < L; p0 += (int64_t)kNT * U) {
+          Code<MF> cw[U];
+#pragma unroll
+          for (int u = 0; u < U; u++) {  // every load of the batch in flight first
+            const int64_t pos = p0 + (int64_t)u * kNT;
+            cw[u] = Code<MF>{};
+            if (pos < L) cw[u] = load_code<MF>(codes, off + pos);
+          }
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            const int64_t pos = p0 + (int64_t)u * kNT;
+            if (pos >= L) continue;
+            bool skip = false;
+            if (excl) {  // :328-334
+              const int32_t id = ids[off + pos];
+              skip = id >= 0 && id < excl_len && excl[id] != 0;
+            }
+            uint32_t key = kSentinel;
+            if (!skip) {
+              key = f2key(adc_words<MF>(cw[u], lut, base));
+              atomicAdd(&hist[key >> 21], 1);
+            }
+            seg[n + pos] = key;
+          }
         }
-        uint32_t key = kSentinel;
-        if (!skip) {
-          key = f2key(adc_sum<MF>(codes + (off + pos) * M, lut, M, base));
-          atomicAdd(&hist[key >> 21], 1);
+      } else {
+        for (int64_t pos = tid; pos < L; pos += kNT) {
+          bool skip = false;
+          if (excl) {  // :328-334
+            const int32_t id = ids[off + pos];
+            skip = id >= 0 && id < excl_len && excl[id] != 0;
+          }
+          uint32_t key = kSentinel;
+          if (!skip) {
+            key = f2key(adc_sum<MF>(codes + (off + pos) * M, lut, M, base));
+            atomicAdd(&hist[key >> 21], 1);
+          }
+          seg[n + pos] = key;
         }
-        seg[n + pos] = key;
       }
       n += L;
       __syncthreads();  // LUT and base are rebuilt for the next list
@@ -845,6 +937,333 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
 }
 
 __global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
+
+// ---------------------------------------------------------------------------
+// Stream scan: a query's probed lists are scanned as one stream of entries
+// with a single LUT per query, so no per-list LUT is built:
+//   dist(q, e) = base[p] + pre[e] + sum_m lut[m][code_m]   (m in order)
+// Plain IP is the reference's own form (index_ivf_pq.mojo:288-293, 316-341):
+// base = -<q, c_l>, lut = -<q_m, c_mk>, no pre term. Mixed L2 uses the
+// residual decomposition
+//   |q - c_l - r|^2 = |q - c_l|^2 + sum_m T[l][m][k_m] - 2 sum_m <q_m, c_mk>
+// with pre[e] = sum_m T[l][m][code_m] computed once per stored entry, so each
+// entry costs its code, id and pre reads and M LDS lookups. DEEP selects the
+// top-kk once per query from keys written to the workgroup's HBM segment (as
+// ivfpq_deep_scan_kernel); otherwise a running top-kk with a threshold
+// filter (as ivfpq_query_scan_kernel).
+constexpr int kStreamMaxProbes = 1024;
+
+struct StreamLayout {
+  int lut, rv, poff, loff, base, cand, tmp, scr, hist, st, total;
+  __host__ __device__ StreamLayout(int M, int d, int kk, int maxp, bool deep) {
+    lut = 0;
+    rv = lut + M * kKsub * 4;
+    poff = (rv + 4 * d + 15) & ~15;
+    loff = (poff + 4 * (maxp + 1) + 15) & ~15;
+    base = loff + 8 * maxp;
+    cand = (base + 4 * maxp + 15) & ~15;
+    if (deep) {  // cand holds the winners, hist the digit histogram
+      tmp = cand;
+      scr = cand;
+      hist = cand + 8 * next_pow2(kk);
+      st = hist + 4 * 2048;
+      total = st + (int)sizeof(DeepShared);
+    } else {
+      tmp = cand + 8 * (kk + kCandCap);
+      scr = tmp + 8 * next_pow2(kk);
+      hist = scr;
+      st = (scr + (int)sizeof(SelectScratch64) + 15) & ~15;
+      total = st + (int)sizeof(QueryScanShared);
+    }
+  }
+};
+
+// Entry of stream position j: advances the thread's probe cursor p (j only
+// grows per thread) and returns the entry's CSR index.
+__device__ __forceinline__ int64_t stream_entry(const int *poff, const int64_t *loff, int64_t j,
+                                                int &p) {
+  while (poff[p + 1] <= j) p++;
+  return loff[p] + (j - poff[p]);
+}
+
+template <int MF, bool IP, bool DEEP>
+__global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
+    const float *__restrict__ q, int64_t nq, int d, int M, const float *__restrict__ coarse,
+    const float *__restrict__ pq, const float *__restrict__ pre,
+    const int64_t *__restrict__ list_off, const uint8_t *__restrict__ codes,
+    const int32_t *__restrict__ ids, const int32_t *__restrict__ probes, int maxp,
+    const int32_t *__restrict__ nprobe_q, int nprobe_all, int kk, int k,
+    const uint8_t *__restrict__ excl, int64_t excl_len, uint64_t *__restrict__ out_keys,
+    float *__restrict__ D, int64_t *__restrict__ I, float pad_dist, uint32_t *__restrict__ seg_base,
+    int64_t seg_cap, unsigned *__restrict__ counter) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const StreamLayout lay(M, d, kk, maxp, DEEP);
+  float *lut = reinterpret_cast<float *>(smem + lay.lut);
+  float *rv = reinterpret_cast<float *>(smem + lay.rv);
+  int *poff = reinterpret_cast<int *>(smem + lay.poff);
+  int64_t *loff = reinterpret_cast<int64_t *>(smem + lay.loff);
+  float *basev = reinterpret_cast<float *>(smem + lay.base);
+  uint64_t *cand = reinterpret_cast<uint64_t *>(smem + lay.cand);
+  uint64_t *tmp = reinterpret_cast<uint64_t *>(smem + lay.tmp);
+  SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(smem + lay.scr);
+  int *hist = reinterpret_cast<int *>(smem + lay.hist);
+  DeepShared &ds = *reinterpret_cast<DeepShared *>(smem + lay.st);
+  QueryScanShared *st = reinterpret_cast<QueryScanShared *>(smem + lay.st);
+  const int tid = threadIdx.x, nlut = M * kKsub, dsub = d / M;
+  constexpr bool kMixed = !IP;
+  uint32_t *seg = DEEP ? seg_base + (int64_t)blockIdx.x * seg_cap : nullptr;
+
+  for (;;) {
+    int64_t qi;
+    if constexpr (DEEP) {
+      __syncthreads();
+      if (tid == 0) ds.q = (int64_t)atomicAdd(counter, 1u);
+      __syncthreads();
+      qi = ds.q;
+      if (qi >= nq) break;
+    } else {
+      qi = blockIdx.x;
+    }
+    const float *qq = q + qi * d;
+    int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
+    np = min(max(np, 1), maxp);
+
+    // per-query LUT
+    for (int j = tid; j < d; j += kNT) rv[j] = qq[j];
+    if constexpr (DEEP) {
+      for (int b = tid; b < 2048; b += kNT) hist[b] = 0;
+    } else {
+      for (int i = tid; i < kk; i += kNT) cand[i] = kSentinel64;
+      if (tid == 0) {
+        st->cnt = 0;
+        st->thr = kSentinel64;
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < nlut; e += kNT) {
+      const int m = e >> 8;
+      const float *c = pq + (int64_t)e * dsub;
+      if (IP) {
+        lut[e] = -ip_pq(rv + m * dsub, c, dsub);  // pq.mojo:285-292
+      } else {
+        float dot = 0.0f;
+        for (int j = 0; j < dsub; j++) dot = fma_(rv[m * dsub + j], c[j], dot);
+        lut[e] = -2.0f * dot;
+      }
+    }
+    // probe table: stream offsets, CSR starts, per-list base
+    int carry = 0;
+    for (int p0 = 0; p0 < np; p0 += kNT) {
+      const int p = p0 + tid;
+      int len = 0;
+      if (p < np) {
+        const int l = probes[qi * maxp + p];
+        float b = 0.0f;
+        int64_t off = 0;
+        if (l >= 0) {
+          off = list_off[l];
+          len = (int)min<int64_t>(list_off[l + 1] - off, seg_cap);
+          const float *cl = coarse + (int64_t)l * d;
+          b = IP ? -ip_wn<8>(rv, cl, d) : l2_w8(rv, cl, d);  // :322-324 / |q - c_l|^2
+        }
+        loff[p] = off;
+        basev[p] = b;
+      }
+      int total;
+      const int ex = block_excl_scan(len, DEEP ? ds.sh : reinterpret_cast<int *>(scr->sh), total);
+      if (p < np) poff[p] = carry + ex;
+      carry += total;
+    }
+    if (tid == 0) poff[np] = carry;
+    __syncthreads();
+    const int64_t n = DEEP ? min<int64_t>(carry, seg_cap) : carry;
+
+    // scan the stream
+    int cur = 0;
+    constexpr int U = (MF > 0) ? 4 : 1;
+    for (int64_t r0 = 0; r0 < n; r0 += (int64_t)kNT * U) {
+      const uint64_t thr = DEEP ? kSentinel64 : st->thr;
+      Code<MF> cw[U];
+      int32_t idv[U];
+      float initv[U];
+      int64_t ev[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {  // every load of the batch in flight first
+        const int64_t j = r0 + (int64_t)u * kNT + tid;
+        cw[u] = Code<MF>{};
+        idv[u] = -1;
+        initv[u] = 0.0f;
+        ev[u] = 0;
+        if (j < n) {
+          const int64_t e = stream_entry(poff, loff, j, cur);
+          ev[u] = e;
+          if (MF > 0) cw[u] = load_code<MF>(codes, e);
+          if (!DEEP || excl) idv[u] = ids[e];
+          initv[u] = basev[cur];
+          if (kMixed) initv[u] = initv[u] + pre[e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t j = r0 + (int64_t)u * kNT + tid;
+        if (j >= n) continue;
+        const int32_t id = idv[u];
+        const bool skip = excl && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
+        float dist = 0.0f;
+        if (!skip) {
+          if constexpr (MF > 0) dist = adc_words<MF>(cw[u], lut, initv[u]);
+          else dist = adc_sum<0>(codes + ev[u] * M, lut, M, initv[u]);
+        }
+        if constexpr (DEEP) {
+          uint32_t key = kSentinel;
+          if (!skip) {
+            key = f2key(dist);
+            atomicAdd(&hist[key >> 21], 1);
+          }
+          seg[j] = key;
+        } else if (!skip) {
+          const uint64_t key = pack_key(dist, (uint32_t)id);
+          if (key < thr) cand[kk + atomicAdd(&st->cnt, 1)] = key;
+        }
+      }
+      if constexpr (!DEEP) {
+        __syncthreads();
+        if (st->cnt > kCandCap - kNT * U) flush_candidates(cand, tmp, kk, *st, *scr);
+      }
+    }
+
+    if constexpr (!DEEP) {
+      if (st->cnt > 0) flush_candidates(cand, tmp, kk, *st, *scr);
+      __syncthreads();
+      if (out_keys) {
+        for (int t = tid; t < kk; t += kNT) out_keys[qi * kk + t] = cand[t];
+      } else {
+        for (int t = tid; t < k; t += kNT) {
+          const uint64_t kv = cand[t];
+          float dv = pad_dist;
+          int64_t iv = -1;
+          if (kv != kSentinel64) {
+            dv = key2f((uint32_t)(kv >> 32));
+            if (IP) dv = -dv;  // :357-360
+            iv = (int64_t)(uint32_t)(kv & 0xFFFFFFFFu);
+          }
+          D[qi * k + t] = dv;
+          I[qi * k + t] = iv;
+        }
+      }
+      break;
+    } else {
+      __syncthreads();
+      // kk-th smallest key T (as ivfpq_deep_scan_kernel)
+      {
+        int local = 0;
+        for (int b = tid * 8; b < tid * 8 + 8; b++) local += hist[b];
+        const int valid = block_sum(local, ds.sh);
+        if (tid == 0) ds.valid = valid;
+      }
+      __syncthreads();
+      const bool take_all = ds.valid <= kk;
+      uint32_t T = 0u, Tid = 0xFFFFFFFFu;
+      if (!take_all) {
+        int bin, before, count, remaining = kk;
+        deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
+        remaining -= before;
+        uint32_t prefix = (uint32_t)bin << 21, mask = 0xFFE00000u;
+        deep_key_pass(seg, n, 10, 11, prefix, mask, hist);
+        deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
+        remaining -= before;
+        prefix |= (uint32_t)bin << 10;
+        mask |= 0x001FFC00u;
+        deep_key_pass(seg, n, 0, 10, prefix, mask, hist);
+        deep_find_bin(hist, 1024, remaining, ds, bin, before, count);
+        remaining -= before;
+        T = prefix | (uint32_t)bin;
+        if (remaining < count) {  // ties at T: the lowest ids win
+          uint32_t iprefix = 0u, imask = 0u;
+          const int shifts[3] = {21, 10, 0};
+          const int nbits[3] = {11, 11, 10};
+          for (int pass = 0; pass < 3; pass++) {
+            const int nb = 1 << nbits[pass];
+            const uint32_t dm = (uint32_t)nb - 1u;
+            for (int b = tid; b < nb; b += kNT) hist[b] = 0;
+            __syncthreads();
+            int c2 = 0;
+            for (int64_t j = tid; j < n; j += kNT) {
+              const int64_t e = stream_entry(poff, loff, j, c2);
+              if (seg[j] != T) continue;
+              const uint32_t id = (uint32_t)ids[e];
+              if ((id & imask) == iprefix) atomicAdd(&hist[(id >> shifts[pass]) & dm], 1);
+            }
+            __syncthreads();
+            int ibin, ibefore, icount;
+            deep_find_bin(hist, nb, remaining, ds, ibin, ibefore, icount);
+            remaining -= ibefore;
+            iprefix |= (uint32_t)ibin << shifts[pass];
+            imask |= dm << shifts[pass];
+          }
+          Tid = iprefix;
+        }
+      }
+      if (tid == 0) ds.cnt = 0;
+      __syncthreads();
+      {
+        int c2 = 0;
+        for (int64_t j = tid; j < n; j += kNT) {
+          const int64_t e = stream_entry(poff, loff, j, c2);
+          const uint32_t key = seg[j];
+          if (key == kSentinel || (!take_all && key > T)) continue;
+          const uint32_t id = (uint32_t)ids[e];
+          if (take_all || key < T || id <= Tid) {
+            const int slot = atomicAdd(&ds.cnt, 1);
+            if (slot < kk) cand[slot] = ((uint64_t)key << 32) | id;
+          }
+        }
+      }
+      __syncthreads();
+      const int cnt = min(ds.cnt, kk);
+      const int np2 = next_pow2(kk);
+      for (int i = cnt + tid; i < np2; i += kNT) cand[i] = kSentinel64;
+      __syncthreads();
+      bitonic_sort_u64(cand, np2);
+      if (out_keys) {
+        for (int t = tid; t < kk; t += kNT) out_keys[qi * kk + t] = cand[t];
+      } else {
+        for (int t = tid; t < k; t += kNT) {
+          const uint64_t kv = cand[t];
+          float dv = pad_dist;
+          int64_t iv = -1;
+          if (kv != kSentinel64) {
+            dv = key2f((uint32_t)(kv >> 32));
+            if (IP) dv = -dv;
+            iv = (int64_t)(uint32_t)(kv & 0xFFFFFFFFu);
+          }
+          D[qi * k + t] = dv;
+          I[qi * k + t] = iv;
+        }
+      }
+    }
+  }
+}
+
+// pre[e] = sum_m T[l][m][code_m] for every stored entry e of list l (mixed
+// L2 stream scan), summed in m order.
+__global__ void entry_terms_kernel(const int64_t *__restrict__ list_off, int nlist,
+                                   const uint8_t *__restrict__ codes, int M,
+                                   const float *__restrict__ ptab, int64_t ntotal,
+                                   float *__restrict__ pre) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ntotal) return;
+  int lo = 0, hi = nlist;  // list l with list_off[l] <= e < list_off[l + 1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (list_off[mid] <= e) lo = mid; else hi = mid;
+  }
+  const float *t = ptab + (int64_t)lo * M * kKsub;
+  const uint8_t *c = codes + e * M;
+  float acc = 0.0f;
+  for (int m = 0; m < M; m++) acc += t[m * kKsub + c[m]];
+  pre[e] = acc;
+}
 
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> (mixed L2 tables).
 __global__ void precompute_tables_kernel(const float *__restrict__ coarse, const float *__restrict__ pq,
@@ -987,6 +1406,48 @@ void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int
   MVG_LAUNCH_CHECK("ivfpq_deep_scan_kernel");
 }
 
+// Stream scan (plain IP, mixed L2). Returns false when the shape does not
+// fit its LDS layout; the caller then takes the per-list kernels.
+template <int MF, bool IP, bool DEEP>
+bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                   uint64_t *out_keys, IvfWork &w, hipStream_t s) {
+  const StreamLayout lay(a.M, a.d, kk, maxp, DEEP);
+  const size_t sm = (size_t)lay.total;
+  if (sm > 150 * 1024 || maxp > kStreamMaxProbes) return false;
+  const void *fn = (const void *)ivfpq_stream_scan_kernel<MF, IP, DEEP>;
+  if (sm > 64 * 1024)
+    MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  int64_t grid = a.nq, cap = INT32_MAX;
+  uint32_t *seg = nullptr;
+  unsigned *counter = nullptr;
+  if (DEEP) {
+    int per_cu = 0;
+    MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
+    cap = std::max<int64_t>(a.deep_cap, 1);
+    grid = std::min<int64_t>((int64_t)std::max(1, per_cu) * device_cus(), a.nq);
+    grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(4) << 30) / (4 * cap)));
+    w.deep_seg.reserve(sizeof(uint32_t) * (size_t)(grid * cap));
+    w.deep_counter.reserve(sizeof(unsigned));
+    seg = w.deep_seg.as<uint32_t>();
+    counter = w.deep_counter.as<unsigned>();
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, counter);
+    MVG_LAUNCH_CHECK("reset_counter_kernel");
+  }
+  hipLaunchKernelGGL((ivfpq_stream_scan_kernel<MF, IP, DEEP>), dim3((unsigned)grid), dim3(kNT), sm, s,
+                     a.q, a.nq, a.d, a.M, a.coarse, a.pq, a.pre, a.list_off, a.codes, a.ids, probes,
+                     maxp, a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D,
+                     a.I, kIvfPadDist, seg, cap, counter);
+  MVG_LAUNCH_CHECK("ivfpq_stream_scan_kernel");
+  return true;
+}
+
+template <int MF, bool IP>
+bool launch_stream_any(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk, bool deep,
+                       uint64_t *out_keys, IvfWork &w, hipStream_t s) {
+  return deep ? launch_stream<MF, IP, true>(a, maxp, probes, kk, out_keys, w, s)
+              : launch_stream<MF, IP, false>(a, maxp, probes, kk, out_keys, w, s);
+}
+
 template <int MF, bool IP, bool MIXED>
 void launch_deep(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk, uint64_t *out_keys,
                  IvfWork &w, hipStream_t s) {
@@ -1011,6 +1472,14 @@ void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, i
   hipLaunchKernelGGL(precompute_tables_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, s, coarse, pq,
                      d, M, nlist, ptab);
   MVG_LAUNCH_CHECK("precompute_tables_kernel");
+}
+
+void precompute_entry_terms(const int64_t *list_off, int nlist, const uint8_t *codes, int M,
+                            const float *ptab, int64_t ntotal, float *pre, hipStream_t s) {
+  if (ntotal == 0) return;
+  hipLaunchKernelGGL(entry_terms_kernel, dim3(ceil_div(ntotal, 256)), dim3(256), 0, s, list_off, nlist,
+                     codes, M, ptab, ntotal, pre);
+  MVG_LAUNCH_CHECK("entry_terms_kernel");
 }
 
 void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors, int64_t nrows,
@@ -1101,7 +1570,21 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     const bool deep = kk >= (a.deep_min > 0 ? a.deep_min : kDeepMinCandidates);
     WorkQueue wq{};
     const WorkQueue *wqp = nullptr;
-    if (deep) {
+    bool streamed = false;
+    if ((ip || (mixed && a.pre)) && (deep || a.wq_entries == 0)) {
+#define MVG_STREAM(MFV)                                                                       \
+  streamed = ip ? launch_stream_any<MFV, true>(a, maxp, probes, kk, deep, keys, w, s)         \
+                : launch_stream_any<MFV, false>(a, maxp, probes, kk, deep, keys, w, s);
+      switch (a.M) {
+        case 8: MVG_STREAM(8) break;
+        case 12: MVG_STREAM(12) break;
+        case 16: MVG_STREAM(16) break;
+        default: MVG_STREAM(0) break;
+      }
+#undef MVG_STREAM
+    }
+    if (streamed) {
+    } else if (deep) {
 #define MVG_DEEP(MFV)                                                                \
   if (ip) launch_deep<MFV, true, false>(a, maxp, probes, kk, keys, w, s);           \
   else if (mixed) launch_deep<MFV, false, true>(a, maxp, probes, kk, keys, w, s);   \
@@ -1143,7 +1626,7 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   if (ip) launch_query_scan<MFV, true, false>(a, maxp, probes, kk, keys, wqp, s);       \
   else if (mixed) launch_query_scan<MFV, false, true>(a, maxp, probes, kk, keys, wqp, s); \
   else launch_query_scan<MFV, false, false>(a, maxp, probes, kk, keys, wqp, s);
-    if (!deep) {
+    if (!deep && !streamed) {
       switch (a.M) {
         case 8: MVG_QSCAN(8) break;
         case 12: MVG_QSCAN(12) break;

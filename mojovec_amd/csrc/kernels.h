@@ -65,7 +65,11 @@ struct IvfSearchArgs {
   int64_t ntotal;  // rows of `vectors`
   int64_t deep_cap;  // sum of the nprobe largest lists: most entries one query visits
   int deep_min;      // k * k_factor from which the deep-pool scan runs (0: default)
+  const float *pre;  // mixed L2: per-entry sum_m T[l][m][code_m] [ntotal] (nullable)
 };
+// pre[e] = sum_m ptab[l][m][code_m] for every entry of the CSR lists.
+void precompute_entry_terms(const int64_t *list_off, int nlist, const uint8_t *codes, int M,
+                            const float *ptab, int64_t ntotal, float *pre, hipStream_t s);
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> for the mixed L2 scan.
 void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, int nlist,
                            float *ptab, hipStream_t s);

@@ -211,14 +211,18 @@ def test_ivfpq_add_and_search_match_oracle(gpu, metric, M, split):
         assert_same(Dg, Ig, Do, Io)
 
 
-@pytest.mark.parametrize("wq", [0, 300, 4096])
-def test_ivfpq_per_query_nprobe_exclusion_padding(gpu, wq):
+@pytest.mark.parametrize("wq,metric", [(0, 0), (300, 0), (4096, 0), (0, 1)])
+def test_ivfpq_per_query_nprobe_exclusion_padding(gpu, wq, metric):
     x = clustered(6, 4000, 32, nclusters=30)
-    oi, gi = _pair(gpu, x, 32, 64, 8, 0)
+    if metric == 1:
+        x = x / 255.0
+    oi, gi = _pair(gpu, x, 32, 64, 8, metric)
     gi.set_work_queue(wq)
     oi.add(x)
     gi.add(x)
     q = clustered(7, 40, 32, nclusters=30)
+    if metric == 1:
+        q = q / 255.0
     npq = np.random.default_rng(0).integers(1, 70, 40).astype(np.int32)
     ex = np.zeros(4000, np.uint8)
     ex[::7] = 1
