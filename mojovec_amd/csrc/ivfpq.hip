@@ -725,13 +725,32 @@ __device__ void deep_find_bin(const int *hist, int nb, int remaining, DeepShared
 
 // One MSD digit pass over the segment: histogram of the digit at `shift`
 // (nbits wide) of the valid keys whose bits under `mask` equal `prefix`.
+// The segment is 16-byte aligned; each thread keeps 4 x 4 keys in flight.
 __device__ void deep_key_pass(const uint32_t *__restrict__ seg, int64_t n, int shift, int nbits,
                               uint32_t prefix, uint32_t mask, int *hist) {
   const int tid = threadIdx.x, nb = 1 << nbits;
   const uint32_t dm = (uint32_t)nb - 1u;
   for (int b = tid; b < nb; b += kNT) hist[b] = 0;
   __syncthreads();
-  for (int64_t i = tid; i < n; i += kNT) {
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(seg);
+  const int64_t n4 = n >> 2;
+  constexpr int U = 4;
+  for (int64_t i0 = tid; i0 < n4; i0 += (int64_t)kNT * U) {
+    uint4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + (int64_t)u * kNT;
+      v[u] = i < n4 ? s4[i] : make_uint4(kSentinel, kSentinel, kSentinel, kSentinel);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t kv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (kv[c] != kSentinel && (kv[c] & mask) == prefix) atomicAdd(&hist[(kv[c] >> shift) & dm], 1);
+    }
+  }
+  for (int64_t i = (n4 << 2) + tid; i < n; i += kNT) {
     const uint32_t key = seg[i];
     if (key != kSentinel && (key & mask) == prefix) atomicAdd(&hist[(key >> shift) & dm], 1);
   }
@@ -900,13 +919,24 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
         const int64_t off = list_off[l];
         const int64_t L = min<int64_t>(list_off[l + 1] - off, seg_cap - base_i);
         if (L <= 0) continue;
-        for (int64_t pos = tid; pos < L; pos += kNT) {
-          const uint32_t key = seg[base_i + pos];
-          if (key == kSentinel || (!take_all && key > T)) continue;
-          const uint32_t id = (uint32_t)ids[off + pos];
-          if (take_all || key < T || id <= Tid) {
-            const int slot = atomicAdd(&ds.cnt, 1);
-            if (slot < kk) out[slot] = ((uint64_t)key << 32) | id;
+        constexpr int UC = 8;
+        for (int64_t p0 = tid; p0 < L; p0 += (int64_t)kNT * UC) {
+          uint32_t kv[UC];
+#pragma unroll
+          for (int u = 0; u < UC; u++) {
+            const int64_t pos = p0 + (int64_t)u * kNT;
+            kv[u] = pos < L ? seg[base_i + pos] : kSentinel;
+          }
+#pragma unroll
+          for (int u = 0; u < UC; u++) {
+            const uint32_t key = kv[u];
+            if (key == kSentinel || (!take_all && key > T)) continue;
+            const int64_t pos = p0 + (int64_t)u * kNT;
+            const uint32_t id = (uint32_t)ids[off + pos];
+            if (take_all || key < T || id <= Tid) {
+              const int slot = atomicAdd(&ds.cnt, 1);
+              if (slot < kk) out[slot] = ((uint64_t)key << 32) | id;
+            }
           }
         }
         base_i += L;
@@ -1208,11 +1238,36 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
       __syncthreads();
       {
         int c2 = 0;
-        for (int64_t j = tid; j < n; j += kNT) {
-          const int64_t e = stream_entry(poff, loff, j, c2);
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(seg);
+        const int64_t n4 = n >> 2;
+        constexpr int UC = 4;
+        for (int64_t i0 = tid; i0 < n4; i0 += (int64_t)kNT * UC) {
+          uint4 v[UC];
+#pragma unroll
+          for (int u = 0; u < UC; u++) {
+            const int64_t i = i0 + (int64_t)u * kNT;
+            v[u] = i < n4 ? s4[i] : make_uint4(kSentinel, kSentinel, kSentinel, kSentinel);
+          }
+#pragma unroll
+          for (int u = 0; u < UC; u++) {
+            const uint32_t kv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              const uint32_t key = kv[c];
+              if (key == kSentinel || (!take_all && key > T)) continue;
+              const int64_t j = ((i0 + (int64_t)u * kNT) << 2) + c;
+              const uint32_t id = (uint32_t)ids[stream_entry(poff, loff, j, c2)];
+              if (take_all || key < T || id <= Tid) {
+                const int slot = atomicAdd(&ds.cnt, 1);
+                if (slot < kk) cand[slot] = ((uint64_t)key << 32) | id;
+              }
+            }
+          }
+        }
+        for (int64_t j = (n4 << 2) + tid; j < n; j += kNT) {
           const uint32_t key = seg[j];
           if (key == kSentinel || (!take_all && key > T)) continue;
-          const uint32_t id = (uint32_t)ids[e];
+          const uint32_t id = (uint32_t)ids[stream_entry(poff, loff, j, c2)];
           if (take_all || key < T || id <= Tid) {
             const int slot = atomicAdd(&ds.cnt, 1);
             if (slot < kk) cand[slot] = ((uint64_t)key << 32) | id;
@@ -1390,7 +1445,7 @@ void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int
     MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   int per_cu = 0;
   MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
-  const int64_t cap = std::max<int64_t>(a.deep_cap, 1);
+  const int64_t cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);  // 16-byte aligned segments
   int64_t grid = std::min<int64_t>((int64_t)std::max(1, per_cu) * device_cus(), a.nq);
   // at most 4 GiB of key segments: fewer persistent workgroups beyond that
   grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(4) << 30) / (4 * cap)));
@@ -1423,7 +1478,7 @@ bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int 
   if (DEEP) {
     int per_cu = 0;
     MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
-    cap = std::max<int64_t>(a.deep_cap, 1);
+    cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);  // 16-byte aligned segments
     grid = std::min<int64_t>((int64_t)std::max(1, per_cu) * device_cus(), a.nq);
     grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(4) << 30) / (4 * cap)));
     w.deep_seg.reserve(sizeof(uint32_t) * (size_t)(grid * cap));
