@@ -988,17 +988,25 @@ struct StreamLayout {
   __host__ __device__ StreamLayout(int M, int d, int kk, int maxp, bool deep) {
     lut = 0;
     rv = lut + M * kKsub * 4;
-    poff = (rv + 4 * d + 15) & ~15;
+    int tables_end = (rv + 4 * d + 15) & ~15;
+    if (deep) {
+      // the winners (cand) reuse the LUT space once the scan is done
+      cand = 0;
+      const int out_end = 8 * next_pow2(kk);
+      if (out_end > tables_end) tables_end = out_end;
+    }
+    poff = tables_end;
     loff = (poff + 4 * (maxp + 1) + 15) & ~15;
     base = loff + 8 * maxp;
-    cand = (base + 4 * maxp + 15) & ~15;
-    if (deep) {  // cand holds the winners, hist the digit histogram
+    const int after = (base + 4 * maxp + 15) & ~15;
+    if (deep) {
       tmp = cand;
       scr = cand;
-      hist = cand + 8 * next_pow2(kk);
+      hist = after;
       st = hist + 4 * 2048;
       total = st + (int)sizeof(DeepShared);
     } else {
+      cand = after;
       tmp = cand + 8 * (kk + kCandCap);
       scr = tmp + 8 * next_pow2(kk);
       hist = scr;
