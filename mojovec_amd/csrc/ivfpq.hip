@@ -668,6 +668,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
 // query can visit), and takes queries from a device counter.
 constexpr int kDeepMinCandidates = 513;  // default: the deep form for k * k_factor >= this
 constexpr int kDeepBatch = 4;            // list entries per thread with loads in flight
+constexpr int kMfmaCoarseMinLists = 4096;  // mixed form: MFMA coarse above this nlist
 
 struct DeepShared {
   int sh[16];  // block_excl_scan / digit search scratch
@@ -1601,8 +1602,10 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   // 1. coarse top-maxp (IndexFlat.search over the centroids, :264).
   w.probes.reserve(sizeof(int32_t) * (size_t)a.nq * maxp);
   int32_t *probes = w.probes.as<int32_t>();
-  if (a.precision != 0 && a.coarse_bf16) {
-    // mixed: bf16 GEMM-form coarse distances on MFMA, FP32 accumulate
+  if (a.precision != 0 && a.coarse_bf16 && a.nlist > kMfmaCoarseMinLists) {
+    // mixed: bf16 GEMM-form coarse distances on MFMA, FP32 accumulate. Below
+    // kMfmaCoarseMinLists centroids the contraction is negligible and the
+    // exact FP32 kernel's per-chunk radix select is the faster path.
     const int rb = gemm_row_bytes(kGemmBf16L2, a.d);
     w.qbf.reserve((size_t)a.nq * rb);
     w.qnorm.reserve(sizeof(float) * a.nq);

@@ -339,6 +339,25 @@ def test_ivfpq_mixed_form_error_bound_and_rerank(gpu):
     gi.set_search_mode("plain")
 
 
+def test_ivfpq_mixed_mfma_coarse_large_nlist(gpu):
+    # above 4096 lists the mixed form takes its coarse top-nprobe from the
+    # bf16 GEMM-form distance on MFMA: the probe sets may differ from the FP32
+    # ones only at near-ties, so results stay within the mixed-form bound
+    x = clustered(40, 9000, 16, nclusters=60)
+    oi, gi = _pair(gpu, x, 16, 4200, 4, 0, niter=1)
+    oi.add(x)
+    gi.add(x)
+    q = clustered(41, 40, 16, nclusters=60)
+    gi.nprobe = 64
+    gi.set_search_mode("mixed")
+    Dm, Im = gi.search(q, 10)
+    Do, Io = oi.search(q, 10, 64)
+    overlap = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(Im.tolist(), Io.tolist())])
+    assert overlap > 0.9
+    assert np.allclose(np.sort(Dm, 1)[:, :5], np.sort(Do, 1)[:, :5], rtol=1e-3, atol=1.0)
+    gi.set_search_mode("plain")
+
+
 def test_ivfpq_untrained_and_empty(gpu):
     gi = gpu.IndexIVFPQ(8, 4, 2)
     D, I = gi.search(np.zeros((2, 8), np.float32), 3)
