@@ -35,7 +35,7 @@ RECALL_BAR = 0.95
 # PQ 16 x 8 ADC alone ranks this Gaussian-mixture data coarsely (recall@10
 # ~0.17 at nprobe 64): the 0.95 bar needs a deep exact-rerank pool, up to
 # k * k_factor = 4000 candidates (the device top-k limit is 4096).
-KF_SWEEP = (1, 4, 16, 64, 128, 256, 400)
+KF_SWEEP = (1, 4, 16, 64, 128, 256, 288, 320, 400)
 DEEP_MIN = 513  # k * k_factor from which the library runs the deep-pool scan kernel
 
 
