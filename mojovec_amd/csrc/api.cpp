@@ -684,6 +684,7 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.ntotal = h->ntotal;
   a.deep_cap = h->top_sum.empty() ? 0 : h->top_sum[(size_t)a.nprobe];
   a.deep_min = h->deep_min;
+  a.deep_pool = h->deep_pool;
   if (h->precision != 0) {
     if (h->tables_dirty) throw Error(mvg::kState, "mixed-precision tables are not prepared");
     if (h->metric == MVG_METRIC_L2) {
@@ -753,10 +754,11 @@ int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item) {
   API_END
 }
 
-int mvg_ivfpq_set_deep_select(mvg_ivfpq *h, int min_candidates) {
+int mvg_ivfpq_set_deep_select(mvg_ivfpq *h, int min_candidates, int pool) {
   API_BEGIN
-  require(h && min_candidates >= 0, "invalid arguments");
+  require(h && min_candidates >= 0 && (pool == 0 || pool == 1), "invalid arguments");
   h->deep_min = min_candidates;
+  h->deep_pool = pool;
   API_END
 }
 

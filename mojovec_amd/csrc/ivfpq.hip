@@ -984,9 +984,13 @@ __global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
 // filter (as ivfpq_query_scan_kernel).
 constexpr int kStreamMaxProbes = 1024;
 
+constexpr int kThreshAppend = 2048;  // appended keys between threshold flushes (mode 2)
+constexpr int kThreshJ = (kMaxCandidates + kThreshAppend + kNT - 1) / kNT;
+
 struct StreamLayout {
-  int lut, rv, poff, loff, base, cand, tmp, scr, hist, st, total;
-  __host__ __device__ StreamLayout(int M, int d, int kk, int maxp, bool deep) {
+  int lut, rv, poff, loff, base, cand, tmp, scr, hist, sh, st, total;
+  __host__ __device__ StreamLayout(int M, int d, int kk, int maxp, int mode) {
+    const bool deep = mode == 1;
     lut = 0;
     rv = lut + M * kKsub * 4;
     int tables_end = (rv + 4 * d + 15) & ~15;
@@ -1000,12 +1004,21 @@ struct StreamLayout {
     loff = (poff + 4 * (maxp + 1) + 15) & ~15;
     base = loff + 8 * maxp;
     const int after = (base + 4 * maxp + 15) & ~15;
+    sh = 0;
     if (deep) {
       tmp = cand;
       scr = cand;
       hist = after;
       st = hist + 4 * 2048;
       total = st + (int)sizeof(DeepShared);
+    } else if (mode == 2) {  // running candidate set with threshold flushes
+      cand = after;
+      tmp = cand;
+      scr = cand;
+      hist = cand + 8 * (kk + kThreshAppend);
+      sh = hist + 4 * 2048;
+      st = sh + 64;
+      total = st + (int)sizeof(QueryScanShared);
     } else {
       cand = after;
       tmp = cand + 8 * (kk + kCandCap);
@@ -1017,6 +1030,118 @@ struct StreamLayout {
   }
 };
 
+// Bin of a digit histogram hist[0..nb) (nb a multiple of kNT) holding the
+// remaining-th key: returns it with the count before it and its size.
+// sh: 16 ints of scratch (block_excl_scan uses 0..kNW, this 8..10).
+__device__ int find_bin_sh(const int *hist, int nb, int remaining, int *sh, int &before, int &count) {
+  const int tid = threadIdx.x;
+  const int per = nb / kNT;
+  int local = 0;
+  for (int b = 0; b < per; b++) local += hist[tid * per + b];
+  int total;
+  const int excl = block_excl_scan(local, sh, total);
+  if (excl < remaining && remaining <= excl + local) {
+    int c = excl;
+    for (int b = 0; b < per; b++) {
+      const int h = hist[tid * per + b];
+      if (c + h >= remaining) {
+        sh[8] = tid * per + b;
+        sh[9] = c;
+        sh[10] = h;
+        break;
+      }
+      c += h;
+    }
+  }
+  __syncthreads();
+  const int bin = sh[8];
+  before = sh[9];
+  count = sh[10];
+  __syncthreads();
+  return bin;
+}
+
+// Running candidate set for large k * k_factor: keeps the kk smallest of the
+// unique keys cand[0..n) (n <= kMaxCandidates + kThreshAppend, sentinels
+// allowed) in cand[0..kk), unordered and sentinel padded, and returns the
+// kk-th smallest key (kSentinel64 while fewer than kk are valid). Radix
+// select on the distance half (11/11/10-bit digits), then, when the kk-th
+// distance is shared across the cut, on the ids carrying it. No sort.
+__device__ uint64_t threshold_compact(uint64_t *cand, int n, int kk, int *hist, int *sh) {
+  const int tid = threadIdx.x;
+  uint64_t r[kThreshJ];
+  int mine = 0;
+#pragma unroll
+  for (int j = 0; j < kThreshJ; j++) {
+    const int i = tid + j * kNT;
+    r[j] = i < n ? cand[i] : kSentinel64;
+    mine += r[j] != kSentinel64;
+  }
+  const int valid = block_sum(mine, sh);
+  uint64_t thr = kSentinel64;
+  if (valid > kk) {
+    const int shifts[3] = {21, 10, 0};
+    const int nbits[3] = {11, 11, 10};
+    int remaining = kk, before = 0, count = 0;
+    uint32_t prefix = 0u, mask = 0u;
+#pragma unroll
+    for (int pass = 0; pass < 3; pass++) {
+      const int nb = 1 << nbits[pass];
+      const uint32_t dm = (uint32_t)nb - 1u;
+      for (int b = tid; b < nb; b += kNT) hist[b] = 0;
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kThreshJ; j++) {
+        const uint32_t hi = (uint32_t)(r[j] >> 32);
+        if (r[j] != kSentinel64 && (hi & mask) == prefix) atomicAdd(&hist[(hi >> shifts[pass]) & dm], 1);
+      }
+      __syncthreads();
+      const int bin = find_bin_sh(hist, nb, remaining, sh, before, count);
+      remaining -= before;
+      prefix |= (uint32_t)bin << shifts[pass];
+      mask |= dm << shifts[pass];
+    }
+    uint32_t tid_cut = 0xFFFFFFFFu;
+    if (remaining < count) {  // ties at the kk-th distance: lower ids win
+      uint32_t iprefix = 0u, imask = 0u;
+#pragma unroll
+      for (int pass = 0; pass < 3; pass++) {
+        const int nb = 1 << nbits[pass];
+        const uint32_t dm = (uint32_t)nb - 1u;
+        for (int b = tid; b < nb; b += kNT) hist[b] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kThreshJ; j++) {
+          const uint32_t hi = (uint32_t)(r[j] >> 32), lo = (uint32_t)r[j];
+          if (r[j] != kSentinel64 && hi == prefix && (lo & imask) == iprefix)
+            atomicAdd(&hist[(lo >> shifts[pass]) & dm], 1);
+        }
+        __syncthreads();
+        const int bin = find_bin_sh(hist, nb, remaining, sh, before, count);
+        remaining -= before;
+        iprefix |= (uint32_t)bin << shifts[pass];
+        imask |= dm << shifts[pass];
+      }
+      tid_cut = iprefix;
+    }
+    thr = ((uint64_t)prefix << 32) | tid_cut;
+  }
+  if (tid == 0) sh[11] = 0;
+  __syncthreads();  // every read of cand is in registers
+#pragma unroll
+  for (int j = 0; j < kThreshJ; j++) {
+    if (r[j] != kSentinel64 && r[j] <= thr) {
+      const int slot = atomicAdd(&sh[11], 1);
+      if (slot < kk) cand[slot] = r[j];
+    }
+  }
+  __syncthreads();
+  const int cnt = min(sh[11], kk);
+  for (int i = cnt + tid; i < kk; i += kNT) cand[i] = kSentinel64;
+  __syncthreads();
+  return thr;
+}
+
 // Entry of stream position j: advances the thread's probe cursor p (j only
 // grows per thread) and returns the entry's CSR index.
 __device__ __forceinline__ int64_t stream_entry(const int *poff, const int64_t *loff, int64_t j,
@@ -1025,7 +1150,7 @@ __device__ __forceinline__ int64_t stream_entry(const int *poff, const int64_t *
   return loff[p] + (j - poff[p]);
 }
 
-template <int MF, bool IP, bool DEEP>
+template <int MF, bool IP, int MODE>
 __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
     const float *__restrict__ q, int64_t nq, int d, int M, const float *__restrict__ coarse,
     const float *__restrict__ pq, const float *__restrict__ pre,
@@ -1036,7 +1161,11 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
     float *__restrict__ D, int64_t *__restrict__ I, float pad_dist, uint32_t *__restrict__ seg_base,
     int64_t seg_cap, unsigned *__restrict__ counter) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const StreamLayout lay(M, d, kk, maxp, DEEP);
+  // MODE 0: running top-kk, rank / radix merge per flush; 1: deep pool in
+  // HBM (DEEP); 2: running candidate set with threshold flushes (THRESH).
+  constexpr bool DEEP = MODE == 1;
+  constexpr bool THRESH = MODE == 2;
+  const StreamLayout lay(M, d, kk, maxp, MODE);
   float *lut = reinterpret_cast<float *>(smem + lay.lut);
   float *rv = reinterpret_cast<float *>(smem + lay.rv);
   int *poff = reinterpret_cast<int *>(smem + lay.poff);
@@ -1048,6 +1177,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
   int *hist = reinterpret_cast<int *>(smem + lay.hist);
   DeepShared &ds = *reinterpret_cast<DeepShared *>(smem + lay.st);
   QueryScanShared *st = reinterpret_cast<QueryScanShared *>(smem + lay.st);
+  int *tsh = reinterpret_cast<int *>(smem + lay.sh);
   const int tid = threadIdx.x, nlut = M * kKsub, dsub = d / M;
   constexpr bool kMixed = !IP;
   uint32_t *seg = DEEP ? seg_base + (int64_t)blockIdx.x * seg_cap : nullptr;
@@ -1109,7 +1239,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
         basev[p] = b;
       }
       int total;
-      const int ex = block_excl_scan(len, DEEP ? ds.sh : reinterpret_cast<int *>(scr->sh), total);
+      const int ex = block_excl_scan(len, DEEP ? ds.sh : (THRESH ? tsh : reinterpret_cast<int *>(scr->sh)), total);
       if (p < np) poff[p] = carry + ex;
       carry += total;
     }
@@ -1165,14 +1295,33 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
           if (key < thr) cand[kk + atomicAdd(&st->cnt, 1)] = key;
         }
       }
-      if constexpr (!DEEP) {
+      if constexpr (THRESH) {
+        __syncthreads();
+        if (st->cnt > kThreshAppend - kNT * U) {
+          const uint64_t t = threshold_compact(cand, kk + st->cnt, kk, hist, tsh);
+          if (tid == 0) {
+            st->cnt = 0;
+            st->thr = t;
+          }
+          __syncthreads();
+        }
+      } else if constexpr (!DEEP) {
         __syncthreads();
         if (st->cnt > kCandCap - kNT * U) flush_candidates(cand, tmp, kk, *st, *scr);
       }
     }
 
     if constexpr (!DEEP) {
-      if (st->cnt > 0) flush_candidates(cand, tmp, kk, *st, *scr);
+      if constexpr (THRESH) {
+        __syncthreads();
+        threshold_compact(cand, kk + st->cnt, kk, hist, tsh);
+        const int np2 = next_pow2(kk);  // cand holds kk + kThreshAppend >= np2 keys
+        for (int i = kk + tid; i < np2; i += kNT) cand[i] = kSentinel64;
+        __syncthreads();
+        bitonic_sort_u64(cand, np2);
+      } else {
+        if (st->cnt > 0) flush_candidates(cand, tmp, kk, *st, *scr);
+      }
       __syncthreads();
       if (out_keys) {
         for (int t = tid; t < kk; t += kNT) out_keys[qi * kk + t] = cand[t];
@@ -1472,13 +1621,14 @@ void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int
 
 // Stream scan (plain IP, mixed L2). Returns false when the shape does not
 // fit its LDS layout; the caller then takes the per-list kernels.
-template <int MF, bool IP, bool DEEP>
+template <int MF, bool IP, int MODE>
 bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                    uint64_t *out_keys, IvfWork &w, hipStream_t s) {
-  const StreamLayout lay(a.M, a.d, kk, maxp, DEEP);
+  constexpr bool DEEP = MODE == 1;
+  const StreamLayout lay(a.M, a.d, kk, maxp, MODE);
   const size_t sm = (size_t)lay.total;
   if (sm > 150 * 1024 || maxp > kStreamMaxProbes) return false;
-  const void *fn = (const void *)ivfpq_stream_scan_kernel<MF, IP, DEEP>;
+  const void *fn = (const void *)ivfpq_stream_scan_kernel<MF, IP, MODE>;
   if (sm > 64 * 1024)
     MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   int64_t grid = a.nq, cap = INT32_MAX;
@@ -1497,7 +1647,7 @@ bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int 
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, counter);
     MVG_LAUNCH_CHECK("reset_counter_kernel");
   }
-  hipLaunchKernelGGL((ivfpq_stream_scan_kernel<MF, IP, DEEP>), dim3((unsigned)grid), dim3(kNT), sm, s,
+  hipLaunchKernelGGL((ivfpq_stream_scan_kernel<MF, IP, MODE>), dim3((unsigned)grid), dim3(kNT), sm, s,
                      a.q, a.nq, a.d, a.M, a.coarse, a.pq, a.pre, a.list_off, a.codes, a.ids, probes,
                      maxp, a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D,
                      a.I, kIvfPadDist, seg, cap, counter);
@@ -1506,10 +1656,11 @@ bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int 
 }
 
 template <int MF, bool IP>
-bool launch_stream_any(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk, bool deep,
+bool launch_stream_any(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk, int mode,
                        uint64_t *out_keys, IvfWork &w, hipStream_t s) {
-  return deep ? launch_stream<MF, IP, true>(a, maxp, probes, kk, out_keys, w, s)
-              : launch_stream<MF, IP, false>(a, maxp, probes, kk, out_keys, w, s);
+  if (mode == 1) return launch_stream<MF, IP, 1>(a, maxp, probes, kk, out_keys, w, s);
+  if (mode == 2) return launch_stream<MF, IP, 2>(a, maxp, probes, kk, out_keys, w, s);
+  return launch_stream<MF, IP, 0>(a, maxp, probes, kk, out_keys, w, s);
 }
 
 template <int MF, bool IP, bool MIXED>
@@ -1638,9 +1789,11 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     const WorkQueue *wqp = nullptr;
     bool streamed = false;
     if ((ip || (mixed && a.pre)) && (deep || a.wq_entries == 0)) {
+      // large pools: deep pool in HBM or running set with threshold flushes
+      const int smode = !deep ? 0 : (a.deep_pool ? 1 : 2);
 #define MVG_STREAM(MFV)                                                                       \
-  streamed = ip ? launch_stream_any<MFV, true>(a, maxp, probes, kk, deep, keys, w, s)         \
-                : launch_stream_any<MFV, false>(a, maxp, probes, kk, deep, keys, w, s);
+  streamed = ip ? launch_stream_any<MFV, true>(a, maxp, probes, kk, smode, keys, w, s)        \
+                : launch_stream_any<MFV, false>(a, maxp, probes, kk, smode, keys, w, s);
       switch (a.M) {
         case 8: MVG_STREAM(8) break;
         case 12: MVG_STREAM(12) break;

@@ -66,6 +66,7 @@ struct IvfSearchArgs {
   int64_t deep_cap;  // sum of the nprobe largest lists: most entries one query visits
   int deep_min;      // k * k_factor from which the deep-pool scan runs (0: default)
   const float *pre;  // mixed L2: per-entry sum_m T[l][m][code_m] [ntotal] (nullable)
+  int deep_pool;     // stream scan, large pools: 1 keys in HBM, 0 running set in LDS
 };
 // pre[e] = sum_m ptab[l][m][code_m] for every entry of the CSR lists.
 void precompute_entry_terms(const int64_t *list_off, int nlist, const uint8_t *codes, int M,

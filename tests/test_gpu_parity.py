@@ -256,8 +256,9 @@ def test_ivfpq_long_lists_cross_scan_chunks(gpu, mode):
         assert_same(Dg, Ig, Do, Io)
 
 
-@pytest.mark.parametrize("d,M,metric", [(128, 16, 0), (64, 8, 0), (32, 8, 1), (48, 6, 0)])
-def test_ivfpq_deep_select_matches_oracle(gpu, d, M, metric):
+@pytest.mark.parametrize("pool", ["lds", "hbm"])
+@pytest.mark.parametrize("d,M,metric", [(128, 16, 0), (64, 8, 0), (32, 8, 1), (48, 6, 1)])
+def test_ivfpq_deep_select_matches_oracle(gpu, d, M, metric, pool):
     # deep-pool scan (keys in HBM, one radix select per query) on every
     # k * k_factor: same canonical top-k as the oracle
     x = clustered(20 + d, 6000, d, nclusters=40)
@@ -266,7 +267,7 @@ def test_ivfpq_deep_select_matches_oracle(gpu, d, M, metric):
     oi, gi = _pair(gpu, x, d, 32, M, metric)
     oi.add(x)
     gi.add(x)
-    gi.set_deep_select(1)
+    gi.set_deep_select(1, pool)
     q = clustered(21 + d, 30, d, nclusters=40)
     if metric == 1:
         q = q / 255.0
@@ -284,19 +285,25 @@ def test_ivfpq_deep_select_matches_oracle(gpu, d, M, metric):
     assert not np.isin(Ig[Ig >= 0], np.nonzero(ex)[0]).any()
 
 
-def test_ivfpq_deep_select_ties_break_by_lower_id(gpu):
+@pytest.mark.parametrize("metric", [0, 1])
+def test_ivfpq_deep_select_ties_break_by_lower_id(gpu, metric):
     # every row 12 times: runs of equal codes and equal ADC distances, so the
     # k-th key is shared by keys on both sides of the cut
     x = np.repeat(clustered(30, 400, 16, nclusters=8), 12, axis=0)
-    oi, gi = _pair(gpu, x, 16, 8, 4, 0)
+    if metric == 1:
+        x = x / 255.0
+    oi, gi = _pair(gpu, x, 16, 8, 4, metric)
     oi.add(x)
     gi.add(x)
     q = clustered(31, 20, 16, nclusters=8)
+    if metric == 1:
+        q = q / 255.0
     gi.nprobe = 8
-    for k in (515, 1001, 7):
+    for k in (515, 1001, 7, 2500):
         Do, Io = oi.search(q, k, 8)
-        for deep_min in (1, 100000):  # deep form, running top-k form
-            gi.set_deep_select(deep_min)
+        # deep form with either pool, and the running top-k form
+        for deep_min, pool in ((1, "lds"), (1, "hbm"), (100000, "lds")):
+            gi.set_deep_select(deep_min, pool)
             Dg, Ig = gi.search(q, k)
             assert_same(Dg, Ig, Do, Io)
 
@@ -308,12 +315,15 @@ def test_ivfpq_deep_select_mixed_equals_running_form(gpu):
     gi.set_search_mode("mixed")
     gi.nprobe = 12
     q = clustered(33, 40, 64, nclusters=40)
-    for k, kf in ((600, 1), (10, 80)):
-        gi.set_deep_select(1)
+    for k, kf in ((600, 1), (10, 80), (3000, 1)):
+        gi.set_deep_select(1, "lds")
         Dd, Id = gi.search(q, k, k_factor=kf)
+        gi.set_deep_select(1, "hbm")
+        Dh, Ih = gi.search(q, k, k_factor=kf)
         gi.set_deep_select(100000)
         Dr, Ir = gi.search(q, k, k_factor=kf)
         assert np.array_equal(Id, Ir) and np.array_equal(Dd, Dr)
+        assert np.array_equal(Ih, Ir) and np.array_equal(Dh, Dr)
     gi.set_search_mode("plain")
 
 

@@ -32,7 +32,7 @@ def test_server_results_match_alone(gpu):
     srv.close()
 
 
-@pytest.mark.parametrize("wq", [0, 1024, "deep"])
+@pytest.mark.parametrize("wq", [0, 1024, "deep", "deep-hbm"])
 def test_captured_search_graph_replays_match_eager(gpu, wq):
     import torch
 
@@ -41,8 +41,8 @@ def test_captured_search_graph_replays_match_eager(gpu, wq):
     ix = gpu.IndexIVFPQ(32, 64, 8)
     ix.train(x, niter_coarse=4, niter_pq=4)
     ix.add(x)
-    if wq == "deep":  # every k * k_factor takes the deep-pool scan
-        ix.set_deep_select(1)
+    if wq in ("deep", "deep-hbm"):  # every k * k_factor takes the deep-pool scan
+        ix.set_deep_select(1, "hbm" if wq == "deep-hbm" else "lds")
     else:
         ix.set_work_queue(wq)
     nq, k = 300, 10
