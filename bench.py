@@ -93,8 +93,8 @@ def main(argv=None):
     ap.add_argument("--fixed", default=None,
                     help="skip the sweep: 'plain:NPROBE:KF,mixed:NPROBE:KF' (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--deep-pool", default="lds", choices=["lds", "hbm"],
-                    help="large k * k_factor: running set in LDS (default) or keys in HBM")
+    ap.add_argument("--deep-pool", default="hbm", choices=["lds", "hbm"],
+                    help="large k * k_factor: keys in HBM (default) or a running set in LDS")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one 1 GiB device-to-device tensor copy: the reference "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")

@@ -157,11 +157,11 @@ int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan);
 int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item);
 /* Deep candidate pools (no reference counterpart): from k * k_factor >=
  * min_candidates (0 = default 513) the scan selects its candidates without
- * a sorted running top set. pool 0 (default): a running candidate set in
- * LDS, re-thresholded by radix select when its append buffer fills (inner
- * product and mixed forms; the plain L2 form uses pool 1); pool 1: every
- * visited entry's key is stored in HBM and selected once per query. Results
- * are identical in every case. */
+ * a sorted running top set. pool 0 (default): every visited entry's key is
+ * stored in HBM and the top k * k_factor selected once per query by radix
+ * select; pool 1: a running candidate set in LDS, re-thresholded by radix
+ * select when its append buffer fills (inner-product and mixed forms; the
+ * plain L2 form always uses pool 0). Results are identical in every case. */
 int mvg_ivfpq_set_deep_select(mvg_ivfpq *h, int min_candidates, int pool);
 /* Stage timing of the next searches (no reference counterpart; used by the
  * benchmark): when on, each search records HIP events on its stream around

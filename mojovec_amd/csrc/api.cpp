@@ -684,7 +684,7 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.ntotal = h->ntotal;
   a.deep_cap = h->top_sum.empty() ? 0 : h->top_sum[(size_t)a.nprobe];
   a.deep_min = h->deep_min;
-  a.deep_pool = h->deep_pool;
+  a.deep_pool = h->deep_pool == 0 ? 1 : 0;  // ABI pool 0: keys in HBM; 1: running set in LDS
   if (h->precision != 0) {
     if (h->tables_dirty) throw Error(mvg::kState, "mixed-precision tables are not prepared");
     if (h->metric == MVG_METRIC_L2) {

@@ -241,15 +241,15 @@ class IndexIVFPQ:
         """Persistent (query, entry-range) work queue for skewed lists; 0 off."""
         _lib.check(_lib.load().mvg_ivfpq_set_work_queue(self._h, int(entries_per_item)))
 
-    def set_deep_select(self, min_candidates: int = 0, pool: str = "lds") -> None:
+    def set_deep_select(self, min_candidates: int = 0, pool: str = "hbm") -> None:
         """k * k_factor from which the scan selects candidates without a
-        sorted running top set (0: default 513); pool "lds" re-thresholds a
-        running set in LDS, "hbm" stores every key in HBM and selects once per
-        query. Same results."""
+        sorted running top set (0: default 513); pool "hbm" stores every key
+        in HBM and selects once per query, "lds" re-thresholds a running set
+        in LDS. Same results."""
         if pool not in ("lds", "hbm"):
             raise ValueError('pool must be "lds" or "hbm"')
         _lib.check(_lib.load().mvg_ivfpq_set_deep_select(self._h, int(min_candidates),
-                                                         1 if pool == "hbm" else 0))
+                                                         0 if pool == "hbm" else 1))
 
     def train(self, x, niter_coarse: int = 15, niter_pq: int = 15, seed: int = KMEANS_SEED) -> None:
         x = _f32(x, self.d)
