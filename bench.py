@@ -98,9 +98,13 @@ def main(argv=None):
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with one 1 GiB device-to-device tensor copy: the reference "
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
-    ap.add_argument("--workload", default="ivfpq", choices=["ivfpq", "flat", "sq8", "batch"],
-                    help="ivfpq: the headline 1M x 128 IVF-PQ line; flat / sq8: the Flat "
-                         "100k x 128 and SQ8 1M x 768 side measurements")
+    ap.add_argument("--no-flat", action="store_true",
+                    help="skip the 1M x 128 exact FP32 Flat vs SQ8 (int8 MFMA) leg")
+    ap.add_argument("--workload", default="ivfpq",
+                    choices=["ivfpq", "flat", "sq8", "batch", "serve"],
+                    help="ivfpq: the headline 1M x 128 line; flat / sq8 / batch: the Flat "
+                         "100k x 128, SQ8 1M x 768 and skewed 10M x 96 side measurements; "
+                         "serve: continuous batching under Poisson arrivals")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures a single MI355X; --gpus {args.gpus} is not supported: "
@@ -113,6 +117,8 @@ def main(argv=None):
         return bench_sq8(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import torch
 
@@ -228,9 +234,10 @@ def main(argv=None):
 
     cpu = None if args.no_cpu else cpu_baseline(
         ix, x_h, q.cpu().numpy(), best["plain"]["nprobe"], best["plain"]["k_factor"], args.cpu_seconds)
+    flat = None if args.no_flat else flat_1m(mv, torch, x_h, q.cpu().numpy(), gt, args)
 
     line = {
-        "metric": "QPS@recall10>=0.95, 1M x 128 IVF-PQ: mixed vs FP32 GPU; %HBM roofline of the list scan",
+        "metric": "QPS@recall10\u22650.95, 1M\u00d7128 IVF-PQ/SQ8: mixed vs FP32 GPU; %HBM roofline",
         "value": round(qps, 1),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -253,6 +260,7 @@ def main(argv=None):
             "recall_bar": RECALL_BAR,
             "forms": forms,
             "mixed_over_plain": None if ratio is None else round(ratio, 3),
+            "flat_1Mx128": flat,
             "train_s": round(t_train, 3),
             "add_s": round(t_add, 3),
             "sweep": sweeps,
@@ -278,6 +286,125 @@ def main(argv=None):
         torch.cuda.synchronize()
         dst.copy_(src)  # reads 1 GiB, writes 1 GiB
         torch.cuda.synchronize()
+    return 0
+
+
+def flat_1m(mv, torch, x_h, q_h, gt, args):
+    """The same 1M x 128 base and 10k queries searched exhaustively: plain
+    FP32 Flat (reference operation order, VALU) against SQ8 with affine uint8
+    codes contracted on int8 MFMA. On this integer data in [0, 255] the SQ8
+    calibration is min 0, max 255, scale 1: the codes equal the vectors and
+    the int8 distance is exact. Host query/result buffers (PCIe included)
+    for both forms."""
+    out = {}
+    steps = max(2, args.steps // 5)
+    fx = mv.IndexFlat(D)
+    fx.add(x_h)
+    res = {}
+
+    def run_plain():
+        res["p"] = fx.search(q_h, K)
+
+    dt = _timed(torch, run_plain, steps, 1)
+    ids = torch.from_numpy(res["p"][1]).to(gt.device)
+    out["plain_fp32"] = {"qps": round(NQ / dt, 1), "ms": round(dt * 1e3, 3),
+                         "recall_at_10": round(recall_at(torch, gt, ids), 4),
+                         "achieved_tflops": round(2.0 * N * NQ * D / dt / 1e12, 2), "peak": 157.3}
+    del fx
+    sx = mv.IndexFlatSQ8(D, 0)
+    sx.add(x_h)
+    for kf in (1, 2):
+        def run_sq8():
+            res["s"] = sx.search(q_h, K, k_factor=kf)
+
+        dt = _timed(torch, run_sq8, steps, 1)
+        ids = torch.from_numpy(res["s"][1]).to(gt.device)
+        out[f"sq8_int8_mfma_kf{kf}"] = {"qps": round(NQ / dt, 1), "ms": round(dt * 1e3, 3),
+                                        "recall_at_10": round(recall_at(torch, gt, ids), 4),
+                                        "achieved_tops": round(2.0 * N * NQ * D / dt / 1e12, 2),
+                                        "peak": 5000.0}
+    del sx
+    base = out["plain_fp32"]["qps"]
+    out["sq8_over_plain"] = round(out["sq8_int8_mfma_kf1"]["qps"] / base, 3)
+    log(f"flat 1M x 128: {out}")
+    return out
+
+
+def bench_serve(args):
+    """Continuous batching on the 1M x 128 IVF-PQ index (mixed form):
+    Poisson query arrivals at 10k..500k queries/s, requests of log-uniform
+    size in [1, 4096] queries, k in {10, 100}, each at the nprobe / k_factor
+    that reaches recall@10 >= 0.95 (nprobe 64; k * k_factor ~2,900). Reports
+    per-request p50 / p99 latency (submit to results on the host) and the
+    sustained rate for each offered rate."""
+    import queue
+    import threading
+
+    import torch
+
+    import mojovec_amd as mv
+
+    dev = torch.device("cuda:0")
+    x, q, sample = make_data(torch, dev)
+    x_h, q_h = x.cpu().numpy(), q.cpu().numpy()
+    ix = mv.IndexIVFPQ(D, NLIST, M)
+    ix.train(sample.cpu().numpy(), niter_coarse=NITER_COARSE, niter_pq=NITER_PQ)
+    ix.add(x_h)
+    ix.set_search_mode("mixed")
+    gt = ground_truth(torch, x, q, K).cpu().numpy()
+    log("serve index ready")
+    settings = {10: (64, 288), 100: (64, 29)}
+    srv = mv.Server(ix, slots=4, max_batch=4096, mem_budget=32 << 30)
+    rng = np.random.default_rng(41)
+    out = {}
+    for rate in (10_000, 100_000, 300_000, 500_000):
+        duration = 2.0
+        reqs, t, pos = [], 0.0, 0
+        while t < duration:
+            size = int(min(4096, max(1, round(np.exp(rng.uniform(0.0, np.log(4096)))))))
+            k = int(rng.choice([10, 100]))
+            reqs.append((t, pos % NQ, size, k))
+            pos += size
+            t += rng.exponential(size / rate)
+        tickets = queue.Queue()
+        t0 = time.perf_counter()
+
+        def submitter():
+            for (at, start, size, k) in reqs:
+                wait = at - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+                idx = (np.arange(size) + start) % NQ
+                nprobe, kf = settings[k]
+                tickets.put((srv.submit(q_h[idx], k=k, nprobe=nprobe, k_factor=kf), idx))
+            tickets.put(None)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, hits, nqs = [], 0, 0
+        while True:
+            item = tickets.get()
+            if item is None:
+                break
+            ticket, idx = item
+            Dr, Ir, ms = srv.wait(ticket)
+            lat.append(ms)
+            hits += int((Ir[:, :10, None] == gt[idx][:, None, :]).any(2).sum())
+            nqs += len(idx)
+        t_end = time.perf_counter()
+        th.join()
+        lat = np.asarray(lat)
+        out[str(rate)] = {"requests": len(reqs), "queries": nqs,
+                          "sustained_qps": round(nqs / (t_end - t0), 1),
+                          "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                          "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                          "recall_at_10": round(hits / (10 * nqs), 4)}
+        log(f"serve offered {rate}/s: {out[str(rate)]}")
+    stats = srv.stats()
+    srv.close()
+    print(json.dumps({"workload": "continuous batching, IVF-PQ 1M x 128 (mixed form), Poisson "
+                                  "arrivals, request sizes log-uniform [1, 4096], k in {10, 100}",
+                      "rates": out, "server": stats}), flush=True)
     return 0
 
 
