@@ -778,9 +778,10 @@ int mvg_ivfpq_read_profile(mvg_ivfpq *h, float *stage_ms, int64_t *scanned_entri
   MVG_HIP(hipEventSynchronize(pf.ev[3]));
   for (int i = 0; i < 3; i++)
     if (stage_ms) MVG_HIP(hipEventElapsedTime(&stage_ms[i], pf.ev[i], pf.ev[i + 1]));
-  if (scanned_entries) {
+  if (scanned_entries) {  // the count is written on the search's stream after ev[3]
     unsigned long long v = 0;
-    MVG_HIP(hipMemcpy(&v, pf.counter.ptr, sizeof(v), hipMemcpyDeviceToHost));
+    MVG_HIP(hipMemcpyAsync(&v, pf.counter.ptr, sizeof(v), hipMemcpyDeviceToHost, pf.stream));
+    MVG_HIP(hipStreamSynchronize(pf.stream));
     *scanned_entries = (int64_t)v;
   }
   API_END

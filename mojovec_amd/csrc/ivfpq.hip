@@ -1901,6 +1901,7 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
                        maxp, a.nprobe_per_query, a.nprobe, a.list_off,
                        pf.counter.as<unsigned long long>());
     MVG_LAUNCH_CHECK("scanned_entries_kernel");
+    pf.stream = s;
     pf.recorded = true;
   }
 }

@@ -82,6 +82,7 @@ struct IvfProfile {
   bool recorded = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   DevBuf counter;
+  hipStream_t stream = nullptr;  // stream of the recorded search (the counter is written there)
 };
 struct IvfWork {
   DevBuf coarse_part, probes, part, keys, qbf, qnorm;
