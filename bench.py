@@ -267,7 +267,8 @@ def main(argv=None):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "ivfpq_deep_scan_kernel" if K * kf >= DEEP_MIN else "ivfpq_query_scan_kernel",
+            "kernel": ("ivfpq_stream_scan_kernel" if head == "mixed" else
+                       "ivfpq_deep_scan_kernel" if K * kf >= DEEP_MIN else "ivfpq_query_scan_kernel"),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
