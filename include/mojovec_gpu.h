@@ -41,6 +41,7 @@ extern "C" {
 typedef struct mvg_flat mvg_flat;
 typedef struct mvg_sq8 mvg_sq8;
 typedef struct mvg_ivfpq mvg_ivfpq;
+typedef struct mvg_ivfflat mvg_ivfflat;
 typedef struct mvg_collection mvg_collection;
 
 /* ---- library ---------------------------------------------------------- */
@@ -180,6 +181,46 @@ int mvg_ivfpq_shape(mvg_ivfpq *h, int *d, int *nlist, int *M, int *metric);
 int mvg_ivfpq_list_sizes(mvg_ivfpq *h, int64_t *sizes);
 int mvg_ivfpq_get_list(mvg_ivfpq *h, int list, int64_t *ids, uint8_t *codes);
 void mvg_ivfpq_destroy(mvg_ivfpq *h);
+
+/* ---- IVF-Flat: IndexIVFFlat (mojovec/index/index_ivf_flat.mojo:10-244) -- */
+/* IndexIVFFlat.__init__ (:29-52) with an IndexFlat quantizer of the same
+ * metric; nprobe starts at 1 (:46). */
+int mvg_ivfflat_create(int d, int nlist, int metric, mvg_ivfflat **out);
+/* IndexIVFFlat.train (:65-77): coarse k-means, which the reference fixes at
+ * 15 iterations (:73); explicit here. No-op once trained (:71). */
+int mvg_ivfflat_train(mvg_ivfflat *h, const float *x, int64_t n, int niter, uint64_t seed);
+/* Installs coarse centroids [nlist x d] and marks the index trained (the
+ * quantizer.add + is_trained of :75-77). */
+int mvg_ivfflat_set_centroids(mvg_ivfflat *h, const float *coarse);
+int mvg_ivfflat_get_centroids(mvg_ivfflat *h, float *coarse);
+/* IndexIVFFlat.add / add_with_ids (:79-137): ids NULL gives ntotal..
+ * ntotal+n-1; explicit ids must lie in [0, 2^31 - 1]. Rows go to their
+ * nearest centroid's list in input order; nothing happens while untrained
+ * (:102-104). */
+int mvg_ivfflat_add(mvg_ivfflat *h, const float *x, int64_t n, const int64_t *ids);
+/* The public `nprobe` field (:19); clamped to nlist at search time (:178).
+ * nprobe above 4096 is rejected at search time. */
+int mvg_ivfflat_set_nprobe(mvg_ivfflat *h, int nprobe);
+/* IndexIVFFlat.search (:139-244): L2 = l2_distance_simd[8], IP =
+ * inner_product_simd[4] (:222-225); ascending, ties by lower id; pad
+ * (1e38, -1); IP un-negated over found entries (:241-243). k <= 4096. The
+ * reference accepts a filter and ignores it (:149-156); exclude (nullable)
+ * here skips ids with exclude[id] != 0, as IVF-PQ does. */
+int mvg_ivfflat_search(mvg_ivfflat *h, const float *q, int64_t nq, int k, const uint8_t *exclude,
+                       int64_t exclude_len, float *distances, int64_t *labels);
+/* Same search with every pointer in device memory, enqueued on `stream`
+ * (NULL = the handle's stream) without synchronising. */
+int mvg_ivfflat_search_device(mvg_ivfflat *h, const float *q, int64_t nq, int k,
+                              const uint8_t *exclude, int64_t exclude_len, float *distances,
+                              int64_t *labels, void *stream);
+/* State (ntotal, is_trained, nprobe) and construction parameters. */
+int mvg_ivfflat_info(mvg_ivfflat *h, int64_t *ntotal, int *trained, int *nprobe);
+int mvg_ivfflat_shape(mvg_ivfflat *h, int *d, int *nlist, int *metric);
+/* Inverted-list sizes [nlist] and one list's ids / FP32 rows in append order
+ * (ArrayInvertedLists with code_size 4d, :52). */
+int mvg_ivfflat_list_sizes(mvg_ivfflat *h, int64_t *sizes);
+int mvg_ivfflat_get_list(mvg_ivfflat *h, int list, int64_t *ids, float *rows);
+void mvg_ivfflat_destroy(mvg_ivfflat *h);
 
 /* ---- captured search batch (no reference counterpart) ------------------ */
 /* Records one whole device-pointer search (coarse, work-queue build, scan,

@@ -12,6 +12,7 @@ from .index import (
     METRIC_L2,
     IndexFlat,
     IndexFlatSQ8,
+    IndexIVFFlat,
     IndexIVFPQ,
     KMeans,
     ProductQuantizer,
@@ -29,6 +30,7 @@ __all__ = [
     "QueryResult",
     "IndexFlat",
     "IndexFlatSQ8",
+    "IndexIVFFlat",
     "IndexIVFPQ",
     "KMeans",
     "ProductQuantizer",

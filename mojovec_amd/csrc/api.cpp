@@ -944,3 +944,238 @@ void mvg_graph_destroy(mvg_graph *g) {
 }
 
 }  // extern "C"
+
+// ---- IVF-Flat (mojovec/index/index_ivf_flat.mojo) ------------------------------
+
+namespace {
+
+// Device CSR of the host lists; each list is written in id order (stable).
+void ivfflat_sync(mvg_ivfflat *h) {
+  if (!h->dirty) return;
+  const int d = h->d;
+  std::vector<int64_t> off(h->nlist + 1, 0);
+  for (int l = 0; l < h->nlist; l++) off[l + 1] = off[l] + (int64_t)h->lids[l].size();
+  const int64_t total = off[h->nlist];
+  std::vector<int32_t> ids((size_t)std::max<int64_t>(total, 1));
+  std::vector<float> rows((size_t)std::max<int64_t>(total * d, 1));
+  std::vector<int32_t> order;
+  h->max_list = 0;
+  for (int l = 0; l < h->nlist; l++) {
+    const auto &li = h->lids[l];
+    const int64_t L = (int64_t)li.size();
+    h->max_list = std::max(h->max_list, L);
+    order.resize(L);
+    for (int64_t j = 0; j < L; j++) order[j] = (int32_t)j;
+    if (!std::is_sorted(li.begin(), li.end()))
+      std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return li[a] < li[b]; });
+    for (int64_t j = 0; j < L; j++) {
+      ids[off[l] + j] = li[order[j]];
+      std::copy(h->lrows[l].begin() + (size_t)order[j] * d, h->lrows[l].begin() + (size_t)(order[j] + 1) * d,
+                rows.begin() + (size_t)(off[l] + j) * d);
+    }
+  }
+  upload(h->list_off, off.data(), off.size(), h->s);
+  upload(h->ids, ids.data(), ids.size(), h->s);
+  upload(h->rows, rows.data(), rows.size(), h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  h->dirty = false;
+}
+
+void ivfflat_search_on(mvg_ivfflat *h, const float *q, int64_t nq, int k, const uint8_t *exclude,
+                       int64_t exclude_len, float *distances, int64_t *labels, hipStream_t s) {
+  mvg::IvfFlatArgs a{};
+  a.q = q;
+  a.nq = nq;
+  a.d = h->d;
+  a.nlist = h->nlist;
+  a.metric = h->metric;
+  a.coarse = h->coarse.as<float>();
+  a.list_off = h->list_off.as<int64_t>();
+  a.rows = h->rows.as<float>();
+  a.ids = h->ids.as<int32_t>();
+  a.ntotal = h->ntotal;
+  a.max_list = h->max_list;
+  a.k = k;
+  a.nprobe = h->nprobe;
+  a.excl = exclude_len > 0 ? exclude : nullptr;
+  a.excl_len = exclude_len;
+  a.D = distances;
+  a.I = labels;
+  mvg::ivfflat_search(a, h->work, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mvg_ivfflat_create(int d, int nlist, int metric, mvg_ivfflat **out) {
+  API_BEGIN
+  require(out != nullptr, "invalid arguments");
+  require(d > 0, "IVF-Flat dimension must be positive.");
+  require(nlist > 0 && nlist <= 1000000, "IVF-Flat nlist must be between 1 and 1000000.");
+  require(metric == MVG_METRIC_L2 || metric == MVG_METRIC_IP, "metric must be 0 (L2) or 1 (IP).");
+  auto h = std::make_unique<mvg_ivfflat>();
+  h->d = d;
+  h->nlist = nlist;
+  h->metric = metric;
+  h->lids.resize(nlist);
+  h->lrows.resize(nlist);
+  h->s = new_stream();
+  h->coarse.reserve(sizeof(float) * (size_t)nlist * d);
+  *out = h.release();
+  API_END
+}
+
+int mvg_ivfflat_train(mvg_ivfflat *h, const float *x, int64_t n, int niter, uint64_t seed) {
+  API_BEGIN
+  require(h && n >= 0 && niter >= 0, "invalid arguments");
+  if (h->trained || n == 0) return MVG_OK;  // index_ivf_flat.mojo:71
+  const int d = h->d;
+  float *dx = upload(h->tx, x, (size_t)n * d, h->s);
+  std::vector<int64_t> rows;  // KMeans(d, nlist, 15).train(x) (:73-76)
+  const int64_t off = kmeans_pick(n, h->nlist, seed, rows);
+  std::vector<float> c0((size_t)h->nlist * d);
+  for (int c = 0; c < h->nlist; c++) std::copy(x + rows[c] * d, x + rows[c] * d + d, c0.begin() + (size_t)c * d);
+  float *dc = h->coarse.as<float>();
+  MVG_HIP(hipMemcpyAsync(dc, c0.data(), sizeof(float) * c0.size(), hipMemcpyHostToDevice, h->s));
+  mvg::kmeans_fit(dx, n, 1, d, h->nlist, niter, off, dc, h->kw, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  h->trained = true;
+  API_END
+}
+
+int mvg_ivfflat_set_centroids(mvg_ivfflat *h, const float *coarse) {
+  API_BEGIN
+  require(h && coarse, "invalid arguments");
+  MVG_HIP(hipMemcpyAsync(h->coarse.ptr, coarse, sizeof(float) * (size_t)h->nlist * h->d,
+                         hipMemcpyHostToDevice, h->s));
+  MVG_HIP(hipStreamSynchronize(h->s));
+  h->trained = true;
+  API_END
+}
+
+int mvg_ivfflat_get_centroids(mvg_ivfflat *h, float *coarse) {
+  API_BEGIN
+  require(h && coarse, "invalid arguments");
+  download(coarse, h->coarse.ptr, (size_t)h->nlist * h->d, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_ivfflat_add(mvg_ivfflat *h, const float *x, int64_t n, const int64_t *ids) {
+  API_BEGIN
+  require(h && n >= 0, "invalid arguments");
+  if (!h->trained || n == 0) return MVG_OK;  // index_ivf_flat.mojo:102-104
+  if (ids) {
+    for (int64_t i = 0; i < n; i++)
+      require(ids[i] >= 0 && ids[i] <= INT32_MAX, "IVF-Flat ids must be in [0, 2^31 - 1]");
+  } else if (h->ntotal + n - 1 > INT32_MAX) {
+    throw Error(mvg::kInvalid, "IVF-Flat sequential ids must stay below 2^31");
+  }
+  const int d = h->d;
+  const float *dx = upload(h->tx, x, (size_t)n * d, h->s);
+  h->tlab.reserve(sizeof(int32_t) * n);
+  int32_t *dl = h->tlab.as<int32_t>();
+  // quantizer.search(x, 1) (:110-116)
+  mvg::flat_argmin(h->coarse.as<float>(), h->nlist, d, h->metric, false, dx, n, dl, nullptr, h->s);
+  std::vector<int32_t> lab(n);
+  download(lab.data(), dl, (size_t)n, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  for (int64_t i = 0; i < n; i++) {  // :123-135, append order
+    const int l = lab[i];
+    if (l < 0 || l >= h->nlist) continue;
+    h->lids[l].push_back((int32_t)(ids ? ids[i] : h->ntotal + i));
+    h->lrows[l].insert(h->lrows[l].end(), x + i * d, x + (i + 1) * d);
+  }
+  h->ntotal += n;  // :137
+  h->dirty = true;
+  API_END
+}
+
+int mvg_ivfflat_set_nprobe(mvg_ivfflat *h, int nprobe) {
+  API_BEGIN
+  require(h && nprobe >= 1, "nprobe must be at least 1");
+  h->nprobe = nprobe;
+  API_END
+}
+
+int mvg_ivfflat_search(mvg_ivfflat *h, const float *q, int64_t nq, int k, const uint8_t *exclude,
+                       int64_t exclude_len, float *distances, int64_t *labels) {
+  API_BEGIN
+  require(h && nq >= 0 && k >= 1, "invalid arguments");
+  if (nq == 0) return MVG_OK;
+  if (!h->trained || h->ntotal == 0) {  // index_ivf_flat.mojo:171-175
+    for (int64_t i = 0; i < nq * k; i++) {
+      distances[i] = mvg::kIvfPadDist;
+      labels[i] = -1;
+    }
+    return MVG_OK;
+  }
+  std::lock_guard<std::mutex> lk(h->search_mu);
+  ivfflat_sync(h);
+  const float *dq = upload(h->q, q, (size_t)nq * h->d, h->s);
+  const uint8_t *dx = exclude && exclude_len > 0 ? upload(h->excl, exclude, (size_t)exclude_len, h->s) : nullptr;
+  h->D.reserve(sizeof(float) * nq * k);
+  h->I.reserve(sizeof(int64_t) * nq * k);
+  ivfflat_search_on(h, dq, nq, k, dx, dx ? exclude_len : 0, h->D.as<float>(), h->I.as<int64_t>(), h->s);
+  download(distances, h->D.ptr, (size_t)nq * k, h->s);
+  download(labels, h->I.ptr, (size_t)nq * k, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_ivfflat_search_device(mvg_ivfflat *h, const float *q, int64_t nq, int k,
+                              const uint8_t *exclude, int64_t exclude_len, float *distances,
+                              int64_t *labels, void *stream) {
+  API_BEGIN
+  require(h && nq >= 0 && k >= 1, "invalid arguments");
+  require(h->trained && h->ntotal > 0, "search_device needs a trained, non-empty index");
+  hipStream_t s = stream ? (hipStream_t)stream : h->s;
+  std::lock_guard<std::mutex> lk(h->search_mu);
+  ivfflat_sync(h);
+  ivfflat_search_on(h, q, nq, k, exclude, exclude_len, distances, labels, s);
+  API_END
+}
+
+int mvg_ivfflat_info(mvg_ivfflat *h, int64_t *ntotal, int *trained, int *nprobe) {
+  API_BEGIN
+  require(h, "invalid handle");
+  if (ntotal) *ntotal = h->ntotal;
+  if (trained) *trained = h->trained ? 1 : 0;
+  if (nprobe) *nprobe = h->nprobe;
+  API_END
+}
+
+int mvg_ivfflat_shape(mvg_ivfflat *h, int *d, int *nlist, int *metric) {
+  API_BEGIN
+  require(h, "invalid handle");
+  if (d) *d = h->d;
+  if (nlist) *nlist = h->nlist;
+  if (metric) *metric = h->metric;
+  API_END
+}
+
+int mvg_ivfflat_list_sizes(mvg_ivfflat *h, int64_t *sizes) {
+  API_BEGIN
+  require(h && sizes, "invalid arguments");
+  for (int l = 0; l < h->nlist; l++) sizes[l] = (int64_t)h->lids[l].size();
+  API_END
+}
+
+int mvg_ivfflat_get_list(mvg_ivfflat *h, int list, int64_t *ids, float *rows) {
+  API_BEGIN
+  require(h && list >= 0 && list < h->nlist, "list index out of range");
+  const auto &li = h->lids[list];
+  if (ids)
+    for (size_t i = 0; i < li.size(); i++) ids[i] = li[i];
+  if (rows) std::copy(h->lrows[list].begin(), h->lrows[list].end(), rows);
+  API_END
+}
+
+void mvg_ivfflat_destroy(mvg_ivfflat *h) {
+  if (!h) return;
+  if (h->s) (void)hipStreamDestroy(h->s);
+  delete h;
+}
+
+}  // extern "C"

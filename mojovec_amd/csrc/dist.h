@@ -270,6 +270,42 @@ __device__ __forceinline__ float ip_w8_fixed(const float *xr, const float *q) {
   return res;
 }
 
+// Compile-time-dimension form of inner_product_simd[4] (the IVF-Flat IP
+// distance, index_ivf_flat.mojo:225) for D >= 4: four 4-lane accumulators
+// over 16-element blocks, 4-element blocks, the 4-lane tree, a scalar tail.
+template <int D>
+__device__ __forceinline__ float ip_w4_fixed(const float *xr, const float *q) {
+  float p0[4], p1[4], p2[4], p3[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) p0[j] = p1[j] = p2[j] = p3[j] = 0.0f;
+  constexpr int NB4 = D / 16;
+#pragma unroll
+  for (int b = 0; b < NB4; b++) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int i = b * 16;
+      p0[j] = fma_(q[i + j], xr[i + j], p0[j]);
+      p1[j] = fma_(q[i + 4 + j], xr[i + 4 + j], p1[j]);
+      p2[j] = fma_(q[i + 8 + j], xr[i + 8 + j], p2[j]);
+      p3[j] = fma_(q[i + 12 + j], xr[i + 12 + j], p3[j]);
+    }
+  }
+  float v[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) v[j] = ((p0[j] + p1[j]) + p2[j]) + p3[j];
+  constexpr int I1 = NB4 * 16;
+  constexpr int NW4 = (D - I1) / 4;
+#pragma unroll
+  for (int w = 0; w < NW4; w++) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = fma_(q[I1 + w * 4 + j], xr[I1 + w * 4 + j], v[j]);
+  }
+  float res = tree_sum<4>(v);
+#pragma unroll
+  for (int i = I1 + NW4 * 4; i < D; i++) res += q[i] * xr[i];
+  return res;
+}
+
 // ---------------------------------------------------------------------------
 // Wavefront forms of the W = 64 routines used by the exact rerank
 // (index_hnsw.mojo:456-465): lane j owns SIMD lane j; the halving tree is a

@@ -26,6 +26,22 @@ struct mvg_ivfpq {
   std::mutex search_mu;  // serialises searches that share `work`
 };
 
+// IVF-Flat handle: host lists in append order (what list accessors and
+// snapshots see) and a device CSR copy with each list in id order.
+struct mvg_ivfflat {
+  int d = 0, nlist = 0, metric = 0, nprobe = 1;
+  bool trained = false, dirty = true;
+  int64_t ntotal = 0, max_list = 0;
+  std::vector<std::vector<int32_t>> lids;
+  std::vector<std::vector<float>> lrows;
+  mvg::DevBuf coarse, list_off, rows, ids;
+  mvg::DevBuf tx, tlab, q, D, I, excl;
+  mvg::IvfFlatWork work;
+  mvg::KmeansWork kw;
+  hipStream_t s = nullptr;
+  std::mutex search_mu;  // serialises searches that share `work`
+};
+
 namespace mvg {
 
 // Brings the device CSR lists and the mixed-form tables up to date (host

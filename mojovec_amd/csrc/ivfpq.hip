@@ -1534,16 +1534,6 @@ void launch_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk
   MVG_LAUNCH_CHECK("ivfpq_scan_kernel");
 }
 
-int device_cus() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
-    MVG_HIP(hipGetDevice(&dev));
-    MVG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  }
-  return cus;
-}
-
 template <int MF, bool IP, bool MIXED, int DF, bool WQ>
 void launch_query_scan_wq(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                           uint64_t *out_keys, const WorkQueue &wq, hipStream_t s) {
@@ -1680,6 +1670,17 @@ void launch_deep(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk
 }
 
 }  // namespace
+
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    MVG_HIP(hipGetDevice(&dev));
+    MVG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  return cus;
+}
+
 
 void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, int nlist,
                            float *ptab, hipStream_t s) {

@@ -41,6 +41,8 @@ void residuals(const float *x, int64_t n, int d, const float *coarse, const int3
                float *out, hipStream_t s);
 
 // ---- ivfpq.hip ----
+// Compute units of the current device (cached after the first call).
+int device_cus();
 struct IvfSearchArgs {
   const float *q;
   int64_t nq;
@@ -98,6 +100,29 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
 void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors, int64_t nrows,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
             hipStream_t s);
+
+// ---- ivfflat.hip ----
+struct IvfFlatArgs {
+  const float *q;
+  int64_t nq;
+  int d, nlist, metric;
+  const float *coarse;      // [nlist][d]
+  const int64_t *list_off;  // [nlist + 1], CSR over `rows`
+  const float *rows;        // [ntotal][d], each list in id order
+  const int32_t *ids;       // [ntotal]
+  int64_t ntotal, max_list;  // rows, longest list
+  int k, nprobe;
+  const uint8_t *excl;
+  int64_t excl_len;
+  float *D;
+  int64_t *I;
+};
+struct IvfFlatWork {
+  DevBuf coarse_part, probes, cnt, cursor, pair_off, tile_off, range_ctr, pairs, part;
+};
+// IndexIVFFlat.search (index_ivf_flat.mojo:139-244) for a trained, non-empty
+// index; every pointer is device memory. Pads (1e38, -1).
+void ivfflat_search(const IvfFlatArgs &a, IvfFlatWork &w, hipStream_t s);
 
 // ---- gemm.hip ----
 // Kinds of the MFMA GEMM-form distance: 0 bf16 L2, 1 bf16 IP, 2 int8 IP

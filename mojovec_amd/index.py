@@ -342,6 +342,100 @@ class IndexIVFPQ:
             ctypes.c_void_p(stream or None)))
 
 
+class IndexIVFFlat:
+    """IVF index over exact FP32 rows with an HBM-resident CSR list layout
+    (IndexIVFFlat, index_ivf_flat.mojo:10-244)."""
+
+    def __init__(self, d: int, nlist: int, metric: int = METRIC_L2, _handle=None):
+        L = _lib.load()
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _lib.check(L.mvg_ivfflat_create(d, nlist, metric, ctypes.byref(h)))
+        else:
+            h = ctypes.c_void_p(_handle)
+        self._h, self.d, self.nlist, self.metric_type = h, d, nlist, metric
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.load().mvg_ivfflat_destroy(self._h)
+            self._h = None
+
+    def _info(self):
+        n, t, p = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().mvg_ivfflat_info(self._h, ctypes.byref(n), ctypes.byref(t), ctypes.byref(p)))
+        return n.value, bool(t.value), p.value
+
+    @property
+    def ntotal(self) -> int:
+        return self._info()[0]
+
+    @property
+    def is_trained(self) -> bool:
+        return self._info()[1]
+
+    @property
+    def nprobe(self) -> int:
+        return self._info()[2]
+
+    @nprobe.setter
+    def nprobe(self, v: int) -> None:
+        _lib.check(_lib.load().mvg_ivfflat_set_nprobe(self._h, int(v)))
+
+    def train(self, x, niter: int = 15, seed: int = KMEANS_SEED) -> None:
+        """KMeans(d, nlist, 15).train over x (index_ivf_flat.mojo:65-77)."""
+        x = _f32(x, self.d)
+        _lib.check(_lib.load().mvg_ivfflat_train(self._h, _ptr(x), x.shape[0], niter, seed))
+
+    def set_centroids(self, coarse) -> None:
+        c = _f32(coarse, self.d)
+        _lib.check(_lib.load().mvg_ivfflat_set_centroids(self._h, _ptr(c)))
+
+    def centroids(self):
+        c = np.empty((self.nlist, self.d), np.float32)
+        _lib.check(_lib.load().mvg_ivfflat_get_centroids(self._h, _ptr(c)))
+        return c
+
+    def add(self, x) -> None:
+        x = _f32(x, self.d)
+        _lib.check(_lib.load().mvg_ivfflat_add(self._h, _ptr(x), x.shape[0], None))
+
+    def add_with_ids(self, x, ids) -> None:
+        x = _f32(x, self.d)
+        ids = np.ascontiguousarray(ids, np.int64)
+        if ids.shape != (x.shape[0],):
+            raise ValueError("one id per vector is required")
+        _lib.check(_lib.load().mvg_ivfflat_add(self._h, _ptr(x), x.shape[0], _ptr(ids)))
+
+    def list_sizes(self):
+        s = np.empty(self.nlist, np.int64)
+        _lib.check(_lib.load().mvg_ivfflat_list_sizes(self._h, _ptr(s)))
+        return s
+
+    def get_list(self, l: int):
+        n = int(self.list_sizes()[l])
+        ids = np.empty(n, np.int64)
+        rows = np.empty((n, self.d), np.float32)
+        _lib.check(_lib.load().mvg_ivfflat_get_list(self._h, l, _ptr(ids), _ptr(rows)))
+        return ids, rows
+
+    def search(self, x, k: int, exclude=None):
+        x = _f32(x, self.d)
+        nq = x.shape[0]
+        D = np.empty((nq, k), np.float32)
+        I = np.empty((nq, k), np.int64)
+        m, ml = _mask(exclude)
+        _lib.check(_lib.load().mvg_ivfflat_search(self._h, _ptr(x), nq, k, _ptr(m), ml, _ptr(D),
+                                                   _ptr(I)))
+        return D, I
+
+    def search_device(self, q_ptr: int, nq: int, k: int, D_ptr: int, I_ptr: int,
+                      stream: int = 0) -> None:
+        """Search with every buffer already in device memory (raw pointers)."""
+        _lib.check(_lib.load().mvg_ivfflat_search_device(
+            self._h, ctypes.c_void_p(q_ptr), nq, k, None, 0, ctypes.c_void_p(D_ptr),
+            ctypes.c_void_p(I_ptr), ctypes.c_void_p(stream or None)))
+
+
 class SearchGraph:
     """A captured IVF-PQ search batch (mvg_ivfpq_capture); replay with launch()."""
 

@@ -775,23 +775,28 @@ void mvref_ivfpq_get_codebooks(const mvref_ivfpq *h, float *coarse, float *pq) {
   if (pq) memcpy(pq, h->pq, sizeof(float) * (size_t)h->d * PQ_KSUB);
 }
 
-/* Coarse assignment = quantizer.search(x, 1) (index_ivf_pq.mojo:115,172). */
-static void ivf_assign(const mvref_ivfpq *h, const float *x, int64_t n,
-                       int32_t *labels, int nthreads) {
+/* Coarse assignment = quantizer.search(x, 1) (index_ivf_pq.mojo:115,172;
+ * index_ivf_flat.mojo:115): the canonical nearest centroid. */
+static void coarse_assign(const float *coarse, int nlist, int d, int metric,
+                          const float *x, int64_t n, int32_t *labels, int nthreads) {
   int nt = nthreads_or_default(nthreads);
-  int d = h->d;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt)
   for (int64_t i = 0; i < n; i++) {
     float best = 0.0f;
     int bl = -1;
-    for (int l = 0; l < h->nlist; l++) {
-      const float *c = h->coarse + (int64_t)l * d;
-      float dd = h->metric == MVREF_L2 ? mvref_l2_simd(x + i * d, c, d, 8)
-                                       : -mvref_ip_simd(x + i * d, c, d, 8);
+    for (int l = 0; l < nlist; l++) {
+      const float *c = coarse + (int64_t)l * d;
+      float dd = metric == MVREF_L2 ? mvref_l2_simd(x + i * d, c, d, 8)
+                                    : -mvref_ip_simd(x + i * d, c, d, 8);
       if (bl < 0 || dd < best) { best = dd; bl = l; }
     }
     labels[i] = bl;
   }
+}
+
+static void ivf_assign(const mvref_ivfpq *h, const float *x, int64_t n,
+                       int32_t *labels, int nthreads) {
+  coarse_assign(h->coarse, h->nlist, h->d, h->metric, x, n, labels, nthreads);
 }
 
 /* train: index_ivf_pq.mojo:87-136 (coarse k-means, residuals, PQ). */
@@ -976,5 +981,170 @@ void mvref_ivfpq_search(const mvref_ivfpq *h, const float *q, int64_t nq,
       }
     }
     free(pbuf); free(buf); free(buf2); free(lut); free(r);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* IndexIVFFlat: mojovec/index/index_ivf_flat.mojo. Lists hold the raw FP32
+ * rows (code_size = 4d, :52) and their ids in append order. */
+
+struct mvref_ivfflat {
+  int d, nlist, metric, trained;
+  float *coarse; /* nlist x d */
+  int64_t ntotal;
+  int64_t *lsize, *lcap;
+  int64_t **lids;
+  float **lrows;
+};
+
+mvref_ivfflat *mvref_ivfflat_create(int d, int nlist, int metric) {
+  mvref_ivfflat *h = (mvref_ivfflat *)calloc(1, sizeof(mvref_ivfflat));
+  h->d = d; h->nlist = nlist; h->metric = metric;
+  h->coarse = (float *)calloc((size_t)nlist * d, sizeof(float));
+  h->lsize = (int64_t *)calloc((size_t)nlist, sizeof(int64_t));
+  h->lcap = (int64_t *)calloc((size_t)nlist, sizeof(int64_t));
+  h->lids = (int64_t **)calloc((size_t)nlist, sizeof(int64_t *));
+  h->lrows = (float **)calloc((size_t)nlist, sizeof(float *));
+  return h;
+}
+
+void mvref_ivfflat_destroy(mvref_ivfflat *h) {
+  if (!h) return;
+  for (int l = 0; l < h->nlist; l++) { free(h->lids[l]); free(h->lrows[l]); }
+  free(h->lids); free(h->lrows); free(h->lsize); free(h->lcap); free(h->coarse); free(h);
+}
+
+/* quantizer.add(centroids) + is_trained = True (:75-77). */
+void mvref_ivfflat_set_centroids(mvref_ivfflat *h, const float *coarse) {
+  memcpy(h->coarse, coarse, sizeof(float) * (size_t)h->nlist * h->d);
+  h->trained = 1;
+}
+
+void mvref_ivfflat_get_centroids(const mvref_ivfflat *h, float *coarse) {
+  memcpy(coarse, h->coarse, sizeof(float) * (size_t)h->nlist * h->d);
+}
+
+/* train: KMeans(d, nlist, 15).train(x) (:65-77). */
+void mvref_ivfflat_train(mvref_ivfflat *h, const float *x, int64_t n, int niter,
+                         uint64_t seed, int nthreads) {
+  if (h->trained) return; /* :71 */
+  int d = h->d;
+  float *init = (float *)malloc(sizeof(float) * (size_t)h->nlist * d);
+  int64_t off = mvref_kmeans_init(x, n, d, h->nlist, seed, init);
+  mvref_kmeans_train(x, n, d, h->nlist, niter, init, off, 1, h->coarse, NULL, nthreads);
+  free(init);
+  h->trained = 1;
+}
+
+/* add / add_with_ids (:79-137): nearest centroid, one append per row in
+ * input order (ArrayInvertedLists capacity: 16, then doubling); ids default
+ * to ntotal + i; rows of an untrained index are dropped silently (:102-104). */
+void mvref_ivfflat_add(mvref_ivfflat *h, const float *x, int64_t n, const int64_t *ids,
+                       int nthreads) {
+  if (!h->trained || n == 0) return;
+  int d = h->d;
+  int32_t *lab = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+  coarse_assign(h->coarse, h->nlist, d, h->metric, x, n, lab, nthreads);
+  for (int64_t i = 0; i < n; i++) {
+    int l = lab[i];
+    if (l < 0 || l >= h->nlist) continue; /* :125 */
+    if (h->lsize[l] == h->lcap[l]) {
+      int64_t nc = h->lcap[l] ? h->lcap[l] * 2 : 16;
+      h->lids[l] = (int64_t *)realloc(h->lids[l], sizeof(int64_t) * (size_t)nc);
+      h->lrows[l] = (float *)realloc(h->lrows[l], sizeof(float) * (size_t)nc * d);
+      h->lcap[l] = nc;
+    }
+    h->lids[l][h->lsize[l]] = ids ? ids[i] : h->ntotal + i;
+    memcpy(h->lrows[l] + h->lsize[l] * d, x + i * d, sizeof(float) * (size_t)d);
+    h->lsize[l]++;
+  }
+  h->ntotal += n; /* :137 */
+  free(lab);
+}
+
+/* Installs lists built elsewhere (CSR: list_off[nlist+1], int32 ids and FP32
+ * rows in list order), so the CPU baseline times the search alone. */
+void mvref_ivfflat_load_lists(mvref_ivfflat *h, const int64_t *list_off, const int32_t *ids,
+                              const float *rows, int64_t ntotal) {
+  int d = h->d;
+  for (int l = 0; l < h->nlist; l++) {
+    int64_t L = list_off[l + 1] - list_off[l];
+    free(h->lids[l]); free(h->lrows[l]);
+    h->lids[l] = (int64_t *)malloc(sizeof(int64_t) * (size_t)(L > 0 ? L : 1));
+    h->lrows[l] = (float *)malloc(sizeof(float) * (size_t)(L > 0 ? L : 1) * d);
+    for (int64_t j = 0; j < L; j++) h->lids[l][j] = ids[list_off[l] + j];
+    memcpy(h->lrows[l], rows + list_off[l] * d, sizeof(float) * (size_t)L * d);
+    h->lsize[l] = h->lcap[l] = L;
+  }
+  h->ntotal = ntotal;
+  h->trained = 1;
+}
+
+int64_t mvref_ivfflat_list_size(const mvref_ivfflat *h, int list) { return h->lsize[list]; }
+
+void mvref_ivfflat_list(const mvref_ivfflat *h, int list, int64_t *ids, float *rows) {
+  memcpy(ids, h->lids[list], sizeof(int64_t) * (size_t)h->lsize[list]);
+  memcpy(rows, h->lrows[list], sizeof(float) * (size_t)h->lsize[list] * h->d);
+}
+
+/* search (:139-244). Untrained or empty: (1e38, -1) everywhere (:171-175).
+ * nprobe clamped to nlist (:177-178); coarse = IndexFlat top-nprobe (:189);
+ * per probed list every row is scored with l2_distance_simd[8] or
+ * -inner_product_simd[4] (:222-225) into a top-k kept in the canonical
+ * (distance, lower id) order; the tail stays (1e38, -1) (:198-202) and IP
+ * distances are un-negated over the found entries (:241-243). The reference
+ * accepts a filter and ignores it; here a non-null `exclude` drops ids with
+ * exclude[id] != 0, as the IVF-PQ search does (index_ivf_pq.mojo:328-334). */
+void mvref_ivfflat_search(const mvref_ivfflat *h, const float *q, int64_t nq, int k,
+                          int nprobe, const uint8_t *exclude, int64_t exclude_len,
+                          float *D, int64_t *I, int nthreads) {
+  int d = h->d, nlist = h->nlist, ip = h->metric != MVREF_L2;
+  if (!h->trained || h->ntotal == 0) {
+    for (int64_t i = 0; i < nq * k; i++) { D[i] = IVF_PAD_DIST; I[i] = -1; }
+    return;
+  }
+  int np = nprobe > nlist ? nlist : nprobe;
+  if (np < 1) np = 1;
+  int nt = nthreads_or_default(nthreads);
+#pragma omp parallel num_threads(nt)
+  {
+    cand_t *pbuf = (cand_t *)malloc(sizeof(cand_t) * (size_t)np);
+    cand_t *buf = (cand_t *)malloc(sizeof(cand_t) * (size_t)k);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t qi = 0; qi < nq; qi++) {
+      const float *qp = q + qi * d;
+      cheap_t pc;
+      ch_init(&pc, pbuf, np);
+      for (int l = 0; l < nlist; l++) {
+        const float *c = h->coarse + (int64_t)l * d;
+        ch_offer(&pc, ip ? -mvref_ip_simd(qp, c, d, 8) : mvref_l2_simd(qp, c, d, 8), l);
+      }
+      ch_sorted(&pc);
+      cheap_t c;
+      ch_init(&c, buf, k);
+      for (int p = 0; p < pc.size; p++) {
+        int l = (int)pc.h[p].id;
+        const float *rows = h->lrows[l];
+        const int64_t *ids = h->lids[l];
+        for (int64_t j = 0; j < h->lsize[l]; j++) {
+          int64_t id = ids[j];
+          if (exclude && id >= 0 && id < exclude_len && exclude[id] != 0) continue;
+          const float *xv = rows + j * d;
+          ch_offer(&c, ip ? -mvref_ip_simd(qp, xv, d, 4) : mvref_l2_simd(qp, xv, d, 8), id);
+        }
+      }
+      ch_sorted(&c);
+      for (int j = 0; j < k; j++) {
+        if (j < c.size) {
+          float v = c.h[j].d;
+          D[qi * k + j] = ip ? -v : v;
+          I[qi * k + j] = c.h[j].id;
+        } else {
+          D[qi * k + j] = IVF_PAD_DIST;
+          I[qi * k + j] = -1;
+        }
+      }
+    }
+    free(pbuf); free(buf);
   }
 }

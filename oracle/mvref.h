@@ -111,6 +111,24 @@ void mvref_ivfpq_search(const mvref_ivfpq *h, const float *q, int64_t nq,
                         int64_t exclude_len, float *D, int64_t *I,
                         int nthreads);
 
+/* ---- IVF-Flat (mojovec/index/index_ivf_flat.mojo) ---- */
+typedef struct mvref_ivfflat mvref_ivfflat;
+mvref_ivfflat *mvref_ivfflat_create(int d, int nlist, int metric);
+void mvref_ivfflat_destroy(mvref_ivfflat *h);
+void mvref_ivfflat_set_centroids(mvref_ivfflat *h, const float *coarse);
+void mvref_ivfflat_get_centroids(const mvref_ivfflat *h, float *coarse);
+void mvref_ivfflat_train(mvref_ivfflat *h, const float *x, int64_t n, int niter,
+                         uint64_t seed, int nthreads);
+void mvref_ivfflat_add(mvref_ivfflat *h, const float *x, int64_t n, const int64_t *ids,
+                       int nthreads);
+void mvref_ivfflat_load_lists(mvref_ivfflat *h, const int64_t *list_off, const int32_t *ids,
+                              const float *rows, int64_t ntotal);
+int64_t mvref_ivfflat_list_size(const mvref_ivfflat *h, int list);
+void mvref_ivfflat_list(const mvref_ivfflat *h, int list, int64_t *ids, float *rows);
+void mvref_ivfflat_search(const mvref_ivfflat *h, const float *q, int64_t nq, int k,
+                          int nprobe, const uint8_t *exclude, int64_t exclude_len,
+                          float *D, int64_t *I, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

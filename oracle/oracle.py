@@ -79,6 +79,17 @@ def lib():
             "mvref_ivfpq_load_lists": (None, [_vp, _i64p, _i32p, _u8p, _vp, _i64]),
             "mvref_ivfpq_list_size": (_i64, [_vp, _int]),
             "mvref_ivfpq_list": (None, [_vp, _int, _i64p, _u8p]),
+            "mvref_ivfflat_create": (_vp, [_int, _int, _int]),
+            "mvref_ivfflat_destroy": (None, [_vp]),
+            "mvref_ivfflat_set_centroids": (None, [_vp, _f32p]),
+            "mvref_ivfflat_get_centroids": (None, [_vp, _f32p]),
+            "mvref_ivfflat_train": (None, [_vp, _f32p, _i64, _int, ctypes.c_uint64, _int]),
+            "mvref_ivfflat_add": (None, [_vp, _f32p, _i64, _vp, _int]),
+            "mvref_ivfflat_load_lists": (None, [_vp, _i64p, _i32p, _f32p, _i64]),
+            "mvref_ivfflat_list_size": (_i64, [_vp, _int]),
+            "mvref_ivfflat_list": (None, [_vp, _int, _i64p, _f32p]),
+            "mvref_ivfflat_search": (None, [_vp, _f32p, _i64, _int, _int, _vp, _i64, _f32p, _i64p,
+                                            _int]),
             "mvref_ivfpq_search": (None, [_vp, _f32p, _i64, _int, _int, _vp, _int, _vp, _i64,
                                           _f32p, _i64p, _int]),
         }
@@ -349,4 +360,55 @@ class IVFPQIndex:
         ex = None if exclude is None else np.ascontiguousarray(exclude, np.uint8)
         lib().mvref_ivfpq_search(self.h, q, nq, k, nprobe, _ptr(npq), k_factor, _ptr(ex),
                                  0 if ex is None else ex.size, D, I, nthreads)
+        return D, I
+
+
+# ---- IVF-Flat: mojovec/index/index_ivf_flat.mojo ----
+class IVFFlatIndex:
+    def __init__(self, d, nlist, metric=L2):
+        self.d, self.nlist, self.metric = d, nlist, metric
+        self.h = lib().mvref_ivfflat_create(d, nlist, metric)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mvref_ivfflat_destroy(self.h)
+            self.h = None
+
+    def set_centroids(self, coarse):
+        lib().mvref_ivfflat_set_centroids(self.h, _f32(coarse))
+
+    def centroids(self):
+        c = np.empty((self.nlist, self.d), np.float32)
+        lib().mvref_ivfflat_get_centroids(self.h, c)
+        return c
+
+    def train(self, x, niter=15, seed=KMEANS_SEED, nthreads=0):
+        x = _f32(x).reshape(-1, self.d)
+        lib().mvref_ivfflat_train(self.h, x, x.shape[0], niter, seed, nthreads)
+
+    def add(self, x, ids=None, nthreads=0):
+        x = _f32(x).reshape(-1, self.d)
+        idv = None if ids is None else np.ascontiguousarray(ids, np.int64)
+        lib().mvref_ivfflat_add(self.h, x, x.shape[0], _ptr(idv), nthreads)
+
+    def load_lists(self, list_off, ids, rows):
+        list_off = np.ascontiguousarray(list_off, np.int64)
+        lib().mvref_ivfflat_load_lists(self.h, list_off, np.ascontiguousarray(ids, np.int32),
+                                       _f32(rows).reshape(-1), int(list_off[-1]))
+
+    def list(self, l):
+        n = lib().mvref_ivfflat_list_size(self.h, l)
+        ids = np.empty(n, np.int64)
+        rows = np.empty((n, self.d), np.float32)
+        lib().mvref_ivfflat_list(self.h, l, ids, rows)
+        return ids, rows
+
+    def search(self, q, k, nprobe, exclude=None, nthreads=0):
+        q = _f32(q).reshape(-1, self.d)
+        nq = q.shape[0]
+        D = np.empty((nq, k), np.float32)
+        I = np.empty((nq, k), np.int64)
+        ex = None if exclude is None else np.ascontiguousarray(exclude, np.uint8)
+        lib().mvref_ivfflat_search(self.h, q, nq, k, nprobe, _ptr(ex), 0 if ex is None else ex.size,
+                                   D, I, nthreads)
         return D, I

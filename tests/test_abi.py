@@ -19,7 +19,8 @@ def declared_symbols():
 
 def test_header_declares_the_expected_families():
     syms = declared_symbols()
-    for fam in ("mvg_flat_", "mvg_sq8_", "mvg_kmeans_", "mvg_pq_", "mvg_ivfpq_", "mvg_coll_"):
+    for fam in ("mvg_flat_", "mvg_sq8_", "mvg_kmeans_", "mvg_pq_", "mvg_ivfpq_", "mvg_ivfflat_",
+                "mvg_coll_"):
         assert any(s.startswith(fam) for s in syms), fam
 
 
@@ -66,6 +67,12 @@ def test_index_validation_messages():
         mojovec_amd.IndexIVFPQ(10, 4, 3)
     with pytest.raises(mojovec_amd.MojoVecError, match="PQ M must divide dimension"):
         mojovec_amd.ProductQuantizer(10, 3)
+    with pytest.raises(mojovec_amd.MojoVecError, match="IVF-Flat dimension must be positive"):
+        mojovec_amd.IndexIVFFlat(0, 4)
+    with pytest.raises(mojovec_amd.MojoVecError, match="IVF-Flat nlist must be between 1 and 1000000"):
+        mojovec_amd.IndexIVFFlat(8, 0)
+    with pytest.raises(mojovec_amd.MojoVecError, match="metric"):
+        mojovec_amd.IndexIVFFlat(8, 4, 7)
 
 
 def test_no_oracle_in_product_package():

@@ -437,3 +437,120 @@ def test_ivfpq_rerank_with_every_candidate_is_exact_search():
     D, I = ix.search(q, 5, 4, k_factor=120)  # k * k_factor = every row
     Df, If = o.flat_search(x, q, 5)
     assert np.array_equal(I, If) and np.array_equal(D, Df)
+
+
+# ---------------------------------------------------------------- IVF-Flat
+def _canonical_topk_f64(d2, ids, k):
+    """k smallest (distance, id) pairs in lexicographic order, from float64."""
+    order = np.lexsort((ids, d2))[:k]
+    return d2[order], ids[order]
+
+
+def test_ivfflat_accuracy_threshold():
+    # tests/test_accuracy.mojo:96-126: recall@10 >= 0.75 at nlist 64, nprobe 16
+    db, q = _uniform_dataset()
+    _, gt = o.flat_search(db, q, 10)
+    ix = o.IVFFlatIndex(32, 64)
+    ix.train(db)
+    ix.add(db)
+    _, I = ix.search(q, 10, 16)
+    assert _recall(gt, I) >= 0.75
+    _, Iall = ix.search(q, 10, 64)
+    assert _recall(gt, Iall) == 1.0
+
+
+@pytest.mark.parametrize("metric", [o.L2, o.IP])
+def test_ivfflat_exhaustive_probe_is_exact_search(metric):
+    # nprobe = nlist scans every row: on integer data every summation order is
+    # exact, so the result is the float64 brute force with (distance, id) ties.
+    rng = np.random.default_rng(21)
+    x = rint(rng, (1500, 24), 0, 8)  # few distinct values: many exact ties
+    q = rint(rng, (12, 24), 0, 8)
+    ix = o.IVFFlatIndex(24, 6, metric)
+    ix.train(x, niter=4)
+    ids = rng.permutation(100000)[:1500].astype(np.int64)  # custom, unordered ids
+    ix.add(x, ids=ids)
+    D, I = ix.search(q, 30, 6)
+    x64, q64 = x.astype(np.float64), q.astype(np.float64)
+    for i in range(12):
+        dd = ((q64[i] - x64) ** 2).sum(-1) if metric == o.L2 else -(x64 @ q64[i])
+        ed, ei = _canonical_topk_f64(dd, ids, 30)
+        assert np.array_equal(I[i], ei)
+        assert np.array_equal(D[i].astype(np.float64), ed if metric == o.L2 else -ed)
+
+
+@pytest.mark.parametrize("metric", [o.L2, o.IP])
+def test_ivfflat_probes_only_the_nearest_lists(metric):
+    # nprobe < nlist: the result is the brute force over the rows whose list
+    # is among the query's nprobe nearest centroids (canonical ties), and each
+    # row sits in the list of its own nearest centroid.
+    rng = np.random.default_rng(22)
+    x = rint(rng, (3000, 16), 0, 16)
+    q = rint(rng, (20, 16), 0, 16)
+    nlist, nprobe = 12, 3
+    ix = o.IVFFlatIndex(16, nlist, metric)
+    ix.train(x, niter=5)
+    ix.add(x)
+    cen = ix.centroids().astype(np.float64)
+    x64, q64 = x.astype(np.float64), q.astype(np.float64)
+    cd = (((x64[:, None] - cen[None]) ** 2).sum(-1) if metric == o.L2 else -(x64 @ cen.T))
+    owner = np.array([np.lexsort((np.arange(nlist), r))[0] for r in cd])
+    for l in range(nlist):
+        lid, rows = ix.list(l)
+        assert np.array_equal(lid, np.flatnonzero(owner == l))  # append order
+        assert np.array_equal(rows, x[lid])
+    D, I = ix.search(q, 10, nprobe)
+    for i in range(20):
+        qc = ((q64[i] - cen) ** 2).sum(-1) if metric == o.L2 else -(cen @ q64[i])
+        probed = np.lexsort((np.arange(nlist), qc))[:nprobe]
+        rows = np.flatnonzero(np.isin(owner, probed))
+        dd = ((q64[i] - x64[rows]) ** 2).sum(-1) if metric == o.L2 else -(x64[rows] @ q64[i])
+        ed, ei = _canonical_topk_f64(dd, rows.astype(np.int64), 10)
+        assert np.array_equal(I[i], ei)
+        assert np.array_equal(D[i].astype(np.float64), ed if metric == o.L2 else -ed)
+
+
+def test_ivfflat_reference_known_answers():
+    # tests/test_ivf_flat.mojo:69-88: one vector, nlist 1 -> id 0 at distance 0
+    data = np.arange(4, dtype=np.float32)[None]
+    ix = o.IVFFlatIndex(4, 1)
+    ix.train(data)
+    ix.add(data)
+    D, I = ix.search(data, 1, 1)
+    assert I[0, 0] == 0 and D[0, 0] == 0.0
+    # :91-137: k = 5 over one stored row with id 42 -> tail (1e38, -1)
+    zero = np.zeros((1, 4), np.float32)
+    ix = o.IVFFlatIndex(4, 1)
+    ix.train(zero)
+    ix.add(zero, ids=np.array([42]))
+    D, I = ix.search(zero, 5, 1)
+    assert I[0].tolist() == [42, -1, -1, -1, -1]
+    assert D[0, 0] == 0.0 and (D[0, 1:] == np.float32(1e38)).all()
+    # :9-66: custom ids i * 10, nprobe 3 of 10 lists; every result is a stored id
+    rng = np.random.default_rng(101)
+    x = (rng.random((1000, 16)) * 2 - 1).astype(np.float32)
+    ix = o.IVFFlatIndex(16, 10)
+    ix.train(x)
+    ix.add(x, ids=np.arange(1000) * 10)
+    D, I = ix.search(x[:10], 5, 3)
+    assert (I % 10 == 0).all() and (I[:, 0] == np.arange(10) * 10).all()
+    assert (np.diff(D, axis=1) >= 0).all()
+
+
+def test_ivfflat_untrained_empty_oversized_nprobe_and_exclusion():
+    ix = o.IVFFlatIndex(8, 4)
+    rows = np.load(os.path.join(GOLDEN, "ivfpq_rows.npz"))["rows"]
+    ix.add(rows)  # dropped: not trained (index_ivf_flat.mojo:102-104)
+    D, I = ix.search(rows[:2], 3, 2)
+    assert (I == -1).all() and (D == np.float32(1e38)).all()
+    ix.train(rows)
+    D, I = ix.search(rows[:2], 3, 2)  # trained, empty
+    assert (I == -1).all() and (D == np.float32(1e38)).all()
+    ix.add(rows)
+    D1, I1 = ix.search(rows[:3], 4, 4)
+    D2, I2 = ix.search(rows[:3], 4, 400)  # nprobe clamps to nlist
+    assert np.array_equal(I1, I2) and np.array_equal(D1, D2)
+    ex = np.zeros(256, np.uint8)
+    ex[I1[:, 0]] = 1
+    _, Ie = ix.search(rows[:3], 4, 4, exclude=ex)
+    assert np.array_equal(Ie[:, :3], I1[:, 1:])
