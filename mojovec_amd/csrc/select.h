@@ -236,4 +236,85 @@ __device__ int select_sort_u64(const uint64_t *in, int n, int k, Scratch &s,
   return cnt;
 }
 
+// ---------------------------------------------------------------------------
+// Wavefront top-k for k <= 64 over unique u64 composites (one value per lane,
+// sentinel = kSentinel64): bitonic networks across the 64 lanes, no LDS and
+// no workgroup barrier.
+
+// Ascending bitonic sort of the wave's 64 values.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const uint64_t p = __shfl_xor(v, j, 64);
+      const bool asc = (lane & k) == 0;
+      const bool lower = (lane & j) == 0;
+      v = (lower == asc) ? (p < v ? p : v) : (p > v ? p : v);
+    }
+  }
+  return v;
+}
+
+// The 64 smallest of two ascending sequences, ascending (Batcher: the
+// lane-wise minimum of a and reversed b is bitonic; half-cleaners sort it).
+__device__ __forceinline__ uint64_t wave_merge64(uint64_t a, uint64_t b) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t r = __shfl(b, 63 - lane, 64);
+  uint64_t v = a < r ? a : r;
+#pragma unroll
+  for (int j = 32; j > 0; j >>= 1) {
+    const uint64_t p = __shfl_xor(v, j, 64);
+    v = (lane & j) == 0 ? (p < v ? p : v) : (p > v ? p : v);
+  }
+  return v;
+}
+
+// Folds one block of CHUNK (a multiple of 64) u32 keys into a wave's running
+// ascending set `best` of composites key << 32 | (pos0 + index); the k
+// smallest of the union end in lanes 0..k-1. Keys past `len` and sentinel
+// keys are ignored. A key can only enter the top k if it is at most the
+// current k-th best, or -- while fewer than k are held -- at most the k-th
+// smallest of the per-lane minima of this block (k lanes hold a key no larger
+// than that); the few keys under that bound are merged 64 at a time.
+template <int CHUNK>
+__device__ uint64_t wave_fold_keys(const uint32_t *keys, int len, uint32_t pos0, uint64_t best,
+                                   int k) {
+  const int lane = threadIdx.x & 63;
+  constexpr int PER = CHUNK / 64;
+  uint64_t theta = __shfl(best, k - 1, 64);
+  if (theta == kSentinel64) {
+    uint64_t lmin = kSentinel64;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      const int idx = lane + 64 * j;
+      const uint32_t key = idx < len ? keys[idx] : kSentinel;
+      if (key != kSentinel) {
+        const uint64_t c = ((uint64_t)key << 32) | (pos0 + (uint32_t)idx);
+        lmin = c < lmin ? c : lmin;
+      }
+    }
+    theta = __shfl(wave_sort64(lmin), k - 1, 64);
+  }
+  uint32_t mask = 0;
+#pragma unroll
+  for (int j = 0; j < PER; j++) {
+    const int idx = lane + 64 * j;
+    const uint32_t key = idx < len ? keys[idx] : kSentinel;
+    if (key != kSentinel && (((uint64_t)key << 32) | (pos0 + (uint32_t)idx)) <= theta) mask |= 1u << j;
+  }
+  while (__ballot(mask != 0) != 0) {
+    uint64_t c = kSentinel64;
+    if (mask) {
+      const int j = __builtin_ctz(mask);
+      mask &= mask - 1;
+      const int idx = lane + 64 * j;
+      c = ((uint64_t)keys[idx] << 32) | (pos0 + (uint32_t)idx);
+    }
+    best = wave_merge64(best, wave_sort64(c));
+  }
+  return best;
+}
+
 }  // namespace mvg
