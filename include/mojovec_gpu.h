@@ -186,6 +186,11 @@ void mvg_ivfpq_destroy(mvg_ivfpq *h);
 /* IndexIVFFlat.__init__ (:29-52) with an IndexFlat quantizer of the same
  * metric; nprobe starts at 1 (:46). */
 int mvg_ivfflat_create(int d, int nlist, int metric, mvg_ivfflat **out);
+/* The same with the quantizer's own metric (IndexFlat(d) is L2 whatever the
+ * index metric): coarse assignment and probe selection use quantizer_metric,
+ * list scoring uses metric. */
+int mvg_ivfflat_create_with_quantizer(int d, int nlist, int metric, int quantizer_metric,
+                                      mvg_ivfflat **out);
 /* IndexIVFFlat.train (:65-77): coarse k-means, which the reference fixes at
  * 15 iterations (:73); explicit here. No-op once trained (:71). */
 int mvg_ivfflat_train(mvg_ivfflat *h, const float *x, int64_t n, int niter, uint64_t seed);
@@ -215,12 +220,18 @@ int mvg_ivfflat_search_device(mvg_ivfflat *h, const float *q, int64_t nq, int k,
                               int64_t *labels, void *stream);
 /* State (ntotal, is_trained, nprobe) and construction parameters. */
 int mvg_ivfflat_info(mvg_ivfflat *h, int64_t *ntotal, int *trained, int *nprobe);
-int mvg_ivfflat_shape(mvg_ivfflat *h, int *d, int *nlist, int *metric);
+int mvg_ivfflat_shape(mvg_ivfflat *h, int *d, int *nlist, int *metric, int *quantizer_metric);
 /* Inverted-list sizes [nlist] and one list's ids / FP32 rows in append order
  * (ArrayInvertedLists with code_size 4d, :52). */
 int mvg_ivfflat_list_sizes(mvg_ivfflat *h, int64_t *sizes);
 int mvg_ivfflat_get_list(mvg_ivfflat *h, int list, int64_t *ids, float *rows);
 void mvg_ivfflat_destroy(mvg_ivfflat *h);
+/* write_index_ivf_flat / read_index_ivf_flat (mojovec/io/serialization.mojo:
+ * 980-1020) to / from a file, the reference's byte layout (no checksum
+ * trailer at index level). Loading also rejects a quantizer that contradicts
+ * the header, nprobe <= 0 and ids outside [0, 2^31 - 1]. */
+int mvg_ivfflat_save(mvg_ivfflat *h, const char *path);
+int mvg_ivfflat_load(const char *path, mvg_ivfflat **out);
 
 /* ---- captured search batch (no reference counterpart) ------------------ */
 /* Records one whole device-pointer search (coarse, work-queue build, scan,

@@ -103,6 +103,7 @@ SIGNATURES = {
     "mvg_ivfpq_save": (_i, [_p, ctypes.c_char_p]),
     "mvg_ivfpq_load": (_i, [ctypes.c_char_p, ctypes.POINTER(_p)]),
     "mvg_ivfflat_create": (_i, [_i, _i, _i, ctypes.POINTER(_p)]),
+    "mvg_ivfflat_create_with_quantizer": (_i, [_i, _i, _i, _i, ctypes.POINTER(_p)]),
     "mvg_ivfflat_train": (_i, [_p, _p, _i64, _i, _u64]),
     "mvg_ivfflat_set_centroids": (_i, [_p, _p]),
     "mvg_ivfflat_get_centroids": (_i, [_p, _p]),
@@ -111,10 +112,13 @@ SIGNATURES = {
     "mvg_ivfflat_search": (_i, [_p, _p, _i64, _i, _p, _i64, _p, _p]),
     "mvg_ivfflat_search_device": (_i, [_p, _p, _i64, _i, _p, _i64, _p, _p, _p]),
     "mvg_ivfflat_info": (_i, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
-    "mvg_ivfflat_shape": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "mvg_ivfflat_shape": (_i, [_p, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                               ctypes.POINTER(_i)]),
     "mvg_ivfflat_list_sizes": (_i, [_p, _p]),
     "mvg_ivfflat_get_list": (_i, [_p, _i, _p, _p]),
     "mvg_ivfflat_destroy": (None, [_p]),
+    "mvg_ivfflat_save": (_i, [_p, ctypes.c_char_p]),
+    "mvg_ivfflat_load": (_i, [ctypes.c_char_p, ctypes.POINTER(_p)]),
     "mvg_snapshot_checksum": (_i, [_p, _i64, ctypes.POINTER(_u64)]),
 }
 

@@ -989,6 +989,7 @@ void ivfflat_search_on(mvg_ivfflat *h, const float *q, int64_t nq, int k, const 
   a.d = h->d;
   a.nlist = h->nlist;
   a.metric = h->metric;
+  a.coarse_metric = h->quantizer_metric;
   a.coarse = h->coarse.as<float>();
   a.list_off = h->list_off.as<int64_t>();
   a.rows = h->rows.as<float>();
@@ -1009,15 +1010,24 @@ void ivfflat_search_on(mvg_ivfflat *h, const float *q, int64_t nq, int k, const 
 extern "C" {
 
 int mvg_ivfflat_create(int d, int nlist, int metric, mvg_ivfflat **out) {
+  return mvg_ivfflat_create_with_quantizer(d, nlist, metric, metric, out);
+}
+
+int mvg_ivfflat_create_with_quantizer(int d, int nlist, int metric, int quantizer_metric,
+                                      mvg_ivfflat **out) {
   API_BEGIN
   require(out != nullptr, "invalid arguments");
   require(d > 0, "IVF-Flat dimension must be positive.");
   require(nlist > 0 && nlist <= 1000000, "IVF-Flat nlist must be between 1 and 1000000.");
-  require(metric == MVG_METRIC_L2 || metric == MVG_METRIC_IP, "metric must be 0 (L2) or 1 (IP).");
+  require((metric == MVG_METRIC_L2 || metric == MVG_METRIC_IP) &&
+              (quantizer_metric == MVG_METRIC_L2 || quantizer_metric == MVG_METRIC_IP),
+          "metric must be 0 (L2) or 1 (IP).");
   auto h = std::make_unique<mvg_ivfflat>();
   h->d = d;
   h->nlist = nlist;
   h->metric = metric;
+  h->quantizer_metric = quantizer_metric;
+  h->list_capacity.assign((size_t)nlist, 0);
   h->lids.resize(nlist);
   h->lrows.resize(nlist);
   h->s = new_stream();
@@ -1077,7 +1087,8 @@ int mvg_ivfflat_add(mvg_ivfflat *h, const float *x, int64_t n, const int64_t *id
   h->tlab.reserve(sizeof(int32_t) * n);
   int32_t *dl = h->tlab.as<int32_t>();
   // quantizer.search(x, 1) (:110-116)
-  mvg::flat_argmin(h->coarse.as<float>(), h->nlist, d, h->metric, false, dx, n, dl, nullptr, h->s);
+  mvg::flat_argmin(h->coarse.as<float>(), h->nlist, d, h->quantizer_metric, false, dx, n, dl, nullptr,
+                   h->s);
   std::vector<int32_t> lab(n);
   download(lab.data(), dl, (size_t)n, h->s);
   MVG_HIP(hipStreamSynchronize(h->s));
@@ -1146,12 +1157,13 @@ int mvg_ivfflat_info(mvg_ivfflat *h, int64_t *ntotal, int *trained, int *nprobe)
   API_END
 }
 
-int mvg_ivfflat_shape(mvg_ivfflat *h, int *d, int *nlist, int *metric) {
+int mvg_ivfflat_shape(mvg_ivfflat *h, int *d, int *nlist, int *metric, int *quantizer_metric) {
   API_BEGIN
   require(h, "invalid handle");
   if (d) *d = h->d;
   if (nlist) *nlist = h->nlist;
   if (metric) *metric = h->metric;
+  if (quantizer_metric) *quantizer_metric = h->quantizer_metric;
   API_END
 }
 

@@ -30,8 +30,13 @@ struct mvg_ivfpq {
 // snapshots see) and a device CSR copy with each list in id order.
 struct mvg_ivfflat {
   int d = 0, nlist = 0, metric = 0, nprobe = 1;
+  int quantizer_metric = 0;  // the coarse IndexFlat's own metric
   bool trained = false, dirty = true;
   int64_t ntotal = 0, max_list = 0;
+  // Reference storage capacities a snapshot records: the quantizer's rows
+  // and each list's (grown by the reference's rules at save time).
+  int64_t coarse_capacity = 1024;
+  std::vector<int64_t> list_capacity;
   std::vector<std::vector<int32_t>> lids;
   std::vector<std::vector<float>> lrows;
   mvg::DevBuf coarse, list_off, rows, ids;

@@ -334,7 +334,7 @@ void ivfflat_search(const IvfFlatArgs &a, IvfFlatWork &w, hipStream_t s) {
     const int64_t nb = std::min<int64_t>(qb, a.nq - b);
     const float *qb_ptr = a.q + b * a.d;
     const int64_t npairs = nb * np;
-    flat_topk(a.coarse, a.nlist, a.d, a.metric, qb_ptr, nb, np, nullptr, 0, w.coarse_part, 0.0f,
+    flat_topk(a.coarse, a.nlist, a.d, a.coarse_metric, qb_ptr, nb, np, nullptr, 0, w.coarse_part, 0.0f,
               nullptr, nullptr, probes, nullptr, s);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(a.nlist, 256)), dim3(256), 0, s,
                        w.cnt.as<int>(), (int64_t)a.nlist, 0);

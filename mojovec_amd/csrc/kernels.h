@@ -105,7 +105,7 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
 struct IvfFlatArgs {
   const float *q;
   int64_t nq;
-  int d, nlist, metric;
+  int d, nlist, metric, coarse_metric;
   const float *coarse;      // [nlist][d]
   const int64_t *list_off;  // [nlist + 1], CSR over `rows`
   const float *rows;        // [ntotal][d], each list in id order

@@ -1,7 +1,8 @@
-// IVF-PQ snapshots in the reference's on-disk format, so that collections saved
-// by either side load on the other:
+// IVF-PQ and IVF-Flat snapshots in the reference's on-disk format, so that
+// collections and indexes saved by either side load on the other:
 //   collection file  mojovec/api/collection_ivfpq.mojo:335-476
 //   index stream     mojovec/io/serialization.mojo:1029-1090 (IndexIVFPQ),
+//                    :980-1020 (IndexIVFFlat),
 //                    :229-262 (IndexFlat), :903-944 (ArrayInvertedLists),
 //                    :946-975 (ProductQuantizer), :66-80,136-147 (Int, Bool)
 //   checksum trailer mojovec/io/snapshot_file.mojo:8-108
@@ -32,6 +33,7 @@ namespace {
 using mvg::coll_raise;
 
 constexpr int64_t kMagicFlat = 0x4d4a4f46;      // serialization.mojo:21
+constexpr int64_t kMagicIvfFlat = 0x4d4a4f49;   // :23
 constexpr int64_t kMagicIvfPq = 0x4d4a4f50;     // :24
 constexpr int64_t kMagicInvlists = 0x4d4a4f4c;  // :25
 constexpr int64_t kMagicPq = 0x4d4a4f51;        // :26
@@ -178,6 +180,19 @@ class Writer {
   uint64_t sum_ = kSumSeed;
 };
 
+// write_index_flat (serialization.mojo:229-238): `rows` real rows, then the
+// unused capacity as zeros.
+void write_flat(Writer &w, int64_t d, int64_t rows, int64_t capacity, int metric,
+                const std::vector<float> &data) {
+  w.i64(kMagicFlat);
+  w.i64(d);
+  w.i64(rows);
+  w.i64(capacity);
+  w.i64(metric == MVG_METRIC_IP ? 1 : 0);
+  w.bytes(data.data(), sizeof(float) * (size_t)(rows * d));
+  w.zeros(sizeof(float) * (size_t)((capacity - rows) * d));
+}
+
 // Host copy of everything an IndexIVFPQ stream holds.
 struct IvfImage {
   int64_t d = 0, nlist = 0, M = 0, nprobe = 1, ntotal = 0;
@@ -208,14 +223,7 @@ void write_ivf(Writer &w, const mvg_ivfpq *h, const std::vector<float> &coarse,
   w.i64(h->ntotal);
   w.boolean(h->trained);
   w.i64(h->metric == MVG_METRIC_IP ? 1 : 0);
-  // coarse quantizer
-  w.i64(kMagicFlat);
-  w.i64(d);
-  w.i64(qrows);
-  w.i64(coarse_capacity);
-  w.i64(h->metric == MVG_METRIC_IP ? 1 : 0);
-  w.bytes(coarse.data(), sizeof(float) * (size_t)(qrows * d));
-  w.zeros(sizeof(float) * (size_t)((coarse_capacity - qrows) * d));
+  write_flat(w, d, qrows, coarse_capacity, h->metric, coarse);
   // inverted lists: ids widened to the reference's 8-byte Int
   w.i64(kMagicInvlists);
   w.i64(nlist);
@@ -353,8 +361,10 @@ class Reader {
   size_t n_, pos_ = 0;
 };
 
-// read_index_flat (serialization.mojo:240-262).
-void read_flat(Reader &r, IvfImage &img, int64_t &qd, int64_t &qrows, int &qmetric) {
+// read_index_flat (serialization.mojo:240-262): header fields, capacity and
+// the first qrows rows.
+void read_flat(Reader &r, int64_t &qd, int64_t &qrows, int &qmetric, int64_t &capacity,
+               std::vector<float> &rows) {
   if (r.i64() != kMagicFlat) coll_raise("Invalid magic for IndexFlat");
   qd = r.i64();
   validate_dimension(qd);
@@ -369,10 +379,37 @@ void read_flat(Reader &r, IvfImage &img, int64_t &qd, int64_t &qrows, int &qmetr
   const int64_t floats = cap * qd;  // cap, qd bounded above: no overflow
   checked_byte_count(floats, 4);
   const uint8_t *src = r.take((size_t)(floats * 4));
-  img.coarse_capacity = cap;
-  img.coarse_rows = qrows;
-  img.coarse.resize((size_t)(qrows * qd));
-  std::memcpy(img.coarse.data(), src, sizeof(float) * img.coarse.size());
+  capacity = cap;
+  rows.resize((size_t)(qrows * qd));
+  std::memcpy(rows.data(), src, sizeof(float) * rows.size());
+}
+
+// read_invlists (serialization.mojo:916-944) into an index of nlist lists of
+// code_size-byte codes.
+void read_invlists(Reader &r, int64_t nlist_expect, int64_t code_size_expect,
+                   std::vector<int64_t> &capacity, std::vector<std::vector<int64_t>> &ids,
+                   std::vector<std::vector<uint8_t>> &codes) {
+  if (r.i64() != kMagicInvlists) coll_raise("Invalid magic for ArrayInvertedLists");
+  const int64_t nlist = r.i64();
+  check_size_limit(nlist, 1000000);
+  const int64_t code_size = r.i64();
+  check_size_limit(code_size, 65536);
+  if (nlist != nlist_expect || code_size != code_size_expect)
+    coll_raise("Inverted-list shape does not match its index header.");
+  ids.resize((size_t)nlist);
+  codes.resize((size_t)nlist);
+  capacity.resize((size_t)nlist);
+  for (int64_t l = 0; l < nlist; l++) {
+    const int64_t size = r.i64();
+    const int64_t cap = r.i64();
+    check_size_limit(cap, 1000000000);
+    check_size_limit(size, cap);
+    capacity[(size_t)l] = resized_capacity(0, cap);  // resize(i, capacity)
+    ids[(size_t)l].resize((size_t)size);
+    r.array(ids[(size_t)l].data(), size);
+    codes[(size_t)l].resize((size_t)(size * code_size));
+    r.array(codes[(size_t)l].data(), size * code_size);
+  }
 }
 
 // read_index_ivf_pq (serialization.mojo:1046-1090) with read_invlists
@@ -399,32 +436,14 @@ IvfImage read_ivf(Reader &r) {
 
   int64_t qd = 0, qrows = 0;
   int qmetric = 0;
-  read_flat(r, img, qd, qrows, qmetric);
+  read_flat(r, qd, qrows, qmetric, img.coarse_capacity, img.coarse);
+  img.coarse_rows = qrows;
   if (qd != img.d || (img.trained && qrows != img.nlist) || (!img.trained && qrows != 0) ||
       qmetric != img.metric)
     coll_raise("IndexIVFPQ coarse quantizer header mismatch.");
 
-  if (r.i64() != kMagicInvlists) coll_raise("Invalid magic for ArrayInvertedLists");
-  const int64_t nlist = r.i64();
-  check_size_limit(nlist, 1000000);
-  const int64_t code_size = r.i64();
-  check_size_limit(code_size, 65536);
-  if (nlist != img.nlist || code_size != img.M)
-    coll_raise("Inverted-list shape does not match its index header.");
-  img.ids.resize((size_t)nlist);
-  img.codes.resize((size_t)nlist);
-  img.list_capacity.resize((size_t)nlist);
-  for (int64_t l = 0; l < nlist; l++) {
-    const int64_t size = r.i64();
-    const int64_t cap = r.i64();
-    check_size_limit(cap, 1000000000);
-    check_size_limit(size, cap);
-    img.list_capacity[(size_t)l] = resized_capacity(0, cap);  // resize(i, capacity)
-    img.ids[(size_t)l].resize((size_t)size);
-    r.array(img.ids[(size_t)l].data(), size);
-    img.codes[(size_t)l].resize((size_t)(size * code_size));
-    r.array(img.codes[(size_t)l].data(), size * code_size);
-  }
+  read_invlists(r, img.nlist, img.M, img.list_capacity, img.ids, img.codes);
+  const int64_t code_size = img.M;
 
   if (r.i64() != kMagicPq) coll_raise("Invalid magic for ProductQuantizer");
   const int64_t pd = r.i64();
@@ -464,6 +483,99 @@ mvg_ivfpq *install(const IvfImage &img) {
     h->lcodes[(size_t)l] = img.codes[(size_t)l];
   }
   h->ntotal = img.ntotal;
+  h->dirty = true;
+  return h.release();
+}
+
+// ---- IVF-Flat ------------------------------------------------------------------
+
+// write_index_ivf_flat (serialization.mojo:980-993): header, the quantizer
+// IndexFlat with its own metric, inverted lists of code_size 4d whose codes
+// are the raw FP32 rows.
+void write_ivfflat(Writer &w, const mvg_ivfflat *h, const std::vector<float> &coarse,
+                   int64_t coarse_capacity, const std::vector<int64_t> &list_capacity) {
+  const int64_t d = h->d, nlist = h->nlist;
+  w.i64(kMagicIvfFlat);
+  w.i64(d);
+  w.i64(nlist);
+  w.i64(h->nprobe);
+  w.i64(h->ntotal);
+  w.boolean(h->trained);
+  w.i64(h->metric == MVG_METRIC_IP ? 1 : 0);
+  write_flat(w, d, h->trained ? nlist : 0, coarse_capacity, h->quantizer_metric, coarse);
+  w.i64(kMagicInvlists);
+  w.i64(nlist);
+  w.i64(4 * d);
+  std::vector<int64_t> wide;
+  for (int64_t l = 0; l < nlist; l++) {
+    const auto &li = h->lids[(size_t)l];
+    w.i64((int64_t)li.size());
+    w.i64(list_capacity[(size_t)l]);
+    wide.assign(li.begin(), li.end());
+    w.bytes(wide.data(), sizeof(int64_t) * wide.size());
+    w.bytes(h->lrows[(size_t)l].data(), sizeof(float) * h->lrows[(size_t)l].size());
+  }
+}
+
+struct IvfFlatImage {
+  int64_t d = 0, nlist = 0, nprobe = 1, ntotal = 0;
+  bool trained = false;
+  int metric = MVG_METRIC_L2, quantizer_metric = MVG_METRIC_L2;
+  int64_t coarse_capacity = 1024, coarse_rows = 0;
+  std::vector<float> coarse;
+  std::vector<int64_t> list_capacity;
+  std::vector<std::vector<int64_t>> ids;
+  std::vector<std::vector<uint8_t>> rows;  // 4d bytes per entry
+};
+
+// read_index_ivf_flat (serialization.mojo:995-1020). The reference takes any
+// metric value other than 1 as L2 and does not compare the quantizer with the
+// header; here a quantizer whose dimension or row count contradicts the
+// header, a non-positive nprobe and ids outside [0, 2^31 - 1] are rejected,
+// since the index could not search them.
+IvfFlatImage read_ivfflat(Reader &r) {
+  IvfFlatImage img;
+  if (r.i64() != kMagicIvfFlat) coll_raise("Invalid magic for IndexIVFFlat");
+  img.d = r.i64();
+  validate_dimension(img.d);
+  img.nlist = r.i64();
+  check_size_limit(img.nlist, 1000000);
+  img.nprobe = r.i64();
+  check_size_limit(img.nprobe, 1000000);
+  img.ntotal = r.i64();
+  check_size_limit(img.ntotal, 1000000000);
+  img.trained = r.boolean();
+  img.metric = r.i64() == 1 ? MVG_METRIC_IP : MVG_METRIC_L2;
+  int64_t qd = 0;
+  read_flat(r, qd, img.coarse_rows, img.quantizer_metric, img.coarse_capacity, img.coarse);
+  read_invlists(r, img.nlist, 4 * img.d, img.list_capacity, img.ids, img.rows);
+  if (img.nlist <= 0) coll_raise("IVF-Flat nlist must be between 1 and 1000000.");
+  if (img.nprobe <= 0) coll_raise("Invalid IndexIVFFlat nprobe.");
+  if (qd != img.d || (img.trained && img.coarse_rows != img.nlist) ||
+      (!img.trained && img.coarse_rows != 0))
+    coll_raise("IndexIVFFlat coarse quantizer header mismatch.");
+  for (const auto &li : img.ids)
+    for (int64_t id : li)
+      if (id < 0 || id > INT32_MAX) coll_raise("IVF-Flat snapshot contains an id outside [0, 2^31 - 1].");
+  return img;
+}
+
+mvg_ivfflat *install_ivfflat(const IvfFlatImage &img) {
+  mvg_ivfflat *raw = nullptr;
+  mvg::check_index(mvg_ivfflat_create_with_quantizer((int)img.d, (int)img.nlist, img.metric,
+                                                     img.quantizer_metric, &raw));
+  std::unique_ptr<mvg_ivfflat, void (*)(mvg_ivfflat *)> h(raw, mvg_ivfflat_destroy);
+  mvg::check_index(mvg_ivfflat_set_nprobe(h.get(), (int)std::min<int64_t>(img.nprobe, INT32_MAX)));
+  if (img.trained) mvg::check_index(mvg_ivfflat_set_centroids(h.get(), img.coarse.data()));
+  for (int64_t l = 0; l < img.nlist; l++) {
+    h->lids[(size_t)l].assign(img.ids[(size_t)l].begin(), img.ids[(size_t)l].end());
+    const auto &b = img.rows[(size_t)l];
+    h->lrows[(size_t)l].resize(b.size() / sizeof(float));
+    if (!b.empty()) std::memcpy(h->lrows[(size_t)l].data(), b.data(), b.size());
+  }
+  h->ntotal = img.ntotal;
+  h->coarse_capacity = img.coarse_capacity;
+  h->list_capacity = img.list_capacity;
   h->dirty = true;
   return h.release();
 }
@@ -512,6 +624,35 @@ int mvg_ivfpq_load(const char *path, mvg_ivfpq **out) {
   Reader r(f.data(), f.size());
   const IvfImage img = read_ivf(r);
   *out = install(img);
+  COLL_END
+}
+
+int mvg_ivfflat_save(mvg_ivfflat *h, const char *path) {
+  COLL_BEGIN
+  if (!h || !path) coll_raise("invalid arguments");
+  std::vector<float> coarse((size_t)h->nlist * h->d, 0.0f);
+  if (h->trained) mvg::check_index(mvg_ivfflat_get_centroids(h, coarse.data()));
+  const int64_t qcap = grown_flat_capacity(h->coarse_capacity, h->trained ? h->nlist : 0);
+  std::vector<int64_t> lcap((size_t)h->nlist);
+  for (int l = 0; l < h->nlist; l++)
+    lcap[(size_t)l] = appended_capacity(h->list_capacity[(size_t)l], (int64_t)h->lids[(size_t)l].size());
+  write_atomically(path, [&](const std::string &tmp) {
+    Writer w(tmp);
+    write_ivfflat(w, h, coarse, qcap, lcap);
+    w.close(false);
+  });
+  h->coarse_capacity = qcap;
+  h->list_capacity = lcap;
+  COLL_END
+}
+
+int mvg_ivfflat_load(const char *path, mvg_ivfflat **out) {
+  COLL_BEGIN
+  if (!path || !out) coll_raise("invalid arguments");
+  MappedFile f(path);
+  Reader r(f.data(), f.size());
+  const IvfFlatImage img = read_ivfflat(r);
+  *out = install_ivfflat(img);
   COLL_END
 }
 

@@ -346,14 +346,12 @@ class IndexIVFFlat:
     """IVF index over exact FP32 rows with an HBM-resident CSR list layout
     (IndexIVFFlat, index_ivf_flat.mojo:10-244)."""
 
-    def __init__(self, d: int, nlist: int, metric: int = METRIC_L2, _handle=None):
-        L = _lib.load()
-        if _handle is None:
-            h = ctypes.c_void_p()
-            _lib.check(L.mvg_ivfflat_create(d, nlist, metric, ctypes.byref(h)))
-        else:
-            h = ctypes.c_void_p(_handle)
-        self._h, self.d, self.nlist, self.metric_type = h, d, nlist, metric
+    def __init__(self, d: int, nlist: int, metric: int = METRIC_L2, quantizer_metric=None):
+        """quantizer_metric: the coarse IndexFlat's metric (default: metric)."""
+        qm = metric if quantizer_metric is None else quantizer_metric
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().mvg_ivfflat_create_with_quantizer(d, nlist, metric, qm, ctypes.byref(h)))
+        self._h, self.d, self.nlist, self.metric_type, self.quantizer_metric = h, d, nlist, metric, qm
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -405,6 +403,23 @@ class IndexIVFFlat:
         if ids.shape != (x.shape[0],):
             raise ValueError("one id per vector is required")
         _lib.check(_lib.load().mvg_ivfflat_add(self._h, _ptr(x), x.shape[0], _ptr(ids)))
+
+    def save(self, path) -> None:
+        """write_index_ivf_flat (serialization.mojo:980-993) into a file."""
+        _lib.check(_lib.load().mvg_ivfflat_save(self._h, os.fspath(path).encode()))
+
+    @classmethod
+    def load(cls, path) -> "IndexIVFFlat":
+        """read_index_ivf_flat (serialization.mojo:995-1020) from a file."""
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(L.mvg_ivfflat_load(os.fspath(path).encode(), ctypes.byref(h)))
+        vals = [ctypes.c_int() for _ in range(4)]
+        _lib.check(L.mvg_ivfflat_shape(h, *[ctypes.byref(v) for v in vals]))
+        ix = cls.__new__(cls)
+        ix._h = h
+        ix.d, ix.nlist, ix.metric_type, ix.quantizer_metric = (v.value for v in vals)
+        return ix
 
     def list_sizes(self):
         s = np.empty(self.nlist, np.int64)

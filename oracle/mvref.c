@@ -989,7 +989,7 @@ void mvref_ivfpq_search(const mvref_ivfpq *h, const float *q, int64_t nq,
  * rows (code_size = 4d, :52) and their ids in append order. */
 
 struct mvref_ivfflat {
-  int d, nlist, metric, trained;
+  int d, nlist, metric, qmetric, trained; /* qmetric: the quantizer IndexFlat's */
   float *coarse; /* nlist x d */
   int64_t ntotal;
   int64_t *lsize, *lcap;
@@ -997,9 +997,11 @@ struct mvref_ivfflat {
   float **lrows;
 };
 
-mvref_ivfflat *mvref_ivfflat_create(int d, int nlist, int metric) {
+/* IndexIVFFlat[IndexFlat](IndexFlat(d, quantizer_metric), d, nlist, metric)
+ * (:29-52): the quantizer keeps its own metric. */
+mvref_ivfflat *mvref_ivfflat_create(int d, int nlist, int metric, int quantizer_metric) {
   mvref_ivfflat *h = (mvref_ivfflat *)calloc(1, sizeof(mvref_ivfflat));
-  h->d = d; h->nlist = nlist; h->metric = metric;
+  h->d = d; h->nlist = nlist; h->metric = metric; h->qmetric = quantizer_metric;
   h->coarse = (float *)calloc((size_t)nlist * d, sizeof(float));
   h->lsize = (int64_t *)calloc((size_t)nlist, sizeof(int64_t));
   h->lcap = (int64_t *)calloc((size_t)nlist, sizeof(int64_t));
@@ -1044,7 +1046,7 @@ void mvref_ivfflat_add(mvref_ivfflat *h, const float *x, int64_t n, const int64_
   if (!h->trained || n == 0) return;
   int d = h->d;
   int32_t *lab = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
-  coarse_assign(h->coarse, h->nlist, d, h->metric, x, n, lab, nthreads);
+  coarse_assign(h->coarse, h->nlist, d, h->qmetric, x, n, lab, nthreads);
   for (int64_t i = 0; i < n; i++) {
     int l = lab[i];
     if (l < 0 || l >= h->nlist) continue; /* :125 */
@@ -1098,7 +1100,7 @@ void mvref_ivfflat_list(const mvref_ivfflat *h, int list, int64_t *ids, float *r
 void mvref_ivfflat_search(const mvref_ivfflat *h, const float *q, int64_t nq, int k,
                           int nprobe, const uint8_t *exclude, int64_t exclude_len,
                           float *D, int64_t *I, int nthreads) {
-  int d = h->d, nlist = h->nlist, ip = h->metric != MVREF_L2;
+  int d = h->d, nlist = h->nlist, ip = h->metric != MVREF_L2, qip = h->qmetric != MVREF_L2;
   if (!h->trained || h->ntotal == 0) {
     for (int64_t i = 0; i < nq * k; i++) { D[i] = IVF_PAD_DIST; I[i] = -1; }
     return;
@@ -1117,7 +1119,7 @@ void mvref_ivfflat_search(const mvref_ivfflat *h, const float *q, int64_t nq, in
       ch_init(&pc, pbuf, np);
       for (int l = 0; l < nlist; l++) {
         const float *c = h->coarse + (int64_t)l * d;
-        ch_offer(&pc, ip ? -mvref_ip_simd(qp, c, d, 8) : mvref_l2_simd(qp, c, d, 8), l);
+        ch_offer(&pc, qip ? -mvref_ip_simd(qp, c, d, 8) : mvref_l2_simd(qp, c, d, 8), l);
       }
       ch_sorted(&pc);
       cheap_t c;

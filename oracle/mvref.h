@@ -113,7 +113,7 @@ void mvref_ivfpq_search(const mvref_ivfpq *h, const float *q, int64_t nq,
 
 /* ---- IVF-Flat (mojovec/index/index_ivf_flat.mojo) ---- */
 typedef struct mvref_ivfflat mvref_ivfflat;
-mvref_ivfflat *mvref_ivfflat_create(int d, int nlist, int metric);
+mvref_ivfflat *mvref_ivfflat_create(int d, int nlist, int metric, int quantizer_metric);
 void mvref_ivfflat_destroy(mvref_ivfflat *h);
 void mvref_ivfflat_set_centroids(mvref_ivfflat *h, const float *coarse);
 void mvref_ivfflat_get_centroids(const mvref_ivfflat *h, float *coarse);

@@ -79,7 +79,7 @@ def lib():
             "mvref_ivfpq_load_lists": (None, [_vp, _i64p, _i32p, _u8p, _vp, _i64]),
             "mvref_ivfpq_list_size": (_i64, [_vp, _int]),
             "mvref_ivfpq_list": (None, [_vp, _int, _i64p, _u8p]),
-            "mvref_ivfflat_create": (_vp, [_int, _int, _int]),
+            "mvref_ivfflat_create": (_vp, [_int, _int, _int, _int]),
             "mvref_ivfflat_destroy": (None, [_vp]),
             "mvref_ivfflat_set_centroids": (None, [_vp, _f32p]),
             "mvref_ivfflat_get_centroids": (None, [_vp, _f32p]),
@@ -365,9 +365,10 @@ class IVFPQIndex:
 
 # ---- IVF-Flat: mojovec/index/index_ivf_flat.mojo ----
 class IVFFlatIndex:
-    def __init__(self, d, nlist, metric=L2):
+    def __init__(self, d, nlist, metric=L2, quantizer_metric=None):
         self.d, self.nlist, self.metric = d, nlist, metric
-        self.h = lib().mvref_ivfflat_create(d, nlist, metric)
+        qm = metric if quantizer_metric is None else quantizer_metric
+        self.h = lib().mvref_ivfflat_create(d, nlist, metric, qm)
 
     def __del__(self):
         if getattr(self, "h", None):

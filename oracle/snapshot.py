@@ -1,4 +1,5 @@
-"""Restatement of MojoVec's IVF-PQ snapshot file format, for tests only.
+"""Restatement of MojoVec's IVF-PQ and IVF-Flat snapshot file formats, for
+tests only.
 
 TEST INFRASTRUCTURE. Only tests/ may import this module; the product reads
 and writes snapshots in mojovec_amd/csrc/snapshot.cpp.
@@ -12,6 +13,10 @@ Layout (all Int fields 8-byte little-endian, Bool one byte, floats raw FP32):
   IndexIVFPQ  mojovec/io/serialization.mojo:1029-1044
       magic 0x4d4a4f50, d, nlist, M, nprobe, ntotal, is_trained (Bool),
       metric (0 L2 / 1 IP), IndexFlat, ArrayInvertedLists, ProductQuantizer
+  IndexIVFFlat  serialization.mojo:980-993
+      magic 0x4d4a4f49, d, nlist, nprobe, ntotal, is_trained (Bool), metric,
+      IndexFlat (with the quantizer's own metric), ArrayInvertedLists with
+      code_size 4*d whose codes are the raw FP32 rows
   IndexFlat   serialization.mojo:229-238
       magic 0x4d4a4f46, d, ntotal, capacity, metric, capacity*d floats
   invlists    serialization.mojo:903-914
@@ -36,6 +41,7 @@ import struct
 import numpy as np
 
 MAGIC_FLAT = 0x4D4A4F46
+MAGIC_IVF_FLAT = 0x4D4A4F49
 MAGIC_IVF_PQ = 0x4D4A4F50
 MAGIC_INVLISTS = 0x4D4A4F4C
 MAGIC_PQ = 0x4D4A4F51
@@ -113,6 +119,56 @@ def index_bytes(d, nlist, M, nprobe, ntotal, trained, metric, coarse, lists, pq,
     p = np.asarray(pq, np.float32).reshape(-1) if trained else np.zeros(d * 256, np.float32)
     out += p.astype("<f4").tobytes()
     return bytes(out)
+
+
+def ivfflat_bytes(d, nlist, nprobe, ntotal, trained, metric, coarse, lists, quantizer_metric=None,
+                  coarse_capacity=None, list_caps=None) -> bytes:
+    """write_index_ivf_flat. `lists` is a sequence of (ids int64[n], rows
+    float32[n, d]); coarse [nlist, d] (ignored when untrained)."""
+    qm = metric if quantizer_metric is None else quantizer_metric
+    out = bytearray()
+    out += _i64(MAGIC_IVF_FLAT) + _i64(d) + _i64(nlist) + _i64(nprobe) + _i64(ntotal)
+    out += bytes([1 if trained else 0]) + _i64(metric)
+    rows = nlist if trained else 0
+    cap = flat_capacity(rows) if coarse_capacity is None else coarse_capacity
+    out += _i64(MAGIC_FLAT) + _i64(d) + _i64(rows) + _i64(cap) + _i64(qm)
+    cb = np.zeros((cap, d), np.float32)
+    if rows:
+        cb[:rows] = np.asarray(coarse, np.float32).reshape(rows, d)
+    out += cb.tobytes()
+    out += _i64(MAGIC_INVLISTS) + _i64(nlist) + _i64(4 * d)
+    for l, (ids, vecs) in enumerate(lists):
+        ids = np.asarray(ids, np.int64)
+        vecs = np.asarray(vecs, np.float32).reshape(ids.size, d)
+        c = list_capacity(ids.size) if list_caps is None else list_caps[l]
+        out += _i64(ids.size) + _i64(c) + ids.astype("<i8").tobytes() + vecs.astype("<f4").tobytes()
+    return bytes(out)
+
+
+def parse_ivfflat(buf: bytes) -> dict:
+    """Parses an IndexIVFFlat file into plain arrays."""
+    cur = _Cursor(buf)
+    assert cur.i64() == MAGIC_IVF_FLAT
+    r = {k: cur.i64() for k in ("d", "nlist", "nprobe", "ntotal")}
+    r["trained"] = cur.byte()
+    r["metric"] = cur.i64()
+    d = r["d"]
+    assert cur.i64() == MAGIC_FLAT
+    r["q_d"], r["q_rows"], r["q_capacity"], r["q_metric"] = (cur.i64() for _ in range(4))
+    coarse = np.frombuffer(cur.take(4 * r["q_capacity"] * d), "<f4").reshape(r["q_capacity"], d)
+    r["coarse"], r["coarse_tail"] = coarse[: r["q_rows"]], coarse[r["q_rows"]:]
+    assert cur.i64() == MAGIC_INVLISTS
+    r["l_nlist"], r["code_size"] = cur.i64(), cur.i64()
+    lists, caps = [], []
+    for _ in range(r["l_nlist"]):
+        size, cap = cur.i64(), cur.i64()
+        ids = np.frombuffer(cur.take(8 * size), "<i8")
+        vecs = np.frombuffer(cur.take(size * r["code_size"]), "<f4").reshape(size, d)
+        lists.append((ids, vecs))
+        caps.append(cap)
+    r["lists"], r["list_caps"] = lists, caps
+    assert cur.pos == len(buf)
+    return r
 
 
 def collection_bytes(dimension, name, metric_name, user_ids, index: bytes) -> bytes:

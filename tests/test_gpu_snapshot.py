@@ -161,3 +161,74 @@ def test_index_level_save_load(gpu, tmp_path):
     assert r["ntotal"] == 2000 and r["nprobe"] == 5
     with pytest.raises(Exception, match="k_factor"):
         ld.search(q, 10, k_factor=2)  # snapshots carry no FP32 rows
+
+
+# ---------------------------------------------------------------- IVF-Flat
+@pytest.mark.parametrize("metric,qmetric", [(0, 0), (1, 1), (1, 0)])
+def test_ivfflat_save_load_round_trip(gpu, tmp_path, metric, qmetric):
+    d, nlist = 32, 16
+    x = clustered(21, 3000, d)
+    gi = gpu.IndexIVFFlat(d, nlist, metric, quantizer_metric=qmetric)
+    gi.train(x, niter=4)
+    gi.add(x)
+    gi.nprobe = 5
+    q = clustered(22, 25, d)
+    D0, I0 = gi.search(q, 10)
+    gi.save(tmp_path / "ivf.bin")
+    ld = gpu.IndexIVFFlat.load(tmp_path / "ivf.bin")
+    assert (ld.d, ld.nlist, ld.metric_type, ld.quantizer_metric) == (d, nlist, metric, qmetric)
+    assert (ld.nprobe, ld.ntotal, ld.is_trained) == (5, 3000, True)
+    D1, I1 = ld.search(q, 10)
+    assert np.array_equal(I0, I1) and np.array_equal(D0, D1)
+    # the oracle parses the file and, given its content, searches the same
+    r = osnap.parse_ivfflat((tmp_path / "ivf.bin").read_bytes())
+    assert r["metric"] == metric and r["q_metric"] == qmetric and r["code_size"] == 4 * d
+    assert r["q_capacity"] == 1024 and not r["coarse_tail"].any()
+    oi = o.IVFFlatIndex(d, nlist, metric, quantizer_metric=qmetric)
+    oi.set_centroids(r["coarse"])
+    for ids, vecs in r["lists"]:
+        oi.add(np.asarray(vecs), ids=np.asarray(ids))  # re-adding keeps each row's list
+    Do, Io = oi.search(q, 10, 5)
+    assert np.array_equal(I0, Io) and np.array_equal(D0, Do)
+    # load -> save writes the same bytes
+    ld.save(tmp_path / "again.bin")
+    assert (tmp_path / "again.bin").read_bytes() == (tmp_path / "ivf.bin").read_bytes()
+
+
+def test_ivfflat_oracle_written_file_loads_and_searches(gpu, tmp_path):
+    d, nlist = 16, 8
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal((900, d)).astype(np.float32)
+    oi = o.IVFFlatIndex(d, nlist)
+    oi.train(x, niter=3)
+    ids = rng.permutation(5000)[:900]
+    oi.add(x, ids=ids)
+    lists = [oi.list(l) for l in range(nlist)]
+    caps = [osnap.list_capacity(len(li[0])) * 2 for li in lists]  # a grown capacity
+    data = osnap.ivfflat_bytes(d, nlist, 3, 900, True, 0, oi.centroids(), lists,
+                               coarse_capacity=4096, list_caps=caps)
+    (tmp_path / "o.bin").write_bytes(data)
+    gi = gpu.IndexIVFFlat.load(tmp_path / "o.bin")
+    q = rng.standard_normal((30, d)).astype(np.float32)
+    Dg, Ig = gi.search(q, 20)
+    Do, Io = oi.search(q, 20, 3)
+    assert np.array_equal(Ig, Io) and np.array_equal(Dg, Do)
+    gi.save(tmp_path / "g.bin")
+    assert (tmp_path / "g.bin").read_bytes() == data  # capacities kept
+    gi.add(x[:40])  # sequential ids continue from ntotal; capacities grow by the rules
+    gi.save(tmp_path / "g2.bin")
+    r = osnap.parse_ivfflat((tmp_path / "g2.bin").read_bytes())
+    assert r["ntotal"] == 940
+    for l in range(nlist):
+        assert r["list_caps"][l] == osnap.list_capacity(len(r["lists"][l][0]), caps[l])
+
+
+def test_ivfflat_untrained_save_load(gpu, tmp_path):
+    gi = gpu.IndexIVFFlat(8, 4)
+    gi.save(tmp_path / "u.bin")
+    r = osnap.parse_ivfflat((tmp_path / "u.bin").read_bytes())
+    assert r["trained"] == 0 and r["q_rows"] == 0 and r["q_capacity"] == 1024
+    ld = gpu.IndexIVFFlat.load(tmp_path / "u.bin")
+    assert not ld.is_trained and ld.ntotal == 0
+    D, I = ld.search(np.zeros((2, 8), np.float32), 3)
+    assert (I == -1).all() and (D == np.float32(1e38)).all()

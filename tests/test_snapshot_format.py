@@ -206,3 +206,62 @@ def test_oracle_capacity_rules():
     assert osnap.list_capacity(0) == 0
     assert osnap.flat_capacity(1024) == 1024 and osnap.flat_capacity(1025) == 2048
     assert osnap.flat_capacity(16384) == 16384
+
+
+# ---------------------------------------------------------------- IVF-Flat files
+def make_ivfflat(**over):
+    rng = np.random.default_rng(3)
+    ids = np.arange(10)
+    lists = [(ids[l::NLIST], rng.standard_normal((len(ids[l::NLIST]), D))) for l in range(NLIST)]
+    args = dict(d=D, nlist=NLIST, nprobe=2, ntotal=10, trained=True, metric=0,
+                coarse=rng.standard_normal((NLIST, D)), lists=lists)
+    args.update(over)
+    return osnap.ivfflat_bytes(**args)
+
+
+def ivfflat_error(tmp_path, data: bytes) -> str:
+    p = tmp_path / "ix.ivfflat"
+    p.write_bytes(data)
+    h = ctypes.c_void_p()
+    rc = _lib.load().mvg_ivfflat_load(str(p).encode(), ctypes.byref(h))
+    assert rc != _lib.MVG_OK, "crafted IVF-Flat file was accepted"
+    return _lib.load().mvg_last_error().decode()
+
+
+def test_ivfflat_load_checks(tmp_path):
+    # read_index_ivf_flat (serialization.mojo:995-1020) and its parts
+    data = make_ivfflat()
+    assert ivfflat_error(tmp_path, patch_i64(data, 0, 1)) == "Invalid magic for IndexIVFFlat"
+    assert ivfflat_error(tmp_path, patch_i64(data, 8, 0)) == "dimension must be between 1 and 65536."
+    assert ivfflat_error(tmp_path, patch_i64(data, 16, 1000001)).startswith("Security Error")
+    assert ivfflat_error(tmp_path, patch_i64(data, 24, 1000001)).startswith("Security Error")
+    assert ivfflat_error(tmp_path, patch_i64(data, 32, -1)).startswith("Security Error")
+    b = bytearray(data)
+    b[40] = 3
+    assert ivfflat_error(tmp_path, bytes(b)) == "Invalid serialized Bool value."
+    q = 49  # the quantizer IndexFlat
+    assert ivfflat_error(tmp_path, patch_i64(data, q, 5)) == "Invalid magic for IndexFlat"
+    inv = q + 40 + 4 * 1024 * D
+    assert ivfflat_error(tmp_path, patch_i64(data, inv, 9)) == "Invalid magic for ArrayInvertedLists"
+    assert ivfflat_error(tmp_path, patch_i64(data, inv + 16, 4 * D + 1)) == \
+        "Inverted-list shape does not match its index header."
+    assert ivfflat_error(tmp_path, data[:-3]) == "Unexpected end of serialized data."
+    # checks beyond the reference reader: a quantizer contradicting the
+    # header, nprobe 0, ids the device lists cannot hold
+    assert ivfflat_error(tmp_path, patch_i64(data, q + 16, NLIST - 1)) == \
+        "IndexIVFFlat coarse quantizer header mismatch."
+    assert ivfflat_error(tmp_path, patch_i64(data, 24, 0)) == "Invalid IndexIVFFlat nprobe."
+    rng = np.random.default_rng(4)
+    lists = [(np.array([1 << 31]), rng.standard_normal((1, D)))] + \
+        [(np.arange(0), np.zeros((0, D)))] * (NLIST - 1)
+    assert ivfflat_error(tmp_path, make_ivfflat(lists=lists)) == \
+        "IVF-Flat snapshot contains an id outside [0, 2^31 - 1]."
+
+
+def test_ivfflat_oracle_layout_sizes():
+    # header 49 bytes, IndexFlat 40 + 4 * capacity * d, invlists 24 + per
+    # list 16 + 8 * size + 4 * d * size
+    data = make_ivfflat()
+    assert len(data) == 49 + 40 + 4 * 1024 * D + 24 + NLIST * 16 + 10 * (8 + 4 * D)
+    r = osnap.parse_ivfflat(data)
+    assert r["code_size"] == 4 * D and r["q_capacity"] == 1024 and r["list_caps"] == [16] * NLIST
