@@ -198,20 +198,21 @@ def main(argv=None):
 
     def profiled(form, nprobe, kf, reps=5):
         """Stage times from HIP events on the search stream: coarse, scan,
-        merge + rerank; and the list entries the scan visits."""
+        merge + rerank; the list entries the scan visits and the keys the
+        deep-pool stream scan writes to HBM."""
         _lib.check(L.mvg_ivfpq_set_profiling(ix._h, 1))
         stage = (ctypes.c_float * 3)()
-        scanned = ctypes.c_int64()
+        scanned, pooled = ctypes.c_int64(), ctypes.c_int64()
         rows = []
         search(form, nprobe, kf)
         for _ in range(reps):
             search(form, nprobe, kf)
-            _lib.check(L.mvg_ivfpq_read_profile(ix._h, stage, ctypes.byref(scanned)))
-            rows.append((stage[0], stage[1], stage[2], scanned.value))
+            _lib.check(L.mvg_ivfpq_read_profile(ix._h, stage, ctypes.byref(scanned), ctypes.byref(pooled)))
+            rows.append((stage[0], stage[1], stage[2], scanned.value, pooled.value))
         _lib.check(L.mvg_ivfpq_set_profiling(ix._h, 0))
         r = np.mean(np.asarray(rows, np.float64), axis=0)
         return {"coarse_ms": round(r[0], 4), "scan_ms": round(r[1], 4),
-                "merge_rerank_ms": round(r[2], 4)}, float(r[3])
+                "merge_rerank_ms": round(r[2], 4), "pool_writes": int(r[4])}, float(r[3])
 
     # headline form: mixed when it reaches the recall bar, else plain
     head = "mixed" if best["mixed"]["recall"] >= RECALL_BAR else "plain"

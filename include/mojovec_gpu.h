@@ -166,11 +166,13 @@ int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item);
 int mvg_ivfpq_set_deep_select(mvg_ivfpq *h, int min_candidates, int pool);
 /* Stage timing of the next searches (no reference counterpart; used by the
  * benchmark): when on, each search records HIP events on its stream around
- * the coarse, scan and merge/rerank stages and counts the list entries the
- * scan visits. read_profile returns the last search's stage_ms[3] and that
- * count. */
+ * the coarse, scan and merge/rerank stages, counts the list entries the scan
+ * visits and the keys the deep-pool stream scan writes to its HBM pools.
+ * read_profile returns the last search's stage_ms[3] and those two counts
+ * (each output nullable). */
 int mvg_ivfpq_set_profiling(mvg_ivfpq *h, int on);
-int mvg_ivfpq_read_profile(mvg_ivfpq *h, float *stage_ms, int64_t *scanned_entries);
+int mvg_ivfpq_read_profile(mvg_ivfpq *h, float *stage_ms, int64_t *scanned_entries,
+                           int64_t *pool_writes);
 /* State: ntotal, trained flag, nprobe. */
 int mvg_ivfpq_info(mvg_ivfpq *h, int64_t *ntotal, int *trained, int *nprobe);
 /* Construction parameters: d, nlist, M, metric (IndexIVFPQ fields d, nlist,

@@ -770,7 +770,8 @@ int mvg_ivfpq_set_profiling(mvg_ivfpq *h, int on) {
   API_END
 }
 
-int mvg_ivfpq_read_profile(mvg_ivfpq *h, float *stage_ms, int64_t *scanned_entries) {
+int mvg_ivfpq_read_profile(mvg_ivfpq *h, float *stage_ms, int64_t *scanned_entries,
+                           int64_t *pool_writes) {
   API_BEGIN
   require(h, "invalid handle");
   auto &pf = h->work.prof;
@@ -778,11 +779,12 @@ int mvg_ivfpq_read_profile(mvg_ivfpq *h, float *stage_ms, int64_t *scanned_entri
   MVG_HIP(hipEventSynchronize(pf.ev[3]));
   for (int i = 0; i < 3; i++)
     if (stage_ms) MVG_HIP(hipEventElapsedTime(&stage_ms[i], pf.ev[i], pf.ev[i + 1]));
-  if (scanned_entries) {  // the count is written on the search's stream after ev[3]
-    unsigned long long v = 0;
-    MVG_HIP(hipMemcpyAsync(&v, pf.counter.ptr, sizeof(v), hipMemcpyDeviceToHost, pf.stream));
+  if (scanned_entries || pool_writes) {  // written on the search's stream, up to after ev[3]
+    unsigned long long v[2] = {0, 0};
+    MVG_HIP(hipMemcpyAsync(v, pf.counter.ptr, sizeof(v), hipMemcpyDeviceToHost, pf.stream));
     MVG_HIP(hipStreamSynchronize(pf.stream));
-    *scanned_entries = (int64_t)v;
+    if (scanned_entries) *scanned_entries = (int64_t)v[0];
+    if (pool_writes) *pool_writes = (int64_t)v[1];
   }
   API_END
 }

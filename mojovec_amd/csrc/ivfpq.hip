@@ -662,7 +662,8 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
 //   3. if T is shared by more keys than are taken, three passes over the
 //      ids of those keys find the largest id taken (lower ids win ties);
 //   4. one walk over the lists emits (key, id) of the winners, which are
-//      sorted in LDS and written out.
+//      sorted in LDS and written out (unsorted when the exact rerank, which
+//      selects over all of them, follows).
 // The workgroups are persistent: each owns one segment of seg_cap keys,
 // where seg_cap is the sum of the max_nprobe largest lists (the most any
 // query can visit), and takes queries from a device counter.
@@ -673,6 +674,8 @@ constexpr int kMfmaCoarseMinLists = 4096;  // mixed form: MFMA coarse above this
 struct DeepShared {
   int sh[16];  // block_excl_scan / digit search scratch
   int cnt, valid, need_eq, ties;
+  int nseg;       // compacted segment: keys written so far
+  uint32_t bound;  // top-digit bound of the keys worth writing (stream scan)
   int64_t q;
   float base;
   uint32_t T, Tid;
@@ -758,6 +761,155 @@ __device__ void deep_key_pass(const uint32_t *__restrict__ seg, int64_t n, int s
   __syncthreads();
 }
 
+__global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
+
+__global__ void fill_zero_kernel(int *p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Query order of the persistent deep-pool scans: queries grouped by their
+// nearest list (a counting sort on probes[q][0]) and handed out from
+// kQueryRanges contiguous ranges, one per XCD (consecutive workgroups are
+// placed on consecutive XCDs), so the queries in flight on one XCD probe
+// mostly the same lists and find their codes in its L2.
+constexpr int kQueryRanges = 8;
+
+__global__ void order_count_kernel(const int32_t *__restrict__ probes, int64_t nq, int maxp,
+                                   int nlist, int *__restrict__ cnt) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  int l = probes[q * maxp];
+  if (l < 0 || l >= nlist) l = 0;
+  atomicAdd(&cnt[l], 1);
+}
+
+// One workgroup: exclusive offsets of the per-list counts (the scatter
+// advances them as cursors); clears the range counters.
+__global__ __launch_bounds__(kNT) void order_offsets_kernel(int *__restrict__ cnt, int nlist,
+                                                            int *__restrict__ off,
+                                                            unsigned *__restrict__ ranges) {
+  __shared__ int sh[kNW + 1];
+  const int tid = threadIdx.x;
+  if (tid < kQueryRanges) ranges[tid] = 0u;
+  int carry = 0;
+  for (int base = 0; base < nlist; base += kNT) {
+    const int l = base + tid;
+    const int c = l < nlist ? cnt[l] : 0;
+    int total;
+    const int ex = block_excl_scan(c, sh, total);
+    if (l < nlist) off[l] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ void order_scatter_kernel(const int32_t *__restrict__ probes, int64_t nq, int maxp,
+                                     int nlist, int *__restrict__ off, int32_t *__restrict__ order) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  int l = probes[q * maxp];
+  if (l < 0 || l >= nlist) l = 0;
+  order[atomicAdd(&off[l], 1)] = (int32_t)q;
+}
+
+// Next query of a persistent workgroup (nq when its range is drained).
+__device__ __forceinline__ int64_t next_query(const int32_t *__restrict__ order, int64_t nq,
+                                              unsigned *__restrict__ ranges) {
+  const int r = blockIdx.x % kQueryRanges;
+  const int64_t beg = nq * r / kQueryRanges, end = nq * (r + 1) / kQueryRanges;
+  const int64_t t = beg + (int64_t)atomicAdd(&ranges[r], 1u);
+  return t < end ? (int64_t)order[t] : nq;
+}
+
+// Appends v to out[] where `take`, through one LDS counter with one atomic
+// per wavefront (called by every active lane); slots at or past cap are
+// dropped.
+__device__ __forceinline__ void wave_append(bool take, uint64_t v, int *counter, uint64_t *out,
+                                            int cap) {
+  const uint64_t m = __ballot(take);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int at = 0;
+  if (lane == leader) at = atomicAdd(counter, (int)__popcll(m));
+  at = __shfl(at, leader, 64);
+  if (take) {
+    const int slot = at + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    if (slot < cap) out[slot] = v;
+  }
+}
+
+// Upper bound on the top 11-bit digit of the kk-th smallest key, from one
+// wavefront's read of the running digit histogram while other waves still
+// add to it: counts only grow (and are exact in every bin at or below the
+// current bound), so the first bin at which the read's running sum reaches
+// kk is at or above the final bin of the kk-th key. 2047 (no bound) while
+// fewer than kk keys are counted.
+__device__ uint32_t wave_digit_bound(const int *hist, int kk) {
+  const int lane = threadIdx.x & 63;
+  const int4 *h4 = reinterpret_cast<const int4 *>(hist + lane * 32);
+  int local = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int4 v = h4[i];
+    local += (v.x + v.y) + (v.z + v.w);
+  }
+  int incl = local;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (__shfl(incl, 63, 64) < kk) return 2047u;
+  const int L = __ffsll((unsigned long long)__ballot(incl >= kk)) - 1;
+  // The 32 bins of lane L, one per lane, on top of the count before them;
+  // re-read counts can only be larger, which keeps the bin a valid bound.
+  const int before = __shfl(incl - local, L, 64);
+  int c = lane < 32 ? hist[L * 32 + lane] : 0;
+#pragma unroll
+  for (int o = 1; o < 32; o <<= 1) {
+    const int y = __shfl_up(c, o, 64);
+    if (lane >= o) c += y;
+  }
+  const uint64_t hit = __ballot(lane < 32 && before + c >= kk);
+  return hit ? (uint32_t)(L * 32 + __ffsll((unsigned long long)hit) - 1) : (uint32_t)(L * 32 + 31);
+}
+
+// One MSD digit pass over a compacted segment of (key << 32 | entry) words:
+// histogram of the key digit at `shift` (nbits wide) of the keys whose bits
+// under `mask` equal `prefix`. The segment is 16-byte aligned.
+__device__ void deep_key_pass64(const uint64_t *__restrict__ seg, int64_t n, int shift, int nbits,
+                                uint32_t prefix, uint32_t mask, int *hist) {
+  const int tid = threadIdx.x, nb = 1 << nbits;
+  const uint32_t dm = (uint32_t)nb - 1u;
+  for (int b = tid; b < nb; b += kNT) hist[b] = 0;
+  __syncthreads();
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(seg);
+  const int64_t n2 = n >> 1;
+  constexpr int U = 4;
+  for (int64_t i0 = tid; i0 < n2; i0 += (int64_t)kNT * U) {
+    uint4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t i = i0 + (int64_t)u * kNT;
+      v[u] = i < n2 ? s4[i] : make_uint4(0u, kSentinel, 0u, kSentinel);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint32_t kv[2] = {v[u].y, v[u].w};  // high words: the keys
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        if (kv[c] != kSentinel && (kv[c] & mask) == prefix) atomicAdd(&hist[(kv[c] >> shift) & dm], 1);
+    }
+  }
+  if ((n & 1) && tid == 0) {
+    const uint32_t key = (uint32_t)(seg[n - 1] >> 32);
+    if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & dm], 1);
+  }
+  __syncthreads();
+}
+
 template <int MF, bool IP, bool MIXED, int DF>
 __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
     const float *__restrict__ q, int64_t nq, int d, int M, const float *__restrict__ coarse,
@@ -767,7 +919,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
     const int32_t *__restrict__ nprobe_q, int nprobe_all, int kk, int k,
     const uint8_t *__restrict__ excl, int64_t excl_len, uint64_t *__restrict__ out_keys,
     float *__restrict__ D, int64_t *__restrict__ I, float pad_dist, uint32_t *__restrict__ seg_base,
-    int64_t seg_cap, unsigned *__restrict__ counter) {
+    int64_t seg_cap, unsigned *__restrict__ counter, const int32_t *__restrict__ order) {
   extern __shared__ __align__(16) unsigned char smem[];
   const DeepLayout lay(M, d, kk, MIXED && !IP);
   float *lut = reinterpret_cast<float *>(smem + lay.lut);
@@ -784,7 +936,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
 
   for (;;) {
     __syncthreads();
-    if (tid == 0) ds.q = (int64_t)atomicAdd(counter, 1u);
+    if (tid == 0) ds.q = next_query(order, nq, counter);
     __syncthreads();
     const int64_t qi = ds.q;
     if (qi >= nq) break;
@@ -931,13 +1083,14 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
 #pragma unroll
           for (int u = 0; u < UC; u++) {
             const uint32_t key = kv[u];
-            if (key == kSentinel || (!take_all && key > T)) continue;
-            const int64_t pos = p0 + (int64_t)u * kNT;
-            const uint32_t id = (uint32_t)ids[off + pos];
-            if (take_all || key < T || id <= Tid) {
-              const int slot = atomicAdd(&ds.cnt, 1);
-              if (slot < kk) out[slot] = ((uint64_t)key << 32) | id;
+            bool take = false;
+            uint32_t id = 0u;
+            if (key != kSentinel && (take_all || key <= T)) {
+              const int64_t pos = p0 + (int64_t)u * kNT;
+              id = (uint32_t)ids[off + pos];
+              take = take_all || key < T || id <= Tid;
             }
+            wave_append(take, ((uint64_t)key << 32) | id, &ds.cnt, out, kk);
           }
         }
         base_i += L;
@@ -945,9 +1098,11 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
     }
     __syncthreads();
     const int cnt = min(ds.cnt, kk);
-    for (int i = cnt + tid; i < np2; i += kNT) out[i] = kSentinel64;
+    // the rerank that reads out_keys selects over all of them: no sort
+    const int fill_to = out_keys ? kk : np2;
+    for (int i = cnt + tid; i < fill_to; i += kNT) out[i] = kSentinel64;
     __syncthreads();
-    bitonic_sort_u64(out, np2);
+    if (!out_keys) bitonic_sort_u64(out, np2);
     if (out_keys) {
       for (int t = tid; t < kk; t += kNT) out_keys[qi * kk + t] = out[t];
     } else {
@@ -967,7 +1122,6 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
   }
 }
 
-__global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
 
 // ---------------------------------------------------------------------------
 // Stream scan: a query's probed lists are scanned as one stream of entries
@@ -979,9 +1133,15 @@ __global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
 //   |q - c_l - r|^2 = |q - c_l|^2 + sum_m T[l][m][k_m] - 2 sum_m <q_m, c_mk>
 // with pre[e] = sum_m T[l][m][code_m] computed once per stored entry, so each
 // entry costs its code, id and pre reads and M LDS lookups. DEEP selects the
-// top-kk once per query from keys written to the workgroup's HBM segment (as
-// ivfpq_deep_scan_kernel); otherwise a running top-kk with a threshold
-// filter (as ivfpq_query_scan_kernel).
+// top-kk once per query with the digit passes of ivfpq_deep_scan_kernel, but
+// over a compacted segment: only keys whose top digit is at most a running
+// bound on the kk-th key's digit are appended to the workgroup's HBM segment
+// as (key << 32 | entry) words and counted in the LDS digit histogram. The
+// bound only tightens (the waves take turns recomputing it from the
+// histogram), so every key at or below it was appended and its bins hold
+// exact counts: the bin of the kk-th key is found from them as from a full
+// histogram. Otherwise a running top-kk with a threshold filter (as
+// ivfpq_query_scan_kernel).
 constexpr int kStreamMaxProbes = 1024;
 
 constexpr int kThreshAppend = 2048;  // appended keys between threshold flushes (mode 2)
@@ -1151,7 +1311,7 @@ __device__ __forceinline__ int64_t stream_entry(const int *poff, const int64_t *
 }
 
 template <int MF, bool IP, int MODE>
-__global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void ivfpq_stream_scan_kernel(
     const float *__restrict__ q, int64_t nq, int d, int M, const float *__restrict__ coarse,
     const float *__restrict__ pq, const float *__restrict__ pre,
     const int64_t *__restrict__ list_off, const uint8_t *__restrict__ codes,
@@ -1159,7 +1319,8 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
     const int32_t *__restrict__ nprobe_q, int nprobe_all, int kk, int k,
     const uint8_t *__restrict__ excl, int64_t excl_len, uint64_t *__restrict__ out_keys,
     float *__restrict__ D, int64_t *__restrict__ I, float pad_dist, uint32_t *__restrict__ seg_base,
-    int64_t seg_cap, unsigned *__restrict__ counter) {
+    int64_t seg_cap, unsigned *__restrict__ counter, unsigned long long *__restrict__ pool_writes,
+    const int32_t *__restrict__ order) {
   extern __shared__ __align__(16) unsigned char smem[];
   // MODE 0: running top-kk, rank / radix merge per flush; 1: deep pool in
   // HBM (DEEP); 2: running candidate set with threshold flushes (THRESH).
@@ -1180,13 +1341,15 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
   int *tsh = reinterpret_cast<int *>(smem + lay.sh);
   const int tid = threadIdx.x, nlut = M * kKsub, dsub = d / M;
   constexpr bool kMixed = !IP;
-  uint32_t *seg = DEEP ? seg_base + (int64_t)blockIdx.x * seg_cap : nullptr;
+  // DEEP: this workgroup's compacted segment of (key << 32 | entry) words
+  uint64_t *seg = DEEP ? reinterpret_cast<uint64_t *>(seg_base) + (int64_t)blockIdx.x * seg_cap : nullptr;
+  const int wave = tid >> 6, lane = tid & 63;
 
   for (;;) {
     int64_t qi;
     if constexpr (DEEP) {
       __syncthreads();
-      if (tid == 0) ds.q = (int64_t)atomicAdd(counter, 1u);
+      if (tid == 0) ds.q = next_query(order, nq, counter);
       __syncthreads();
       qi = ds.q;
       if (qi >= nq) break;
@@ -1201,6 +1364,10 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
     for (int j = tid; j < d; j += kNT) rv[j] = qq[j];
     if constexpr (DEEP) {
       for (int b = tid; b < 2048; b += kNT) hist[b] = 0;
+      if (tid == 0) {
+        ds.nseg = 0;
+        ds.bound = 2047u;
+      }
     } else {
       for (int i = tid; i < kk; i += kNT) cand[i] = kSentinel64;
       if (tid == 0) {
@@ -1250,8 +1417,13 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
     // scan the stream
     int cur = 0;
     constexpr int U = (MF > 0) ? 4 : 1;
+    int round = 0;
     for (int64_t r0 = 0; r0 < n; r0 += (int64_t)kNT * U) {
       const uint64_t thr = DEEP ? kSentinel64 : st->thr;
+      // DEEP: keys whose top digit exceeds this bound cannot be among the kk
+      // smallest; only the others are written (the histogram counts all).
+      uint32_t bound = 2047u;
+      if constexpr (DEEP) bound = __hip_atomic_load(&ds.bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       Code<MF> cw[U];
       int32_t idv[U];
       float initv[U];
@@ -1272,27 +1444,62 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
           if (kMixed) initv[u] = initv[u] + pre[e];
         }
       }
+      uint32_t putk[U];  // DEEP: keys to append (sentinel: none)
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int64_t j = r0 + (int64_t)u * kNT + tid;
-        if (j >= n) continue;
-        const int32_t id = idv[u];
-        const bool skip = excl && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
-        float dist = 0.0f;
-        if (!skip) {
-          if constexpr (MF > 0) dist = adc_words<MF>(cw[u], lut, initv[u]);
-          else dist = adc_sum<0>(codes + ev[u] * M, lut, M, initv[u]);
-        }
-        if constexpr (DEEP) {
-          uint32_t key = kSentinel;
+        bool put = false;
+        uint32_t key = kSentinel;
+        if (j < n) {
+          const int32_t id = idv[u];
+          const bool skip = excl && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
+          float dist = 0.0f;
           if (!skip) {
-            key = f2key(dist);
-            atomicAdd(&hist[key >> 21], 1);
+            if constexpr (MF > 0) dist = adc_words<MF>(cw[u], lut, initv[u]);
+            else dist = adc_sum<0>(codes + ev[u] * M, lut, M, initv[u]);
           }
-          seg[j] = key;
-        } else if (!skip) {
-          const uint64_t key = pack_key(dist, (uint32_t)id);
-          if (key < thr) cand[kk + atomicAdd(&st->cnt, 1)] = key;
+          if constexpr (DEEP) {
+            if (!skip) {
+              key = f2key(dist);
+              put = (key >> 21) <= bound;
+              if (put) atomicAdd(&hist[key >> 21], 1);
+            }
+          } else if (!skip) {
+            const uint64_t pk = pack_key(dist, (uint32_t)id);
+            if (pk < thr) cand[kk + atomicAdd(&st->cnt, 1)] = pk;
+          }
+        }
+        putk[u] = put ? key : kSentinel;
+      }
+      if constexpr (DEEP) {
+        // appends: one LDS atomic per wavefront and round; the keys of one u
+        // land in consecutive words in lane order
+        uint64_t mu[U];
+        int wtotal = 0;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          mu[u] = __ballot(putk[u] != kSentinel);
+          wtotal += (int)__popcll(mu[u]);
+        }
+        if (wtotal > 0) {
+          int at = 0;
+          if (lane == 0) at = atomicAdd(&ds.nseg, wtotal);
+          at = __shfl(at, 0, 64);
+#pragma unroll
+          for (int u = 0; u < U; u++) {
+            if (putk[u] != kSentinel) {
+              const int off = (int)__builtin_amdgcn_mbcnt_hi(
+                  (uint32_t)(mu[u] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mu[u], 0u));
+              seg[at + off] = ((uint64_t)putk[u] << 32) | (uint32_t)ev[u];
+            }
+            at += (int)__popcll(mu[u]);
+          }
+        }
+        // the waves take turns tightening the bound, one refresh per four rounds
+        ++round;
+        if ((round & 3) == 0 && ((round >> 2) & (kNW - 1)) == wave) {
+          const uint32_t b = wave_digit_bound(hist, kk);
+          if (lane == 0) __hip_atomic_store(&ds.bound, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
       if constexpr (THRESH) {
@@ -1352,17 +1559,19 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
       __syncthreads();
       const bool take_all = ds.valid <= kk;
       uint32_t T = 0u, Tid = 0xFFFFFFFFu;
+      const int64_t nseg = ds.nseg;  // every key at or below the final bound is here
+      if (pool_writes && tid == 0) atomicAdd(pool_writes, (unsigned long long)nseg);
       if (!take_all) {
         int bin, before, count, remaining = kk;
         deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
         remaining -= before;
         uint32_t prefix = (uint32_t)bin << 21, mask = 0xFFE00000u;
-        deep_key_pass(seg, n, 10, 11, prefix, mask, hist);
+        deep_key_pass64(seg, nseg, 10, 11, prefix, mask, hist);
         deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
         remaining -= before;
         prefix |= (uint32_t)bin << 10;
         mask |= 0x001FFC00u;
-        deep_key_pass(seg, n, 0, 10, prefix, mask, hist);
+        deep_key_pass64(seg, nseg, 0, 10, prefix, mask, hist);
         deep_find_bin(hist, 1024, remaining, ds, bin, before, count);
         remaining -= before;
         T = prefix | (uint32_t)bin;
@@ -1375,11 +1584,10 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
             const uint32_t dm = (uint32_t)nb - 1u;
             for (int b = tid; b < nb; b += kNT) hist[b] = 0;
             __syncthreads();
-            int c2 = 0;
-            for (int64_t j = tid; j < n; j += kNT) {
-              const int64_t e = stream_entry(poff, loff, j, c2);
-              if (seg[j] != T) continue;
-              const uint32_t id = (uint32_t)ids[e];
+            for (int64_t j = tid; j < nseg; j += kNT) {
+              const uint64_t w = seg[j];
+              if ((uint32_t)(w >> 32) != T) continue;
+              const uint32_t id = (uint32_t)ids[(uint32_t)w];
               if ((id & imask) == iprefix) atomicAdd(&hist[(id >> shifts[pass]) & dm], 1);
             }
             __syncthreads();
@@ -1395,49 +1603,45 @@ __global__ __launch_bounds__(kNT) void ivfpq_stream_scan_kernel(
       if (tid == 0) ds.cnt = 0;
       __syncthreads();
       {
-        int c2 = 0;
         const uint4 *s4 = reinterpret_cast<const uint4 *>(seg);
-        const int64_t n4 = n >> 2;
+        const int64_t n2 = nseg >> 1;
         constexpr int UC = 4;
-        for (int64_t i0 = tid; i0 < n4; i0 += (int64_t)kNT * UC) {
+        for (int64_t i0 = tid; i0 < n2 + (nseg & 1); i0 += (int64_t)kNT * UC) {
           uint4 v[UC];
 #pragma unroll
           for (int u = 0; u < UC; u++) {
             const int64_t i = i0 + (int64_t)u * kNT;
-            v[u] = i < n4 ? s4[i] : make_uint4(kSentinel, kSentinel, kSentinel, kSentinel);
+            v[u] = i < n2 ? s4[i] : make_uint4(0u, kSentinel, 0u, kSentinel);
+            if (i == n2 && (nseg & 1)) {  // the odd last word
+              const uint64_t w = seg[nseg - 1];
+              v[u] = make_uint4((uint32_t)w, (uint32_t)(w >> 32), 0u, kSentinel);
+            }
           }
 #pragma unroll
           for (int u = 0; u < UC; u++) {
-            const uint32_t kv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+            const uint32_t kv[2] = {v[u].y, v[u].w}, ev2[2] = {v[u].x, v[u].z};
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
+            for (int c = 0; c < 2; c++) {
               const uint32_t key = kv[c];
-              if (key == kSentinel || (!take_all && key > T)) continue;
-              const int64_t j = ((i0 + (int64_t)u * kNT) << 2) + c;
-              const uint32_t id = (uint32_t)ids[stream_entry(poff, loff, j, c2)];
-              if (take_all || key < T || id <= Tid) {
-                const int slot = atomicAdd(&ds.cnt, 1);
-                if (slot < kk) cand[slot] = ((uint64_t)key << 32) | id;
+              bool take = false;
+              uint32_t id = 0u;
+              if (key != kSentinel && (take_all || key <= T)) {
+                id = (uint32_t)ids[ev2[c]];
+                take = take_all || key < T || id <= Tid;
               }
+              wave_append(take, ((uint64_t)key << 32) | id, &ds.cnt, cand, kk);
             }
-          }
-        }
-        for (int64_t j = (n4 << 2) + tid; j < n; j += kNT) {
-          const uint32_t key = seg[j];
-          if (key == kSentinel || (!take_all && key > T)) continue;
-          const uint32_t id = (uint32_t)ids[stream_entry(poff, loff, j, c2)];
-          if (take_all || key < T || id <= Tid) {
-            const int slot = atomicAdd(&ds.cnt, 1);
-            if (slot < kk) cand[slot] = ((uint64_t)key << 32) | id;
           }
         }
       }
       __syncthreads();
       const int cnt = min(ds.cnt, kk);
       const int np2 = next_pow2(kk);
-      for (int i = cnt + tid; i < np2; i += kNT) cand[i] = kSentinel64;
+      // the rerank that reads out_keys selects over all of them: no sort
+      const int fill_to = out_keys ? kk : np2;
+      for (int i = cnt + tid; i < fill_to; i += kNT) cand[i] = kSentinel64;
       __syncthreads();
-      bitonic_sort_u64(cand, np2);
+      if (!out_keys) bitonic_sort_u64(cand, np2);
       if (out_keys) {
         for (int t = tid; t < kk; t += kNT) out_keys[qi * kk + t] = cand[t];
       } else {
@@ -1582,6 +1786,36 @@ void launch_query_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, 
   launch_query_scan_df<MF, IP, MIXED, 0>(a, maxp, probes, kk, out_keys, wq, s);
 }
 
+// Builds the query order and clears the range counters of a persistent
+// deep-pool scan; returns the order.
+const int32_t *build_query_order(const IvfSearchArgs &a, int maxp, const int32_t *probes,
+                                 IvfWork &w, hipStream_t s) {
+  w.ord_cnt.reserve(sizeof(int) * (size_t)a.nlist);
+  w.ord_off.reserve(sizeof(int) * (size_t)a.nlist);
+  w.ord.reserve(sizeof(int32_t) * (size_t)a.nq);
+  w.deep_counter.reserve(sizeof(unsigned) * kQueryRanges);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(ceil_div(a.nlist, 256)), dim3(256), 0, s,
+                     w.ord_cnt.as<int>(), (int64_t)a.nlist);
+  MVG_LAUNCH_CHECK("fill_zero_kernel");
+  hipLaunchKernelGGL(order_count_kernel, dim3(ceil_div(a.nq, 256)), dim3(256), 0, s, probes, a.nq,
+                     maxp, a.nlist, w.ord_cnt.as<int>());
+  MVG_LAUNCH_CHECK("order_count_kernel");
+  hipLaunchKernelGGL(order_offsets_kernel, dim3(1), dim3(kNT), 0, s, w.ord_cnt.as<int>(), a.nlist,
+                     w.ord_off.as<int>(), w.deep_counter.as<unsigned>());
+  MVG_LAUNCH_CHECK("order_offsets_kernel");
+  hipLaunchKernelGGL(order_scatter_kernel, dim3(ceil_div(a.nq, 256)), dim3(256), 0, s, probes, a.nq,
+                     maxp, a.nlist, w.ord_off.as<int>(), w.ord.as<int32_t>());
+  MVG_LAUNCH_CHECK("order_scatter_kernel");
+  return w.ord.as<int32_t>();
+}
+
+// Persistent grid: every XCD range needs workgroups, so a multiple of
+// kQueryRanges (at least one per range).
+int64_t persistent_grid(int per_cu, int64_t limit) {
+  int64_t g = std::min<int64_t>((int64_t)std::max(1, per_cu) * device_cus(), limit);
+  return std::max<int64_t>(kQueryRanges, g / kQueryRanges * kQueryRanges);
+}
+
 template <int MF, bool IP, bool MIXED, int DF>
 void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                     uint64_t *out_keys, IvfWork &w, hipStream_t s) {
@@ -1594,18 +1828,15 @@ void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int
   int per_cu = 0;
   MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
   const int64_t cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);  // 16-byte aligned segments
-  int64_t grid = std::min<int64_t>((int64_t)std::max(1, per_cu) * device_cus(), a.nq);
   // at most 4 GiB of key segments: fewer persistent workgroups beyond that
-  grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(4) << 30) / (4 * cap)));
+  const int64_t grid = persistent_grid(per_cu, (int64_t(4) << 30) / (4 * cap));
   w.deep_seg.reserve(sizeof(uint32_t) * (size_t)(grid * cap));
-  w.deep_counter.reserve(sizeof(unsigned));
+  const int32_t *order = build_query_order(a, maxp, probes, w, s);
   unsigned *counter = w.deep_counter.as<unsigned>();
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, counter);
-  MVG_LAUNCH_CHECK("reset_counter_kernel");
   hipLaunchKernelGGL((ivfpq_deep_scan_kernel<MF, IP, MIXED, DF>), dim3((unsigned)grid), dim3(kNT), sm,
                      s, a.q, a.nq, a.d, a.M, a.coarse, a.pq, a.ptab, a.list_off, a.codes, a.ids,
                      probes, maxp, a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len,
-                     out_keys, a.D, a.I, kIvfPadDist, w.deep_seg.as<uint32_t>(), cap, counter);
+                     out_keys, a.D, a.I, kIvfPadDist, w.deep_seg.as<uint32_t>(), cap, counter, order);
   MVG_LAUNCH_CHECK("ivfpq_deep_scan_kernel");
 }
 
@@ -1624,23 +1855,23 @@ bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int 
   int64_t grid = a.nq, cap = INT32_MAX;
   uint32_t *seg = nullptr;
   unsigned *counter = nullptr;
+  const int32_t *order = nullptr;
   if (DEEP) {
     int per_cu = 0;
     MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
     cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);  // 16-byte aligned segments
-    grid = std::min<int64_t>((int64_t)std::max(1, per_cu) * device_cus(), a.nq);
-    grid = std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t(4) << 30) / (4 * cap)));
-    w.deep_seg.reserve(sizeof(uint32_t) * (size_t)(grid * cap));
-    w.deep_counter.reserve(sizeof(unsigned));
+    // at most 4 GiB of segments of 8-byte (key, entry) words
+    grid = persistent_grid(per_cu, (int64_t(4) << 30) / (8 * cap));
+    w.deep_seg.reserve(sizeof(uint64_t) * (size_t)(grid * cap));
     seg = w.deep_seg.as<uint32_t>();
+    order = build_query_order(a, maxp, probes, w, s);
     counter = w.deep_counter.as<unsigned>();
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, counter);
-    MVG_LAUNCH_CHECK("reset_counter_kernel");
   }
   hipLaunchKernelGGL((ivfpq_stream_scan_kernel<MF, IP, MODE>), dim3((unsigned)grid), dim3(kNT), sm, s,
                      a.q, a.nq, a.d, a.M, a.coarse, a.pq, a.pre, a.list_off, a.codes, a.ids, probes,
                      maxp, a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D,
-                     a.I, kIvfPadDist, seg, cap, counter);
+                     a.I, kIvfPadDist, seg, cap, counter,
+                     DEEP && w.prof.on ? w.prof.counter.as<unsigned long long>() + 1 : nullptr, order);
   MVG_LAUNCH_CHECK("ivfpq_stream_scan_kernel");
   return true;
 }
@@ -1749,6 +1980,9 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   if (pf.on) {
     for (auto &e : pf.ev)
       if (!e) MVG_HIP(hipEventCreate(&e));
+    // [0] list entries the scan visits, [1] keys written to deep pools
+    pf.counter.reserve(2 * sizeof(unsigned long long));
+    MVG_HIP(hipMemsetAsync(pf.counter.ptr, 0, 2 * sizeof(unsigned long long), s));
     MVG_HIP(hipEventRecord(pf.ev[0], s));
   }
   // 1. coarse top-maxp (IndexFlat.search over the centroids, :264).
@@ -1895,8 +2129,6 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   }
   if (pf.on) {
     MVG_HIP(hipEventRecord(pf.ev[3], s));
-    pf.counter.reserve(sizeof(unsigned long long));
-    MVG_HIP(hipMemsetAsync(pf.counter.ptr, 0, sizeof(unsigned long long), s));
     const int64_t nt = a.nq * maxp;
     hipLaunchKernelGGL(scanned_entries_kernel, dim3(ceil_div(nt, 256)), dim3(256), 0, s, probes, a.nq,
                        maxp, a.nprobe_per_query, a.nprobe, a.list_off,

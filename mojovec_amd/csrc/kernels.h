@@ -89,12 +89,13 @@ struct IvfProfile {
 struct IvfWork {
   DevBuf coarse_part, probes, part, keys, qbf, qnorm;
   DevBuf wq_poff, wq_nitems, wq_rq, wq_off, wq_gthr, wq_counter;
-  DevBuf deep_seg, deep_counter;  // deep candidate pools: key segments, query counter
+  DevBuf deep_seg, deep_counter;  // deep candidate pools: key segments, query range counters
+  DevBuf ord_cnt, ord_off, ord;    // query order of the persistent deep scans
   IvfProfile prof;
 };
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
 
-// Exact FP32 rerank of kk sorted candidate keys per query to the top-k
+// Exact FP32 rerank of kk candidate keys per query (any order) to the top-k
 // (index_hnsw.mojo:449-496 generalised to k_factor).
 // Candidate ids at or above nrows (not rows of `vectors`) are dropped.
 void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors, int64_t nrows,
