@@ -80,6 +80,20 @@ __device__ __forceinline__ float adc_words(const Code<MF> &c, const float *lut, 
   return dist;
 }
 
+// adc_words of U entries at once: each entry's sum keeps the m order, and the
+// U independent chains interleave so that U LUT lookups are in flight.
+template <int MF, int U>
+__device__ __forceinline__ void adc_words_multi(const Code<MF> (&c)[U], const float *lut,
+                                                const float (&init)[U], float (&out)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; u++) out[u] = init[u];
+#pragma unroll
+  for (int m = 0; m < MF; m++) {
+#pragma unroll
+    for (int u = 0; u < U; u++) out[u] += lut[m * kKsub + ((c[u].w[m >> 2] >> (8 * (m & 3))) & 0xFF)];
+  }
+}
+
 // One workgroup per (query, probe rank).
 template <int MF, bool IP>
 __global__ __launch_bounds__(kNT) void ivfpq_scan_kernel(
@@ -1421,7 +1435,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
     for (int64_t r0 = 0; r0 < n; r0 += (int64_t)kNT * U) {
       const uint64_t thr = DEEP ? kSentinel64 : st->thr;
       // DEEP: keys whose top digit exceeds this bound cannot be among the kk
-      // smallest; only the others are written (the histogram counts all).
+      // smallest; only the others are written and counted.
       uint32_t bound = 2047u;
       if constexpr (DEEP) bound = __hip_atomic_load(&ds.bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       Code<MF> cw[U];
@@ -1444,6 +1458,10 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
           if (kMixed) initv[u] = initv[u] + pre[e];
         }
       }
+      // Fixed M: the U entries' sums without branches (entries past n score
+      // zero codes and are dropped below), so their lookups interleave.
+      float distv[U];
+      if constexpr (MF > 0) adc_words_multi<MF, U>(cw, lut, initv, distv);
       uint32_t putk[U];  // DEEP: keys to append (sentinel: none)
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -1455,7 +1473,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
           const bool skip = excl && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
           float dist = 0.0f;
           if (!skip) {
-            if constexpr (MF > 0) dist = adc_words<MF>(cw[u], lut, initv[u]);
+            if constexpr (MF > 0) dist = distv[u];
             else dist = adc_sum<0>(codes + ev[u] * M, lut, M, initv[u]);
           }
           if constexpr (DEEP) {
