@@ -668,19 +668,24 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
 // Deep candidate pools (k * k_factor in the thousands): a running top-kk
 // would re-select and re-sort thousands of keys on every buffer flush, so
 // the selection is done once per query instead.
-//   1. scan: every entry's u32 distance key is written, in probe order, to
-//      the workgroup's private key segment in HBM, and the top 11-bit digit
-//      of each key is histogrammed in LDS;
+//   1. scan: an entry's u32 distance key is appended, with its CSR entry
+//      index, to the workgroup's private segment in HBM -- and its top
+//      11-bit digit counted in an LDS histogram -- when that digit is at most
+//      a running bound on the digit of the kk-th smallest key; the bound is
+//      recomputed from the histogram as the scan goes and only tightens, so
+//      every key that can be among the kk smallest is appended (a fraction
+//      of the entries once it settles);
 //   2. two more digit passes over the segment find the kk-th smallest key T
 //      and how many keys equal to T it takes (MSD radix select);
 //   3. if T is shared by more keys than are taken, three passes over the
 //      ids of those keys find the largest id taken (lower ids win ties);
-//   4. one walk over the lists emits (key, id) of the winners, which are
+//   4. one pass over the segment emits (key, id) of the winners, which are
 //      sorted in LDS and written out (unsorted when the exact rerank, which
 //      selects over all of them, follows).
-// The workgroups are persistent: each owns one segment of seg_cap keys,
+// The workgroups are persistent: each owns one segment of seg_cap words,
 // where seg_cap is the sum of the max_nprobe largest lists (the most any
-// query can visit), and takes queries from a device counter.
+// query can visit), and takes queries from per-XCD ranges of a clustered
+// query order (next_query).
 constexpr int kDeepMinCandidates = 513;  // default: the deep form for k * k_factor >= this
 constexpr int kDeepBatch = 4;            // list entries per thread with loads in flight
 constexpr int kMfmaCoarseMinLists = 4096;  // mixed form: MFMA coarse above this nlist
@@ -741,39 +746,6 @@ __device__ void deep_find_bin(const int *hist, int nb, int remaining, DeepShared
   __syncthreads();
 }
 
-// One MSD digit pass over the segment: histogram of the digit at `shift`
-// (nbits wide) of the valid keys whose bits under `mask` equal `prefix`.
-// The segment is 16-byte aligned; each thread keeps 4 x 4 keys in flight.
-__device__ void deep_key_pass(const uint32_t *__restrict__ seg, int64_t n, int shift, int nbits,
-                              uint32_t prefix, uint32_t mask, int *hist) {
-  const int tid = threadIdx.x, nb = 1 << nbits;
-  const uint32_t dm = (uint32_t)nb - 1u;
-  for (int b = tid; b < nb; b += kNT) hist[b] = 0;
-  __syncthreads();
-  const uint4 *s4 = reinterpret_cast<const uint4 *>(seg);
-  const int64_t n4 = n >> 2;
-  constexpr int U = 4;
-  for (int64_t i0 = tid; i0 < n4; i0 += (int64_t)kNT * U) {
-    uint4 v[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int64_t i = i0 + (int64_t)u * kNT;
-      v[u] = i < n4 ? s4[i] : make_uint4(kSentinel, kSentinel, kSentinel, kSentinel);
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const uint32_t kv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-#pragma unroll
-      for (int c = 0; c < 4; c++)
-        if (kv[c] != kSentinel && (kv[c] & mask) == prefix) atomicAdd(&hist[(kv[c] >> shift) & dm], 1);
-    }
-  }
-  for (int64_t i = (n4 << 2) + tid; i < n; i += kNT) {
-    const uint32_t key = seg[i];
-    if (key != kSentinel && (key & mask) == prefix) atomicAdd(&hist[(key >> shift) & dm], 1);
-  }
-  __syncthreads();
-}
 
 __global__ void reset_counter_kernel(unsigned *counter) { *counter = 0u; }
 
@@ -924,6 +896,102 @@ __device__ void deep_key_pass64(const uint64_t *__restrict__ seg, int64_t n, int
   __syncthreads();
 }
 
+// Selects the kk smallest (key, id) pairs -- ids taken from ids[entry] --
+// of a compacted segment of nseg (key << 32 | entry) words, given the digit
+// histogram of its keys, into out[] in no particular order; ds.cnt ends as
+// the number found (out past min(ds.cnt, kk) is not written). Every key of
+// the query at or below the top-digit bin of the kk-th key must be in the
+// segment: the MSD radix select (digit passes over the segment, id passes
+// for ties at the kk-th key) then runs as over the full key set.
+__device__ void deep_select_segment(const uint64_t *__restrict__ seg, int64_t nseg,
+                                    const int32_t *__restrict__ ids, int kk, int *hist,
+                                    DeepShared &ds, uint64_t *out) {
+  const int tid = threadIdx.x;
+  {
+    int local = 0;
+    for (int b = tid * 8; b < tid * 8 + 8; b++) local += hist[b];
+    const int valid = block_sum(local, ds.sh);
+    if (tid == 0) ds.valid = valid;
+  }
+  __syncthreads();
+  const bool take_all = ds.valid <= kk;
+  uint32_t T = 0u, Tid = 0xFFFFFFFFu;
+  if (!take_all) {
+    int bin, before, count, remaining = kk;
+    deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
+    remaining -= before;
+    uint32_t prefix = (uint32_t)bin << 21, mask = 0xFFE00000u;
+    deep_key_pass64(seg, nseg, 10, 11, prefix, mask, hist);
+    deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
+    remaining -= before;
+    prefix |= (uint32_t)bin << 10;
+    mask |= 0x001FFC00u;
+    deep_key_pass64(seg, nseg, 0, 10, prefix, mask, hist);
+    deep_find_bin(hist, 1024, remaining, ds, bin, before, count);
+    remaining -= before;
+    T = prefix | (uint32_t)bin;
+    if (remaining < count) {  // ties at T: the lowest ids win
+      uint32_t iprefix = 0u, imask = 0u;
+      const int shifts[3] = {21, 10, 0};
+      const int nbits[3] = {11, 11, 10};
+      for (int pass = 0; pass < 3; pass++) {
+        const int nb = 1 << nbits[pass];
+        const uint32_t dm = (uint32_t)nb - 1u;
+        for (int b = tid; b < nb; b += kNT) hist[b] = 0;
+        __syncthreads();
+        for (int64_t j = tid; j < nseg; j += kNT) {
+          const uint64_t w = seg[j];
+          if ((uint32_t)(w >> 32) != T) continue;
+          const uint32_t id = (uint32_t)ids[(uint32_t)w];
+          if ((id & imask) == iprefix) atomicAdd(&hist[(id >> shifts[pass]) & dm], 1);
+        }
+        __syncthreads();
+        int ibin, ibefore, icount;
+        deep_find_bin(hist, nb, remaining, ds, ibin, ibefore, icount);
+        remaining -= ibefore;
+        iprefix |= (uint32_t)ibin << shifts[pass];
+        imask |= dm << shifts[pass];
+      }
+      Tid = iprefix;
+    }
+  }
+  if (tid == 0) ds.cnt = 0;
+  __syncthreads();
+  {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(seg);
+    const int64_t n2 = nseg >> 1;
+    constexpr int UC = 4;
+    for (int64_t i0 = tid; i0 < n2 + (nseg & 1); i0 += (int64_t)kNT * UC) {
+      uint4 v[UC];
+#pragma unroll
+      for (int u = 0; u < UC; u++) {
+        const int64_t i = i0 + (int64_t)u * kNT;
+        v[u] = i < n2 ? s4[i] : make_uint4(0u, kSentinel, 0u, kSentinel);
+        if (i == n2 && (nseg & 1)) {  // the odd last word
+          const uint64_t w = seg[nseg - 1];
+          v[u] = make_uint4((uint32_t)w, (uint32_t)(w >> 32), 0u, kSentinel);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UC; u++) {
+        const uint32_t kv[2] = {v[u].y, v[u].w}, ev2[2] = {v[u].x, v[u].z};
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const uint32_t key = kv[c];
+          bool take = false;
+          uint32_t id = 0u;
+          if (key != kSentinel && (take_all || key <= T)) {
+            id = (uint32_t)ids[ev2[c]];
+            take = take_all || key < T || id <= Tid;
+          }
+          wave_append(take, ((uint64_t)key << 32) | id, &ds.cnt, out, kk);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <int MF, bool IP, bool MIXED, int DF>
 __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
     const float *__restrict__ q, int64_t nq, int d, int M, const float *__restrict__ coarse,
@@ -942,8 +1010,9 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
   uint64_t *out = reinterpret_cast<uint64_t *>(smem + lay.out);
   int *hist = reinterpret_cast<int *>(smem + lay.hist);
   DeepShared &ds = *reinterpret_cast<DeepShared *>(smem + lay.st);
-  uint32_t *seg = seg_base + (int64_t)blockIdx.x * seg_cap;
-  const int tid = threadIdx.x;
+  // compacted segment of (key << 32 | entry) words, as in the stream scan
+  uint64_t *seg = reinterpret_cast<uint64_t *>(seg_base) + (int64_t)blockIdx.x * seg_cap;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int np2 = next_pow2(kk);
   float cbr[LutShape<MF, IP, MIXED, DF>::kRegs];
   load_reg_codebook<MF, IP, MIXED, DF>(cbr, pq);
@@ -958,9 +1027,14 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
     int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
     np = min(max(np, 1), maxp);
     for (int b = tid; b < 2048; b += kNT) hist[b] = 0;
+    if (tid == 0) {
+      ds.nseg = 0;
+      ds.bound = 2047u;
+    }
     build_query_tables<IP, MIXED>(qq, d, M, pq, rv, lut, qtab);
 
-    // 1. scan: keys to the segment, top digit histogram
+    // 1. scan: keys whose top digit is at most the running bound go to the
+    //    compacted segment and the digit histogram (see the stream scan)
     int64_t n = 0;
     for (int p = 0; p < np; p++) {
       const int l = probes[qi * maxp + p];
@@ -970,31 +1044,38 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
       if (L <= 0) continue;
       const float base = build_list_lut<MF, IP, MIXED, DF>(qq, coarse, l, d, M, pq, ptab, rv, lut,
                                                            qtab, cbr, &ds.base);
+      const uint32_t bound = ds.bound;  // set before the LUT build's barrier
       if constexpr (MF > 0) {
         constexpr int U = kDeepBatch;
         for (int64_t p0 = tid; p0 < L; p0 += (int64_t)kNT * U) {
           Code<MF> cw[U];
+          float initv[U], distv[U];
 #pragma unroll
           for (int u = 0; u < U; u++) {  // every load of the batch in flight first
             const int64_t pos = p0 + (int64_t)u * kNT;
             cw[u] = Code<MF>{};
+            initv[u] = base;
             if (pos < L) cw[u] = load_code<MF>(codes, off + pos);
           }
+          adc_words_multi<MF, U>(cw, lut, initv, distv);  // positions past L are dropped
 #pragma unroll
           for (int u = 0; u < U; u++) {
             const int64_t pos = p0 + (int64_t)u * kNT;
-            if (pos >= L) continue;
-            bool skip = false;
-            if (excl) {  // :328-334
-              const int32_t id = ids[off + pos];
-              skip = id >= 0 && id < excl_len && excl[id] != 0;
-            }
+            bool put = false;
             uint32_t key = kSentinel;
-            if (!skip) {
-              key = f2key(adc_words<MF>(cw[u], lut, base));
-              atomicAdd(&hist[key >> 21], 1);
+            if (pos < L) {
+              bool skip = false;
+              if (excl) {  // :328-334
+                const int32_t id = ids[off + pos];
+                skip = id >= 0 && id < excl_len && excl[id] != 0;
+              }
+              if (!skip) {
+                key = f2key(distv[u]);
+                put = (key >> 21) <= bound;
+                if (put) atomicAdd(&hist[key >> 21], 1);
+              }
             }
-            seg[n + pos] = key;
+            wave_append(put, ((uint64_t)key << 32) | (uint32_t)(off + pos), &ds.nseg, seg, INT32_MAX);
           }
         }
       } else {
@@ -1004,113 +1085,28 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
             const int32_t id = ids[off + pos];
             skip = id >= 0 && id < excl_len && excl[id] != 0;
           }
+          bool put = false;
           uint32_t key = kSentinel;
           if (!skip) {
             key = f2key(adc_sum<MF>(codes + (off + pos) * M, lut, M, base));
-            atomicAdd(&hist[key >> 21], 1);
+            put = (key >> 21) <= bound;
+            if (put) atomicAdd(&hist[key >> 21], 1);
           }
-          seg[n + pos] = key;
+          wave_append(put, ((uint64_t)key << 32) | (uint32_t)(off + pos), &ds.nseg, seg, INT32_MAX);
         }
       }
       n += L;
       __syncthreads();  // LUT and base are rebuilt for the next list
-    }
-
-    // 2. kk-th smallest key T and the count of T-keys it takes
-    {
-      int local = 0;
-      for (int b = tid * 8; b < tid * 8 + 8; b++) local += hist[b];
-      const int valid = block_sum(local, ds.sh);
-      if (tid == 0) ds.valid = valid;
-    }
-    __syncthreads();
-    const bool take_all = ds.valid <= kk;
-    uint32_t T = 0u, Tid = 0xFFFFFFFFu;
-    if (!take_all) {
-      int bin, before, count, remaining = kk;
-      deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
-      remaining -= before;
-      uint32_t prefix = (uint32_t)bin << 21, mask = 0xFFE00000u;
-      deep_key_pass(seg, n, 10, 11, prefix, mask, hist);
-      deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
-      remaining -= before;
-      prefix |= (uint32_t)bin << 10;
-      mask |= 0x001FFC00u;
-      deep_key_pass(seg, n, 0, 10, prefix, mask, hist);
-      deep_find_bin(hist, 1024, remaining, ds, bin, before, count);
-      remaining -= before;
-      T = prefix | (uint32_t)bin;
-      // 3. ties at T: the `remaining` lowest ids among `count` keys equal to T
-      if (remaining < count) {
-        uint32_t iprefix = 0u, imask = 0u;
-        const int shifts[3] = {21, 10, 0};
-        const int nbits[3] = {11, 11, 10};
-        for (int pass = 0; pass < 3; pass++) {
-          const int nb = 1 << nbits[pass];
-          const uint32_t dm = (uint32_t)nb - 1u;
-          for (int b = tid; b < nb; b += kNT) hist[b] = 0;
-          __syncthreads();
-          int64_t base_i = 0;
-          for (int p = 0; p < np; p++) {
-            const int l = probes[qi * maxp + p];
-            if (l < 0) continue;
-            const int64_t off = list_off[l];
-            const int64_t L = min<int64_t>(list_off[l + 1] - off, seg_cap - base_i);
-            if (L <= 0) continue;
-            for (int64_t pos = tid; pos < L; pos += kNT) {
-              if (seg[base_i + pos] != T) continue;
-              const uint32_t id = (uint32_t)ids[off + pos];
-              if ((id & imask) == iprefix) atomicAdd(&hist[(id >> shifts[pass]) & dm], 1);
-            }
-            base_i += L;
-          }
-          __syncthreads();
-          int ibin, ibefore, icount;
-          deep_find_bin(hist, nb, remaining, ds, ibin, ibefore, icount);
-          remaining -= ibefore;
-          iprefix |= (uint32_t)ibin << shifts[pass];
-          imask |= dm << shifts[pass];
-        }
-        Tid = iprefix;
-      }
-    }
-
-    // 4. emit the winners, sort, write out
-    if (tid == 0) ds.cnt = 0;
-    __syncthreads();
-    {
-      int64_t base_i = 0;
-      for (int p = 0; p < np; p++) {
-        const int l = probes[qi * maxp + p];
-        if (l < 0) continue;
-        const int64_t off = list_off[l];
-        const int64_t L = min<int64_t>(list_off[l + 1] - off, seg_cap - base_i);
-        if (L <= 0) continue;
-        constexpr int UC = 8;
-        for (int64_t p0 = tid; p0 < L; p0 += (int64_t)kNT * UC) {
-          uint32_t kv[UC];
-#pragma unroll
-          for (int u = 0; u < UC; u++) {
-            const int64_t pos = p0 + (int64_t)u * kNT;
-            kv[u] = pos < L ? seg[base_i + pos] : kSentinel;
-          }
-#pragma unroll
-          for (int u = 0; u < UC; u++) {
-            const uint32_t key = kv[u];
-            bool take = false;
-            uint32_t id = 0u;
-            if (key != kSentinel && (take_all || key <= T)) {
-              const int64_t pos = p0 + (int64_t)u * kNT;
-              id = (uint32_t)ids[off + pos];
-              take = take_all || key < T || id <= Tid;
-            }
-            wave_append(take, ((uint64_t)key << 32) | id, &ds.cnt, out, kk);
-          }
-        }
-        base_i += L;
+      if (wave == 0) {  // tighten the bound for the next list
+        const uint32_t b = wave_digit_bound(hist, kk);
+        if (lane == 0) ds.bound = b;
       }
     }
     __syncthreads();
+    const int64_t nseg = ds.nseg;
+
+    // 2.-4. kk-th smallest key, ties by id, emit (shared with the stream scan)
+    deep_select_segment(seg, nseg, ids, kk, hist, ds, out);
     const int cnt = min(ds.cnt, kk);
     // the rerank that reads out_keys selects over all of them: no sort
     const int fill_to = out_keys ? kk : np2;
@@ -1567,91 +1563,9 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
       break;
     } else {
       __syncthreads();
-      // kk-th smallest key T (as ivfpq_deep_scan_kernel)
-      {
-        int local = 0;
-        for (int b = tid * 8; b < tid * 8 + 8; b++) local += hist[b];
-        const int valid = block_sum(local, ds.sh);
-        if (tid == 0) ds.valid = valid;
-      }
-      __syncthreads();
-      const bool take_all = ds.valid <= kk;
-      uint32_t T = 0u, Tid = 0xFFFFFFFFu;
       const int64_t nseg = ds.nseg;  // every key at or below the final bound is here
       if (pool_writes && tid == 0) atomicAdd(pool_writes, (unsigned long long)nseg);
-      if (!take_all) {
-        int bin, before, count, remaining = kk;
-        deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
-        remaining -= before;
-        uint32_t prefix = (uint32_t)bin << 21, mask = 0xFFE00000u;
-        deep_key_pass64(seg, nseg, 10, 11, prefix, mask, hist);
-        deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
-        remaining -= before;
-        prefix |= (uint32_t)bin << 10;
-        mask |= 0x001FFC00u;
-        deep_key_pass64(seg, nseg, 0, 10, prefix, mask, hist);
-        deep_find_bin(hist, 1024, remaining, ds, bin, before, count);
-        remaining -= before;
-        T = prefix | (uint32_t)bin;
-        if (remaining < count) {  // ties at T: the lowest ids win
-          uint32_t iprefix = 0u, imask = 0u;
-          const int shifts[3] = {21, 10, 0};
-          const int nbits[3] = {11, 11, 10};
-          for (int pass = 0; pass < 3; pass++) {
-            const int nb = 1 << nbits[pass];
-            const uint32_t dm = (uint32_t)nb - 1u;
-            for (int b = tid; b < nb; b += kNT) hist[b] = 0;
-            __syncthreads();
-            for (int64_t j = tid; j < nseg; j += kNT) {
-              const uint64_t w = seg[j];
-              if ((uint32_t)(w >> 32) != T) continue;
-              const uint32_t id = (uint32_t)ids[(uint32_t)w];
-              if ((id & imask) == iprefix) atomicAdd(&hist[(id >> shifts[pass]) & dm], 1);
-            }
-            __syncthreads();
-            int ibin, ibefore, icount;
-            deep_find_bin(hist, nb, remaining, ds, ibin, ibefore, icount);
-            remaining -= ibefore;
-            iprefix |= (uint32_t)ibin << shifts[pass];
-            imask |= dm << shifts[pass];
-          }
-          Tid = iprefix;
-        }
-      }
-      if (tid == 0) ds.cnt = 0;
-      __syncthreads();
-      {
-        const uint4 *s4 = reinterpret_cast<const uint4 *>(seg);
-        const int64_t n2 = nseg >> 1;
-        constexpr int UC = 4;
-        for (int64_t i0 = tid; i0 < n2 + (nseg & 1); i0 += (int64_t)kNT * UC) {
-          uint4 v[UC];
-#pragma unroll
-          for (int u = 0; u < UC; u++) {
-            const int64_t i = i0 + (int64_t)u * kNT;
-            v[u] = i < n2 ? s4[i] : make_uint4(0u, kSentinel, 0u, kSentinel);
-            if (i == n2 && (nseg & 1)) {  // the odd last word
-              const uint64_t w = seg[nseg - 1];
-              v[u] = make_uint4((uint32_t)w, (uint32_t)(w >> 32), 0u, kSentinel);
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < UC; u++) {
-            const uint32_t kv[2] = {v[u].y, v[u].w}, ev2[2] = {v[u].x, v[u].z};
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-              const uint32_t key = kv[c];
-              bool take = false;
-              uint32_t id = 0u;
-              if (key != kSentinel && (take_all || key <= T)) {
-                id = (uint32_t)ids[ev2[c]];
-                take = take_all || key < T || id <= Tid;
-              }
-              wave_append(take, ((uint64_t)key << 32) | id, &ds.cnt, cand, kk);
-            }
-          }
-        }
-      }
+      deep_select_segment(seg, nseg, ids, kk, hist, ds, cand);
       __syncthreads();
       const int cnt = min(ds.cnt, kk);
       const int np2 = next_pow2(kk);
@@ -1846,9 +1760,9 @@ void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int
   int per_cu = 0;
   MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
   const int64_t cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);  // 16-byte aligned segments
-  // at most 4 GiB of key segments: fewer persistent workgroups beyond that
-  const int64_t grid = persistent_grid(per_cu, (int64_t(4) << 30) / (4 * cap));
-  w.deep_seg.reserve(sizeof(uint32_t) * (size_t)(grid * cap));
+  // at most 4 GiB of segments of 8-byte (key, entry) words
+  const int64_t grid = persistent_grid(per_cu, (int64_t(4) << 30) / (8 * cap));
+  w.deep_seg.reserve(sizeof(uint64_t) * (size_t)(grid * cap));
   const int32_t *order = build_query_order(a, maxp, probes, w, s);
   unsigned *counter = w.deep_counter.as<unsigned>();
   hipLaunchKernelGGL((ivfpq_deep_scan_kernel<MF, IP, MIXED, DF>), dim3((unsigned)grid), dim3(kNT), sm,
