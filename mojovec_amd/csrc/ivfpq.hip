@@ -1312,14 +1312,6 @@ __device__ uint64_t threshold_compact(uint64_t *cand, int n, int kk, int *hist, 
   return thr;
 }
 
-// Entry of stream position j: advances the thread's probe cursor p (j only
-// grows per thread) and returns the entry's CSR index.
-__device__ __forceinline__ int64_t stream_entry(const int *poff, const int64_t *loff, int64_t j,
-                                                int &p) {
-  while (poff[p + 1] <= j) p++;
-  return loff[p] + (j - poff[p]);
-}
-
 template <int MF, bool IP, int MODE>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void ivfpq_stream_scan_kernel(
     const float *__restrict__ q, int64_t nq, int d, int M, const float *__restrict__ coarse,
@@ -1424,8 +1416,13 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
     __syncthreads();
     const int64_t n = DEEP ? min<int64_t>(carry, seg_cap) : carry;
 
-    // scan the stream
+    // scan the stream. Each thread's positions only grow, so it keeps its
+    // current probe in registers -- the stream position where it ends, the
+    // CSR offset of its entries and its base term -- and reads the probe
+    // table only when it crosses into the next list.
     int cur = 0;
+    int64_t cur_end = poff[1], cur_delta = loff[0] - poff[0];
+    float cur_base = basev[0];
     constexpr int U = (MF > 0) ? 4 : 1;
     int round = 0;
     for (int64_t r0 = 0; r0 < n; r0 += (int64_t)kNT * U) {
@@ -1446,11 +1443,17 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
         initv[u] = 0.0f;
         ev[u] = 0;
         if (j < n) {
-          const int64_t e = stream_entry(poff, loff, j, cur);
+          while (j >= cur_end) {
+            cur++;
+            cur_end = poff[cur + 1];
+            cur_delta = loff[cur] - poff[cur];
+            cur_base = basev[cur];
+          }
+          const int64_t e = j + cur_delta;
           ev[u] = e;
           if (MF > 0) cw[u] = load_code<MF>(codes, e);
           if (!DEEP || excl) idv[u] = ids[e];
-          initv[u] = basev[cur];
+          initv[u] = cur_base;
           if (kMixed) initv[u] = initv[u] + pre[e];
         }
       }
