@@ -670,7 +670,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
 // the selection is done once per query instead.
 //   1. scan: an entry's u32 distance key is appended, with its CSR entry
 //      index, to the workgroup's private segment in HBM -- and its top
-//      11-bit digit counted in an LDS histogram -- when that digit is at most
+//      12-bit digit counted in an LDS histogram -- when that digit is at most
 //      a running bound on the digit of the kk-th smallest key; the bound is
 //      recomputed from the histogram as the scan goes and only tightens, so
 //      every key that can be among the kk smallest is appended (a fraction
@@ -689,6 +689,12 @@ __global__ __launch_bounds__(kNT) void ivfpq_query_scan_kernel(
 constexpr int kDeepMinCandidates = 513;  // default: the deep form for k * k_factor >= this
 constexpr int kDeepBatch = 4;            // list entries per thread with loads in flight
 constexpr int kMfmaCoarseMinLists = 4096;  // mixed form: MFMA coarse above this nlist
+// Top digit of the deep-pool keys (eighth-octave bins of the distance); the
+// other 20 bits are two 10-bit digits.
+constexpr int kTopBits = 12;
+constexpr int kTopBins = 1 << kTopBits;
+constexpr int kTopShift = 32 - kTopBits;
+constexpr uint32_t kNoBound = kTopBins - 1;
 
 struct DeepShared {
   int sh[16];  // block_excl_scan / digit search scratch
@@ -700,18 +706,21 @@ struct DeepShared {
   uint32_t T, Tid;
 };
 
+// LDS of the deep-pool scans. The winners (kk keys, or next_pow2(kk) when
+// they are sorted in place) overlay the tables and the digit histogram,
+// which are dead once the selection has found its cut.
 struct DeepLayout {
   int lut, qtab, rv, out, hist, st, total;
-  __host__ __device__ DeepLayout(int M, int d, int kk, bool has_qtab) {
+  __host__ __device__ DeepLayout(int M, int d, int kk, bool has_qtab, bool sorted) {
     const int nlut = M * kKsub * 4;
     lut = 0;
     qtab = lut + nlut;
     rv = qtab + (has_qtab ? nlut : 0);
-    const int tables_end = rv + 4 * (d + 4);
-    out = 0;  // the winners reuse the table space once the scan is done
-    const int out_end = 8 * next_pow2(kk);
-    hist = ((tables_end > out_end ? tables_end : out_end) + 15) & ~15;
-    st = hist + 4 * 2048;
+    hist = (rv + 4 * (d + 4) + 15) & ~15;
+    const int tables_end = hist + 4 * kTopBins;
+    out = 0;
+    const int out_end = 8 * (sorted ? next_pow2(kk) : kk);
+    st = ((tables_end > out_end ? tables_end : out_end) + 15) & ~15;
     total = st + (int)sizeof(DeepShared);
   }
 };
@@ -826,18 +835,19 @@ __device__ __forceinline__ void wave_append(bool take, uint64_t v, int *counter,
   }
 }
 
-// Upper bound on the top 11-bit digit of the kk-th smallest key, from one
+// Upper bound on the top digit (kTopBits) of the kk-th smallest key, from one
 // wavefront's read of the running digit histogram while other waves still
 // add to it: counts only grow (and are exact in every bin at or below the
 // current bound), so the first bin at which the read's running sum reaches
-// kk is at or above the final bin of the kk-th key. 2047 (no bound) while
-// fewer than kk keys are counted.
+// kk is at or above the final bin of the kk-th key. kNoBound while fewer than
+// kk keys are counted.
 __device__ uint32_t wave_digit_bound(const int *hist, int kk) {
+  constexpr int PER = kTopBins / 64;  // bins per lane
   const int lane = threadIdx.x & 63;
-  const int4 *h4 = reinterpret_cast<const int4 *>(hist + lane * 32);
+  const int4 *h4 = reinterpret_cast<const int4 *>(hist + lane * PER);
   int local = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
+  for (int i = 0; i < PER / 4; i++) {
     const int4 v = h4[i];
     local += (v.x + v.y) + (v.z + v.w);
   }
@@ -847,19 +857,19 @@ __device__ uint32_t wave_digit_bound(const int *hist, int kk) {
     const int y = __shfl_up(incl, o, 64);
     if (lane >= o) incl += y;
   }
-  if (__shfl(incl, 63, 64) < kk) return 2047u;
+  if (__shfl(incl, 63, 64) < kk) return kNoBound;
   const int L = __ffsll((unsigned long long)__ballot(incl >= kk)) - 1;
-  // The 32 bins of lane L, one per lane, on top of the count before them;
+  // The bins of lane L, one per lane, on top of the count before them;
   // re-read counts can only be larger, which keeps the bin a valid bound.
   const int before = __shfl(incl - local, L, 64);
-  int c = lane < 32 ? hist[L * 32 + lane] : 0;
+  int c = lane < PER ? hist[L * PER + lane] : 0;
 #pragma unroll
-  for (int o = 1; o < 32; o <<= 1) {
+  for (int o = 1; o < 64; o <<= 1) {
     const int y = __shfl_up(c, o, 64);
     if (lane >= o) c += y;
   }
-  const uint64_t hit = __ballot(lane < 32 && before + c >= kk);
-  return hit ? (uint32_t)(L * 32 + __ffsll((unsigned long long)hit) - 1) : (uint32_t)(L * 32 + 31);
+  const uint64_t hit = __ballot(lane < PER && before + c >= kk);
+  return hit ? (uint32_t)(L * PER + __ffsll((unsigned long long)hit) - 1) : (uint32_t)(L * PER + PER - 1);
 }
 
 // One MSD digit pass over a compacted segment of (key << 32 | entry) words:
@@ -908,8 +918,9 @@ __device__ void deep_select_segment(const uint64_t *__restrict__ seg, int64_t ns
                                     DeepShared &ds, uint64_t *out) {
   const int tid = threadIdx.x;
   {
+    constexpr int PER = kTopBins / kNT;
     int local = 0;
-    for (int b = tid * 8; b < tid * 8 + 8; b++) local += hist[b];
+    for (int b = tid * PER; b < tid * PER + PER; b++) local += hist[b];
     const int valid = block_sum(local, ds.sh);
     if (tid == 0) ds.valid = valid;
   }
@@ -918,14 +929,14 @@ __device__ void deep_select_segment(const uint64_t *__restrict__ seg, int64_t ns
   uint32_t T = 0u, Tid = 0xFFFFFFFFu;
   if (!take_all) {
     int bin, before, count, remaining = kk;
-    deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
+    deep_find_bin(hist, kTopBins, remaining, ds, bin, before, count);
     remaining -= before;
-    uint32_t prefix = (uint32_t)bin << 21, mask = 0xFFE00000u;
-    deep_key_pass64(seg, nseg, 10, 11, prefix, mask, hist);
-    deep_find_bin(hist, 2048, remaining, ds, bin, before, count);
+    uint32_t prefix = (uint32_t)bin << kTopShift, mask = ~0u << kTopShift;
+    deep_key_pass64(seg, nseg, 10, 10, prefix, mask, hist);
+    deep_find_bin(hist, 1024, remaining, ds, bin, before, count);
     remaining -= before;
     prefix |= (uint32_t)bin << 10;
-    mask |= 0x001FFC00u;
+    mask |= 0x000FFC00u;
     deep_key_pass64(seg, nseg, 0, 10, prefix, mask, hist);
     deep_find_bin(hist, 1024, remaining, ds, bin, before, count);
     remaining -= before;
@@ -1003,7 +1014,7 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
     float *__restrict__ D, int64_t *__restrict__ I, float pad_dist, uint32_t *__restrict__ seg_base,
     int64_t seg_cap, unsigned *__restrict__ counter, const int32_t *__restrict__ order) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const DeepLayout lay(M, d, kk, MIXED && !IP);
+  const DeepLayout lay(M, d, kk, MIXED && !IP, out_keys == nullptr);
   float *lut = reinterpret_cast<float *>(smem + lay.lut);
   float *qtab = reinterpret_cast<float *>(smem + lay.qtab);
   float *rv = reinterpret_cast<float *>(smem + lay.rv);
@@ -1026,10 +1037,10 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
     const float *qq = q + qi * d;
     int np = nprobe_q ? nprobe_q[qi] : nprobe_all;
     np = min(max(np, 1), maxp);
-    for (int b = tid; b < 2048; b += kNT) hist[b] = 0;
+    for (int b = tid; b < kTopBins; b += kNT) hist[b] = 0;
     if (tid == 0) {
       ds.nseg = 0;
-      ds.bound = 2047u;
+      ds.bound = kNoBound;
     }
     build_query_tables<IP, MIXED>(qq, d, M, pq, rv, lut, qtab);
 
@@ -1071,8 +1082,8 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
               }
               if (!skip) {
                 key = f2key(distv[u]);
-                put = (key >> 21) <= bound;
-                if (put) atomicAdd(&hist[key >> 21], 1);
+                put = (key >> kTopShift) <= bound;
+                if (put) atomicAdd(&hist[key >> kTopShift], 1);
               }
             }
             wave_append(put, ((uint64_t)key << 32) | (uint32_t)(off + pos), &ds.nseg, seg, INT32_MAX);
@@ -1089,8 +1100,8 @@ __global__ __launch_bounds__(kNT) void ivfpq_deep_scan_kernel(
           uint32_t key = kSentinel;
           if (!skip) {
             key = f2key(adc_sum<MF>(codes + (off + pos) * M, lut, M, base));
-            put = (key >> 21) <= bound;
-            if (put) atomicAdd(&hist[key >> 21], 1);
+            put = (key >> kTopShift) <= bound;
+            if (put) atomicAdd(&hist[key >> kTopShift], 1);
           }
           wave_append(put, ((uint64_t)key << 32) | (uint32_t)(off + pos), &ds.nseg, seg, INT32_MAX);
         }
@@ -1159,16 +1170,20 @@ constexpr int kThreshJ = (kMaxCandidates + kThreshAppend + kNT - 1) / kNT;
 
 struct StreamLayout {
   int lut, rv, poff, loff, base, cand, tmp, scr, hist, sh, st, total;
-  __host__ __device__ StreamLayout(int M, int d, int kk, int maxp, int mode) {
+  __host__ __device__ StreamLayout(int M, int d, int kk, int maxp, int mode, bool sorted = true) {
     const bool deep = mode == 1;
     lut = 0;
     rv = lut + M * kKsub * 4;
     int tables_end = (rv + 4 * d + 15) & ~15;
     if (deep) {
-      // the winners (cand) reuse the LUT space once the scan is done
+      // the digit histogram follows the tables; the winners (cand: kk keys,
+      // next_pow2(kk) when sorted in place) overlay both once the selection
+      // has found its cut (as DeepLayout)
+      hist = tables_end;
+      tables_end = hist + 4 * kTopBins;
       cand = 0;
-      const int out_end = 8 * next_pow2(kk);
-      if (out_end > tables_end) tables_end = out_end;
+      const int out_end = 8 * (sorted ? next_pow2(kk) : kk);
+      if (out_end > tables_end) tables_end = (out_end + 15) & ~15;
     }
     poff = tables_end;
     loff = (poff + 4 * (maxp + 1) + 15) & ~15;
@@ -1178,8 +1193,7 @@ struct StreamLayout {
     if (deep) {
       tmp = cand;
       scr = cand;
-      hist = after;
-      st = hist + 4 * 2048;
+      st = after;
       total = st + (int)sizeof(DeepShared);
     } else if (mode == 2) {  // running candidate set with threshold flushes
       cand = after;
@@ -1328,7 +1342,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
   // HBM (DEEP); 2: running candidate set with threshold flushes (THRESH).
   constexpr bool DEEP = MODE == 1;
   constexpr bool THRESH = MODE == 2;
-  const StreamLayout lay(M, d, kk, maxp, MODE);
+  const StreamLayout lay(M, d, kk, maxp, MODE, out_keys == nullptr);
   float *lut = reinterpret_cast<float *>(smem + lay.lut);
   float *rv = reinterpret_cast<float *>(smem + lay.rv);
   int *poff = reinterpret_cast<int *>(smem + lay.poff);
@@ -1365,10 +1379,10 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
     // per-query LUT
     for (int j = tid; j < d; j += kNT) rv[j] = qq[j];
     if constexpr (DEEP) {
-      for (int b = tid; b < 2048; b += kNT) hist[b] = 0;
+      for (int b = tid; b < kTopBins; b += kNT) hist[b] = 0;
       if (tid == 0) {
         ds.nseg = 0;
-        ds.bound = 2047u;
+        ds.bound = kNoBound;
       }
     } else {
       for (int i = tid; i < kk; i += kNT) cand[i] = kSentinel64;
@@ -1429,7 +1443,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
       const uint64_t thr = DEEP ? kSentinel64 : st->thr;
       // DEEP: keys whose top digit exceeds this bound cannot be among the kk
       // smallest; only the others are written and counted.
-      uint32_t bound = 2047u;
+      uint32_t bound = kNoBound;
       if constexpr (DEEP) bound = __hip_atomic_load(&ds.bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       Code<MF> cw[U];
       int32_t idv[U];
@@ -1478,8 +1492,8 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
           if constexpr (DEEP) {
             if (!skip) {
               key = f2key(dist);
-              put = (key >> 21) <= bound;
-              if (put) atomicAdd(&hist[key >> 21], 1);
+              put = (key >> kTopShift) <= bound;
+              if (put) atomicAdd(&hist[key >> kTopShift], 1);
             }
           } else if (!skip) {
             const uint64_t pk = pack_key(dist, (uint32_t)id);
@@ -1754,7 +1768,7 @@ int64_t persistent_grid(int per_cu, int64_t limit) {
 template <int MF, bool IP, bool MIXED, int DF>
 void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                     uint64_t *out_keys, IvfWork &w, hipStream_t s) {
-  const DeepLayout lay(a.M, a.d, kk, MIXED && !IP);
+  const DeepLayout lay(a.M, a.d, kk, MIXED && !IP, out_keys == nullptr);
   const size_t sm = (size_t)lay.total;
   if (sm > 150 * 1024) throw Error(kInvalid, "IVF-PQ deep scan workspace exceeds LDS");
   const void *fn = (const void *)ivfpq_deep_scan_kernel<MF, IP, MIXED, DF>;
@@ -1781,7 +1795,7 @@ template <int MF, bool IP, int MODE>
 bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                    uint64_t *out_keys, IvfWork &w, hipStream_t s) {
   constexpr bool DEEP = MODE == 1;
-  const StreamLayout lay(a.M, a.d, kk, maxp, MODE);
+  const StreamLayout lay(a.M, a.d, kk, maxp, MODE, out_keys == nullptr);
   const size_t sm = (size_t)lay.total;
   if (sm > 150 * 1024 || maxp > kStreamMaxProbes) return false;
   const void *fn = (const void *)ivfpq_stream_scan_kernel<MF, IP, MODE>;
