@@ -184,6 +184,20 @@ __global__ __launch_bounds__(kNT) void ivfpq_scan_kernel(
   }
 }
 
+// Query of a rerank workgroup. Without an order, workgroup b reranks query b.
+// With one (the clustered order of the deep-pool scans), the grid is
+// kRerankRanges * ceil(nq / kRerankRanges) and the workgroups of one XCD
+// (consecutive workgroups go to consecutive XCDs) take a contiguous stretch of
+// the order, so queries that share lists share their rows in that XCD's L2.
+// Returns -1 for the grid's padding.
+constexpr int kRerankRanges = 8;
+__device__ __forceinline__ int64_t rerank_query(int64_t nq, const int32_t *__restrict__ order) {
+  if (!order) return blockIdx.x;
+  const int64_t per = (nq + kRerankRanges - 1) / kRerankRanges;
+  const int64_t t = (int64_t)(blockIdx.x % kRerankRanges) * per + blockIdx.x / kRerankRanges;
+  return t < nq ? (int64_t)order[t] : -1;
+}
+
 template <bool IP>
 __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q, int d,
                                                      const float *__restrict__ vectors,
@@ -191,14 +205,16 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
                                                      const uint64_t *__restrict__ cand, int kk,
                                                      int k, int buf_n, float pad_dist,
                                                      float *__restrict__ D,
-                                                     int64_t *__restrict__ I) {
+                                                     int64_t *__restrict__ I, int64_t nq,
+                                                     const int32_t *__restrict__ order) {
   extern __shared__ __align__(16) unsigned char smem[];
   float *qs = reinterpret_cast<float *>(smem);
   uint64_t *buf = reinterpret_cast<uint64_t *>(qs + ((d + 1) & ~1));
   uint64_t *top = buf + buf_n;
   SelectScratch *scr = reinterpret_cast<SelectScratch *>(top + buf_n);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t qi = blockIdx.x;
+  const int64_t qi = rerank_query(nq, order);
+  if (qi < 0) return;
   for (int j = tid; j < d; j += kNT) qs[j] = q[qi * d + j];
   for (int t = tid; t < buf_n; t += kNT) buf[t] = kSentinel64;
   __syncthreads();
@@ -235,14 +251,16 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
                                                            const uint64_t *__restrict__ cand,
                                                            int kk, int k, float pad_dist,
                                                            float *__restrict__ D,
-                                                           int64_t *__restrict__ I) {
+                                                           int64_t *__restrict__ I, int64_t nq,
+                                                           const int32_t *__restrict__ order) {
   extern __shared__ __align__(16) unsigned char smem[];
   float *qs = reinterpret_cast<float *>(smem);                      // d
   uint64_t *buf = reinterpret_cast<uint64_t *>(qs + d);             // kk
   uint64_t *top = buf + ((kk + 1) & ~1);                            // next_pow2(k)
   SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(top + next_pow2(k));
   const int tid = threadIdx.x, glane = tid & 15, group = tid >> 4;
-  const int64_t qi = blockIdx.x;
+  const int64_t qi = rerank_query(nq, order);
+  if (qi < 0) return;
   for (int j = tid; j < d; j += kNT) qs[j] = q[qi * d + j];
   __syncthreads();
   const uint64_t *cq = cand + qi * kk;
@@ -1743,6 +1761,7 @@ const int32_t *build_query_order(const IvfSearchArgs &a, int maxp, const int32_t
   w.ord_off.reserve(sizeof(int) * (size_t)a.nlist);
   w.ord.reserve(sizeof(int32_t) * (size_t)a.nq);
   w.deep_counter.reserve(sizeof(unsigned) * kQueryRanges);
+  w.ord_ready = true;
   hipLaunchKernelGGL(fill_zero_kernel, dim3(ceil_div(a.nlist, 256)), dim3(256), 0, s,
                      w.ord_cnt.as<int>(), (int64_t)a.nlist);
   MVG_LAUNCH_CHECK("fill_zero_kernel");
@@ -1880,9 +1899,11 @@ void precompute_entry_terms(const int64_t *list_off, int nlist, const uint8_t *c
 
 void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors, int64_t nrows,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
-            hipStream_t s) {
+            hipStream_t s, const int32_t *order) {
   if (nq == 0) return;
   const bool ip = metric == kMetricIP;
+  const unsigned grid =
+      (unsigned)(order ? kRerankRanges * ((nq + kRerankRanges - 1) / kRerankRanges) : nq);
   if (d >= 64 && d % 64 == 0) {
     const size_t sm = sizeof(float) * d + sizeof(uint64_t) * (size_t)(((kk + 1) & ~1) + next_pow2(k)) +
                       sizeof(SelectScratch64);
@@ -1891,11 +1912,11 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
       if (sm > 64 * 1024)
         MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
       if (ip)
-        hipLaunchKernelGGL(rerank_group_kernel<true>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d,
-                           vectors, nrows, cand, kk, k, pad_dist, D, I);
+        hipLaunchKernelGGL(rerank_group_kernel<true>, dim3(grid), dim3(kNT), sm, s, q, d, vectors,
+                           nrows, cand, kk, k, pad_dist, D, I, nq, order);
       else
-        hipLaunchKernelGGL(rerank_group_kernel<false>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d,
-                           vectors, nrows, cand, kk, k, pad_dist, D, I);
+        hipLaunchKernelGGL(rerank_group_kernel<false>, dim3(grid), dim3(kNT), sm, s, q, d, vectors,
+                           nrows, cand, kk, k, pad_dist, D, I, nq, order);
       MVG_LAUNCH_CHECK("rerank_group_kernel");
       return;
     }
@@ -1908,16 +1929,17 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
   if (sm > 64 * 1024)
     MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   if (ip)
-    hipLaunchKernelGGL(rerank_kernel<true>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d, vectors,
-                       nrows, cand, kk, k, buf_n, pad_dist, D, I);
+    hipLaunchKernelGGL(rerank_kernel<true>, dim3(grid), dim3(kNT), sm, s, q, d, vectors, nrows,
+                       cand, kk, k, buf_n, pad_dist, D, I, nq, order);
   else
-    hipLaunchKernelGGL(rerank_kernel<false>, dim3((unsigned)nq), dim3(kNT), sm, s, q, d, vectors,
-                       nrows, cand, kk, k, buf_n, pad_dist, D, I);
+    hipLaunchKernelGGL(rerank_kernel<false>, dim3(grid), dim3(kNT), sm, s, q, d, vectors, nrows,
+                       cand, kk, k, buf_n, pad_dist, D, I, nq, order);
   MVG_LAUNCH_CHECK("rerank_kernel");
 }
 
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   if (a.nq == 0) return;
+  w.ord_ready = false;  // set when a deep-pool scan builds the clustered query order
   const int kf = a.k_factor > 1 ? a.k_factor : 1;
   const int64_t kk64 = (int64_t)a.k * kf;
   if (kk64 > kMaxCandidates) throw Error(kInvalid, "k * k_factor exceeds the device limit of 4096");
@@ -2048,7 +2070,8 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     }
     if (pf.on) MVG_HIP(hipEventRecord(pf.ev[2], s));
     if (kf > 1)
-      rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, keys, kk, a.k, kIvfPadDist, a.D, a.I, s);
+      rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, keys, kk, a.k, kIvfPadDist, a.D, a.I, s,
+             w.ord_ready ? w.ord.as<int32_t>() : nullptr);
   } else {
   // 2. per-(query, probe) LUT + ADC scan + list-local top-kk.
   w.part.reserve(sizeof(uint64_t) * (size_t)a.nq * maxp * kk);

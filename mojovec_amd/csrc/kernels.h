@@ -91,6 +91,7 @@ struct IvfWork {
   DevBuf wq_poff, wq_nitems, wq_rq, wq_off, wq_gthr, wq_counter;
   DevBuf deep_seg, deep_counter;  // deep candidate pools: key segments, query range counters
   DevBuf ord_cnt, ord_off, ord;    // query order of the persistent deep scans
+  bool ord_ready = false;          // `ord` holds this search's order
   IvfProfile prof;
 };
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
@@ -98,9 +99,11 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
 // Exact FP32 rerank of kk candidate keys per query (any order) to the top-k
 // (index_hnsw.mojo:449-496 generalised to k_factor).
 // Candidate ids at or above nrows (not rows of `vectors`) are dropped.
+// order (nullable, [nq]): a query order whose neighbours share candidate
+// rows; each XCD then reranks a contiguous stretch of it.
 void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors, int64_t nrows,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
-            hipStream_t s);
+            hipStream_t s, const int32_t *order = nullptr);
 
 // ---- ivfflat.hip ----
 struct IvfFlatArgs {
