@@ -663,6 +663,33 @@ def bench_sq8(args):
                                      "peak_tops": 5000.0, "frac": round(tops / 5000.0, 4)}
             log(f"sq8 {name} kf={kf}: {out[f'{name}_kf{kf}']}")
         del ix
+        # the same rows in an FP32 Flat index: plain FP32 and the BF16 / FP8
+        # E4M3 MFMA candidate forms with the exact FP32 rerank
+        fx = mv.IndexFlat(d, metric)
+        fx.add(x_h)
+        fname = "ip" if metric == 1 else "l2"
+        for form, kfs, peak in (("plain", (1,), 157.3), ("mixed-bf16", (2, 4), 2500.0),
+                                ("mixed-fp8", (2, 4), 5000.0)):
+            for kf in kfs:
+                fx.set_search_mode(form, k_factor=kf)
+                res = {}
+
+                def run():
+                    res["r"] = fx.search(q_h, k)
+
+                dt = _timed(torch, run, 1 if form == "plain" else max(1, args.steps // 4), 1)
+                D = torch.from_numpy(res["r"][0]).to(dev).double()
+                ids = torch.from_numpy(res["r"][1]).to(dev)
+                exact = -gt_d if metric == 1 else gt_d
+                tops = 2.0 * n * nq * d / dt / 1e12
+                key = f"flat_{fname}_{form}_kf{kf}"
+                out[key] = {"qps": round(nq / dt, 1), "ms": round(dt * 1e3, 2),
+                            "recall_at_10": round(recall_at(torch, gt_i, ids), 4),
+                            "max_top1_dist_err": float((D[:, 0] - exact[:, 0]).abs().max()),
+                            "achieved_tflops": round(tops, 1), "peak_tflops": peak,
+                            "frac": round(tops / peak, 4)}
+                log(f"sq8 {key}: {out[key]}")
+        del fx
     print(json.dumps({"workload": "SQ8 Flat 1,000,000 x 768 unit-norm, 10,000 queries, k=10 "
                                   "(host buffers, PCIe included)", "forms": out}), flush=True)
     return 0

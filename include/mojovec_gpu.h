@@ -61,10 +61,17 @@ int mvg_flat_add(mvg_flat *h, const float *x, int64_t n);
 int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_t *exclude,
                     int64_t exclude_len, float *distances, int64_t *labels);
 /* Search form (no reference counterpart). precision 0: plain FP32 in the
- * reference's operation order; 1: mixed, candidates from a bf16 GEMM-form
- * distance on MFMA (FP32 accumulate), reranked with exact FP32 distances when
- * k_factor > 1. */
+ * reference's operation order; 1: mixed, candidates from a BF16 GEMM-form
+ * distance on MFMA (FP32 accumulate); 2: mixed with FP8 E4M3 operands (each
+ * row scaled by a power of two, FP32 accumulate). Mixed candidates are
+ * reranked with exact FP32 distances when k_factor > 1. */
 int mvg_flat_set_search_mode(mvg_flat *h, int precision, int k_factor);
+/* FP8 E4M3 row encoding of the mixed Flat form (no reference counterpart):
+ * codes [n x roundup(d, 64)] (zero padded), inv_scale [n] = 2^-e where 2^e
+ * puts the row's largest magnitude in [224, 448], norms [n] (nullable) of the
+ * dequantized rows. */
+int mvg_fp8_encode(const float *x, int64_t n, int d, uint8_t *codes, float *inv_scale,
+                   float *norms);
 int64_t mvg_flat_ntotal(const mvg_flat *h);
 void mvg_flat_destroy(mvg_flat *h);
 

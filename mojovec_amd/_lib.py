@@ -42,6 +42,7 @@ SIGNATURES = {
     "mvg_flat_add": (_i, [_p, _p, _i64]),
     "mvg_flat_search": (_i, [_p, _p, _i64, _i, _p, _i64, _p, _p]),
     "mvg_flat_set_search_mode": (_i, [_p, _i, _i]),
+    "mvg_fp8_encode": (_i, [_p, _i64, _i, _p, _p, _p]),
     "mvg_flat_ntotal": (_i64, [_p]),
     "mvg_flat_destroy": (None, [_p]),
     "mvg_sq8_create": (_i, [_i, _i, ctypes.POINTER(_p)]),

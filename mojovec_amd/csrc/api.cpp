@@ -125,9 +125,10 @@ void mvg::set_last_error(const std::string &msg) { g_last_error = msg; }
 struct mvg_flat {
   int d = 0, metric = 0;
   int64_t ntotal = 0;
-  int precision = 0, k_factor = 1;
-  bool bf16_dirty = true;
+  int precision = 0, k_factor = 1;  // precision 0 FP32, 1 BF16 candidates, 2 FP8 E4M3 candidates
+  bool bf16_dirty = true, fp8_dirty = true;
   DevBuf base, part, q, D, I, excl, base_bf16, base_norm, qbf, qnorm, keys;
+  DevBuf base_fp8, base_fp8_inv, base_fp8_norm, qinv;
   hipStream_t s = nullptr;
 };
 
@@ -215,7 +216,7 @@ int mvg_flat_add(mvg_flat *h, const float *x, int64_t n) {
   MVG_HIP(hipMemcpyAsync((char *)h->base.ptr + keep, x, need - keep, hipMemcpyHostToDevice, h->s));
   MVG_HIP(hipStreamSynchronize(h->s));
   h->ntotal += n;
-  h->bf16_dirty = true;
+  h->bf16_dirty = h->fp8_dirty = true;
   API_END
 }
 
@@ -233,27 +234,52 @@ int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_
                    dx ? exclude_len : 0, h->part, 0.0f, h->D.as<float>(), h->I.as<int64_t>(),
                    nullptr, nullptr, h->s);
   } else {
-    // mixed: bf16 GEMM-form candidates on MFMA, optional exact FP32 rerank
-    const int rb = mvg::gemm_row_bytes(mvg::kGemmBf16L2, h->d);
-    if (h->bf16_dirty) {
-      h->base_bf16.reserve((size_t)h->ntotal * rb);
-      h->base_norm.reserve(sizeof(float) * h->ntotal);
-      mvg::to_bf16(h->base.as<float>(), h->ntotal, h->d, h->base_bf16.as<uint16_t>(),
-                   h->base_norm.as<float>(), h->s);
-      h->bf16_dirty = false;
-    }
-    h->qbf.reserve((size_t)nq * rb);
+    // mixed: BF16 or FP8 E4M3 GEMM-form candidates on MFMA (FP32 accumulate),
+    // optional exact FP32 rerank
+    const bool ip = h->metric == MVG_METRIC_IP;
+    const bool fp8 = h->precision == 2;
+    const int kind = fp8 ? (ip ? mvg::kGemmFp8IP : mvg::kGemmFp8L2) : (ip ? mvg::kGemmBf16IP : mvg::kGemmBf16L2);
+    const int rb = mvg::gemm_row_bytes(kind, h->d);
+    const uint8_t *xrows, *qrows;
+    const float *xnorm, *xinv = nullptr, *qinv = nullptr;
     h->qnorm.reserve(sizeof(float) * nq);
-    mvg::to_bf16(dq, nq, h->d, h->qbf.as<uint16_t>(), h->qnorm.as<float>(), h->s);
+    h->qbf.reserve((size_t)nq * rb);
+    if (fp8) {
+      if (h->fp8_dirty) {
+        h->base_fp8.reserve((size_t)h->ntotal * rb);
+        h->base_fp8_inv.reserve(sizeof(float) * h->ntotal);
+        h->base_fp8_norm.reserve(sizeof(float) * h->ntotal);
+        mvg::to_fp8(h->base.as<float>(), h->ntotal, h->d, h->base_fp8.as<uint8_t>(),
+                    h->base_fp8_inv.as<float>(), h->base_fp8_norm.as<float>(), h->s, ip);
+        h->fp8_dirty = false;
+      }
+      h->qinv.reserve(sizeof(float) * nq);
+      mvg::to_fp8(dq, nq, h->d, h->qbf.as<uint8_t>(), h->qinv.as<float>(), h->qnorm.as<float>(), h->s,
+                  ip);
+      xrows = h->base_fp8.as<uint8_t>();
+      xnorm = h->base_fp8_norm.as<float>();
+      xinv = h->base_fp8_inv.as<float>();
+      qinv = h->qinv.as<float>();
+    } else {
+      if (h->bf16_dirty) {
+        h->base_bf16.reserve((size_t)h->ntotal * rb);
+        h->base_norm.reserve(sizeof(float) * h->ntotal);
+        mvg::to_bf16(h->base.as<float>(), h->ntotal, h->d, h->base_bf16.as<uint16_t>(),
+                     h->base_norm.as<float>(), h->s);
+        h->bf16_dirty = false;
+      }
+      mvg::to_bf16(dq, nq, h->d, h->qbf.as<uint16_t>(), h->qnorm.as<float>(), h->s);
+      xrows = h->base_bf16.as<uint8_t>();
+      xnorm = h->base_norm.as<float>();
+    }
+    qrows = h->qbf.as<uint8_t>();
     const int kf = h->k_factor;
     const int kk = k * kf;
     if (kk > mvg::kMaxCandidates) throw Error(mvg::kInvalid, "k * k_factor exceeds the device limit of 4096");
-    const bool ip = h->metric == MVG_METRIC_IP;
     int nsplit = 0;
-    mvg::gemm_topk(ip ? mvg::kGemmBf16IP : mvg::kGemmBf16L2, h->qbf.as<uint8_t>(),
-                   h->base_bf16.as<uint8_t>(), nq, h->ntotal, h->d, h->qnorm.as<float>(),
-                   h->base_norm.as<float>(), nullptr, nullptr, nullptr, nullptr, 0.0f, kk, dx,
-                   dx ? exclude_len : 0, h->part, nsplit, h->s);
+    mvg::gemm_topk(kind, qrows, xrows, nq, h->ntotal, h->d, h->qnorm.as<float>(), xnorm, nullptr,
+                   nullptr, nullptr, nullptr, 0.0f, kk, dx, dx ? exclude_len : 0, h->part, nsplit,
+                   h->s, qinv, xinv);
     if (kf == 1) {
       mvg::launch_merge(h->part.as<uint64_t>(), nq, (int64_t)nsplit * kk, k, ip ? 1 : 0, 0.0f,
                         h->D.as<float>(), h->I.as<int64_t>(), nullptr, nullptr, h->s);
@@ -273,7 +299,7 @@ int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_
 
 int mvg_flat_set_search_mode(mvg_flat *h, int precision, int k_factor) {
   API_BEGIN
-  require(h && (precision == 0 || precision == 1) && k_factor >= 1, "invalid search mode");
+  require(h && precision >= 0 && precision <= 2 && k_factor >= 1, "invalid search mode");
   h->precision = precision;
   h->k_factor = k_factor;
   API_END
@@ -450,6 +476,27 @@ int mvg_pq_train(const float *x, int64_t n, int d, int M, int niter, uint64_t se
   bcb.reserve(sizeof(float) * (size_t)d * mvg::kKsub);
   pq_train_device(dx, n, d, M, niter, seed, bcb.as<float>(), kw, s);
   download(codebooks, bcb.ptr, (size_t)d * mvg::kKsub, s);
+  MVG_HIP(hipStreamSynchronize(s));
+  API_END
+}
+
+int mvg_fp8_encode(const float *x, int64_t n, int d, uint8_t *codes, float *inv_scale,
+                   float *norms) {
+  API_BEGIN
+  require(n >= 0 && d > 0 && codes && inv_scale, "invalid arguments");
+  if (n == 0) return MVG_OK;
+  hipStream_t s = new_stream();
+  struct Guard { hipStream_t s; ~Guard() { (void)hipStreamDestroy(s); } } g{s};
+  const int rb = mvg::gemm_row_bytes(mvg::kGemmFp8L2, d);
+  DevBuf bx, bc, bi, bn;
+  const float *dx = upload(bx, x, (size_t)n * d, s);
+  bc.reserve((size_t)n * rb);
+  bi.reserve(sizeof(float) * n);
+  bn.reserve(sizeof(float) * n);
+  mvg::to_fp8(dx, n, d, bc.as<uint8_t>(), bi.as<float>(), bn.as<float>(), s);
+  download(codes, bc.ptr, (size_t)n * rb, s);
+  download(inv_scale, bi.ptr, (size_t)n, s);
+  if (norms) download(norms, bn.ptr, (size_t)n, s);
   MVG_HIP(hipStreamSynchronize(s));
   API_END
 }

@@ -9,6 +9,14 @@
 //             uint8 codes rebased to int8 (s = u ^ 0x80 = u - 128):
 //             q.x = s_q.s_x + 128 (sum s_q + sum s_x) + 16384 d, exact in int32
 //             (sq8.mojo:76-79, distances.mojo:437-451)
+//   FP8_L2  : |q|^2 + |x|^2 - 2 q.x, FP8 E4M3 operands, FP32 accumulate; each
+//             row is stored scaled by a power of two that puts its largest
+//             magnitude in [224, 448], and q.x = acc * (1/s_q) * (1/s_x)
+//   FP8_IP  : -q.x ~ c_q + c_x - q~.x~ with the same operands, where
+//             c = (|v~|^2 - |v|^2) / 2 over the dequantized row v~: since
+//             -q.x = (|q - x|^2 - |q|^2 - |x|^2) / 2, the estimate's error is
+//             (x - q).(dq - dx) for quantization errors dq, dx -- small for
+//             near neighbours, where -q~.x~ alone errs by q.dx + x.dq
 // A workgroup owns 16 queries and one contiguous split of the base rows. Each
 // of its four wavefronts computes 16 x 64 tiles with v_mfma 16x16 instructions
 // (K = 32 bf16 or 64 int8 per instruction), and each query keeps its running
@@ -25,12 +33,13 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef long longx1;
 
 constexpr int kQT = 16;        // queries per workgroup
 constexpr int kStepCols = 256;  // base rows per step (4 waves x 4 tiles x 16)
 constexpr int kGCap = 384;      // per-query append capacity
 
-enum GemmKind { BF16_L2 = 0, BF16_IP = 1, I8_IP = 2, U8_L2 = 3 };
+enum GemmKind { BF16_L2 = 0, BF16_IP = 1, I8_IP = 2, U8_L2 = 3, FP8_L2 = 4, FP8_IP = 5 };
 
 struct GemmLayout {
   int qtile, top, tmp, cnt, thr, scr, total;
@@ -79,7 +88,8 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
     const uint32_t *__restrict__ qn_u, const uint32_t *__restrict__ qs_u,
     const uint32_t *__restrict__ xn_u, const uint32_t *__restrict__ xs_u, float scale_sq, int kk,
     int64_t split_len, const uint8_t *__restrict__ excl, int64_t excl_len,
-    uint64_t *__restrict__ part, int S) {
+    uint64_t *__restrict__ part, int S, const float *__restrict__ qinv,
+    const float *__restrict__ xinv) {
   extern __shared__ __align__(16) unsigned char smem[];
   const GemmLayout lay(row_bytes, kk);
   uint8_t *qtile = smem + lay.qtile;
@@ -137,6 +147,13 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
         if (KIND == BF16_L2 || KIND == BF16_IP) {
           accf[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
                                                             __builtin_bit_cast(bf16x8, b), accf[t], 0, 0, 0);
+        } else if (KIND == FP8_L2 || KIND == FP8_IP) {
+          // two K = 32 steps over the lane's 16 bytes; A and B lanes hold the
+          // same byte positions, so each k is paired exactly once
+          const long alo = (long)(((uint64_t)a.y << 32) | a.x), ahi = (long)(((uint64_t)a.w << 32) | a.z);
+          const long blo = (long)(((uint64_t)b.y << 32) | b.x), bhi = (long)(((uint64_t)b.w << 32) | b.z);
+          accf[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(alo, blo, accf[t], 0, 0, 0);
+          accf[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(ahi, bhi, accf[t], 0, 0, 0);
         } else {
           uint4 aa = a;
           if (KIND == U8_L2) {  // u - 128 as int8
@@ -162,6 +179,10 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
           dist = (qnorm[q0 + qi] + xnorm[col]) - 2.0f * accf[t][j];
         } else if (KIND == BF16_IP) {
           dist = -accf[t][j];
+        } else if (KIND == FP8_L2) {
+          dist = (qnorm[q0 + qi] + xnorm[col]) - 2.0f * (accf[t][j] * (qinv[q0 + qi] * xinv[col]));
+        } else if (KIND == FP8_IP) {
+          dist = (qnorm[q0 + qi] + xnorm[col]) - accf[t][j] * (qinv[q0 + qi] * xinv[col]);
         } else if (KIND == I8_IP) {
           dist = __fmul_rn(-(float)acci[t][j], scale_sq);
         } else {
@@ -195,6 +216,345 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide form for kk <= 48: a workgroup owns 64 queries (16 per wavefront) and
+// one split of the base rows, so each base byte fetched serves 64 queries
+// (four times the reuse of the 16-query form above). The 256 rows of a step
+// are staged through LDS 128 bytes of k at a time (32 KB per slice, double
+// buffered, the next slice's global loads in flight while the current one
+// feeds the MFMAs; one barrier per slice), and every wavefront meets the 256
+// columns with its 16 query rows in 16 MFMA tiles. Selection stays inside the
+// wavefront: a distance is compared with its query's running kk-th distance
+// in registers, the few that pass are appended to a 32-slot LDS buffer per
+// query, and a buffer near full is sorted and merged into the query's sorted
+// top kk with the lane-wide bitonic networks (no workgroup barrier).
+constexpr int kWQ = 64;                   // queries per workgroup
+constexpr int kWCap = 32;                 // append slots per query
+constexpr int kWMaxK = 48;                // largest kk of the wide form
+constexpr int kBPlane = kStepCols * 16;   // one 16-byte group of a staged slice
+constexpr int kBSlice = 8 * kBPlane;      // 128 bytes of k for 256 rows
+
+struct WideLayout {
+  int apitch, a, b, top, buf, cnt, total;
+  __host__ __device__ WideLayout(int row_bytes, int kk) {
+    apitch = row_bytes + 16;  // padded rows: the 16 lanes of a fragment read hit distinct banks
+    a = 0;
+    b = (kWQ * apitch + 15) & ~15;
+    top = b + 2 * kBSlice;
+    buf = top + 8 * kWQ * kk;
+    cnt = buf + 8 * kWQ * kWCap;
+    total = cnt + 4 * kWQ;
+  }
+};
+
+// Column swizzle of group plane g (0..7): the eight groups of one column, and
+// of two neighbouring columns, land in distinct banks when written, while a
+// fragment read (16 consecutive columns of one plane) stays one contiguous
+// 256-byte run.
+__device__ __forceinline__ int wide_swz(int g) { return (int)((0xFCA96530u >> (4 * g)) & 0xFu); }
+
+// Merges query row qq's append buffer into its sorted top kk; the whole
+// wavefront calls it with the same qq.
+__device__ __forceinline__ void wide_flush(uint64_t *top, uint64_t *buf, int *cnt, int qq, int kk) {
+  const int lane = threadIdx.x & 63;
+  const int c = cnt[qq];
+  uint64_t cand = lane < c ? buf[qq * kWCap + lane] : kSentinel64;
+  cand = wave_sort64(cand);
+  const uint64_t cur = lane < kk ? top[qq * kk + lane] : kSentinel64;
+  const uint64_t m = wave_merge64(cur, cand);
+  if (lane < kk) top[qq * kk + lane] = m;
+  if (lane == 0) cnt[qq] = 0;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
+    const uint8_t *__restrict__ Q, const uint8_t *__restrict__ X, int64_t nq, int64_t n,
+    int row_bytes, int d_true, const float *__restrict__ qnorm, const float *__restrict__ xnorm,
+    const uint32_t *__restrict__ qn_u, const uint32_t *__restrict__ qs_u,
+    const uint32_t *__restrict__ xn_u, const uint32_t *__restrict__ xs_u, float scale_sq, int kk,
+    int64_t split_len, const uint8_t *__restrict__ excl, int64_t excl_len,
+    uint64_t *__restrict__ part, int S, const float *__restrict__ qinv,
+    const float *__restrict__ xinv) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const WideLayout lay(row_bytes, kk);
+  uint8_t *atile = smem + lay.a;
+  uint8_t *bst = smem + lay.b;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lrow = lane & 15, lgrp = lane >> 4;
+  // this wavefront's 16 query rows
+  uint64_t *top = reinterpret_cast<uint64_t *>(smem + lay.top) + wave * 16 * kk;
+  uint64_t *buf = reinterpret_cast<uint64_t *>(smem + lay.buf) + wave * 16 * kWCap;
+  int *cnt = reinterpret_cast<int *>(smem + lay.cnt) + wave * 16;
+
+  const int64_t q0 = (int64_t)blockIdx.x * kWQ;
+  const int split = blockIdx.y;
+  const int64_t b_begin = (int64_t)split * split_len;
+  const int64_t b_end = min(n, b_begin + split_len);
+  const int qcount = (int)min<int64_t>(kWQ, nq - q0);
+  const int wq = wave * 16;  // first query row of this wavefront in the tile
+
+  const int row_words = row_bytes / 16;
+  for (int i = tid; i < kWQ * row_words; i += kNT) {
+    const int qi = i / row_words, wi = i - qi * row_words;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (qi < qcount) v = reinterpret_cast<const uint4 *>(Q + (q0 + qi) * row_bytes)[wi];
+    if (KIND == U8_L2) {  // u - 128 as int8
+      v.x ^= 0x80808080u; v.y ^= 0x80808080u; v.z ^= 0x80808080u; v.w ^= 0x80808080u;
+    }
+    *reinterpret_cast<uint4 *>(atile + qi * lay.apitch + wi * 16) = v;
+  }
+  for (int i = lane; i < 16 * kk; i += 64) top[i] = kSentinel64;
+  if (lane < 16) cnt[lane] = 0;
+
+  // per-lane state of the lane's four queries (rows 4 * lgrp + j). The
+  // integer kinds test candidates with an exact integer prefilter derived
+  // from the running kk-th distance (a superset: the rounding of the FP32
+  // distance is covered by a margin), then the exact key on the rare pass.
+  float thr_f[4], qa[4], qb[4];
+  int64_t qc[4], qsx[4];
+  int ithr[4];
+  bool qok[4];
+  int cnt_r[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int qi = wq + 4 * lgrp + j;
+    qok[j] = qi < qcount;
+    const int64_t qg = q0 + (qok[j] ? qi : 0);
+    thr_f[j] = INFINITY;
+    qa[j] = (KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP) ? qnorm[qg] : 0.0f;
+    qb[j] = (KIND == FP8_L2 || KIND == FP8_IP) ? qinv[qg] : 0.0f;
+    qc[j] = KIND == U8_L2 ? (int64_t)qn_u[qg] : 0;
+    qsx[j] = KIND == U8_L2 ? (int64_t)qs_u[qg] - 128 * (int64_t)d_true : 0;
+    ithr[j] = KIND == I8_IP ? -(1 << 30) : -600000000;  // no bound yet: everything passes
+  }
+  // I8_IP: dist = fmul_rn(-acc, s) <= thr  =>  acc >= floor(-thr / s) - 2.
+  // U8_L2: dist = fmul_rn(max(v, 0), s) with v = qn + xn - 2 acc - 256 (sq + sx)
+  // - 32768 d, and dist <= thr  =>  v <= vthr = floor(thr / s (1 + 2^-20)) + 4,
+  // i.e. 2 acc >= ithr + (xn - 256 sx) with ithr = qn - 256 sq - 32768 d - vthr
+  // (clamped: |acc| <= 128^2 d and d <= 1264 keep every sum inside int32).
+#define MVG_INT_THR(J)                                                                       \
+  {                                                                                          \
+    if (KIND == I8_IP) {                                                                     \
+      const float qt = -thr_f[J] / scale_sq;                                                 \
+      ithr[J] = thr_f[J] == INFINITY ? -(1 << 30)                                            \
+                                     : (int)fminf(fmaxf(floorf(qt), -1073741824.0f), 1073741824.0f) - 2; \
+    } else if (KIND == U8_L2) {                                                              \
+      int64_t vthr;                                                                          \
+      if (thr_f[J] == INFINITY) vthr = (int64_t)1 << 40;                                     \
+      else if (thr_f[J] < 0.0f) vthr = -((int64_t)1 << 40);                                  \
+      else {  /* r is NaN for scale 0: no bound */                                          \
+        const float r = thr_f[J] / scale_sq;                                                 \
+        vthr = r <= 1.0e12f ? (int64_t)floorf(r + r * 0x1p-20f) + 4 : ((int64_t)1 << 40);    \
+      }                                                                                      \
+      const int64_t c = qc[J] - 256 * qsx[J] - 32768 * (int64_t)d_true - vthr;               \
+      ithr[J] = (int)max<int64_t>(-600000000, min<int64_t>(c, 300000000));                   \
+    }                                                                                        \
+  }
+
+  const int nks = row_bytes / 64;    // 64-byte k steps of a row
+  const int nsl = (nks + 1) >> 1;    // 128-byte slices per step
+  const int srow = tid >> 3, sgrp = tid & 7;  // staging: 8 columns x 16 bytes per thread
+  const int sswz = wide_swz(sgrp);
+  uint4 p0, p1, p2, p3, p4, p5, p6, p7;
+  // loads slice SL of the step at B0 (the group past an odd row end re-reads
+  // the row's last group; its MFMA is skipped)
+#define MVG_LOAD_SLICE(B0, SL)                                                          \
+  {                                                                                     \
+    const int ko = min((SL) * 128 + sgrp * 16, row_bytes - 16);                          \
+    const uint8_t *src = X + ko;                                                         \
+    const int64_t lastc = b_end - 1;                                                     \
+    const int64_t c0 = (B0) + srow;                                                      \
+    p0 = *reinterpret_cast<const uint4 *>(src + min(c0, lastc) * row_bytes);             \
+    p1 = *reinterpret_cast<const uint4 *>(src + min(c0 + 32, lastc) * row_bytes);        \
+    p2 = *reinterpret_cast<const uint4 *>(src + min(c0 + 64, lastc) * row_bytes);        \
+    p3 = *reinterpret_cast<const uint4 *>(src + min(c0 + 96, lastc) * row_bytes);        \
+    p4 = *reinterpret_cast<const uint4 *>(src + min(c0 + 128, lastc) * row_bytes);       \
+    p5 = *reinterpret_cast<const uint4 *>(src + min(c0 + 160, lastc) * row_bytes);       \
+    p6 = *reinterpret_cast<const uint4 *>(src + min(c0 + 192, lastc) * row_bytes);       \
+    p7 = *reinterpret_cast<const uint4 *>(src + min(c0 + 224, lastc) * row_bytes);       \
+  }
+#define MVG_STORE_SLICE(SB)                                                              \
+  {                                                                                      \
+    if (KIND == U8_L2) {                                                                 \
+      p0.x ^= 0x80808080u; p0.y ^= 0x80808080u; p0.z ^= 0x80808080u; p0.w ^= 0x80808080u; \
+      p1.x ^= 0x80808080u; p1.y ^= 0x80808080u; p1.z ^= 0x80808080u; p1.w ^= 0x80808080u; \
+      p2.x ^= 0x80808080u; p2.y ^= 0x80808080u; p2.z ^= 0x80808080u; p2.w ^= 0x80808080u; \
+      p3.x ^= 0x80808080u; p3.y ^= 0x80808080u; p3.z ^= 0x80808080u; p3.w ^= 0x80808080u; \
+      p4.x ^= 0x80808080u; p4.y ^= 0x80808080u; p4.z ^= 0x80808080u; p4.w ^= 0x80808080u; \
+      p5.x ^= 0x80808080u; p5.y ^= 0x80808080u; p5.z ^= 0x80808080u; p5.w ^= 0x80808080u; \
+      p6.x ^= 0x80808080u; p6.y ^= 0x80808080u; p6.z ^= 0x80808080u; p6.w ^= 0x80808080u; \
+      p7.x ^= 0x80808080u; p7.y ^= 0x80808080u; p7.z ^= 0x80808080u; p7.w ^= 0x80808080u; \
+    }                                                                                    \
+    uint8_t *plane = bst + (SB) * kBSlice + sgrp * kBPlane;                              \
+    *reinterpret_cast<uint4 *>(plane + ((srow ^ sswz) * 16)) = p0;                       \
+    *reinterpret_cast<uint4 *>(plane + (((srow + 32) ^ sswz) * 16)) = p1;                \
+    *reinterpret_cast<uint4 *>(plane + (((srow + 64) ^ sswz) * 16)) = p2;                \
+    *reinterpret_cast<uint4 *>(plane + (((srow + 96) ^ sswz) * 16)) = p3;                \
+    *reinterpret_cast<uint4 *>(plane + (((srow + 128) ^ sswz) * 16)) = p4;               \
+    *reinterpret_cast<uint4 *>(plane + (((srow + 160) ^ sswz) * 16)) = p5;               \
+    *reinterpret_cast<uint4 *>(plane + (((srow + 192) ^ sswz) * 16)) = p6;               \
+    *reinterpret_cast<uint4 *>(plane + (((srow + 224) ^ sswz) * 16)) = p7;               \
+  }
+
+  MVG_LOAD_SLICE(b_begin, 0)
+  MVG_STORE_SLICE(0)
+  __syncthreads();
+  int sb = 0;
+  const uint8_t *arow = atile + (wq + lrow) * lay.apitch + lgrp * 16;
+  floatx4 accf[16];
+  intx4 acci[16];
+#pragma unroll
+  for (int t = 0; t < 16; t++) {
+    accf[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    acci[t] = intx4{0, 0, 0, 0};
+  }
+  for (int64_t b0 = b_begin; b0 < b_end; b0 += kStepCols) {
+    for (int sl = 0; sl < nsl; sl++) {
+      int nsl1 = sl + 1;
+      int64_t nb0 = b0;
+      if (nsl1 == nsl) {
+        nsl1 = 0;
+        nb0 = b0 + kStepCols;
+      }
+      const bool more = nb0 < b_end;
+      if (more) MVG_LOAD_SLICE(nb0, nsl1)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int ks = 2 * sl + h;
+      if (ks < nks) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(arow + ks * 64);
+        const int g = 4 * h + lgrp;
+        const uint8_t *plane = bst + sb * kBSlice + g * kBPlane + ((lrow ^ wide_swz(g)) * 16);
+        // the 16 fragment reads are issued together, ahead of their MFMAs
+        uint4 bfr[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) bfr[t] = *reinterpret_cast<const uint4 *>(plane + t * 256);
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+          const uint4 b = bfr[t];
+          if (KIND == BF16_L2 || KIND == BF16_IP) {
+            accf[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                              __builtin_bit_cast(bf16x8, b), accf[t], 0, 0, 0);
+          } else if (KIND == FP8_L2 || KIND == FP8_IP) {
+            const long alo = (long)(((uint64_t)a.y << 32) | a.x), ahi = (long)(((uint64_t)a.w << 32) | a.z);
+            const long blo = (long)(((uint64_t)b.y << 32) | b.x), bhi = (long)(((uint64_t)b.w << 32) | b.z);
+            accf[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(alo, blo, accf[t], 0, 0, 0);
+            accf[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(ahi, bhi, accf[t], 0, 0, 0);
+          } else {  // int8 codes (U8_L2 rebased when staged)
+            acci[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(intx4, a),
+                                                            __builtin_bit_cast(intx4, b), acci[t], 0, 0, 0);
+          }
+        }
+        // schedule: the A and 16 B fragment reads first, then the MFMAs
+        __builtin_amdgcn_sched_group_barrier(0x100, 17, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, KIND == FP8_L2 || KIND == FP8_IP ? 32 : 16, 0);
+      }
+    }
+      if (more) MVG_STORE_SLICE(sb ^ 1)
+      __syncthreads();
+      sb ^= 1;
+    }
+    {
+      const int64_t pb0 = b0;
+      const int lim = (int)min<int64_t>(kStepCols, b_end - pb0) - 1;
+#pragma unroll
+      for (int t = 0; t < 16; t++) {
+        const int64_t col = pb0 + t * 16 + lrow;
+        const bool col_ok = t * 16 + lrow <= lim && !(excl && col < excl_len && excl[col] > 0);
+        const int64_t cc = col_ok ? col : b_begin;
+        float xa = 0.0f, xb = 0.0f;
+        int xcorr = 0;
+        if (KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP) xa = xnorm[cc];
+        if (KIND == FP8_L2 || KIND == FP8_IP) xb = xinv[cc];
+        if (KIND == U8_L2) xcorr = (int)xn_u[cc] - 256 * ((int)xs_u[cc] - 128 * d_true);
+        float dist[4];
+        bool pass[4];
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (KIND == I8_IP) {
+            pass[j] = acci[t][j] >= ithr[j];
+          } else if (KIND == U8_L2) {
+            pass[j] = 2 * acci[t][j] >= ithr[j] + xcorr;
+          } else {
+            if (KIND == BF16_L2) {
+              dist[j] = (qa[j] + xa) - 2.0f * accf[t][j];
+            } else if (KIND == BF16_IP) {
+              dist[j] = -accf[t][j];
+            } else if (KIND == FP8_L2) {
+              dist[j] = (qa[j] + xa) - 2.0f * (accf[t][j] * (qb[j] * xb));
+            } else {
+              dist[j] = (qa[j] + xa) - accf[t][j] * (qb[j] * xb);
+            }
+            pass[j] = dist[j] <= thr_f[j];
+          }
+          pass[j] = pass[j] && col_ok && qok[j];
+          any |= pass[j];
+        }
+        if (__ballot(any) == 0) continue;
+        const uint64_t gmask = 0xFFFFull << (16 * lgrp);
+        const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          if (KIND == I8_IP) {
+            dist[j] = __fmul_rn(-(float)acci[t][j], scale_sq);
+          } else if (KIND == U8_L2) {
+            const int64_t sx = (int64_t)xs_u[cc] - 128 * (int64_t)d_true;
+            const int64_t dot = (int64_t)acci[t][j] + 128 * (qsx[j] + sx) + 16384 * (int64_t)d_true;
+            const int64_t v = qc[j] + (int64_t)xn_u[cc] - 2 * dot;
+            dist[j] = __fmul_rn((float)(uint32_t)(v > 0 ? v : 0), scale_sq);
+          }
+          const uint64_t key = pack_key(dist[j], (uint32_t)col);
+          // exact key order against the stored kk-th key (ties: lower id)
+          const bool take = pass[j] && dist[j] <= thr_f[j] &&
+                            (thr_f[j] == INFINITY || key < top[(4 * lgrp + j) * kk + kk - 1]);
+          const uint64_t m = __ballot(take);
+          if (m == 0) continue;
+          if (take) buf[(4 * lgrp + j) * kWCap + cnt_r[j] + __popcll(m & gmask & below)] = key;
+          cnt_r[j] += __popcll(m & gmask);
+        }
+        // room for the next tile's 16 columns per query, else flush
+        bool full = false;
+#pragma unroll
+        for (int j = 0; j < 4; j++) full |= cnt_r[j] > kWCap - 16;
+        if (__ballot(full) != 0) {
+          if (lrow == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) cnt[4 * lgrp + j] = cnt_r[j];
+          }
+          for (int qq = 0; qq < 16; qq++)
+            if (cnt[qq] > kWCap - 16) wide_flush(top, buf, cnt, qq, kk);
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            cnt_r[j] = cnt[4 * lgrp + j];
+            const uint64_t th = top[(4 * lgrp + j) * kk + kk - 1];
+            thr_f[j] = th == kSentinel64 ? INFINITY : key2f((uint32_t)(th >> 32));
+            MVG_INT_THR(j)
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 16; t++) {
+        accf[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        acci[t] = intx4{0, 0, 0, 0};
+      }
+    }
+  }
+#undef MVG_INT_THR
+#undef MVG_LOAD_SLICE
+#undef MVG_STORE_SLICE
+  if (lrow == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) cnt[4 * lgrp + j] = cnt_r[j];
+  }
+  for (int qq = 0; qq < 16; qq++)
+    if (cnt[qq] > 0) wide_flush(top, buf, cnt, qq, kk);
+  for (int qq = 0; qq < 16; qq++) {
+    if (wq + qq >= qcount) break;
+    if (lane < kk) part[((q0 + wq + qq) * S + split) * (int64_t)kk + lane] = top[qq * kk + lane];
+  }
+}
+
 // FP32 rows -> bf16 rows padded to row_elems (round to nearest even), plus
 // the FP32 squared norm of the rounded row.
 __global__ void to_bf16_kernel(const float *__restrict__ x, int64_t n, int d, int row_elems,
@@ -218,7 +578,68 @@ __global__ void to_bf16_kernel(const float *__restrict__ x, int64_t n, int d, in
   if (lane == 0 && norms) norms[row] = acc;
 }
 
+// FP32 rows -> FP8 E4M3 rows padded to row_bytes (zero bytes), each scaled
+// by the power of two 2^e that puts its largest magnitude in [224, 448];
+// inv[row] = 2^-e, and norms (nullable) = FP32 squared norm of the
+// dequantized row v~ (half_diff false) or (|v~|^2 - |v|^2) / 2 (half_diff
+// true, summed as (v~ - v)(v~ + v) per element). One wavefront per row.
+__global__ void to_fp8_kernel(const float *__restrict__ x, int64_t n, int d, int row_bytes,
+                              uint8_t *__restrict__ out, float *__restrict__ inv,
+                              float *__restrict__ norms, bool half_diff) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const float *xr = x + row * d;
+  float mx = 0.0f;
+  for (int j = lane; j < d; j += 64) mx = fmaxf(mx, fabsf(xr[j]));
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  int e = 0;
+  if (mx > 0.0f) {
+    int ex;
+    (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
+    e = 9 - ex;             // mx * 2^e in [256, 512) ... one step down if >= 448
+    if (ldexpf(mx, e) > 448.0f) e -= 1;
+  }
+  const float s = ldexpf(1.0f, e), is = ldexpf(1.0f, -e);
+  float acc = 0.0f;
+  uint8_t *orow = out + row * row_bytes;
+  for (int j0 = 4 * lane; j0 < row_bytes; j0 += 256) {
+    float v[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[c] = j0 + c < d ? xr[j0 + c] * s : 0.0f;
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], w, true);
+    const float r[4] = {__builtin_amdgcn_cvt_f32_fp8(w, 0) * is, __builtin_amdgcn_cvt_f32_fp8(w, 1) * is,
+                        __builtin_amdgcn_cvt_f32_fp8(w, 2) * is, __builtin_amdgcn_cvt_f32_fp8(w, 3) * is};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      if (half_diff) {
+        const float xv = v[c] * is;  // the original value (exact: power-of-two scaling)
+        acc = fma_(r[c] - xv, r[c] + xv, acc);
+      } else {
+        acc = fma_(r[c], r[c], acc);
+      }
+    }
+    *reinterpret_cast<int *>(orow + j0) = w;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) {
+    inv[row] = is;
+    if (norms) norms[row] = half_diff ? 0.5f * acc : acc;
+  }
+}
+
 }  // namespace
+
+void to_fp8(const float *x, int64_t n, int d, uint8_t *out, float *inv, float *norms,
+            hipStream_t s, bool half_diff) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(to_fp8_kernel, dim3(ceil_div(n, 4)), dim3(256), 0, s, x, n, d,
+                     gemm_row_bytes(FP8_L2, d), out, inv, norms, half_diff);
+  MVG_LAUNCH_CHECK("to_fp8_kernel");
+}
 
 int gemm_row_bytes(int kind, int d) {
   const int bytes = (kind == BF16_L2 || kind == BF16_IP) ? 2 * d : d;
@@ -236,8 +657,53 @@ void to_bf16(const float *x, int64_t n, int d, uint16_t *out, float *norms, hipS
 void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t n, int d,
                const float *qnorm, const float *xnorm, const uint32_t *qn_u, const uint32_t *qs_u,
                const uint32_t *xn_u, const uint32_t *xs_u, float scale_sq, int kk, const uint8_t *excl, int64_t excl_len, DevBuf &part,
-               int &nsplit, hipStream_t s) {
+               int &nsplit, hipStream_t s, const float *qinv, const float *xinv) {
   const int row_bytes = gemm_row_bytes(kind, d);
+  const WideLayout wlay(row_bytes, kk);
+  if (kk <= kWMaxK && nq > kQT && wlay.total <= 160 * 1024) {
+    const int qtiles = ceil_div(nq, kWQ);
+    // Splits of the base per query tile: one workgroup per CU at a time, so a
+    // CU streams ceil(qtiles * S / cus) / S of the base; every split costs a
+    // selection warm-up and kk more candidates to merge, so take the fewest
+    // splits within 10% of the best streaming balance.
+    const int cus = std::max(1, device_cus());
+    const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(n, kStepCols)));
+    double best = 1e30;
+    for (int64_t c = 1; c <= smax; c++) best = std::min(best, (double)ceil_div(qtiles * c, cus) / c);
+    int64_t S = 1;
+    while (S < smax && (double)ceil_div(qtiles * S, cus) / S > 1.1 * best) S++;
+    int64_t split_len = ((ceil_div(n, S) + kStepCols - 1) / kStepCols) * kStepCols;
+    S = ceil_div(n, split_len);
+    nsplit = (int)S;
+    part.reserve(sizeof(uint64_t) * (size_t)nq * S * kk);
+    const void *fn;
+    switch (kind) {
+      case BF16_L2: fn = (const void *)gemm_wide_kernel<BF16_L2>; break;
+      case BF16_IP: fn = (const void *)gemm_wide_kernel<BF16_IP>; break;
+      case I8_IP: fn = (const void *)gemm_wide_kernel<I8_IP>; break;
+      case FP8_L2: fn = (const void *)gemm_wide_kernel<FP8_L2>; break;
+      case FP8_IP: fn = (const void *)gemm_wide_kernel<FP8_IP>; break;
+      default: fn = (const void *)gemm_wide_kernel<U8_L2>; break;
+    }
+    if (wlay.total > 64 * 1024)
+      MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, wlay.total));
+    dim3 grid(qtiles, (unsigned)S);
+#define MVG_WIDE(K)                                                                               \
+  hipLaunchKernelGGL(gemm_wide_kernel<K>, grid, dim3(kNT), wlay.total, s, Q, X, nq, n, row_bytes, d, \
+                     qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len, excl,        \
+                     excl_len, part.as<uint64_t>(), (int)S, qinv, xinv);
+    switch (kind) {
+      case BF16_L2: MVG_WIDE(BF16_L2) break;
+      case BF16_IP: MVG_WIDE(BF16_IP) break;
+      case I8_IP: MVG_WIDE(I8_IP) break;
+      case FP8_L2: MVG_WIDE(FP8_L2) break;
+      case FP8_IP: MVG_WIDE(FP8_IP) break;
+      default: MVG_WIDE(U8_L2) break;
+    }
+#undef MVG_WIDE
+    MVG_LAUNCH_CHECK("gemm_wide_kernel");
+    return;
+  }
   const GemmLayout lay(row_bytes, kk);
   if (lay.total > 150 * 1024) throw Error(kInvalid, "GEMM top-k workspace exceeds LDS");
   const int qtiles = ceil_div(nq, kQT);
@@ -252,6 +718,8 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
     case BF16_L2: fn = (const void *)gemm_topk_kernel<BF16_L2>; break;
     case BF16_IP: fn = (const void *)gemm_topk_kernel<BF16_IP>; break;
     case I8_IP: fn = (const void *)gemm_topk_kernel<I8_IP>; break;
+    case FP8_L2: fn = (const void *)gemm_topk_kernel<FP8_L2>; break;
+    case FP8_IP: fn = (const void *)gemm_topk_kernel<FP8_IP>; break;
     default: fn = (const void *)gemm_topk_kernel<U8_L2>; break;
   }
   if (lay.total > 64 * 1024)
@@ -261,11 +729,13 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
   hipLaunchKernelGGL(gemm_topk_kernel<K>, grid, dim3(kNT), lay.total, s, Q, X, nq, n, row_bytes, d, \
                      qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len, excl,        \
                      excl_len,                                                                     \
-                     part.as<uint64_t>(), (int)S);
+                     part.as<uint64_t>(), (int)S, qinv, xinv);
   switch (kind) {
     case BF16_L2: MVG_GEMM(BF16_L2) break;
     case BF16_IP: MVG_GEMM(BF16_IP) break;
     case I8_IP: MVG_GEMM(I8_IP) break;
+    case FP8_L2: MVG_GEMM(FP8_L2) break;
+    case FP8_IP: MVG_GEMM(FP8_IP) break;
     default: MVG_GEMM(U8_L2) break;
   }
 #undef MVG_GEMM

@@ -130,19 +130,27 @@ void ivfflat_search(const IvfFlatArgs &a, IvfFlatWork &w, hipStream_t s);
 
 // ---- gemm.hip ----
 // Kinds of the MFMA GEMM-form distance: 0 bf16 L2, 1 bf16 IP, 2 int8 IP
-// (symmetric SQ8), 3 uint8 L2 rebased to int8 (affine SQ8).
-enum { kGemmBf16L2 = 0, kGemmBf16IP = 1, kGemmI8IP = 2, kGemmU8L2 = 3 };
+// (symmetric SQ8), 3 uint8 L2 rebased to int8 (affine SQ8), 4 FP8 E4M3 L2,
+// 5 FP8 E4M3 IP (rows scaled per row by a power of two).
+enum { kGemmBf16L2 = 0, kGemmBf16IP = 1, kGemmI8IP = 2, kGemmU8L2 = 3, kGemmFp8L2 = 4, kGemmFp8IP = 5 };
 // Bytes per padded operand row (multiple of 64).
 int gemm_row_bytes(int kind, int d);
 // FP32 -> bf16 rows padded to gemm_row_bytes; norms (nullable) of the
 // rounded rows.
 void to_bf16(const float *x, int64_t n, int d, uint16_t *out, float *norms, hipStream_t s);
+// FP32 -> FP8 E4M3 rows padded to gemm_row_bytes, each scaled by 2^e so that
+// its largest magnitude lies in [224, 448]; inv[row] = 2^-e; norms (nullable)
+// of the dequantized rows v~, or (|v~|^2 - |v|^2) / 2 with half_diff (the
+// per-row terms of the FP8 inner-product estimate).
+void to_fp8(const float *x, int64_t n, int d, uint8_t *out, float *inv, float *norms,
+            hipStream_t s, bool half_diff = false);
 // Per-(query, base split) top-kk keys part[nq][nsplit][kk] of the GEMM-form
 // distance; reduce with launch_merge over nsplit * kk candidates.
 void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t n, int d,
                const float *qnorm, const float *xnorm, const uint32_t *qn_u, const uint32_t *qs_u,
                const uint32_t *xn_u, const uint32_t *xs_u, float scale_sq, int kk,
-               const uint8_t *excl, int64_t excl_len, DevBuf &part, int &nsplit, hipStream_t s);
+               const uint8_t *excl, int64_t excl_len, DevBuf &part, int &nsplit, hipStream_t s,
+               const float *qinv = nullptr, const float *xinv = nullptr);
 
 // ---- sq8.hip ----
 void sq8_minmax(const float *x, int64_t count, float *out2, DevBuf &tmp, hipStream_t s);

@@ -56,9 +56,10 @@ class IndexFlat:
         return int(_lib.load().mvg_flat_ntotal(self._h))
 
     def set_search_mode(self, precision: str = "plain", k_factor: int = 1) -> None:
-        """"plain" FP32 (reference order) or "mixed" (bf16 MFMA candidates,
-        exact FP32 rerank of k * k_factor when k_factor > 1)."""
-        p = {"plain": 0, "mixed": 1}[precision]
+        """"plain" FP32 (reference order), "mixed" / "mixed-bf16" (BF16 MFMA
+        candidates) or "mixed-fp8" (FP8 E4M3 MFMA candidates); mixed forms
+        rerank k * k_factor candidates exactly in FP32 when k_factor > 1."""
+        p = {"plain": 0, "mixed": 1, "mixed-bf16": 1, "mixed-fp8": 2}[precision]
         _lib.check(_lib.load().mvg_flat_set_search_mode(self._h, p, int(k_factor)))
 
     def add(self, x) -> None:

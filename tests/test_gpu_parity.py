@@ -400,6 +400,24 @@ def test_sq8_codes_and_search_match_oracle(gpu, metric, d, use_mfma):
         assert_same(Dg, Ig, Do, Io)
 
 
+@pytest.mark.parametrize("metric", [0, 1])
+def test_sq8_many_query_tiles_with_exclusion(gpu, metric):
+    # 300 queries span several 64-query workgroup tiles of the int8 MFMA form
+    # (last one partial); codes of a few distinct values give many exact ties.
+    rng = np.random.default_rng(60 + metric)
+    x = rng.integers(0, 6, (15000, 48)).astype(np.float32)
+    q = rng.integers(0, 6, (300, 48)).astype(np.float32)
+    ex = (rng.random(15000) < 0.25).astype(np.uint8)
+    gx = gpu.IndexFlatSQ8(48, metric)
+    ox = o.SQ8Index(48, metric)
+    gx.add(x)
+    ox.add(x)
+    for k, kf in [(1, 1), (10, 1), (16, 4), (64, 1), (10, 6)]:
+        Dg, Ig = gx.search(q, k, k_factor=kf, exclude=ex)
+        Do, Io = ox.search(q, k, k_factor=kf, exclude=ex)
+        assert_same(Dg, Ig, Do, Io)
+
+
 def test_sq8_ip_ordering_and_scale(gpu):
     ix = gpu.IndexFlatSQ8(4, 1)
     ix.add(np.array([[1, 0, 0, 0], [-1, 0, 0, 0], [0.5, 0, 0, 0]], np.float32))
