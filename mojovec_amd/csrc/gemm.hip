@@ -235,12 +235,13 @@ constexpr int kBPlane = kStepCols * 16;   // one 16-byte group of a staged slice
 constexpr int kBSlice = 8 * kBPlane;      // 128 bytes of k for 256 rows
 
 struct WideLayout {
-  int apitch, a, b, top, buf, cnt, total;
+  int apitch, a, b, colv, top, buf, cnt, total;
   __host__ __device__ WideLayout(int row_bytes, int kk) {
     apitch = row_bytes + 16;  // padded rows: the 16 lanes of a fragment read hit distinct banks
     a = 0;
     b = (kWQ * apitch + 15) & ~15;
-    top = b + 2 * kBSlice;
+    colv = b + 2 * kBSlice;    // per-column terms of a step, three steps in turn
+    top = colv + 3 * kStepCols * 8;
     buf = top + 8 * kWQ * kk;
     cnt = buf + 8 * kWQ * kWCap;
     total = cnt + 4 * kWQ;
@@ -396,8 +397,31 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
     *reinterpret_cast<uint4 *>(plane + (((srow + 224) ^ sswz) * 16)) = p7;               \
   }
 
+  // Per-column terms of a step (norm, scale or code-sum correction), loaded
+  // with the step's first slice (one column per thread) and kept in LDS until
+  // the step's epilogue. Three buffers in turn: with one slice per step the
+  // terms of step s + 2 are written while a slower wavefront may still read
+  // those of step s.
+  constexpr bool kColTerms = KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP || KIND == U8_L2;
+  uint2 *colv = reinterpret_cast<uint2 *>(smem + lay.colv);
+  uint2 cterm = make_uint2(0, 0);
+#define MVG_LOAD_TERMS(B0)                                                                   \
+  if (kColTerms) {                                                                           \
+    const int64_t c = (B0) + tid;                                                            \
+    cterm = make_uint2(0, 0);                                                                \
+    if (c < b_end) {                                                                         \
+      if (KIND == U8_L2)                                                                     \
+        cterm.x = (uint32_t)((int)xn_u[c] - 256 * ((int)xs_u[c] - 128 * d_true));           \
+      else                                                                                   \
+        cterm.x = __float_as_uint(xnorm[c]);                                                 \
+      if (KIND == FP8_L2 || KIND == FP8_IP) cterm.y = __float_as_uint(xinv[c]);              \
+    }                                                                                        \
+  }
+  int sp = 0;  // buffer of the current step's terms (0..2)
   MVG_LOAD_SLICE(b_begin, 0)
+  MVG_LOAD_TERMS(b_begin)
   MVG_STORE_SLICE(0)
+  if (kColTerms) colv[tid] = cterm;
   __syncthreads();
   int sb = 0;
   const uint8_t *arow = atile + (wq + lrow) * lay.apitch + lgrp * 16;
@@ -418,6 +442,7 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
       }
       const bool more = nb0 < b_end;
       if (more) MVG_LOAD_SLICE(nb0, nsl1)
+      if (more && nsl1 == 0) MVG_LOAD_TERMS(nb0)
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int ks = 2 * sl + h;
@@ -451,11 +476,14 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
       }
     }
       if (more) MVG_STORE_SLICE(sb ^ 1)
+      if (kColTerms && more && nsl1 == 0) colv[(sp == 2 ? 0 : sp + 1) * kStepCols + tid] = cterm;
       __syncthreads();
       sb ^= 1;
     }
     {
       const int64_t pb0 = b0;
+      const uint2 *cv = colv + sp * kStepCols + lrow;
+      sp = sp == 2 ? 0 : sp + 1;
       const int lim = (int)min<int64_t>(kStepCols, b_end - pb0) - 1;
 #pragma unroll
       for (int t = 0; t < 16; t++) {
@@ -464,9 +492,12 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
         const int64_t cc = col_ok ? col : b_begin;
         float xa = 0.0f, xb = 0.0f;
         int xcorr = 0;
-        if (KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP) xa = xnorm[cc];
-        if (KIND == FP8_L2 || KIND == FP8_IP) xb = xinv[cc];
-        if (KIND == U8_L2) xcorr = (int)xn_u[cc] - 256 * ((int)xs_u[cc] - 128 * d_true);
+        if (kColTerms) {
+          const uint2 v = cv[t * 16];
+          xa = __uint_as_float(v.x);
+          xb = __uint_as_float(v.y);
+          xcorr = (int)v.x;
+        }
         float dist[4];
         bool pass[4];
         bool any = false;
@@ -541,6 +572,7 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
     }
   }
 #undef MVG_INT_THR
+#undef MVG_LOAD_TERMS
 #undef MVG_LOAD_SLICE
 #undef MVG_STORE_SLICE
   if (lrow == 0) {
