@@ -63,8 +63,10 @@ int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_
 /* Search form (no reference counterpart). precision 0: plain FP32 in the
  * reference's operation order; 1: mixed, candidates from a BF16 GEMM-form
  * distance on MFMA (FP32 accumulate); 2: mixed with FP8 E4M3 operands (each
- * row scaled by a power of two, FP32 accumulate). Mixed candidates are
- * reranked with exact FP32 distances when k_factor > 1. */
+ * row scaled by a power of two, FP32 accumulate; inner products are scored
+ * as c_q + c_x - q~.x~ with c = (|v~|^2 - |v|^2) / 2 over the dequantized
+ * row v~). Mixed candidates are reranked with exact FP32 distances when
+ * k_factor > 1; with k_factor 1 the candidate distances are returned. */
 int mvg_flat_set_search_mode(mvg_flat *h, int precision, int k_factor);
 /* FP8 E4M3 row encoding of the mixed Flat form (no reference counterpart):
  * codes [n x roundup(d, 64)] (zero padded), inv_scale [n] = 2^-e where 2^e
