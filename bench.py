@@ -101,11 +101,12 @@ def main(argv=None):
     ap.add_argument("--no-flat", action="store_true",
                     help="skip the 1M x 128 exact FP32 Flat vs SQ8 (int8 MFMA) leg")
     ap.add_argument("--workload", default="ivfpq",
-                    choices=["ivfpq", "flat", "sq8", "batch", "serve", "ivfflat"],
+                    choices=["ivfpq", "flat", "sq8", "batch", "serve", "ivfflat", "pairdist"],
                     help="ivfpq: the headline 1M x 128 line; flat / sq8 / batch: the Flat "
                          "100k x 128, SQ8 1M x 768 and skewed 10M x 96 side measurements; "
                          "serve: continuous batching under Poisson arrivals; ivfflat: exact "
-                         "FP32 IVF-Flat on the 1M x 128 data")
+                         "FP32 IVF-Flat on the 1M x 128 data; pairdist: the batched distance "
+                         "service of a CPU graph search (Flat 1M x 128, SQ8 1M x 768)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures a single MI355X; --gpus {args.gpus} is not supported: "
@@ -116,6 +117,8 @@ def main(argv=None):
         return bench_flat(args)
     if args.workload == "sq8":
         return bench_sq8(args)
+    if args.workload == "pairdist":
+        return bench_pairdist(args)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "serve":
@@ -691,6 +694,49 @@ def bench_sq8(args):
                 log(f"sq8 {key}: {out[key]}")
         del fx
     print(json.dumps({"workload": "SQ8 Flat 1,000,000 x 768 unit-norm, 10,000 queries, k=10 "
+                                  "(host buffers, PCIe included)", "forms": out}), flush=True)
+    return 0
+
+
+def bench_pairdist(args):
+    """Batched distance service (FlatDistanceComputer / SQ8DistanceComputer
+    for a CPU graph search): 10,000 queries x 256 candidate rows drawn
+    uniformly from 1M rows (2.56M pairs per call), Flat 1M x 128 (both FP32
+    orders) and SQ8 cosine 1M x 768; host buffers (ids, queries and results
+    cross PCIe). Algorithmic bytes per pair: the row (4 d for FP32, d for SQ8
+    codes); the query rows stay in cache."""
+    import numpy as np
+    import torch
+
+    import mojovec_amd as mv
+
+    rng = np.random.default_rng(41)
+    nq, per = 10_000, 256
+    out = {}
+    for name, d, sq8 in (("flat_l2_1Mx128", 128, False), ("sq8_cosine_1Mx768", 768, True)):
+        x = rng.standard_normal((1_000_000, d), dtype=np.float32)
+        q = rng.standard_normal((nq, d), dtype=np.float32)
+        ids = rng.integers(0, 1_000_000, (nq, per))
+        if sq8:
+            ix = mv.IndexFlatSQ8(d, 1)
+            ix.add(x / np.linalg.norm(x, axis=1, keepdims=True))
+            forms = (("sq8", lambda: ix.distances(q, ids)),)
+            row_bytes = d
+        else:
+            ix = mv.IndexFlat(d)
+            ix.add(x)
+            forms = (("distance", lambda: ix.distances(q, ids, order="distance")),
+                     ("batch4", lambda: ix.distances(q, ids, order="batch4")))
+            row_bytes = 4 * d
+        log(f"pairdist {name} index ready")
+        for form, fn in forms:
+            dt = _timed(torch, fn, max(2, args.steps // 2), 1)
+            pairs = nq * per
+            out[f"{name}_{form}"] = {"pairs_per_s": round(pairs / dt, 1), "ms": round(dt * 1e3, 3),
+                                     "row_gb_per_s_wall": round(pairs * row_bytes / dt / 1e9, 1)}
+            log(f"pairdist {name} {form}: {out[f'{name}_{form}']}")
+        del ix, x
+    print(json.dumps({"workload": "distance service, 10,000 queries x 256 random rows of 1M "
                                   "(host buffers, PCIe included)", "forms": out}), flush=True)
     return 0
 

@@ -74,6 +74,20 @@ int mvg_flat_set_search_mode(mvg_flat *h, int precision, int k_factor);
  * dequantized rows. */
 int mvg_fp8_encode(const float *x, int64_t n, int d, uint8_t *codes, float *inv_scale,
                    float *norms);
+/* Batched distance service for a CPU graph search (HNSW build and search
+ * stay on the CPU; their candidate evaluations can be sent here in batches).
+ * FlatDistanceComputer (mojovec/index/index_flat.mojo:36-115) for nq x
+ * per_query pairs: out[i * per_query + j] = distance of query i to row
+ * ids[i * per_query + j], bit-identical to distance() (order 0:
+ * l2_distance_simd[8] / -inner_product_simd[8], :60-68) or distance_batch4()
+ * (order 1: l2_distance_simd_batch4[8] / -inner_product_simd_batch4[8],
+ * :70-90; the form hnsw_graph.mojo:315 calls). ids must lie in [0, ntotal). */
+int mvg_flat_distances(mvg_flat *h, const float *q, int64_t nq, const int64_t *ids,
+                       int64_t per_query, int order, float *out);
+/* FlatDistanceComputer.symmetric_distance (index_flat.mojo:92-99) for row
+ * pairs (a[p], b[p]). */
+int mvg_flat_pair_distances(mvg_flat *h, const int64_t *a, const int64_t *b, int64_t npairs,
+                            float *out);
 int64_t mvg_flat_ntotal(const mvg_flat *h);
 void mvg_flat_destroy(mvg_flat *h);
 
@@ -90,6 +104,15 @@ int mvg_sq8_search(mvg_sq8 *h, const float *q, int64_t nq, int k, int k_factor,
                    const uint8_t *exclude, int64_t exclude_len, float *distances,
                    int64_t *labels);
 /* Calibration state: global_min, global_max, scale (:203-206). */
+/* SQ8DistanceComputer (index_flat_sq8.mojo:21-190) for nq x per_query
+ * pairs: the query is encoded as get_distance_computer does (:527-560), the
+ * distance is the code distance of distance() / distance_batch4() (equal:
+ * exact code dots), sq8.mojo:62-79. ids must lie in [0, ntotal). */
+int mvg_sq8_distances(mvg_sq8 *h, const float *q, int64_t nq, const int64_t *ids,
+                      int64_t per_query, float *out);
+/* SQ8DistanceComputer.symmetric_distance (index_flat_sq8.mojo:148-177). */
+int mvg_sq8_pair_distances(mvg_sq8 *h, const int64_t *a, const int64_t *b, int64_t npairs,
+                           float *out);
 int mvg_sq8_params(const mvg_sq8 *h, float *gmin, float *gmax, float *scale, int64_t *ntotal);
 /* Codes [ntotal x d] and code norms [ntotal] (codes_u8 / norms_u32). */
 int mvg_sq8_get_codes(mvg_sq8 *h, uint8_t *codes, uint32_t *norms);

@@ -42,12 +42,16 @@ SIGNATURES = {
     "mvg_flat_add": (_i, [_p, _p, _i64]),
     "mvg_flat_search": (_i, [_p, _p, _i64, _i, _p, _i64, _p, _p]),
     "mvg_flat_set_search_mode": (_i, [_p, _i, _i]),
+    "mvg_flat_distances": (_i, [_p, _p, _i64, _p, _i64, _i, _p]),
+    "mvg_flat_pair_distances": (_i, [_p, _p, _p, _i64, _p]),
     "mvg_fp8_encode": (_i, [_p, _i64, _i, _p, _p, _p]),
     "mvg_flat_ntotal": (_i64, [_p]),
     "mvg_flat_destroy": (None, [_p]),
     "mvg_sq8_create": (_i, [_i, _i, ctypes.POINTER(_p)]),
     "mvg_sq8_add": (_i, [_p, _p, _i64]),
     "mvg_sq8_search": (_i, [_p, _p, _i64, _i, _i, _p, _i64, _p, _p]),
+    "mvg_sq8_distances": (_i, [_p, _p, _i64, _p, _i64, _p]),
+    "mvg_sq8_pair_distances": (_i, [_p, _p, _p, _i64, _p]),
     "mvg_sq8_params": (_i, [_p, ctypes.POINTER(_f), ctypes.POINTER(_f), ctypes.POINTER(_f),
                             ctypes.POINTER(_i64)]),
     "mvg_sq8_get_codes": (_i, [_p, _p, _p]),

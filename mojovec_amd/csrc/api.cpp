@@ -129,6 +129,7 @@ struct mvg_flat {
   bool bf16_dirty = true, fp8_dirty = true;
   DevBuf base, part, q, D, I, excl, base_bf16, base_norm, qbf, qnorm, keys;
   DevBuf base_fp8, base_fp8_inv, base_fp8_norm, qinv;
+  DevBuf pair_a, pair_b, pair_out;  // batched distance service
   hipStream_t s = nullptr;
 };
 
@@ -137,12 +138,20 @@ struct mvg_sq8 {
   int64_t ntotal = 0;
   float gmin = 3.40282347e38f, gmax = -3.40282347e38f, scale = 1.0f;  // index_flat_sq8.mojo:234-236
   DevBuf f32, codes, norms, sums, tmp, q, D, I, excl;
+  DevBuf pair_a, pair_b, pair_out;  // batched distance service
   int use_mfma = 1;
   mvg::Sq8Work work;
   hipStream_t s = nullptr;
 };
 
 namespace {
+
+// Rejects ids outside [0, n): the distance service reads rows by id.
+void check_ids(const int64_t *ids, int64_t count, int64_t n, const char *what) {
+  for (int64_t i = 0; i < count; i++)
+    if (ids[i] < 0 || ids[i] >= n)
+      throw Error(mvg::kInvalid, std::string(what) + " must lie in [0, ntotal)");
+}
 
 void ivf_sync_lists(mvg_ivfpq *h) {
   if (!h->dirty) return;
@@ -305,6 +314,43 @@ int mvg_flat_set_search_mode(mvg_flat *h, int precision, int k_factor) {
   API_END
 }
 
+int mvg_flat_distances(mvg_flat *h, const float *q, int64_t nq, const int64_t *ids,
+                       int64_t per_query, int order, float *out) {
+  API_BEGIN
+  require(h && nq >= 0 && per_query >= 0, "invalid arguments");
+  require(order == 0 || order == 1, "order must be 0 (distance) or 1 (distance_batch4).");
+  const int64_t np = nq * per_query;
+  if (np == 0) return MVG_OK;
+  require(q && ids && out, "invalid arguments");
+  check_ids(ids, np, h->ntotal, "ids");
+  const float *dq = upload(h->q, q, (size_t)nq * h->d, h->s);
+  const int64_t *di = upload(h->pair_a, ids, (size_t)np, h->s);
+  h->pair_out.reserve(sizeof(float) * np);
+  mvg::flat_pair_distances(dq, nullptr, per_query, h->base.as<float>(), di, np, h->d, h->metric,
+                           order, h->pair_out.as<float>(), h->s);
+  download(out, h->pair_out.ptr, (size_t)np, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_flat_pair_distances(mvg_flat *h, const int64_t *a, const int64_t *b, int64_t npairs,
+                            float *out) {
+  API_BEGIN
+  require(h && npairs >= 0, "invalid arguments");
+  if (npairs == 0) return MVG_OK;
+  require(a && b && out, "invalid arguments");
+  check_ids(a, npairs, h->ntotal, "ids");
+  check_ids(b, npairs, h->ntotal, "ids");
+  const int64_t *da = upload(h->pair_a, a, (size_t)npairs, h->s);
+  const int64_t *db = upload(h->pair_b, b, (size_t)npairs, h->s);
+  h->pair_out.reserve(sizeof(float) * npairs);
+  mvg::flat_pair_distances(h->base.as<float>(), da, 0, h->base.as<float>(), db, npairs, h->d,
+                           h->metric, 0, h->pair_out.as<float>(), h->s);
+  download(out, h->pair_out.ptr, (size_t)npairs, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
 int64_t mvg_flat_ntotal(const mvg_flat *h) { return h ? h->ntotal : 0; }
 
 void mvg_flat_destroy(mvg_flat *h) {
@@ -396,6 +442,54 @@ int mvg_sq8_search(mvg_sq8 *h, const float *q, int64_t nq, int k, int k_factor,
   mvg::sq8_search(a, h->work, h->s);
   download(distances, h->D.ptr, (size_t)nq * k, h->s);
   download(labels, h->I.ptr, (size_t)nq * k, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_sq8_distances(mvg_sq8 *h, const float *q, int64_t nq, const int64_t *ids,
+                      int64_t per_query, float *out) {
+  API_BEGIN
+  require(h && nq >= 0 && per_query >= 0, "invalid arguments");
+  const int64_t np = nq * per_query;
+  if (np == 0) return MVG_OK;
+  require(q && ids && out, "invalid arguments");
+  check_ids(ids, np, h->ntotal, "ids");
+  const bool ip = h->metric == MVG_METRIC_IP;
+  const int ld = mvg::sq8_row_stride(h->d);
+  const float *dq = upload(h->q, q, (size_t)nq * h->d, h->s);
+  h->work.qcodes.reserve((size_t)nq * ld);
+  h->work.qnorms.reserve(sizeof(uint32_t) * nq);
+  h->work.qsums.reserve(sizeof(uint32_t) * nq);
+  // the query codec of get_distance_computer (index_flat_sq8.mojo:536-555)
+  mvg::sq8_encode(dq, nq, h->d, ip ? 1 : 0, ip ? 0.0f : h->gmin, 1.0f / h->scale,
+                  h->work.qcodes.as<uint8_t>(), h->work.qnorms.as<uint32_t>(),
+                  h->work.qsums.as<uint32_t>(), h->s);
+  const int64_t *di = upload(h->pair_a, ids, (size_t)np, h->s);
+  h->pair_out.reserve(sizeof(float) * np);
+  mvg::sq8_pair_distances(h->work.qcodes.as<uint8_t>(), h->work.qnorms.as<uint32_t>(), nullptr,
+                          per_query, h->codes.as<uint8_t>(), h->norms.as<uint32_t>(), di, np, ld,
+                          h->metric, h->scale * h->scale, h->pair_out.as<float>(), h->s);
+  download(out, h->pair_out.ptr, (size_t)np, h->s);
+  MVG_HIP(hipStreamSynchronize(h->s));
+  API_END
+}
+
+int mvg_sq8_pair_distances(mvg_sq8 *h, const int64_t *a, const int64_t *b, int64_t npairs,
+                           float *out) {
+  API_BEGIN
+  require(h && npairs >= 0, "invalid arguments");
+  if (npairs == 0) return MVG_OK;
+  require(a && b && out, "invalid arguments");
+  check_ids(a, npairs, h->ntotal, "ids");
+  check_ids(b, npairs, h->ntotal, "ids");
+  const int64_t *da = upload(h->pair_a, a, (size_t)npairs, h->s);
+  const int64_t *db = upload(h->pair_b, b, (size_t)npairs, h->s);
+  h->pair_out.reserve(sizeof(float) * npairs);
+  mvg::sq8_pair_distances(h->codes.as<uint8_t>(), h->norms.as<uint32_t>(), da, 0,
+                          h->codes.as<uint8_t>(), h->norms.as<uint32_t>(), db, npairs,
+                          mvg::sq8_row_stride(h->d), h->metric, h->scale * h->scale,
+                          h->pair_out.as<float>(), h->s);
+  download(out, h->pair_out.ptr, (size_t)npairs, h->s);
   MVG_HIP(hipStreamSynchronize(h->s));
   API_END
 }

@@ -182,4 +182,17 @@ struct Sq8Work {
 };
 void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s);
 
+// ---- pairdist.hip ----
+// out[p] = distance of query row (qid ? qid[p] : p / per_query) of qbase to
+// row xid[p] of xbase, as FlatDistanceComputer.distance (order 0) or
+// distance_batch4 (order 1) computes it; IP distances are negated.
+void flat_pair_distances(const float *qbase, const int64_t *qid, int64_t per_query,
+                         const float *xbase, const int64_t *xid, int64_t npairs, int d, int metric,
+                         int order, float *out, hipStream_t s);
+// The SQ8DistanceComputer code distance of encoded rows (stride ld).
+void sq8_pair_distances(const uint8_t *qcodes, const uint32_t *qnorms, const int64_t *qid,
+                        int64_t per_query, const uint8_t *codes, const uint32_t *norms,
+                        const int64_t *xid, int64_t npairs, int ld, int metric, float scale_sq,
+                        float *out, hipStream_t s);
+
 }  // namespace mvg

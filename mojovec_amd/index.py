@@ -37,6 +37,23 @@ def _mask(exclude):
     return m, m.size
 
 
+def _ids2(ids, nq):
+    ids = np.ascontiguousarray(ids, np.int64)
+    if ids.ndim == 1 and nq == 1:
+        ids = ids.reshape(1, -1)
+    if ids.ndim != 2 or ids.shape[0] != nq:
+        raise ValueError("ids must have shape [nq, per_query]")
+    return ids
+
+
+def _pairs(a, b):
+    a = np.ascontiguousarray(a, np.int64).reshape(-1)
+    b = np.ascontiguousarray(b, np.int64).reshape(-1)
+    if a.size != b.size:
+        raise ValueError("a and b must have the same length")
+    return a, b
+
+
 class IndexFlat:
     """Exact brute-force index (IndexFlat, index_flat.mojo:117-330)."""
 
@@ -74,6 +91,25 @@ class IndexFlat:
         m, ml = _mask(exclude)
         _lib.check(_lib.load().mvg_flat_search(self._h, _ptr(x), nq, k, _ptr(m), ml, _ptr(D), _ptr(I)))
         return D, I
+
+    def distances(self, x, ids, order: str = "distance"):
+        """FlatDistanceComputer distances (index_flat.mojo:36-115) of each
+        query row to the rows ids[i] ([nq][per_query]): "distance" or
+        "batch4" (distance_batch4) summation order."""
+        x = _f32(x, self.d)
+        ids = _ids2(ids, x.shape[0])
+        o = {"distance": 0, "batch4": 1}[order]
+        out = np.empty(ids.shape, np.float32)
+        _lib.check(_lib.load().mvg_flat_distances(self._h, _ptr(x), ids.shape[0], _ptr(ids), ids.shape[1], o,
+                                                  _ptr(out)))
+        return out
+
+    def pair_distances(self, a, b):
+        """FlatDistanceComputer.symmetric_distance for row pairs (a[p], b[p])."""
+        a, b = _pairs(a, b)
+        out = np.empty(a.size, np.float32)
+        _lib.check(_lib.load().mvg_flat_pair_distances(self._h, _ptr(a), _ptr(b), a.size, _ptr(out)))
+        return out
 
 
 class IndexFlatSQ8:
@@ -119,6 +155,23 @@ class IndexFlatSQ8:
         nr = np.empty(n, np.uint32)
         _lib.check(_lib.load().mvg_sq8_get_codes(self._h, _ptr(c), _ptr(nr)))
         return c, nr
+
+    def distances(self, x, ids):
+        """SQ8DistanceComputer distances (index_flat_sq8.mojo:21-190) of each
+        query row to the rows ids[i] ([nq][per_query])."""
+        x = _f32(x, self.d)
+        ids = _ids2(ids, x.shape[0])
+        out = np.empty(ids.shape, np.float32)
+        _lib.check(_lib.load().mvg_sq8_distances(self._h, _ptr(x), ids.shape[0], _ptr(ids), ids.shape[1],
+                                                 _ptr(out)))
+        return out
+
+    def pair_distances(self, a, b):
+        """SQ8DistanceComputer.symmetric_distance for row pairs (a[p], b[p])."""
+        a, b = _pairs(a, b)
+        out = np.empty(a.size, np.float32)
+        _lib.check(_lib.load().mvg_sq8_pair_distances(self._h, _ptr(a), _ptr(b), a.size, _ptr(out)))
+        return out
 
     def search(self, x, k: int, k_factor: int = 1, exclude=None):
         x = _f32(x, self.d)

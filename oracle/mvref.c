@@ -208,6 +208,67 @@ void mvref_ip_simd_batch(const float *x, const float *y, int64_t n, int64_t d,
   for (int64_t i = 0; i < n; i++) out[i] = mvref_ip_simd(x + i * d, y + i * d, d, W);
 }
 
+/* l2_distance_simd_batch4[W], one of its four rows: distances.mojo:541-605.
+ * One W-lane accumulator over W-element blocks (fma(diff, diff, acc), diff =
+ * x - y), the halving tree, then a scalar tail res += diff * diff. */
+float mvref_l2_simd_b4(const float *x, const float *y, int64_t d, int W) {
+  float acc[64];
+  for (int j = 0; j < W; j++) acc[j] = 0.0f;
+  int64_t i = 0;
+  for (; i + W <= d; i += W) /* :556-576 */
+    for (int j = 0; j < W; j++) {
+      float df = x[i + j] - y[i + j];
+      acc[j] = fmaf(df, df, acc[j]);
+    }
+  float res = reduce_add(acc, W); /* :578-581 */
+  for (; i < d; i++) { /* :583-597 */
+    float df = x[i] - y[i];
+    res += df * df;
+  }
+  return res;
+}
+
+/* inner_product_simd_batch4[W], one row: distances.mojo:608-660 (acc =
+ * fma(x, y, acc) over W-element blocks, halving tree, res += x * y tail). */
+float mvref_ip_simd_b4(const float *x, const float *y, int64_t d, int W) {
+  float acc[64];
+  for (int j = 0; j < W; j++) acc[j] = 0.0f;
+  int64_t i = 0;
+  for (; i + W <= d; i += W)
+    for (int j = 0; j < W; j++) acc[j] = fmaf(x[i + j], y[i + j], acc[j]);
+  float res = reduce_add(acc, W);
+  for (; i < d; i++) res += x[i] * y[i];
+  return res;
+}
+
+/* FlatDistanceComputer (index_flat.mojo:36-115, SIMD_WIDTH = 8 at :9) over a
+ * batch: out[p] = distance(q[p / per_query], row ids[p]). order 0 is
+ * distance() (:60-68: l2_distance_simd[8] / -inner_product_simd[8]), order 1
+ * distance_batch4() (:70-90: l2_distance_simd_batch4[8] /
+ * -inner_product_simd_batch4[8]). */
+void mvref_flat_distances(const float *base, int d, int metric, const float *q,
+                          int64_t nq, const int64_t *ids, int64_t per_query,
+                          int order, float *out) {
+  for (int64_t p = 0; p < nq * per_query; p++) {
+    const float *qv = q + (p / per_query) * d, *xv = base + ids[p] * d;
+    float v;
+    if (order == 0)
+      v = metric == MVREF_L2 ? mvref_l2_simd(qv, xv, d, 8) : -mvref_ip_simd(qv, xv, d, 8);
+    else
+      v = metric == MVREF_L2 ? mvref_l2_simd_b4(xv, qv, d, 8) : -mvref_ip_simd_b4(xv, qv, d, 8);
+    out[p] = v;
+  }
+}
+
+/* FlatDistanceComputer.symmetric_distance (index_flat.mojo:92-99). */
+void mvref_flat_pair_distances(const float *base, int d, int metric, const int64_t *a,
+                               const int64_t *b, int64_t npairs, float *out) {
+  for (int64_t p = 0; p < npairs; p++) {
+    const float *xa = base + a[p] * d, *xb = base + b[p] * d;
+    out[p] = metric == MVREF_L2 ? mvref_l2_simd(xa, xb, d, 8) : -mvref_ip_simd(xa, xb, d, 8);
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Reference max-heap (mojovec/utils/heap.mojo:17-213), literal restatement. */
 
@@ -726,6 +787,47 @@ void mvref_sq8_search(const mvref_sq8 *h, const float *q, int64_t nq, int k,
       }
     }
     free(buf); free(buf2); free(qc);
+  }
+}
+
+/* SQ8DistanceComputer (index_flat_sq8.mojo:21-190) over a batch: the query
+ * is encoded as get_distance_computer does (:527-560), then distance() /
+ * distance_batch4() (:80-146, the same values: code dots are exact):
+ * IP -Float32(dot_i8) * scale^2, L2 Float32(l2_from_dot) * scale^2
+ * (sq8.mojo:62-79). */
+void mvref_sq8_distances(const mvref_sq8 *h, const float *q, int64_t nq, const int64_t *ids,
+                         int64_t per_query, float *out) {
+  int d = h->d, ip = h->metric != MVREF_L2;
+  float inv = 1.0f / h->scale, ssq = h->scale * h->scale;
+  uint8_t *qc = (uint8_t *)malloc((size_t)d);
+  for (int64_t qi = 0; qi < nq; qi++) {
+    uint32_t qn = mvref_sq8_encode_vec(q + qi * d, d, ip, ip ? 0.0f : h->gmin, inv, qc);
+    for (int64_t j = 0; j < per_query; j++) {
+      const uint8_t *xc = h->codes + ids[qi * per_query + j] * d;
+      float v;
+      if (ip) {
+        v = -(float)mvref_dot_i8((const int8_t *)qc, (const int8_t *)xc, d) * ssq;
+      } else {
+        uint32_t xn = h->norms[ids[qi * per_query + j]];
+        v = (float)mvref_l2_from_dot(qn, xn, mvref_dot_u8(qc, xc, d)) * ssq;
+      }
+      out[qi * per_query + j] = v;
+    }
+  }
+  free(qc);
+}
+
+/* SQ8DistanceComputer.symmetric_distance (index_flat_sq8.mojo:148-177). */
+void mvref_sq8_pair_distances(const mvref_sq8 *h, const int64_t *a, const int64_t *b,
+                              int64_t npairs, float *out) {
+  int d = h->d, ip = h->metric != MVREF_L2;
+  float ssq = h->scale * h->scale;
+  for (int64_t p = 0; p < npairs; p++) {
+    const uint8_t *ca = h->codes + a[p] * d, *cb = h->codes + b[p] * d;
+    if (ip)
+      out[p] = -(float)mvref_dot_i8((const int8_t *)ca, (const int8_t *)cb, d) * ssq;
+    else
+      out[p] = (float)mvref_l2_from_dot(h->norms[a[p]], h->norms[b[p]], mvref_dot_u8(ca, cb, d)) * ssq;
   }
 }
 

@@ -26,6 +26,18 @@ enum { MVREF_L2 = 0, MVREF_IP = 1 };
 /* ---- distances (mojovec/utils/distances.mojo) ---- */
 float mvref_l2_simd(const float *x, const float *y, int64_t d, int W);
 float mvref_ip_simd(const float *x, const float *y, int64_t d, int W);
+/* l2_distance_simd_batch4[W] / inner_product_simd_batch4[W] for one row
+ * (distances.mojo:541-660), W <= 64. */
+float mvref_l2_simd_b4(const float *x, const float *y, int64_t d, int W);
+float mvref_ip_simd_b4(const float *x, const float *y, int64_t d, int W);
+/* FlatDistanceComputer.distance (order 0) / distance_batch4 (order 1) over
+ * nq x per_query (query, row id) pairs, and symmetric_distance over row
+ * pairs (index_flat.mojo:36-115). */
+void mvref_flat_distances(const float *base, int d, int metric, const float *q,
+                          int64_t nq, const int64_t *ids, int64_t per_query,
+                          int order, float *out);
+void mvref_flat_pair_distances(const float *base, int d, int metric, const int64_t *a,
+                               const int64_t *b, int64_t npairs, float *out);
 float mvref_l2_short4(const float *x, const float *y, int64_t d);
 float mvref_ip_short4(const float *x, const float *y, int64_t d);
 uint32_t mvref_dot_u8(const uint8_t *x, const uint8_t *y, int64_t d);
@@ -82,6 +94,12 @@ int mvref_sq8_add(mvref_sq8 *h, const float *x, int64_t n);
 void mvref_sq8_params(const mvref_sq8 *h, float *gmin, float *gmax,
                       float *scale, int64_t *ntotal);
 void mvref_sq8_codes(const mvref_sq8 *h, uint8_t *codes, uint32_t *norms);
+/* SQ8DistanceComputer.distance / symmetric_distance over a batch
+ * (index_flat_sq8.mojo:21-190, :527-560). */
+void mvref_sq8_distances(const mvref_sq8 *h, const float *q, int64_t nq, const int64_t *ids,
+                         int64_t per_query, float *out);
+void mvref_sq8_pair_distances(const mvref_sq8 *h, const int64_t *a, const int64_t *b,
+                              int64_t npairs, float *out);
 void mvref_sq8_search(const mvref_sq8 *h, const float *q, int64_t nq, int k,
                       int k_factor, const uint8_t *exclude,
                       int64_t exclude_len, float *D, int64_t *I,

@@ -45,6 +45,12 @@ def lib():
         sig = {
             "mvref_l2_simd": (ctypes.c_float, [_f32p, _f32p, _i64, _int]),
             "mvref_ip_simd": (ctypes.c_float, [_f32p, _f32p, _i64, _int]),
+            "mvref_l2_simd_b4": (ctypes.c_float, [_f32p, _f32p, _i64, _int]),
+            "mvref_ip_simd_b4": (ctypes.c_float, [_f32p, _f32p, _i64, _int]),
+            "mvref_flat_distances": (None, [_f32p, _int, _int, _f32p, _i64, _i64p, _i64, _int, _f32p]),
+            "mvref_flat_pair_distances": (None, [_f32p, _int, _int, _i64p, _i64p, _i64, _f32p]),
+            "mvref_sq8_distances": (None, [_vp, _f32p, _i64, _i64p, _i64, _f32p]),
+            "mvref_sq8_pair_distances": (None, [_vp, _i64p, _i64p, _i64, _f32p]),
             "mvref_l2_short4": (ctypes.c_float, [_f32p, _f32p, _i64]),
             "mvref_ip_short4": (ctypes.c_float, [_f32p, _f32p, _i64]),
             "mvref_dot_u8": (ctypes.c_uint32, [_u8p, _u8p, _i64]),
@@ -121,6 +127,38 @@ def l2_simd(x, y, W=8):
 def ip_simd(x, y, W=8):
     x, y = _f32(x), _f32(y)
     return float(lib().mvref_ip_simd(x, y, x.size, W))
+
+
+def l2_simd_b4(x, y, W=8):
+    """One row of l2_distance_simd_batch4[W] (distances.mojo:541-605)."""
+    x, y = _f32(x), _f32(y)
+    return float(lib().mvref_l2_simd_b4(x, y, x.size, W))
+
+
+def ip_simd_b4(x, y, W=8):
+    """One row of inner_product_simd_batch4[W] (distances.mojo:608-660)."""
+    x, y = _f32(x), _f32(y)
+    return float(lib().mvref_ip_simd_b4(x, y, x.size, W))
+
+
+def flat_distances(base, q, ids, metric=L2, order=0):
+    """FlatDistanceComputer.distance (order 0) / distance_batch4 (order 1) for
+    ids [nq][per_query] (index_flat.mojo:36-115)."""
+    base, q = _f32(base), _f32(q)
+    ids = np.ascontiguousarray(ids, np.int64)
+    nq, per = ids.shape
+    out = np.empty((nq, per), np.float32)
+    lib().mvref_flat_distances(base, base.shape[1], metric, q, nq, ids, per, order, out)
+    return out
+
+
+def flat_pair_distances(base, a, b, metric=L2):
+    """FlatDistanceComputer.symmetric_distance (index_flat.mojo:92-99)."""
+    base = _f32(base)
+    a, b = np.ascontiguousarray(a, np.int64), np.ascontiguousarray(b, np.int64)
+    out = np.empty(a.size, np.float32)
+    lib().mvref_flat_pair_distances(base, base.shape[1], metric, a, b, a.size, out)
+    return out
 
 
 def l2_short4(x, y):
@@ -270,6 +308,14 @@ def sq8_encode(x, symmetric, minimum, inverse_scale):
     return (code.view(np.int8) if symmetric else code), int(norm)
 
 
+def sq8_encode_vec(x, symmetric, minimum, inverse_scale):
+    """_encode_sq8_vector (sq8.mojo:13-37): (codes, UInt32 code norm)."""
+    x = _f32(x).reshape(-1)
+    code = np.empty(x.size, np.uint8)
+    n = lib().mvref_sq8_encode_vec(x, x.size, int(bool(symmetric)), float(minimum), float(inverse_scale), code)
+    return code, int(n)
+
+
 class SQ8Index:
     def __init__(self, d, metric=L2):
         self.d = d
@@ -297,6 +343,21 @@ class SQ8Index:
         nr = np.empty(n, np.uint32)
         lib().mvref_sq8_codes(self.h, c, nr)
         return c, nr
+
+    def distances(self, q, ids):
+        """SQ8DistanceComputer.distance for ids [nq][per_query]."""
+        q = _f32(q).reshape(-1, self.d)
+        ids = np.ascontiguousarray(ids, np.int64)
+        out = np.empty(ids.shape, np.float32)
+        lib().mvref_sq8_distances(self.h, q, ids.shape[0], ids, ids.shape[1], out)
+        return out
+
+    def pair_distances(self, a, b):
+        """SQ8DistanceComputer.symmetric_distance over row pairs."""
+        a, b = np.ascontiguousarray(a, np.int64), np.ascontiguousarray(b, np.int64)
+        out = np.empty(a.size, np.float32)
+        lib().mvref_sq8_pair_distances(self.h, a, b, a.size, out)
+        return out
 
     def search(self, q, k, k_factor=1, exclude=None, nthreads=0):
         q = _f32(q).reshape(-1, self.d)

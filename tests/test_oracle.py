@@ -554,3 +554,180 @@ def test_ivfflat_untrained_empty_oversized_nprobe_and_exclusion():
     ex[I1[:, 0]] = 1
     _, Ie = ix.search(rows[:3], 4, 4, exclude=ex)
     assert np.array_equal(Ie[:, :3], I1[:, 1:])
+
+
+# ---- distance computers for a graph search (HNSW-facing batch service) ----
+# The two FP32 summation orders are pinned against an exact rational
+# emulation written from the reference text: every binary32 operation is
+# rounded exactly (round to nearest, ties to even) from its exact rational
+# value, fma rounding once.
+from fractions import Fraction
+
+
+def _ex_r32(v):
+    """Exact binary32 rounding of a rational (normal range)."""
+    if v == 0:
+        return Fraction(0)
+    sign = -1 if v < 0 else 1
+    a = abs(v)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    while Fraction(2) ** e > a:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    m = a * Fraction(2) ** (23 - e)
+    q, r = divmod(m.numerator, m.denominator)
+    if 2 * r > m.denominator or (2 * r == m.denominator and q % 2 == 1):
+        q += 1
+    return sign * Fraction(q) * Fraction(2) ** (e - 23)
+
+
+def _ex_tree(v):  # SIMD.reduce_add as the halving tree
+    v = list(v)
+    h = len(v) // 2
+    while h >= 1:
+        for j in range(h):
+            v[j] = _ex_r32(v[j] + v[j + h])
+        h //= 2
+    return v[0]
+
+
+def _ex_term(a, b, ip):
+    return _ex_r32(a * b) if ip else _ex_r32(_ex_r32(a - b) ** 2)
+
+
+def _ex_fma_term(a, b, acc, ip):
+    return _ex_r32(a * b + acc) if ip else _ex_r32(_ex_r32(a - b) ** 2 + acc)
+
+
+def _ex_emul_single8(x, y, ip):
+    """l2_distance_simd[8] / inner_product_simd[8] (distances.mojo:43-228)."""
+    d = len(x)
+    if d < 8:
+        r = Fraction(0)
+        for i in range(d):
+            r = _ex_r32(r + _ex_term(x[i], y[i], ip))
+        return r
+    acc = [[Fraction(0)] * 8 for _ in range(4)]
+    i = 0
+    while i <= d - 32:
+        for b in range(4):
+            for j in range(8):
+                acc[b][j] = _ex_fma_term(x[i + 8 * b + j], y[i + 8 * b + j], acc[b][j], ip)
+        i += 32
+    v = [_ex_r32(_ex_r32(_ex_r32(acc[0][j] + acc[1][j]) + acc[2][j]) + acc[3][j]) for j in range(8)]
+    while i <= d - 8:
+        v = [_ex_fma_term(x[i + j], y[i + j], v[j], ip) for j in range(8)]
+        i += 8
+    res = _ex_tree(v)
+    while i <= d - 4:
+        res = _ex_r32(res + _ex_tree([_ex_term(x[i + j], y[i + j], ip) for j in range(4)]))
+        i += 4
+    while i < d:
+        res = _ex_r32(res + _ex_term(x[i], y[i], ip))
+        i += 1
+    return res
+
+
+def _ex_emul_batch8(x, y, ip):
+    """One row of l2_distance_simd_batch4[8] / inner_product_simd_batch4[8]
+    (distances.mojo:541-660)."""
+    d = len(x)
+    acc = [Fraction(0)] * 8
+    i = 0
+    while i + 8 <= d:
+        acc = [_ex_fma_term(x[i + j], y[i + j], acc[j], ip) for j in range(8)]
+        i += 8
+    res = _ex_tree(acc)
+    while i < d:
+        res = _ex_r32(res + _ex_term(x[i], y[i], ip))
+        i += 1
+    return res
+
+
+@pytest.mark.parametrize("d", [5, 8, 13, 32, 36, 45, 64, 71])
+def test_distance_orders_match_exact_emulation(d):
+    rng = np.random.default_rng(900 + d)
+    differ = 0
+    for trial in range(4):
+        x = (rng.standard_normal(d) * rng.choice([1.0, 37.0])).astype(np.float32)
+        y = (rng.standard_normal(d) * 3.0).astype(np.float32)
+        xf = [Fraction(float(v)) for v in x]
+        yf = [Fraction(float(v)) for v in y]
+        for ip in (False, True):
+            s_exp = float(_ex_emul_single8(xf, yf, ip))
+            b_exp = float(_ex_emul_batch8(xf, yf, ip))
+            s_got = o.ip_simd(x, y, 8) if ip else o.l2_simd(x, y, 8)
+            b_got = o.ip_simd_b4(x, y, 8) if ip else o.l2_simd_b4(x, y, 8)
+            assert np.float32(s_got) == np.float32(s_exp), (d, trial, ip, "single")
+            assert np.float32(b_got) == np.float32(b_exp), (d, trial, ip, "batch4")
+            differ += s_exp != b_exp
+    if d >= 32:  # the two orders really differ once 32-element blocks exist
+        assert differ > 0
+
+
+def test_flat_distance_computer_batches():
+    rng = np.random.default_rng(31)
+    base = rng.standard_normal((300, 45)).astype(np.float32)
+    q = rng.standard_normal((7, 45)).astype(np.float32)
+    ids = rng.integers(0, 300, (7, 20))
+    for metric in (o.L2, o.IP):
+        for order in (0, 1):
+            got = o.flat_distances(base, q, ids, metric, order)
+            fn = (o.l2_simd if order == 0 else o.l2_simd_b4) if metric == o.L2 else \
+                 (o.ip_simd if order == 0 else o.ip_simd_b4)
+            for i in range(7):
+                for j in range(20):
+                    v = fn(q[i], base[ids[i, j]], 8) if order == 0 else fn(base[ids[i, j]], q[i], 8)
+                    assert got[i, j] == np.float32(v if metric == o.L2 else -v)
+    # integer rows: every order is exact, so both equal the float64 brute force
+    bi = rng.integers(-20, 21, (50, 36)).astype(np.float32)
+    qi = rng.integers(-20, 21, (3, 36)).astype(np.float32)
+    idx = rng.integers(0, 50, (3, 9))
+    exact_l2 = ((qi[:, None, :].astype(np.float64) - bi[idx]) ** 2).sum(-1)
+    exact_ip = -(qi[:, None, :].astype(np.float64) * bi[idx]).sum(-1)
+    for order in (0, 1):
+        assert np.array_equal(o.flat_distances(bi, qi, idx, o.L2, order), exact_l2.astype(np.float32))
+        assert np.array_equal(o.flat_distances(bi, qi, idx, o.IP, order), exact_ip.astype(np.float32))
+    a, b = rng.integers(0, 50, 30), rng.integers(0, 50, 30)
+    assert np.array_equal(o.flat_pair_distances(bi, a, b, o.L2),
+                          ((bi[a].astype(np.float64) - bi[b]) ** 2).sum(-1).astype(np.float32))
+
+
+def test_sq8_distance_computer_batches():
+    rng = np.random.default_rng(32)
+    x = rng.standard_normal((400, 24)).astype(np.float32)
+    q = rng.standard_normal((5, 24)).astype(np.float32)
+    ids = rng.integers(0, 400, (5, 30))
+    for metric in (o.L2, o.IP):
+        ix = o.SQ8Index(24, metric)
+        ix.add(x)
+        gmin, _, scale, _ = ix.params()
+        codes, norms = ix.codes()
+        got = ix.distances(q, ids)
+        ssq = np.float32(scale) * np.float32(scale)
+        for i in range(5):
+            # the query codec of get_distance_computer (index_flat_sq8.mojo:536-555)
+            qc, qn = o.sq8_encode_vec(q[i], metric != o.L2, 0.0 if metric != o.L2 else gmin,
+                                      np.float32(1.0) / np.float32(scale))
+            for j in range(30):
+                c = codes[ids[i, j]]
+                if metric == o.IP:
+                    dot = int((qc.view(np.int8).astype(np.int64) * c.view(np.int8).astype(np.int64)).sum())
+                    want = np.float32(-np.float32(dot)) * ssq
+                else:
+                    dot = int((qc.astype(np.int64) * c.astype(np.int64)).sum())
+                    v = max(int(qn) + int(norms[ids[i, j]]) - 2 * dot, 0)
+                    want = np.float32(v) * ssq
+                assert got[i, j] == np.float32(want), (metric, i, j)
+        a, b = rng.integers(0, 400, 40), rng.integers(0, 400, 40)
+        gp = ix.pair_distances(a, b)
+        for p in range(40):
+            ca, cb = codes[a[p]], codes[b[p]]
+            if metric == o.IP:
+                dot = int((ca.view(np.int8).astype(np.int64) * cb.view(np.int8).astype(np.int64)).sum())
+                want = np.float32(-np.float32(dot)) * ssq
+            else:
+                dot = int((ca.astype(np.int64) * cb.astype(np.int64)).sum())
+                want = np.float32(max(int(norms[a[p]]) + int(norms[b[p]]) - 2 * dot, 0)) * ssq
+            assert gp[p] == np.float32(want)
