@@ -376,6 +376,71 @@ __device__ float group16_dist_w64(const float *q, const float *x, int d, int gla
   return (v.x + v.z) + (v.y + v.w);
 }
 
+// group16_dist_w64 of R rows at once (each row's operations exactly as
+// above; the rows' loads are issued together, so R rows are in flight per
+// group). BIG: d may reach the four-accumulator blocks (d >= 256); without it
+// those blocks are compiled out (d < 256 never enters them). Results valid in
+// glane 0.
+template <bool IP, int R, bool BIG>
+__device__ __forceinline__ void group16_distR_w64(const float *q, const float *const (&x)[R], int d,
+                                                  int glane, float (&r)[R]) {
+  const float4 *q4 = reinterpret_cast<const float4 *>(q);
+  const int n4 = d >> 2;
+  int i = 0;
+  float4 v[R];
+  if constexpr (BIG) {
+    float4 a[R][4];
+#pragma unroll
+    for (int t = 0; t < R; t++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) a[t][b] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (; i + 64 <= n4; i += 64) {
+      float4 xv[R][4];
+#pragma unroll
+      for (int t = 0; t < R; t++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) xv[t][b] = reinterpret_cast<const float4 *>(x[t])[i + 16 * b + glane];
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const float4 qv = q4[i + 16 * b + glane];
+#pragma unroll
+        for (int t = 0; t < R; t++) w64_step<IP>(a[t][b], qv, xv[t][b]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      v[t].x = ((a[t][0].x + a[t][1].x) + a[t][2].x) + a[t][3].x;
+      v[t].y = ((a[t][0].y + a[t][1].y) + a[t][2].y) + a[t][3].y;
+      v[t].z = ((a[t][0].z + a[t][1].z) + a[t][2].z) + a[t][3].z;
+      v[t].w = ((a[t][0].w + a[t][1].w) + a[t][2].w) + a[t][3].w;
+    }
+  } else {
+    // ((0 + 0) + 0) + 0 of the empty four-accumulator stage
+#pragma unroll
+    for (int t = 0; t < R; t++) v[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  for (; i + 16 <= n4; i += 16) {
+    float4 xv[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) xv[t] = reinterpret_cast<const float4 *>(x[t])[i + glane];
+    const float4 qv = q4[i + glane];
+#pragma unroll
+    for (int t = 0; t < R; t++) w64_step<IP>(v[t], qv, xv[t]);
+  }
+#pragma unroll
+  for (int h = 8; h >= 1; h >>= 1) {
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      v[t].x = v[t].x + __shfl_down(v[t].x, h, 16);
+      v[t].y = v[t].y + __shfl_down(v[t].y, h, 16);
+      v[t].z = v[t].z + __shfl_down(v[t].z, h, 16);
+      v[t].w = v[t].w + __shfl_down(v[t].w, h, 16);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < R; t++) r[t] = (v[t].x + v[t].z) + (v[t].y + v[t].w);
+}
+
 template <bool IP>
 __device__ float wave_dist_w64(const float *q, const float *x, int d, int lane) {
   if (d < 64) {

@@ -241,10 +241,47 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
   }
 }
 
+// Exact distances of a query's kk candidates into buf (sentinel for empty or
+// invalid slots): sixteen 16-lane groups, each scoring R candidates per step
+// so that R rows per group are in flight.
+template <bool IP, int R, bool BIG>
+__device__ __forceinline__ void rerank_rows(const float *qs, const float *__restrict__ vectors,
+                                            int64_t nrows, const uint64_t *__restrict__ cq, int kk,
+                                            int d, uint64_t *buf) {
+  constexpr int G = kNT / 16;
+  const int glane = threadIdx.x & 15, group = threadIdx.x >> 4;
+  for (int c = group; c < kk; c += R * G) {
+    uint32_t id[R];
+    bool ok[R];
+    const float *x[R];
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      const int ct = c + t * G;
+      const uint64_t kv = ct < kk ? cq[ct] : kSentinel64;
+      id[t] = (uint32_t)(kv & 0xFFFFFFFFu);
+      ok[t] = kv != kSentinel64 && (int64_t)id[t] < nrows;
+      x[t] = ok[t] ? vectors + (int64_t)id[t] * d : qs;  // an invalid slot reads the query row
+      any |= ok[t];
+    }
+    float e[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) e[t] = 0.0f;
+    if (any) group16_distR_w64<IP, R, BIG>(qs, x, d, glane, e);
+    if (glane == 0) {
+#pragma unroll
+      for (int t = 0; t < R; t++) {
+        const int ct = c + t * G;
+        if (ct < kk) buf[ct] = ok[t] ? pack_key(IP ? -e[t] : e[t], id[t]) : kSentinel64;
+      }
+    }
+  }
+}
+
 // Rerank for d a multiple of 64: sixteen 16-lane groups per workgroup each
 // score one candidate row at a time (group16_dist_w64, bit-identical to the
 // wavefront form), so 16 rows per workgroup are in flight instead of 4.
-template <bool IP>
+template <bool IP, bool SMALLD>
 __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restrict__ q, int d,
                                                            const float *__restrict__ vectors,
                                                            int64_t nrows,
@@ -258,20 +295,16 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
   uint64_t *buf = reinterpret_cast<uint64_t *>(qs + d);             // kk
   uint64_t *top = buf + ((kk + 1) & ~1);                            // next_pow2(k)
   SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(top + next_pow2(k));
-  const int tid = threadIdx.x, glane = tid & 15, group = tid >> 4;
+  const int tid = threadIdx.x;
   const int64_t qi = rerank_query(nq, order);
   if (qi < 0) return;
   for (int j = tid; j < d; j += kNT) qs[j] = q[qi * d + j];
   __syncthreads();
   const uint64_t *cq = cand + qi * kk;
-  for (int c = group; c < kk; c += kNT / 16) {
-    const uint64_t key = cq[c];
-    const uint32_t id = (uint32_t)(key & 0xFFFFFFFFu);
-    const bool valid = key != kSentinel64 && (int64_t)id < nrows;
-    float ex = 0.0f;
-    if (valid) ex = group16_dist_w64<IP>(qs, vectors + (int64_t)id * d, d, glane);
-    if (glane == 0) buf[c] = valid ? pack_key(IP ? -ex : ex, id) : kSentinel64;
-  }
+  if constexpr (SMALLD)  // d < 256: four rows per group, no four-accumulator blocks
+    rerank_rows<IP, 4, false>(qs, vectors, nrows, cq, kk, d, buf);
+  else
+    rerank_rows<IP, 2, true>(qs, vectors, nrows, cq, kk, d, buf);
   __syncthreads();
   const int cnt = select_sort_u64(buf, kk, k, *scr, top);
   for (int t = tid; t < k; t += kNT) {
@@ -1446,18 +1479,20 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
     }
     if (tid == 0) poff[np] = carry;
     __syncthreads();
-    const int64_t n = DEEP ? min<int64_t>(carry, seg_cap) : carry;
+    // one query's stream positions and CSR entries fit in 32 bits (entries are
+    // int32 ids), which keeps the per-entry index arithmetic 32-bit
+    const int n = DEEP ? (int)min<int64_t>(carry, seg_cap) : carry;
 
     // scan the stream. Each thread's positions only grow, so it keeps its
     // current probe in registers -- the stream position where it ends, the
     // CSR offset of its entries and its base term -- and reads the probe
     // table only when it crosses into the next list.
     int cur = 0;
-    int64_t cur_end = poff[1], cur_delta = loff[0] - poff[0];
+    int cur_end = poff[1], cur_delta = (int)(loff[0] - poff[0]);
     float cur_base = basev[0];
     constexpr int U = (MF > 0) ? 4 : 1;
     int round = 0;
-    for (int64_t r0 = 0; r0 < n; r0 += (int64_t)kNT * U) {
+    for (int r0 = 0; r0 < n; r0 += kNT * U) {
       const uint64_t thr = DEEP ? kSentinel64 : st->thr;
       // DEEP: keys whose top digit exceeds this bound cannot be among the kk
       // smallest; only the others are written and counted.
@@ -1466,10 +1501,10 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
       Code<MF> cw[U];
       int32_t idv[U];
       float initv[U];
-      int64_t ev[U];
+      uint32_t ev[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {  // every load of the batch in flight first
-        const int64_t j = r0 + (int64_t)u * kNT + tid;
+        const int j = r0 + u * kNT + tid;
         cw[u] = Code<MF>{};
         idv[u] = -1;
         initv[u] = 0.0f;
@@ -1478,10 +1513,10 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
           while (j >= cur_end) {
             cur++;
             cur_end = poff[cur + 1];
-            cur_delta = loff[cur] - poff[cur];
+            cur_delta = (int)(loff[cur] - poff[cur]);
             cur_base = basev[cur];
           }
-          const int64_t e = j + cur_delta;
+          const uint32_t e = (uint32_t)(j + cur_delta);
           ev[u] = e;
           if (MF > 0) cw[u] = load_code<MF>(codes, e);
           if (!DEEP || excl) idv[u] = ids[e];
@@ -1496,7 +1531,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
       uint32_t putk[U];  // DEEP: keys to append (sentinel: none)
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int64_t j = r0 + (int64_t)u * kNT + tid;
+        const int j = r0 + u * kNT + tid;
         bool put = false;
         uint32_t key = kSentinel;
         if (j < n) {
@@ -1505,7 +1540,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
           float dist = 0.0f;
           if (!skip) {
             if constexpr (MF > 0) dist = distv[u];
-            else dist = adc_sum<0>(codes + ev[u] * M, lut, M, initv[u]);
+            else dist = adc_sum<0>(codes + (int64_t)ev[u] * M, lut, M, initv[u]);
           }
           if constexpr (DEEP) {
             if (!skip) {
@@ -1908,15 +1943,21 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
     const size_t sm = sizeof(float) * d + sizeof(uint64_t) * (size_t)(((kk + 1) & ~1) + next_pow2(k)) +
                       sizeof(SelectScratch64);
     if (sm <= 150 * 1024) {
-      const void *fn = ip ? (const void *)rerank_group_kernel<true> : (const void *)rerank_group_kernel<false>;
-      if (sm > 64 * 1024)
-        MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-      if (ip)
-        hipLaunchKernelGGL(rerank_group_kernel<true>, dim3(grid), dim3(kNT), sm, s, q, d, vectors,
-                           nrows, cand, kk, k, pad_dist, D, I, nq, order);
-      else
-        hipLaunchKernelGGL(rerank_group_kernel<false>, dim3(grid), dim3(kNT), sm, s, q, d, vectors,
-                           nrows, cand, kk, k, pad_dist, D, I, nq, order);
+      const bool small = d < 256;
+#define MVG_RR(IPV, SV)                                                                          \
+  {                                                                                              \
+    if (sm > 64 * 1024)                                                                          \
+      MVG_HIP(hipFuncSetAttribute((const void *)rerank_group_kernel<IPV, SV>,                    \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));         \
+    hipLaunchKernelGGL((rerank_group_kernel<IPV, SV>), dim3(grid), dim3(kNT), sm, s, q, d, vectors, \
+                       nrows, cand, kk, k, pad_dist, D, I, nq, order);                           \
+  }
+      if (ip) {
+        if (small) MVG_RR(true, true) else MVG_RR(true, false)
+      } else {
+        if (small) MVG_RR(false, true) else MVG_RR(false, false)
+      }
+#undef MVG_RR
       MVG_LAUNCH_CHECK("rerank_group_kernel");
       return;
     }
