@@ -37,6 +37,13 @@ RECALL_BAR = 0.95
 # k * k_factor = 4000 candidates (the device top-k limit is 4096).
 KF_SWEEP = (1, 4, 16, 64, 128, 256, 288, 320, 400)
 DEEP_MIN = 513  # k * k_factor from which the library runs the deep-pool scan kernel
+# HBM bytes per launch of the mixed scan kernel at the headline settings, from
+# two rocprofv3 --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the
+# scales a 1 GiB device copy gives for these counters); the counters cannot
+# be read inside this process, so the value is the profiled one.
+SCAN_TRAFFIC_PROFILED = {"at": ("mixed", 64, 288), "bytes": 5_128_894 * 1024 * 2 + 1_362_262 * 1024,
+                         "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, "
+                                   "profiles/ivfpq_1Mx128_hbm_traffic.md"}
 
 
 def log(msg):
@@ -280,7 +287,10 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": (SCAN_TRAFFIC_PROFILED["bytes"] if (head, nprobe, kf) == SCAN_TRAFFIC_PROFILED["at"]
+                        else None),
+            "traffic_source": (SCAN_TRAFFIC_PROFILED["source"]
+                               if (head, nprobe, kf) == SCAN_TRAFFIC_PROFILED["at"] else None),
             "scan_ms": round(scan_s * 1e3, 4),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
             "scanned_entries_per_launch": int(entries),
