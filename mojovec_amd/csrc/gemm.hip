@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Wide form for kk <= 48: a workgroup owns 64 queries (16 per wavefront) and
+// Wide form for kk <= 64: a workgroup owns 64 queries (16 per wavefront) and
 // one split of the base rows, so each base byte fetched serves 64 queries
 // (four times the reuse of the 16-query form above). The 256 rows of a step
 // are staged through LDS 128 bytes of k at a time (32 KB per slice, double
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
 // top kk with the lane-wide bitonic networks (no workgroup barrier).
 constexpr int kWQ = 64;                   // queries per workgroup
 constexpr int kWCap = 32;                 // append slots per query
-constexpr int kWMaxK = 48;                // largest kk of the wide form
+constexpr int kWMaxK = 64;                // largest kk of the wide form (LDS permitting)
 constexpr int kBPlane = kStepCols * 16;   // one 16-byte group of a staged slice
 constexpr int kBSlice = 8 * kBPlane;      // 128 bytes of k for 256 rows
 
