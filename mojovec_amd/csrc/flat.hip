@@ -80,6 +80,16 @@ __global__ __launch_bounds__(kNT) void flat_chunk_topk_kernel(
   __syncthreads();
 
   const int clen = (int)min<int64_t>(kCB, n - c0);
+  if (k <= 64) {
+    // wavefront top-k: each wavefront folds its queries' chunk keys into the
+    // k smallest (key, row) composites held one per lane, no workgroup barrier
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int qi = wave; qi < qcount; qi += kNW) {
+      const uint64_t best = wave_fold_keys<kCB>(keys + qi * kCB, clen, (uint32_t)c0, kSentinel64, k);
+      if (lane < k) part[((q0 + qi) * nchunks + chunk) * (int64_t)k + lane] = best;
+    }
+    return;
+  }
   for (int qi = 0; qi < qcount; qi++) {
     const int cnt = select_topk_u32(keys + qi * kCB, clen, k, *scr, sel);
     uint64_t *dst = part + ((q0 + qi) * nchunks + chunk) * (int64_t)k;
