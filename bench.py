@@ -327,7 +327,7 @@ def flat_1m(mv, torch, x_h, q_h, gt, args):
     ids = torch.from_numpy(res["p"][1]).to(gt.device)
     out["plain_fp32"] = {"qps": round(NQ / dt, 1), "ms": round(dt * 1e3, 3),
                          "recall_at_10": round(recall_at(torch, gt, ids), 4),
-                         "achieved_tflops": round(2.0 * N * NQ * D / dt / 1e12, 2), "peak": 157.3}
+                         "achieved_tflops": round(2.0 * N * NQ * D / dt / 1e12, 2), "peak": FP32_VALU_PEAK}
     del fx
     sx = mv.IndexFlatSQ8(D, 0)
     sx.add(x_h)
@@ -598,7 +598,7 @@ def bench_flat(args):
     ix = mv.IndexFlat(d)
     ix.add(x_h)
     out = {}
-    for form, peak, unit in (("plain", 157.3, "TFLOP/s"), ("mixed", 2500.0, "TFLOP/s")):
+    for form, peak, unit in (("plain", FP32_VALU_PEAK, "TFLOP/s"), ("mixed", 2500.0, "TFLOP/s")):
         ix.set_search_mode(form)
         res = {}
 
@@ -681,7 +681,7 @@ def bench_sq8(args):
         fx = mv.IndexFlat(d, metric)
         fx.add(x_h)
         fname = "ip" if metric == 1 else "l2"
-        for form, kfs, peak in (("plain", (1,), 157.3), ("mixed-bf16", (2, 4), 2500.0),
+        for form, kfs, peak in (("plain", (1,), FP32_VALU_PEAK), ("mixed-bf16", (2, 4), 2500.0),
                                 ("mixed-fp8", (2, 4), 5000.0)):
             for kf in kfs:
                 fx.set_search_mode(form, k_factor=kf)
