@@ -108,10 +108,12 @@ def main(argv=None):
     ap.add_argument("--no-flat", action="store_true",
                     help="skip the 1M x 128 exact FP32 Flat vs SQ8 (int8 MFMA) leg")
     ap.add_argument("--workload", default="ivfpq",
-                    choices=["ivfpq", "flat", "sq8", "batch", "serve", "ivfflat", "pairdist"],
+                    choices=["ivfpq", "flat", "sq8", "batch", "serve", "serve_sq8", "ivfflat",
+                             "pairdist"],
                     help="ivfpq: the headline 1M x 128 line; flat / sq8 / batch: the Flat "
                          "100k x 128, SQ8 1M x 768 and skewed 10M x 96 side measurements; "
-                         "serve: continuous batching under Poisson arrivals; ivfflat: exact "
+                         "serve / serve_sq8: continuous batching under Poisson arrivals "
+                         "(IVF-PQ 1M x 128, SQ8 cosine 1M x 768); ivfflat: exact "
                          "FP32 IVF-Flat on the 1M x 128 data; pairdist: the batched distance "
                          "service of a CPU graph search (Flat 1M x 128, SQ8 1M x 768)")
     args = ap.parse_args(argv)
@@ -130,6 +132,8 @@ def main(argv=None):
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "serve_sq8":
+        return bench_serve_sq8(args)
     if args.workload == "ivfflat":
         return bench_ivfflat(args)
 
@@ -355,9 +359,6 @@ def bench_serve(args):
     that reaches recall@10 >= 0.95 (nprobe 64; k * k_factor ~2,900). Reports
     per-request p50 / p99 latency (submit to results on the host) and the
     sustained rate for each offered rate."""
-    import queue
-    import threading
-
     import torch
 
     import mojovec_amd as mv
@@ -373,15 +374,33 @@ def bench_serve(args):
     log("serve index ready")
     settings = {10: (64, 288), 100: (64, 29)}
     srv = mv.Server(ix, slots=4, max_batch=4096, mem_budget=32 << 30)
-    rng = np.random.default_rng(41)
+    out = serve_poisson(srv, q_h, gt, settings, (10_000, 100_000, 300_000, 500_000), seed=41)
+    stats = srv.stats()
+    srv.close()
+    print(json.dumps({"workload": "continuous batching, IVF-PQ 1M x 128 (mixed form), Poisson "
+                                  "arrivals, request sizes log-uniform [1, 4096], k in {10, 100}",
+                      "rates": out, "server": stats}), flush=True)
+    return 0
+
+
+def serve_poisson(srv, q_h, gt, settings, rates, seed, duration=2.0):
+    """Offers each rate for `duration` seconds: Poisson request arrivals,
+    request sizes log-uniform in [1, 4096] queries, k drawn from the keys of
+    `settings` (k -> (nprobe, k_factor)). One thread submits on schedule, the
+    caller's thread waits in order; latency is submit to results on the host."""
+    import queue
+    import threading
+
+    nq_all = q_h.shape[0]
+    rng = np.random.default_rng(seed)
+    ks = sorted(settings)
     out = {}
-    for rate in (10_000, 100_000, 300_000, 500_000):
-        duration = 2.0
+    for rate in rates:
         reqs, t, pos = [], 0.0, 0
         while t < duration:
             size = int(min(4096, max(1, round(np.exp(rng.uniform(0.0, np.log(4096)))))))
-            k = int(rng.choice([10, 100]))
-            reqs.append((t, pos % NQ, size, k))
+            k = int(rng.choice(ks))
+            reqs.append((t, pos % nq_all, size, k))
             pos += size
             t += rng.exponential(size / rate)
         tickets = queue.Queue()
@@ -392,7 +411,7 @@ def bench_serve(args):
                 wait = at - (time.perf_counter() - t0)
                 if wait > 0:
                     time.sleep(wait)
-                idx = (np.arange(size) + start) % NQ
+                idx = (np.arange(size) + start) % nq_all
                 nprobe, kf = settings[k]
                 tickets.put((srv.submit(q_h[idx], k=k, nprobe=nprobe, k_factor=kf), idx))
             tickets.put(None)
@@ -407,7 +426,7 @@ def bench_serve(args):
             ticket, idx = item
             Dr, Ir, ms = srv.wait(ticket)
             lat.append(ms)
-            hits += int((Ir[:, :10, None] == gt[idx][:, None, :]).any(2).sum())
+            hits += int((Ir[:, :10, None] == gt[idx][:, None, :10]).any(2).sum())
             nqs += len(idx)
         t_end = time.perf_counter()
         th.join()
@@ -418,10 +437,39 @@ def bench_serve(args):
                           "p99_ms": round(float(np.percentile(lat, 99)), 3),
                           "recall_at_10": round(hits / (10 * nqs), 4)}
         log(f"serve offered {rate}/s: {out[str(rate)]}")
+    return out
+
+
+def bench_serve_sq8(args):
+    """Continuous batching on the SQ8 cosine 1M x 768 index (int8 MFMA
+    candidates, exact FP32 rerank of k * 2): the embedding data of the sq8
+    workload, Poisson arrivals at 10k..300k queries/s, request sizes
+    log-uniform in [1, 4096], k in {10, 100}; p50 / p99 latency, sustained
+    rate and recall@10 against FP64 brute force."""
+    import torch
+
+    import mojovec_amd as mv
+
+    dev = torch.device("cuda:0")
+    x, q = sq8_data(torch, dev)
+    x_h, q_h = x.cpu().numpy(), q.cpu().numpy()
+    x64 = x.double()
+    gt = []
+    for s in range(0, q.shape[0], 250):
+        gt.append(torch.topk(-(q[s:s + 250].double() @ x64.T), 10, dim=1, largest=False)[1])
+    del x64
+    gt = torch.cat(gt).cpu().numpy()
+    ix = mv.IndexFlatSQ8(x.shape[1], 1)
+    ix.add(x_h)
+    log("serve sq8 index ready")
+    srv = mv.Server(ix, slots=4, max_batch=4096, mem_budget=16 << 30)
+    out = serve_poisson(srv, q_h, gt, {10: (1, 2), 100: (1, 2)}, (10_000, 50_000, 100_000, 200_000, 300_000),
+                        seed=43)
     stats = srv.stats()
     srv.close()
-    print(json.dumps({"workload": "continuous batching, IVF-PQ 1M x 128 (mixed form), Poisson "
-                                  "arrivals, request sizes log-uniform [1, 4096], k in {10, 100}",
+    print(json.dumps({"workload": "continuous batching, SQ8 cosine 1M x 768 (int8 MFMA, FP32 rerank "
+                                  "of 2k), Poisson arrivals, request sizes log-uniform [1, 4096], "
+                                  "k in {10, 100}",
                       "rates": out, "server": stats}), flush=True)
     return 0
 
@@ -617,6 +665,22 @@ def bench_flat(args):
     return 0
 
 
+def sq8_data(torch, dev):
+    """1M unit-norm rows and 10k queries of d = 768 drawn around 2,000
+    random directions with an angular spread of 0.3 rad (seeded)."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(21)
+    n, nq, d = 1_000_000, 10_000, 768
+    dirs = torch.nn.functional.normalize(torch.randn((2000, d), generator=g, device=dev), dim=1)
+
+    def draw(m):
+        lab = torch.randint(0, 2000, (m,), generator=g, device=dev)
+        v = dirs[lab] + (0.3 / d ** 0.5) * torch.randn((m, d), generator=g, device=dev)
+        return torch.nn.functional.normalize(v, dim=1).contiguous()
+
+    return draw(n), draw(nq)
+
+
 def bench_sq8(args):
     """SQ8 Flat with exact rerank: 1M x 768 unit-norm embeddings (2,000
     directions, angular spread 0.3 rad), cosine (int8 codes) and L2 (uint8
@@ -627,17 +691,8 @@ def bench_sq8(args):
     import mojovec_amd as mv
 
     dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(21)
-    n, nq, d, k = 1_000_000, 10_000, 768, 10
-    dirs = torch.nn.functional.normalize(torch.randn((2000, d), generator=g, device=dev), dim=1)
-
-    def draw(m):
-        lab = torch.randint(0, 2000, (m,), generator=g, device=dev)
-        v = dirs[lab] + (0.3 / d ** 0.5) * torch.randn((m, d), generator=g, device=dev)
-        return torch.nn.functional.normalize(v, dim=1).contiguous()
-
-    x, q = draw(n), draw(nq)
+    x, q = sq8_data(torch, dev)
+    n, nq, d, k = x.shape[0], q.shape[0], x.shape[1], 10
     log("sq8 data ready")
     x_h, q_h = x.cpu().numpy(), q.cpu().numpy()
     out = {}

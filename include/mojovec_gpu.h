@@ -289,7 +289,7 @@ int mvg_ivfpq_autotune(mvg_ivfpq *h, const float *q, int64_t nq, int k,
 void mvg_graph_destroy(mvg_graph *g);
 
 /* ---- continuous-batching request loop (no reference counterpart) ------- */
-/* For serving callers of an HBM-resident IVF-PQ index. A dispatcher thread
+/* For serving callers of an HBM-resident IVF-PQ or Flat SQ8 index. A dispatcher thread
  * packs waiting requests with equal (k, k_factor) into batches of at most
  * max_batch_queries queries, admits a batch when one of `slots` (1..16)
  * streams is free and its device-memory estimate fits mem_budget_bytes, and
@@ -298,6 +298,10 @@ void mvg_graph_destroy(mvg_graph *g);
 typedef struct mvg_server mvg_server;
 int mvg_server_create(mvg_ivfpq *index, int slots, int max_batch_queries, int64_t mem_budget_bytes,
                       mvg_server **out);
+/* The same loop over a Flat SQ8 index (mvg_sq8_search per batch; a request's
+ * nprobe is ignored). */
+int mvg_server_create_sq8(mvg_sq8 *index, int slots, int max_batch_queries,
+                          int64_t mem_budget_bytes, mvg_server **out);
 /* Copies q [nq x d]; distances/labels [nq x k] must stay valid until
  * mvg_server_wait(ticket) returns. */
 int mvg_server_submit(mvg_server *s, const float *q, int64_t nq, int k, int nprobe, int k_factor,

@@ -133,17 +133,6 @@ struct mvg_flat {
   hipStream_t s = nullptr;
 };
 
-struct mvg_sq8 {
-  int d = 0, metric = 0;
-  int64_t ntotal = 0;
-  float gmin = 3.40282347e38f, gmax = -3.40282347e38f, scale = 1.0f;  // index_flat_sq8.mojo:234-236
-  DevBuf f32, codes, norms, sums, tmp, q, D, I, excl;
-  DevBuf pair_a, pair_b, pair_out;  // batched distance service
-  int use_mfma = 1;
-  mvg::Sq8Work work;
-  hipStream_t s = nullptr;
-};
-
 namespace {
 
 // Rejects ids outside [0, n): the distance service reads rows by id.

@@ -81,6 +81,26 @@ __device__ void gemm_flush(uint64_t *top, uint64_t *tmp, int kk, int cnt, Select
   if (tid == 0) thr_out = top[kk - 1];
 }
 
+// Workgroup -> (query tile, base split) for a 1-D grid of 8 * per workgroups.
+// Workgroups go to the eight XCDs in turn (linear id mod 8) and each XCD has
+// its own L2, so XCD j takes the contiguous range [j * per, (j + 1) * per) of
+// the split-major order: the workgroups resident on one XCD walk the same base
+// rows for different query tiles, and one memory fetch serves all of them.
+// False for the padding workgroups past qtiles * S.
+__device__ __forceinline__ bool xcd_tile(int64_t nq, int rows, int S, int64_t &qt, int &split) {
+  const int64_t qtiles = (nq + rows - 1) / rows;
+  const int64_t per = gridDim.x >> 3;
+  const int64_t L = blockIdx.x;
+  const int64_t G = (L & 7) * per + (L >> 3);
+  if (G >= qtiles * S) return false;
+  split = (int)(G / qtiles);
+  qt = G - (int64_t)split * qtiles;
+  return true;
+}
+
+// Grid size for xcd_tile: qtiles * S rounded up to a multiple of 8.
+inline unsigned xcd_grid(int64_t qtiles, int64_t S) { return (unsigned)(8 * ((qtiles * S + 7) / 8)); }
+
 template <int KIND>
 __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
     const uint8_t *__restrict__ Q, const uint8_t *__restrict__ X, int64_t nq, int64_t n,
@@ -101,8 +121,10 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
   const int stride = kk + kGCap;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t q0 = (int64_t)blockIdx.x * kQT;
-  const int split = blockIdx.y;
+  int64_t qt;
+  int split;
+  if (!xcd_tile(nq, kQT, S, qt, split)) return;
+  const int64_t q0 = qt * kQT;
   const int64_t b_begin = (int64_t)split * split_len;
   const int64_t b_end = min(n, b_begin + split_len);
   const int qcount = (int)min<int64_t>(kQT, nq - q0);
@@ -287,8 +309,10 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
   uint64_t *buf = reinterpret_cast<uint64_t *>(smem + lay.buf) + wave * 16 * kWCap;
   int *cnt = reinterpret_cast<int *>(smem + lay.cnt) + wave * 16;
 
-  const int64_t q0 = (int64_t)blockIdx.x * kWQ;
-  const int split = blockIdx.y;
+  int64_t qt;
+  int split;
+  if (!xcd_tile(nq, kWQ, S, qt, split)) return;
+  const int64_t q0 = qt * kWQ;
   const int64_t b_begin = (int64_t)split * split_len;
   const int64_t b_end = min(n, b_begin + split_len);
   const int qcount = (int)min<int64_t>(kWQ, nq - q0);
@@ -719,7 +743,7 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
     }
     if (wlay.total > 64 * 1024)
       MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, wlay.total));
-    dim3 grid(qtiles, (unsigned)S);
+    const dim3 grid(xcd_grid(qtiles, S));
 #define MVG_WIDE(K)                                                                               \
   hipLaunchKernelGGL(gemm_wide_kernel<K>, grid, dim3(kNT), wlay.total, s, Q, X, nq, n, row_bytes, d, \
                      qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len, excl,        \
@@ -756,7 +780,7 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
   }
   if (lay.total > 64 * 1024)
     MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lay.total));
-  dim3 grid(qtiles, (unsigned)S);
+  const dim3 grid(xcd_grid(qtiles, S));
 #define MVG_GEMM(K)                                                                               \
   hipLaunchKernelGGL(gemm_topk_kernel<K>, grid, dim3(kNT), lay.total, s, Q, X, nq, n, row_bytes, d, \
                      qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len, excl,        \

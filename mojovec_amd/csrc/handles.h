@@ -1,5 +1,5 @@
-// Internal layout of the IVF-PQ handle, shared by the C ABI (api.cpp) and the
-// request loop (server.cpp).
+// Internal layout of the IVF-PQ and SQ8 handles, shared by the C ABI (api.cpp)
+// and the request loop (server.cpp).
 #pragma once
 
 #include <mutex>
@@ -24,6 +24,19 @@ struct mvg_ivfpq {
   mvg::KmeansWork kw;
   hipStream_t s = nullptr;
   std::mutex search_mu;  // serialises searches that share `work`
+};
+
+// Flat SQ8 handle: FP32 rows (exact rerank), codes [n][sq8_row_stride(d)],
+// integer code norms and sums, the global range and scale.
+struct mvg_sq8 {
+  int d = 0, metric = 0;
+  int64_t ntotal = 0;
+  float gmin = 3.40282347e38f, gmax = -3.40282347e38f, scale = 1.0f;  // index_flat_sq8.mojo:234-236
+  mvg::DevBuf f32, codes, norms, sums, tmp, q, D, I, excl;
+  mvg::DevBuf pair_a, pair_b, pair_out;  // batched distance service
+  int use_mfma = 1;
+  mvg::Sq8Work work;
+  hipStream_t s = nullptr;
 };
 
 // IVF-Flat handle: host lists in append order (what list accessors and

@@ -1,14 +1,16 @@
-// Continuous-batching request loop over an HBM-resident IVF-PQ index.
+// Continuous-batching request loop over an HBM-resident IVF-PQ or Flat SQ8
+// index.
 //
-// Requests (a block of queries with its own k, nprobe and k_factor) queue on
-// the host. A dispatcher thread packs waiting requests that share (k,
+// Requests (a block of queries with its own k, nprobe and k_factor; nprobe is
+// ignored by a Flat SQ8 index) queue on the host. A dispatcher thread packs waiting requests that share (k,
 // k_factor) into a batch (per-query nprobe travels with each query), admits
 // the batch when a slot is free and the batch's device-memory estimate fits
 // the budget, and enqueues it on the slot's stream with the slot's own
 // workspace, so batches overlap on the device. Completed slots are retired as
 // soon as their event fires; a request's result depends only on its own
-// queries (every search kernel works per query), so it is the same alone and
-// in any mix.
+// queries (every search kernel works per query, and a split of the base rows
+// only changes which partial lists hold a candidate, not the canonical
+// (distance, id) top-k they merge to), so it is the same alone and in any mix.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -56,6 +58,7 @@ struct Slot {
   hipStream_t s = nullptr;
   hipEvent_t done = nullptr;
   mvg::IvfWork work;
+  mvg::Sq8Work swork;
   mvg::DevBuf q, np, D, I;
   PinnedBuf<float> hq, hD;
   PinnedBuf<int32_t> hnp;
@@ -69,7 +72,9 @@ struct Slot {
 }  // namespace
 
 struct mvg_server {
-  mvg_ivfpq *ix = nullptr;
+  mvg_ivfpq *ix = nullptr;  // exactly one of ix, sq is set
+  mvg_sq8 *sq = nullptr;
+  int d = 0;
   int device = 0;
   int max_batch = 4096;
   int64_t budget = 0, in_use = 0;
@@ -87,10 +92,18 @@ struct mvg_server {
 
 namespace {
 
-int64_t batch_bytes(const mvg_ivfpq *ix, int64_t nq, int k, int kf, int maxp) {
+int64_t batch_bytes(const mvg_server *sv, int64_t nq, int k, int kf, int maxp) {
   const int64_t kk = (int64_t)k * kf;
+  if (sv->sq) {
+    // queries, query codes and norms, outputs, merged candidates, and the
+    // per-split partial lists: at most 64 splits per query in the 64-query
+    // form, at most 2048 workgroups of 16 queries in the 16-query form
+    const int64_t ld = mvg::sq8_row_stride(sv->d);
+    return nq * ((int64_t)sv->d * 4 + ld + 8 + (int64_t)k * 12 + kk * 8 * 65) +
+           (int64_t)2048 * 16 * kk * 8 + (1 << 20);
+  }
   // queries, probes, per-query candidates (work queue: up to 8 items), outputs
-  return nq * ((int64_t)ix->d * 6 + (int64_t)maxp * 12 + kk * 8 * 9 + (int64_t)k * 12) + (1 << 20);
+  return nq * ((int64_t)sv->d * 6 + (int64_t)maxp * 12 + kk * 8 * 9 + (int64_t)k * 12) + (1 << 20);
 }
 
 void retire(mvg_server *sv, Slot &sl) {
@@ -143,8 +156,8 @@ std::vector<Request *> take_batch(mvg_server *sv, int64_t &bytes) {
       continue;
     }
     if (!batch.empty() && nq + r->nq > sv->max_batch) break;
-    const int mp = std::max(maxp, std::min(r->nprobe, sv->ix->nlist));
-    const int64_t b = batch_bytes(sv->ix, nq + r->nq, k, kf, mp);
+    const int mp = sv->ix ? std::max(maxp, std::min(r->nprobe, sv->ix->nlist)) : 1;
+    const int64_t b = batch_bytes(sv, nq + r->nq, k, kf, mp);
     // admission by memory: the first request of an idle server always runs
     if (sv->in_use + b > sv->budget && !(batch.empty() && sv->inflight == 0)) break;
     batch.push_back(r);
@@ -157,7 +170,7 @@ std::vector<Request *> take_batch(mvg_server *sv, int64_t &bytes) {
 }
 
 void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t bytes) {
-  const int d = sv->ix->d;
+  const int d = sv->d;
   int64_t nq = 0;
   int maxp = 1;
   for (Request *r : batch) nq += r->nq;
@@ -169,7 +182,7 @@ void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t byt
   int64_t off = 0;
   for (Request *r : batch) {
     std::copy(r->q.begin(), r->q.end(), sl.hq.ptr + off * d);
-    const int np = std::min(r->nprobe, sv->ix->nlist);
+    const int np = sv->ix ? std::min(r->nprobe, sv->ix->nlist) : 1;
     maxp = std::max(maxp, np);
     std::fill(sl.hnp.ptr + off, sl.hnp.ptr + off + r->nq, np);
     off += r->nq;
@@ -179,9 +192,32 @@ void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t byt
   sl.D.reserve(sizeof(float) * nq * k);
   sl.I.reserve(sizeof(int64_t) * nq * k);
   MVG_HIP(hipMemcpyAsync(sl.q.ptr, sl.hq.ptr, sizeof(float) * nq * d, hipMemcpyHostToDevice, sl.s));
-  MVG_HIP(hipMemcpyAsync(sl.np.ptr, sl.hnp.ptr, sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.s));
-  mvg::ivf_search_with(sv->ix, sl.q.as<float>(), nq, k, sl.np.as<int32_t>(), maxp, kf, nullptr, 0,
-                       sl.D.as<float>(), sl.I.as<int64_t>(), sl.work, sl.s);
+  if (sv->ix)
+    MVG_HIP(hipMemcpyAsync(sl.np.ptr, sl.hnp.ptr, sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.s));
+  if (sv->ix) {
+    mvg::ivf_search_with(sv->ix, sl.q.as<float>(), nq, k, sl.np.as<int32_t>(), maxp, kf, nullptr, 0,
+                         sl.D.as<float>(), sl.I.as<int64_t>(), sl.work, sl.s);
+  } else {
+    const mvg_sq8 *h = sv->sq;
+    mvg::Sq8SearchArgs a{};
+    a.q = sl.q.as<float>();
+    a.nq = nq;
+    a.d = h->d;
+    a.metric = h->metric;
+    a.n = h->ntotal;
+    a.codes = h->codes.as<uint8_t>();
+    a.norms = h->norms.as<uint32_t>();
+    a.sums = h->sums.as<uint32_t>();
+    a.vectors = h->f32.as<float>();
+    a.gmin = h->gmin;
+    a.scale = h->scale;
+    a.k = k;
+    a.k_factor = kf;
+    a.D = sl.D.as<float>();
+    a.I = sl.I.as<int64_t>();
+    a.use_mfma = h->use_mfma;
+    mvg::sq8_search(a, sl.swork, sl.s);
+  }
   MVG_HIP(hipMemcpyAsync(sl.hD.ptr, sl.D.ptr, sizeof(float) * nq * k, hipMemcpyDeviceToHost, sl.s));
   MVG_HIP(hipMemcpyAsync(sl.hI.ptr, sl.I.ptr, sizeof(int64_t) * nq * k, hipMemcpyDeviceToHost, sl.s));
   MVG_HIP(hipEventRecord(sl.done, sl.s));
@@ -244,6 +280,32 @@ void dispatcher(mvg_server *sv) {
   }
 }
 
+// Validates the settings, creates the slots' streams and starts the dispatcher.
+int start_server(mvg_ivfpq *ix, mvg_sq8 *sq, int d, int slots, int max_batch_queries,
+                 int64_t mem_budget_bytes, mvg_server **out) {
+  if (slots < 1 || slots > 16 || max_batch_queries < 1 || mem_budget_bytes <= 0 || !out)
+    throw mvg::Error(MVG_ERR_INVALID,
+                     "server needs 1..16 slots, max_batch_queries >= 1 and a positive memory "
+                     "budget");
+  auto sv = std::make_unique<mvg_server>();
+  sv->ix = ix;
+  sv->sq = sq;
+  sv->d = d;
+  MVG_HIP(hipGetDevice(&sv->device));
+  sv->max_batch = max_batch_queries;
+  sv->budget = mem_budget_bytes;
+  for (int i = 0; i < slots; i++) {
+    auto sl = std::make_unique<Slot>();
+    MVG_HIP(hipStreamCreateWithFlags(&sl->s, hipStreamNonBlocking));
+    MVG_HIP(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
+    sv->slots.push_back(std::move(sl));
+  }
+  mvg_server *raw = sv.get();
+  sv->th = std::thread(dispatcher, raw);
+  *out = sv.release();
+  return MVG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -251,28 +313,11 @@ extern "C" {
 int mvg_server_create(mvg_ivfpq *ix, int slots, int max_batch_queries, int64_t mem_budget_bytes,
                       mvg_server **out) {
   try {
-    if (!ix || slots < 1 || slots > 16 || max_batch_queries < 1 || mem_budget_bytes <= 0)
-      throw mvg::Error(MVG_ERR_INVALID,
-                       "server needs an index, 1..16 slots, max_batch_queries >= 1 and a positive "
-                       "memory budget");
+    if (!ix) throw mvg::Error(MVG_ERR_INVALID, "server needs an index");
     if (!ix->trained || ix->ntotal == 0)
       throw mvg::Error(MVG_ERR_STATE, "server needs a trained, non-empty index");
     mvg::ivf_prepare(ix);
-    auto sv = std::make_unique<mvg_server>();
-    sv->ix = ix;
-    MVG_HIP(hipGetDevice(&sv->device));
-    sv->max_batch = max_batch_queries;
-    sv->budget = mem_budget_bytes;
-    for (int i = 0; i < slots; i++) {
-      auto sl = std::make_unique<Slot>();
-      MVG_HIP(hipStreamCreateWithFlags(&sl->s, hipStreamNonBlocking));
-      MVG_HIP(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
-      sv->slots.push_back(std::move(sl));
-    }
-    mvg_server *raw = sv.get();
-    sv->th = std::thread(dispatcher, raw);
-    *out = sv.release();
-    return MVG_OK;
+    return start_server(ix, nullptr, ix->d, slots, max_batch_queries, mem_budget_bytes, out);
   } catch (const mvg::Error &e) {
     mvg::set_last_error(e.what());
     return e.code;
@@ -282,6 +327,20 @@ int mvg_server_create(mvg_ivfpq *ix, int slots, int max_batch_queries, int64_t m
   }
 }
 
+int mvg_server_create_sq8(mvg_sq8 *ix, int slots, int max_batch_queries, int64_t mem_budget_bytes,
+                          mvg_server **out) {
+  try {
+    if (!ix) throw mvg::Error(MVG_ERR_INVALID, "server needs an index");
+    if (ix->ntotal == 0) throw mvg::Error(MVG_ERR_STATE, "server needs a non-empty index");
+    return start_server(nullptr, ix, ix->d, slots, max_batch_queries, mem_budget_bytes, out);
+  } catch (const mvg::Error &e) {
+    mvg::set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception &e) {
+    mvg::set_last_error(e.what());
+    return MVG_ERR_DEVICE;
+  }
+}
 int mvg_server_submit(mvg_server *sv, const float *q, int64_t nq, int k, int nprobe, int k_factor,
                       float *distances, int64_t *labels, int64_t *ticket) {
   if (!sv || nq < 1 || k < 1 || nprobe < 1 || k_factor < 1 || !distances || !labels) {
@@ -293,7 +352,7 @@ int mvg_server_submit(mvg_server *sv, const float *q, int64_t nq, int k, int npr
     return MVG_ERR_INVALID;
   }
   auto r = std::make_unique<Request>();
-  r->q.assign(q, q + nq * sv->ix->d);
+  r->q.assign(q, q + nq * sv->d);
   r->nq = nq;
   r->k = k;
   r->nprobe = nprobe;

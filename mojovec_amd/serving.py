@@ -1,4 +1,4 @@
-"""Continuous-batching server over an HBM-resident IVF-PQ index
+"""Continuous-batching server over an HBM-resident IVF-PQ or Flat SQ8 index
 (mvg_server_* in include/mojovec_gpu.h).
 
     srv = Server(index, slots=4, max_batch=8192, mem_budget=8 << 30)
@@ -16,8 +16,12 @@ from . import _lib
 
 class Server:
     def __init__(self, index, slots: int = 4, max_batch: int = 8192, mem_budget: int = 8 << 30):
+        from .index import IndexFlatSQ8
+
         h = ctypes.c_void_p()
-        _lib.check(_lib.load().mvg_server_create(index._h, slots, max_batch, mem_budget, ctypes.byref(h)))
+        create = _lib.load().mvg_server_create_sq8 if isinstance(index, IndexFlatSQ8) else \
+            _lib.load().mvg_server_create
+        _lib.check(create(index._h, slots, max_batch, mem_budget, ctypes.byref(h)))
         self._h = h
         self._index = index  # keeps the index alive
         self._out = {}

@@ -80,3 +80,29 @@ def test_server_mixed_precision_and_work_queue(gpu):
     Da, Ia = ix.search(q, 10, k_factor=4)
     assert np.array_equal(np.concatenate([r[1] for r in res]), Ia)
     srv.close()
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+@pytest.mark.parametrize("use_mfma", [True, False])
+def test_sq8_server_results_match_alone(gpu, metric, use_mfma):
+    # requests of 1..300 queries share batches of up to 700; the 16- and
+    # 64-query MFMA forms and the split count all change with the batch size
+    rng = np.random.default_rng(8 + metric)
+    x = rng.standard_normal((6000, 96)).astype(np.float32)
+    ix = gpu.IndexFlatSQ8(96, metric)
+    ix.add(x)
+    ix.set_search_mode(use_mfma)
+    srv = gpu.Server(ix, slots=3, max_batch=700, mem_budget=256 << 20)
+    reqs = []
+    for i in range(30):
+        nq = int(rng.integers(1, 300))
+        q = rng.standard_normal((nq, 96)).astype(np.float32)
+        k = int(rng.choice([1, 10, 50]))
+        kf = int(rng.choice([1, 4]))
+        reqs.append((q, k, kf, srv.submit(q, k=k, nprobe=1, k_factor=kf)))
+    outs = [(q, k, kf, srv.wait(t)) for q, k, kf, t in reqs]
+    assert srv.stats()["completed_queries"] == sum(q.shape[0] for q, *_ in reqs)
+    for q, k, kf, (D, I, lat) in outs:
+        Da, Ia = ix.search(q, k, k_factor=kf)
+        assert np.array_equal(I, Ia) and np.array_equal(D, Da), (q.shape[0], k, kf)
+    srv.close()
