@@ -182,6 +182,12 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
  * split_scan 1 selects one workgroup per (query, list) with a merge pass
  * instead of one workgroup per query (same results). */
 int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan);
+/* Mixed-form pre-rerank (no reference counterpart): with stored rows, the
+ * mixed form scores its k * k_factor candidates by the SQ8 code distance of
+ * the rows (global range and scale as IndexFlatSQ8 computes them,
+ * index_flat_sq8.mojo:317-367) and reranks only the best max(64, per_k * k)
+ * in exact FP32. per_k 0 reranks every candidate in FP32. Default 8. */
+int mvg_ivfpq_set_prefilter(mvg_ivfpq *h, int per_k);
 /* Skewed-list load balance (no reference counterpart): entries_per_item > 0
  * runs the scan as persistent workgroups over a device-built queue of
  * (query, entry-range) items of about that many list entries (at most 8 per

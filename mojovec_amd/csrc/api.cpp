@@ -730,6 +730,7 @@ int mvg_ivfpq_add(mvg_ivfpq *h, const float *x, int64_t n) {
   }
   h->ntotal += n;
   h->dirty = true;
+  h->vq_dirty = true;
   API_END
 }
 
@@ -772,6 +773,27 @@ void mvg::ivf_prepare(mvg_ivfpq *h) {
     MVG_HIP(hipStreamSynchronize(h->s));
     h->tables_dirty = false;
     h->pre_dirty = true;
+  }
+  if (h->precision != 0 && h->prefilter > 0 && h->keep_vectors && h->vq_dirty && h->ntotal > 0) {
+    // SQ8 codes of the stored rows (index_flat_sq8.mojo:317-367 range and scale)
+    const int64_t n = h->ntotal;
+    const int d = h->d, ld = mvg::sq8_row_stride(d);
+    float mm[2];
+    mvg::sq8_minmax(h->vectors.as<float>(), n * d, mm, h->vq_tmp, h->s);
+    const bool sym = h->metric != MVG_METRIC_L2;
+    if (!sym) {
+      h->vq_scale = mm[1] == mm[0] ? 1.0f : (mm[1] - mm[0]) / 255.0f;
+    } else {
+      const float mx = std::max(std::fabs(mm[0]), std::fabs(mm[1]));
+      h->vq_scale = mx > 0.0f ? mx / 127.0f : 1.0f;
+    }
+    h->vq_min = sym ? 0.0f : mm[0];
+    h->vq_codes.reserve((size_t)n * ld + 16);
+    h->vq_norms.reserve(sizeof(uint32_t) * (size_t)n);
+    mvg::sq8_encode(h->vectors.as<float>(), n, d, sym ? 1 : 0, h->vq_min, 1.0f / h->vq_scale,
+                    h->vq_codes.as<uint8_t>(), h->vq_norms.as<uint32_t>(), nullptr, h->s);
+    MVG_HIP(hipStreamSynchronize(h->s));
+    h->vq_dirty = false;
   }
   if (h->precision != 0 && h->metric == MVG_METRIC_L2 && h->pre_dirty) {
     h->entry_pre.reserve(sizeof(float) * (size_t)std::max<int64_t>(h->ntotal, 1));
@@ -824,6 +846,14 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
     }
     a.coarse_bf16 = h->coarse_bf16.as<uint16_t>();
     a.coarse_norm = h->coarse_norm.as<float>();
+    if (h->prefilter > 0 && a.vectors && h->ntotal > 0) {
+      if (h->vq_dirty) throw Error(mvg::kState, "pre-rerank row codes are not prepared");
+      a.vq_codes = h->vq_codes.as<uint8_t>();
+      a.vq_norms = h->vq_norms.as<uint32_t>();
+      a.vq_min = h->vq_min;
+      a.vq_scale = h->vq_scale;
+      a.prefilter = h->prefilter;
+    }
   }
   mvg::ivfpq_search(a, work, s);
 }
@@ -874,6 +904,13 @@ int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan) {
   require(h && (precision == 0 || precision == 1), "precision must be 0 (plain) or 1 (mixed)");
   h->precision = precision;
   h->split_scan = split_scan != 0;
+  API_END
+}
+
+int mvg_ivfpq_set_prefilter(mvg_ivfpq *h, int per_k) {
+  API_BEGIN
+  require(h && per_k >= 0, "per_k must be >= 0");
+  h->prefilter = per_k;
   API_END
 }
 

@@ -22,6 +22,12 @@ struct mvg_ivfpq {
   mvg::DevBuf tx, tlab, tcodes, tres, q, D, I, excl, npq;
   mvg::IvfWork work;
   mvg::KmeansWork kw;
+  // mixed form: SQ8 codes of the stored rows for the pre-rerank (global range
+  // as IndexFlatSQ8 computes it); rebuilt when rows were added
+  int prefilter = 8;  // pre-rerank keeps max(64, prefilter * k) candidates; 0: off
+  bool vq_dirty = true;
+  float vq_min = 0.0f, vq_scale = 1.0f;
+  mvg::DevBuf vq_codes, vq_norms, vq_tmp;
   hipStream_t s = nullptr;
   std::mutex search_mu;  // serialises searches that share `work`
 };

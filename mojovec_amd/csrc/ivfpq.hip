@@ -320,6 +320,89 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
   }
 }
 
+// Mixed-form pre-rerank: the SQ8 code distance of each of a query's kk
+// candidates (the integer SQ8 distance of index_flat_sq8.mojo:80-177 over
+// codes of the stored rows), then the kk2 best of them, sorted, as the
+// candidates of the exact FP32 rerank. Eight lanes per candidate row, 16-byte
+// code chunks, two rows per group in flight; keys are (integer distance << 32
+// | id), so equal code distances keep the lower id.
+template <bool IP>
+__global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
+    const uint8_t *__restrict__ qcodes, const uint32_t *__restrict__ qnorms, int ld,
+    const uint8_t *__restrict__ vcodes, const uint32_t *__restrict__ vnorms, int64_t nrows,
+    const uint64_t *__restrict__ cand, int kk, int kk2, uint64_t *__restrict__ out, int64_t nq,
+    const int32_t *__restrict__ order) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint64_t *buf = reinterpret_cast<uint64_t *>(smem);  // kk
+  uint64_t *top = buf + ((kk + 1) & ~1);               // next_pow2(kk2)
+  SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(top + next_pow2(kk2));
+  const int64_t qi = rerank_query(nq, order);
+  if (qi < 0) return;
+  constexpr int G = kNT / 8, R = 2;
+  const int tid = threadIdx.x, gl = tid & 7, group = tid >> 3;
+  const uint4 *qa = reinterpret_cast<const uint4 *>(qcodes + qi * ld);
+  const int words = ld / 16;
+  const uint64_t *cq = cand + qi * kk;
+  for (int c = group; c < kk; c += R * G) {
+    uint32_t id[R], acc[R];
+    bool ok[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+      const int ct = c + t * G;
+      const uint64_t kv = ct < kk ? cq[ct] : kSentinel64;
+      id[t] = (uint32_t)(kv & 0xFFFFFFFFu);
+      ok[t] = kv != kSentinel64 && (int64_t)id[t] < nrows;
+      acc[t] = 0;
+    }
+    for (int w = gl; w < words; w += 8) {
+      const uint4 a = qa[w];
+#pragma unroll
+      for (int t = 0; t < R; t++) {
+        if (!ok[t]) continue;
+        const uint4 b = reinterpret_cast<const uint4 *>(vcodes + (int64_t)id[t] * ld)[w];
+        if (IP) {
+          int v = (int)acc[t];
+          v = __builtin_amdgcn_sdot4((int)a.x, (int)b.x, v, false);
+          v = __builtin_amdgcn_sdot4((int)a.y, (int)b.y, v, false);
+          v = __builtin_amdgcn_sdot4((int)a.z, (int)b.z, v, false);
+          v = __builtin_amdgcn_sdot4((int)a.w, (int)b.w, v, false);
+          acc[t] = (uint32_t)v;
+        } else {
+          acc[t] = __builtin_amdgcn_udot4(a.x, b.x, acc[t], false);
+          acc[t] = __builtin_amdgcn_udot4(a.y, b.y, acc[t], false);
+          acc[t] = __builtin_amdgcn_udot4(a.z, b.z, acc[t], false);
+          acc[t] = __builtin_amdgcn_udot4(a.w, b.w, acc[t], false);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+#pragma unroll
+      for (int o = 4; o >= 1; o >>= 1) acc[t] += (uint32_t)__shfl_xor((int)acc[t], o, 8);
+    }
+    if (gl == 0) {
+#pragma unroll
+      for (int t = 0; t < R; t++) {
+        const int ct = c + t * G;
+        if (ct >= kk) continue;
+        uint32_t key = 0xFFFFFFFFu;
+        if (ok[t]) {
+          if (IP) {
+            key = (uint32_t)((int64_t)0x80000000LL - (int64_t)(int32_t)acc[t]);
+          } else {
+            const int64_t v = (int64_t)qnorms[qi] + (int64_t)vnorms[id[t]] - 2 * (int64_t)acc[t];
+            key = (uint32_t)(v > 0 ? v : 0);
+          }
+        }
+        buf[ct] = ok[t] ? ((uint64_t)key << 32 | id[t]) : kSentinel64;
+      }
+    }
+  }
+  __syncthreads();
+  const int cnt = select_sort_u64(buf, kk, kk2, *scr, top);
+  for (int t = tid; t < kk2; t += kNT) out[qi * kk2 + t] = t < cnt ? top[t] : kSentinel64;
+}
+
 // ---------------------------------------------------------------------------
 // Per-query scan: one workgroup walks the query's probed lists in coarse
 // order and keeps a running canonical top-kk of u64 (key << 32 | id) in LDS.
@@ -1978,6 +2061,48 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
   MVG_LAUNCH_CHECK("rerank_kernel");
 }
 
+int prefilter_width(int k, int kk, int per_k) {
+  if (per_k <= 0) return kk;
+  return (int)std::min<int64_t>(kk, std::max<int64_t>(64, (int64_t)per_k * k));
+}
+
+// Mixed-form rerank of kk candidates per query: with SQ8 row codes and
+// kk > prefilter_width, the SQ8 pre-rerank narrows them first.
+void rerank_mixed(const IvfSearchArgs &a, IvfWork &w, const uint64_t *keys, int kk,
+                  const int32_t *order, hipStream_t s) {
+  const int kk2 = prefilter_width(a.k, kk, a.prefilter);
+  if (!a.vq_codes || kk2 >= kk) {
+    rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, keys, kk, a.k, kIvfPadDist, a.D, a.I, s,
+           order);
+    return;
+  }
+  const bool ip = a.metric == kMetricIP;
+  const int ld = sq8_row_stride(a.d);
+  w.vq_qcodes.reserve((size_t)a.nq * ld);
+  w.vq_qnorms.reserve(sizeof(uint32_t) * a.nq);
+  w.vq_keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk2);
+  sq8_encode(a.q, a.nq, a.d, ip ? 1 : 0, ip ? 0.0f : a.vq_min, 1.0f / a.vq_scale,
+             w.vq_qcodes.as<uint8_t>(), w.vq_qnorms.as<uint32_t>(), nullptr, s);
+  const size_t sm = sizeof(uint64_t) * (size_t)(((kk + 1) & ~1) + next_pow2(kk2)) + sizeof(SelectScratch64);
+  if (sm > 150 * 1024) throw Error(kInvalid, "pre-rerank workspace exceeds LDS");
+  const unsigned grid =
+      (unsigned)(order ? kRerankRanges * ((a.nq + kRerankRanges - 1) / kRerankRanges) : a.nq);
+  const void *fn = ip ? (const void *)sq8_prefilter_kernel<true> : (const void *)sq8_prefilter_kernel<false>;
+  if (sm > 64 * 1024)
+    MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  if (ip)
+    hipLaunchKernelGGL(sq8_prefilter_kernel<true>, dim3(grid), dim3(kNT), sm, s, w.vq_qcodes.as<uint8_t>(),
+                       w.vq_qnorms.as<uint32_t>(), ld, a.vq_codes, a.vq_norms, a.ntotal, keys, kk, kk2,
+                       w.vq_keys.as<uint64_t>(), a.nq, order);
+  else
+    hipLaunchKernelGGL(sq8_prefilter_kernel<false>, dim3(grid), dim3(kNT), sm, s, w.vq_qcodes.as<uint8_t>(),
+                       w.vq_qnorms.as<uint32_t>(), ld, a.vq_codes, a.vq_norms, a.ntotal, keys, kk, kk2,
+                       w.vq_keys.as<uint64_t>(), a.nq, order);
+  MVG_LAUNCH_CHECK("sq8_prefilter_kernel");
+  rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, w.vq_keys.as<uint64_t>(), kk2, a.k, kIvfPadDist,
+         a.D, a.I, s, order);
+}
+
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   if (a.nq == 0) return;
   w.ord_ready = false;  // set when a deep-pool scan builds the clustered query order
@@ -2110,9 +2235,14 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
                      wq.item_off);
     }
     if (pf.on) MVG_HIP(hipEventRecord(pf.ev[2], s));
-    if (kf > 1)
-      rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, keys, kk, a.k, kIvfPadDist, a.D, a.I, s,
-             w.ord_ready ? w.ord.as<int32_t>() : nullptr);
+    if (kf > 1) {
+      const int32_t *ord = w.ord_ready ? w.ord.as<int32_t>() : nullptr;
+      if (a.precision != 0)
+        rerank_mixed(a, w, keys, kk, ord, s);
+      else
+        rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, keys, kk, a.k, kIvfPadDist, a.D, a.I, s,
+               ord);
+    }
   } else {
   // 2. per-(query, probe) LUT + ADC scan + list-local top-kk.
   w.part.reserve(sizeof(uint64_t) * (size_t)a.nq * maxp * kk);
@@ -2136,8 +2266,11 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
     launch_merge(part, a.nq, (int64_t)maxp * kk, kk, 0, kIvfPadDist, nullptr, nullptr, nullptr,
                  w.keys.as<uint64_t>(), s);
-    rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, w.keys.as<uint64_t>(), kk, a.k, kIvfPadDist, a.D,
-           a.I, s);
+    if (a.precision != 0)
+      rerank_mixed(a, w, w.keys.as<uint64_t>(), kk, nullptr, s);
+    else
+      rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, w.keys.as<uint64_t>(), kk, a.k, kIvfPadDist,
+             a.D, a.I, s);
   }
   }
   if (pf.on) {

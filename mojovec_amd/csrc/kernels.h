@@ -69,6 +69,12 @@ struct IvfSearchArgs {
   int deep_min;      // k * k_factor from which the deep-pool scan runs (0: default)
   const float *pre;  // mixed L2: per-entry sum_m T[l][m][code_m] [ntotal] (nullable)
   int deep_pool;     // stream scan, large pools: 1 keys in HBM, 0 running set in LDS
+  // mixed form: SQ8 codes of the stored rows (nullable) for the pre-rerank,
+  // which keeps max(64, prefilter * k) of the k * k_factor candidates
+  const uint8_t *vq_codes;  // [ntotal][sq8_row_stride(d)]
+  const uint32_t *vq_norms;
+  float vq_min, vq_scale;
+  int prefilter;
 };
 // pre[e] = sum_m ptab[l][m][code_m] for every entry of the CSR lists.
 void precompute_entry_terms(const int64_t *list_off, int nlist, const uint8_t *codes, int M,
@@ -91,6 +97,7 @@ struct IvfWork {
   DevBuf wq_poff, wq_nitems, wq_rq, wq_off, wq_gthr, wq_counter;
   DevBuf deep_seg, deep_counter;  // deep candidate pools: key segments, query range counters
   DevBuf ord_cnt, ord_off, ord;    // query order of the persistent deep scans
+  DevBuf vq_qcodes, vq_qnorms, vq_keys;  // mixed pre-rerank: query codes, narrowed candidates
   bool ord_ready = false;          // `ord` holds this search's order
   IvfProfile prof;
 };

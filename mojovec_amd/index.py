@@ -291,6 +291,12 @@ class IndexIVFPQ:
         p = {"plain": 0, "mixed": 1}[precision]
         _lib.check(_lib.load().mvg_ivfpq_set_search_mode(self._h, p, int(split_scan)))
 
+    def set_prefilter(self, per_k: int = 8) -> None:
+        """Mixed form: narrow the k * k_factor candidates to max(64, per_k * k)
+        by the SQ8 code distance of the stored rows before the exact FP32
+        rerank; 0 reranks every candidate in FP32."""
+        _lib.check(_lib.load().mvg_ivfpq_set_prefilter(self._h, int(per_k)))
+
     def set_work_queue(self, entries_per_item: int = 0) -> None:
         """Persistent (query, entry-range) work queue for skewed lists; 0 off."""
         _lib.check(_lib.load().mvg_ivfpq_set_work_queue(self._h, int(entries_per_item)))

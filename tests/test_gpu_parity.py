@@ -343,9 +343,68 @@ def test_ivfpq_mixed_form_error_bound_and_rerank(gpu):
     assert np.allclose(Dm, Do, rtol=1e-4, atol=0.5)
     overlap = np.mean([len(set(a) & set(b)) / 50 for a, b in zip(Im.tolist(), Io.tolist())])
     assert overlap > 0.95
+    gi.set_prefilter(0)  # no SQ8 pre-rerank: all candidates in FP32
     Dr, Ir = gi.search(q[:5], 5, k_factor=800)  # 4000 of 6000 candidates reranked
     Df, If = o.flat_search(x, q[:5], 5)
     assert np.array_equal(Ir, If) and np.array_equal(Dr, Df)
+    gi.set_search_mode("plain")
+
+
+@pytest.mark.parametrize("metric", [0, 1])
+def test_ivfpq_mixed_prefilter_exact_codes(gpu, metric):
+    # Integer rows whose range pins the SQ8 scale to 1 (L2: [0, 255]; IP:
+    # max |x| = 127): the codes are the rows, the SQ8 pre-rerank orders
+    # candidates by the exact distance with the same id tie-break, so
+    # narrowing k * k_factor candidates to max(64, 8k) before the FP32 rerank
+    # changes nothing.
+    x = clustered(60 + metric, 6000, 32, nclusters=30)
+    if metric == 0:
+        x[0, :], x[1, :] = 0.0, 255.0
+    else:
+        x = np.clip(x - 128.0, -127.0, 127.0)
+        x[0, :] = 127.0
+    _, gi = _pair(gpu, x, 32, 16, 8, metric)
+    gi.add(x)
+    q = clustered(70 + metric, 40, 32, nclusters=30)
+    if metric == 1:
+        q = np.clip(q - 128.0, -127.0, 127.0)
+    gi.nprobe = 8
+    gi.set_search_mode("mixed")
+    for k, kf in ((10, 50), (100, 12), (1, 300)):
+        gi.set_prefilter(0)
+        Df, If = gi.search(q, k, k_factor=kf)
+        gi.set_prefilter(8)
+        Dp, Ip = gi.search(q, k, k_factor=kf)
+        assert np.array_equal(Ip, If) and np.array_equal(Dp, Df), (k, kf)
+    gi.set_search_mode("plain")
+
+
+def test_ivfpq_mixed_prefilter_float_rows(gpu):
+    # Rows the codes only approximate: the narrowed candidates still hold
+    # nearly all of the FP32 top-k, and every returned distance is the FP32
+    # rerank distance of its row.
+    rng = np.random.default_rng(80)
+    x = (clustered(81, 6000, 32, nclusters=30) + rng.uniform(-0.5, 0.5, (6000, 32))).astype(np.float32)
+    _, gi = _pair(gpu, x, 32, 16, 8, 0)
+    gi.add(x)
+    q = clustered(82, 50, 32, nclusters=30)
+    gi.nprobe = 8
+    gi.set_search_mode("mixed")
+    gi.set_prefilter(0)
+    Df, If = gi.search(q, 10, k_factor=60)
+    gi.set_prefilter(8)
+    Dp, Ip = gi.search(q, 10, k_factor=60)
+    overlap = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(Ip.tolist(), If.tolist())])
+    assert overlap > 0.99
+    for i in range(q.shape[0]):  # a row kept by both has the same FP32 rerank distance
+        full = dict(zip(If[i].tolist(), Df[i].tolist()))
+        for j, dv in zip(Ip[i].tolist(), Dp[i].tolist()):
+            if j in full:
+                assert full[j] == dv
+    Da, Ia = o.flat_search(x, q, 6000)  # every row's distance: within FP32 rounding
+    for i in range(q.shape[0]):
+        exact = dict(zip(Ia[i].tolist(), Da[i].tolist()))
+        assert np.allclose([exact[j] for j in Ip[i].tolist()], Dp[i], rtol=1e-6)
     gi.set_search_mode("plain")
 
 
