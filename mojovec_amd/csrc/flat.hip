@@ -183,11 +183,11 @@ __global__ __launch_bounds__(kNT) void flat_argmin_kernel(
 __global__ __launch_bounds__(kNT) void merge_topk_kernel(
     const uint64_t *__restrict__ cand, int64_t ncand, int k, int negate, float pad_dist,
     float *__restrict__ D, int64_t *__restrict__ I64, int32_t *__restrict__ I32,
-    uint64_t *__restrict__ out_keys, const int64_t *__restrict__ seg_off) {
+    uint64_t *__restrict__ out_keys, const int64_t *__restrict__ seg_off, int cap) {
   extern __shared__ __align__(16) unsigned char smem[];
-  uint64_t *buf = reinterpret_cast<uint64_t *>(smem);  // kMergeCap
-  uint64_t *top = buf + kMergeCap;                       // kMaxCandidates
-  SelectScratch64 &scr = *reinterpret_cast<SelectScratch64 *>(top + kMaxCandidates);
+  uint64_t *buf = reinterpret_cast<uint64_t *>(smem);  // cap
+  uint64_t *top = buf + cap;                             // next_pow2(k)
+  SelectScratch64 &scr = *reinterpret_cast<SelectScratch64 *>(top + next_pow2(k));
   const int tid = threadIdx.x;
   const int64_t qi = blockIdx.x;
   // Fixed-size candidate rows, or (seg_off) rows of seg_off[q+1]-seg_off[q]
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kNT) void merge_topk_kernel(
   const uint64_t *src = seg_off ? cand + seg_off[qi] * ncand : cand + qi * ncand;
   const int64_t n = seg_off ? (seg_off[qi + 1] - seg_off[qi]) * ncand : ncand;
   int cur = 0;
-  const int room = kMergeCap - k;
+  const int room = cap - k;
   for (int64_t b = 0; b < n; b += room) {
     const int len = (int)min<int64_t>(room, n - b);
     for (int i = tid; i < cur; i += kNT) buf[i] = top[i];
@@ -226,15 +226,21 @@ void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int ne
                   hipStream_t s, const int64_t *seg_off) {
   if (nq == 0) return;
   if (k > kMaxCandidates) throw Error(kInvalid, "k exceeds the device top-k limit of 4096");
-  const size_t sm = sizeof(uint64_t) * (kMergeCap + kMaxCandidates) + sizeof(SelectScratch64);
+  // The buffer holds the running top k plus the next candidates; a merge of
+  // fixed-size rows smaller than kMergeCap - k sizes it to one pass, so small
+  // merges keep several workgroups per CU.
+  const int cap = seg_off ? kMergeCap
+                          : (int)std::min<int64_t>(kMergeCap, std::max<int64_t>(ncand, 1) + k);
+  const size_t sm = sizeof(uint64_t) * (cap + next_pow2(k)) + sizeof(SelectScratch64);
   static bool attr_set = false;
   if (!attr_set) {
+    const size_t most = sizeof(uint64_t) * (kMergeCap + kMaxCandidates) + sizeof(SelectScratch64);
     MVG_HIP(hipFuncSetAttribute((const void *)merge_topk_kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)most));
     attr_set = true;
   }
   hipLaunchKernelGGL(merge_topk_kernel, dim3((unsigned)nq), dim3(kNT), sm, s, cand, ncand, k,
-                     negate, pad_dist, D, I64, I32, keys, seg_off);
+                     negate, pad_dist, D, I64, I32, keys, seg_off, cap);
   MVG_LAUNCH_CHECK("merge_topk_kernel");
 }
 

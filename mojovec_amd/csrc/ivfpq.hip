@@ -19,12 +19,6 @@ namespace {
 
 constexpr int kScanChunk = 4096;  // list positions scored per select round
 
-__host__ __device__ inline int next_pow2(int v) {
-  int p = 1;
-  while (p < v) p <<= 1;
-  return p;
-}
-
 template <int MF>
 __device__ __forceinline__ float adc_sum(const uint8_t *__restrict__ code, const float *lut, int M,
                                          float init) {
@@ -324,7 +318,7 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
 // candidates (the integer SQ8 distance of index_flat_sq8.mojo:80-177 over
 // codes of the stored rows), then the kk2 best of them, sorted, as the
 // candidates of the exact FP32 rerank. Eight lanes per candidate row, 16-byte
-// code chunks, two rows per group in flight; keys are (integer distance << 32
+// code chunks, four rows per group in flight; keys are (integer distance << 32
 // | id), so equal code distances keep the lower id.
 template <bool IP>
 __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
@@ -338,7 +332,7 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
   SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(top + next_pow2(kk2));
   const int64_t qi = rerank_query(nq, order);
   if (qi < 0) return;
-  constexpr int G = kNT / 8, R = 2;
+  constexpr int G = kNT / 8, R = 4;
   const int tid = threadIdx.x, gl = tid & 7, group = tid >> 3;
   const uint4 *qa = reinterpret_cast<const uint4 *>(qcodes + qi * ld);
   const int words = ld / 16;

@@ -206,6 +206,13 @@ __device__ void bitonic_sort_u64(uint64_t *keys, int n) {
   }
 }
 
+// Smallest power of two >= v (v >= 1): the output size select_sort_u64 needs.
+__host__ __device__ inline int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
 // Writes the min(k, valid) smallest unique u64 keys of in[0..n) to out[] in
 // ascending order and sentinels to the rest of out[0..k); out must hold
 // next_pow2(k) entries and must not overlap in. Returns the count.
