@@ -161,6 +161,8 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
       const int64_t r = b0 + wave * 64 + t * 16 + lrow;
       rows[t] = r < b_end ? r : b_end - 1;
     }
+    // four k steps per iteration: their 16 row loads are issued together
+#pragma unroll 4
     for (int ks = 0; ks < nsteps; ks++) {
       const uint4 a = *reinterpret_cast<const uint4 *>(qtile + lrow * row_bytes + ks * 64 + lgrp * 16);
 #pragma unroll
@@ -763,8 +765,22 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
   const GemmLayout lay(row_bytes, kk);
   if (lay.total > 150 * 1024) throw Error(kInvalid, "GEMM top-k workspace exceeds LDS");
   const int qtiles = ceil_div(nq, kQT);
-  // Split the base so that the grid holds ~2048 workgroups, at least one step each.
+  // Splits of the base per query tile. Up to 64 candidates per query, a grid
+  // of ~2048 workgroups hides the row loads best. Above that every split
+  // fills its own top kk and pays its own flushes (about kk (1 + ln(split /
+  // kk)) appends per query), so take the fewest splits within 10% of the best
+  // balance over the CUs' resident workgroups (LDS-bound).
   int64_t S = std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, qtiles), ceil_div(n, kStepCols)));
+  if (kk > kWMaxK) {
+    const int cus = std::max(1, device_cus());
+    const int per_cu = std::max(1, std::min(8, (160 * 1024) / std::max(1, lay.total)));
+    const int64_t slots = (int64_t)cus * per_cu;
+    const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(256, ceil_div(n, kStepCols)));
+    double best = 1e30;
+    for (int64_t c = 1; c <= smax; c++) best = std::min(best, (double)ceil_div(qtiles * c, slots) / c);
+    S = 1;
+    while (S < smax && (double)ceil_div(qtiles * S, slots) / S > 1.1 * best) S++;
+  }
   int64_t split_len = ((ceil_div(n, S) + kStepCols - 1) / kStepCols) * kStepCols;
   S = ceil_div(n, split_len);
   nsplit = (int)S;
