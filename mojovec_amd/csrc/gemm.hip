@@ -62,7 +62,7 @@ __device__ void gemm_flush(uint64_t *top, uint64_t *tmp, int kk, int cnt, Select
                            uint64_t &thr_out) {
   const int tid = threadIdx.x;
   const int n = kk + cnt;
-  if (n <= 2 * kNT) {
+  if (n <= kNT / 2) {  // rank merge for small sets, radix select above
     for (int i = tid; i < kk; i += kNT) tmp[i] = kSentinel64;
     __syncthreads();
     for (int i = tid; i < n; i += kNT) {
