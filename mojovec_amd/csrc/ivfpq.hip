@@ -489,8 +489,9 @@ __device__ void flush_candidates(uint64_t *cand, uint64_t *tmp, int kk, QuerySca
   const int tid = threadIdx.x;
   __syncthreads();
   const int n = kk + st.cnt;
-  if (n <= 2 * kNT) {
-    // rank merge: keys are unique, sentinels sort last and are dropped
+  if (n <= kNT / 2) {
+    // rank merge (O(n^2), small sets): keys are unique, sentinels sort last
+    // and are dropped
     for (int i = tid; i < kk; i += kNT) tmp[i] = kSentinel64;
     __syncthreads();
     for (int i = tid; i < n; i += kNT) {
