@@ -72,6 +72,9 @@ __global__ void sq8_encode_kernel(const float *__restrict__ x, int64_t n, int d,
       sum += c;
     }
   }
+  // zero padding up to the stride: contributes nothing to uint8 or int8 dots
+  // (written here, not by a memset node, so a captured search replays it)
+  for (int j = d + lane; j < ld; j += 64) cr[j] = 0;
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) {
     norm += (uint32_t)__shfl_xor((int)norm, o, 64);
@@ -191,8 +194,6 @@ void sq8_encode(const float *x, int64_t n, int d, int symmetric, float minimum, 
                 uint8_t *codes, uint32_t *norms, uint32_t *sums, hipStream_t s) {
   if (n == 0) return;
   const int ld = sq8_row_stride(d);
-  if (ld != d)  // zero padding: contributes nothing to uint8 or int8 dots
-    MVG_HIP(hipMemset2DAsync(codes + d, ld, 0, ld - d, n, s));
   hipLaunchKernelGGL(sq8_encode_kernel, dim3(ceil_div(n, 4)), dim3(256), 0, s, x, n, d, ld,
                      symmetric, minimum, inv_scale, codes, norms, sums);
   MVG_LAUNCH_CHECK("sq8_encode_kernel");

@@ -64,6 +64,38 @@ def test_captured_search_graph_replays_match_eager(gpu, wq):
         assert np.array_equal(D.cpu().numpy(), De), f"replay {rep}"
 
 
+def test_captured_mixed_search_with_prefilter_replays_match_eager(gpu):
+    # mixed form with k * k_factor above the pre-rerank width: the captured
+    # graph holds the query SQ8 encode (d = 32 rows padded to 64 bytes), the
+    # pre-rerank and the FP32 rerank
+    import torch
+
+    rng = np.random.default_rng(17)
+    x = np.rint(np.clip(rng.normal(100, 30, (20000, 32)), 0, 255)).astype(np.float32)
+    ix = gpu.IndexIVFPQ(32, 64, 8)
+    ix.train(x, niter_coarse=4, niter_pq=4)
+    ix.add(x)
+    ix.set_search_mode("mixed")
+    nq, k, kf = 200, 10, 20
+    dev = torch.device("cuda:0")
+    q = torch.zeros((nq, 32), dtype=torch.float32, device=dev)
+    npq = torch.from_numpy(rng.integers(1, 40, nq).astype(np.int32)).to(dev)
+    D = torch.empty((nq, k), dtype=torch.float32, device=dev)
+    I = torch.empty((nq, k), dtype=torch.int64, device=dev)
+    g = ix.capture(q.data_ptr(), nq, k, D.data_ptr(), I.data_ptr(), k_factor=kf,
+                   nprobe_ptr=npq.data_ptr(), max_nprobe=40)
+    for rep in range(4):
+        qn = np.rint(np.clip(rng.normal(100, 30, (nq, 32)), 0, 255)).astype(np.float32)
+        q.copy_(torch.from_numpy(qn))
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        De, Ie = ix.search(qn, k, k_factor=kf, nprobe_per_query=npq.cpu().numpy())
+        assert np.array_equal(I.cpu().numpy(), Ie), f"replay {rep}"
+        assert np.array_equal(D.cpu().numpy(), De), f"replay {rep}"
+    ix.set_search_mode("plain")
+
+
 def test_server_mixed_precision_and_work_queue(gpu):
     rng = np.random.default_rng(6)
     x = rng.standard_normal((10000, 64)).astype(np.float32)
