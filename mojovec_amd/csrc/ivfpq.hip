@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
 // candidates (the integer SQ8 distance of index_flat_sq8.mojo:80-177 over
 // codes of the stored rows), then the kk2 best of them, sorted, as the
 // candidates of the exact FP32 rerank. Eight lanes per candidate row, 16-byte
-// code chunks, four rows per group in flight; keys are (integer distance << 32
+// code chunks, eight rows per group in flight; keys are (integer distance << 32
 // | id), so equal code distances keep the lower id.
 template <bool IP>
 __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
   SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(top + next_pow2(kk2));
   const int64_t qi = rerank_query(nq, order);
   if (qi < 0) return;
-  constexpr int G = kNT / 8, R = 4;
+  constexpr int G = kNT / 8, R = 8;
   const int tid = threadIdx.x, gl = tid & 7, group = tid >> 3;
   const uint4 *qa = reinterpret_cast<const uint4 *>(qcodes + qi * ld);
   const int words = ld / 16;
