@@ -471,9 +471,15 @@ def test_sq8_many_query_tiles_with_exclusion(gpu, metric):
     ox = o.SQ8Index(48, metric)
     gx.add(x)
     ox.add(x)
-    for k, kf in [(1, 1), (10, 1), (16, 4), (64, 1), (10, 6)]:
+    # (100, 2), (50, 3): above 64 candidates the 16-query form, its split
+    # count from the CU balance and flushes by radix select
+    for k, kf in [(1, 1), (10, 1), (16, 4), (64, 1), (10, 6), (100, 2), (50, 3)]:
         Dg, Ig = gx.search(q, k, k_factor=kf, exclude=ex)
         Do, Io = ox.search(q, k, k_factor=kf, exclude=ex)
+        assert_same(Dg, Ig, Do, Io)
+    for nq in (1, 17):  # one tile: the most splits
+        Dg, Ig = gx.search(q[:nq], 100, k_factor=2, exclude=ex)
+        Do, Io = ox.search(q[:nq], 100, k_factor=2, exclude=ex)
         assert_same(Dg, Ig, Do, Io)
 
 
