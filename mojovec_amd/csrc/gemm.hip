@@ -37,18 +37,30 @@ typedef long longx1;
 
 constexpr int kQT = 16;        // queries per workgroup
 constexpr int kStepCols = 256;  // base rows per step (4 waves x 4 tiles x 16)
-constexpr int kGCap = 384;      // per-query append capacity
+// Per-query append capacity: 384 keys; 768 above 64 candidates when that
+// keeps as many workgroups per CU (LDS) and fits in 150 KB, since fewer
+// flushes of a larger buffer cost less there (each is a radix select, O(n)).
+constexpr int kGCap = 384, kGCapLarge = 768;
 
 enum GemmKind { BF16_L2 = 0, BF16_IP = 1, I8_IP = 2, U8_L2 = 3, FP8_L2 = 4, FP8_IP = 5 };
 
 struct GemmLayout {
-  int qtile, top, tmp, cnt, thr, scr, total;
+  int qtile, top, tmp, cnt, thr, scr, total, cap;
   __host__ __device__ GemmLayout(int row_bytes, int kk) {
+    place(row_bytes, kk, kGCap);
+    if (kk > 64) {
+      const int small = total;
+      place(row_bytes, kk, kGCapLarge);
+      if (total > 150 * 1024 || (160 * 1024) / total < (160 * 1024) / small) place(row_bytes, kk, kGCap);
+    }
+  }
+  __host__ __device__ void place(int row_bytes, int kk, int c) {
     int np2 = 1;
     while (np2 < kk) np2 <<= 1;
+    cap = c;
     qtile = 0;
     top = (kQT * row_bytes + 15) & ~15;
-    tmp = top + 8 * kQT * (kk + kGCap);
+    tmp = top + 8 * kQT * (kk + cap);
     cnt = tmp + 8 * np2;
     thr = cnt + 4 * kQT;
     thr = (thr + 7) & ~7;
@@ -113,12 +125,12 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
   extern __shared__ __align__(16) unsigned char smem[];
   const GemmLayout lay(row_bytes, kk);
   uint8_t *qtile = smem + lay.qtile;
-  uint64_t *top = reinterpret_cast<uint64_t *>(smem + lay.top);  // [kQT][kk + kGCap]
+  uint64_t *top = reinterpret_cast<uint64_t *>(smem + lay.top);  // [kQT][kk + lay.cap]
   uint64_t *tmp = reinterpret_cast<uint64_t *>(smem + lay.tmp);
   int *cnt = reinterpret_cast<int *>(smem + lay.cnt);
   uint64_t *thr = reinterpret_cast<uint64_t *>(smem + lay.thr);
   SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(smem + lay.scr);
-  const int stride = kk + kGCap;
+  const int stride = kk + lay.cap;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int64_t qt;
@@ -222,7 +234,7 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
     }
     __syncthreads();
     bool need = false;
-    for (int qi = 0; qi < kQT; qi++) need |= cnt[qi] > kGCap - kStepCols;
+    for (int qi = 0; qi < kQT; qi++) need |= cnt[qi] > lay.cap - kStepCols;
     if (need || b0 + kStepCols >= b_end) {
       for (int qi = 0; qi < qcount; qi++) {
         const int c = cnt[qi];
