@@ -75,6 +75,20 @@ def test_index_validation_messages():
         mojovec_amd.IndexIVFFlat(8, 4, 7)
 
 
+def test_null_handles_are_rejected_before_any_device_work():
+    import ctypes
+
+    import mojovec_amd
+
+    lib = mojovec_amd._lib.load()
+    out = ctypes.c_void_p()
+    for create in (lib.mvg_server_create, lib.mvg_server_create_sq8):
+        assert create(None, 2, 64, 1 << 20, ctypes.byref(out)) != 0
+        assert b"server needs an index" in lib.mvg_last_error()
+    assert lib.mvg_ivfpq_set_prefilter(None, 8) != 0
+    assert lib.mvg_last_error()
+
+
 def test_no_oracle_in_product_package():
     pkg = os.path.join(ROOT, "mojovec_amd")
     for dirpath, _, files in os.walk(pkg):
