@@ -439,6 +439,151 @@ def test_ivfpq_rerank_with_every_candidate_is_exact_search():
     assert np.array_equal(I, If) and np.array_equal(D, Df)
 
 
+def _integer_ivfpq(seed, d, nlist, M, metric, n=1500):
+    """An IVF-PQ index whose codebooks are small integers, so that on integer
+    rows and queries every FP32 sum of the search is exact: the oracle must
+    then equal float64 arithmetic on the reconstructed rows bit for bit."""
+    rng = np.random.default_rng(seed)
+    coarse = rint(rng, (nlist, d), -6, 7)
+    pq = rint(rng, (M, 256, d // M), -3, 4)
+    ix = o.IVFPQIndex(d, nlist, M, metric)
+    ix.set_codebooks(coarse, pq)
+    x = rint(rng, (n, d), -8, 9)
+    ix.add(x)
+    owner = np.full(n, -1)
+    recon = np.zeros((n, d), np.float64)
+    for l in range(nlist):
+        ids, codes = ix.list(l)
+        owner[ids] = l
+        recon[ids] = coarse[l].astype(np.float64) + o.pq_decode(codes, pq)
+    assert (owner >= 0).all()
+    return ix, rng, x, coarse, owner, recon
+
+
+@pytest.mark.parametrize("M", [4, 2])  # dsub 4 (short form) and 8 (W = 4 LUT)
+def test_ivfpq_inner_product_is_minus_dot_with_reconstruction(M):
+    # index_ivf_pq.mojo:288-293,316-324: an inner-product search scores an
+    # entry of list l as -<q, c_l> + sum_m LUT_IP(q)[m][code_m], i.e.
+    # -<q, c_l + decode(code)>, and returns it un-negated (:357-360). Every
+    # sum is exact here, so a dropped coarse term, a wrong sign or a
+    # transposed LUT index changes the result; exact ties order by id.
+    d, nlist = 16, 8
+    ix, rng, x, coarse, owner, recon = _integer_ivfpq(31 + M, d, nlist, M, o.IP)
+    q = rint(rng, (15, d), -8, 9)
+    q64 = q.astype(np.float64)
+    ids_all = np.arange(x.shape[0], dtype=np.int64)
+    k = 40
+    D, I = ix.search(q, k, nlist)  # every list
+    for i in range(q.shape[0]):
+        ed, ei = _canonical_topk_f64(-(recon @ q64[i]), ids_all, k)
+        assert np.array_equal(I[i], ei)
+        assert np.array_equal(D[i].astype(np.float64), -ed)
+    nprobe = 3
+    D, I = ix.search(q, k, nprobe)
+    for i in range(q.shape[0]):
+        probed = np.lexsort((np.arange(nlist), -(coarse.astype(np.float64) @ q64[i])))[:nprobe]
+        rows = np.flatnonzero(np.isin(owner, probed)).astype(np.int64)
+        ed, ei = _canonical_topk_f64(-(recon[rows] @ q64[i]), rows, k)
+        assert np.array_equal(I[i], ei)
+        assert np.array_equal(D[i].astype(np.float64), -ed)
+
+
+@pytest.mark.parametrize("metric", [o.L2, o.IP])
+def test_ivfpq_rerank_reorders_exactly_the_adc_candidates(metric):
+    # k_factor: the k * k_factor ADC candidates of the search, re-scored with
+    # exact distances to the stored rows (index_hnsw.mojo:449-482), best k by
+    # (distance, id). Integer data: the exact distances are float64's.
+    d, nlist, M = 16, 8, 4
+    ix, rng, x, coarse, owner, recon = _integer_ivfpq(41, d, nlist, M, metric)
+    q = rint(rng, (12, d), -8, 9)
+    x64, q64 = x.astype(np.float64), q.astype(np.float64)
+    k, kf, nprobe = 7, 6, 4
+    _, cand = ix.search(q, k * kf, nprobe)
+    D, I = ix.search(q, k, nprobe, k_factor=kf)
+    for i in range(q.shape[0]):
+        c = cand[i][cand[i] >= 0]
+        ex = ((x64[c] - q64[i]) ** 2).sum(-1) if metric == o.L2 else -(x64[c] @ q64[i])
+        ed, ei = _canonical_topk_f64(ex, c, k)
+        assert np.array_equal(I[i], ei)
+        assert np.array_equal(D[i].astype(np.float64), ed if metric == o.L2 else -ed)
+    # every stored row as a candidate: the exact search over the probed lists
+    D, I = ix.search(q, k, nlist, k_factor=x.shape[0])
+    Df, If = o.flat_search(x, q, k, metric)
+    assert np.array_equal(I, If) and np.array_equal(D, Df)
+
+
+def _ip_w_order(x, y, W):
+    """inner_product_simd[W]'s documented order, as _l2_w_order for L2."""
+    x = x.astype(np.float32)
+    y = y.astype(np.float32)
+    d = x.size
+    acc = [np.zeros(W, np.float32) for _ in range(4)]
+    i = 0
+    while i <= d - 4 * W:
+        for t in range(4):
+            p = x[i + t * W:i + (t + 1) * W].astype(np.float64) * y[i + t * W:i + (t + 1) * W]
+            acc[t] = (p + acc[t].astype(np.float64)).astype(np.float32)
+        i += 4 * W
+    v = ((acc[0] + acc[1]) + acc[2]) + acc[3]
+    while i <= d - W:
+        p = x[i:i + W].astype(np.float64) * y[i:i + W]
+        v = (p + v.astype(np.float64)).astype(np.float32)
+        i += W
+    assert i == d
+    return _tree(v)
+
+
+@pytest.mark.parametrize("metric", [o.L2, o.IP])
+def test_sq8_rerank_uses_w64_exact_distances_of_the_code_candidates(metric):
+    # index_hnsw.mojo:433-496: the SQ8 candidates are re-scored with
+    # l2_distance_simd[64] / -inner_product_simd[64] against the FP32 rows and
+    # the best k kept by (distance, id). The W = 64 distances come from the
+    # independent numpy statement of that order (float rows, d = 128: two
+    # 64-wide blocks and the halving tree).
+    rng = np.random.default_rng(51 + metric)
+    d, n = 128, 300
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    q = rng.standard_normal((6, d)).astype(np.float32)
+    ix = o.SQ8Index(d, metric)
+    ix.add(x)
+
+    def exact(qi, rows):
+        if metric == o.L2:
+            return np.array([_l2_w_order(q[qi], x[r], 64) for r in rows], np.float64)
+        return -np.array([_ip_w_order(q[qi], x[r], 64) for r in rows], np.float64)
+
+    k, kf = 5, 4
+    _, cand = ix.search(q, k * kf)
+    D, I = ix.search(q, k, k_factor=kf)
+    for i in range(q.shape[0]):
+        ed, ei = _canonical_topk_f64(exact(i, cand[i]), cand[i], k)
+        assert np.array_equal(I[i], ei)
+        assert np.array_equal(D[i].astype(np.float64), ed if metric == o.L2 else -ed)
+    # k * k_factor covering every row: the W = 64 exact search
+    D, I = ix.search(q[:3], k, k_factor=n)
+    for i in range(3):
+        ed, ei = _canonical_topk_f64(exact(i, np.arange(n)), np.arange(n, dtype=np.int64), k)
+        assert np.array_equal(I[i], ei)
+        assert np.array_equal(D[i].astype(np.float64), ed if metric == o.L2 else -ed)
+
+
+@pytest.mark.parametrize("metric", [o.L2, o.IP])
+def test_sq8_full_rerank_on_integer_rows_is_flat_search(metric):
+    # Integer rows with the full code range (scale 1: codes are the rows):
+    # a rerank over every row equals the exact Flat search, ids and distances.
+    rng = np.random.default_rng(61 + metric)
+    lo, hi = (0, 256) if metric == o.L2 else (-127, 128)
+    x = rint(rng, (400, 24), lo, hi)
+    x[0, 0], x[1, 0] = lo, hi - 1
+    q = rint(rng, (9, 24), lo, hi)
+    ix = o.SQ8Index(24, metric)
+    ix.add(x)
+    assert ix.params()[2] == 1.0
+    D, I = ix.search(q, 12, k_factor=400)
+    Df, If = o.flat_search(x, q, 12, metric)
+    assert np.array_equal(I, If) and np.array_equal(D, Df)
+
+
 # ---------------------------------------------------------------- IVF-Flat
 def _canonical_topk_f64(d2, ids, k):
     """k smallest (distance, id) pairs in lexicographic order, from float64."""
