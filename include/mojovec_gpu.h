@@ -89,6 +89,11 @@ int mvg_flat_distances(mvg_flat *h, const float *q, int64_t nq, const int64_t *i
 int mvg_flat_pair_distances(mvg_flat *h, const int64_t *a, const int64_t *b, int64_t npairs,
                             float *out);
 int64_t mvg_flat_ntotal(const mvg_flat *h);
+/* IndexFlat.get_vector / get_vector_span (index_flat.mojo:212-227): the
+ * stored row `id` (d floats) copied to out; ids must lie in [0, ntotal).
+ * get_vectors gathers rows ids[0..n) into out [n x d]. */
+int mvg_flat_get_vector(mvg_flat *h, int64_t id, float *out);
+int mvg_flat_get_vectors(mvg_flat *h, const int64_t *ids, int64_t n, float *out);
 void mvg_flat_destroy(mvg_flat *h);
 
 /* ---- SQ8 Flat: IndexFlatSQ8 (mojovec/index/index_flat_sq8.mojo:192-582) */
@@ -116,6 +121,11 @@ int mvg_sq8_pair_distances(mvg_sq8 *h, const int64_t *a, const int64_t *b, int64
 int mvg_sq8_params(const mvg_sq8 *h, float *gmin, float *gmax, float *scale, int64_t *ntotal);
 /* Codes [ntotal x d] and code norms [ntotal] (codes_u8 / norms_u32). */
 int mvg_sq8_get_codes(mvg_sq8 *h, uint8_t *codes, uint32_t *norms);
+/* IndexFlatSQ8.get_vector / get_vector_span (index_flat_sq8.mojo:567-582):
+ * the retained FP32 row `id` (codes_f32); get_vectors gathers rows
+ * ids[0..n) into out [n x d]. ids must lie in [0, ntotal). */
+int mvg_sq8_get_vector(mvg_sq8 *h, int64_t id, float *out);
+int mvg_sq8_get_vectors(mvg_sq8 *h, const int64_t *ids, int64_t n, float *out);
 /* Code-dot engine (no reference counterpart): 1 (default) contracts the
  * codes on int8 MFMA (affine uint8 rebased to int8 with an exact correction),
  * 0 uses dot4 VALU instructions. Both give bit-identical results. */
@@ -137,6 +147,10 @@ int mvg_pq_train(const float *x, int64_t n, int d, int M, int niter, uint64_t se
 /* ProductQuantizer.compute_codes (:178-219): codes [n x M]. */
 int mvg_pq_encode(const float *x, int64_t n, int d, int M, const float *codebooks,
                   uint8_t *codes);
+/* ProductQuantizer.decode (:221-250): codes [n x M] -> rows [n x d], each
+ * subvector the codebook entry its code names. */
+int mvg_pq_decode(const uint8_t *codes, int64_t n, int d, int M, const float *codebooks,
+                  float *out);
 /* ProductQuantizer.compute_distance_table (:252-292) for nq vectors:
  * lut [nq x M x 256]. */
 int mvg_pq_lut(const float *q, int64_t nq, int d, int M, const float *codebooks, int metric,
@@ -158,6 +172,13 @@ int mvg_ivfpq_get_codebooks(mvg_ivfpq *h, float *coarse, float *pq);
  * k_factor rerank unless disabled with mvg_ivfpq_set_keep_vectors(h, 0)
  * before the first add. */
 int mvg_ivfpq_add(mvg_ivfpq *h, const float *x, int64_t n);
+/* IndexIVFPQ.add_with_ids (:150-213): as add, with the caller's ids stored in
+ * the lists (ids NULL = add). Ids must lie in [0, 2^31 - 1] (the device id
+ * width); ntotal grows by n and the FP32 rows stay in add order. While rows
+ * are kept for the rerank the ids must be unique across the index (the
+ * rerank finds a row by its id); a duplicate is refused before anything
+ * changes. Searches, the exclusion mask and ties use these ids. */
+int mvg_ivfpq_add_with_ids(mvg_ivfpq *h, const float *x, int64_t n, const int64_t *ids);
 int mvg_ivfpq_set_keep_vectors(mvg_ivfpq *h, int keep);
 /* The public `nprobe` field (:22). Values above nlist are clamped at search
  * time (:253-254). */
@@ -175,6 +196,16 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
                             const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
                             const uint8_t *exclude, int64_t exclude_len, float *distances,
                             int64_t *labels, void *stream);
+/* Batched search with a k per query (no reference counterpart; the
+ * reference's search takes one k per call, :215-224): query i returns
+ * k_per_query[i] results (device [nq], values in 1..max_k) in output rows of
+ * stride max_k, the slots past its own k padded (1e38, -1). Each query's
+ * results equal those of this search with k = k_per_query[i] alone; the
+ * batch runs as one scan for all queries. */
+int mvg_ivfpq_search_device_k(mvg_ivfpq *h, const float *q, int64_t nq, const int32_t *k_per_query,
+                              int max_k, const int32_t *nprobe_per_query, int max_nprobe,
+                              int k_factor, const uint8_t *exclude, int64_t exclude_len,
+                              float *distances, int64_t *labels, void *stream);
 /* Search form (no reference counterpart). precision 0: plain FP32, the
  * reference's residual-LUT arithmetic in its operation order (bit-exact ids);
  * 1: mixed, L2 tables decomposed as |q - c_l|^2 + T[l] + Q[q] (precomputed
@@ -278,12 +309,18 @@ int mvg_ivfflat_load(const char *path, mvg_ivfflat **out);
  * merge, rerank) of a fixed batch shape into a HIP graph with its own
  * workspace; mvg_graph_launch replays it on `stream` (a hipStream_t; NULL =
  * the null stream) against the same query/output buffers, whose contents may
- * change between replays. The index must not be mutated while a graph of it
- * exists. */
+ * change between replays. A graph replays only while its index is unchanged:
+ * after an add, train or codebook change mvg_graph_launch returns
+ * MVG_ERR_STATE (capture again). Destroy graphs before their index. */
 typedef struct mvg_graph mvg_graph;
 int mvg_ivfpq_capture(mvg_ivfpq *h, const float *q, int64_t nq, int k,
                       const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
                       float *distances, int64_t *labels, mvg_graph **out);
+/* The same with a k per query (device k_per_query [nq], output stride max_k),
+ * as mvg_ivfpq_search_device_k: one graph for a batch of mixed k. */
+int mvg_ivfpq_capture_k(mvg_ivfpq *h, const float *q, int64_t nq, const int32_t *k_per_query,
+                        int max_k, const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                        float *distances, int64_t *labels, mvg_graph **out);
 int mvg_graph_launch(mvg_graph *g, void *stream);
 /* Times the scan schedules (one workgroup per query; work queue with 1024,
  * 4096 or 16384 entries per item; one workgroup per (query, list)) on this
@@ -296,11 +333,15 @@ void mvg_graph_destroy(mvg_graph *g);
 
 /* ---- continuous-batching request loop (no reference counterpart) ------- */
 /* For serving callers of an HBM-resident IVF-PQ or Flat SQ8 index. A dispatcher thread
- * packs waiting requests with equal (k, k_factor) into batches of at most
- * max_batch_queries queries, admits a batch when one of `slots` (1..16)
- * streams is free and its device-memory estimate fits mem_budget_bytes, and
- * overlaps the batches on the device. Each request's result equals the same
- * request searched alone. The index must not be mutated while served. */
+ * packs waiting requests with equal k_factor into batches of at most
+ * max_batch_queries queries (requests of different k share a batch, each
+ * query with its own k and nprobe; over a Flat SQ8 index only when their
+ * k * k_factor fall on the same side of 64, where its MFMA form changes),
+ * admits a batch when one of `slots` (1..16) streams is free and its
+ * device-memory estimate fits mem_budget_bytes, and overlaps the batches on
+ * the device. Each request's result equals the same request searched alone.
+ * While a server holds an IVF-PQ index, its mutations and search settings
+ * return MVG_ERR_STATE. */
 typedef struct mvg_server mvg_server;
 int mvg_server_create(mvg_ivfpq *index, int slots, int max_batch_queries, int64_t mem_budget_bytes,
                       mvg_server **out);

@@ -46,6 +46,8 @@ SIGNATURES = {
     "mvg_flat_pair_distances": (_i, [_p, _p, _p, _i64, _p]),
     "mvg_fp8_encode": (_i, [_p, _i64, _i, _p, _p, _p]),
     "mvg_flat_ntotal": (_i64, [_p]),
+    "mvg_flat_get_vector": (_i, [_p, _i64, _p]),
+    "mvg_flat_get_vectors": (_i, [_p, _p, _i64, _p]),
     "mvg_flat_destroy": (None, [_p]),
     "mvg_sq8_create": (_i, [_i, _i, ctypes.POINTER(_p)]),
     "mvg_sq8_add": (_i, [_p, _p, _i64]),
@@ -55,21 +57,26 @@ SIGNATURES = {
     "mvg_sq8_params": (_i, [_p, ctypes.POINTER(_f), ctypes.POINTER(_f), ctypes.POINTER(_f),
                             ctypes.POINTER(_i64)]),
     "mvg_sq8_get_codes": (_i, [_p, _p, _p]),
+    "mvg_sq8_get_vector": (_i, [_p, _i64, _p]),
+    "mvg_sq8_get_vectors": (_i, [_p, _p, _i64, _p]),
     "mvg_sq8_set_search_mode": (_i, [_p, _i]),
     "mvg_sq8_destroy": (None, [_p]),
     "mvg_kmeans_train": (_i, [_p, _i64, _i, _i, _i, _p, _i64, _u64, _p, _p]),
     "mvg_pq_train": (_i, [_p, _i64, _i, _i, _i, _u64, _p]),
     "mvg_pq_encode": (_i, [_p, _i64, _i, _i, _p, _p]),
     "mvg_pq_lut": (_i, [_p, _i64, _i, _i, _p, _i, _p]),
+    "mvg_pq_decode": (_i, [_p, _i64, _i, _i, _p, _p]),
     "mvg_ivfpq_create": (_i, [_i, _i, _i, _i, ctypes.POINTER(_p)]),
     "mvg_ivfpq_train": (_i, [_p, _p, _i64, _i, _i, _u64]),
     "mvg_ivfpq_set_codebooks": (_i, [_p, _p, _p]),
     "mvg_ivfpq_get_codebooks": (_i, [_p, _p, _p]),
     "mvg_ivfpq_add": (_i, [_p, _p, _i64]),
+    "mvg_ivfpq_add_with_ids": (_i, [_p, _p, _i64, _p]),
     "mvg_ivfpq_set_keep_vectors": (_i, [_p, _i]),
     "mvg_ivfpq_set_nprobe": (_i, [_p, _i]),
     "mvg_ivfpq_search": (_i, [_p, _p, _i64, _i, _p, _i, _p, _i64, _p, _p]),
     "mvg_ivfpq_search_device": (_i, [_p, _p, _i64, _i, _p, _i, _i, _p, _i64, _p, _p, _p]),
+    "mvg_ivfpq_search_device_k": (_i, [_p, _p, _i64, _p, _i, _p, _i, _i, _p, _i64, _p, _p, _p]),
     "mvg_ivfpq_set_search_mode": (_i, [_p, _i, _i]),
     "mvg_ivfpq_set_prefilter": (_i, [_p, _i]),
     "mvg_ivfpq_set_work_queue": (_i, [_p, _i]),
@@ -83,6 +90,7 @@ SIGNATURES = {
     "mvg_ivfpq_get_list": (_i, [_p, _i, _p, _p]),
     "mvg_ivfpq_destroy": (None, [_p]),
     "mvg_ivfpq_capture": (_i, [_p, _p, _i64, _i, _p, _i, _i, _p, _p, ctypes.POINTER(_p)]),
+    "mvg_ivfpq_capture_k": (_i, [_p, _p, _i64, _p, _i, _p, _i, _i, _p, _p, ctypes.POINTER(_p)]),
     "mvg_graph_launch": (_i, [_p, _p]),
     "mvg_ivfpq_autotune": (_i, [_p, _p, _i64, _i, _p, _i, _i, _p, _p, ctypes.POINTER(_i),
                                 ctypes.POINTER(_f)]),

@@ -159,6 +159,7 @@ void ivf_sync_lists(mvg_ivfpq *h) {
   h->top_sum.assign(h->nlist + 1, 0);
   for (int l = 0; l < h->nlist; l++) h->top_sum[l + 1] = h->top_sum[l] + sizes[l];
   upload(h->list_off, off.data(), off.size(), h->s);
+  h->ncsr = total;
   // +16 bytes so 16-byte code loads of the last entry stay inside the buffer.
   h->codes.reserve(codes.size() + 16);
   MVG_HIP(hipMemcpyAsync(h->codes.ptr, codes.data(), codes.size(), hipMemcpyHostToDevice, h->s));
@@ -342,6 +343,38 @@ int mvg_flat_pair_distances(mvg_flat *h, const int64_t *a, const int64_t *b, int
 
 int64_t mvg_flat_ntotal(const mvg_flat *h) { return h ? h->ntotal : 0; }
 
+}  // extern "C"
+
+namespace {
+
+// Stored FP32 rows ids[0..n) of a Flat or SQ8 handle into out [n][d].
+void read_rows(const float *base, int d, int64_t ntotal, const int64_t *ids, int64_t n, float *out,
+               DevBuf &dids, DevBuf &dout, hipStream_t s) {
+  require(n >= 0 && (n == 0 || (ids && out)), "invalid arguments");
+  check_ids(ids, n, ntotal, "ids");
+  if (n == 0) return;
+  const int64_t *di = upload(dids, ids, (size_t)n, s);
+  dout.reserve(sizeof(float) * (size_t)n * d);
+  mvg::gather_rows(base, d, di, n, dout.as<float>(), s);
+  download(out, dout.ptr, (size_t)n * d, s);
+  MVG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int mvg_flat_get_vectors(mvg_flat *h, const int64_t *ids, int64_t n, float *out) {
+  API_BEGIN
+  require(h, "invalid handle");
+  read_rows(h->base.as<float>(), h->d, h->ntotal, ids, n, out, h->pair_a, h->pair_out, h->s);
+  API_END
+}
+
+int mvg_flat_get_vector(mvg_flat *h, int64_t id, float *out) {
+  return mvg_flat_get_vectors(h, &id, 1, out);
+}
+
 void mvg_flat_destroy(mvg_flat *h) {
   if (!h) return;
   if (h->s) (void)hipStreamDestroy(h->s);
@@ -493,6 +526,17 @@ int mvg_sq8_params(const mvg_sq8 *h, float *gmin, float *gmax, float *scale, int
   API_END
 }
 
+int mvg_sq8_get_vectors(mvg_sq8 *h, const int64_t *ids, int64_t n, float *out) {
+  API_BEGIN
+  require(h, "invalid handle");
+  read_rows(h->f32.as<float>(), h->d, h->ntotal, ids, n, out, h->pair_a, h->pair_out, h->s);
+  API_END
+}
+
+int mvg_sq8_get_vector(mvg_sq8 *h, int64_t id, float *out) {
+  return mvg_sq8_get_vectors(h, &id, 1, out);
+}
+
 int mvg_sq8_get_codes(mvg_sq8 *h, uint8_t *codes, uint32_t *norms) {
   API_BEGIN
   require(h, "invalid handle");
@@ -599,6 +643,23 @@ int mvg_pq_encode(const float *x, int64_t n, int d, int M, const float *codebook
   API_END
 }
 
+int mvg_pq_decode(const uint8_t *codes, int64_t n, int d, int M, const float *codebooks, float *out) {
+  API_BEGIN
+  require(M > 0 && M <= d && d % M == 0, "PQ M must divide dimension and be between 1 and dimension.");
+  require(n >= 0 && (n == 0 || (codes && codebooks && out)), "invalid arguments");
+  if (n == 0) return MVG_OK;
+  hipStream_t s = new_stream();
+  struct Guard { hipStream_t s; ~Guard() { (void)hipStreamDestroy(s); } } g{s};
+  DevBuf bc, bcb, bo;
+  const uint8_t *dc = upload(bc, codes, (size_t)n * M, s);
+  const float *dcb = upload(bcb, codebooks, (size_t)d * mvg::kKsub, s);
+  bo.reserve(sizeof(float) * (size_t)n * d);
+  mvg::pq_decode(dc, n, d, M, dcb, bo.as<float>(), s);
+  download(out, bo.ptr, (size_t)n * d, s);
+  MVG_HIP(hipStreamSynchronize(s));
+  API_END
+}
+
 int mvg_pq_lut(const float *q, int64_t nq, int d, int M, const float *codebooks, int metric,
                float *lut) {
   API_BEGIN
@@ -642,6 +703,7 @@ int mvg_ivfpq_create(int d, int nlist, int M, int metric, mvg_ivfpq **out) {
 int mvg_ivfpq_set_keep_vectors(mvg_ivfpq *h, int keep) {
   API_BEGIN
   require(h, "invalid handle");
+  mvg::ivf_check_unserved(h);
   if (h->ntotal > 0 && (keep != 0) != h->keep_vectors)
     throw Error(mvg::kState, "keep_vectors can only change before the first add");
   h->keep_vectors = keep != 0;
@@ -652,6 +714,7 @@ int mvg_ivfpq_train(mvg_ivfpq *h, const float *x, int64_t n, int niter_coarse, i
                     uint64_t seed) {
   API_BEGIN
   require(h && n >= 0, "invalid arguments");
+  mvg::ivf_check_unserved(h);
   if (h->trained || n == 0) return MVG_OK;  // index_ivf_pq.mojo:93
   const int d = h->d;
   float *dx = upload(h->tx, x, (size_t)n * d, h->s);
@@ -672,12 +735,14 @@ int mvg_ivfpq_train(mvg_ivfpq *h, const float *x, int64_t n, int niter_coarse, i
   MVG_HIP(hipStreamSynchronize(h->s));
   h->trained = true;
   h->tables_dirty = true;
+  h->generation++;
   API_END
 }
 
 int mvg_ivfpq_set_codebooks(mvg_ivfpq *h, const float *coarse, const float *pq) {
   API_BEGIN
   require(h && coarse && pq, "invalid arguments");
+  mvg::ivf_check_unserved(h);
   MVG_HIP(hipMemcpyAsync(h->coarse.ptr, coarse, sizeof(float) * (size_t)h->nlist * h->d,
                          hipMemcpyHostToDevice, h->s));
   MVG_HIP(hipMemcpyAsync(h->pq.ptr, pq, sizeof(float) * (size_t)h->d * mvg::kKsub,
@@ -685,6 +750,7 @@ int mvg_ivfpq_set_codebooks(mvg_ivfpq *h, const float *coarse, const float *pq) 
   MVG_HIP(hipStreamSynchronize(h->s));
   h->trained = true;
   h->tables_dirty = true;
+  h->generation++;
   API_END
 }
 
@@ -698,10 +764,43 @@ int mvg_ivfpq_get_codebooks(mvg_ivfpq *h, float *coarse, float *pq) {
 }
 
 int mvg_ivfpq_add(mvg_ivfpq *h, const float *x, int64_t n) {
+  return mvg_ivfpq_add_with_ids(h, x, n, nullptr);
+}
+
+int mvg_ivfpq_add_with_ids(mvg_ivfpq *h, const float *x, int64_t n, const int64_t *ids) {
   API_BEGIN
   require(h && n >= 0, "invalid arguments");
+  mvg::ivf_check_unserved(h);
   if (!h->trained || n == 0) return MVG_OK;  // index_ivf_pq.mojo:161
   if (h->ntotal + n > INT32_MAX) throw Error(mvg::kInvalid, "IVF-PQ holds at most 2^31-1 vectors");
+  bool explicit_rows = false;  // some id differs from its row number
+  if (ids) {
+    for (int64_t i = 0; i < n; i++) {
+      require(ids[i] >= 0 && ids[i] <= INT32_MAX, "IVF-PQ ids must be in [0, 2^31 - 1]");
+      explicit_rows |= ids[i] != h->ntotal + i;
+    }
+  }
+  // FP32 rows stay in add order; once an id differs from its row the rerank
+  // needs id -> row, which must then be a function: duplicate ids are refused
+  // while rows are kept (checked before anything changes).
+  const bool map_rows = h->keep_vectors && (h->custom_ids || explicit_rows);
+  std::vector<int32_t> next_map;
+  if (map_rows) {
+    next_map = h->row_of_host;
+    if (!h->custom_ids) {  // every earlier row was added under its own number
+      next_map.resize((size_t)h->ntotal);
+      for (int64_t r = 0; r < h->ntotal; r++) next_map[(size_t)r] = (int32_t)r;
+    }
+    for (int64_t i = 0; i < n; i++) {
+      const int64_t id = ids ? ids[i] : h->ntotal + i;
+      if ((int64_t)next_map.size() <= id) next_map.resize((size_t)id + 1, -1);
+      if (next_map[(size_t)id] >= 0)
+        throw Error(mvg::kInvalid,
+                    "duplicate IVF-PQ id " + std::to_string(id) +
+                        ": ids must be unique while FP32 rows are kept for the rerank");
+      next_map[(size_t)id] = (int32_t)(h->ntotal + i);
+    }
+  }
   const int d = h->d, M = h->M;
   const float *dx = upload(h->tx, x, (size_t)n * d, h->s);
   h->tlab.reserve(sizeof(int32_t) * n);
@@ -725,12 +824,18 @@ int mvg_ivfpq_add(mvg_ivfpq *h, const float *x, int64_t n) {
   for (int64_t i = 0; i < n; i++) {  // :199-211, append order
     const int l = lab[i];
     if (l < 0 || l >= h->nlist) continue;
-    h->lids[l].push_back((int32_t)(h->ntotal + i));
+    h->lids[l].push_back((int32_t)(ids ? ids[i] : h->ntotal + i));
     h->lcodes[l].insert(h->lcodes[l].end(), codes.begin() + i * M, codes.begin() + (i + 1) * M);
+  }
+  if (map_rows) {
+    h->row_of_host.swap(next_map);
+    h->custom_ids = true;
+    h->row_of_dirty = true;
   }
   h->ntotal += n;
   h->dirty = true;
   h->vq_dirty = true;
+  h->generation++;
   API_END
 }
 
@@ -745,14 +850,22 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
                             const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
                             const uint8_t *exclude, int64_t exclude_len, float *distances,
                             int64_t *labels, void *stream) {
+  return mvg_ivfpq_search_device_k(h, q, nq, nullptr, k, nprobe_per_query, max_nprobe, k_factor,
+                                   exclude, exclude_len, distances, labels, stream);
+}
+
+int mvg_ivfpq_search_device_k(mvg_ivfpq *h, const float *q, int64_t nq, const int32_t *k_per_query,
+                              int max_k, const int32_t *nprobe_per_query, int max_nprobe,
+                              int k_factor, const uint8_t *exclude, int64_t exclude_len,
+                              float *distances, int64_t *labels, void *stream) {
   API_BEGIN
-  require(h && nq >= 0 && k >= 1 && k_factor >= 1, "invalid arguments");
+  require(h && nq >= 0 && max_k >= 1 && k_factor >= 1, "invalid arguments");
   require(h->trained && h->ntotal > 0, "search_device needs a trained, non-empty index");
   hipStream_t s = stream ? (hipStream_t)stream : h->s;
   std::lock_guard<std::mutex> lk(h->search_mu);
   mvg::ivf_prepare(h);
-  mvg::ivf_search_with(h, q, nq, k, nprobe_per_query, max_nprobe, k_factor, exclude, exclude_len,
-                       distances, labels, h->work, s);
+  mvg::ivf_search_with(h, q, nq, max_k, nprobe_per_query, max_nprobe, k_factor, exclude, exclude_len,
+                       distances, labels, h->work, s, k_per_query);
   API_END
 }
 
@@ -760,6 +873,11 @@ int mvg_ivfpq_search_device(mvg_ivfpq *h, const float *q, int64_t nq, int k,
 
 void mvg::ivf_prepare(mvg_ivfpq *h) {
   ivf_sync_lists(h);
+  if (h->row_of_dirty) {
+    upload(h->row_of, h->row_of_host.data(), h->row_of_host.size(), h->s);
+    MVG_HIP(hipStreamSynchronize(h->s));
+    h->row_of_dirty = false;
+  }
   if (h->precision != 0 && h->tables_dirty) {
     if (h->metric == MVG_METRIC_L2) {
       h->ptab.reserve(sizeof(float) * (size_t)h->nlist * h->M * mvg::kKsub);
@@ -796,18 +914,25 @@ void mvg::ivf_prepare(mvg_ivfpq *h) {
     h->vq_dirty = false;
   }
   if (h->precision != 0 && h->metric == MVG_METRIC_L2 && h->pre_dirty) {
-    h->entry_pre.reserve(sizeof(float) * (size_t)std::max<int64_t>(h->ntotal, 1));
+    // one term per CSR entry (the lists' own total, whatever ntotal says)
+    h->entry_pre.reserve(sizeof(float) * (size_t)std::max<int64_t>(h->ncsr, 1));
     mvg::precompute_entry_terms(h->list_off.as<int64_t>(), h->nlist, h->codes.as<uint8_t>(), h->M,
-                                h->ptab.as<float>(), h->ntotal, h->entry_pre.as<float>(), h->s);
+                                h->ptab.as<float>(), h->ncsr, h->entry_pre.as<float>(), h->s);
     MVG_HIP(hipStreamSynchronize(h->s));
     h->pre_dirty = false;
   }
 }
 
+void mvg::ivf_check_unserved(const mvg_ivfpq *h) {
+  if (h->servers.load() > 0)
+    throw Error(mvg::kState, "the index is held by a server: destroy the server before changing it");
+}
+
 void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
                           const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
                           const uint8_t *exclude, int64_t exclude_len, float *distances,
-                          int64_t *labels, mvg::IvfWork &work, hipStream_t s) {
+                          int64_t *labels, mvg::IvfWork &work, hipStream_t s,
+                          const int32_t *k_per_query) {
   mvg::IvfSearchArgs a{};
   a.q = q;
   a.nq = nq;
@@ -837,6 +962,12 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.deep_cap = h->top_sum.empty() ? 0 : h->top_sum[(size_t)a.nprobe];
   a.deep_min = h->deep_min;
   a.deep_pool = h->deep_pool == 0 ? 1 : 0;  // ABI pool 0: keys in HBM; 1: running set in LDS
+  a.kq = k_per_query;
+  if (h->custom_ids && a.vectors) {
+    if (h->row_of_dirty) throw Error(mvg::kState, "explicit-id row map is not prepared");
+    a.rows.row_of = h->row_of.as<int32_t>();
+    a.rows.nids = (int64_t)h->row_of_host.size();
+  }
   if (h->precision != 0) {
     if (h->tables_dirty) throw Error(mvg::kState, "mixed-precision tables are not prepared");
     if (h->metric == MVG_METRIC_L2) {
@@ -902,6 +1033,7 @@ int mvg_ivfpq_search(mvg_ivfpq *h, const float *q, int64_t nq, int k,
 int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan) {
   API_BEGIN
   require(h && (precision == 0 || precision == 1), "precision must be 0 (plain) or 1 (mixed)");
+  mvg::ivf_check_unserved(h);
   h->precision = precision;
   h->split_scan = split_scan != 0;
   API_END
@@ -910,6 +1042,7 @@ int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan) {
 int mvg_ivfpq_set_prefilter(mvg_ivfpq *h, int per_k) {
   API_BEGIN
   require(h && per_k >= 0, "per_k must be >= 0");
+  mvg::ivf_check_unserved(h);
   h->prefilter = per_k;
   API_END
 }
@@ -917,6 +1050,7 @@ int mvg_ivfpq_set_prefilter(mvg_ivfpq *h, int per_k) {
 int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item) {
   API_BEGIN
   require(h && entries_per_item >= 0, "entries_per_item must be >= 0");
+  mvg::ivf_check_unserved(h);
   h->wq_entries = entries_per_item;
   API_END
 }
@@ -924,6 +1058,7 @@ int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item) {
 int mvg_ivfpq_set_deep_select(mvg_ivfpq *h, int min_candidates, int pool) {
   API_BEGIN
   require(h && min_candidates >= 0 && (pool == 0 || pool == 1), "invalid arguments");
+  mvg::ivf_check_unserved(h);
   h->deep_min = min_candidates;
   h->deep_pool = pool;
   API_END
@@ -1005,6 +1140,8 @@ struct mvg_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   mvg::IvfWork work;  // the workspace the graph's kernels point into
+  mvg_ivfpq *index = nullptr;
+  uint64_t generation = 0;  // the index's mutation generation at capture
 };
 
 extern "C" {
@@ -1012,7 +1149,15 @@ extern "C" {
 int mvg_ivfpq_capture(mvg_ivfpq *h, const float *q, int64_t nq, int k,
                       const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
                       float *distances, int64_t *labels, mvg_graph **out) {
+  return mvg_ivfpq_capture_k(h, q, nq, nullptr, k, nprobe_per_query, max_nprobe, k_factor, distances,
+                             labels, out);
+}
+
+int mvg_ivfpq_capture_k(mvg_ivfpq *h, const float *q, int64_t nq, const int32_t *k_per_query,
+                        int max_k, const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                        float *distances, int64_t *labels, mvg_graph **out) {
   API_BEGIN
+  const int k = max_k;
   require(h && nq >= 1 && k >= 1 && k_factor >= 1 && out, "invalid arguments");
   require(h->trained && h->ntotal > 0, "capture needs a trained, non-empty index");
   std::lock_guard<std::mutex> lk(h->search_mu);
@@ -1024,12 +1169,12 @@ int mvg_ivfpq_capture(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   g->work.prof.on = false;
   // One eager run sizes every buffer, so the capture allocates nothing.
   mvg::ivf_search_with(h, q, nq, k, nprobe_per_query, max_nprobe, k_factor, nullptr, 0, distances,
-                       labels, g->work, s);
+                       labels, g->work, s, k_per_query);
   MVG_HIP(hipStreamSynchronize(s));
   MVG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
   try {
     mvg::ivf_search_with(h, q, nq, k, nprobe_per_query, max_nprobe, k_factor, nullptr, 0, distances,
-                         labels, g->work, s);
+                         labels, g->work, s, k_per_query);
   } catch (...) {
     hipGraph_t dummy = nullptr;
     (void)hipStreamEndCapture(s, &dummy);
@@ -1039,6 +1184,8 @@ int mvg_ivfpq_capture(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   MVG_HIP(hipStreamEndCapture(s, &g->graph));
   MVG_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   (void)prof;
+  g->index = h;
+  g->generation = h->generation.load();
   *out = g.release();
   API_END
 }
@@ -1101,6 +1248,9 @@ int mvg_ivfpq_autotune(mvg_ivfpq *h, const float *q, int64_t nq, int k,
 int mvg_graph_launch(mvg_graph *g, void *stream) {
   API_BEGIN
   require(g && g->exec, "invalid graph");
+  // the graph's kernels hold the index buffers of capture time
+  if (g->index && g->index->generation.load() != g->generation)
+    throw Error(mvg::kState, "the index changed after this graph was captured: capture it again");
   MVG_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
   API_END
 }

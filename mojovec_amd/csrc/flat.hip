@@ -183,7 +183,8 @@ __global__ __launch_bounds__(kNT) void flat_argmin_kernel(
 __global__ __launch_bounds__(kNT) void merge_topk_kernel(
     const uint64_t *__restrict__ cand, int64_t ncand, int k, int negate, float pad_dist,
     float *__restrict__ D, int64_t *__restrict__ I64, int32_t *__restrict__ I32,
-    uint64_t *__restrict__ out_keys, const int64_t *__restrict__ seg_off, int cap) {
+    uint64_t *__restrict__ out_keys, const int64_t *__restrict__ seg_off, int cap,
+    const int32_t *__restrict__ kq, int kmul) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t *buf = reinterpret_cast<uint64_t *>(smem);  // cap
   uint64_t *top = buf + cap;                             // next_pow2(k)
@@ -203,6 +204,7 @@ __global__ __launch_bounds__(kNT) void merge_topk_kernel(
     __syncthreads();
     cur = select_sort_u64(buf, cur + len, k, scr, top);
   }
+  if (kq) cur = min(cur, (int)min<int64_t>((int64_t)kq[qi] * kmul, k));
   for (int t = tid; t < k; t += kNT) {
     float dv = pad_dist;
     int64_t iv = -1;
@@ -223,7 +225,7 @@ __global__ __launch_bounds__(kNT) void merge_topk_kernel(
 
 void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int negate,
                   float pad_dist, float *D, int64_t *I64, int32_t *I32, uint64_t *keys,
-                  hipStream_t s, const int64_t *seg_off) {
+                  hipStream_t s, const int64_t *seg_off, const int32_t *kq, int kmul) {
   if (nq == 0) return;
   if (k > kMaxCandidates) throw Error(kInvalid, "k exceeds the device top-k limit of 4096");
   // The buffer holds the running top k plus the next candidates; a merge of
@@ -240,7 +242,7 @@ void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int ne
     attr_set = true;
   }
   hipLaunchKernelGGL(merge_topk_kernel, dim3((unsigned)nq), dim3(kNT), sm, s, cand, ncand, k,
-                     negate, pad_dist, D, I64, I32, keys, seg_off, cap);
+                     negate, pad_dist, D, I64, I32, keys, seg_off, cap, kq, kmul);
   MVG_LAUNCH_CHECK("merge_topk_kernel");
 }
 

@@ -2,6 +2,7 @@
 // and the request loop (server.cpp).
 #pragma once
 
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -28,6 +29,19 @@ struct mvg_ivfpq {
   bool vq_dirty = true;
   float vq_min = 0.0f, vq_scale = 1.0f;
   mvg::DevBuf vq_codes, vq_norms, vq_tmp;
+  // Explicit ids (add_with_ids): the lists hold the caller's ids while the
+  // FP32 rows and their SQ8 codes stay in add order, so the rerank finds a
+  // row through row_of[id] (-1: no row). Built once an id differs from its
+  // row number while rows are kept.
+  bool custom_ids = false, row_of_dirty = false;
+  std::vector<int32_t> row_of_host;
+  mvg::DevBuf row_of;
+  int64_t ncsr = 0;  // entries in the device CSR lists
+  // Mutation generation (captured graphs refuse to replay after it moves)
+  // and the number of servers holding the handle (mutations and search
+  // settings are refused while one does).
+  std::atomic<uint64_t> generation{0};
+  std::atomic<int> servers{0};
   hipStream_t s = nullptr;
   std::mutex search_mu;  // serialises searches that share `work`
 };
@@ -74,9 +88,16 @@ void ivf_prepare(mvg_ivfpq *h);
 
 // Enqueues one search on `s` with the caller's workspace; every pointer is
 // device memory. Throws mvg::Error.
+// k_per_query (nullable, device [nq]): per-query result counts <= k, in
+// output rows of stride k.
 void ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
                      const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
                      const uint8_t *exclude, int64_t exclude_len, float *distances,
-                     int64_t *labels, IvfWork &work, hipStream_t s);
+                     int64_t *labels, IvfWork &work, hipStream_t s,
+                     const int32_t *k_per_query = nullptr);
+
+// Throws MVG_ERR_STATE while a server holds the handle (mutations and
+// search settings would change what its batches read).
+void ivf_check_unserved(const mvg_ivfpq *h);
 
 }  // namespace mvg

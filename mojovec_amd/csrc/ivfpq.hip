@@ -192,6 +192,33 @@ __device__ __forceinline__ int64_t rerank_query(int64_t nq, const int32_t *__res
   return t < nq ? (int64_t)order[t] : -1;
 }
 
+// Row of candidate id (RowMap), or -1 when it has none.
+__device__ __forceinline__ int64_t cand_row(uint32_t id, RowMap rm, int64_t nrows) {
+  int64_t r = (int64_t)id;
+  if (rm.row_of) r = r < rm.nids ? (int64_t)rm.row_of[id] : -1;
+  return (r >= 0 && r < nrows) ? r : -1;
+}
+
+// Writes query qi's sorted top keys (cnt valid) as k result slots, padding
+// slots at or past cnt and past the query's own count kq[qi].
+template <bool IP>
+__device__ __forceinline__ void write_results(const uint64_t *top, int cnt, int64_t qi, int k,
+                                              const int32_t *__restrict__ kq, float pad_dist,
+                                              float *__restrict__ D, int64_t *__restrict__ I) {
+  const int kv = kq ? min(kq[qi], k) : k;
+  for (int t = threadIdx.x; t < k; t += kNT) {
+    float dv = pad_dist;
+    int64_t iv = -1;
+    if (t < cnt && t < kv) {
+      dv = key2f((uint32_t)(top[t] >> 32));
+      if (IP) dv = -dv;
+      iv = (int64_t)(uint32_t)(top[t] & 0xFFFFFFFFu);
+    }
+    D[qi * k + t] = dv;
+    I[qi * k + t] = iv;
+  }
+}
+
 template <bool IP>
 __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q, int d,
                                                      const float *__restrict__ vectors,
@@ -200,7 +227,8 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
                                                      int k, int buf_n, float pad_dist,
                                                      float *__restrict__ D,
                                                      int64_t *__restrict__ I, int64_t nq,
-                                                     const int32_t *__restrict__ order) {
+                                                     const int32_t *__restrict__ order, RowMap rm,
+                                                     const int32_t *__restrict__ kq) {
   extern __shared__ __align__(16) unsigned char smem[];
   float *qs = reinterpret_cast<float *>(smem);
   uint64_t *buf = reinterpret_cast<uint64_t *>(qs + ((d + 1) & ~1));
@@ -216,23 +244,14 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
   for (int c = wave; c < kk; c += kNW) {
     const uint64_t key = cq[c];
     const uint32_t id = (uint32_t)(key & 0xFFFFFFFFu);
-    if (key == kSentinel64 || (int64_t)id >= nrows) continue;
-    const float ex = wave_dist_w64<IP>(qs, vectors + (int64_t)id * d, d, lane);
+    const int64_t row = key == kSentinel64 ? -1 : cand_row(id, rm, nrows);
+    if (row < 0) continue;
+    const float ex = wave_dist_w64<IP>(qs, vectors + row * d, d, lane);
     if (lane == 0) buf[c] = pack_key(IP ? -ex : ex, id);
   }
   __syncthreads();
   const int cnt = select_sort_u64(buf, kk, k, *scr, top);
-  for (int t = tid; t < k; t += kNT) {
-    float dv = pad_dist;
-    int64_t iv = -1;
-    if (t < cnt) {
-      dv = key2f((uint32_t)(top[t] >> 32));
-      if (IP) dv = -dv;
-      iv = (int64_t)(uint32_t)(top[t] & 0xFFFFFFFFu);
-    }
-    D[qi * k + t] = dv;
-    I[qi * k + t] = iv;
-  }
+  write_results<IP>(top, cnt, qi, k, kq, pad_dist, D, I);
 }
 
 // Exact distances of a query's kk candidates into buf (sentinel for empty or
@@ -241,7 +260,7 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
 template <bool IP, int R, bool BIG>
 __device__ __forceinline__ void rerank_rows(const float *qs, const float *__restrict__ vectors,
                                             int64_t nrows, const uint64_t *__restrict__ cq, int kk,
-                                            int d, uint64_t *buf) {
+                                            int d, uint64_t *buf, RowMap rm) {
   constexpr int G = kNT / 16;
   const int glane = threadIdx.x & 15, group = threadIdx.x >> 4;
   for (int c = group; c < kk; c += R * G) {
@@ -254,8 +273,9 @@ __device__ __forceinline__ void rerank_rows(const float *qs, const float *__rest
       const int ct = c + t * G;
       const uint64_t kv = ct < kk ? cq[ct] : kSentinel64;
       id[t] = (uint32_t)(kv & 0xFFFFFFFFu);
-      ok[t] = kv != kSentinel64 && (int64_t)id[t] < nrows;
-      x[t] = ok[t] ? vectors + (int64_t)id[t] * d : qs;  // an invalid slot reads the query row
+      const int64_t row = kv == kSentinel64 ? -1 : cand_row(id[t], rm, nrows);
+      ok[t] = row >= 0;
+      x[t] = ok[t] ? vectors + row * d : qs;  // an invalid slot reads the query row
       any |= ok[t];
     }
     float e[R];
@@ -283,7 +303,9 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
                                                            int kk, int k, float pad_dist,
                                                            float *__restrict__ D,
                                                            int64_t *__restrict__ I, int64_t nq,
-                                                           const int32_t *__restrict__ order) {
+                                                           const int32_t *__restrict__ order,
+                                                           RowMap rm,
+                                                           const int32_t *__restrict__ kq) {
   extern __shared__ __align__(16) unsigned char smem[];
   float *qs = reinterpret_cast<float *>(smem);                      // d
   uint64_t *buf = reinterpret_cast<uint64_t *>(qs + d);             // kk
@@ -296,22 +318,12 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
   __syncthreads();
   const uint64_t *cq = cand + qi * kk;
   if constexpr (SMALLD)  // d < 256: four rows per group, no four-accumulator blocks
-    rerank_rows<IP, 4, false>(qs, vectors, nrows, cq, kk, d, buf);
+    rerank_rows<IP, 4, false>(qs, vectors, nrows, cq, kk, d, buf, rm);
   else
-    rerank_rows<IP, 2, true>(qs, vectors, nrows, cq, kk, d, buf);
+    rerank_rows<IP, 2, true>(qs, vectors, nrows, cq, kk, d, buf, rm);
   __syncthreads();
   const int cnt = select_sort_u64(buf, kk, k, *scr, top);
-  for (int t = tid; t < k; t += kNT) {
-    float dv = pad_dist;
-    int64_t iv = -1;
-    if (t < cnt) {
-      dv = key2f((uint32_t)(top[t] >> 32));
-      if (IP) dv = -dv;
-      iv = (int64_t)(uint32_t)(top[t] & 0xFFFFFFFFu);
-    }
-    D[qi * k + t] = dv;
-    I[qi * k + t] = iv;
-  }
+  write_results<IP>(top, cnt, qi, k, kq, pad_dist, D, I);
 }
 
 // Mixed-form pre-rerank: the SQ8 code distance of each of a query's kk
@@ -325,7 +337,8 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
     const uint8_t *__restrict__ qcodes, const uint32_t *__restrict__ qnorms, int ld,
     const uint8_t *__restrict__ vcodes, const uint32_t *__restrict__ vnorms, int64_t nrows,
     const uint64_t *__restrict__ cand, int kk, int kk2, uint64_t *__restrict__ out, int64_t nq,
-    const int32_t *__restrict__ order) {
+    const int32_t *__restrict__ order, RowMap rm, const int32_t *__restrict__ kq, int kf,
+    int per_k) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t *buf = reinterpret_cast<uint64_t *>(smem);  // kk
   uint64_t *top = buf + ((kk + 1) & ~1);               // next_pow2(kk2)
@@ -339,13 +352,15 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
   const uint64_t *cq = cand + qi * kk;
   for (int c = group; c < kk; c += R * G) {
     uint32_t id[R], acc[R];
+    int64_t row[R];
     bool ok[R];
 #pragma unroll
     for (int t = 0; t < R; t++) {
       const int ct = c + t * G;
       const uint64_t kv = ct < kk ? cq[ct] : kSentinel64;
       id[t] = (uint32_t)(kv & 0xFFFFFFFFu);
-      ok[t] = kv != kSentinel64 && (int64_t)id[t] < nrows;
+      row[t] = kv == kSentinel64 ? -1 : cand_row(id[t], rm, nrows);
+      ok[t] = row[t] >= 0;
       acc[t] = 0;
     }
     for (int w = gl; w < words; w += 8) {
@@ -353,7 +368,7 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
 #pragma unroll
       for (int t = 0; t < R; t++) {
         if (!ok[t]) continue;
-        const uint4 b = reinterpret_cast<const uint4 *>(vcodes + (int64_t)id[t] * ld)[w];
+        const uint4 b = reinterpret_cast<const uint4 *>(vcodes + row[t] * ld)[w];
         if (IP) {
           int v = (int)acc[t];
           v = __builtin_amdgcn_sdot4((int)a.x, (int)b.x, v, false);
@@ -384,7 +399,7 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
           if (IP) {
             key = (uint32_t)((int64_t)0x80000000LL - (int64_t)(int32_t)acc[t]);
           } else {
-            const int64_t v = (int64_t)qnorms[qi] + (int64_t)vnorms[id[t]] - 2 * (int64_t)acc[t];
+            const int64_t v = (int64_t)qnorms[qi] + (int64_t)vnorms[row[t]] - 2 * (int64_t)acc[t];
             key = (uint32_t)(v > 0 ? v : 0);
           }
         }
@@ -394,7 +409,14 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
   }
   __syncthreads();
   const int cnt = select_sort_u64(buf, kk, kk2, *scr, top);
-  for (int t = tid; t < kk2; t += kNT) out[qi * kk2 + t] = t < cnt ? top[t] : kSentinel64;
+  // per-query k: the width a search with k = kq[qi] alone would keep
+  int w2 = kk2;
+  if (kq) {
+    const int64_t kqi = kq[qi], kkq = kqi * kf;
+    w2 = (int)(per_k > 0 ? min(kkq, max((int64_t)64, (int64_t)per_k * kqi)) : kkq);
+    w2 = min(w2, kk2);
+  }
+  for (int t = tid; t < kk2; t += kNT) out[qi * kk2 + t] = (t < cnt && t < w2) ? top[t] : kSentinel64;
 }
 
 // ---------------------------------------------------------------------------
@@ -2012,7 +2034,7 @@ void precompute_entry_terms(const int64_t *list_off, int nlist, const uint8_t *c
 
 void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors, int64_t nrows,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
-            hipStream_t s, const int32_t *order) {
+            hipStream_t s, const int32_t *order, RowMap rm, const int32_t *kq) {
   if (nq == 0) return;
   const bool ip = metric == kMetricIP;
   const unsigned grid =
@@ -2028,7 +2050,7 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
       MVG_HIP(hipFuncSetAttribute((const void *)rerank_group_kernel<IPV, SV>,                    \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));         \
     hipLaunchKernelGGL((rerank_group_kernel<IPV, SV>), dim3(grid), dim3(kNT), sm, s, q, d, vectors, \
-                       nrows, cand, kk, k, pad_dist, D, I, nq, order);                           \
+                       nrows, cand, kk, k, pad_dist, D, I, nq, order, rm, kq);                   \
   }
       if (ip) {
         if (small) MVG_RR(true, true) else MVG_RR(true, false)
@@ -2049,10 +2071,10 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
     MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   if (ip)
     hipLaunchKernelGGL(rerank_kernel<true>, dim3(grid), dim3(kNT), sm, s, q, d, vectors, nrows,
-                       cand, kk, k, buf_n, pad_dist, D, I, nq, order);
+                       cand, kk, k, buf_n, pad_dist, D, I, nq, order, rm, kq);
   else
     hipLaunchKernelGGL(rerank_kernel<false>, dim3(grid), dim3(kNT), sm, s, q, d, vectors, nrows,
-                       cand, kk, k, buf_n, pad_dist, D, I, nq, order);
+                       cand, kk, k, buf_n, pad_dist, D, I, nq, order, rm, kq);
   MVG_LAUNCH_CHECK("rerank_kernel");
 }
 
@@ -2068,7 +2090,7 @@ void rerank_mixed(const IvfSearchArgs &a, IvfWork &w, const uint64_t *keys, int 
   const int kk2 = prefilter_width(a.k, kk, a.prefilter);
   if (!a.vq_codes || kk2 >= kk) {
     rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, keys, kk, a.k, kIvfPadDist, a.D, a.I, s,
-           order);
+           order, a.rows, a.kq);
     return;
   }
   const bool ip = a.metric == kMetricIP;
@@ -2088,14 +2110,14 @@ void rerank_mixed(const IvfSearchArgs &a, IvfWork &w, const uint64_t *keys, int 
   if (ip)
     hipLaunchKernelGGL(sq8_prefilter_kernel<true>, dim3(grid), dim3(kNT), sm, s, w.vq_qcodes.as<uint8_t>(),
                        w.vq_qnorms.as<uint32_t>(), ld, a.vq_codes, a.vq_norms, a.ntotal, keys, kk, kk2,
-                       w.vq_keys.as<uint64_t>(), a.nq, order);
+                       w.vq_keys.as<uint64_t>(), a.nq, order, a.rows, a.kq, kk / a.k, a.prefilter);
   else
     hipLaunchKernelGGL(sq8_prefilter_kernel<false>, dim3(grid), dim3(kNT), sm, s, w.vq_qcodes.as<uint8_t>(),
                        w.vq_qnorms.as<uint32_t>(), ld, a.vq_codes, a.vq_norms, a.ntotal, keys, kk, kk2,
-                       w.vq_keys.as<uint64_t>(), a.nq, order);
+                       w.vq_keys.as<uint64_t>(), a.nq, order, a.rows, a.kq, kk / a.k, a.prefilter);
   MVG_LAUNCH_CHECK("sq8_prefilter_kernel");
   rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, w.vq_keys.as<uint64_t>(), kk2, a.k, kIvfPadDist,
-         a.D, a.I, s, order);
+         a.D, a.I, s, order, a.rows, a.kq);
 }
 
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
@@ -2141,11 +2163,15 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   }
   const bool ip = a.metric == kMetricIP;
   if (pf.on) MVG_HIP(hipEventRecord(pf.ev[1], s));
+  // Per-query k (a.kq): every query is scanned for the batch's k * k_factor
+  // candidates, then cut to its own kq * k_factor best (the candidate set of
+  // a search with its k alone) before the rerank or the output.
+  const bool per_k = a.kq != nullptr;
   if (!a.split_scan) {
     // 2'. one workgroup per query: LUT + ADC scan + running top-kk, then the
     // optional rerank.
     uint64_t *keys = nullptr;
-    if (kf > 1) {
+    if (kf > 1 || per_k) {
       w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
       keys = w.keys.as<uint64_t>();
     }
@@ -2222,12 +2248,23 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     }
 #undef MVG_QSCAN
     if (wqp) {  // segmented merge of each query's items
-      if (kf == 1)
+      if (kf == 1 && !per_k)
         launch_merge(wq.part, a.nq, kk, a.k, ip ? 1 : 0, kIvfPadDist, a.D, a.I, nullptr, nullptr, s,
                      wq.item_off);
       else
         launch_merge(wq.part, a.nq, kk, kk, 0, kIvfPadDist, nullptr, nullptr, nullptr, keys, s,
                      wq.item_off);
+    }
+    if (per_k) {  // each query's own kq * k_factor best, sorted
+      w.keys2.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
+      if (kf == 1) {
+        launch_merge(keys, a.nq, kk, kk, ip ? 1 : 0, kIvfPadDist, a.D, a.I, nullptr, nullptr, s, nullptr,
+                     a.kq, 1);
+      } else {
+        launch_merge(keys, a.nq, kk, kk, 0, kIvfPadDist, nullptr, nullptr, nullptr,
+                     w.keys2.as<uint64_t>(), s, nullptr, a.kq, kf);
+        keys = w.keys2.as<uint64_t>();
+      }
     }
     if (pf.on) MVG_HIP(hipEventRecord(pf.ev[2], s));
     if (kf > 1) {
@@ -2236,7 +2273,7 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
         rerank_mixed(a, w, keys, kk, ord, s);
       else
         rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, keys, kk, a.k, kIvfPadDist, a.D, a.I, s,
-               ord);
+               ord, a.rows, a.kq);
     }
   } else {
   // 2. per-(query, probe) LUT + ADC scan + list-local top-kk.
@@ -2253,19 +2290,20 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
   }
 #undef MVG_SCAN
   if (pf.on) MVG_HIP(hipEventRecord(pf.ev[2], s));
-  // 3. per-query merge (and rerank).
+  // 3. per-query merge (and rerank); with per-query k the merge keeps each
+  // query's own kq (or kq * k_factor) best.
   if (kf == 1) {
     launch_merge(part, a.nq, (int64_t)maxp * kk, a.k, ip ? 1 : 0, kIvfPadDist, a.D, a.I, nullptr,
-                 nullptr, s);
+                 nullptr, s, nullptr, a.kq, 1);
   } else {
     w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
     launch_merge(part, a.nq, (int64_t)maxp * kk, kk, 0, kIvfPadDist, nullptr, nullptr, nullptr,
-                 w.keys.as<uint64_t>(), s);
+                 w.keys.as<uint64_t>(), s, nullptr, a.kq, kf);
     if (a.precision != 0)
       rerank_mixed(a, w, w.keys.as<uint64_t>(), kk, nullptr, s);
     else
       rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, w.keys.as<uint64_t>(), kk, a.k, kIvfPadDist,
-             a.D, a.I, s);
+             a.D, a.I, s, nullptr, a.rows, a.kq);
   }
   }
   if (pf.on) {

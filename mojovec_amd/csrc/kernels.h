@@ -15,10 +15,14 @@ void flat_topk(const float *base, int64_t n, int d, int metric, const float *q, 
 void flat_argmin(const float *base, int64_t n, int d, int metric, bool kmeans_mode, const float *q,
                  int64_t nq, int32_t *out_id, float *out_dist, hipStream_t s);
 // Reduce ncand candidate keys per query to the sorted top-k. With seg_off,
-// query q owns units [seg_off[q], seg_off[q+1]) of ncand keys each.
+// query q owns units [seg_off[q], seg_off[q+1]) of ncand keys each. With kq
+// (nullable, [nq]) query q keeps only its kq[q] * kmul (<= k) smallest keys
+// in its k output slots (the rest padded): a prefix of the sorted top-k, so
+// the set a top-(kq[q] * kmul) selection alone would keep.
 void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int negate,
                   float pad_dist, float *D, int64_t *I64, int32_t *I32, uint64_t *keys,
-                  hipStream_t s, const int64_t *seg_off = nullptr);
+                  hipStream_t s, const int64_t *seg_off = nullptr, const int32_t *kq = nullptr,
+                  int kmul = 1);
 
 // ---- kmeans.hip ----
 struct KmeansWork {
@@ -36,6 +40,11 @@ void pq_assign(const float *x, int64_t n, int d, int M, const float *cb, const f
 // LUT[q][m][256] (pq.mojo:252-292) for nq query (or residual) vectors.
 void pq_lut(const float *q, int64_t nq, int d, int M, const float *cb, int metric, float *lut,
             hipStream_t s);
+// ProductQuantizer.decode (pq.mojo:221-250): codes [n][M] -> rows [n][d].
+void pq_decode(const uint8_t *codes, int64_t n, int d, int M, const float *cb, float *out,
+               hipStream_t s);
+// out[i] = base row ids[i] (every id in range; checked by the caller).
+void gather_rows(const float *base, int d, const int64_t *ids, int64_t n, float *out, hipStream_t s);
 // x - coarse[label] row-wise.
 void residuals(const float *x, int64_t n, int d, const float *coarse, const int32_t *labels,
                float *out, hipStream_t s);
@@ -43,6 +52,14 @@ void residuals(const float *x, int64_t n, int d, const float *coarse, const int3
 // ---- ivfpq.hip ----
 // Compute units of the current device (cached after the first call).
 int device_cus();
+// Candidate id -> row of the stored FP32 rows (and of their SQ8 codes):
+// the id itself when row_of is null, else row_of[id], where ids at or above
+// nids and entries of -1 have no row.
+struct RowMap {
+  const int32_t *row_of = nullptr;
+  int64_t nids = 0;
+};
+
 struct IvfSearchArgs {
   const float *q;
   int64_t nq;
@@ -75,6 +92,11 @@ struct IvfSearchArgs {
   const uint32_t *vq_norms;
   float vq_min, vq_scale;
   int prefilter;
+  RowMap rows;  // explicit ids: id -> row of `vectors` / `vq_codes`
+  // per-query k (nullable, device [nq]): query i returns kq[i] <= k results
+  // in rows of stride k (padded past kq[i]); results equal those of a search
+  // with k = kq[i] alone
+  const int32_t *kq;
 };
 // pre[e] = sum_m ptab[l][m][code_m] for every entry of the CSR lists.
 void precompute_entry_terms(const int64_t *list_off, int nlist, const uint8_t *codes, int M,
@@ -93,7 +115,7 @@ struct IvfProfile {
   hipStream_t stream = nullptr;  // stream of the recorded search (the counter is written there)
 };
 struct IvfWork {
-  DevBuf coarse_part, probes, part, keys, qbf, qnorm;
+  DevBuf coarse_part, probes, part, keys, keys2, qbf, qnorm;
   DevBuf wq_poff, wq_nitems, wq_rq, wq_off, wq_gthr, wq_counter;
   DevBuf deep_seg, deep_counter;  // deep candidate pools: key segments, query range counters
   DevBuf ord_cnt, ord_off, ord;    // query order of the persistent deep scans
@@ -105,12 +127,15 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
 
 // Exact FP32 rerank of kk candidate keys per query (any order) to the top-k
 // (index_hnsw.mojo:449-496 generalised to k_factor).
-// Candidate ids at or above nrows (not rows of `vectors`) are dropped.
+// Candidates without a row of `vectors` (see RowMap) are dropped.
 // order (nullable, [nq]): a query order whose neighbours share candidate
 // rows; each XCD then reranks a contiguous stretch of it.
+// kq (nullable, [nq]): per-query result counts <= k; the slots of query i
+// past kq[i] (stride k) are padded.
 void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors, int64_t nrows,
             const uint64_t *cand, int kk, int k, float pad_dist, float *D, int64_t *I,
-            hipStream_t s, const int32_t *order = nullptr);
+            hipStream_t s, const int32_t *order = nullptr, RowMap rm = RowMap{},
+            const int32_t *kq = nullptr);
 
 // ---- ivfflat.hip ----
 struct IvfFlatArgs {
@@ -183,6 +208,9 @@ struct Sq8SearchArgs {
   float *D;
   int64_t *I;
   int use_mfma;  // 1: int8 MFMA contraction; 0: dot4 VALU kernel (same results)
+  // per-query k (nullable, device [nq]): query i returns kq[i] <= k results
+  // in rows of stride k (padded past kq[i])
+  const int32_t *kq;
 };
 struct Sq8Work {
   DevBuf qcodes, qnorms, qsums, part, keys;

@@ -113,6 +113,29 @@ __global__ void pq_lut_kernel(const float *__restrict__ q, int d, int M, int dsu
   }
 }
 
+// ProductQuantizer.decode (pq.mojo:221-250): row i, column m * dsub + j is
+// codebook entry (m, codes[i][m]), component j. One thread per output float.
+__global__ void pq_decode_kernel(const uint8_t *__restrict__ codes, int64_t n, int d, int M,
+                                 const float *__restrict__ cb, float *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * d) return;
+  const int dsub = d / M;
+  const int64_t i = t / d;
+  const int c = (int)(t - i * d);
+  const int m = c / dsub, j = c - m * dsub;
+  out[t] = cb[((int64_t)m * kKsub + codes[i * M + m]) * dsub + j];
+}
+
+// Rows ids[i] of base [.][d] gathered into out [n][d].
+__global__ void gather_rows_kernel(const float *__restrict__ base, int d,
+                                   const int64_t *__restrict__ ids, int64_t n,
+                                   float *__restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * d) return;
+  const int64_t i = t / d;
+  out[t] = base[ids[i] * d + (t - i * d)];
+}
+
 __global__ void residual_kernel(const float *__restrict__ x, int64_t n, int d,
                                 const float *__restrict__ coarse, const int32_t *__restrict__ labels,
                                 float *__restrict__ out) {
@@ -150,6 +173,21 @@ void pq_lut(const float *q, int64_t nq, int d, int M, const float *cb, int metri
   if (nq == 0) return;
   hipLaunchKernelGGL(pq_lut_kernel, dim3((unsigned)nq), dim3(256), 0, s, q, d, M, d / M, cb, metric, lut);
   MVG_LAUNCH_CHECK("pq_lut_kernel");
+}
+
+void pq_decode(const uint8_t *codes, int64_t n, int d, int M, const float *cb, float *out,
+               hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(pq_decode_kernel, dim3(ceil_div(n * d, 256)), dim3(256), 0, s, codes, n, d, M, cb,
+                     out);
+  MVG_LAUNCH_CHECK("pq_decode_kernel");
+}
+
+void gather_rows(const float *base, int d, const int64_t *ids, int64_t n, float *out, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(ceil_div(n * d, 256)), dim3(256), 0, s, base, d, ids, n,
+                     out);
+  MVG_LAUNCH_CHECK("gather_rows_kernel");
 }
 
 void residuals(const float *x, int64_t n, int d, const float *coarse, const int32_t *labels,

@@ -2,11 +2,15 @@
 // index.
 //
 // Requests (a block of queries with its own k, nprobe and k_factor; nprobe is
-// ignored by a Flat SQ8 index) queue on the host. A dispatcher thread packs waiting requests that share (k,
-// k_factor) into a batch (per-query nprobe travels with each query), admits
-// the batch when a slot is free and the batch's device-memory estimate fits
-// the budget, and enqueues it on the slot's stream with the slot's own
-// workspace, so batches overlap on the device. Completed slots are retired as
+// ignored by a Flat SQ8 index) queue on the host. A dispatcher thread packs
+// waiting requests that share k_factor into a batch (per-query k and nprobe
+// travel with each query; a Flat SQ8 batch keeps k * k_factor on one side of
+// 64, where its MFMA form changes), admits the batch when a slot is free and
+// the batch's device-memory estimate fits the budget, and enqueues it on the
+// slot's stream with the slot's own workspace, so batches overlap on the
+// device. A batch is searched for its largest k and each query cut to its
+// own k (the merge keeps a query's own k * k_factor candidates, which is the
+// candidate set of a search with that k alone). Completed slots are retired as
 // soon as their event fires; a request's result depends only on its own
 // queries (every search kernel works per query, and a split of the base rows
 // only changes which partial lists hold a candidate, not the canonical
@@ -59,13 +63,13 @@ struct Slot {
   hipEvent_t done = nullptr;
   mvg::IvfWork work;
   mvg::Sq8Work swork;
-  mvg::DevBuf q, np, D, I;
+  mvg::DevBuf q, np, kq, D, I;
   PinnedBuf<float> hq, hD;
-  PinnedBuf<int32_t> hnp;
+  PinnedBuf<int32_t> hnp, hkq;
   PinnedBuf<int64_t> hI;
   std::vector<Request *> reqs;
   int64_t nq = 0, bytes = 0;
-  int k = 0;
+  int k = 0;  // the batch's largest k: row stride of hD / hI
   bool busy = false;
 };
 
@@ -112,8 +116,10 @@ void retire(mvg_server *sv, Slot &sl) {
   const auto now = Clock::now();
   std::lock_guard<std::mutex> lk(sv->mu);
   for (Request *r : sl.reqs) {
-    std::copy(sl.hD.ptr + off * sl.k, sl.hD.ptr + (off + r->nq) * sl.k, r->D);
-    std::copy(sl.hI.ptr + off * sl.k, sl.hI.ptr + (off + r->nq) * sl.k, r->I);
+    for (int64_t i = 0; i < r->nq; i++) {
+      std::copy(sl.hD.ptr + (off + i) * sl.k, sl.hD.ptr + (off + i) * sl.k + r->k, r->D + i * r->k);
+      std::copy(sl.hI.ptr + (off + i) * sl.k, sl.hI.ptr + (off + i) * sl.k + r->k, r->I + i * r->k);
+    }
     off += r->nq;
     r->t_done = now;
     r->done = true;
@@ -141,28 +147,44 @@ void fail_slot(mvg_server *sv, Slot &sl, const std::string &msg) {
   sv->cv_done.notify_all();
 }
 
+// Whether a request may join a batch whose head is `head`: equal k_factor;
+// over a Flat SQ8 index also k * k_factor on the head's side of 64 (the
+// 64-query MFMA form holds up to 64 candidates per query, larger pools take
+// the 16-query form, so mixing them would slow the small ones).
+bool joins(const mvg_server *sv, const Request *head, const Request *r) {
+  if (r->k_factor != head->k_factor) return false;
+  if (sv->sq) {
+    const bool small_head = (int64_t)head->k * head->k_factor <= 64;
+    const bool small_r = (int64_t)r->k * r->k_factor <= 64;
+    if (small_head != small_r) return false;
+  }
+  return true;
+}
+
 // Takes a batch from the queue front under the lock; empty if none fits.
 std::vector<Request *> take_batch(mvg_server *sv, int64_t &bytes) {
   std::vector<Request *> batch;
   if (sv->pending.empty()) return batch;
   Request *head = sv->pending.front();
-  const int k = head->k, kf = head->k_factor;
+  const int kf = head->k_factor;
   int64_t nq = 0;
-  int maxp = 1;
+  int maxp = 1, maxk = 1;
   for (auto it = sv->pending.begin(); it != sv->pending.end() && batch.size() < 4096;) {
     Request *r = *it;
-    if (r->k != k || r->k_factor != kf) {
+    if (!joins(sv, head, r)) {
       ++it;
       continue;
     }
     if (!batch.empty() && nq + r->nq > sv->max_batch) break;
     const int mp = sv->ix ? std::max(maxp, std::min(r->nprobe, sv->ix->nlist)) : 1;
-    const int64_t b = batch_bytes(sv, nq + r->nq, k, kf, mp);
+    const int mk = std::max(maxk, r->k);
+    const int64_t b = batch_bytes(sv, nq + r->nq, mk, kf, mp);
     // admission by memory: the first request of an idle server always runs
     if (sv->in_use + b > sv->budget && !(batch.empty() && sv->inflight == 0)) break;
     batch.push_back(r);
     nq += r->nq;
     maxp = mp;
+    maxk = mk;
     bytes = b;
     it = sv->pending.erase(it);
   }
@@ -172,11 +194,17 @@ std::vector<Request *> take_batch(mvg_server *sv, int64_t &bytes) {
 void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t bytes) {
   const int d = sv->d;
   int64_t nq = 0;
-  int maxp = 1;
-  for (Request *r : batch) nq += r->nq;
-  const int k = batch[0]->k, kf = batch[0]->k_factor;
+  int maxp = 1, k = 1;
+  bool mixed_k = false;
+  for (Request *r : batch) {
+    nq += r->nq;
+    k = std::max(k, r->k);
+    mixed_k |= r->k != batch[0]->k;
+  }
+  const int kf = batch[0]->k_factor;
   sl.hq.reserve((size_t)nq * d);
   sl.hnp.reserve((size_t)nq);
+  sl.hkq.reserve((size_t)nq);
   sl.hD.reserve((size_t)nq * k);
   sl.hI.reserve((size_t)nq * k);
   int64_t off = 0;
@@ -185,18 +213,29 @@ void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t byt
     const int np = sv->ix ? std::min(r->nprobe, sv->ix->nlist) : 1;
     maxp = std::max(maxp, np);
     std::fill(sl.hnp.ptr + off, sl.hnp.ptr + off + r->nq, np);
+    std::fill(sl.hkq.ptr + off, sl.hkq.ptr + off + r->nq, r->k);
     off += r->nq;
   }
   sl.q.reserve(sizeof(float) * nq * d);
   sl.np.reserve(sizeof(int32_t) * nq);
+  sl.kq.reserve(sizeof(int32_t) * nq);
   sl.D.reserve(sizeof(float) * nq * k);
   sl.I.reserve(sizeof(int64_t) * nq * k);
   MVG_HIP(hipMemcpyAsync(sl.q.ptr, sl.hq.ptr, sizeof(float) * nq * d, hipMemcpyHostToDevice, sl.s));
   if (sv->ix)
     MVG_HIP(hipMemcpyAsync(sl.np.ptr, sl.hnp.ptr, sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.s));
+  const int32_t *kq = nullptr;
+  if (mixed_k) {
+    MVG_HIP(hipMemcpyAsync(sl.kq.ptr, sl.hkq.ptr, sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.s));
+    kq = sl.kq.as<int32_t>();
+  }
   if (sv->ix) {
+    // the handle's search lock: a concurrent eager search cannot prepare
+    // (re-upload) the index buffers while this batch is being enqueued
+    std::lock_guard<std::mutex> lk(sv->ix->search_mu);
+    mvg::ivf_prepare(sv->ix);
     mvg::ivf_search_with(sv->ix, sl.q.as<float>(), nq, k, sl.np.as<int32_t>(), maxp, kf, nullptr, 0,
-                         sl.D.as<float>(), sl.I.as<int64_t>(), sl.work, sl.s);
+                         sl.D.as<float>(), sl.I.as<int64_t>(), sl.work, sl.s, kq);
   } else {
     const mvg_sq8 *h = sv->sq;
     mvg::Sq8SearchArgs a{};
@@ -216,6 +255,7 @@ void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t byt
     a.D = sl.D.as<float>();
     a.I = sl.I.as<int64_t>();
     a.use_mfma = h->use_mfma;
+    a.kq = kq;
     mvg::sq8_search(a, sl.swork, sl.s);
   }
   MVG_HIP(hipMemcpyAsync(sl.hD.ptr, sl.D.ptr, sizeof(float) * nq * k, hipMemcpyDeviceToHost, sl.s));
@@ -316,8 +356,13 @@ int mvg_server_create(mvg_ivfpq *ix, int slots, int max_batch_queries, int64_t m
     if (!ix) throw mvg::Error(MVG_ERR_INVALID, "server needs an index");
     if (!ix->trained || ix->ntotal == 0)
       throw mvg::Error(MVG_ERR_STATE, "server needs a trained, non-empty index");
-    mvg::ivf_prepare(ix);
-    return start_server(ix, nullptr, ix->d, slots, max_batch_queries, mem_budget_bytes, out);
+    {
+      std::lock_guard<std::mutex> lk(ix->search_mu);
+      mvg::ivf_prepare(ix);
+    }
+    const int rc = start_server(ix, nullptr, ix->d, slots, max_batch_queries, mem_budget_bytes, out);
+    if (rc == MVG_OK) ix->servers++;
+    return rc;
   } catch (const mvg::Error &e) {
     mvg::set_last_error(e.what());
     return e.code;
@@ -410,6 +455,7 @@ void mvg_server_destroy(mvg_server *sv) {
     (void)hipEventDestroy(sl->done);
     (void)hipStreamDestroy(sl->s);
   }
+  if (sv->ix) sv->ix->servers--;
   delete sv;
 }
 

@@ -468,10 +468,17 @@ IvfImage read_ivf(Reader &r) {
 // host append lists (uploaded as CSR on first search). Snapshots hold no FP32
 // rows, so a loaded index cannot rerank (k_factor 1 only) until rebuilt.
 mvg_ivfpq *install(const IvfImage &img) {
-  for (int64_t l = 0; l < img.nlist; l++)
+  int64_t entries = 0;
+  for (int64_t l = 0; l < img.nlist; l++) {
+    entries += (int64_t)img.ids[(size_t)l].size();
     for (int64_t id : img.ids[(size_t)l])
       if (id < 0 || id > INT32_MAX) coll_raise("IVF-PQ snapshot contains an invalid internal ID.");
+  }
   if (img.ntotal > INT32_MAX) coll_raise("IVF-PQ holds at most 2^31-1 vectors");
+  // Every add appends each vector to one list (index_ivf_pq.mojo:199-212), so
+  // the lists hold ntotal entries; a file that disagrees is not an index the
+  // reference wrote, and the per-entry device terms are sized by the lists.
+  if (entries != img.ntotal) coll_raise("IVF-PQ inverted lists do not hold ntotal entries.");
   mvg_ivfpq *raw = nullptr;
   mvg::check_index(mvg_ivfpq_create((int)img.d, (int)img.nlist, (int)img.M, img.metric, &raw));
   std::unique_ptr<mvg_ivfpq, void (*)(mvg_ivfpq *)> h(raw, mvg_ivfpq_destroy);

@@ -221,12 +221,13 @@ void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s) {
               a.excl, a.excl_len, w.part, nsplit, s);
     if (kf == 1) {
       launch_merge(w.part.as<uint64_t>(), a.nq, (int64_t)nsplit * kk, a.k, ip ? 1 : 0, 0.0f, a.D, a.I,
-                   nullptr, nullptr, s);
+                   nullptr, nullptr, s, nullptr, a.kq, 1);
     } else {
       w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
       launch_merge(w.part.as<uint64_t>(), a.nq, (int64_t)nsplit * kk, kk, 0, 0.0f, nullptr, nullptr,
-                   nullptr, w.keys.as<uint64_t>(), s);
-      rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.n, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s);
+                   nullptr, w.keys.as<uint64_t>(), s, nullptr, a.kq, kf);
+      rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.n, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s,
+             nullptr, RowMap{}, a.kq);
     }
     return;
   }
@@ -253,12 +254,13 @@ void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s) {
   const uint64_t *part = nchunks > 0 ? w.part.as<uint64_t>() : nullptr;
   if (kf == 1) {
     launch_merge(part, a.nq, (int64_t)nchunks * kk, a.k, ip ? 1 : 0, 0.0f, a.D, a.I, nullptr,
-                 nullptr, s);
+                 nullptr, s, nullptr, a.kq, 1);
   } else {
     w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
     launch_merge(part, a.nq, (int64_t)nchunks * kk, kk, 0, 0.0f, nullptr, nullptr, nullptr,
-                 w.keys.as<uint64_t>(), s);
-    rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.n, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s);
+                 w.keys.as<uint64_t>(), s, nullptr, a.kq, kf);
+    rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.n, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s,
+           nullptr, RowMap{}, a.kq);
   }
 }
 

@@ -111,6 +111,19 @@ class IndexFlat:
         _lib.check(_lib.load().mvg_flat_pair_distances(self._h, _ptr(a), _ptr(b), a.size, _ptr(out)))
         return out
 
+    def get_vector(self, i: int):
+        """IndexFlat.get_vector_span (index_flat.mojo:212-227): row i."""
+        out = np.empty(self.d, np.float32)
+        _lib.check(_lib.load().mvg_flat_get_vector(self._h, int(i), _ptr(out)))
+        return out
+
+    def get_vectors(self, ids):
+        """Stored rows ids[0..n) as [n, d]."""
+        ids = np.ascontiguousarray(ids, np.int64).reshape(-1)
+        out = np.empty((ids.size, self.d), np.float32)
+        _lib.check(_lib.load().mvg_flat_get_vectors(self._h, _ptr(ids), ids.size, _ptr(out)))
+        return out
+
 
 class IndexFlatSQ8:
     """Global-calibration SQ8 index (IndexFlatSQ8, index_flat_sq8.mojo:192-582)."""
@@ -173,6 +186,20 @@ class IndexFlatSQ8:
         _lib.check(_lib.load().mvg_sq8_pair_distances(self._h, _ptr(a), _ptr(b), a.size, _ptr(out)))
         return out
 
+    def get_vector(self, i: int):
+        """IndexFlatSQ8.get_vector_span (index_flat_sq8.mojo:567-582): the
+        retained FP32 row i."""
+        out = np.empty(self.d, np.float32)
+        _lib.check(_lib.load().mvg_sq8_get_vector(self._h, int(i), _ptr(out)))
+        return out
+
+    def get_vectors(self, ids):
+        """Retained FP32 rows ids[0..n) as [n, d]."""
+        ids = np.ascontiguousarray(ids, np.int64).reshape(-1)
+        out = np.empty((ids.size, self.d), np.float32)
+        _lib.check(_lib.load().mvg_sq8_get_vectors(self._h, _ptr(ids), ids.size, _ptr(out)))
+        return out
+
     def search(self, x, k: int, k_factor: int = 1, exclude=None):
         x = _f32(x, self.d)
         nq = x.shape[0]
@@ -231,8 +258,12 @@ class ProductQuantizer:
         return codes
 
     def decode(self, codes):
-        codes = np.asarray(codes, np.uint8).reshape(-1, self.M)
-        return np.concatenate([self.centroids[m][codes[:, m]] for m in range(self.M)], axis=1)
+        """ProductQuantizer.decode (pq.mojo:221-250) on the device."""
+        codes = np.ascontiguousarray(codes, np.uint8).reshape(-1, self.M)
+        out = np.empty((codes.shape[0], self.d), np.float32)
+        cb = np.ascontiguousarray(self.centroids, np.float32)
+        _lib.check(_lib.load().mvg_pq_decode(_ptr(codes), codes.shape[0], self.d, self.M, _ptr(cb), _ptr(out)))
+        return out
 
     def compute_distance_table(self, q, metric: int = METRIC_L2):
         q = _f32(q, self.d)
@@ -330,6 +361,15 @@ class IndexIVFPQ:
         x = _f32(x, self.d)
         _lib.check(_lib.load().mvg_ivfpq_add(self._h, _ptr(x), x.shape[0]))
 
+    def add_with_ids(self, x, ids) -> None:
+        """IndexIVFPQ.add_with_ids (index_ivf_pq.mojo:150-213): ids in
+        [0, 2^31 - 1], unique while FP32 rows are kept."""
+        x = _f32(x, self.d)
+        ids = np.ascontiguousarray(ids, np.int64).reshape(-1)
+        if ids.size != x.shape[0]:
+            raise ValueError("add_with_ids needs one id per vector")
+        _lib.check(_lib.load().mvg_ivfpq_add_with_ids(self._h, _ptr(x), x.shape[0], _ptr(ids)))
+
     def save(self, path) -> None:
         """write_index_ivf_pq (serialization.mojo:1029-1044) into a file."""
         _lib.check(_lib.load().mvg_ivfpq_save(self._h, os.fspath(path).encode()))
@@ -385,21 +425,25 @@ class IndexIVFPQ:
         return self.SCHEDULES[c.value], ms.value
 
     def capture(self, q_ptr: int, nq: int, k: int, D_ptr: int, I_ptr: int, k_factor: int = 1,
-                nprobe_ptr: int = 0, max_nprobe: int = 0) -> "SearchGraph":
-        """Record one device-pointer search of this batch shape as a HIP graph."""
+                nprobe_ptr: int = 0, max_nprobe: int = 0, kq_ptr: int = 0) -> "SearchGraph":
+        """Record one device-pointer search of this batch shape as a HIP graph.
+        kq_ptr (device int32 [nq]): a k per query, k being the largest (the
+        output row stride)."""
         g = ctypes.c_void_p()
-        _lib.check(_lib.load().mvg_ivfpq_capture(
-            self._h, ctypes.c_void_p(q_ptr), nq, k, ctypes.c_void_p(nprobe_ptr or None), max_nprobe,
-            k_factor, ctypes.c_void_p(D_ptr), ctypes.c_void_p(I_ptr), ctypes.byref(g)))
+        _lib.check(_lib.load().mvg_ivfpq_capture_k(
+            self._h, ctypes.c_void_p(q_ptr), nq, ctypes.c_void_p(kq_ptr or None), k,
+            ctypes.c_void_p(nprobe_ptr or None), max_nprobe, k_factor, ctypes.c_void_p(D_ptr),
+            ctypes.c_void_p(I_ptr), ctypes.byref(g)))
         return SearchGraph(g, self)
 
     def search_device(self, q_ptr: int, nq: int, k: int, D_ptr: int, I_ptr: int, k_factor: int = 1,
-                      nprobe_ptr: int = 0, max_nprobe: int = 0, stream: int = 0) -> None:
-        """Search with every buffer already in device memory (raw pointers)."""
-        _lib.check(_lib.load().mvg_ivfpq_search_device(
-            self._h, ctypes.c_void_p(q_ptr), nq, k, ctypes.c_void_p(nprobe_ptr or None), max_nprobe,
-            k_factor, None, 0, ctypes.c_void_p(D_ptr), ctypes.c_void_p(I_ptr),
-            ctypes.c_void_p(stream or None)))
+                      nprobe_ptr: int = 0, max_nprobe: int = 0, stream: int = 0, kq_ptr: int = 0) -> None:
+        """Search with every buffer already in device memory (raw pointers);
+        kq_ptr as in capture."""
+        _lib.check(_lib.load().mvg_ivfpq_search_device_k(
+            self._h, ctypes.c_void_p(q_ptr), nq, ctypes.c_void_p(kq_ptr or None), k,
+            ctypes.c_void_p(nprobe_ptr or None), max_nprobe, k_factor, None, 0, ctypes.c_void_p(D_ptr),
+            ctypes.c_void_p(I_ptr), ctypes.c_void_p(stream or None)))
 
 
 class IndexIVFFlat:

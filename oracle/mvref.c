@@ -842,7 +842,7 @@ struct mvref_ivfpq {
   int64_t *lsize, *lcap;
   int64_t **lids;
   uint8_t **lcodes;
-  float *vectors; /* ntotal x d, kept for the exact rerank */
+  float *vectors; /* FP32 rows by id (vcap rows), kept for the exact rerank */
   int64_t vcap;
 };
 
@@ -923,21 +923,24 @@ void mvref_ivfpq_train(mvref_ivfpq *h, const float *x, int64_t n,
   h->trained = 1;
 }
 
-/* add / add_with_ids: index_ivf_pq.mojo:138-213 (ids ntotal+i, append). */
-void mvref_ivfpq_add(mvref_ivfpq *h, const float *x, int64_t n,
-                     int keep_vectors, int nthreads) {
-  if (!h->trained || n == 0) return;
+/* add / add_with_ids: index_ivf_pq.mojo:138-213. ids NULL: ntotal+i (add,
+ * :146-147); otherwise the caller's ids, appended in input order. The kept
+ * FP32 rows (the k_factor rerank's input, an addition of this build) are
+ * stored by id, so the rerank reads the row added under a candidate's id. */
+void mvref_ivfpq_add_ids(mvref_ivfpq *h, const float *x, int64_t n, const int64_t *ids,
+                         int keep_vectors, int nthreads) {
+  if (!h->trained || n == 0) return; /* :161 */
   int d = h->d, M = h->M;
   int32_t *lab = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
   ivf_assign(h, x, n, lab, nthreads);
   float *res = (float *)malloc(sizeof(float) * (size_t)n * d);
-  for (int64_t i = 0; i < n; i++) {
+  for (int64_t i = 0; i < n; i++) { /* :179-192 residual vs the assigned centroid */
     int l = lab[i] < 0 || lab[i] >= h->nlist ? 0 : lab[i];
     for (int j = 0; j < d; j++) res[i * d + j] = x[i * d + j] - h->coarse[(int64_t)l * d + j];
   }
   uint8_t *codes = (uint8_t *)malloc((size_t)n * M);
   mvref_pq_encode(res, n, d, M, h->pq, codes, nthreads);
-  for (int64_t i = 0; i < n; i++) {
+  for (int64_t i = 0; i < n; i++) { /* :199-211 */
     int l = lab[i];
     if (l < 0 || l >= h->nlist) continue;
     if (h->lsize[l] == h->lcap[l]) {
@@ -946,20 +949,33 @@ void mvref_ivfpq_add(mvref_ivfpq *h, const float *x, int64_t n,
       h->lcodes[l] = (uint8_t *)realloc(h->lcodes[l], (size_t)nc * M);
       h->lcap[l] = nc;
     }
-    h->lids[l][h->lsize[l]] = h->ntotal + i;
+    h->lids[l][h->lsize[l]] = ids ? ids[i] : h->ntotal + i;
     memcpy(h->lcodes[l] + h->lsize[l] * M, codes + i * M, (size_t)M);
     h->lsize[l]++;
   }
   if (keep_vectors) {
-    if (h->ntotal + n > h->vcap) {
-      int64_t nc = h->vcap * 2 > h->ntotal + n ? h->vcap * 2 : h->ntotal + n;
+    int64_t top = h->ntotal + n; /* rows needed: one past the largest id */
+    if (ids)
+      for (int64_t i = 0; i < n; i++)
+        if (ids[i] + 1 > top) top = ids[i] + 1;
+    if (top > h->vcap) {
+      int64_t nc = h->vcap * 2 > top ? h->vcap * 2 : top;
       h->vectors = (float *)realloc(h->vectors, sizeof(float) * (size_t)nc * d);
+      memset(h->vectors + h->vcap * d, 0, sizeof(float) * (size_t)(nc - h->vcap) * d);
       h->vcap = nc;
     }
-    memcpy(h->vectors + h->ntotal * d, x, sizeof(float) * (size_t)n * d);
+    for (int64_t i = 0; i < n; i++) {
+      int64_t id = ids ? ids[i] : h->ntotal + i;
+      memcpy(h->vectors + id * d, x + i * d, sizeof(float) * (size_t)d);
+    }
   }
-  h->ntotal += n;
+  h->ntotal += n; /* :213 */
   free(codes); free(res); free(lab);
+}
+
+void mvref_ivfpq_add(mvref_ivfpq *h, const float *x, int64_t n,
+                     int keep_vectors, int nthreads) {
+  mvref_ivfpq_add_ids(h, x, n, NULL, keep_vectors, nthreads);
 }
 
 /* Installs inverted lists built elsewhere (CSR: list_off[nlist+1], ids and

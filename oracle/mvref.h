@@ -116,6 +116,8 @@ void mvref_ivfpq_train(mvref_ivfpq *h, const float *x, int64_t n,
                        int nthreads);
 void mvref_ivfpq_get_codebooks(const mvref_ivfpq *h, float *coarse,
                                float *pq);
+void mvref_ivfpq_add_ids(mvref_ivfpq *h, const float *x, int64_t n, const int64_t *ids,
+                         int keep_vectors, int nthreads);
 void mvref_ivfpq_add(mvref_ivfpq *h, const float *x, int64_t n,
                      int keep_vectors, int nthreads);
 void mvref_ivfpq_load_lists(mvref_ivfpq *h, const int64_t *list_off, const int32_t *ids,

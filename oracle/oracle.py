@@ -82,6 +82,7 @@ def lib():
             "mvref_ivfpq_train": (None, [_vp, _f32p, _i64, _int, _int, ctypes.c_uint64, _int]),
             "mvref_ivfpq_get_codebooks": (None, [_vp, _f32p, _f32p]),
             "mvref_ivfpq_add": (None, [_vp, _f32p, _i64, _int, _int]),
+            "mvref_ivfpq_add_ids": (None, [_vp, _f32p, _i64, _vp, _int, _int]),
             "mvref_ivfpq_load_lists": (None, [_vp, _i64p, _i32p, _u8p, _vp, _i64]),
             "mvref_ivfpq_list_size": (_i64, [_vp, _int]),
             "mvref_ivfpq_list": (None, [_vp, _int, _i64p, _u8p]),
@@ -397,6 +398,12 @@ class IVFPQIndex:
     def add(self, x, keep_vectors=True, nthreads=0):
         x = _f32(x).reshape(-1, self.d)
         lib().mvref_ivfpq_add(self.h, x, x.shape[0], int(keep_vectors), nthreads)
+
+    def add_with_ids(self, x, ids, keep_vectors=True, nthreads=0):
+        """IndexIVFPQ.add_with_ids (index_ivf_pq.mojo:150-213)."""
+        x = _f32(x).reshape(-1, self.d)
+        ids = np.ascontiguousarray(ids, np.int64).reshape(-1)
+        lib().mvref_ivfpq_add_ids(self.h, x, x.shape[0], _ptr(ids), int(keep_vectors), nthreads)
 
     def load_lists(self, list_off, ids, codes, vectors=None):
         list_off = np.ascontiguousarray(list_off, np.int64)

@@ -87,6 +87,19 @@ def test_null_handles_are_rejected_before_any_device_work():
         assert b"server needs an index" in lib.mvg_last_error()
     assert lib.mvg_ivfpq_set_prefilter(None, 8) != 0
     assert lib.mvg_last_error()
+    # the later entry points check their handle first as well
+    assert lib.mvg_ivfpq_add_with_ids(None, None, 1, None) != 0
+    assert lib.mvg_flat_get_vectors(None, None, 0, None) != 0
+    assert lib.mvg_sq8_get_vector(None, 0, None) != 0
+    assert lib.mvg_ivfpq_search_device_k(None, None, 1, None, 1, None, 1, 1, None, 0, None, None,
+                                         None) != 0
+    assert lib.mvg_ivfpq_capture_k(None, None, 1, None, 1, None, 1, 1, None, None,
+                                   ctypes.byref(out)) != 0
+    assert lib.mvg_graph_launch(None, None) != 0
+    # PQ decode validates its shape before touching a device; n = 0 is a no-op
+    assert lib.mvg_pq_decode(None, 0, 8, 2, None, None) == 0
+    assert lib.mvg_pq_decode(None, 4, 8, 3, None, None) != 0
+    assert b"PQ M must divide dimension" in lib.mvg_last_error()
 
 
 def test_no_oracle_in_product_package():

