@@ -206,6 +206,23 @@ int mvg_ivfpq_search_device_k(mvg_ivfpq *h, const float *q, int64_t nq, const in
                               int max_k, const int32_t *nprobe_per_query, int max_nprobe,
                               int k_factor, const uint8_t *exclude, int64_t exclude_len,
                               float *distances, int64_t *labels, void *stream);
+/* Allocation-free form of mvg_ivfpq_search_device. No reference
+ * counterpart: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results; its workspace (probe lists, candidate keys,
+ * pools) comes from one fixed 64 MiB device array in the library, one per
+ * GPU, held until the process ends, and the call makes no device allocation
+ * or free. A request whose workspace exceeds 64 MiB returns MVG_ERR_INVALID
+ * (checked before anything is enqueued for the probe lists and candidate
+ * keys, then piece by piece; results are then not written). The index must
+ * be prepared: after a change, run one mvg_ivfpq_search or
+ * mvg_ivfpq_search_device first, else MVG_ERR_STATE. Every call on one GPU
+ * shares the array: calls that overlap on a GPU (other threads or streams)
+ * race. */
+int mvg_ivfpq_search_device_noalloc(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                                    const int32_t *nprobe_per_query, int max_nprobe, int k_factor,
+                                    const uint8_t *exclude, int64_t exclude_len, float *distances,
+                                    int64_t *labels, void *stream);
 /* Search form (no reference counterpart). precision 0: plain FP32, the
  * reference's residual-LUT arithmetic in its operation order (bit-exact ids);
  * 1: mixed, L2 tables decomposed as |q - c_l|^2 + T[l] + Q[q] (precomputed

@@ -871,6 +871,74 @@ int mvg_ivfpq_search_device_k(mvg_ivfpq *h, const float *q, int64_t nq, const in
 
 }  // extern "C"
 
+// ---- allocation-free search ---------------------------------------------------
+
+namespace {
+
+// Workspace of mvg_ivfpq_search_device_noalloc: one fixed array per GPU,
+// held until the process ends, shared by every call on that GPU.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+std::vector<char *> g_scratch_ptr;  // device address of `scratch`, per device
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current (the host symbol is only a stand-in, not an address).
+char *scratch_base() {
+  int dev = 0;
+  MVG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_ptr[(size_t)dev]) {
+    void *p = nullptr;
+    MVG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(mvg::kDevice, "scratch array has no device address");
+    g_scratch_ptr[(size_t)dev] = static_cast<char *>(p);
+  }
+  return g_scratch_ptr[(size_t)dev];
+}
+
+}  // namespace
+
+extern "C" int mvg_ivfpq_search_device_noalloc(mvg_ivfpq *h, const float *q, int64_t nq, int k,
+                                               const int32_t *nprobe_per_query, int max_nprobe,
+                                               int k_factor, const uint8_t *exclude,
+                                               int64_t exclude_len, float *distances,
+                                               int64_t *labels, void *stream) {
+  API_BEGIN
+  require(h && nq >= 0 && k >= 1 && k_factor >= 1, "invalid arguments");
+  require(h->trained && h->ntotal > 0, "search_device needs a trained, non-empty index");
+  hipStream_t s = stream ? (hipStream_t)stream : h->s;
+  std::lock_guard<std::mutex> lk(h->search_mu);
+  // Preparing the index (CSR upload, mixed-form tables) allocates: it must
+  // have happened already, through any other search of the current state.
+  const bool mixed = h->precision != 0;
+  if (h->dirty || h->row_of_dirty || (mixed && h->tables_dirty) ||
+      (mixed && h->metric == MVG_METRIC_L2 && h->pre_dirty) ||
+      (mixed && h->prefilter > 0 && h->keep_vectors && h->vq_dirty))
+    throw Error(mvg::kState,
+                "the index is not prepared: run one search (mvg_ivfpq_search or "
+                "mvg_ivfpq_search_device) after the last change before the allocation-free one");
+  // The first pieces every search takes (probe lists, candidate keys) checked
+  // before anything is enqueued; the rest is checked piece by piece.
+  const int maxp = std::min(nprobe_per_query ? max_nprobe : h->nprobe, h->nlist);
+  const size_t first = sizeof(int32_t) * (size_t)nq * (size_t)std::max(maxp, 1) +
+                       (k_factor > 1 ? sizeof(uint64_t) * (size_t)nq * k * k_factor : 0);
+  if (first > kScratchBytes)
+    throw Error(mvg::kInvalid, "the request needs more than the " + std::to_string(kScratchBytes >> 20) +
+                                   " MiB scratch of the allocation-free entry point");
+  if (nq == 0) return MVG_OK;
+  mvg::Arena arena;
+  arena.base = scratch_base();
+  arena.size = kScratchBytes;
+  mvg::IvfWork work;
+  work.bind(&arena);
+  mvg::ivf_search_with(h, q, nq, k, nprobe_per_query, max_nprobe, k_factor, exclude, exclude_len,
+                       distances, labels, work, s);
+  API_END
+}
+
 void mvg::ivf_prepare(mvg_ivfpq *h) {
   ivf_sync_lists(h);
   if (h->row_of_dirty) {

@@ -38,16 +38,34 @@ inline void hip_check(hipError_t e, const char *what) {
 #define MVG_HIP(call) ::mvg::hip_check((call), #call)
 #define MVG_LAUNCH_CHECK(name) ::mvg::hip_check(hipGetLastError(), name)
 
-// Grow-only device buffer. Contents are not preserved by reserve().
+// Fixed device region handed out front to back (256-byte aligned pieces);
+// nothing is ever freed back. Lets a search run with no device allocation.
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+  void *take(size_t n) {
+    const size_t at = (used + 255) & ~size_t(255);
+    if (at > size || n > size - at)
+      throw Error(kInvalid, "the request needs more than the " + std::to_string(size >> 20) +
+                                " MiB scratch of the allocation-free entry point");
+    used = at + n;
+    return base + at;
+  }
+  size_t left() const { return used < size ? size - used : 0; }
+};
+
+// Grow-only device buffer. Contents are not preserved by reserve(). Bound
+// to an arena, it takes its memory from there (no hipMalloc / hipFree).
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
+  Arena *arena = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !arena) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
   }
@@ -55,12 +73,17 @@ struct DevBuf {
     if (n <= bytes) return;
     release();
     if (n == 0) return;
-    MVG_HIP(hipMalloc(&ptr, n));
+    if (arena) {
+      ptr = arena->take(n);
+    } else {
+      MVG_HIP(hipMalloc(&ptr, n));
+    }
     bytes = n;
   }
   // Grow while keeping the first `keep` bytes.
   void grow_keep(size_t n, size_t keep, hipStream_t s) {
     if (n <= bytes) return;
+    if (arena) throw Error(kInvalid, "arena buffers cannot grow in place");
     void *np = nullptr;
     MVG_HIP(hipMalloc(&np, n));
     if (ptr && keep) MVG_HIP(hipMemcpyAsync(np, ptr, keep, hipMemcpyDeviceToDevice, s));

@@ -1919,6 +1919,17 @@ int64_t persistent_grid(int per_cu, int64_t limit) {
   return std::max<int64_t>(kQueryRanges, g / kQueryRanges * kQueryRanges);
 }
 
+// Most persistent workgroups whose pool segments of `cap` words fit: 4 GiB
+// of segments, or half of what is left of a fixed workspace (the query
+// order and the rerank take their buffers after the segments). A workgroup
+// pulls queries from the shared order, so fewer of them change only the
+// schedule, not the results.
+int64_t segment_grid_limit(const DevBuf &seg, int64_t cap) {
+  int64_t limit = (int64_t(4) << 30) / (8 * cap);
+  if (seg.arena) limit = std::min<int64_t>(limit, (int64_t)(seg.arena->left() / 2) / (8 * cap));
+  return limit;
+}
+
 template <int MF, bool IP, bool MIXED, int DF>
 void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                     uint64_t *out_keys, IvfWork &w, hipStream_t s) {
@@ -1931,8 +1942,7 @@ void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int
   int per_cu = 0;
   MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
   const int64_t cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);  // 16-byte aligned segments
-  // at most 4 GiB of segments of 8-byte (key, entry) words
-  const int64_t grid = persistent_grid(per_cu, (int64_t(4) << 30) / (8 * cap));
+  const int64_t grid = persistent_grid(per_cu, segment_grid_limit(w.deep_seg, cap));
   w.deep_seg.reserve(sizeof(uint64_t) * (size_t)(grid * cap));
   const int32_t *order = build_query_order(a, maxp, probes, w, s);
   unsigned *counter = w.deep_counter.as<unsigned>();
@@ -1963,8 +1973,7 @@ bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int 
     int per_cu = 0;
     MVG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kNT, sm));
     cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);  // 16-byte aligned segments
-    // at most 4 GiB of segments of 8-byte (key, entry) words
-    grid = persistent_grid(per_cu, (int64_t(4) << 30) / (8 * cap));
+    grid = persistent_grid(per_cu, segment_grid_limit(w.deep_seg, cap));
     w.deep_seg.reserve(sizeof(uint64_t) * (size_t)(grid * cap));
     seg = w.deep_seg.as<uint32_t>();
     order = build_query_order(a, maxp, probes, w, s);

@@ -122,6 +122,13 @@ struct IvfWork {
   DevBuf vq_qcodes, vq_qnorms, vq_keys;  // mixed pre-rerank: query codes, narrowed candidates
   bool ord_ready = false;          // `ord` holds this search's order
   IvfProfile prof;
+  // Every workspace buffer takes its memory from `a` (before any reserve).
+  void bind(Arena *a) {
+    for (DevBuf *b : {&coarse_part, &probes, &part, &keys, &keys2, &qbf, &qnorm, &wq_poff, &wq_nitems,
+                      &wq_rq, &wq_off, &wq_gthr, &wq_counter, &deep_seg, &deep_counter, &ord_cnt,
+                      &ord_off, &ord, &vq_qcodes, &vq_qnorms, &vq_keys, &prof.counter})
+      b->arena = a;
+  }
 };
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
 
