@@ -91,6 +91,23 @@ def recall_at(torch, gt, ids):
     return float(hit.mean() / gt.shape[1])
 
 
+def quality(torch, x, q, D, I, gt, metric=0):
+    """Each result against FP64: recall@k and its loss (1 - recall) against
+    the FP64 brute-force ids, and the largest absolute and relative error of
+    a returned distance against the FP64 distance of the returned row."""
+    ids = I.clamp(min=0)
+    xs = x[ids.reshape(-1)].double().reshape(ids.shape[0], ids.shape[1], -1)
+    qq = q.double()[:, None, :]
+    exact = ((xs - qq) ** 2).sum(-1) if metric == 0 else (xs * qq).sum(-1)
+    ok = I >= 0
+    err = (D.double() - exact).abs()[ok]
+    rel = err / exact.abs()[ok].clamp(min=1e-30 if metric == 1 else 1.0)
+    rec = recall_at(torch, gt, I)
+    return {"recall": round(rec, 4), "recall_loss_vs_fp64": round(1.0 - rec, 4),
+            "max_dist_err_vs_fp64": float(err.max()) if err.numel() else 0.0,
+            "max_rel_dist_err_vs_fp64": float(rel.max()) if rel.numel() else 0.0}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +124,8 @@ def main(argv=None):
                          "kernel that scales FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--no-flat", action="store_true",
                     help="skip the 1M x 128 exact FP32 Flat vs SQ8 (int8 MFMA) leg")
+    ap.add_argument("--no-sq8", action="store_true",
+                    help="skip the SQ8 cosine 1M x 768 (int8 MFMA + FP32 rerank) leg")
     ap.add_argument("--workload", default="ivfpq",
                     choices=["ivfpq", "flat", "sq8", "batch", "serve", "serve_sq8", "ivfflat",
                              "pairdist"],
@@ -210,6 +229,19 @@ def main(argv=None):
         torch.cuda.synchronize()
         return time.perf_counter() - t0, recall_at(torch, gt, Id)
 
+    q_h = q.cpu().numpy()
+
+    def host_qps(form, nprobe, kf, reps=3):
+        """The same search through the host-buffer entry point: the 5 MB of
+        queries and 0.8 MB of results cross PCIe inside each call."""
+        ix.set_search_mode(form)
+        ix.nprobe = nprobe
+        ix.search(q_h, K, k_factor=kf)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ix.search(q_h, K, k_factor=kf)
+        return round(NQ * reps / (time.perf_counter() - t0), 1)
+
     def profiled(form, nprobe, kf, reps=5):
         """Stage times from HIP events on the search stream: coarse, scan,
         merge + rerank; the list entries the scan visits and the keys the
@@ -235,8 +267,12 @@ def main(argv=None):
     log(f"timing {head} nprobe={nprobe} k_factor={kf}")
     elapsed, rec = timed(head, nprobe, kf)
     qps = NQ * args.steps / elapsed
+    qual = quality(torch, x, q, Dd, Id, gt)
     o_elapsed, o_rec = timed(other, best[other]["nprobe"], best[other]["k_factor"])
     o_qps = NQ * args.steps / o_elapsed
+    o_qual = quality(torch, x, q, Dd, Id, gt)
+    h_qps = host_qps(head, nprobe, kf)
+    o_h_qps = host_qps(other, best[other]["nprobe"], best[other]["k_factor"])
     stages, entries = profiled(head, nprobe, kf)
     o_stages, _ = profiled(other, best[other]["nprobe"], best[other]["k_factor"])
     scan_s = stages["scan_ms"] / 1e3
@@ -244,15 +280,22 @@ def main(argv=None):
     achieved = bytes_per_launch / scan_s / 1e9
     log(f"{head} qps={qps:.0f} recall={rec:.4f} stages={stages} scan {achieved:.0f} GB/s; "
         f"{other} qps={o_qps:.0f} recall={o_rec:.4f} stages={o_stages}")
-    forms = {head: {**best[head], "qps": round(qps, 1), "recall": round(rec, 4), "stages": stages},
-             other: {**best[other], "qps": round(o_qps, 1), "recall": round(o_rec, 4),
+    forms = {head: {**best[head], "qps": round(qps, 1), "qps_host_buffers": h_qps, **qual,
+                    "stages": stages},
+             other: {**best[other], "qps": round(o_qps, 1), "qps_host_buffers": o_h_qps, **o_qual,
                      "stages": o_stages}}
     plain_qps = forms["plain"]["qps"]
     ratio = forms["mixed"]["qps"] / plain_qps if forms["mixed"]["recall"] >= RECALL_BAR else None
 
     cpu = None if args.no_cpu else cpu_baseline(
-        ix, x_h, q.cpu().numpy(), best["plain"]["nprobe"], best["plain"]["k_factor"], args.cpu_seconds)
-    flat = None if args.no_flat else flat_1m(mv, torch, x_h, q.cpu().numpy(), gt, args)
+        ix, x_h, q_h, best["plain"]["nprobe"], best["plain"]["k_factor"], args.cpu_seconds)
+    flat = None if args.no_flat else flat_1m(mv, torch, x_h, q_h, gt, args)
+    del ix
+    sq8 = None
+    if not args.no_sq8:
+        del x, sample
+        torch.cuda.empty_cache()
+        sq8 = sq8_leg(mv, torch, args)
 
     line = {
         "metric": "QPS@recall10\u22650.95, 1M\u00d7128 IVF-PQ/SQ8: mixed vs FP32 GPU; %HBM roofline",
@@ -276,9 +319,17 @@ def main(argv=None):
             "k_factor": kf,
             "recall_at_10": round(rec, 4),
             "recall_bar": RECALL_BAR,
+            "recall_loss_vs_fp64": qual["recall_loss_vs_fp64"],
+            "max_dist_err_vs_fp64": qual["max_dist_err_vs_fp64"],
+            "qps_definition": "value: queries / wall time of device-pointer searches of the 10k "
+                              "batch with the index and the queries resident in HBM (the "
+                              "host-to-device copy of the queries is outside the timed region); "
+                              "forms.*.qps_host_buffers: the same search through the host-buffer "
+                              "entry point, PCIe copies of queries and results included",
             "forms": forms,
             "mixed_over_plain": None if ratio is None else round(ratio, 3),
             "flat_1Mx128": flat,
+            "sq8_cosine_1Mx768": sq8,
             "train_s": round(t_train, 3),
             "add_s": round(t_add, 3),
             "sweep": sweeps,
@@ -474,9 +525,23 @@ def bench_serve_sq8(args):
     return 0
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(ix, x_h, q_h, nprobe, kf, budget_s):
-    """CPU oracle search (C restatement of the reference IVF-PQ search,
-    parallel over queries) on the same index, a bounded query sample."""
+    """CPU oracle search (C restatement of the reference IVF-PQ search) on the
+    same index and settings, timed on the host cores over bounded query
+    samples, three ways: parallel over queries on every host thread (the
+    strongest CPU form; `value`), the reference's own threading (coarse step
+    parallel, LUT + ADC loop serial, index_ivf_pq.mojo:256-276) and one
+    thread."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import oracle as o
 
@@ -494,17 +559,74 @@ def cpu_baseline(ix, x_h, q_h, nprobe, kf, budget_s):
         ids[off[l]:off[l + 1]] = li
         codes[off[l]:off[l + 1]] = lc
     oi.load_lists(off, ids, codes, x_h if kf > 1 else None)
-    n = 64
-    t0 = time.perf_counter()
-    oi.search(q_h[:n], K, nprobe, k_factor=kf, nthreads=threads)
-    dt = time.perf_counter() - t0
-    n = int(min(NQ, max(n, n * budget_s / max(dt, 1e-6))))
-    log(f"cpu baseline: {n} queries on {threads} threads")
-    t0 = time.perf_counter()
-    oi.search(q_h[:n], K, nprobe, k_factor=kf, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": round(n / dt, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"{n} of the 10,000 queries, same index, nprobe={nprobe}, k_factor={kf}"}
+
+    def rate(fn, budget, n0):
+        """Queries/s of fn(n) on a sample sized to about `budget` seconds."""
+        t0 = time.perf_counter()
+        fn(n0)
+        dt = time.perf_counter() - t0
+        n = int(min(NQ, max(n0, n0 * budget / max(dt, 1e-6))))
+        t0 = time.perf_counter()
+        fn(n)
+        return n, n / (time.perf_counter() - t0)
+
+    n_par, v_par = rate(lambda n: oi.search(q_h[:n], K, nprobe, k_factor=kf, nthreads=threads),
+                        budget_s, 64)
+    log(f"cpu baseline: {n_par} queries on {threads} threads: {v_par:.0f}/s")
+    n_ref, v_ref = rate(lambda n: oi.search_refstruct(q_h[:n], K, nprobe, k_factor=kf, nthreads=threads),
+                        budget_s / 3, 8)
+    log(f"cpu baseline, reference threading: {n_ref} queries: {v_ref:.0f}/s")
+    n_one, v_one = rate(lambda n: oi.search(q_h[:n], K, nprobe, k_factor=kf, nthreads=1), budget_s / 3, 8)
+    log(f"cpu baseline, 1 thread: {n_one} queries: {v_one:.0f}/s")
+    return {"value": round(v_par, 1), "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"{n_par} of the 10,000 queries, same index, nprobe={nprobe}, k_factor={kf}",
+            "cpu_model": cpu_model(),
+            "variants": {
+                "per_query_parallel": {"qps": round(v_par, 1), "threads": threads, "queries": n_par},
+                "reference_threading": {"qps": round(v_ref, 1), "threads": threads, "queries": n_ref,
+                                        "note": "coarse step parallel over queries, LUT + ADC loop "
+                                                "serial (index_ivf_pq.mojo:256-276)"},
+                "single_thread": {"qps": round(v_one, 1), "threads": 1, "queries": n_one}}}
+
+
+def sq8_leg(mv, torch, args):
+    """SQ8 cosine on 1M x 768 unit-norm embeddings (the sq8 workload's data):
+    int8 codes contracted on int8 MFMA, the exact FP32 rerank of k * 2
+    candidates, 10,000 queries, k = 10, host buffers (PCIe included);
+    recall and distance error against FP64 brute force."""
+    dev = torch.device("cuda:0")
+    x, q = sq8_data(torch, dev)
+    n, nq, d, k = x.shape[0], q.shape[0], x.shape[1], 10
+    x64 = x.double()
+    gt = []
+    for s0 in range(0, nq, 500):
+        gt.append(torch.topk(q[s0:s0 + 500].double() @ x64.T, k, dim=1)[1])
+    del x64
+    gt = torch.cat(gt)
+    x_h, q_h = x.cpu().numpy(), q.cpu().numpy()
+    ix = mv.IndexFlatSQ8(d, 1)
+    ix.add(x_h)
+    out = {"workload": "SQ8 cosine (symmetric int8 codes, int8 MFMA) + exact FP32 rerank, "
+                       "1,000,000 x 768 unit-norm, 10,000 queries, k=10, host buffers"}
+    for kf in (1, 2):
+        res = {}
+
+        def run():
+            res["r"] = ix.search(q_h, k, k_factor=kf)
+
+        dt = _timed(torch, run, 3, 1)
+        Dt = torch.from_numpy(res["r"][0]).to(dev)
+        It = torch.from_numpy(res["r"][1]).to(dev)
+        qa = quality(torch, x, q, Dt, It, gt, metric=1)
+        if kf == 1:  # SQ8 code distances, not FP32 ones
+            qa.pop("max_dist_err_vs_fp64"), qa.pop("max_rel_dist_err_vs_fp64")
+        tops = 2.0 * n * nq * d / dt / 1e12
+        out[f"k_factor_{kf}"] = {"qps": round(nq / dt, 1), "ms": round(dt * 1e3, 2), **qa,
+                                 "achieved_tops": round(tops, 1), "peak_tops": 5000.0,
+                                 "frac": round(tops / 5000.0, 4)}
+        log(f"sq8 cosine 1M x 768 kf={kf}: {out[f'k_factor_{kf}']}")
+    del ix
+    return out
 
 
 FP32_VALU_PEAK = 78.6  # TFLOP/s: MI355X FP32 FMA without packed math (157.3 with it)
@@ -810,8 +932,10 @@ def bench_batch(args):
     """Skewed batch: IVF-PQ on 10M x 96 integer-valued mixture (20,000
     clusters, Zipf 1.2), nlist 16,384 (k-means 25 iterations on 655,360
     rows), PQ 12 x 8; 50,000 queries, each with nprobe ~ U{4..128} and
-    k ~ {1, 10, 100}. Queries are grouped by k; each group's schedule is
-    autotuned, then the group is captured in a HIP graph and replayed."""
+    k ~ {1, 10, 100}. The whole batch is one search with a k per query: its
+    scan schedule is autotuned, then it is captured in one HIP graph and
+    replayed; k_factor 1 (ADC distances) and 4 (exact FP32 rerank of each
+    query's 4k best)."""
     import torch
 
     import mojovec_amd as mv
@@ -851,52 +975,59 @@ def bench_batch(args):
     sizes = ix.list_sizes()
     rng = np.random.default_rng(32)
     nprobe = rng.integers(4, 129, nq).astype(np.int32)
-    kq = rng.choice(np.array([1, 10, 100]), nq)
-    L = _lib.load()
-    total_s, groups = 0.0, {}
-    for k in (1, 10, 100):
-        sel = np.nonzero(kq == k)[0]
-        qg = q[torch.from_numpy(sel).to(dev)].contiguous()
-        npg = torch.from_numpy(nprobe[sel]).to(dev)
-        ng = len(sel)
-        D = torch.empty((ng, k), dtype=torch.float32, device=dev)
-        I = torch.empty((ng, k), dtype=torch.int64, device=dev)
-        sched, ms = ix.autotune(qg.data_ptr(), ng, k, D.data_ptr(), I.data_ptr(), k_factor=1,
-                                nprobe_ptr=npg.data_ptr(), max_nprobe=128)
-        graph = ix.capture(qg.data_ptr(), ng, k, D.data_ptr(), I.data_ptr(), k_factor=1,
-                           nprobe_ptr=npg.data_ptr(), max_nprobe=128)
+    kq = rng.choice(np.array([1, 10, 100]), nq).astype(np.int32)
+    # exact top-10 of 500 queries per k group (FP32 brute force on the GPU;
+    # integer data, so exact), for recall@min(k, 10)
+    sample = {int(k): np.nonzero(kq == k)[0][:500] for k in (1, 10, 100)}
+
+    def exact_top(qq, k):
+        acc = None
+        for xs in range(0, n, 2_000_000):
+            xx = x[xs:xs + 2_000_000]
+            d2 = (qq * qq).sum(1)[:, None] + (xx * xx).sum(1)[None, :] - 2.0 * (qq @ xx.T)
+            v, i = torch.topk(d2, k, dim=1, largest=False)
+            i = i + xs
+            if acc is None:
+                acc = (v, i)
+            else:
+                vv, ii = torch.cat([acc[0], v], 1), torch.cat([acc[1], i], 1)
+                o = torch.argsort(vv, 1)[:, :k]
+                acc = (torch.gather(vv, 1, o), torch.gather(ii, 1, o))
+        return acc[1]
+
+    gts = {}
+    for k, sel in sample.items():
+        qs = q[torch.from_numpy(sel).to(dev)]
+        gts[k] = torch.cat([exact_top(qs[s0:s0 + 100], min(k, 10)) for s0 in range(0, len(sel), 100)])
+    # one batch: every query with its own nprobe and k, one autotuned
+    # schedule, one captured graph (k_per_query, output rows of stride 100)
+    qd = q.contiguous()
+    npd = torch.from_numpy(nprobe).to(dev)
+    kqd = torch.from_numpy(kq).to(dev)
+    kmax = 100
+    Dd = torch.empty((nq, kmax), dtype=torch.float32, device=dev)
+    Id = torch.empty((nq, kmax), dtype=torch.int64, device=dev)
+    runs = {}
+    for kf in (1, 4):
+        sched, ms = ix.autotune(qd.data_ptr(), nq, kmax, Dd.data_ptr(), Id.data_ptr(), k_factor=kf,
+                                nprobe_ptr=npd.data_ptr(), max_nprobe=128)
+        graph = ix.capture(qd.data_ptr(), nq, kmax, Dd.data_ptr(), Id.data_ptr(), k_factor=kf,
+                           nprobe_ptr=npd.data_ptr(), max_nprobe=128, kq_ptr=kqd.data_ptr())
         dt = _timed(torch, graph.launch, max(2, args.steps // 4), 1)
-        total_s += dt
-        # recall@min(k,10) on 500 queries of the group against exact search
-        sub = qg[:500]
-        best = []
-        for s0 in range(0, sub.shape[0], 100):
-            qq = sub[s0:s0 + 100]
-            acc = None
-            for xs in range(0, n, 2_000_000):
-                xx = x[xs:xs + 2_000_000]
-                d2 = (qq * qq).sum(1)[:, None] + (xx * xx).sum(1)[None, :] - 2.0 * (qq @ xx.T)
-                v, i = torch.topk(d2, k, dim=1, largest=False)
-                i = i + xs
-                if acc is None:
-                    acc = (v, i)
-                else:
-                    vv = torch.cat([acc[0], v], 1)
-                    ii = torch.cat([acc[1], i], 1)
-                    o = torch.argsort(vv, 1)[:, :k]
-                    acc = (torch.gather(vv, 1, o), torch.gather(ii, 1, o))
-            best.append(acc[1])
-        gt = torch.cat(best)
-        rec = recall_at(torch, gt, I[:500])
-        groups[str(k)] = {"queries": ng, "ms": round(dt * 1e3, 3), "qps": round(ng / dt, 1),
-                          "schedule": sched, "recall_at_k": round(rec, 4)}
-        log(f"batch k={k}: {groups[str(k)]}")
+        del graph
+        rec = {}
+        for k, sel in sample.items():
+            got = Id[torch.from_numpy(sel).to(dev)][:, :min(k, 10)]
+            rec[str(k)] = round(recall_at(torch, gts[k], got), 4)
+        runs[f"k_factor_{kf}"] = {"qps": round(nq / dt, 1), "ms": round(dt * 1e3, 3), "schedule": sched,
+                                  "recall_at_min_k_10": rec}
+        log(f"batch k_factor={kf}: {runs[f'k_factor_{kf}']}")
     print(json.dumps({"workload": "IVF-PQ 10M x 96, nlist 16384, PQ 12x8, 50,000 queries, nprobe "
-                                  "U{4..128}, k in {1,10,100} (graph replay per k group)",
-                      "qps": round(nq / total_s, 1), "total_ms": round(total_s * 1e3, 3),
+                                  "U{4..128} and k in {1,10,100} per query, one captured graph "
+                                  "per batch",
                       "train_s": round(t_train, 1), "add_s": round(t_add, 1),
                       "list_len_max": int(sizes.max()), "list_len_mean": float(sizes.mean()),
-                      "groups": groups}), flush=True)
+                      "runs": runs}), flush=True)
     return 0
 
 

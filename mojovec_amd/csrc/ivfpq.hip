@@ -192,10 +192,13 @@ __device__ __forceinline__ int64_t rerank_query(int64_t nq, const int32_t *__res
   return t < nq ? (int64_t)order[t] : -1;
 }
 
-// Row of candidate id (RowMap), or -1 when it has none.
+// Row of candidate id, or -1 when it has none: the id itself, or with MAP
+// (explicit ids) rm.row_of[id]. A template flag so that the unmapped
+// kernels keep one register per candidate.
+template <bool MAP>
 __device__ __forceinline__ int64_t cand_row(uint32_t id, RowMap rm, int64_t nrows) {
   int64_t r = (int64_t)id;
-  if (rm.row_of) r = r < rm.nids ? (int64_t)rm.row_of[id] : -1;
+  if (MAP) r = r < rm.nids ? (int64_t)rm.row_of[id] : -1;
   return (r >= 0 && r < nrows) ? r : -1;
 }
 
@@ -219,7 +222,7 @@ __device__ __forceinline__ void write_results(const uint64_t *top, int cnt, int6
   }
 }
 
-template <bool IP>
+template <bool IP, bool MAP>
 __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q, int d,
                                                      const float *__restrict__ vectors,
                                                      int64_t nrows,
@@ -244,7 +247,7 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
   for (int c = wave; c < kk; c += kNW) {
     const uint64_t key = cq[c];
     const uint32_t id = (uint32_t)(key & 0xFFFFFFFFu);
-    const int64_t row = key == kSentinel64 ? -1 : cand_row(id, rm, nrows);
+    const int64_t row = key == kSentinel64 ? -1 : cand_row<MAP>(id, rm, nrows);
     if (row < 0) continue;
     const float ex = wave_dist_w64<IP>(qs, vectors + row * d, d, lane);
     if (lane == 0) buf[c] = pack_key(IP ? -ex : ex, id);
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(kNT) void rerank_kernel(const float *__restrict__ q
 // Exact distances of a query's kk candidates into buf (sentinel for empty or
 // invalid slots): sixteen 16-lane groups, each scoring R candidates per step
 // so that R rows per group are in flight.
-template <bool IP, int R, bool BIG>
+template <bool IP, int R, bool BIG, bool MAP>
 __device__ __forceinline__ void rerank_rows(const float *qs, const float *__restrict__ vectors,
                                             int64_t nrows, const uint64_t *__restrict__ cq, int kk,
                                             int d, uint64_t *buf, RowMap rm) {
@@ -273,7 +276,7 @@ __device__ __forceinline__ void rerank_rows(const float *qs, const float *__rest
       const int ct = c + t * G;
       const uint64_t kv = ct < kk ? cq[ct] : kSentinel64;
       id[t] = (uint32_t)(kv & 0xFFFFFFFFu);
-      const int64_t row = kv == kSentinel64 ? -1 : cand_row(id[t], rm, nrows);
+      const int64_t row = kv == kSentinel64 ? -1 : cand_row<MAP>(id[t], rm, nrows);
       ok[t] = row >= 0;
       x[t] = ok[t] ? vectors + row * d : qs;  // an invalid slot reads the query row
       any |= ok[t];
@@ -295,7 +298,7 @@ __device__ __forceinline__ void rerank_rows(const float *qs, const float *__rest
 // Rerank for d a multiple of 64: sixteen 16-lane groups per workgroup each
 // score one candidate row at a time (group16_dist_w64, bit-identical to the
 // wavefront form), so 16 rows per workgroup are in flight instead of 4.
-template <bool IP, bool SMALLD>
+template <bool IP, bool SMALLD, bool MAP>
 __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restrict__ q, int d,
                                                            const float *__restrict__ vectors,
                                                            int64_t nrows,
@@ -318,9 +321,9 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
   __syncthreads();
   const uint64_t *cq = cand + qi * kk;
   if constexpr (SMALLD)  // d < 256: four rows per group, no four-accumulator blocks
-    rerank_rows<IP, 4, false>(qs, vectors, nrows, cq, kk, d, buf, rm);
+    rerank_rows<IP, 4, false, MAP>(qs, vectors, nrows, cq, kk, d, buf, rm);
   else
-    rerank_rows<IP, 2, true>(qs, vectors, nrows, cq, kk, d, buf, rm);
+    rerank_rows<IP, 2, true, MAP>(qs, vectors, nrows, cq, kk, d, buf, rm);
   __syncthreads();
   const int cnt = select_sort_u64(buf, kk, k, *scr, top);
   write_results<IP>(top, cnt, qi, k, kq, pad_dist, D, I);
@@ -332,7 +335,7 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
 // candidates of the exact FP32 rerank. Eight lanes per candidate row, 16-byte
 // code chunks, eight rows per group in flight; keys are (integer distance << 32
 // | id), so equal code distances keep the lower id.
-template <bool IP>
+template <bool IP, bool MAP>
 __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
     const uint8_t *__restrict__ qcodes, const uint32_t *__restrict__ qnorms, int ld,
     const uint8_t *__restrict__ vcodes, const uint32_t *__restrict__ vnorms, int64_t nrows,
@@ -352,15 +355,20 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
   const uint64_t *cq = cand + qi * kk;
   for (int c = group; c < kk; c += R * G) {
     uint32_t id[R], acc[R];
-    int64_t row[R];
+    int32_t row[R];  // rows < 2^31 (the device id width)
     bool ok[R];
 #pragma unroll
     for (int t = 0; t < R; t++) {
       const int ct = c + t * G;
       const uint64_t kv = ct < kk ? cq[ct] : kSentinel64;
       id[t] = (uint32_t)(kv & 0xFFFFFFFFu);
-      row[t] = kv == kSentinel64 ? -1 : cand_row(id[t], rm, nrows);
-      ok[t] = row[t] >= 0;
+      if constexpr (MAP) {
+        row[t] = kv == kSentinel64 ? -1 : (int32_t)cand_row<true>(id[t], rm, nrows);
+        ok[t] = row[t] >= 0;
+      } else {  // the row is the id: one register
+        row[t] = (int32_t)id[t];
+        ok[t] = kv != kSentinel64 && (int64_t)id[t] < nrows;
+      }
       acc[t] = 0;
     }
     for (int w = gl; w < words; w += 8) {
@@ -368,7 +376,7 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
 #pragma unroll
       for (int t = 0; t < R; t++) {
         if (!ok[t]) continue;
-        const uint4 b = reinterpret_cast<const uint4 *>(vcodes + row[t] * ld)[w];
+        const uint4 b = reinterpret_cast<const uint4 *>(vcodes + (int64_t)row[t] * ld)[w];
         if (IP) {
           int v = (int)acc[t];
           v = __builtin_amdgcn_sdot4((int)a.x, (int)b.x, v, false);
@@ -2053,19 +2061,22 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
                       sizeof(SelectScratch64);
     if (sm <= 150 * 1024) {
       const bool small = d < 256;
-#define MVG_RR(IPV, SV)                                                                          \
+#define MVG_RR(IPV, SV, MV)                                                                      \
   {                                                                                              \
     if (sm > 64 * 1024)                                                                          \
-      MVG_HIP(hipFuncSetAttribute((const void *)rerank_group_kernel<IPV, SV>,                    \
+      MVG_HIP(hipFuncSetAttribute((const void *)rerank_group_kernel<IPV, SV, MV>,                \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));         \
-    hipLaunchKernelGGL((rerank_group_kernel<IPV, SV>), dim3(grid), dim3(kNT), sm, s, q, d, vectors, \
-                       nrows, cand, kk, k, pad_dist, D, I, nq, order, rm, kq);                   \
+    hipLaunchKernelGGL((rerank_group_kernel<IPV, SV, MV>), dim3(grid), dim3(kNT), sm, s, q, d,   \
+                       vectors, nrows, cand, kk, k, pad_dist, D, I, nq, order, rm, kq);          \
   }
+#define MVG_RR2(IPV, SV)                                                                         \
+  if (rm.row_of) MVG_RR(IPV, SV, true) else MVG_RR(IPV, SV, false)
       if (ip) {
-        if (small) MVG_RR(true, true) else MVG_RR(true, false)
+        if (small) MVG_RR2(true, true) else MVG_RR2(true, false)
       } else {
-        if (small) MVG_RR(false, true) else MVG_RR(false, false)
+        if (small) MVG_RR2(false, true) else MVG_RR2(false, false)
       }
+#undef MVG_RR2
 #undef MVG_RR
       MVG_LAUNCH_CHECK("rerank_group_kernel");
       return;
@@ -2075,15 +2086,20 @@ void rerank(const float *q, int64_t nq, int d, int metric, const float *vectors,
   while (buf_n < kk || buf_n < k) buf_n <<= 1;
   const size_t sm = sizeof(float) * ((d + 1) & ~1) + 2 * sizeof(uint64_t) * buf_n + sizeof(SelectScratch);
   if (sm > 150 * 1024) throw Error(kInvalid, "rerank workspace exceeds LDS");
-  const void *fn = ip ? (const void *)rerank_kernel<true> : (const void *)rerank_kernel<false>;
-  if (sm > 64 * 1024)
-    MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-  if (ip)
-    hipLaunchKernelGGL(rerank_kernel<true>, dim3(grid), dim3(kNT), sm, s, q, d, vectors, nrows,
-                       cand, kk, k, buf_n, pad_dist, D, I, nq, order, rm, kq);
-  else
-    hipLaunchKernelGGL(rerank_kernel<false>, dim3(grid), dim3(kNT), sm, s, q, d, vectors, nrows,
-                       cand, kk, k, buf_n, pad_dist, D, I, nq, order, rm, kq);
+#define MVG_RK(IPV, MV)                                                                          \
+  {                                                                                              \
+    if (sm > 64 * 1024)                                                                          \
+      MVG_HIP(hipFuncSetAttribute((const void *)rerank_kernel<IPV, MV>,                          \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));         \
+    hipLaunchKernelGGL((rerank_kernel<IPV, MV>), dim3(grid), dim3(kNT), sm, s, q, d, vectors,    \
+                       nrows, cand, kk, k, buf_n, pad_dist, D, I, nq, order, rm, kq);            \
+  }
+  if (ip) {
+    if (rm.row_of) MVG_RK(true, true) else MVG_RK(true, false)
+  } else {
+    if (rm.row_of) MVG_RK(false, true) else MVG_RK(false, false)
+  }
+#undef MVG_RK
   MVG_LAUNCH_CHECK("rerank_kernel");
 }
 
@@ -2113,17 +2129,22 @@ void rerank_mixed(const IvfSearchArgs &a, IvfWork &w, const uint64_t *keys, int 
   if (sm > 150 * 1024) throw Error(kInvalid, "pre-rerank workspace exceeds LDS");
   const unsigned grid =
       (unsigned)(order ? kRerankRanges * ((a.nq + kRerankRanges - 1) / kRerankRanges) : a.nq);
-  const void *fn = ip ? (const void *)sq8_prefilter_kernel<true> : (const void *)sq8_prefilter_kernel<false>;
-  if (sm > 64 * 1024)
-    MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-  if (ip)
-    hipLaunchKernelGGL(sq8_prefilter_kernel<true>, dim3(grid), dim3(kNT), sm, s, w.vq_qcodes.as<uint8_t>(),
-                       w.vq_qnorms.as<uint32_t>(), ld, a.vq_codes, a.vq_norms, a.ntotal, keys, kk, kk2,
-                       w.vq_keys.as<uint64_t>(), a.nq, order, a.rows, a.kq, kk / a.k, a.prefilter);
-  else
-    hipLaunchKernelGGL(sq8_prefilter_kernel<false>, dim3(grid), dim3(kNT), sm, s, w.vq_qcodes.as<uint8_t>(),
-                       w.vq_qnorms.as<uint32_t>(), ld, a.vq_codes, a.vq_norms, a.ntotal, keys, kk, kk2,
-                       w.vq_keys.as<uint64_t>(), a.nq, order, a.rows, a.kq, kk / a.k, a.prefilter);
+#define MVG_PF(IPV, MV)                                                                          \
+  {                                                                                              \
+    if (sm > 64 * 1024)                                                                          \
+      MVG_HIP(hipFuncSetAttribute((const void *)sq8_prefilter_kernel<IPV, MV>,                   \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));         \
+    hipLaunchKernelGGL((sq8_prefilter_kernel<IPV, MV>), dim3(grid), dim3(kNT), sm, s,            \
+                       w.vq_qcodes.as<uint8_t>(), w.vq_qnorms.as<uint32_t>(), ld, a.vq_codes,    \
+                       a.vq_norms, a.ntotal, keys, kk, kk2, w.vq_keys.as<uint64_t>(), a.nq,      \
+                       order, a.rows, a.kq, kk / a.k, a.prefilter);                              \
+  }
+  if (ip) {
+    if (a.rows.row_of) MVG_PF(true, true) else MVG_PF(true, false)
+  } else {
+    if (a.rows.row_of) MVG_PF(false, true) else MVG_PF(false, false)
+  }
+#undef MVG_PF
   MVG_LAUNCH_CHECK("sq8_prefilter_kernel");
   rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.ntotal, w.vq_keys.as<uint64_t>(), kk2, a.k, kIvfPadDist,
          a.D, a.I, s, order, a.rows, a.kq);

@@ -1011,16 +1011,92 @@ void mvref_ivfpq_list(const mvref_ivfpq *h, int list, int64_t *ids, uint8_t *cod
   memcpy(codes, h->lcodes[list], (size_t)h->lsize[list] * h->M);
 }
 
+/* Coarse top-np of one query: IndexFlat search over the centroids (:264). */
+static void ivfpq_probe(const mvref_ivfpq *h, const float *qp, int np, cheap_t *pc, cand_t *pbuf) {
+  int d = h->d, ip = h->metric != MVREF_L2;
+  ch_init(pc, pbuf, np);
+  for (int l = 0; l < h->nlist; l++) {
+    const float *c = h->coarse + (int64_t)l * d;
+    ch_offer(pc, ip ? -mvref_ip_simd(qp, c, d, 8) : mvref_l2_simd(qp, c, d, 8), l);
+  }
+  ch_sorted(pc);
+}
+
+/* Per-query part of the search after the coarse step: LUT, ADC scan of the
+ * probed lists, optional exact rerank, output row (:276-360). */
+static void ivfpq_scan_query(const mvref_ivfpq *h, const float *qp, const cheap_t *pc, int k,
+                             int kf, const uint8_t *exclude, int64_t exclude_len, cand_t *buf,
+                             cand_t *buf2, float *lut, float *r, float *Drow, int64_t *Irow) {
+  int d = h->d, M = h->M, ip = h->metric != MVREF_L2;
+  int kk = k * kf;
+  if (ip) mvref_pq_lut(qp, d, M, h->pq, MVREF_IP, lut); /* :288-293 */
+  cheap_t c;
+  ch_init(&c, buf, kk);
+  for (int p = 0; p < pc->size; p++) {
+    int l = (int)pc->h[p].id;
+    int64_t L = h->lsize[l];
+    if (L == 0) continue;
+    const float *cen = h->coarse + (int64_t)l * d;
+    float coarse_d = 0.0f;
+    if (!ip) { /* :306-315 residual LUT */
+      for (int j = 0; j < d; j++) r[j] = qp[j] - cen[j];
+      mvref_pq_lut(r, d, M, h->pq, MVREF_L2, lut);
+    } else { /* :316-324 */
+      coarse_d = -mvref_ip_simd(qp, cen, d, 8);
+    }
+    const uint8_t *codes = h->lcodes[l];
+    const int64_t *ids = h->lids[l];
+    for (int64_t j = 0; j < L; j++) { /* :326-347 ADC scan */
+      int64_t id = ids[j];
+      if (exclude && id >= 0 && id < exclude_len && exclude[id] != 0) continue;
+      const uint8_t *cp = codes + j * M;
+      float dist = coarse_d;
+      for (int m = 0; m < M; m++) dist += lut[m * PQ_KSUB + cp[m]];
+      ch_offer(&c, dist, id);
+    }
+  }
+  ch_sorted(&c);
+  cheap_t *res = &c;
+  cheap_t r2;
+  if (kf > 1 && h->vectors) {
+    ch_init(&r2, buf2, k);
+    for (int j = 0; j < c.size; j++) {
+      const float *xv = h->vectors + c.h[j].id * d;
+      float ex = ip ? -mvref_ip_simd(qp, xv, d, 64) : mvref_l2_simd(qp, xv, d, 64);
+      ch_offer(&r2, ex, c.h[j].id);
+    }
+    ch_sorted(&r2);
+    res = &r2;
+  }
+  for (int j = 0; j < k; j++) {
+    if (j < res->size) {
+      float v = res->h[j].d;
+      Drow[j] = ip ? -v : v;
+      Irow[j] = res->h[j].id;
+    } else {
+      Drow[j] = IVF_PAD_DIST;
+      Irow[j] = -1;
+    }
+  }
+}
+
+static int ivfpq_np(const mvref_ivfpq *h, int nprobe, const int32_t *npq, int64_t qi) {
+  int np = npq ? npq[qi] : nprobe;
+  if (np > h->nlist) np = h->nlist; /* :253-254 */
+  return np < 1 ? 1 : np;
+}
+
 /* search: index_ivf_pq.mojo:225-360 with canonical ties, optional per-query
  * nprobe and the k_factor exact rerank (l2_distance_simd[64] /
  * inner_product_simd[64], as the SQ8 rerank of index_hnsw.mojo:449-482).
- * Pad (1e38, -1); IP distances un-negated over found entries (:357-360). */
+ * Pad (1e38, -1); IP distances un-negated over found entries (:357-360).
+ * Parallel over queries (coarse step and scan). */
 void mvref_ivfpq_search(const mvref_ivfpq *h, const float *q, int64_t nq,
                         int k, int nprobe, const int32_t *nprobe_per_query,
                         int k_factor, const uint8_t *exclude,
                         int64_t exclude_len, float *D, int64_t *I,
                         int nthreads) {
-  int d = h->d, M = h->M, nlist = h->nlist, ip = h->metric != MVREF_L2;
+  int d = h->d, M = h->M, nlist = h->nlist;
   int kf = k_factor > 1 ? k_factor : 1;
   int kk = k * kf;
   if (!h->trained || h->ntotal == 0) { /* :247-251 */
@@ -1037,69 +1113,57 @@ void mvref_ivfpq_search(const mvref_ivfpq *h, const float *q, int64_t nq,
     float *r = (float *)malloc(sizeof(float) * (size_t)d);
 #pragma omp for schedule(dynamic, 1)
     for (int64_t qi = 0; qi < nq; qi++) {
-      const float *qp = q + qi * d;
-      int np = nprobe_per_query ? nprobe_per_query[qi] : nprobe;
-      if (np > nlist) np = nlist; /* :253-254 */
-      if (np < 1) np = 1;
-      cheap_t pc; /* coarse top-nprobe: IndexFlat search (:264) */
-      ch_init(&pc, pbuf, np);
-      for (int l = 0; l < nlist; l++) {
-        const float *c = h->coarse + (int64_t)l * d;
-        ch_offer(&pc, ip ? -mvref_ip_simd(qp, c, d, 8) : mvref_l2_simd(qp, c, d, 8), l);
-      }
-      ch_sorted(&pc);
-      if (ip) mvref_pq_lut(qp, d, M, h->pq, MVREF_IP, lut); /* :288-293 */
-      cheap_t c;
-      ch_init(&c, buf, kk);
-      for (int p = 0; p < pc.size; p++) {
-        int l = (int)pc.h[p].id;
-        int64_t L = h->lsize[l];
-        if (L == 0) continue;
-        const float *cen = h->coarse + (int64_t)l * d;
-        float coarse_d = 0.0f;
-        if (!ip) { /* :306-315 residual LUT */
-          for (int j = 0; j < d; j++) r[j] = qp[j] - cen[j];
-          mvref_pq_lut(r, d, M, h->pq, MVREF_L2, lut);
-        } else { /* :316-324 */
-          coarse_d = -mvref_ip_simd(qp, cen, d, 8);
-        }
-        const uint8_t *codes = h->lcodes[l];
-        const int64_t *ids = h->lids[l];
-        for (int64_t j = 0; j < L; j++) { /* :326-347 ADC scan */
-          int64_t id = ids[j];
-          if (exclude && id >= 0 && id < exclude_len && exclude[id] != 0) continue;
-          const uint8_t *cp = codes + j * M;
-          float dist = coarse_d;
-          for (int m = 0; m < M; m++) dist += lut[m * PQ_KSUB + cp[m]];
-          ch_offer(&c, dist, id);
-        }
-      }
-      ch_sorted(&c);
-      cheap_t *res = &c;
-      cheap_t r2;
-      if (kf > 1 && h->vectors) {
-        ch_init(&r2, buf2, k);
-        for (int j = 0; j < c.size; j++) {
-          const float *xv = h->vectors + c.h[j].id * d;
-          float ex = ip ? -mvref_ip_simd(qp, xv, d, 64) : mvref_l2_simd(qp, xv, d, 64);
-          ch_offer(&r2, ex, c.h[j].id);
-        }
-        ch_sorted(&r2);
-        res = &r2;
-      }
-      for (int j = 0; j < k; j++) {
-        if (j < res->size) {
-          float v = res->h[j].d;
-          D[qi * k + j] = ip ? -v : v;
-          I[qi * k + j] = res->h[j].id;
-        } else {
-          D[qi * k + j] = IVF_PAD_DIST;
-          I[qi * k + j] = -1;
-        }
-      }
+      cheap_t pc;
+      ivfpq_probe(h, q + qi * d, ivfpq_np(h, nprobe, nprobe_per_query, qi), &pc, pbuf);
+      ivfpq_scan_query(h, q + qi * d, &pc, k, kf, exclude, exclude_len, buf, buf2, lut, r,
+                       D + qi * k, I + qi * k);
     }
     free(pbuf); free(buf); free(buf2); free(lut); free(r);
   }
+}
+
+/* The same search with the reference's threading (index_ivf_pq.mojo:
+ * 256-276): the coarse IndexFlat search runs parallel over queries (as
+ * IndexFlat._search_impl's parallelize), then the LUT + ADC loop over the
+ * queries runs serially on one thread (`for i in range(n)`, :276). Same
+ * results as mvref_ivfpq_search; a CPU-baseline timing variant. */
+void mvref_ivfpq_search_refstruct(const mvref_ivfpq *h, const float *q, int64_t nq, int k,
+                                  int nprobe, int k_factor, float *D, int64_t *I, int nthreads) {
+  int d = h->d, M = h->M, nlist = h->nlist;
+  int kf = k_factor > 1 ? k_factor : 1;
+  int kk = k * kf;
+  if (!h->trained || h->ntotal == 0) {
+    for (int64_t i = 0; i < nq * k; i++) { D[i] = IVF_PAD_DIST; I[i] = -1; }
+    return;
+  }
+  int np = ivfpq_np(h, nprobe, NULL, 0);
+  cand_t *probes = (cand_t *)malloc(sizeof(cand_t) * (size_t)nq * np);
+  int *counts = (int *)malloc(sizeof(int) * (size_t)nq);
+  int nt = nthreads_or_default(nthreads);
+#pragma omp parallel num_threads(nt)
+  {
+    cand_t *pbuf = (cand_t *)malloc(sizeof(cand_t) * (size_t)nlist);
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t qi = 0; qi < nq; qi++) {
+      cheap_t pc;
+      ivfpq_probe(h, q + qi * d, np, &pc, pbuf);
+      memcpy(probes + qi * np, pc.h, sizeof(cand_t) * (size_t)pc.size);
+      counts[qi] = pc.size;
+    }
+    free(pbuf);
+  }
+  cand_t *buf = (cand_t *)malloc(sizeof(cand_t) * (size_t)kk);
+  cand_t *buf2 = (cand_t *)malloc(sizeof(cand_t) * (size_t)k);
+  float *lut = (float *)malloc(sizeof(float) * (size_t)M * PQ_KSUB);
+  float *r = (float *)malloc(sizeof(float) * (size_t)d);
+  for (int64_t qi = 0; qi < nq; qi++) {
+    cheap_t pc;
+    pc.h = probes + qi * np;
+    pc.size = counts[qi];
+    pc.cap = np;
+    ivfpq_scan_query(h, q + qi * d, &pc, k, kf, NULL, 0, buf, buf2, lut, r, D + qi * k, I + qi * k);
+  }
+  free(buf); free(buf2); free(lut); free(r); free(probes); free(counts);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -130,6 +130,10 @@ void mvref_ivfpq_search(const mvref_ivfpq *h, const float *q, int64_t nq,
                         int k_factor, const uint8_t *exclude,
                         int64_t exclude_len, float *D, int64_t *I,
                         int nthreads);
+/* The search with the reference's threading: coarse step parallel over
+ * queries, LUT + ADC loop serial (a CPU-baseline timing variant). */
+void mvref_ivfpq_search_refstruct(const mvref_ivfpq *h, const float *q, int64_t nq, int k,
+                                  int nprobe, int k_factor, float *D, int64_t *I, int nthreads);
 
 /* ---- IVF-Flat (mojovec/index/index_ivf_flat.mojo) ---- */
 typedef struct mvref_ivfflat mvref_ivfflat;

@@ -99,6 +99,8 @@ def lib():
                                             _int]),
             "mvref_ivfpq_search": (None, [_vp, _f32p, _i64, _int, _int, _vp, _int, _vp, _i64,
                                           _f32p, _i64p, _int]),
+            "mvref_ivfpq_search_refstruct": (None, [_vp, _f32p, _i64, _int, _int, _int, _f32p, _i64p,
+                                                    _int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -428,6 +430,16 @@ class IVFPQIndex:
         ex = None if exclude is None else np.ascontiguousarray(exclude, np.uint8)
         lib().mvref_ivfpq_search(self.h, q, nq, k, nprobe, _ptr(npq), k_factor, _ptr(ex),
                                  0 if ex is None else ex.size, D, I, nthreads)
+        return D, I
+
+    def search_refstruct(self, q, k, nprobe, k_factor=1, nthreads=0):
+        """The search with the reference's threading (index_ivf_pq.mojo:
+        256-276): coarse step parallel, LUT + ADC loop serial."""
+        q = _f32(q).reshape(-1, self.d)
+        nq = q.shape[0]
+        D = np.empty((nq, k), np.float32)
+        I = np.empty((nq, k), np.int64)
+        lib().mvref_ivfpq_search_refstruct(self.h, q, nq, k, nprobe, k_factor, D, I, nthreads)
         return D, I
 
 

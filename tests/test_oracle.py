@@ -411,6 +411,21 @@ def test_ivfpq_exhaustive_equals_adc_bruteforce():
         assert (got <= kth + 1e-3).all()
 
 
+def test_ivfpq_reference_threading_variant_gives_the_same_results():
+    # the CPU-baseline variant with the reference's threading (coarse
+    # parallel, scan serial) is the same search
+    rng = np.random.default_rng(13)
+    x = rint(rng, (3000, 16), 0, 64)
+    ix = o.IVFPQIndex(16, 12, 4)
+    ix.train(x, niter_coarse=3, niter_pq=3)
+    ix.add(x)
+    q = rint(rng, (40, 16), 0, 64)
+    for kf in (1, 3):
+        D1, I1 = ix.search(q, 7, 5, k_factor=kf)
+        D2, I2 = ix.search_refstruct(q, 7, 5, k_factor=kf)
+        assert np.array_equal(I1, I2) and np.array_equal(D1, D2)
+
+
 def test_ivfpq_padding_exclusion_and_untrained():
     ix = o.IVFPQIndex(8, 4, 2)
     D, I = ix.search(np.zeros((2, 8), np.float32), 3, 2)
