@@ -1608,15 +1608,18 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
       if constexpr (DEEP) bound = __hip_atomic_load(&ds.bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       Code<MF> cw[U];
       int32_t idv[U];
-      float initv[U];
+      float initv[U], prev[U];
       uint32_t ev[U];
+      // Entry of each of the U positions (the probe cursor moves at list
+      // crossings only); positions past n read entry 0, which exists whenever
+      // n > 0, and are dropped below. The loads are issued unconditionally
+      // and nothing uses them until all U are in flight: a load inside the
+      // position test made the compiler wait for each entry before the next.
 #pragma unroll
-      for (int u = 0; u < U; u++) {  // every load of the batch in flight first
+      for (int u = 0; u < U; u++) {
         const int j = r0 + u * kNT + tid;
-        cw[u] = Code<MF>{};
-        idv[u] = -1;
-        initv[u] = 0.0f;
         ev[u] = 0;
+        initv[u] = 0.0f;
         if (j < n) {
           while (j >= cur_end) {
             cur++;
@@ -1624,14 +1627,20 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
             cur_delta = (int)(loff[cur] - poff[cur]);
             cur_base = basev[cur];
           }
-          const uint32_t e = (uint32_t)(j + cur_delta);
-          ev[u] = e;
-          if (MF > 0) cw[u] = load_code<MF>(codes, e);
-          if (!DEEP || excl) idv[u] = ids[e];
+          ev[u] = (uint32_t)(j + cur_delta);
           initv[u] = cur_base;
-          if (kMixed) initv[u] = initv[u] + pre[e];
         }
       }
+#pragma unroll
+      for (int u = 0; u < U; u++) {  // every load of the batch in flight first
+        if (MF > 0) cw[u] = load_code<MF>(codes, ev[u]);
+        else cw[u] = Code<MF>{};
+        idv[u] = (!DEEP || excl) ? ids[ev[u]] : -1;
+        prev[u] = kMixed ? pre[ev[u]] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (kMixed) initv[u] = initv[u] + prev[u];
       // Fixed M: the U entries' sums without branches (entries past n score
       // zero codes and are dropped below), so their lookups interleave.
       float distv[U];
