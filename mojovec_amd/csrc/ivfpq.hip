@@ -74,18 +74,38 @@ __device__ __forceinline__ float adc_words(const Code<MF> &c, const float *lut, 
   return dist;
 }
 
+// LDS word at byte address a (address space 3). The scan kernels keep their
+// LUT at the start of dynamic LDS and declare no static LDS, so the table's
+// LDS address is 0: with the address a known constant plus the code byte
+// times 4, the compiler selects and shifts the byte in one SDWA op and puts
+// the subquantizer offset in the ds_read immediate. (Through the generic
+// pointer it kept a separate add of the late-resolved dynamic-LDS base.)
+typedef __attribute__((address_space(3))) const float lds_float;
+__device__ __forceinline__ float lds_word(uint32_t a) {
+  return *reinterpret_cast<lds_float *>(static_cast<uintptr_t>(a));
+}
+
+template <int MF, int U, int M_>
+__device__ __forceinline__ void adc_step(const Code<MF> (&c)[U], float (&out)[U]) {
+  if constexpr (M_ < MF) {
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      out[u] += lds_word(M_ * kKsub * 4 + (((c[u].w[M_ >> 2] >> (8 * (M_ & 3))) & 0xFFu) << 2));
+    adc_step<MF, U, M_ + 1>(c, out);
+  }
+}
+
 // adc_words of U entries at once: each entry's sum keeps the m order, and the
-// U independent chains interleave so that U LUT lookups are in flight.
+// U independent chains interleave so that U LUT lookups are in flight. lut
+// must be the LUT at LDS address 0 (the callers' layouts put it first in
+// dynamic LDS, and they declare no static LDS).
 template <int MF, int U>
 __device__ __forceinline__ void adc_words_multi(const Code<MF> (&c)[U], const float *lut,
                                                 const float (&init)[U], float (&out)[U]) {
+  (void)lut;
 #pragma unroll
   for (int u = 0; u < U; u++) out[u] = init[u];
-#pragma unroll
-  for (int m = 0; m < MF; m++) {
-#pragma unroll
-    for (int u = 0; u < U; u++) out[u] += lut[m * kKsub + ((c[u].w[m >> 2] >> (8 * (m & 3))) & 0xFF)];
-  }
+  adc_step<MF, U, 0>(c, out);
 }
 
 // One workgroup per (query, probe rank).
