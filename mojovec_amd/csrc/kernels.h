@@ -26,7 +26,7 @@ void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int ne
 
 // ---- kmeans.hip ----
 struct KmeansWork {
-  DevBuf sums, counts, assign;
+  DevBuf assign, keys, rows, keys_sorted, rows_sorted, seg, sort_tmp;
 };
 // Lloyd iterations over S independent subspaces (S = 1: coarse quantizer;
 // S = M: PQ codebooks) of dsub columns each, starting from `cent`

@@ -5,9 +5,17 @@
 // in row order, chunks are then reduced in chunk order, and a centroid is
 // sum * (1/count) in FP32; an empty cluster is reseeded from row
 // (sample_offset + (iter+1)*k + c) % n. The accumulation is FP64 here (the
-// reference uses FP32), rounded to FP32 once before the scale. One thread owns
-// one (chunk, column) accumulator stream, so the sums are deterministic and
-// equal to the CPU restatement's.
+// reference uses FP32), rounded to FP32 once before the scale.
+//
+// The sums keep exactly that order while spreading the work over every
+// (centroid, column): the rows are stably sorted by their assigned centroid
+// (LSD radix sort: within one centroid the rows stay in row order), then one
+// thread per (subspace, centroid, column) walks its centroid's rows in row
+// order, summing each chunk's rows from 0.0 and adding the chunk partials to
+// the total in chunk order -- the same FP64 additions, in the same order, as
+// one accumulator per (chunk, centroid, column) reduced over the chunks.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "dist.h"
 #include "kernels.h"
 
@@ -17,52 +25,70 @@ namespace {
 
 constexpr int kChunks = 32;  // kmeans.mojo:84
 
-__global__ void kmeans_chunk_sums_kernel(const float *__restrict__ x, int64_t n, int S, int dsub,
-                                         int k, const int32_t *__restrict__ assign,
-                                         double *__restrict__ sums, int64_t *__restrict__ counts,
-                                         int64_t chunk_size) {
+// Sort keys of (row, subspace): subspace-major centroid index, and the row.
+__global__ void kmeans_keys_kernel(const int32_t *__restrict__ assign, int64_t n, int S, int k,
+                                   uint32_t *__restrict__ keys, int32_t *__restrict__ rows) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t total = (int64_t)S * kChunks * dsub;
-  if (t >= total) return;
-  const int j = (int)(t % dsub);
-  const int ch = (int)((t / dsub) % kChunks);
-  const int s = (int)(t / ((int64_t)dsub * kChunks));
-  const int ld = S * dsub;
-  const int64_t start = (int64_t)ch * chunk_size;
-  const int64_t end = min(start + chunk_size, n);
-  double *sm = sums + ((int64_t)(s * kChunks + ch) * k) * dsub;
-  int64_t *cm = counts + (int64_t)(s * kChunks + ch) * k;
-  for (int64_t i = start; i < end; i++) {
-    const int c = assign[i * S + s];
-    sm[(int64_t)c * dsub + j] += (double)x[i * ld + (int64_t)s * dsub + j];
-    if (j == 0) cm[c] += 1;
-  }
+  if (t >= n * S) return;
+  const int64_t i = t / S;
+  const int s = (int)(t - i * S);
+  keys[t] = (uint32_t)s * (uint32_t)k + (uint32_t)assign[t];
+  rows[t] = (int32_t)i;
 }
 
-__global__ void kmeans_finalize_kernel(const float *__restrict__ x, int64_t n, int S, int dsub,
-                                       int k, const double *__restrict__ sums,
-                                       const int64_t *__restrict__ counts, int iter,
-                                       int64_t sample_offset, float *__restrict__ cent) {
+// seg[key] = first sorted position with that key or a larger one (key in
+// [0, nkeys]; seg[nkeys] = total).
+__global__ void kmeans_segments_kernel(const uint32_t *__restrict__ sorted, int64_t total,
+                                       int64_t nkeys, int64_t *__restrict__ seg) {
+  const int64_t key = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (key > nkeys) return;
+  int64_t lo = 0, hi = total;  // lower bound of key
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)sorted[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  seg[key] = lo;
+}
+
+// One thread per (subspace, centroid, column): FP64 sum of the centroid's
+// rows in row order, restarted at each 32-row-chunk boundary and added to the
+// total in chunk order (kmeans.mojo:149-195), then sum * (1/count) in FP32
+// (:197-210) or the reseed row of an empty cluster (:211-220).
+__global__ void kmeans_update_kernel(const float *__restrict__ x, int64_t n, int S, int dsub, int k,
+                                     const int32_t *__restrict__ rows,
+                                     const int64_t *__restrict__ seg, int64_t chunk_size, int iter,
+                                     int64_t sample_offset, float *__restrict__ cent) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)S * k * dsub;
   if (t >= total) return;
   const int j = (int)(t % dsub);
-  const int c = (int)((t / dsub) % k);
-  const int s = (int)(t / ((int64_t)dsub * k));
-  int64_t count = 0;
-  for (int ch = 0; ch < kChunks; ch++) count += counts[(int64_t)(s * kChunks + ch) * k + c];
+  const int64_t key = t / dsub;  // s * k + c
+  const int s = (int)(key / k);
+  const int64_t ld = (int64_t)S * dsub;
+  const int64_t lo = seg[key], hi = seg[key + 1];
   float v;
-  if (count > 0) {
-    double acc = 0.0;
-    for (int ch = 0; ch < kChunks; ch++)
-      acc += sums[((int64_t)(s * kChunks + ch) * k + c) * dsub + j];
-    const float inv = __fdiv_rn(1.0f, (float)count);  // kmeans.mojo:201
+  if (hi > lo) {
+    double acc = 0.0, part = 0.0;
+    int64_t cur = -1;
+    for (int64_t p = lo; p < hi; p++) {
+      const int64_t i = rows[p];
+      const int64_t ch = i / chunk_size;
+      if (ch != cur) {
+        acc += part;
+        part = 0.0;
+        cur = ch;
+      }
+      part += (double)x[i * ld + (int64_t)s * dsub + j];
+    }
+    acc += part;
+    const float inv = __fdiv_rn(1.0f, (float)(hi - lo));  // kmeans.mojo:201
     v = __fmul_rn((float)acc, inv);
   } else {
+    const int c = (int)(key - (int64_t)s * k);
     const int64_t src = (sample_offset + (int64_t)(iter + 1) * k + c) % n;  // :214-216
-    v = x[src * (int64_t)(S * dsub) + (int64_t)s * dsub + j];
+    v = x[src * ld + (int64_t)s * dsub + j];
   }
-  cent[((int64_t)s * k + c) * dsub + j] = v;
+  cent[key * dsub + j] = v;
 }
 
 // codes / assignment of (row, subspace m) = argmin over 256 centroids with
@@ -201,12 +227,22 @@ void residuals(const float *x, int64_t n, int d, const float *coarse, const int3
 void kmeans_fit(const float *x, int64_t n, int S, int dsub, int k, int niter,
                 int64_t sample_offset, float *cent, KmeansWork &w, hipStream_t s) {
   if (n == 0 || niter <= 0) return;
+  if (n * S > INT32_MAX) throw Error(kInvalid, "k-means holds at most 2^31-1 (row, subspace) pairs");
   const int64_t chunk_size = (n + kChunks - 1) / kChunks;
-  const size_t nsum = (size_t)S * kChunks * k * dsub;
-  const size_t ncnt = (size_t)S * kChunks * k;
-  w.sums.reserve(sizeof(double) * nsum);
-  w.counts.reserve(sizeof(int64_t) * ncnt);
-  w.assign.reserve(sizeof(int32_t) * (size_t)n * S);
+  const int64_t total = n * S, nkeys = (int64_t)S * k;
+  unsigned end_bit = 1;
+  while (end_bit < 32 && ((int64_t)1 << end_bit) < nkeys) end_bit++;
+  w.assign.reserve(sizeof(int32_t) * (size_t)total);
+  w.keys.reserve(sizeof(uint32_t) * (size_t)total);
+  w.rows.reserve(sizeof(int32_t) * (size_t)total);
+  w.keys_sorted.reserve(sizeof(uint32_t) * (size_t)total);
+  w.rows_sorted.reserve(sizeof(int32_t) * (size_t)total);
+  w.seg.reserve(sizeof(int64_t) * (size_t)(nkeys + 1));
+  size_t tmp_bytes = 0;
+  MVG_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, w.keys.as<uint32_t>(), w.keys_sorted.as<uint32_t>(),
+                                    w.rows.as<int32_t>(), w.rows_sorted.as<int32_t>(), (size_t)total, 0,
+                                    end_bit, s));
+  w.sort_tmp.reserve(std::max<size_t>(tmp_bytes, 1));
   int32_t *assign = w.assign.as<int32_t>();
   for (int it = 0; it < niter; it++) {
     if (S == 1) {
@@ -215,16 +251,21 @@ void kmeans_fit(const float *x, int64_t n, int S, int dsub, int k, int niter,
       if (k != kKsub) throw Error(kInvalid, "PQ training uses 256 centroids per subspace");
       pq_assign(x, n, S * dsub, S, cent, nullptr, nullptr, nullptr, assign, s);
     }
-    MVG_HIP(hipMemsetAsync(w.sums.ptr, 0, sizeof(double) * nsum, s));
-    MVG_HIP(hipMemsetAsync(w.counts.ptr, 0, sizeof(int64_t) * ncnt, s));
-    const int64_t nt1 = (int64_t)S * kChunks * dsub;
-    hipLaunchKernelGGL(kmeans_chunk_sums_kernel, dim3(ceil_div(nt1, 64)), dim3(64), 0, s, x, n, S,
-                       dsub, k, assign, w.sums.as<double>(), w.counts.as<int64_t>(), chunk_size);
-    MVG_LAUNCH_CHECK("kmeans_chunk_sums_kernel");
-    const int64_t nt2 = (int64_t)S * k * dsub;
-    hipLaunchKernelGGL(kmeans_finalize_kernel, dim3(ceil_div(nt2, 256)), dim3(256), 0, s, x, n, S,
-                       dsub, k, w.sums.as<double>(), w.counts.as<int64_t>(), it, sample_offset, cent);
-    MVG_LAUNCH_CHECK("kmeans_finalize_kernel");
+    hipLaunchKernelGGL(kmeans_keys_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, s, assign, n, S, k,
+                       w.keys.as<uint32_t>(), w.rows.as<int32_t>());
+    MVG_LAUNCH_CHECK("kmeans_keys_kernel");
+    size_t tb = w.sort_tmp.bytes;
+    MVG_HIP(rocprim::radix_sort_pairs(w.sort_tmp.ptr, tb, w.keys.as<uint32_t>(), w.keys_sorted.as<uint32_t>(),
+                                      w.rows.as<int32_t>(), w.rows_sorted.as<int32_t>(), (size_t)total, 0,
+                                      end_bit, s));
+    hipLaunchKernelGGL(kmeans_segments_kernel, dim3(ceil_div(nkeys + 1, 256)), dim3(256), 0, s,
+                       w.keys_sorted.as<uint32_t>(), total, nkeys, w.seg.as<int64_t>());
+    MVG_LAUNCH_CHECK("kmeans_segments_kernel");
+    const int64_t nt = nkeys * dsub;
+    hipLaunchKernelGGL(kmeans_update_kernel, dim3(ceil_div(nt, 256)), dim3(256), 0, s, x, n, S, dsub, k,
+                       w.rows_sorted.as<int32_t>(), w.seg.as<int64_t>(), chunk_size, it, sample_offset,
+                       cent);
+    MVG_LAUNCH_CHECK("kmeans_update_kernel");
   }
 }
 
