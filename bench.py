@@ -126,6 +126,8 @@ def main(argv=None):
                     help="skip the 1M x 128 exact FP32 Flat vs SQ8 (int8 MFMA) leg")
     ap.add_argument("--no-sq8", action="store_true",
                     help="skip the SQ8 cosine 1M x 768 (int8 MFMA + FP32 rerank) leg")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving leg (both indexes resident, one request loop)")
     ap.add_argument("--workload", default="ivfpq",
                     choices=["ivfpq", "flat", "sq8", "batch", "serve", "serve_sq8", "ivfflat",
                              "pairdist"],
@@ -290,12 +292,15 @@ def main(argv=None):
     cpu = None if args.no_cpu else cpu_baseline(
         ix, x_h, q_h, best["plain"]["nprobe"], best["plain"]["k_factor"], args.cpu_seconds)
     flat = None if args.no_flat else flat_1m(mv, torch, x_h, q_h, gt, args)
-    del ix
-    sq8 = None
+    sq8 = serve = None
     if not args.no_sq8:
         del x, sample
         torch.cuda.empty_cache()
-        sq8 = sq8_leg(mv, torch, args)
+        sq8, sx, sq_q_h, sq_gt = sq8_leg(mv, torch, args)
+        if not args.no_serve:
+            serve = serve_both(mv, ix, sx, q_h, gt.cpu().numpy(), sq_q_h, sq_gt)
+        del sx
+    del ix
 
     line = {
         "metric": "QPS@recall10\u22650.95, 1M\u00d7128 IVF-PQ/SQ8: mixed vs FP32 GPU; %HBM roofline",
@@ -330,6 +335,7 @@ def main(argv=None):
             "mixed_over_plain": None if ratio is None else round(ratio, 3),
             "flat_1Mx128": flat,
             "sq8_cosine_1Mx768": sq8,
+            "serving_both_indexes": serve,
             "train_s": round(t_train, 3),
             "add_s": round(t_add, 3),
             "sweep": sweeps,
@@ -401,6 +407,81 @@ def flat_1m(mv, torch, x_h, q_h, gt, args):
     out["sq8_over_plain"] = round(out["sq8_int8_mfma_kf1"]["qps"] / base, 3)
     log(f"flat 1M x 128: {out}")
     return out
+
+
+def serve_both(mv, ix, sx, q_ivf, gt_ivf, q_sq8, gt_sq8, rates=(20_000, 100_000, 200_000),
+               duration=2.0, seed=47):
+    """Both indexes resident and served by one request loop (one dispatcher,
+    four slots, one memory budget): Poisson arrivals at each offered rate,
+    request sizes log-uniform in [1, 4096], each request to the IVF-PQ
+    1M x 128 index (mixed form; k in {10, 100} at the nprobe / k_factor that
+    reach recall@10 >= 0.95) or to the SQ8 cosine 1M x 768 index (k in
+    {10, 100}, exact rerank of 2k) with equal odds. Latency is submit to
+    results on the host; recall@10 against FP64 brute force per index."""
+    import queue
+    import threading
+
+    ix.set_search_mode("mixed")
+    srv = mv.Server.both(ix, sx, slots=4, max_batch=4096, mem_budget=32 << 30)
+    kinds = {"ivfpq_1Mx128": (ix, q_ivf, gt_ivf, {10: (64, 288), 100: (64, 29)}),
+             "sq8_cosine_1Mx768": (sx, q_sq8, gt_sq8, {10: (1, 2), 100: (1, 2)})}
+    names = sorted(kinds)
+    rng = np.random.default_rng(seed)
+    out = {}
+    for rate in rates:
+        reqs, t, pos = [], 0.0, {n: 0 for n in names}
+        while t < duration:
+            size = int(min(4096, max(1, round(np.exp(rng.uniform(0.0, np.log(4096)))))))
+            name = names[int(rng.integers(0, 2))]
+            k = int(rng.choice([10, 100]))
+            reqs.append((t, name, pos[name], size, k))
+            pos[name] += size
+            t += rng.exponential(size / rate)
+        tickets = queue.Queue()
+        t0 = time.perf_counter()
+
+        def submitter():
+            for (at, name, start, size, k) in reqs:
+                wait = at - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+                target, qh, _, settings = kinds[name]
+                idx = (np.arange(size) + start) % qh.shape[0]
+                nprobe, kf = settings[k]
+                tickets.put((srv.submit(qh[idx], k=k, nprobe=nprobe, k_factor=kf, target=target), name, idx))
+            tickets.put(None)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat = {n: [] for n in names}
+        hits = {n: 0 for n in names}
+        nqs = {n: 0 for n in names}
+        while True:
+            item = tickets.get()
+            if item is None:
+                break
+            ticket, name, idx = item
+            _, Ir, ms = srv.wait(ticket)
+            lat[name].append(ms)
+            hits[name] += int((Ir[:, :10, None] == kinds[name][2][idx][:, None, :10]).any(2).sum())
+            nqs[name] += len(idx)
+        t_end = time.perf_counter()
+        th.join()
+        row = {"sustained_qps": round(sum(nqs.values()) / (t_end - t0), 1)}
+        for n in names:
+            la = np.asarray(lat[n])
+            row[n] = {"queries": nqs[n], "p50_ms": round(float(np.percentile(la, 50)), 3),
+                      "p99_ms": round(float(np.percentile(la, 99)), 3),
+                      "recall_at_10": round(hits[n] / (10 * max(nqs[n], 1)), 4)}
+        out[str(rate)] = row
+        log(f"serve both offered {rate}/s: {row}")
+    stats = srv.stats()
+    srv.close()
+    ix.set_search_mode("plain")
+    return {"workload": "one request loop over both resident indexes: IVF-PQ 1M x 128 (mixed form) and "
+                        "SQ8 cosine 1M x 768; Poisson arrivals, request sizes log-uniform [1, 4096], "
+                        "k in {10, 100}, equal odds per index",
+            "rates": out, "server": stats}
 
 
 def bench_serve(args):
@@ -625,8 +706,7 @@ def sq8_leg(mv, torch, args):
                                  "achieved_tops": round(tops, 1), "peak_tops": 5000.0,
                                  "frac": round(tops / 5000.0, 4)}
         log(f"sq8 cosine 1M x 768 kf={kf}: {out[f'k_factor_{kf}']}")
-    del ix
-    return out
+    return out, ix, q_h, gt.cpu().numpy()
 
 
 FP32_VALU_PEAK = 78.6  # TFLOP/s: MI355X FP32 FMA without packed math (157.3 with it)

@@ -240,7 +240,9 @@ int mvg_ivfpq_set_prefilter(mvg_ivfpq *h, int per_k);
  * runs the scan as persistent workgroups over a device-built queue of
  * (query, entry-range) items of about that many list entries (at most 8 per
  * query), merged per query afterwards; 0 (default) scans one query per
- * workgroup. Results are identical. */
+ * workgroup. Results are identical in the plain form; in the mixed form the
+ * queue's kernel sums each ADC distance from per-list tables rather than the
+ * per-entry terms, so distances agree up to FP32 rounding. */
 int mvg_ivfpq_set_work_queue(mvg_ivfpq *h, int entries_per_item);
 /* Deep candidate pools (no reference counterpart): from k * k_factor >=
  * min_candidates (0 = default 513) the scan selects its candidates without
@@ -366,10 +368,20 @@ int mvg_server_create(mvg_ivfpq *index, int slots, int max_batch_queries, int64_
  * nprobe is ignored). */
 int mvg_server_create_sq8(mvg_sq8 *index, int slots, int max_batch_queries,
                           int64_t mem_budget_bytes, mvg_server **out);
+/* One loop over both kinds of index (either pointer may be NULL): the same
+ * dispatcher, slots and memory budget serve requests of the IVF-PQ index and
+ * of the Flat SQ8 index; a batch holds requests of one index. */
+int mvg_server_create_multi(mvg_ivfpq *ivfpq, mvg_sq8 *sq8, int slots, int max_batch_queries,
+                            int64_t mem_budget_bytes, mvg_server **out);
 /* Copies q [nq x d]; distances/labels [nq x k] must stay valid until
- * mvg_server_wait(ticket) returns. */
+ * mvg_server_wait(ticket) returns. The request goes to the server's only
+ * index (the IVF-PQ one when it holds both). */
 int mvg_server_submit(mvg_server *s, const float *q, int64_t nq, int k, int nprobe, int k_factor,
                       float *distances, int64_t *labels, int64_t *ticket);
+/* The same for a named index: target 0 = the IVF-PQ index, 1 = the Flat SQ8
+ * index (q has that index's d columns). */
+int mvg_server_submit_to(mvg_server *s, int target, const float *q, int64_t nq, int k, int nprobe,
+                         int k_factor, float *distances, int64_t *labels, int64_t *ticket);
 /* Blocks until the request is done; latency_ms = submit to completion. */
 int mvg_server_wait(mvg_server *s, int64_t ticket, double *latency_ms);
 int mvg_server_stats(mvg_server *s, int64_t *completed_queries, int64_t *batches,

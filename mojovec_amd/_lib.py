@@ -98,7 +98,9 @@ SIGNATURES = {
     "mvg_graph_destroy": (None, [_p]),
     "mvg_server_create": (_i, [_p, _i, _i, _i64, ctypes.POINTER(_p)]),
     "mvg_server_create_sq8": (_i, [_p, _i, _i, _i64, ctypes.POINTER(_p)]),
+    "mvg_server_create_multi": (_i, [_p, _p, _i, _i, _i64, ctypes.POINTER(_p)]),
     "mvg_server_submit": (_i, [_p, _p, _i64, _i, _i, _i, _p, _p, ctypes.POINTER(_i64)]),
+    "mvg_server_submit_to": (_i, [_p, _i, _p, _i64, _i, _i, _i, _p, _p, ctypes.POINTER(_i64)]),
     "mvg_server_wait": (_i, [_p, _i64, ctypes.POINTER(ctypes.c_double)]),
     "mvg_server_stats": (_i, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "mvg_server_destroy": (None, [_p]),

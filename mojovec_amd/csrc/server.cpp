@@ -1,5 +1,7 @@
-// Continuous-batching request loop over an HBM-resident IVF-PQ or Flat SQ8
-// index.
+// Continuous-batching request loop over HBM-resident indexes: an IVF-PQ
+// index, a Flat SQ8 index, or both served by the same loop (one dispatcher,
+// one set of slots, one memory budget; each request names its index and a
+// batch holds requests of one index).
 //
 // Requests (a block of queries with its own k, nprobe and k_factor; nprobe is
 // ignored by a Flat SQ8 index) queue on the host. A dispatcher thread packs
@@ -31,6 +33,7 @@ using Clock = std::chrono::steady_clock;
 
 struct Request {
   int64_t ticket = 0;
+  int target = 0;  // 0: the IVF-PQ index, 1: the Flat SQ8 index
   std::vector<float> q;
   int64_t nq = 0;
   int k = 10, nprobe = 1, k_factor = 1;
@@ -76,9 +79,8 @@ struct Slot {
 }  // namespace
 
 struct mvg_server {
-  mvg_ivfpq *ix = nullptr;  // exactly one of ix, sq is set
+  mvg_ivfpq *ix = nullptr;  // at least one of ix, sq is set
   mvg_sq8 *sq = nullptr;
-  int d = 0;
   int device = 0;
   int max_batch = 4096;
   int64_t budget = 0, in_use = 0;
@@ -96,18 +98,22 @@ struct mvg_server {
 
 namespace {
 
-int64_t batch_bytes(const mvg_server *sv, int64_t nq, int k, int kf, int maxp) {
+// Row width of a request's index.
+int target_d(const mvg_server *sv, int target) { return target == 0 ? sv->ix->d : sv->sq->d; }
+
+int64_t batch_bytes(const mvg_server *sv, int target, int64_t nq, int k, int kf, int maxp) {
   const int64_t kk = (int64_t)k * kf;
-  if (sv->sq) {
+  const int64_t d = target_d(sv, target);
+  if (target == 1) {
     // queries, query codes and norms, outputs, merged candidates, and the
     // per-split partial lists: at most 64 splits per query in the 64-query
     // form, at most 2048 workgroups of 16 queries in the 16-query form
-    const int64_t ld = mvg::sq8_row_stride(sv->d);
-    return nq * ((int64_t)sv->d * 4 + ld + 8 + (int64_t)k * 12 + kk * 8 * 65) +
-           (int64_t)2048 * 16 * kk * 8 + (1 << 20);
+    const int64_t ld = mvg::sq8_row_stride((int)d);
+    return nq * (d * 4 + ld + 8 + (int64_t)k * 12 + kk * 8 * 65) + (int64_t)2048 * 16 * kk * 8 +
+           (1 << 20);
   }
   // queries, probes, per-query candidates (work queue: up to 8 items), outputs
-  return nq * ((int64_t)sv->d * 6 + (int64_t)maxp * 12 + kk * 8 * 9 + (int64_t)k * 12) + (1 << 20);
+  return nq * (d * 6 + (int64_t)maxp * 12 + kk * 8 * 9 + (int64_t)k * 12) + (1 << 20);
 }
 
 void retire(mvg_server *sv, Slot &sl) {
@@ -152,8 +158,9 @@ void fail_slot(mvg_server *sv, Slot &sl, const std::string &msg) {
 // 64-query MFMA form holds up to 64 candidates per query, larger pools take
 // the 16-query form, so mixing them would slow the small ones).
 bool joins(const mvg_server *sv, const Request *head, const Request *r) {
-  if (r->k_factor != head->k_factor) return false;
-  if (sv->sq) {
+  (void)sv;
+  if (r->target != head->target || r->k_factor != head->k_factor) return false;
+  if (head->target == 1) {
     const bool small_head = (int64_t)head->k * head->k_factor <= 64;
     const bool small_r = (int64_t)r->k * r->k_factor <= 64;
     if (small_head != small_r) return false;
@@ -176,9 +183,9 @@ std::vector<Request *> take_batch(mvg_server *sv, int64_t &bytes) {
       continue;
     }
     if (!batch.empty() && nq + r->nq > sv->max_batch) break;
-    const int mp = sv->ix ? std::max(maxp, std::min(r->nprobe, sv->ix->nlist)) : 1;
+    const int mp = head->target == 0 ? std::max(maxp, std::min(r->nprobe, sv->ix->nlist)) : 1;
     const int mk = std::max(maxk, r->k);
-    const int64_t b = batch_bytes(sv, nq + r->nq, mk, kf, mp);
+    const int64_t b = batch_bytes(sv, head->target, nq + r->nq, mk, kf, mp);
     // admission by memory: the first request of an idle server always runs
     if (sv->in_use + b > sv->budget && !(batch.empty() && sv->inflight == 0)) break;
     batch.push_back(r);
@@ -192,7 +199,9 @@ std::vector<Request *> take_batch(mvg_server *sv, int64_t &bytes) {
 }
 
 void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t bytes) {
-  const int d = sv->d;
+  const int target = batch[0]->target;
+  const int d = target_d(sv, target);
+  mvg_ivfpq *ix = target == 0 ? sv->ix : nullptr;
   int64_t nq = 0;
   int maxp = 1, k = 1;
   bool mixed_k = false;
@@ -210,7 +219,7 @@ void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t byt
   int64_t off = 0;
   for (Request *r : batch) {
     std::copy(r->q.begin(), r->q.end(), sl.hq.ptr + off * d);
-    const int np = sv->ix ? std::min(r->nprobe, sv->ix->nlist) : 1;
+    const int np = ix ? std::min(r->nprobe, ix->nlist) : 1;
     maxp = std::max(maxp, np);
     std::fill(sl.hnp.ptr + off, sl.hnp.ptr + off + r->nq, np);
     std::fill(sl.hkq.ptr + off, sl.hkq.ptr + off + r->nq, r->k);
@@ -222,19 +231,19 @@ void launch(mvg_server *sv, Slot &sl, std::vector<Request *> &batch, int64_t byt
   sl.D.reserve(sizeof(float) * nq * k);
   sl.I.reserve(sizeof(int64_t) * nq * k);
   MVG_HIP(hipMemcpyAsync(sl.q.ptr, sl.hq.ptr, sizeof(float) * nq * d, hipMemcpyHostToDevice, sl.s));
-  if (sv->ix)
+  if (ix)
     MVG_HIP(hipMemcpyAsync(sl.np.ptr, sl.hnp.ptr, sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.s));
   const int32_t *kq = nullptr;
   if (mixed_k) {
     MVG_HIP(hipMemcpyAsync(sl.kq.ptr, sl.hkq.ptr, sizeof(int32_t) * nq, hipMemcpyHostToDevice, sl.s));
     kq = sl.kq.as<int32_t>();
   }
-  if (sv->ix) {
+  if (ix) {
     // the handle's search lock: a concurrent eager search cannot prepare
     // (re-upload) the index buffers while this batch is being enqueued
-    std::lock_guard<std::mutex> lk(sv->ix->search_mu);
-    mvg::ivf_prepare(sv->ix);
-    mvg::ivf_search_with(sv->ix, sl.q.as<float>(), nq, k, sl.np.as<int32_t>(), maxp, kf, nullptr, 0,
+    std::lock_guard<std::mutex> lk(ix->search_mu);
+    mvg::ivf_prepare(ix);
+    mvg::ivf_search_with(ix, sl.q.as<float>(), nq, k, sl.np.as<int32_t>(), maxp, kf, nullptr, 0,
                          sl.D.as<float>(), sl.I.as<int64_t>(), sl.work, sl.s, kq);
   } else {
     const mvg_sq8 *h = sv->sq;
@@ -321,7 +330,7 @@ void dispatcher(mvg_server *sv) {
 }
 
 // Validates the settings, creates the slots' streams and starts the dispatcher.
-int start_server(mvg_ivfpq *ix, mvg_sq8 *sq, int d, int slots, int max_batch_queries,
+int start_server(mvg_ivfpq *ix, mvg_sq8 *sq, int slots, int max_batch_queries,
                  int64_t mem_budget_bytes, mvg_server **out) {
   if (slots < 1 || slots > 16 || max_batch_queries < 1 || mem_budget_bytes <= 0 || !out)
     throw mvg::Error(MVG_ERR_INVALID,
@@ -330,7 +339,6 @@ int start_server(mvg_ivfpq *ix, mvg_sq8 *sq, int d, int slots, int max_batch_que
   auto sv = std::make_unique<mvg_server>();
   sv->ix = ix;
   sv->sq = sq;
-  sv->d = d;
   MVG_HIP(hipGetDevice(&sv->device));
   sv->max_batch = max_batch_queries;
   sv->budget = mem_budget_bytes;
@@ -350,18 +358,19 @@ int start_server(mvg_ivfpq *ix, mvg_sq8 *sq, int d, int slots, int max_batch_que
 
 extern "C" {
 
-int mvg_server_create(mvg_ivfpq *ix, int slots, int max_batch_queries, int64_t mem_budget_bytes,
-                      mvg_server **out) {
+int mvg_server_create_multi(mvg_ivfpq *ix, mvg_sq8 *sq, int slots, int max_batch_queries,
+                            int64_t mem_budget_bytes, mvg_server **out) {
   try {
-    if (!ix) throw mvg::Error(MVG_ERR_INVALID, "server needs an index");
-    if (!ix->trained || ix->ntotal == 0)
+    if (!ix && !sq) throw mvg::Error(MVG_ERR_INVALID, "server needs an index");
+    if (ix && (!ix->trained || ix->ntotal == 0))
       throw mvg::Error(MVG_ERR_STATE, "server needs a trained, non-empty index");
-    {
+    if (sq && sq->ntotal == 0) throw mvg::Error(MVG_ERR_STATE, "server needs a non-empty index");
+    if (ix) {
       std::lock_guard<std::mutex> lk(ix->search_mu);
       mvg::ivf_prepare(ix);
     }
-    const int rc = start_server(ix, nullptr, ix->d, slots, max_batch_queries, mem_budget_bytes, out);
-    if (rc == MVG_OK) ix->servers++;
+    const int rc = start_server(ix, sq, slots, max_batch_queries, mem_budget_bytes, out);
+    if (rc == MVG_OK && ix) ix->servers++;
     return rc;
   } catch (const mvg::Error &e) {
     mvg::set_last_error(e.what());
@@ -372,24 +381,32 @@ int mvg_server_create(mvg_ivfpq *ix, int slots, int max_batch_queries, int64_t m
   }
 }
 
-int mvg_server_create_sq8(mvg_sq8 *ix, int slots, int max_batch_queries, int64_t mem_budget_bytes,
-                          mvg_server **out) {
-  try {
-    if (!ix) throw mvg::Error(MVG_ERR_INVALID, "server needs an index");
-    if (ix->ntotal == 0) throw mvg::Error(MVG_ERR_STATE, "server needs a non-empty index");
-    return start_server(nullptr, ix, ix->d, slots, max_batch_queries, mem_budget_bytes, out);
-  } catch (const mvg::Error &e) {
-    mvg::set_last_error(e.what());
-    return e.code;
-  } catch (const std::exception &e) {
-    mvg::set_last_error(e.what());
-    return MVG_ERR_DEVICE;
+int mvg_server_create(mvg_ivfpq *ix, int slots, int max_batch_queries, int64_t mem_budget_bytes,
+                      mvg_server **out) {
+  if (!ix) {
+    mvg::set_last_error("server needs an index");
+    return MVG_ERR_INVALID;
   }
+  return mvg_server_create_multi(ix, nullptr, slots, max_batch_queries, mem_budget_bytes, out);
 }
-int mvg_server_submit(mvg_server *sv, const float *q, int64_t nq, int k, int nprobe, int k_factor,
-                      float *distances, int64_t *labels, int64_t *ticket) {
-  if (!sv || nq < 1 || k < 1 || nprobe < 1 || k_factor < 1 || !distances || !labels) {
+
+int mvg_server_create_sq8(mvg_sq8 *sq, int slots, int max_batch_queries, int64_t mem_budget_bytes,
+                          mvg_server **out) {
+  if (!sq) {
+    mvg::set_last_error("server needs an index");
+    return MVG_ERR_INVALID;
+  }
+  return mvg_server_create_multi(nullptr, sq, slots, max_batch_queries, mem_budget_bytes, out);
+}
+
+int mvg_server_submit_to(mvg_server *sv, int target, const float *q, int64_t nq, int k, int nprobe,
+                         int k_factor, float *distances, int64_t *labels, int64_t *ticket) {
+  if (!sv || nq < 1 || k < 1 || nprobe < 1 || k_factor < 1 || !q || !distances || !labels || !ticket) {
     mvg::set_last_error("invalid request");
+    return MVG_ERR_INVALID;
+  }
+  if ((target == 0 && !sv->ix) || (target == 1 && !sv->sq) || target < 0 || target > 1) {
+    mvg::set_last_error("the server holds no index for this target");
     return MVG_ERR_INVALID;
   }
   if ((int64_t)k * k_factor > mvg::kMaxCandidates) {
@@ -397,7 +414,8 @@ int mvg_server_submit(mvg_server *sv, const float *q, int64_t nq, int k, int npr
     return MVG_ERR_INVALID;
   }
   auto r = std::make_unique<Request>();
-  r->q.assign(q, q + nq * sv->d);
+  r->target = target;
+  r->q.assign(q, q + nq * target_d(sv, target));
   r->nq = nq;
   r->k = k;
   r->nprobe = nprobe;
@@ -414,6 +432,15 @@ int mvg_server_submit(mvg_server *sv, const float *q, int64_t nq, int k, int npr
   }
   sv->cv_work.notify_one();
   return MVG_OK;
+}
+
+int mvg_server_submit(mvg_server *sv, const float *q, int64_t nq, int k, int nprobe, int k_factor,
+                      float *distances, int64_t *labels, int64_t *ticket) {
+  if (!sv) {
+    mvg::set_last_error("invalid request");
+    return MVG_ERR_INVALID;
+  }
+  return mvg_server_submit_to(sv, sv->ix ? 0 : 1, q, nq, k, nprobe, k_factor, distances, labels, ticket);
 }
 
 int mvg_server_wait(mvg_server *sv, int64_t ticket, double *latency_ms) {

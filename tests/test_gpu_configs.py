@@ -410,3 +410,41 @@ def test_server_over_sq8_d768_cosine(gpu, emb768):
             assert_same(D, I, Da, Ia)
     finally:
         srv.close()
+
+
+def test_one_loop_serves_ivfpq_128_and_sq8_768(gpu, ivfpq128, emb768):
+    # both layouts resident and served by one dispatcher, one set of slots and
+    # one memory budget; requests of the two indexes arrive interleaved, with
+    # mixed k, and each result equals the same request searched alone
+    x, q, oi, gi = ivfpq128
+    xe, qe = emb768
+    sx = gpu.IndexFlatSQ8(768, 1)
+    sx.add(xe)
+    gi.set_search_mode("mixed")
+    rng = np.random.default_rng(250)
+    srv = gpu.Server.both(gi, sx, slots=4, max_batch=400, mem_budget=512 << 20)
+    try:
+        reqs = []
+        for i in range(40):
+            k = int(rng.choice([10, 100]))
+            nq = int(rng.integers(1, 120))
+            if i % 2 == 0:
+                qq = clustered(260 + i, nq, 128, nclusters=120)
+                nprobe, kf = int(rng.integers(8, 48)), int(rng.choice([1, 4, 30]))
+                reqs.append(("ivf", qq, k, nprobe, kf, srv.submit(qq, k=k, nprobe=nprobe, k_factor=kf)))
+            else:
+                qq = unit_mixture(300 + i, nq, 768)
+                reqs.append(("sq8", qq, k, 1, 2, srv.submit(qq, k=k, nprobe=1, k_factor=2, target=sx)))
+        outs = [(kind, qq, k, npb, kf, srv.wait(t)) for kind, qq, k, npb, kf, t in reqs]
+        st = srv.stats()
+        assert st["completed_queries"] == sum(r[1].shape[0] for r in reqs)
+        for kind, qq, k, npb, kf, (D, I, lat) in outs:
+            if kind == "ivf":
+                gi.nprobe = npb
+                Da, Ia = gi.search(qq, k, k_factor=kf)
+            else:
+                Da, Ia = sx.search(qq, k, k_factor=kf)
+            assert_same(D, I, Da, Ia)
+    finally:
+        srv.close()
+        gi.set_search_mode("plain")
