@@ -41,7 +41,7 @@ DEEP_MIN = 513  # k * k_factor from which the library runs the deep-pool scan ke
 # two rocprofv3 --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the
 # scales a 1 GiB device copy gives for these counters); the counters cannot
 # be read inside this process, so the value is the profiled one.
-SCAN_TRAFFIC_PROFILED = {"at": ("mixed", 64, 288), "bytes": 5_155_072 * 1024 * 2 + 1_362_004 * 1024,
+SCAN_TRAFFIC_PROFILED = {"at": ("mixed", 64, 288), "bytes": 5_505_973 * 1024 * 2 + 1_362_437 * 1024,
                          "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, "
                                    "profiles/ivfpq_1Mx128_hbm_traffic.md"}
 
