@@ -236,6 +236,16 @@ int mvg_ivfpq_set_search_mode(mvg_ivfpq *h, int precision, int split_scan);
  * index_flat_sq8.mojo:317-367) and reranks only the best max(64, per_k * k)
  * in exact FP32. per_k 0 reranks every candidate in FP32. Default 8. */
 int mvg_ivfpq_set_prefilter(mvg_ivfpq *h, int per_k);
+/* Mixed-form scan tables (no reference counterpart; pq.mojo:252-292 builds
+ * the FP32 tables this quantizes): bits 8 makes the mixed L2 deep-pool scan
+ * (k * k_factor at or above the deep-pool threshold, M in {8, 12, 16}) that
+ * is followed by the exact rerank (k_factor > 1) sum 8-bit tables: each
+ * subquantizer's FP32 table minus its minimum, in steps of one scale per
+ * query (the largest range over 255), rounded, summed exactly in integers.
+ * Its candidate distances are then within M * scale / 2 of the FP32 table
+ * sums; the returned distances are the rerank's. Every other form and shape
+ * keeps FP32 tables. bits 32 (default): FP32 tables everywhere. */
+int mvg_ivfpq_set_table_bits(mvg_ivfpq *h, int bits);
 /* Skewed-list load balance (no reference counterpart): entries_per_item > 0
  * runs the scan as persistent workgroups over a device-built queue of
  * (query, entry-range) items of about that many list entries (at most 8 per

@@ -80,6 +80,7 @@ SIGNATURES = {
     "mvg_ivfpq_search_device_noalloc": (_i, [_p, _p, _i64, _i, _p, _i, _i, _p, _i64, _p, _p, _p]),
     "mvg_ivfpq_set_search_mode": (_i, [_p, _i, _i]),
     "mvg_ivfpq_set_prefilter": (_i, [_p, _i]),
+    "mvg_ivfpq_set_table_bits": (_i, [_p, _i]),
     "mvg_ivfpq_set_work_queue": (_i, [_p, _i]),
     "mvg_ivfpq_set_deep_select": (_i, [_p, _i, _i]),
     "mvg_ivfpq_set_profiling": (_i, [_p, _i]),

@@ -916,6 +916,7 @@ extern "C" int mvg_ivfpq_search_device_noalloc(mvg_ivfpq *h, const float *q, int
   const bool mixed = h->precision != 0;
   if (h->dirty || h->row_of_dirty || (mixed && h->tables_dirty) ||
       (mixed && h->metric == MVG_METRIC_L2 && h->pre_dirty) ||
+      (mixed && h->metric == MVG_METRIC_L2 && h->table_bits == 8 && !h->pre8_ready) ||
       (mixed && h->prefilter > 0 && h->keep_vectors && h->vq_dirty))
     throw Error(mvg::kState,
                 "the index is not prepared: run one search (mvg_ivfpq_search or "
@@ -988,6 +989,16 @@ void mvg::ivf_prepare(mvg_ivfpq *h) {
                                 h->ptab.as<float>(), h->ncsr, h->entry_pre.as<float>(), h->s);
     MVG_HIP(hipStreamSynchronize(h->s));
     h->pre_dirty = false;
+    h->pre8_ready = false;
+  }
+  if (h->precision != 0 && h->metric == MVG_METRIC_L2 && h->table_bits == 8 && !h->pre8_ready) {
+    h->entry_pre8.reserve(sizeof(float) * (size_t)std::max<int64_t>(h->ncsr, 1));
+    h->table_mu.reserve(sizeof(float) * (size_t)h->d);
+    mvg::precompute_centered_terms(h->coarse.as<float>(), h->nlist, h->d, h->codes.as<uint8_t>(), h->M,
+                                   h->pq.as<float>(), h->entry_pre.as<float>(), h->ncsr,
+                                   h->table_mu.as<float>(), h->entry_pre8.as<float>(), h->s);
+    MVG_HIP(hipStreamSynchronize(h->s));
+    h->pre8_ready = true;
   }
 }
 
@@ -1053,6 +1064,12 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
       a.vq_scale = h->vq_scale;
       a.prefilter = h->prefilter;
     }
+    a.table_bits = h->table_bits;
+    if (h->metric == MVG_METRIC_L2 && h->table_bits == 8) {
+      if (!h->pre8_ready) throw Error(mvg::kState, "8-bit table terms are not prepared");
+      a.pre8 = h->entry_pre8.as<float>();
+      a.mu = h->table_mu.as<float>();
+    }
   }
   mvg::ivfpq_search(a, work, s);
 }
@@ -1112,6 +1129,14 @@ int mvg_ivfpq_set_prefilter(mvg_ivfpq *h, int per_k) {
   require(h && per_k >= 0, "per_k must be >= 0");
   mvg::ivf_check_unserved(h);
   h->prefilter = per_k;
+  API_END
+}
+
+int mvg_ivfpq_set_table_bits(mvg_ivfpq *h, int bits) {
+  API_BEGIN
+  require(h && (bits == 8 || bits == 32), "bits must be 8 or 32");
+  mvg::ivf_check_unserved(h);
+  h->table_bits = bits;
   API_END
 }
 

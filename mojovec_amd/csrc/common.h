@@ -107,7 +107,8 @@ __host__ __device__ inline uint32_t f2key(float f) {
   f = f + 0.0f;
   uint32_t u;
   memcpy(&u, &f, 4);
-  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  // negative: all bits flipped; otherwise the sign bit set
+  return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
 }
 
 __host__ __device__ inline float key2f(uint32_t k) {

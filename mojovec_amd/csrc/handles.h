@@ -16,6 +16,8 @@ struct mvg_ivfpq {
   std::vector<int64_t> top_sum;  // top_sum[p]: entries in the p largest lists
   mvg::DevBuf ptab, coarse_bf16, coarse_norm, entry_pre;
   bool pre_dirty = true;  // entry_pre out of date with the lists or tables
+  mvg::DevBuf entry_pre8, table_mu;  // 8-bit tables: centred entry terms, centre
+  bool pre8_ready = false;
   int64_t ntotal = 0;
   std::vector<std::vector<int32_t>> lids;   // inverted lists, append order
   std::vector<std::vector<uint8_t>> lcodes;
@@ -25,6 +27,7 @@ struct mvg_ivfpq {
   mvg::KmeansWork kw;
   // mixed form: SQ8 codes of the stored rows for the pre-rerank (global range
   // as IndexFlatSQ8 computes it); rebuilt when rows were added
+  int table_bits = 32;  // mixed L2 deep-pool scan with a rerank: 8 or 32
   int prefilter = 8;  // pre-rerank keeps max(64, prefilter * k) candidates; 0: off
   bool vq_dirty = true;
   float vq_min = 0.0f, vq_scale = 1.0f;

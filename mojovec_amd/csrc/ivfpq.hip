@@ -40,6 +40,12 @@ __device__ __forceinline__ float adc_sum(const uint8_t *__restrict__ code, const
   return dist;
 }
 
+// Compile-time bool passed to generic lambdas as an argument.
+template <bool B>
+struct BoolC {
+  static constexpr bool value = B;
+};
+
 // Code words of one entry for the fixed-M scans (MF = 8, 12 or 16), loaded
 // ahead of the LUT lookups so that a thread keeps several entries' loads in
 // flight; adc_words sums them in the same order as adc_sum.
@@ -85,12 +91,25 @@ __device__ __forceinline__ float lds_word(uint32_t a) {
   return *reinterpret_cast<lds_float *>(static_cast<uintptr_t>(a));
 }
 
+// Byte 0 of w times 4 in one SDWA op. The compiler selects SDWA for bytes
+// 1-3 but writes byte 0 as a shift and a mask (two ops).
+__device__ __forceinline__ uint32_t byte0_x4(uint32_t w) {
+  uint32_t a;
+  asm("v_lshlrev_b32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+      : "=v"(a)
+      : "v"(2u), "v"(w));
+  return a;
+}
+
 template <int MF, int U, int M_>
 __device__ __forceinline__ void adc_step(const Code<MF> (&c)[U], float (&out)[U]) {
   if constexpr (M_ < MF) {
 #pragma unroll
-    for (int u = 0; u < U; u++)
-      out[u] += lds_word(M_ * kKsub * 4 + (((c[u].w[M_ >> 2] >> (8 * (M_ & 3))) & 0xFFu) << 2));
+    for (int u = 0; u < U; u++) {
+      const uint32_t a = (M_ & 3) == 0 ? byte0_x4(c[u].w[M_ >> 2])
+                                       : (((c[u].w[M_ >> 2] >> (8 * (M_ & 3))) & 0xFFu) << 2);
+      out[u] += lds_word(M_ * kKsub * 4 + a);
+    }
     adc_step<MF, U, M_ + 1>(c, out);
   }
 }
@@ -106,6 +125,46 @@ __device__ __forceinline__ void adc_words_multi(const Code<MF> (&c)[U], const fl
 #pragma unroll
   for (int u = 0; u < U; u++) out[u] = init[u];
   adc_step<MF, U, 0>(c, out);
+}
+
+// 8-bit tables (mixed L2 deep-pool form): byte at LDS address a, with the
+// table at LDS address 0 as for lds_word. A 256-entry byte table spans 64
+// dwords, so a wavefront's 64 lookups into one subquantizer's table meet at
+// most two dwords per bank, against four for the FP32 table.
+typedef __attribute__((address_space(3))) const uint8_t lds_uint8;
+__device__ __forceinline__ uint32_t lds_byte(uint32_t a) {
+  return *reinterpret_cast<lds_uint8 *>(static_cast<uintptr_t>(a));
+}
+
+// Sums of the U entries' 8-bit table values, exact in integers (at most
+// 16 * 255), so their order is free: the lookups are issued first, then
+// added pairwise.
+template <int MF, int U>
+__device__ __forceinline__ void adc_q8_multi(const Code<MF> (&c)[U], uint32_t (&out)[U]) {
+  uint32_t t[U][MF];
+#pragma unroll
+  for (int m = 0; m < MF; m++)
+#pragma unroll
+    for (int u = 0; u < U; u++) t[u][m] = lds_byte(m * kKsub + ((c[u].w[m >> 2] >> (8 * (m & 3))) & 0xFFu));
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    uint32_t acc = 0u;
+#pragma unroll
+    for (int m = 0; m < MF; m += 2) acc += t[u][m] + t[u][m + 1];
+    out[u] = acc;
+  }
+}
+
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
 }
 
 // One workgroup per (query, probe rank).
@@ -1348,11 +1407,16 @@ constexpr int kThreshAppend = 2048;  // appended keys between threshold flushes 
 constexpr int kThreshJ = (kMaxCandidates + kThreshAppend + kNT - 1) / kNT;
 
 struct StreamLayout {
-  int lut, rv, poff, loff, base, cand, tmp, scr, hist, sh, st, total;
-  __host__ __device__ StreamLayout(int M, int d, int kk, int maxp, int mode, bool sorted = true) {
+  int lut, rv, poff, loff, base, cand, tmp, scr, hist, sh, st, qt, total;
+  // q8 (deep pool only): 8-bit tables of M * 256 bytes at 0; the FP32
+  // tables they are quantized from are staged in the digit histogram's
+  // space (M <= 16), and qt holds each subquantizer's minimum and range.
+  __host__ __device__ StreamLayout(int M, int d, int kk, int maxp, int mode, bool sorted = true,
+                                   bool q8 = false) {
     const bool deep = mode == 1;
     lut = 0;
-    rv = lut + M * kKsub * 4;
+    qt = 0;
+    rv = (lut + M * kKsub * (q8 ? 1 : 4) + 15) & ~15;
     int tables_end = (rv + 4 * d + 15) & ~15;
     if (deep) {
       // the digit histogram follows the tables; the winners (cand: kk keys,
@@ -1374,6 +1438,10 @@ struct StreamLayout {
       scr = cand;
       st = after;
       total = st + (int)sizeof(DeepShared);
+      if (q8) {
+        qt = (total + 15) & ~15;
+        total = qt + 2 * 16 * 4;
+      }
     } else if (mode == 2) {  // running candidate set with threshold flushes
       cand = after;
       tmp = cand;
@@ -1505,8 +1573,12 @@ __device__ uint64_t threshold_compact(uint64_t *cand, int n, int kk, int *hist, 
   return thr;
 }
 
-template <int MF, bool IP, int MODE>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void ivfpq_stream_scan_kernel(
+// Four waves per SIMD for the L2 deep-pool form (its few spills are outside
+// the scan loop); three for the others, whose scan loops would spill at four.
+// Q8 (L2 deep pool, fixed M, exact rerank after the scan): 8-bit tables.
+#define DEEP_MODE_L2(IP, MODE) (!(IP) && (MODE) == 1)
+template <int MF, bool IP, int MODE, bool Q8 = false>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(!IP && MODE == 1 ? 4 : 3))) void ivfpq_stream_scan_kernel(
     const float *__restrict__ q, int64_t nq, int d, int M, const float *__restrict__ coarse,
     const float *__restrict__ pq, const float *__restrict__ pre,
     const int64_t *__restrict__ list_off, const uint8_t *__restrict__ codes,
@@ -1515,13 +1587,16 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
     const uint8_t *__restrict__ excl, int64_t excl_len, uint64_t *__restrict__ out_keys,
     float *__restrict__ D, int64_t *__restrict__ I, float pad_dist, uint32_t *__restrict__ seg_base,
     int64_t seg_cap, unsigned *__restrict__ counter, unsigned long long *__restrict__ pool_writes,
-    const int32_t *__restrict__ order) {
+    const int32_t *__restrict__ order, const float *__restrict__ mu) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // Q8: pre holds the centred entry terms and the tables are taken over
+  // q - mu (mu: column means of the coarse centroids).
   // MODE 0: running top-kk, rank / radix merge per flush; 1: deep pool in
   // HBM (DEEP); 2: running candidate set with threshold flushes (THRESH).
   constexpr bool DEEP = MODE == 1;
   constexpr bool THRESH = MODE == 2;
-  const StreamLayout lay(M, d, kk, maxp, MODE, out_keys == nullptr);
+  static_assert(!Q8 || (DEEP_MODE_L2(IP, MODE) && MF > 0), "8-bit tables: L2 deep pool, fixed M");
+  const StreamLayout lay(M, d, kk, maxp, MODE, out_keys == nullptr, Q8);
   float *lut = reinterpret_cast<float *>(smem + lay.lut);
   float *rv = reinterpret_cast<float *>(smem + lay.rv);
   int *poff = reinterpret_cast<int *>(smem + lay.poff);
@@ -1558,7 +1633,8 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
     // per-query LUT
     for (int j = tid; j < d; j += kNT) rv[j] = qq[j];
     if constexpr (DEEP) {
-      for (int b = tid; b < kTopBins; b += kNT) hist[b] = 0;
+      if constexpr (!Q8)  // Q8: the histogram's space stages the FP32 tables first
+        for (int b = tid; b < kTopBins; b += kNT) hist[b] = 0;
       if (tid == 0) {
         ds.nseg = 0;
         ds.bound = kNoBound;
@@ -1571,16 +1647,59 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
       }
     }
     __syncthreads();
+    float *ftab = Q8 ? reinterpret_cast<float *>(hist) : lut;
     for (int e = tid; e < nlut; e += kNT) {
       const int m = e >> 8;
       const float *c = pq + (int64_t)e * dsub;
       if (IP) {
-        lut[e] = -ip_pq(rv + m * dsub, c, dsub);  // pq.mojo:285-292
+        ftab[e] = -ip_pq(rv + m * dsub, c, dsub);  // pq.mojo:285-292
+      } else if (Q8) {
+        float dot = 0.0f;
+        for (int j = 0; j < dsub; j++) dot = fma_(rv[m * dsub + j] - mu[m * dsub + j], c[j], dot);
+        ftab[e] = -2.0f * dot;
       } else {
         float dot = 0.0f;
         for (int j = 0; j < dsub; j++) dot = fma_(rv[m * dsub + j], c[j], dot);
-        lut[e] = -2.0f * dot;
+        ftab[e] = -2.0f * dot;
       }
+    }
+    // Q8: each subquantizer's table minus its minimum, in steps of one
+    // scale per query (the largest range over 255), rounded to bytes. A sum
+    // of M table values is then q8off + q8scale * (sum of the bytes), within
+    // M * q8scale / 2; the exact rerank recomputes the distances it returns.
+    float q8scale = 1.0f, q8off = 0.0f;
+    if constexpr (Q8) {
+      __syncthreads();
+      float *tmin = reinterpret_cast<float *>(smem + lay.qt), *trng = tmin + 16;
+      for (int m = wave; m < MF; m += kNW) {
+        float mn = ftab[m * kKsub + lane], mx = mn;
+#pragma unroll
+        for (int c = lane + 64; c < kKsub; c += 64) {
+          const float v = ftab[m * kKsub + c];
+          mn = fminf(mn, v);
+          mx = fmaxf(mx, v);
+        }
+        mn = wave_minf(mn);
+        mx = wave_maxf(mx);
+        if (lane == 0) {
+          tmin[m] = mn;
+          trng[m] = mx - mn;
+        }
+      }
+      __syncthreads();
+      float big = 0.0f;
+#pragma unroll
+      for (int m = 0; m < MF; m++) {
+        big = fmaxf(big, trng[m]);
+        q8off += tmin[m];
+      }
+      const float inv = big > 0.0f ? 255.0f / big : 0.0f;
+      q8scale = big > 0.0f ? big / 255.0f : 1.0f;
+      uint8_t *tab8 = smem + lay.lut;
+      for (int e = tid; e < nlut; e += kNT)
+        tab8[e] = (uint8_t)min(255, (int)rintf((ftab[e] - tmin[e >> 8]) * inv));
+      __syncthreads();
+      for (int b = tid; b < kTopBins; b += kNT) hist[b] = 0;
     }
     // probe table: stream offsets, CSR starts, per-list base
     int carry = 0;
@@ -1596,6 +1715,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
           len = (int)min<int64_t>(list_off[l + 1] - off, seg_cap);
           const float *cl = coarse + (int64_t)l * d;
           b = IP ? -ip_wn<8>(rv, cl, d) : l2_w8(rv, cl, d);  // :322-324 / |q - c_l|^2
+          if (Q8) b += q8off;
         }
         loff[p] = off;
         basev[p] = b;
@@ -1620,7 +1740,23 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
     float cur_base = basev[0];
     constexpr int U = (MF > 0) ? 4 : 1;
     int round = 0;
-    for (int r0 = 0; r0 < n; r0 += kNT * U) {
+    // One round: U positions per thread. FULL: every position of the round is
+    // below n, so nothing is tested per position; EX: the exclusion mask is
+    // set. Both are fixed per call of the body, which keeps the position and
+    // mask tests out of the rounds that do not need them.
+    // DEEP: a round's appends (segment slot, -1 for none) wait in registers
+    // until the next round has issued its loads
+    int pend_slot[U];
+    uint32_t pend_key[U], pend_ev[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      pend_slot[u] = -1;
+      pend_key[u] = 0u;
+      pend_ev[u] = 0u;
+    }
+    auto scan_round = [&](const int r0, auto full_c, auto ex_c) __attribute__((always_inline)) {
+      constexpr bool FULL = decltype(full_c)::value;
+      constexpr bool EX = decltype(ex_c)::value;
       const uint64_t thr = DEEP ? kSentinel64 : st->thr;
       // DEEP: keys whose top digit exceeds this bound cannot be among the kk
       // smallest; only the others are written and counted.
@@ -1635,28 +1771,49 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
       // n > 0, and are dropped below. The loads are issued unconditionally
       // and nothing uses them until all U are in flight: a load inside the
       // position test made the compiler wait for each entry before the next.
+      // A lane whose last position of the round is still in its current
+      // list takes all U from the registers; only a lane that crosses reads
+      // the probe table.
+      const int jlast = FULL ? r0 + (U - 1) * kNT + tid : min(r0 + (U - 1) * kNT + tid, n - 1);
+      if (jlast >= cur_end) {
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int j = r0 + u * kNT + tid;
-        ev[u] = 0;
-        initv[u] = 0.0f;
-        if (j < n) {
-          while (j >= cur_end) {
-            cur++;
-            cur_end = poff[cur + 1];
-            cur_delta = (int)(loff[cur] - poff[cur]);
-            cur_base = basev[cur];
+        for (int u = 0; u < U; u++) {
+          const int j = r0 + u * kNT + tid;
+          ev[u] = 0;
+          initv[u] = 0.0f;
+          if (FULL || j < n) {
+            while (j >= cur_end) {
+              cur++;
+              cur_end = poff[cur + 1];
+              cur_delta = (int)(loff[cur] - poff[cur]);
+              cur_base = basev[cur];
+            }
+            ev[u] = (uint32_t)(j + cur_delta);
+            initv[u] = cur_base;
           }
-          ev[u] = (uint32_t)(j + cur_delta);
-          initv[u] = cur_base;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int j = r0 + u * kNT + tid;
+          ev[u] = (FULL || j < n) ? (uint32_t)(j + cur_delta) : 0u;
+          initv[u] = (FULL || j < n) ? cur_base : 0.0f;
         }
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {  // every load of the batch in flight first
         if (MF > 0) cw[u] = load_code<MF>(codes, ev[u]);
         else cw[u] = Code<MF>{};
-        idv[u] = (!DEEP || excl) ? ids[ev[u]] : -1;
+        idv[u] = (!DEEP || EX) ? ids[ev[u]] : -1;
         prev[u] = kMixed ? pre[ev[u]] : 0.0f;
+      }
+      // DEEP: the previous round's appends, issued behind this round's loads.
+      // Stored at the end of their own round, their data registers went to
+      // the next loads, which then waited for the stores to complete.
+      if constexpr (DEEP) {
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (pend_slot[u] >= 0) seg[pend_slot[u]] = ((uint64_t)pend_key[u] << 32) | pend_ev[u];
       }
 #pragma unroll
       for (int u = 0; u < U; u++)
@@ -1664,16 +1821,23 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
       // Fixed M: the U entries' sums without branches (entries past n score
       // zero codes and are dropped below), so their lookups interleave.
       float distv[U];
-      if constexpr (MF > 0) adc_words_multi<MF, U>(cw, lut, initv, distv);
+      if constexpr (Q8) {
+        uint32_t sv[U];
+        adc_q8_multi<MF, U>(cw, sv);
+#pragma unroll
+        for (int u = 0; u < U; u++) distv[u] = __builtin_fmaf(q8scale, (float)sv[u], initv[u]);
+      } else if constexpr (MF > 0) {
+        adc_words_multi<MF, U>(cw, lut, initv, distv);
+      }
       uint32_t putk[U];  // DEEP: keys to append (sentinel: none)
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const int j = r0 + u * kNT + tid;
         bool put = false;
         uint32_t key = kSentinel;
-        if (j < n) {
+        if (FULL || j < n) {
           const int32_t id = idv[u];
-          const bool skip = excl && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
+          const bool skip = EX && id >= 0 && id < excl_len && excl[id] != 0;  // :328-334
           float dist = 0.0f;
           if (!skip) {
             if constexpr (MF > 0) dist = distv[u];
@@ -1702,6 +1866,8 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
           mu[u] = __ballot(putk[u] != kSentinel);
           wtotal += (int)__popcll(mu[u]);
         }
+#pragma unroll
+        for (int u = 0; u < U; u++) pend_slot[u] = -1;
         if (wtotal > 0) {
           int at = 0;
           if (lane == 0) at = atomicAdd(&ds.nseg, wtotal);
@@ -1711,7 +1877,9 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
             if (putk[u] != kSentinel) {
               const int off = (int)__builtin_amdgcn_mbcnt_hi(
                   (uint32_t)(mu[u] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mu[u], 0u));
-              seg[at + off] = ((uint64_t)putk[u] << 32) | (uint32_t)ev[u];
+              pend_slot[u] = at + off;
+              pend_key[u] = putk[u];
+              pend_ev[u] = ev[u];
             }
             at += (int)__popcll(mu[u]);
           }
@@ -1737,6 +1905,20 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void i
         __syncthreads();
         if (st->cnt > kCandCap - kNT * U) flush_candidates(cand, tmp, kk, *st, *scr);
       }
+    };
+    // rounds whose U x kNT positions all lie below n, then the partial one
+    const int nfull = n - n % (kNT * U);
+    if (excl) {
+      for (int r0 = 0; r0 < nfull; r0 += kNT * U) scan_round(r0, BoolC<true>{}, BoolC<true>{});
+      if (nfull < n) scan_round(nfull, BoolC<false>{}, BoolC<true>{});
+    } else {
+      for (int r0 = 0; r0 < nfull; r0 += kNT * U) scan_round(r0, BoolC<true>{}, BoolC<false>{});
+      if (nfull < n) scan_round(nfull, BoolC<false>{}, BoolC<false>{});
+    }
+    if constexpr (DEEP) {  // the last round's appends
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        if (pend_slot[u] >= 0) seg[pend_slot[u]] = ((uint64_t)pend_key[u] << 32) | pend_ev[u];
     }
 
     if constexpr (!DEEP) {
@@ -1819,6 +2001,35 @@ __global__ void entry_terms_kernel(const int64_t *__restrict__ list_off, int nli
   float acc = 0.0f;
   for (int m = 0; m < M; m++) acc += t[m * kKsub + c[m]];
   pre[e] = acc;
+}
+
+// Column means of the coarse centroids: the 8-bit tables' centre.
+__global__ void column_mean_kernel(const float *__restrict__ x, int n, int d, float *__restrict__ mu) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= d) return;
+  double acc = 0.0;
+  for (int i = 0; i < n; i++) acc += x[(int64_t)i * d + j];
+  mu[j] = n > 0 ? (float)(acc / n) : 0.0f;
+}
+
+// Entry terms of the 8-bit tables: pre[e] - 2 <mu, r_e> with r_e the
+// entry's PQ reconstruction, so that pre8[e] + sum_m -2 <q_m - mu_m, r_e,m>
+// is the same distance with tables over q - mu (about half the range of
+// tables over q for data away from the origin).
+__global__ void entry_terms_centered_kernel(const uint8_t *__restrict__ codes, int M,
+                                            const float *__restrict__ pq, int dsub,
+                                            const float *__restrict__ mu,
+                                            const float *__restrict__ pre, int64_t n,
+                                            float *__restrict__ pre8) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint8_t *c = codes + e * M;
+  float acc = 0.0f;
+  for (int m = 0; m < M; m++) {
+    const float *cw = pq + ((int64_t)m * kKsub + c[m]) * dsub;
+    for (int j = 0; j < dsub; j++) acc = fma_(mu[m * dsub + j], cw[j], acc);
+  }
+  pre8[e] = pre[e] - 2.0f * acc;
 }
 
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> (mixed L2 tables).
@@ -1992,14 +2203,14 @@ void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int
 
 // Stream scan (plain IP, mixed L2). Returns false when the shape does not
 // fit its LDS layout; the caller then takes the per-list kernels.
-template <int MF, bool IP, int MODE>
-bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
-                   uint64_t *out_keys, IvfWork &w, hipStream_t s) {
+template <int MF, bool IP, int MODE, bool Q8>
+bool launch_stream_t(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                     uint64_t *out_keys, IvfWork &w, hipStream_t s) {
   constexpr bool DEEP = MODE == 1;
-  const StreamLayout lay(a.M, a.d, kk, maxp, MODE, out_keys == nullptr);
+  const StreamLayout lay(a.M, a.d, kk, maxp, MODE, out_keys == nullptr, Q8);
   const size_t sm = (size_t)lay.total;
   if (sm > 150 * 1024 || maxp > kStreamMaxProbes) return false;
-  const void *fn = (const void *)ivfpq_stream_scan_kernel<MF, IP, MODE>;
+  const void *fn = (const void *)ivfpq_stream_scan_kernel<MF, IP, MODE, Q8>;
   if (sm > 64 * 1024)
     MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   int64_t grid = a.nq, cap = INT32_MAX;
@@ -2016,13 +2227,26 @@ bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int 
     order = build_query_order(a, maxp, probes, w, s);
     counter = w.deep_counter.as<unsigned>();
   }
-  hipLaunchKernelGGL((ivfpq_stream_scan_kernel<MF, IP, MODE>), dim3((unsigned)grid), dim3(kNT), sm, s,
-                     a.q, a.nq, a.d, a.M, a.coarse, a.pq, a.pre, a.list_off, a.codes, a.ids, probes,
+  hipLaunchKernelGGL((ivfpq_stream_scan_kernel<MF, IP, MODE, Q8>), dim3((unsigned)grid), dim3(kNT), sm, s,
+                     a.q, a.nq, a.d, a.M, a.coarse, a.pq, Q8 ? a.pre8 : a.pre, a.list_off, a.codes, a.ids, probes,
                      maxp, a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D,
                      a.I, kIvfPadDist, seg, cap, counter,
-                     DEEP && w.prof.on ? w.prof.counter.as<unsigned long long>() + 1 : nullptr, order);
+                     DEEP && w.prof.on ? w.prof.counter.as<unsigned long long>() + 1 : nullptr, order,
+                     Q8 ? a.mu : nullptr);
   MVG_LAUNCH_CHECK("ivfpq_stream_scan_kernel");
   return true;
+}
+
+// 8-bit tables only where an exact rerank follows the scan (k_factor > 1):
+// the scan's distances then only choose the candidates.
+template <int MF, bool IP, int MODE>
+bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                   uint64_t *out_keys, IvfWork &w, hipStream_t s) {
+  if constexpr (DEEP_MODE_L2(IP, MODE) && MF > 0) {
+    if (a.table_bits == 8 && a.pre8 && a.mu && a.k_factor > 1 && out_keys)
+      return launch_stream_t<MF, IP, MODE, true>(a, maxp, probes, kk, out_keys, w, s);
+  }
+  return launch_stream_t<MF, IP, MODE, false>(a, maxp, probes, kk, out_keys, w, s);
 }
 
 template <int MF, bool IP>
@@ -2068,6 +2292,17 @@ void precompute_ivf_tables(const float *coarse, const float *pq, int d, int M, i
   hipLaunchKernelGGL(precompute_tables_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, s, coarse, pq,
                      d, M, nlist, ptab);
   MVG_LAUNCH_CHECK("precompute_tables_kernel");
+}
+
+void precompute_centered_terms(const float *coarse, int nlist, int d, const uint8_t *codes, int M,
+                               const float *pq, const float *pre, int64_t n, float *mu, float *pre8,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(column_mean_kernel, dim3(ceil_div(d, 64)), dim3(64), 0, s, coarse, nlist, d, mu);
+  MVG_LAUNCH_CHECK("column_mean_kernel");
+  if (n == 0) return;
+  hipLaunchKernelGGL(entry_terms_centered_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, codes, M, pq,
+                     d / M, mu, pre, n, pre8);
+  MVG_LAUNCH_CHECK("entry_terms_centered_kernel");
 }
 
 void precompute_entry_terms(const int64_t *list_off, int nlist, const uint8_t *codes, int M,

@@ -92,6 +92,12 @@ struct IvfSearchArgs {
   const uint32_t *vq_norms;
   float vq_min, vq_scale;
   int prefilter;
+  // mixed L2 deep-pool scan followed by the exact rerank: 8 selects 8-bit
+  // tables (per-subquantizer offset, one scale per query, exact integer
+  // sums); anything else FP32 tables
+  int table_bits;
+  const float *pre8;  // with table_bits 8: centred entry terms [ncsr] and
+  const float *mu;    // the centre [d] (nullable: FP32 tables)
   RowMap rows;  // explicit ids: id -> row of `vectors` / `vq_codes`
   // per-query k (nullable, device [nq]): query i returns kq[i] <= k results
   // in rows of stride k (padded past kq[i]); results equal those of a search
@@ -99,6 +105,11 @@ struct IvfSearchArgs {
   const int32_t *kq;
 };
 // pre[e] = sum_m ptab[l][m][code_m] for every entry of the CSR lists.
+// 8-bit tables of the mixed L2 form: mu = column means of the coarse
+// centroids [d], pre8[e] = pre[e] - 2 <mu, PQ reconstruction of entry e>.
+void precompute_centered_terms(const float *coarse, int nlist, int d, const uint8_t *codes, int M,
+                               const float *pq, const float *pre, int64_t n, float *mu, float *pre8,
+                               hipStream_t s);
 void precompute_entry_terms(const int64_t *list_off, int nlist, const uint8_t *codes, int M,
                             const float *ptab, int64_t ntotal, float *pre, hipStream_t s);
 // T[l][m][k] = |c_mk|^2 + 2 <c_l,m, c_mk> for the mixed L2 scan.

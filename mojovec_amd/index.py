@@ -328,6 +328,11 @@ class IndexIVFPQ:
         rerank; 0 reranks every candidate in FP32."""
         _lib.check(_lib.load().mvg_ivfpq_set_prefilter(self._h, int(per_k)))
 
+    def set_table_bits(self, bits: int = 32) -> None:
+        """Mixed form: 8 sums 8-bit tables in the L2 deep-pool scan that the
+        exact rerank follows (k_factor > 1); 32 keeps FP32 tables."""
+        _lib.check(_lib.load().mvg_ivfpq_set_table_bits(self._h, int(bits)))
+
     def set_work_queue(self, entries_per_item: int = 0) -> None:
         """Persistent (query, entry-range) work queue for skewed lists; 0 off."""
         _lib.check(_lib.load().mvg_ivfpq_set_work_queue(self._h, int(entries_per_item)))

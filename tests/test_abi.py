@@ -86,6 +86,7 @@ def test_null_handles_are_rejected_before_any_device_work():
         assert create(None, 2, 64, 1 << 20, ctypes.byref(out)) != 0
         assert b"server needs an index" in lib.mvg_last_error()
     assert lib.mvg_ivfpq_set_prefilter(None, 8) != 0
+    assert lib.mvg_ivfpq_set_table_bits(None, 8) != 0
     assert lib.mvg_last_error()
     # the later entry points check their handle first as well
     assert lib.mvg_ivfpq_add_with_ids(None, None, 1, None) != 0

@@ -149,6 +149,76 @@ def test_headline_mixed_prefilter_on_float_rows_bounds(gpu):
     assert np.max(np.abs(Dp - exact) / np.maximum(exact, 1.0)) <= 1e-5
 
 
+def _headline_pool(gi, k=10):
+    """The most probes whose largest lists still fit the 4096-candidate limit,
+    and the k_factor that makes every probed entry a candidate."""
+    sizes = np.sort(gi.list_sizes())[::-1]
+    nprobe = int(np.searchsorted(np.cumsum(sizes), 4090, side="right"))
+    kf = int(math.ceil(sizes[:nprobe].sum() / k))
+    assert nprobe >= 2 and 513 <= k * kf <= 4096
+    return nprobe, kf
+
+
+@pytest.mark.parametrize("pre", [0, 8])
+def test_8bit_tables_every_probed_entry_is_exact(ivfpq128, pre):
+    # 8-bit tables in the deep-pool scan only choose the candidates: with
+    # every probed entry a candidate, the rerank returns the exact search over
+    # the probed rows, as with FP32 tables.
+    x, q, oi, gi = ivfpq128
+    nprobe, kf = _headline_pool(gi)
+    gi.nprobe = nprobe
+    Do, Io = oi.search(q, 10, nprobe, k_factor=kf)
+    gi.set_search_mode("mixed")
+    gi.set_table_bits(8)
+    gi.set_prefilter(pre)
+    try:
+        Dg, Ig = gi.search(q, 10, k_factor=kf)
+    finally:
+        gi.set_table_bits(32)
+        gi.set_prefilter(8)
+        gi.set_search_mode("plain")
+    assert_same(Dg, Ig, Do, Io)
+
+
+def test_8bit_tables_at_kk_2880_match_fp32_tables(ivfpq128):
+    # The measured pool width over a third of the entries: the 8-bit tables'
+    # candidates give the FP32 tables' top-10 (recall >= 0.99 between the
+    # two), and a row both return carries the same exact rerank distance.
+    x, q, oi, gi = ivfpq128
+    gi.nprobe = 16
+    gi.set_search_mode("mixed")
+    try:
+        D32, I32 = gi.search(q, 10, k_factor=288)
+        gi.set_table_bits(8)
+        D8, I8 = gi.search(q, 10, k_factor=288)
+    finally:
+        gi.set_table_bits(32)
+        gi.set_search_mode("plain")
+    assert recall(I32, I8) >= 0.99
+    for i in range(q.shape[0]):
+        d32 = dict(zip(I32[i].tolist(), D32[i].tolist()))
+        for j, dv in zip(I8[i].tolist(), D8[i].tolist()):
+            if j in d32:
+                assert d32[j] == dv
+
+
+@pytest.mark.parametrize("k,kf", [(600, 1), (10, 40)])
+def test_8bit_tables_leave_other_shapes_on_fp32_tables(ivfpq128, k, kf):
+    # No rerank (k_factor 1, the scan's distances are returned) or a pool
+    # under the deep-pool threshold: the FP32 tables run, results unchanged.
+    x, q, oi, gi = ivfpq128
+    gi.nprobe = 16
+    gi.set_search_mode("mixed")
+    try:
+        D32, I32 = gi.search(q, k, k_factor=kf)
+        gi.set_table_bits(8)
+        D8, I8 = gi.search(q, k, k_factor=kf)
+    finally:
+        gi.set_table_bits(32)
+        gi.set_search_mode("plain")
+    assert_same(D8, I8, D32, I32)
+
+
 # ------------------------------------------------------- Flat at d = 768
 @pytest.fixture(scope="module")
 def emb768():
