@@ -145,7 +145,12 @@ __device__ __forceinline__ void adc_q8_multi(const Code<MF> (&c)[U], uint32_t (&
 #pragma unroll
   for (int m = 0; m < MF; m++)
 #pragma unroll
-    for (int u = 0; u < U; u++) t[u][m] = lds_byte(m * kKsub + ((c[u].w[m >> 2] >> (8 * (m & 3))) & 0xFFu));
+    for (int u = 0; u < U; u++) {
+      t[u][m] = lds_byte(m * kKsub + ((c[u].w[m >> 2] >> (8 * (m & 3))) & 0xFFu));
+      // ds_read_u8 zero-extends; without this (empty) operand barrier the
+      // compiler narrowed the sums to 16 bits and masked every byte again
+      asm("" : "+v"(t[u][m]));
+    }
 #pragma unroll
   for (int u = 0; u < U; u++) {
     uint32_t acc = 0u;
