@@ -1038,6 +1038,7 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.split_scan = h->split_scan;
   a.wq_entries = h->wq_entries;
   a.ntotal = h->ntotal;
+  a.ncsr = h->ncsr;
   a.deep_cap = h->top_sum.empty() ? 0 : h->top_sum[(size_t)a.nprobe];
   a.deep_min = h->deep_min;
   a.deep_pool = h->deep_pool == 0 ? 1 : 0;  // ABI pool 0: keys in HBM; 1: running set in LDS

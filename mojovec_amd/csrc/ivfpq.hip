@@ -72,6 +72,27 @@ __device__ __forceinline__ Code<MF> load_code(const uint8_t *__restrict__ codes,
   return c;
 }
 
+// load_code with a 32-bit byte offset (the index's code bytes below 4 GiB):
+// base + 32-bit offset addressing instead of a 64-bit address per load.
+template <int MF>
+__device__ __forceinline__ Code<MF> load_code32(const uint8_t *__restrict__ codes, uint32_t entry) {
+  Code<MF> c;
+  const uint8_t *p = codes + (uint32_t)(entry * (uint32_t)MF);
+  if constexpr (MF == 16) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(p);
+    c.w[0] = v.x; c.w[1] = v.y; c.w[2] = v.z; c.w[3] = v.w;
+  } else if constexpr (MF == 8) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(p);
+    c.w[0] = v.x; c.w[1] = v.y;
+  } else if constexpr (MF == 12) {
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(p);
+    c.w[0] = w[0]; c.w[1] = w[1]; c.w[2] = w[2];
+  } else {
+    c.w[0] = 0u;
+  }
+  return c;
+}
+
 template <int MF>
 __device__ __forceinline__ float adc_words(const Code<MF> &c, const float *lut, float init) {
   float dist = init;
@@ -158,6 +179,14 @@ __device__ __forceinline__ void adc_q8_multi(const Code<MF> (&c)[U], uint32_t (&
     for (int m = 0; m < MF; m += 2) acc += t[u][m] + t[u][m + 1];
     out[u] = acc;
   }
+}
+
+// One pool word (key << 32 | entry) at segment slot i.
+template <bool OFF32>
+__device__ __forceinline__ void seg_put(uint64_t *seg, int i, uint32_t key, uint32_t entry) {
+  const uint64_t v = ((uint64_t)key << 32) | entry;
+  if constexpr (OFF32) *reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(seg) + ((uint32_t)i << 3)) = v;
+  else seg[i] = v;
 }
 
 __device__ __forceinline__ float wave_minf(float v) {
@@ -1581,8 +1610,10 @@ __device__ uint64_t threshold_compact(uint64_t *cand, int n, int kk, int *hist, 
 // Four waves per SIMD for the L2 deep-pool form (its few spills are outside
 // the scan loop); three for the others, whose scan loops would spill at four.
 // Q8 (L2 deep pool, fixed M, exact rerank after the scan): 8-bit tables.
+// OFF32 (with Q8): code bytes, entry terms and pool segments addressed by
+// 32-bit offsets from their base (the host checks that they fit).
 #define DEEP_MODE_L2(IP, MODE) (!(IP) && (MODE) == 1)
-template <int MF, bool IP, int MODE, bool Q8 = false>
+template <int MF, bool IP, int MODE, bool Q8 = false, bool OFF32 = false>
 __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(!IP && MODE == 1 ? 4 : 3))) void ivfpq_stream_scan_kernel(
     const float *__restrict__ q, int64_t nq, int d, int M, const float *__restrict__ coarse,
     const float *__restrict__ pq, const float *__restrict__ pre,
@@ -1601,6 +1632,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(!IP && MODE
   constexpr bool DEEP = MODE == 1;
   constexpr bool THRESH = MODE == 2;
   static_assert(!Q8 || (DEEP_MODE_L2(IP, MODE) && MF > 0), "8-bit tables: L2 deep pool, fixed M");
+  static_assert(!OFF32 || Q8, "32-bit offsets: the 8-bit table form");
   const StreamLayout lay(M, d, kk, maxp, MODE, out_keys == nullptr, Q8);
   float *lut = reinterpret_cast<float *>(smem + lay.lut);
   float *rv = reinterpret_cast<float *>(smem + lay.rv);
@@ -1807,10 +1839,16 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(!IP && MODE
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {  // every load of the batch in flight first
-        if (MF > 0) cw[u] = load_code<MF>(codes, ev[u]);
-        else cw[u] = Code<MF>{};
-        idv[u] = (!DEEP || EX) ? ids[ev[u]] : -1;
-        prev[u] = kMixed ? pre[ev[u]] : 0.0f;
+        if constexpr (OFF32) {
+          cw[u] = load_code32<MF>(codes, ev[u]);
+          idv[u] = EX ? *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(ids) + (uint32_t)(ev[u] << 2)) : -1;
+          prev[u] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(pre) + (uint32_t)(ev[u] << 2));
+        } else {
+          if (MF > 0) cw[u] = load_code<MF>(codes, ev[u]);
+          else cw[u] = Code<MF>{};
+          idv[u] = (!DEEP || EX) ? ids[ev[u]] : -1;
+          prev[u] = kMixed ? pre[ev[u]] : 0.0f;
+        }
       }
       // DEEP: the previous round's appends, issued behind this round's loads.
       // Stored at the end of their own round, their data registers went to
@@ -1818,7 +1856,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(!IP && MODE
       if constexpr (DEEP) {
 #pragma unroll
         for (int u = 0; u < U; u++)
-          if (pend_slot[u] >= 0) seg[pend_slot[u]] = ((uint64_t)pend_key[u] << 32) | pend_ev[u];
+          if (pend_slot[u] >= 0) seg_put<OFF32>(seg, pend_slot[u], pend_key[u], pend_ev[u]);
       }
 #pragma unroll
       for (int u = 0; u < U; u++)
@@ -1923,7 +1961,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(!IP && MODE
     if constexpr (DEEP) {  // the last round's appends
 #pragma unroll
       for (int u = 0; u < U; u++)
-        if (pend_slot[u] >= 0) seg[pend_slot[u]] = ((uint64_t)pend_key[u] << 32) | pend_ev[u];
+        if (pend_slot[u] >= 0) seg_put<OFF32>(seg, pend_slot[u], pend_key[u], pend_ev[u]);
     }
 
     if constexpr (!DEEP) {
@@ -2208,14 +2246,14 @@ void launch_deep_df(const IvfSearchArgs &a, int maxp, const int32_t *probes, int
 
 // Stream scan (plain IP, mixed L2). Returns false when the shape does not
 // fit its LDS layout; the caller then takes the per-list kernels.
-template <int MF, bool IP, int MODE, bool Q8>
+template <int MF, bool IP, int MODE, bool Q8, bool OFF32 = false>
 bool launch_stream_t(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                      uint64_t *out_keys, IvfWork &w, hipStream_t s) {
   constexpr bool DEEP = MODE == 1;
   const StreamLayout lay(a.M, a.d, kk, maxp, MODE, out_keys == nullptr, Q8);
   const size_t sm = (size_t)lay.total;
   if (sm > 150 * 1024 || maxp > kStreamMaxProbes) return false;
-  const void *fn = (const void *)ivfpq_stream_scan_kernel<MF, IP, MODE, Q8>;
+  const void *fn = (const void *)ivfpq_stream_scan_kernel<MF, IP, MODE, Q8, OFF32>;
   if (sm > 64 * 1024)
     MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   int64_t grid = a.nq, cap = INT32_MAX;
@@ -2232,7 +2270,7 @@ bool launch_stream_t(const IvfSearchArgs &a, int maxp, const int32_t *probes, in
     order = build_query_order(a, maxp, probes, w, s);
     counter = w.deep_counter.as<unsigned>();
   }
-  hipLaunchKernelGGL((ivfpq_stream_scan_kernel<MF, IP, MODE, Q8>), dim3((unsigned)grid), dim3(kNT), sm, s,
+  hipLaunchKernelGGL((ivfpq_stream_scan_kernel<MF, IP, MODE, Q8, OFF32>), dim3((unsigned)grid), dim3(kNT), sm, s,
                      a.q, a.nq, a.d, a.M, a.coarse, a.pq, Q8 ? a.pre8 : a.pre, a.list_off, a.codes, a.ids, probes,
                      maxp, a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D,
                      a.I, kIvfPadDist, seg, cap, counter,
@@ -2248,8 +2286,14 @@ template <int MF, bool IP, int MODE>
 bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                    uint64_t *out_keys, IvfWork &w, hipStream_t s) {
   if constexpr (DEEP_MODE_L2(IP, MODE) && MF > 0) {
-    if (a.table_bits == 8 && a.pre8 && a.mu && a.k_factor > 1 && out_keys)
+    if (a.table_bits == 8 && a.pre8 && a.mu && a.k_factor > 1 && out_keys) {
+      // 32-bit offsets when the codes (the largest per-entry array) and a
+      // pool segment stay below 4 GiB
+      const int64_t cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);
+      if ((a.ncsr + 1) * MF < (int64_t(1) << 32) && cap * 8 < (int64_t(1) << 32))
+        return launch_stream_t<MF, IP, MODE, true, true>(a, maxp, probes, kk, out_keys, w, s);
       return launch_stream_t<MF, IP, MODE, true>(a, maxp, probes, kk, out_keys, w, s);
+    }
   }
   return launch_stream_t<MF, IP, MODE, false>(a, maxp, probes, kk, out_keys, w, s);
 }

@@ -82,6 +82,7 @@ struct IvfSearchArgs {
   const float *coarse_norm;     // and their squared norms
   int wq_entries;  // > 0: persistent work queue, items of about this many entries
   int64_t ntotal;  // rows of `vectors`
+  int64_t ncsr;    // list entries (CSR length of codes / ids / pre)
   int64_t deep_cap;  // sum of the nprobe largest lists: most entries one query visits
   int deep_min;      // k * k_factor from which the deep-pool scan runs (0: default)
   const float *pre;  // mixed L2: per-entry sum_m T[l][m][code_m] [ntotal] (nullable)
