@@ -37,11 +37,11 @@ RECALL_BAR = 0.95
 # k * k_factor = 4000 candidates (the device top-k limit is 4096).
 KF_SWEEP = (1, 4, 16, 64, 128, 256, 288, 320, 400)
 DEEP_MIN = 513  # k * k_factor from which the library runs the deep-pool scan kernel
-# HBM bytes per launch of the mixed scan kernel at the headline settings, from
+# HBM bytes per launch of the 8-bit-table scan kernel at the headline settings, from
 # two rocprofv3 --pmc passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the
 # scales a 1 GiB device copy gives for these counters); the counters cannot
 # be read inside this process, so the value is the profiled one.
-SCAN_TRAFFIC_PROFILED = {"at": ("mixed", 64, 288), "bytes": 5_505_973 * 1024 * 2 + 1_362_437 * 1024,
+SCAN_TRAFFIC_PROFILED = {"at": ("mixed8", 64, 288), "bytes": 5_464_962 * 1024 * 2 + 1_384_909 * 1024,
                          "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, "
                                    "profiles/ivfpq_1Mx128_hbm_traffic.md"}
 
