@@ -303,6 +303,169 @@ __device__ __forceinline__ void wide_flush(uint64_t *top, uint64_t *buf, int *cn
   if (lane == 0) cnt[qq] = 0;
 }
 
+// Per-lane selection state of one wavefront's 16 query rows in the wide
+// forms: the lane's four queries are rows 4 * lgrp + j of the wavefront. The
+// integer kinds test candidates with an exact integer prefilter derived from
+// the running kk-th distance (a superset: the rounding of the FP32 distance
+// is covered by a margin), then the exact key on the rare pass.
+template <int KIND>
+struct WideSel {
+  float thr_f[4], qa[4], qb[4];
+  int64_t qc[4], qsx[4];
+  int ithr[4];
+  bool qok[4];
+  int cnt_r[4];
+
+  __device__ __forceinline__ void init(int64_t q0, int wq, int qcount, int lgrp, const float *qnorm,
+                                       const float *qinv, const uint32_t *qn_u, const uint32_t *qs_u,
+                                       int d_true) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int qi = wq + 4 * lgrp + j;
+      qok[j] = qi < qcount;
+      const int64_t qg = q0 + (qok[j] ? qi : 0);
+      thr_f[j] = INFINITY;
+      qa[j] = (KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP) ? qnorm[qg] : 0.0f;
+      qb[j] = (KIND == FP8_L2 || KIND == FP8_IP) ? qinv[qg] : 0.0f;
+      qc[j] = KIND == U8_L2 ? (int64_t)qn_u[qg] : 0;
+      qsx[j] = KIND == U8_L2 ? (int64_t)qs_u[qg] - 128 * (int64_t)d_true : 0;
+      ithr[j] = KIND == I8_IP ? -(1 << 30) : -600000000;  // no bound yet: everything passes
+      cnt_r[j] = 0;
+    }
+  }
+
+  // I8_IP: dist = fmul_rn(-acc, s) <= thr  =>  acc >= floor(-thr / s) - 2.
+  // U8_L2: dist = fmul_rn(max(v, 0), s) with v = qn + xn - 2 acc - 256 (sq + sx)
+  // - 32768 d, and dist <= thr  =>  v <= vthr = floor(thr / s (1 + 2^-20)) + 4,
+  // i.e. 2 acc >= ithr + (xn - 256 sx) with ithr = qn - 256 sq - 32768 d - vthr
+  // (clamped: |acc| <= 128^2 d and d <= 1264 keep every sum inside int32).
+  __device__ __forceinline__ void int_thr(int j, float scale_sq, int d_true) {
+    if (KIND == I8_IP) {
+      const float qt = -thr_f[j] / scale_sq;
+      ithr[j] = thr_f[j] == INFINITY ? -(1 << 30)
+                                     : (int)fminf(fmaxf(floorf(qt), -1073741824.0f), 1073741824.0f) - 2;
+    } else if (KIND == U8_L2) {
+      int64_t vthr;
+      if (thr_f[j] == INFINITY) vthr = (int64_t)1 << 40;
+      else if (thr_f[j] < 0.0f) vthr = -((int64_t)1 << 40);
+      else {  // r is NaN for scale 0: no bound
+        const float r = thr_f[j] / scale_sq;
+        vthr = r <= 1.0e12f ? (int64_t)floorf(r + r * 0x1p-20f) + 4 : ((int64_t)1 << 40);
+      }
+      const int64_t c = qc[j] - 256 * qsx[j] - 32768 * (int64_t)d_true - vthr;
+      ithr[j] = (int)max<int64_t>(-600000000, min<int64_t>(c, 300000000));
+    }
+  }
+
+  // The candidates of one step: tile t holds columns pb0 + t * 16 + lrow
+  // (valid up to pb0 + lim), cv the step's per-column terms at this lane's
+  // column. Passing keys go to the queries' append buffers; a buffer near
+  // full is merged into its query's top kk and the bounds are refreshed.
+  __device__ __forceinline__ void step(const floatx4 (&accf)[16], const intx4 (&acci)[16], int64_t pb0,
+                                       const uint2 *cv, int lim, int64_t b_begin, const uint8_t *excl,
+                                       int64_t excl_len, const uint32_t *xn_u, const uint32_t *xs_u,
+                                       float scale_sq, int d_true, uint64_t *top, uint64_t *buf, int *cnt,
+                                       int kk) {
+    constexpr bool kColTerms = KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP || KIND == U8_L2;
+    const int lane = threadIdx.x & 63, lrow = lane & 15, lgrp = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      const int64_t col = pb0 + t * 16 + lrow;
+      const bool col_ok = t * 16 + lrow <= lim && !(excl && col < excl_len && excl[col] > 0);
+      const int64_t cc = col_ok ? col : b_begin;
+      float xa = 0.0f, xb = 0.0f;
+      int xcorr = 0;
+      if (kColTerms) {
+        const uint2 v = cv[t * 16];
+        xa = __uint_as_float(v.x);
+        xb = __uint_as_float(v.y);
+        xcorr = (int)v.x;
+      }
+      float dist[4];
+      bool pass[4];
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (KIND == I8_IP) {
+          pass[j] = acci[t][j] >= ithr[j];
+        } else if (KIND == U8_L2) {
+          pass[j] = 2 * acci[t][j] >= ithr[j] + xcorr;
+        } else {
+          if (KIND == BF16_L2) {
+            dist[j] = (qa[j] + xa) - 2.0f * accf[t][j];
+          } else if (KIND == BF16_IP) {
+            dist[j] = -accf[t][j];
+          } else if (KIND == FP8_L2) {
+            dist[j] = (qa[j] + xa) - 2.0f * (accf[t][j] * (qb[j] * xb));
+          } else {
+            dist[j] = (qa[j] + xa) - accf[t][j] * (qb[j] * xb);
+          }
+          pass[j] = dist[j] <= thr_f[j];
+        }
+        pass[j] = pass[j] && col_ok && qok[j];
+        any |= pass[j];
+      }
+      if (__ballot(any) == 0) continue;
+      const uint64_t gmask = 0xFFFFull << (16 * lgrp);
+      const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (KIND == I8_IP) {
+          dist[j] = __fmul_rn(-(float)acci[t][j], scale_sq);
+        } else if (KIND == U8_L2) {
+          const int64_t sx = (int64_t)xs_u[cc] - 128 * (int64_t)d_true;
+          const int64_t dot = (int64_t)acci[t][j] + 128 * (qsx[j] + sx) + 16384 * (int64_t)d_true;
+          const int64_t v = qc[j] + (int64_t)xn_u[cc] - 2 * dot;
+          dist[j] = __fmul_rn((float)(uint32_t)(v > 0 ? v : 0), scale_sq);
+        }
+        const uint64_t key = pack_key(dist[j], (uint32_t)col);
+        // exact key order against the stored kk-th key (ties: lower id)
+        const bool take = pass[j] && dist[j] <= thr_f[j] &&
+                          (thr_f[j] == INFINITY || key < top[(4 * lgrp + j) * kk + kk - 1]);
+        const uint64_t m = __ballot(take);
+        if (m == 0) continue;
+        if (take) buf[(4 * lgrp + j) * kWCap + cnt_r[j] + __popcll(m & gmask & below)] = key;
+        cnt_r[j] += __popcll(m & gmask);
+      }
+      // room for the next tile's 16 columns per query, else flush
+      bool full = false;
+#pragma unroll
+      for (int j = 0; j < 4; j++) full |= cnt_r[j] > kWCap - 16;
+      if (__ballot(full) != 0) {
+        if (lrow == 0) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) cnt[4 * lgrp + j] = cnt_r[j];
+        }
+        for (int qq = 0; qq < 16; qq++)
+          if (cnt[qq] > kWCap - 16) wide_flush(top, buf, cnt, qq, kk);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          cnt_r[j] = cnt[4 * lgrp + j];
+          const uint64_t th = top[(4 * lgrp + j) * kk + kk - 1];
+          thr_f[j] = th == kSentinel64 ? INFINITY : key2f((uint32_t)(th >> 32));
+          int_thr(j, scale_sq, d_true);
+        }
+      }
+    }
+  }
+
+  // Last merges, then the wavefront's 16 queries' top kk of this split.
+  __device__ __forceinline__ void finish(uint64_t *top, uint64_t *buf, int *cnt, int kk, int64_t q0, int wq,
+                                         int qcount, uint64_t *part, int S, int split) {
+    const int lane = threadIdx.x & 63, lrow = lane & 15, lgrp = lane >> 4;
+    if (lrow == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) cnt[4 * lgrp + j] = cnt_r[j];
+    }
+    for (int qq = 0; qq < 16; qq++)
+      if (cnt[qq] > 0) wide_flush(top, buf, cnt, qq, kk);
+    for (int qq = 0; qq < 16; qq++) {
+      if (wq + qq >= qcount) break;
+      if (lane < kk) part[((q0 + wq + qq) * S + split) * (int64_t)kk + lane] = top[qq * kk + lane];
+    }
+  }
+};
+
 template <int KIND>
 __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
     const uint8_t *__restrict__ Q, const uint8_t *__restrict__ X, int64_t nq, int64_t n,
@@ -345,50 +508,8 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
   for (int i = lane; i < 16 * kk; i += 64) top[i] = kSentinel64;
   if (lane < 16) cnt[lane] = 0;
 
-  // per-lane state of the lane's four queries (rows 4 * lgrp + j). The
-  // integer kinds test candidates with an exact integer prefilter derived
-  // from the running kk-th distance (a superset: the rounding of the FP32
-  // distance is covered by a margin), then the exact key on the rare pass.
-  float thr_f[4], qa[4], qb[4];
-  int64_t qc[4], qsx[4];
-  int ithr[4];
-  bool qok[4];
-  int cnt_r[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int qi = wq + 4 * lgrp + j;
-    qok[j] = qi < qcount;
-    const int64_t qg = q0 + (qok[j] ? qi : 0);
-    thr_f[j] = INFINITY;
-    qa[j] = (KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP) ? qnorm[qg] : 0.0f;
-    qb[j] = (KIND == FP8_L2 || KIND == FP8_IP) ? qinv[qg] : 0.0f;
-    qc[j] = KIND == U8_L2 ? (int64_t)qn_u[qg] : 0;
-    qsx[j] = KIND == U8_L2 ? (int64_t)qs_u[qg] - 128 * (int64_t)d_true : 0;
-    ithr[j] = KIND == I8_IP ? -(1 << 30) : -600000000;  // no bound yet: everything passes
-  }
-  // I8_IP: dist = fmul_rn(-acc, s) <= thr  =>  acc >= floor(-thr / s) - 2.
-  // U8_L2: dist = fmul_rn(max(v, 0), s) with v = qn + xn - 2 acc - 256 (sq + sx)
-  // - 32768 d, and dist <= thr  =>  v <= vthr = floor(thr / s (1 + 2^-20)) + 4,
-  // i.e. 2 acc >= ithr + (xn - 256 sx) with ithr = qn - 256 sq - 32768 d - vthr
-  // (clamped: |acc| <= 128^2 d and d <= 1264 keep every sum inside int32).
-#define MVG_INT_THR(J)                                                                       \
-  {                                                                                          \
-    if (KIND == I8_IP) {                                                                     \
-      const float qt = -thr_f[J] / scale_sq;                                                 \
-      ithr[J] = thr_f[J] == INFINITY ? -(1 << 30)                                            \
-                                     : (int)fminf(fmaxf(floorf(qt), -1073741824.0f), 1073741824.0f) - 2; \
-    } else if (KIND == U8_L2) {                                                              \
-      int64_t vthr;                                                                          \
-      if (thr_f[J] == INFINITY) vthr = (int64_t)1 << 40;                                     \
-      else if (thr_f[J] < 0.0f) vthr = -((int64_t)1 << 40);                                  \
-      else {  /* r is NaN for scale 0: no bound */                                          \
-        const float r = thr_f[J] / scale_sq;                                                 \
-        vthr = r <= 1.0e12f ? (int64_t)floorf(r + r * 0x1p-20f) + 4 : ((int64_t)1 << 40);    \
-      }                                                                                      \
-      const int64_t c = qc[J] - 256 * qsx[J] - 32768 * (int64_t)d_true - vthr;               \
-      ithr[J] = (int)max<int64_t>(-600000000, min<int64_t>(c, 300000000));                   \
-    }                                                                                        \
-  }
+  WideSel<KIND> sel;
+  sel.init(q0, wq, qcount, lgrp, qnorm, qinv, qn_u, qs_u, d_true);
 
   const int nks = row_bytes / 64;    // 64-byte k steps of a row
   const int nsl = (nks + 1) >> 1;    // 128-byte slices per step
@@ -519,89 +640,10 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
       sb ^= 1;
     }
     {
-      const int64_t pb0 = b0;
       const uint2 *cv = colv + sp * kStepCols + lrow;
       sp = sp == 2 ? 0 : sp + 1;
-      const int lim = (int)min<int64_t>(kStepCols, b_end - pb0) - 1;
-#pragma unroll
-      for (int t = 0; t < 16; t++) {
-        const int64_t col = pb0 + t * 16 + lrow;
-        const bool col_ok = t * 16 + lrow <= lim && !(excl && col < excl_len && excl[col] > 0);
-        const int64_t cc = col_ok ? col : b_begin;
-        float xa = 0.0f, xb = 0.0f;
-        int xcorr = 0;
-        if (kColTerms) {
-          const uint2 v = cv[t * 16];
-          xa = __uint_as_float(v.x);
-          xb = __uint_as_float(v.y);
-          xcorr = (int)v.x;
-        }
-        float dist[4];
-        bool pass[4];
-        bool any = false;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if (KIND == I8_IP) {
-            pass[j] = acci[t][j] >= ithr[j];
-          } else if (KIND == U8_L2) {
-            pass[j] = 2 * acci[t][j] >= ithr[j] + xcorr;
-          } else {
-            if (KIND == BF16_L2) {
-              dist[j] = (qa[j] + xa) - 2.0f * accf[t][j];
-            } else if (KIND == BF16_IP) {
-              dist[j] = -accf[t][j];
-            } else if (KIND == FP8_L2) {
-              dist[j] = (qa[j] + xa) - 2.0f * (accf[t][j] * (qb[j] * xb));
-            } else {
-              dist[j] = (qa[j] + xa) - accf[t][j] * (qb[j] * xb);
-            }
-            pass[j] = dist[j] <= thr_f[j];
-          }
-          pass[j] = pass[j] && col_ok && qok[j];
-          any |= pass[j];
-        }
-        if (__ballot(any) == 0) continue;
-        const uint64_t gmask = 0xFFFFull << (16 * lgrp);
-        const uint64_t below = (1ull << lane) - 1;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if (KIND == I8_IP) {
-            dist[j] = __fmul_rn(-(float)acci[t][j], scale_sq);
-          } else if (KIND == U8_L2) {
-            const int64_t sx = (int64_t)xs_u[cc] - 128 * (int64_t)d_true;
-            const int64_t dot = (int64_t)acci[t][j] + 128 * (qsx[j] + sx) + 16384 * (int64_t)d_true;
-            const int64_t v = qc[j] + (int64_t)xn_u[cc] - 2 * dot;
-            dist[j] = __fmul_rn((float)(uint32_t)(v > 0 ? v : 0), scale_sq);
-          }
-          const uint64_t key = pack_key(dist[j], (uint32_t)col);
-          // exact key order against the stored kk-th key (ties: lower id)
-          const bool take = pass[j] && dist[j] <= thr_f[j] &&
-                            (thr_f[j] == INFINITY || key < top[(4 * lgrp + j) * kk + kk - 1]);
-          const uint64_t m = __ballot(take);
-          if (m == 0) continue;
-          if (take) buf[(4 * lgrp + j) * kWCap + cnt_r[j] + __popcll(m & gmask & below)] = key;
-          cnt_r[j] += __popcll(m & gmask);
-        }
-        // room for the next tile's 16 columns per query, else flush
-        bool full = false;
-#pragma unroll
-        for (int j = 0; j < 4; j++) full |= cnt_r[j] > kWCap - 16;
-        if (__ballot(full) != 0) {
-          if (lrow == 0) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) cnt[4 * lgrp + j] = cnt_r[j];
-          }
-          for (int qq = 0; qq < 16; qq++)
-            if (cnt[qq] > kWCap - 16) wide_flush(top, buf, cnt, qq, kk);
-#pragma unroll
-          for (int j = 0; j < 4; j++) {
-            cnt_r[j] = cnt[4 * lgrp + j];
-            const uint64_t th = top[(4 * lgrp + j) * kk + kk - 1];
-            thr_f[j] = th == kSentinel64 ? INFINITY : key2f((uint32_t)(th >> 32));
-            MVG_INT_THR(j)
-          }
-        }
-      }
+      sel.step(accf, acci, b0, cv, (int)min<int64_t>(kStepCols, b_end - b0) - 1, b_begin, excl,
+               excl_len, xn_u, xs_u, scale_sq, d_true, top, buf, cnt, kk);
 #pragma unroll
       for (int t = 0; t < 16; t++) {
         accf[t] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -609,20 +651,244 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
       }
     }
   }
-#undef MVG_INT_THR
 #undef MVG_LOAD_TERMS
 #undef MVG_LOAD_SLICE
 #undef MVG_STORE_SLICE
-  if (lrow == 0) {
+  sel.finish(top, buf, cnt, kk, q0, wq, qcount, part, S, split);
+}
+
+// Wide form for row sizes fixed at compile time: NKS 64-byte k steps per row
+// (the 1M x 768 layouts: int8 and FP8 rows NKS 12, BF16 rows NKS 24). Each
+// lane holds its query row's fragments for the whole row in registers, so
+// LDS keeps only two staged base slices and the selection state, and a
+// 64-query tile of 1,536-byte BF16 rows fits. A step of 256 base rows is
+// unrolled; slice loads run DEPTH slices ahead through a ring of registers,
+// and the barrier after each slice waits for the LDS writes only
+// (__syncthreads' fence would also wait for the loads in flight). FP8 runs
+// the 16x16x128 f8f6f4 MFMA with unit scales: one instruction per 128 bytes
+// of k, the lane group g holding bytes [32 g, 32 g + 32) of the slice.
+struct WideRegLayout {
+  int b, colv, top, buf, cnt, total;
+  __host__ __device__ explicit WideRegLayout(int kk) {
+    b = 0;
+    colv = b + 2 * kBSlice;
+    top = colv + 3 * kStepCols * 8;
+    buf = top + 8 * kWQ * kk;
+    cnt = buf + 8 * kWQ * kWCap;
+    total = cnt + 4 * kWQ;
+  }
+};
+
+typedef int intx8 __attribute__((ext_vector_type(8)));
+
+template <int KIND, int NKS>
+__global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
+    const uint8_t *__restrict__ Q, const uint8_t *__restrict__ X, int64_t nq, int64_t n,
+    int d_true, const float *__restrict__ qnorm, const float *__restrict__ xnorm,
+    const uint32_t *__restrict__ qn_u, const uint32_t *__restrict__ qs_u,
+    const uint32_t *__restrict__ xn_u, const uint32_t *__restrict__ xs_u, float scale_sq, int kk,
+    int64_t split_len, const uint8_t *__restrict__ excl, int64_t excl_len,
+    uint64_t *__restrict__ part, int S, const float *__restrict__ qinv,
+    const float *__restrict__ xinv) {
+  static_assert(NKS % 2 == 0, "whole 128-byte slices");
+  constexpr int NSL = NKS / 2;                                    // slices per row
+  constexpr bool F8 = KIND == FP8_L2 || KIND == FP8_IP;
+  // ring of slice loads: three deep where the registers allow (the FP8
+  // operands of the 128-k MFMA take eight registers each and spill at three)
+  constexpr int DEPTH = (NKS <= 12 && NSL % 3 == 0 && !F8) ? 3 : 2;
+  static_assert(NSL % DEPTH == 0, "a step's slices in whole turns of the ring");
+  constexpr int ROW = NKS * 64;                                   // row bytes
+  constexpr bool kColTerms = KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP || KIND == U8_L2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const WideRegLayout lay(kk);
+  uint8_t *bst = smem + lay.b;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lrow = lane & 15, lgrp = lane >> 4;
+  uint64_t *top = reinterpret_cast<uint64_t *>(smem + lay.top) + wave * 16 * kk;
+  uint64_t *buf = reinterpret_cast<uint64_t *>(smem + lay.buf) + wave * 16 * kWCap;
+  int *cnt = reinterpret_cast<int *>(smem + lay.cnt) + wave * 16;
+
+  int64_t qt;
+  int split;
+  if (!xcd_tile(nq, kWQ, S, qt, split)) return;
+  const int64_t q0 = qt * kWQ;
+  const int64_t b_begin = (int64_t)split * split_len;
+  const int64_t b_end = min(n, b_begin + split_len);
+  const int qcount = (int)min<int64_t>(kWQ, nq - q0);
+  const int wq = wave * 16;
+
+  // the lane's query row (wq + lrow): fragment h of slice sl at areg[2 sl + h]
+  // (FP8: the 32 bytes of slice sl at a8[sl])
+  uint4 areg[F8 ? 1 : NKS];
+  intx8 a8[F8 ? NSL : 1];
+  {
+    const int qi = wq + lrow;
+    const bool ok = qi < qcount;
+    const uint8_t *qr = Q + (q0 + (ok ? qi : 0)) * ROW;
+    if constexpr (F8) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) cnt[4 * lgrp + j] = cnt_r[j];
+      for (int sl = 0; sl < NSL; sl++) {
+        const intx4 lo = ok ? *reinterpret_cast<const intx4 *>(qr + sl * 128 + lgrp * 32) : intx4{0, 0, 0, 0};
+        const intx4 hi = ok ? *reinterpret_cast<const intx4 *>(qr + sl * 128 + lgrp * 32 + 16) : intx4{0, 0, 0, 0};
+        a8[sl] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NKS; i++) {
+        uint4 v = ok ? *reinterpret_cast<const uint4 *>(qr + i * 64 + lgrp * 16) : make_uint4(0, 0, 0, 0);
+        if (KIND == U8_L2) {  // u - 128 as int8
+          v.x ^= 0x80808080u; v.y ^= 0x80808080u; v.z ^= 0x80808080u; v.w ^= 0x80808080u;
+        }
+        areg[i] = v;
+      }
+    }
   }
-  for (int qq = 0; qq < 16; qq++)
-    if (cnt[qq] > 0) wide_flush(top, buf, cnt, qq, kk);
-  for (int qq = 0; qq < 16; qq++) {
-    if (wq + qq >= qcount) break;
-    if (lane < kk) part[((q0 + wq + qq) * S + split) * (int64_t)kk + lane] = top[qq * kk + lane];
+  for (int i = lane; i < 16 * kk; i += 64) top[i] = kSentinel64;
+  if (lane < 16) cnt[lane] = 0;
+  WideSel<KIND> sel;
+  sel.init(q0, wq, qcount, lgrp, qnorm, qinv, qn_u, qs_u, d_true);
+
+  const int srow = tid >> 3, sgrp = tid & 7;  // staging: 8 columns x 16 bytes per thread
+  const int sswz = wide_swz(sgrp);
+  uint4 pr[DEPTH][8];
+  auto load_slice = [&](uint4(&p)[8], int64_t B0, int SL) __attribute__((always_inline)) {
+    const uint8_t *src = X + SL * 128 + sgrp * 16;
+    const int64_t lastc = b_end - 1, c0 = B0 + srow;
+#pragma unroll
+    for (int i = 0; i < 8; i++) p[i] = *reinterpret_cast<const uint4 *>(src + min(c0 + 32 * i, lastc) * ROW);
+  };
+  auto store_slice = [&](uint4(&p)[8], int SB) __attribute__((always_inline)) {
+    uint8_t *plane = bst + SB * kBSlice + sgrp * kBPlane;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      uint4 v = p[i];
+      if (KIND == U8_L2) {
+        v.x ^= 0x80808080u; v.y ^= 0x80808080u; v.z ^= 0x80808080u; v.w ^= 0x80808080u;
+      }
+      *reinterpret_cast<uint4 *>(plane + (((srow + 32 * i) ^ sswz) * 16)) = v;
+    }
+  };
+  uint2 *colv = reinterpret_cast<uint2 *>(smem + lay.colv);
+  uint2 cterm = make_uint2(0, 0);
+  // column terms of the step at B0 (columns past b_end read the last row's:
+  // the epilogue drops those columns)
+  auto load_terms = [&](int64_t B0) __attribute__((always_inline)) {
+    if (kColTerms) {
+      const int64_t c = min(B0 + tid, b_end - 1);
+      if (KIND == U8_L2)
+        cterm.x = (uint32_t)((int)xn_u[c] - 256 * ((int)xs_u[c] - 128 * d_true));
+      else
+        cterm.x = __float_as_uint(xnorm[c]);
+      if (F8) cterm.y = __float_as_uint(xinv[c]);
+    }
+  };
+
+  // slices 0 .. DEPTH - 1 of the first step in flight, slice 0 staged
+  load_terms(b_begin);
+#pragma unroll
+  for (int r = 0; r < DEPTH; r++) load_slice(pr[r], b_begin, r);
+  store_slice(pr[0], 0);
+  if (kColTerms) colv[tid] = cterm;
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+
+  floatx4 accf[16];
+  intx4 acci[16];
+#pragma unroll
+  for (int t = 0; t < 16; t++) {
+    accf[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    acci[t] = intx4{0, 0, 0, 0};
   }
+  int sb = 0, sp = 0;
+  for (int64_t b0 = b_begin; b0 < b_end; b0 += kStepCols) {
+#pragma unroll
+    for (int sl = 0; sl < NSL; sl++) {
+      // Slice sl + DEPTH (of this step or the next) into the ring slot that
+      // slice sl leaves. Past the split's end the loads re-read its last row
+      // and nothing uses them: no branch, so that a step stays one block (at
+      // a branch per slice the compiler sank the step's MFMAs below its last
+      // barrier and held every slice's fragments at once).
+      {
+        const int tsl = sl + DEPTH < NSL ? sl + DEPTH : sl + DEPTH - NSL;
+        const int64_t tb0 = min(sl + DEPTH < NSL ? b0 : b0 + kStepCols, b_end - 1);
+        load_slice(pr[sl % DEPTH], tb0, tsl);
+        if (tsl == 0) load_terms(tb0);
+      }
+      const uint8_t *slice = bst + sb * kBSlice;
+      if constexpr (F8) {
+        const int g0 = 2 * lgrp, g1 = 2 * lgrp + 1;
+        const uint8_t *p0 = slice + g0 * kBPlane + ((lrow ^ wide_swz(g0)) * 16);
+        const uint8_t *p1 = slice + g1 * kBPlane + ((lrow ^ wide_swz(g1)) * 16);
+#pragma unroll
+        for (int quarter = 0; quarter < 4; quarter++) {
+          intx8 bv[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const intx4 lo = *reinterpret_cast<const intx4 *>(p0 + (4 * quarter + t) * 256);
+            const intx4 hi = *reinterpret_cast<const intx4 *>(p1 + (4 * quarter + t) * 256);
+            bv[t] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+          }
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+            accf[4 * quarter + t] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                a8[sl], bv[t], accf[4 * quarter + t], 0, 0, 0, 127, 0, 127);
+          __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+        }
+      } else {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const uint4 a = areg[2 * sl + h];
+          const int g = 4 * h + lgrp;
+          const uint8_t *plane = slice + g * kBPlane + ((lrow ^ wide_swz(g)) * 16);
+#pragma unroll
+          for (int half = 0; half < 2; half++) {
+            uint4 bfr[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) bfr[t] = *reinterpret_cast<const uint4 *>(plane + (8 * half + t) * 256);
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+              const int tt = 8 * half + t;
+              if (KIND == BF16_L2 || KIND == BF16_IP)
+                accf[tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                                   __builtin_bit_cast(bf16x8, bfr[t]),
+                                                                   accf[tt], 0, 0, 0);
+              else
+                acci[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(intx4, a),
+                                                                 __builtin_bit_cast(intx4, bfr[t]),
+                                                                 acci[tt], 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+          }
+        }
+      }
+      // stage slice sl + 1 (the next step's slice 0 after the last; past the
+      // split's end a buffer nobody reads)
+      store_slice(pr[(sl + 1) % DEPTH], sb ^ 1);
+      if (kColTerms && sl + 1 == NSL) colv[(sp == 2 ? 0 : sp + 1) * kStepCols + tid] = cterm;
+      // the slice's MFMAs issue before its barrier (the scheduler had moved
+      // the FP8 ones below the step's last barrier, every fragment live; for
+      // those the accumulators pass through an empty volatile asm here)
+      if constexpr (F8) {
+#pragma unroll
+        for (int t = 0; t < 16; t += 4)
+          asm volatile("" : "+a"(accf[t]), "+a"(accf[t + 1]), "+a"(accf[t + 2]), "+a"(accf[t + 3]));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      sb ^= 1;
+    }
+    const uint2 *cv = colv + sp * kStepCols + lrow;
+    sp = sp == 2 ? 0 : sp + 1;
+    sel.step(accf, acci, b0, cv, (int)min<int64_t>(kStepCols, b_end - b0) - 1, b_begin, excl, excl_len,
+             xn_u, xs_u, scale_sq, d_true, top, buf, cnt, kk);
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      accf[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      acci[t] = intx4{0, 0, 0, 0};
+    }
+  }
+  sel.finish(top, buf, cnt, kk, q0, wq, qcount, part, S, split);
 }
 
 // FP32 rows -> bf16 rows padded to row_elems (round to nearest even), plus
@@ -730,7 +996,13 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
                int &nsplit, hipStream_t s, const float *qinv, const float *xinv) {
   const int row_bytes = gemm_row_bytes(kind, d);
   const WideLayout wlay(row_bytes, kk);
-  if (kk <= kWMaxK && nq > kQT && wlay.total <= 160 * 1024) {
+  // the register-resident query tile for the 768-byte int8 / FP8 rows and the
+  // 1,536-byte BF16 rows (d = 768)
+  const bool reg12 = row_bytes == 768 && (kind == I8_IP || kind == U8_L2 || kind == FP8_L2 || kind == FP8_IP);
+  const bool reg24 = row_bytes == 1536 && (kind == BF16_L2 || kind == BF16_IP);
+  const WideRegLayout rlay(kk);
+  const bool reg = (reg12 || reg24) && rlay.total <= 160 * 1024;
+  if (kk <= kWMaxK && nq > kQT && (reg || wlay.total <= 160 * 1024)) {
     const int qtiles = ceil_div(nq, kWQ);
     // Splits of the base per query tile: one workgroup per CU at a time, so a
     // CU streams ceil(qtiles * S / cus) / S of the base; every split costs a
@@ -746,6 +1018,35 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
     S = ceil_div(n, split_len);
     nsplit = (int)S;
     part.reserve(sizeof(uint64_t) * (size_t)nq * S * kk);
+    const dim3 grid(xcd_grid(qtiles, S));
+    if (reg) {
+      const void *rf;
+      switch (kind) {
+        case BF16_L2: rf = (const void *)gemm_wide_reg_kernel<BF16_L2, 24>; break;
+        case BF16_IP: rf = (const void *)gemm_wide_reg_kernel<BF16_IP, 24>; break;
+        case I8_IP: rf = (const void *)gemm_wide_reg_kernel<I8_IP, 12>; break;
+        case FP8_L2: rf = (const void *)gemm_wide_reg_kernel<FP8_L2, 12>; break;
+        case FP8_IP: rf = (const void *)gemm_wide_reg_kernel<FP8_IP, 12>; break;
+        default: rf = (const void *)gemm_wide_reg_kernel<U8_L2, 12>; break;
+      }
+      if (rlay.total > 64 * 1024)
+        MVG_HIP(hipFuncSetAttribute(rf, hipFuncAttributeMaxDynamicSharedMemorySize, rlay.total));
+#define MVG_WREG(K, N)                                                                             \
+  hipLaunchKernelGGL((gemm_wide_reg_kernel<K, N>), grid, dim3(kNT), rlay.total, s, Q, X, nq, n, d,   \
+                     qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len, excl, excl_len, \
+                     part.as<uint64_t>(), (int)S, qinv, xinv);
+      switch (kind) {
+        case BF16_L2: MVG_WREG(BF16_L2, 24) break;
+        case BF16_IP: MVG_WREG(BF16_IP, 24) break;
+        case I8_IP: MVG_WREG(I8_IP, 12) break;
+        case FP8_L2: MVG_WREG(FP8_L2, 12) break;
+        case FP8_IP: MVG_WREG(FP8_IP, 12) break;
+        default: MVG_WREG(U8_L2, 12) break;
+      }
+#undef MVG_WREG
+      MVG_LAUNCH_CHECK("gemm_wide_reg_kernel");
+      return;
+    }
     const void *fn;
     switch (kind) {
       case BF16_L2: fn = (const void *)gemm_wide_kernel<BF16_L2>; break;
@@ -757,7 +1058,6 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
     }
     if (wlay.total > 64 * 1024)
       MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, wlay.total));
-    const dim3 grid(xcd_grid(qtiles, S));
 #define MVG_WIDE(K)                                                                               \
   hipLaunchKernelGGL(gemm_wide_kernel<K>, grid, dim3(kNT), wlay.total, s, Q, X, nq, n, row_bytes, d, \
                      qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len, excl,        \
