@@ -666,7 +666,12 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
 // and the barrier after each slice waits for the LDS writes only
 // (__syncthreads' fence would also wait for the loads in flight). FP8 runs
 // the 16x16x128 f8f6f4 MFMA with unit scales: one instruction per 128 bytes
-// of k, the lane group g holding bytes [32 g, 32 g + 32) of the slice.
+// of k. Its lane group g holds bytes [16 g, 16 g + 16) and [64 + 16 g,
+// 80 + 16 g) of the slice for both operands (any k order common to A and B
+// gives the same dot product; these are the planes the int8 and BF16 reads
+// take, without the bank conflicts of planes 2 g and 2 g + 1). Base slices
+// load through a buffer resource over the split's rows: 32-bit offsets, and
+// rows past the split's end read as zeros (the epilogue drops them).
 struct WideRegLayout {
   int b, colv, top, buf, cnt, total;
   __host__ __device__ explicit WideRegLayout(int kk) {
@@ -728,8 +733,8 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
     if constexpr (F8) {
 #pragma unroll
       for (int sl = 0; sl < NSL; sl++) {
-        const intx4 lo = ok ? *reinterpret_cast<const intx4 *>(qr + sl * 128 + lgrp * 32) : intx4{0, 0, 0, 0};
-        const intx4 hi = ok ? *reinterpret_cast<const intx4 *>(qr + sl * 128 + lgrp * 32 + 16) : intx4{0, 0, 0, 0};
+        const intx4 lo = ok ? *reinterpret_cast<const intx4 *>(qr + sl * 128 + lgrp * 16) : intx4{0, 0, 0, 0};
+        const intx4 hi = ok ? *reinterpret_cast<const intx4 *>(qr + sl * 128 + 64 + lgrp * 16) : intx4{0, 0, 0, 0};
         a8[sl] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
       }
     } else {
@@ -751,11 +756,15 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
   const int srow = tid >> 3, sgrp = tid & 7;  // staging: 8 columns x 16 bytes per thread
   const int sswz = wide_swz(sgrp);
   uint4 pr[DEPTH][8];
+  // the split's rows [b_begin, b_end) as one buffer (the host keeps a split
+  // below 2 GiB); flags word: raw byte addressing, as for gfx9 buffers
+  const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(X + b_begin * ROW), (short)0, (int)((b_end - b_begin) * ROW), 0x00020000);
   auto load_slice = [&](uint4(&p)[8], int64_t B0, int SL) __attribute__((always_inline)) {
-    const uint8_t *src = X + SL * 128 + sgrp * 16;
-    const int64_t lastc = b_end - 1, c0 = B0 + srow;
+    const int vo = (int)(B0 - b_begin + srow) * ROW + SL * 128 + sgrp * 16;
 #pragma unroll
-    for (int i = 0; i < 8; i++) p[i] = *reinterpret_cast<const uint4 *>(src + min(c0 + 32 * i, lastc) * ROW);
+    for (int i = 0; i < 8; i++)
+      p[i] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xrs, vo, i * 32 * ROW, 0));
   };
   auto store_slice = [&](uint4(&p)[8], int SB) __attribute__((always_inline)) {
     uint8_t *plane = bst + SB * kBSlice + sgrp * kBPlane;
@@ -809,13 +818,13 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
       // barrier and held every slice's fragments at once).
       {
         const int tsl = sl + DEPTH < NSL ? sl + DEPTH : sl + DEPTH - NSL;
-        const int64_t tb0 = min(sl + DEPTH < NSL ? b0 : b0 + kStepCols, b_end - 1);
+        const int64_t tb0 = min(sl + DEPTH < NSL ? b0 : b0 + kStepCols, b_end);
         load_slice(pr[sl % DEPTH], tb0, tsl);
         if (tsl == 0) load_terms(tb0);
       }
       const uint8_t *slice = bst + sb * kBSlice;
       if constexpr (F8) {
-        const int g0 = 2 * lgrp, g1 = 2 * lgrp + 1;
+        const int g0 = lgrp, g1 = lgrp + 4;
         const uint8_t *p0 = slice + g0 * kBPlane + ((lrow ^ wide_swz(g0)) * 16);
         const uint8_t *p1 = slice + g1 * kBPlane + ((lrow ^ wide_swz(g1)) * 16);
 #pragma unroll
@@ -1014,6 +1023,9 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
     for (int64_t c = 1; c <= smax; c++) best = std::min(best, (double)ceil_div(qtiles * c, cus) / c);
     int64_t S = 1;
     while (S < smax && (double)ceil_div(qtiles * S, cus) / S > 1.1 * best) S++;
+    // the register form addresses a split's rows with 32-bit buffer offsets:
+    // at most 2 GiB per split (a step past its end included)
+    if (reg) S = std::max<int64_t>(S, ceil_div(n * (int64_t)row_bytes, (int64_t(1) << 31) - 512 * (int64_t)row_bytes));
     int64_t split_len = ((ceil_div(n, S) + kStepCols - 1) / kStepCols) * kStepCols;
     S = ceil_div(n, split_len);
     nsplit = (int)S;
