@@ -22,6 +22,8 @@
 // (K = 32 bf16 or 64 int8 per instruction), and each query keeps its running
 // top-kk in LDS behind a threshold, so only candidates that beat the current
 // kk-th key are appended and merged.
+#include <utility>
+
 #include "dist.h"
 #include "kernels.h"
 #include "select.h"
@@ -291,10 +293,32 @@ struct WideLayout {
 __device__ __forceinline__ int wide_swz(int g) { return (int)((0xFCA96530u >> (4 * g)) & 0xFu); }
 
 // Merges query row qq's append buffer into its sorted top kk; the whole
-// wavefront calls it with the same qq.
+// wavefront calls it with the same qq. RANK: the rank merge below when the
+// two fit in one wavefront (the uint8 kind keeps the sorting network: with
+// the rank merge its kernels spilled).
+template <bool RANK>
 __device__ __forceinline__ void wide_flush(uint64_t *top, uint64_t *buf, int *cnt, int qq, int kk) {
   const int lane = threadIdx.x & 63;
   const int c = cnt[qq];
+  if (RANK && kk + c <= 64) {
+    // Rank merge: lanes 0..kk-1 hold the top set, the next c lanes the
+    // appended keys. Each lane counts the keys below its own, read as
+    // same-address broadcasts from LDS (independent reads; the sorting
+    // network's shuffles are LDS round trips in a dependent chain, about
+    // twenty times slower here), and stores its key at that rank. Real keys
+    // are distinct (one per column); slots past the real keys were the
+    // sentinel already and stay so.
+    const uint64_t *t = top + qq * kk, *b = buf + qq * kWCap;
+    const uint64_t v = lane < kk ? t[lane] : (lane < kk + c ? b[lane - kk] : kSentinel64);
+    int rank = 0;
+#pragma unroll 4
+    for (int i = 0; i < kk; i++) rank += t[i] < v;
+#pragma unroll 4
+    for (int i = 0; i < c; i++) rank += b[i] < v;
+    if (v != kSentinel64 && rank < kk) top[qq * kk + rank] = v;
+    if (lane == 0) cnt[qq] = 0;
+    return;
+  }
   uint64_t cand = lane < c ? buf[qq * kWCap + lane] : kSentinel64;
   cand = wave_sort64(cand);
   const uint64_t cur = lane < kk ? top[qq * kk + lane] : kSentinel64;
@@ -357,22 +381,30 @@ struct WideSel {
     }
   }
 
-  // The candidates of one step: tile t holds columns pb0 + t * 16 + lrow
-  // (valid up to pb0 + lim), cv the step's per-column terms at this lane's
-  // column. Passing keys go to the queries' append buffers; a buffer near
-  // full is merged into its query's top kk and the bounds are refreshed.
-  __device__ __forceinline__ void step(const floatx4 (&accf)[16], const intx4 (&acci)[16], int64_t pb0,
-                                       const uint2 *cv, int lim, int64_t b_begin, const uint8_t *excl,
-                                       int64_t excl_len, const uint32_t *xn_u, const uint32_t *xs_u,
-                                       float scale_sq, int d_true, uint64_t *top, uint64_t *buf, int *cnt,
-                                       int kk) {
+  // Candidates of tiles [T0, T1) of one step: tile t holds columns pb0 +
+  // t * 16 + lrow (valid up to pb0 + lim), cv the step's per-column terms at
+  // this lane's column. The tests of all the chunk's tiles come first and
+  // take no branch while nothing passes (straight-line code the scheduler
+  // can interleave with MFMAs); passing keys go to the queries' append
+  // buffers, and a buffer near full is merged into its query's top kk and
+  // the bounds are refreshed.
+  template <int T0, int T1>
+  __device__ __forceinline__ void tiles(const floatx4 (&accf)[16], const intx4 (&acci)[16], int64_t pb0,
+                                        const uint2 *cv, int lim, int64_t b_begin, const uint8_t *excl,
+                                        int64_t excl_len, const uint32_t *xn_u, const uint32_t *xs_u,
+                                        float scale_sq, int d_true, uint64_t *top, uint64_t *buf, int *cnt,
+                                        int kk) {
     constexpr bool kColTerms = KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP || KIND == U8_L2;
+    constexpr int NT = T1 - T0;
     const int lane = threadIdx.x & 63, lrow = lane & 15, lgrp = lane >> 4;
+    float dist[NT][4];
+    bool pass[NT][4];
+    bool any_chunk = false;
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
+    for (int u = 0; u < NT; u++) {
+      const int t = T0 + u;
       const int64_t col = pb0 + t * 16 + lrow;
       const bool col_ok = t * 16 + lrow <= lim && !(excl && col < excl_len && excl[col] > 0);
-      const int64_t cc = col_ok ? col : b_begin;
       float xa = 0.0f, xb = 0.0f;
       int xcorr = 0;
       if (kColTerms) {
@@ -381,46 +413,53 @@ struct WideSel {
         xb = __uint_as_float(v.y);
         xcorr = (int)v.x;
       }
-      float dist[4];
-      bool pass[4];
-      bool any = false;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (KIND == I8_IP) {
-          pass[j] = acci[t][j] >= ithr[j];
+          pass[u][j] = acci[t][j] >= ithr[j];
         } else if (KIND == U8_L2) {
-          pass[j] = 2 * acci[t][j] >= ithr[j] + xcorr;
+          pass[u][j] = 2 * acci[t][j] >= ithr[j] + xcorr;
         } else {
           if (KIND == BF16_L2) {
-            dist[j] = (qa[j] + xa) - 2.0f * accf[t][j];
+            dist[u][j] = (qa[j] + xa) - 2.0f * accf[t][j];
           } else if (KIND == BF16_IP) {
-            dist[j] = -accf[t][j];
+            dist[u][j] = -accf[t][j];
           } else if (KIND == FP8_L2) {
-            dist[j] = (qa[j] + xa) - 2.0f * (accf[t][j] * (qb[j] * xb));
+            dist[u][j] = (qa[j] + xa) - 2.0f * (accf[t][j] * (qb[j] * xb));
           } else {
-            dist[j] = (qa[j] + xa) - accf[t][j] * (qb[j] * xb);
+            dist[u][j] = (qa[j] + xa) - accf[t][j] * (qb[j] * xb);
           }
-          pass[j] = dist[j] <= thr_f[j];
+          pass[u][j] = dist[u][j] <= thr_f[j];
         }
-        pass[j] = pass[j] && col_ok && qok[j];
-        any |= pass[j];
+        pass[u][j] = pass[u][j] && col_ok && qok[j];
+        any_chunk |= pass[u][j];
       }
+    }
+    if (__ballot(any_chunk) == 0) return;
+#pragma unroll
+    for (int u = 0; u < NT; u++) {
+      const int t = T0 + u;
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < 4; j++) any |= pass[u][j];
       if (__ballot(any) == 0) continue;
+      const int64_t col = pb0 + t * 16 + lrow;
+      const int64_t cc = pass[u][0] || pass[u][1] || pass[u][2] || pass[u][3] ? col : b_begin;
       const uint64_t gmask = 0xFFFFull << (16 * lgrp);
       const uint64_t below = (1ull << lane) - 1;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (KIND == I8_IP) {
-          dist[j] = __fmul_rn(-(float)acci[t][j], scale_sq);
+          dist[u][j] = __fmul_rn(-(float)acci[t][j], scale_sq);
         } else if (KIND == U8_L2) {
           const int64_t sx = (int64_t)xs_u[cc] - 128 * (int64_t)d_true;
           const int64_t dot = (int64_t)acci[t][j] + 128 * (qsx[j] + sx) + 16384 * (int64_t)d_true;
           const int64_t v = qc[j] + (int64_t)xn_u[cc] - 2 * dot;
-          dist[j] = __fmul_rn((float)(uint32_t)(v > 0 ? v : 0), scale_sq);
+          dist[u][j] = __fmul_rn((float)(uint32_t)(v > 0 ? v : 0), scale_sq);
         }
-        const uint64_t key = pack_key(dist[j], (uint32_t)col);
+        const uint64_t key = pack_key(dist[u][j], (uint32_t)col);
         // exact key order against the stored kk-th key (ties: lower id)
-        const bool take = pass[j] && dist[j] <= thr_f[j] &&
+        const bool take = pass[u][j] && dist[u][j] <= thr_f[j] &&
                           (thr_f[j] == INFINITY || key < top[(4 * lgrp + j) * kk + kk - 1]);
         const uint64_t m = __ballot(take);
         if (m == 0) continue;
@@ -436,8 +475,10 @@ struct WideSel {
 #pragma unroll
           for (int j = 0; j < 4; j++) cnt[4 * lgrp + j] = cnt_r[j];
         }
+        // one copy of the merge (an unrolled loop inlined 16 and spilled)
+#pragma unroll 1
         for (int qq = 0; qq < 16; qq++)
-          if (cnt[qq] > kWCap - 16) wide_flush(top, buf, cnt, qq, kk);
+          if (cnt[qq] > kWCap - 16) wide_flush<KIND != U8_L2>(top, buf, cnt, qq, kk);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           cnt_r[j] = cnt[4 * lgrp + j];
@@ -449,6 +490,18 @@ struct WideSel {
     }
   }
 
+  // All 16 tiles of a step, four at a time.
+  __device__ __forceinline__ void step(const floatx4 (&accf)[16], const intx4 (&acci)[16], int64_t pb0,
+                                       const uint2 *cv, int lim, int64_t b_begin, const uint8_t *excl,
+                                       int64_t excl_len, const uint32_t *xn_u, const uint32_t *xs_u,
+                                       float scale_sq, int d_true, uint64_t *top, uint64_t *buf, int *cnt,
+                                       int kk) {
+    tiles<0, 4>(accf, acci, pb0, cv, lim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true, top, buf, cnt, kk);
+    tiles<4, 8>(accf, acci, pb0, cv, lim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true, top, buf, cnt, kk);
+    tiles<8, 12>(accf, acci, pb0, cv, lim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true, top, buf, cnt, kk);
+    tiles<12, 16>(accf, acci, pb0, cv, lim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true, top, buf, cnt, kk);
+  }
+
   // Last merges, then the wavefront's 16 queries' top kk of this split.
   __device__ __forceinline__ void finish(uint64_t *top, uint64_t *buf, int *cnt, int kk, int64_t q0, int wq,
                                          int qcount, uint64_t *part, int S, int split) {
@@ -457,8 +510,9 @@ struct WideSel {
 #pragma unroll
       for (int j = 0; j < 4; j++) cnt[4 * lgrp + j] = cnt_r[j];
     }
+#pragma unroll 1
     for (int qq = 0; qq < 16; qq++)
-      if (cnt[qq] > 0) wide_flush(top, buf, cnt, qq, kk);
+      if (cnt[qq] > 0) wide_flush<KIND != U8_L2>(top, buf, cnt, qq, kk);
     for (int qq = 0; qq < 16; qq++) {
       if (wq + qq >= qcount) break;
       if (lane < kk) part[((q0 + wq + qq) * S + split) * (int64_t)kk + lane] = top[qq * kk + lane];
@@ -686,6 +740,16 @@ struct WideRegLayout {
 
 typedef int intx8 __attribute__((ext_vector_type(8)));
 
+// f(I) for I = 0 .. N-1 with I a compile-time constant (std::integral_constant).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
 template <int KIND, int NKS>
 __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
     const uint8_t *__restrict__ Q, const uint8_t *__restrict__ X, int64_t nq, int64_t n,
@@ -704,6 +768,10 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
   static_assert(NSL % DEPTH == 0, "a step's slices in whole turns of the ring");
   constexpr int ROW = NKS * 64;                                   // row bytes
   constexpr bool kColTerms = KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP || KIND == U8_L2;
+  // PIPE: a step's selection runs in chunks during the next step's slices,
+  // beside their MFMAs (BF16 rows: 12 slices of MFMA work per step to hide
+  // it behind; measured slower for the 6-slice int8 / FP8 steps)
+  constexpr bool PIPE = NSL >= 12;
   extern __shared__ __align__(16) unsigned char smem[];
   const WideRegLayout lay(kk);
   uint8_t *bst = smem + lay.b;
@@ -802,15 +870,22 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
 
   floatx4 accf[16];
   intx4 acci[16];
+  // the previous step's accumulators, selected in chunks during this step's
+  // slices (its MFMAs run meanwhile); plim -1: no previous step yet
+  floatx4 paccf[16];
+  intx4 pacci[16];
 #pragma unroll
   for (int t = 0; t < 16; t++) {
-    accf[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-    acci[t] = intx4{0, 0, 0, 0};
+    accf[t] = paccf[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    acci[t] = pacci[t] = intx4{0, 0, 0, 0};
   }
+  int64_t pb0 = b_begin;
+  int plim = -1;
+  const uint2 *pcv = colv + lrow;
   int sb = 0, sp = 0;
   for (int64_t b0 = b_begin; b0 < b_end; b0 += kStepCols) {
-#pragma unroll
-    for (int sl = 0; sl < NSL; sl++) {
+    static_for<NSL>([&](auto slc) __attribute__((always_inline)) {
+      constexpr int sl = decltype(slc)::value;
       // Slice sl + DEPTH (of this step or the next) into the ring slot that
       // slice sl leaves. Past the split's end the loads re-read its last row
       // and nothing uses them: no branch, so that a step stays one block (at
@@ -871,6 +946,11 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
           }
         }
       }
+      // tiles [16 sl / NSL, 16 (sl + 1) / NSL) of the previous step
+      if constexpr (PIPE)
+        sel.template tiles<(16 * sl) / NSL, (16 * (sl + 1)) / NSL>(
+            paccf, pacci, pb0, pcv, plim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true, top, buf,
+            cnt, kk);
       // stage slice sl + 1 (the next step's slice 0 after the last; past the
       // split's end a buffer nobody reads)
       store_slice(pr[(sl + 1) % DEPTH], sb ^ 1);
@@ -886,17 +966,27 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
       __builtin_amdgcn_sched_barrier(0);
       asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
       sb ^= 1;
-    }
-    const uint2 *cv = colv + sp * kStepCols + lrow;
+    });
+    pcv = colv + sp * kStepCols + lrow;
     sp = sp == 2 ? 0 : sp + 1;
-    sel.step(accf, acci, b0, cv, (int)min<int64_t>(kStepCols, b_end - b0) - 1, b_begin, excl, excl_len,
-             xn_u, xs_u, scale_sq, d_true, top, buf, cnt, kk);
+    pb0 = b0;
+    plim = (int)min<int64_t>(kStepCols, b_end - b0) - 1;
+    if constexpr (!PIPE)
+      sel.step(accf, acci, pb0, pcv, plim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true, top, buf,
+               cnt, kk);
 #pragma unroll
     for (int t = 0; t < 16; t++) {
+      if constexpr (PIPE) {
+        paccf[t] = accf[t];
+        pacci[t] = acci[t];
+      }
       accf[t] = floatx4{0.f, 0.f, 0.f, 0.f};
       acci[t] = intx4{0, 0, 0, 0};
     }
   }
+  if constexpr (PIPE)
+    sel.step(paccf, pacci, pb0, pcv, plim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true, top, buf,
+             cnt, kk);
   sel.finish(top, buf, cnt, kk, q0, wq, qcount, part, S, split);
 }
 
