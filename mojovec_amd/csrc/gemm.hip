@@ -335,6 +335,7 @@ __device__ __forceinline__ void wide_flush(uint64_t *top, uint64_t *buf, int *cn
 template <int KIND>
 struct WideSel {
   float thr_f[4], qa[4], qb[4];
+  uint64_t thr_k[4];  // each query's kk-th key (the sentinel while fewer): changes at merges only
   int64_t qc[4], qsx[4];
   int ithr[4];
   bool qok[4];
@@ -349,6 +350,7 @@ struct WideSel {
       qok[j] = qi < qcount;
       const int64_t qg = q0 + (qok[j] ? qi : 0);
       thr_f[j] = INFINITY;
+      thr_k[j] = kSentinel64;
       qa[j] = (KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP) ? qnorm[qg] : 0.0f;
       qb[j] = (KIND == FP8_L2 || KIND == FP8_IP) ? qinv[qg] : 0.0f;
       qc[j] = KIND == U8_L2 ? (int64_t)qn_u[qg] : 0;
@@ -458,9 +460,9 @@ struct WideSel {
           dist[u][j] = __fmul_rn((float)(uint32_t)(v > 0 ? v : 0), scale_sq);
         }
         const uint64_t key = pack_key(dist[u][j], (uint32_t)col);
-        // exact key order against the stored kk-th key (ties: lower id)
-        const bool take = pass[u][j] && dist[u][j] <= thr_f[j] &&
-                          (thr_f[j] == INFINITY || key < top[(4 * lgrp + j) * kk + kk - 1]);
+        // exact key order against the kk-th key (ties: lower id), held in
+        // registers (an LDS read here put its latency on every append)
+        const bool take = pass[u][j] && dist[u][j] <= thr_f[j] && key < thr_k[j];
         const uint64_t m = __ballot(take);
         if (m == 0) continue;
         if (take) buf[(4 * lgrp + j) * kWCap + cnt_r[j] + __popcll(m & gmask & below)] = key;
@@ -483,6 +485,7 @@ struct WideSel {
         for (int j = 0; j < 4; j++) {
           cnt_r[j] = cnt[4 * lgrp + j];
           const uint64_t th = top[(4 * lgrp + j) * kk + kk - 1];
+          thr_k[j] = th;
           thr_f[j] = th == kSentinel64 ? INFINITY : key2f((uint32_t)(th >> 32));
           int_thr(j, scale_sq, d_true);
         }
