@@ -117,9 +117,10 @@ def main(argv=None):
     ap.add_argument("--fixed", default=None,
                     help="skip the sweep: 'plain:NPROBE:KF,mixed:NPROBE:KF' (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--mixed-tables", default="both", choices=["both", "32", "8"],
+    ap.add_argument("--mixed-tables", default="all", choices=["all", "both", "32", "8", "16"],
                     help="mixed-form variants swept: FP32 tables ('mixed'), 8-bit tables in "
-                         "the deep-pool scan that the exact rerank follows ('mixed8'), or both")
+                         "the deep-pool scan that the exact rerank follows ('mixed8'), the FP16 "
+                         "MFMA tile scan ('mixed16'); 'all' (default) or 'both' (the first two)")
     ap.add_argument("--deep-pool", default="hbm", choices=["lds", "hbm"],
                     help="large k * k_factor: keys in HBM (default) or a running set in LDS")
     ap.add_argument("--calib-copy", action="store_true",
@@ -192,12 +193,14 @@ def main(argv=None):
     Id = torch.empty((NQ, K), dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
 
-    mixed_forms = {"both": ("mixed", "mixed8"), "32": ("mixed",), "8": ("mixed8",)}[args.mixed_tables]
+    mixed_forms = {"all": ("mixed", "mixed8", "mixed16"), "both": ("mixed", "mixed8"), "32": ("mixed",),
+                   "8": ("mixed8",), "16": ("mixed16",)}[args.mixed_tables]
 
     def set_form(form):
-        """plain; mixed (FP32 tables); mixed8 (8-bit tables where a rerank follows)"""
+        """plain; mixed (FP32 tables); mixed8 (8-bit tables where a rerank
+        follows); mixed16 (FP16 MFMA tile scan where a rerank follows)"""
         ix.set_search_mode("plain" if form == "plain" else "mixed")
-        ix.set_table_bits(8 if form == "mixed8" else 32)
+        ix.set_table_bits({"mixed8": 8, "mixed16": 16}.get(form, 32))
 
     def search(form, nprobe, kf):
         set_form(form)

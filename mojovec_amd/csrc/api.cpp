@@ -917,6 +917,7 @@ extern "C" int mvg_ivfpq_search_device_noalloc(mvg_ivfpq *h, const float *q, int
   if (h->dirty || h->row_of_dirty || (mixed && h->tables_dirty) ||
       (mixed && h->metric == MVG_METRIC_L2 && h->pre_dirty) ||
       (mixed && h->metric == MVG_METRIC_L2 && h->table_bits == 8 && !h->pre8_ready) ||
+      (mixed && h->metric == MVG_METRIC_L2 && h->table_bits == 16 && !h->tile_ready) ||
       (mixed && h->prefilter > 0 && h->keep_vectors && h->vq_dirty))
     throw Error(mvg::kState,
                 "the index is not prepared: run one search (mvg_ivfpq_search or "
@@ -960,6 +961,7 @@ void mvg::ivf_prepare(mvg_ivfpq *h) {
     MVG_HIP(hipStreamSynchronize(h->s));
     h->tables_dirty = false;
     h->pre_dirty = true;
+    h->tile_ready = false;
   }
   if (h->precision != 0 && h->prefilter > 0 && h->keep_vectors && h->vq_dirty && h->ntotal > 0) {
     // SQ8 codes of the stored rows (index_flat_sq8.mojo:317-367 range and scale)
@@ -999,6 +1001,48 @@ void mvg::ivf_prepare(mvg_ivfpq *h) {
                                    h->table_mu.as<float>(), h->entry_pre8.as<float>(), h->s);
     MVG_HIP(hipStreamSynchronize(h->s));
     h->pre8_ready = true;
+  }
+  if (h->precision != 0 && h->metric == MVG_METRIC_L2 && h->table_bits == 16 && !h->tile_ready) {
+    // FP16 codebooks and the list order of the MFMA tile scan (PQ with
+    // 8-dimensional subspaces and M in {8, 12, 16}; other shapes keep the
+    // FP32 tables)
+    const int dsub = h->d / h->M;
+    h->cb16.release();
+    if (dsub == 8 && (h->M == 8 || h->M == 12 || h->M == 16)) {
+      std::vector<float> pqh((size_t)h->d * mvg::kKsub);
+      download(pqh.data(), h->pq.ptr, pqh.size(), h->s);
+      MVG_HIP(hipStreamSynchronize(h->s));
+      float mx = 0.0f;
+      for (float v : pqh) mx = std::max(mx, std::fabs(v));
+      h->cb16.reserve(sizeof(uint16_t) * pqh.size());
+      h->cb16_inv = mvg::tile_codebook(h->pq.as<float>(), h->M, dsub, mx, h->cb16.as<uint16_t>(), h->s);
+      // Super-clusters of the coarse centroids (Lloyd from evenly spaced
+      // centroids); lists are ranked by (super-cluster, list id), and queries
+      // sorted by the rank of their nearest list form tiles whose probe sets
+      // overlap.
+      const int nl = h->nlist, d = h->d;
+      const int nsup = std::max(1, std::min(nl / 16, 4096));
+      h->tile_cent.reserve(sizeof(float) * (size_t)nsup * d);
+      for (int c = 0; c < nsup; c++)
+        MVG_HIP(hipMemcpyAsync(h->tile_cent.as<float>() + (size_t)c * d,
+                               h->coarse.as<float>() + (size_t)((int64_t)c * nl / nsup) * d, sizeof(float) * d,
+                               hipMemcpyDeviceToDevice, h->s));
+      if (nsup > 1)
+        mvg::kmeans_fit(h->coarse.as<float>(), nl, 1, d, nsup, 10, 0, h->tile_cent.as<float>(), h->kw, h->s);
+      mvg::DevBuf dsup;
+      dsup.reserve(sizeof(int32_t) * (size_t)nl);
+      mvg::flat_argmin(h->tile_cent.as<float>(), nsup, d, mvg::kMetricL2, true, h->coarse.as<float>(), nl,
+                       dsup.as<int32_t>(), nullptr, h->s);
+      std::vector<int32_t> sup((size_t)nl), rank((size_t)nl), byrank((size_t)nl);
+      download(sup.data(), dsup.ptr, sup.size(), h->s);
+      MVG_HIP(hipStreamSynchronize(h->s));
+      for (int l = 0; l < nl; l++) byrank[l] = l;
+      std::stable_sort(byrank.begin(), byrank.end(), [&](int x, int y) { return sup[x] < sup[y]; });
+      for (int r = 0; r < nl; r++) rank[byrank[r]] = r;
+      upload(h->list_rank, rank.data(), rank.size(), h->s);
+    }
+    MVG_HIP(hipStreamSynchronize(h->s));
+    h->tile_ready = true;
   }
 }
 
@@ -1066,6 +1110,14 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
       a.prefilter = h->prefilter;
     }
     a.table_bits = h->table_bits;
+    if (h->metric == MVG_METRIC_L2 && h->table_bits == 16) {
+      if (!h->tile_ready) throw Error(mvg::kState, "tile-scan codebooks are not prepared");
+      if (h->cb16.ptr) {
+        a.cb16 = h->cb16.as<uint16_t>();
+        a.cb16_inv = h->cb16_inv;
+        a.list_rank = h->list_rank.as<int32_t>();
+      }
+    }
     if (h->metric == MVG_METRIC_L2 && h->table_bits == 8) {
       if (!h->pre8_ready) throw Error(mvg::kState, "8-bit table terms are not prepared");
       a.pre8 = h->entry_pre8.as<float>();
@@ -1135,7 +1187,7 @@ int mvg_ivfpq_set_prefilter(mvg_ivfpq *h, int per_k) {
 
 int mvg_ivfpq_set_table_bits(mvg_ivfpq *h, int bits) {
   API_BEGIN
-  require(h && (bits == 8 || bits == 32), "bits must be 8 or 32");
+  require(h && (bits == 8 || bits == 16 || bits == 32), "bits must be 8, 16 or 32");
   mvg::ivf_check_unserved(h);
   h->table_bits = bits;
   API_END

@@ -27,7 +27,12 @@ struct mvg_ivfpq {
   mvg::KmeansWork kw;
   // mixed form: SQ8 codes of the stored rows for the pre-rerank (global range
   // as IndexFlatSQ8 computes it); rebuilt when rows were added
-  int table_bits = 32;  // mixed L2 deep-pool scan with a rerank: 8 or 32
+  int table_bits = 32;  // mixed L2 scan with a rerank: 8, 16 (FP16 MFMA tiles) or 32
+  // table_bits 16: FP16 codebooks of the tile scan (-2 * pq * 2^e, cb16_inv =
+  // 2^-e) and each list's rank in the super-cluster order of the centroids
+  mvg::DevBuf cb16, list_rank, tile_cent;
+  float cb16_inv = 1.0f;
+  bool tile_ready = false;
   int prefilter = 8;  // pre-rerank keeps max(64, prefilter * k) candidates; 0: off
   bool vq_dirty = true;
   float vq_min = 0.0f, vq_scale = 1.0f;

@@ -1042,13 +1042,19 @@ __global__ void fill_zero_kernel(int *p, int64_t n) {
 // mostly the same lists and find their codes in its L2.
 constexpr int kQueryRanges = 8;
 
-__global__ void order_count_kernel(const int32_t *__restrict__ probes, int64_t nq, int maxp,
-                                   int nlist, int *__restrict__ cnt) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
+// Bucket of a query: its nearest list, or that list's rank (nullable).
+__device__ __forceinline__ int order_bucket(const int32_t *__restrict__ probes, int64_t q, int maxp,
+                                            int nlist, const int32_t *__restrict__ rank) {
   int l = probes[q * maxp];
   if (l < 0 || l >= nlist) l = 0;
-  atomicAdd(&cnt[l], 1);
+  return rank ? rank[l] : l;
+}
+
+__global__ void order_count_kernel(const int32_t *__restrict__ probes, int64_t nq, int maxp,
+                                   int nlist, int *__restrict__ cnt, const int32_t *__restrict__ rank) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  atomicAdd(&cnt[order_bucket(probes, q, maxp, nlist, rank)], 1);
 }
 
 // One workgroup: exclusive offsets of the per-list counts (the scatter
@@ -1071,12 +1077,11 @@ __global__ __launch_bounds__(kNT) void order_offsets_kernel(int *__restrict__ cn
 }
 
 __global__ void order_scatter_kernel(const int32_t *__restrict__ probes, int64_t nq, int maxp,
-                                     int nlist, int *__restrict__ off, int32_t *__restrict__ order) {
+                                     int nlist, int *__restrict__ off, int32_t *__restrict__ order,
+                                     const int32_t *__restrict__ rank) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
-  int l = probes[q * maxp];
-  if (l < 0 || l >= nlist) l = 0;
-  order[atomicAdd(&off[l], 1)] = (int32_t)q;
+  order[atomicAdd(&off[order_bucket(probes, q, maxp, nlist, rank)], 1)] = (int32_t)q;
 }
 
 // Next query of a persistent workgroup (nq when its range is drained).
@@ -1623,8 +1628,10 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(!IP && MODE
     const uint8_t *__restrict__ excl, int64_t excl_len, uint64_t *__restrict__ out_keys,
     float *__restrict__ D, int64_t *__restrict__ I, float pad_dist, uint32_t *__restrict__ seg_base,
     int64_t seg_cap, unsigned *__restrict__ counter, unsigned long long *__restrict__ pool_writes,
-    const int32_t *__restrict__ order, const float *__restrict__ mu) {
+    const int32_t *__restrict__ order, const float *__restrict__ mu,
+    const unsigned *__restrict__ nq_dev) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // nq_dev (DEEP, nullable): only order[0..*nq_dev) is scanned.
   // Q8: pre holds the centred entry terms and the tables are taken over
   // q - mu (mu: column means of the coarse centroids).
   // MODE 0: running top-kk, rank / radix merge per flush; 1: deep pool in
@@ -1656,7 +1663,17 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(!IP && MODE
     int64_t qi;
     if constexpr (DEEP) {
       __syncthreads();
-      if (tid == 0) ds.q = next_query(order, nq, counter);
+      if (tid == 0) {
+        if (nq_dev) {  // a listed subset: order[0..*nq_dev) in the same ranges
+          const int64_t nl = min<int64_t>(nq, (int64_t)*nq_dev);
+          const int r = blockIdx.x % kQueryRanges;
+          const int64_t beg = nl * r / kQueryRanges, end = nl * (r + 1) / kQueryRanges;
+          const int64_t t = beg + (int64_t)atomicAdd(&counter[r], 1u);
+          ds.q = t < end ? (int64_t)order[t] : nq;
+        } else {
+          ds.q = next_query(order, nq, counter);
+        }
+      }
       __syncthreads();
       qi = ds.q;
       if (qi >= nq) break;
@@ -2182,7 +2199,7 @@ void launch_query_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, 
 // Builds the query order and clears the range counters of a persistent
 // deep-pool scan; returns the order.
 const int32_t *build_query_order(const IvfSearchArgs &a, int maxp, const int32_t *probes,
-                                 IvfWork &w, hipStream_t s) {
+                                 IvfWork &w, hipStream_t s, const int32_t *rank = nullptr) {
   w.ord_cnt.reserve(sizeof(int) * (size_t)a.nlist);
   w.ord_off.reserve(sizeof(int) * (size_t)a.nlist);
   w.ord.reserve(sizeof(int32_t) * (size_t)a.nq);
@@ -2192,13 +2209,13 @@ const int32_t *build_query_order(const IvfSearchArgs &a, int maxp, const int32_t
                      w.ord_cnt.as<int>(), (int64_t)a.nlist);
   MVG_LAUNCH_CHECK("fill_zero_kernel");
   hipLaunchKernelGGL(order_count_kernel, dim3(ceil_div(a.nq, 256)), dim3(256), 0, s, probes, a.nq,
-                     maxp, a.nlist, w.ord_cnt.as<int>());
+                     maxp, a.nlist, w.ord_cnt.as<int>(), rank);
   MVG_LAUNCH_CHECK("order_count_kernel");
   hipLaunchKernelGGL(order_offsets_kernel, dim3(1), dim3(kNT), 0, s, w.ord_cnt.as<int>(), a.nlist,
                      w.ord_off.as<int>(), w.deep_counter.as<unsigned>());
   MVG_LAUNCH_CHECK("order_offsets_kernel");
   hipLaunchKernelGGL(order_scatter_kernel, dim3(ceil_div(a.nq, 256)), dim3(256), 0, s, probes, a.nq,
-                     maxp, a.nlist, w.ord_off.as<int>(), w.ord.as<int32_t>());
+                     maxp, a.nlist, w.ord_off.as<int>(), w.ord.as<int32_t>(), rank);
   MVG_LAUNCH_CHECK("order_scatter_kernel");
   return w.ord.as<int32_t>();
 }
@@ -2275,9 +2292,37 @@ bool launch_stream_t(const IvfSearchArgs &a, int maxp, const int32_t *probes, in
                      maxp, a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D,
                      a.I, kIvfPadDist, seg, cap, counter,
                      DEEP && w.prof.on ? w.prof.counter.as<unsigned long long>() + 1 : nullptr, order,
-                     Q8 ? a.mu : nullptr);
+                     Q8 ? a.mu : nullptr, nullptr);
   MVG_LAUNCH_CHECK("ivfpq_stream_scan_kernel");
   return true;
+}
+
+// The FP32-table deep-pool stream scan (mixed L2) over list[0..*count).
+template <int MF>
+void launch_deep_subset_t(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                          uint64_t *out_keys, const int32_t *list, const unsigned *count, IvfWork &w,
+                          hipStream_t s) {
+  const StreamLayout lay(a.M, a.d, kk, maxp, 1, false, false);
+  const size_t sm = (size_t)lay.total;
+  if (sm > 150 * 1024 || maxp > kStreamMaxProbes)
+    throw Error(kInvalid, "IVF-PQ deep scan workspace exceeds LDS");
+  const void *fn = (const void *)ivfpq_stream_scan_kernel<MF, false, 1, false, false>;
+  if (sm > 64 * 1024)
+    MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
+  const int64_t cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);
+  const int64_t grid = std::max<int64_t>(kQueryRanges, std::min<int64_t>(64, segment_grid_limit(w.deep_seg, cap)) /
+                                                           kQueryRanges * kQueryRanges);
+  w.deep_seg.reserve(sizeof(uint64_t) * (size_t)(grid * cap));
+  w.deep_counter.reserve(sizeof(unsigned) * kQueryRanges);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<int *>(w.deep_counter.ptr),
+                     (int64_t)kQueryRanges);
+  MVG_LAUNCH_CHECK("fill_zero_kernel");
+  hipLaunchKernelGGL((ivfpq_stream_scan_kernel<MF, false, 1, false, false>), dim3((unsigned)grid), dim3(kNT), sm,
+                     s, a.q, a.nq, a.d, a.M, a.coarse, a.pq, a.pre, a.list_off, a.codes, a.ids, probes, maxp,
+                     a.nprobe_per_query, a.nprobe, kk, a.k, a.excl, a.excl_len, out_keys, a.D, a.I,
+                     kIvfPadDist, w.deep_seg.as<uint32_t>(), cap, w.deep_counter.as<unsigned>(),
+                     w.prof.on ? w.prof.counter.as<unsigned long long>() + 1 : nullptr, list, nullptr, count);
+  MVG_LAUNCH_CHECK("ivfpq_stream_scan_kernel");
 }
 
 // 8-bit tables only where an exact rerank follows the scan (k_factor > 1):
@@ -2323,6 +2368,22 @@ void launch_deep(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk
 }
 
 }  // namespace
+
+const int32_t *build_tile_order(const IvfSearchArgs &a, int maxp, const int32_t *probes,
+                                const int32_t *rank, IvfWork &w, hipStream_t s) {
+  return build_query_order(a, maxp, probes, w, s, rank);
+}
+
+void launch_deep_subset(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                        uint64_t *out_keys, const int32_t *list, const unsigned *count, IvfWork &w,
+                        hipStream_t s) {
+  switch (a.M) {
+    case 8: launch_deep_subset_t<8>(a, maxp, probes, kk, out_keys, list, count, w, s); break;
+    case 12: launch_deep_subset_t<12>(a, maxp, probes, kk, out_keys, list, count, w, s); break;
+    case 16: launch_deep_subset_t<16>(a, maxp, probes, kk, out_keys, list, count, w, s); break;
+    default: launch_deep_subset_t<0>(a, maxp, probes, kk, out_keys, list, count, w, s); break;
+  }
+}
 
 int device_cus() {
   static int cus = 0;
@@ -2524,7 +2585,9 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     WorkQueue wq{};
     const WorkQueue *wqp = nullptr;
     bool streamed = false;
-    if ((ip || (mixed && a.pre)) && (deep || a.wq_entries == 0)) {
+    // FP16 MFMA tile scan: mixed L2 candidates that the exact rerank follows
+    if (mixed && a.table_bits == 16 && kf > 1 && keys) streamed = launch_tile_scan(a, maxp, probes, kk, keys, w, s);
+    if (!streamed && (ip || (mixed && a.pre)) && (deep || a.wq_entries == 0)) {
       // large pools: deep pool in HBM or running set with threshold flushes
       const int smode = !deep ? 0 : (a.deep_pool ? 1 : 2);
 #define MVG_STREAM(MFV)                                                                       \

@@ -104,6 +104,12 @@ struct IvfSearchArgs {
   // in rows of stride k (padded past kq[i]); results equal those of a search
   // with k = kq[i] alone
   const int32_t *kq;
+  // table_bits 16 (mixed L2 with a rerank): FP16 codebooks [M][256][dsub] of
+  // -2 * pq scaled by 2^e (cb16_inv = 2^-e) for the MFMA tile scan, and each
+  // list's rank in the super-cluster order that groups queries into tiles
+  const uint16_t *cb16;
+  float cb16_inv;
+  const int32_t *list_rank;
 };
 // pre[e] = sum_m ptab[l][m][code_m] for every entry of the CSR lists.
 // 8-bit tables of the mixed L2 form: mu = column means of the coarse
@@ -132,17 +138,42 @@ struct IvfWork {
   DevBuf deep_seg, deep_counter;  // deep candidate pools: key segments, query range counters
   DevBuf ord_cnt, ord_off, ord;    // query order of the persistent deep scans
   DevBuf vq_qcodes, vq_qnorms, vq_keys;  // mixed pre-rerank: query codes, narrowed candidates
+  // MFMA tile scan: boundary buffers, tile range counters and the list of
+  // queries left to the deep-pool scan
+  DevBuf tile_bnd, tile_ctr, tile_ovf, tile_pre;
   bool ord_ready = false;          // `ord` holds this search's order
   IvfProfile prof;
   // Every workspace buffer takes its memory from `a` (before any reserve).
   void bind(Arena *a) {
     for (DevBuf *b : {&coarse_part, &probes, &part, &keys, &keys2, &qbf, &qnorm, &wq_poff, &wq_nitems,
                       &wq_rq, &wq_off, &wq_gthr, &wq_counter, &deep_seg, &deep_counter, &ord_cnt,
-                      &ord_off, &ord, &vq_qcodes, &vq_qnorms, &vq_keys, &prof.counter})
+                      &ord_off, &ord, &vq_qcodes, &vq_qnorms, &vq_keys, &tile_bnd, &tile_ctr,
+                      &tile_ovf, &tile_pre, &prof.counter})
       b->arena = a;
   }
 };
 void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s);
+
+// ---- ivfpq_tile.hip ----
+// FP16 codebooks for the tile scan: out[m][c][j] = -2 * pq[m][c][j] * 2^e,
+// 2^e putting the largest magnitude in [2^14, 2^15); returns 2^-e.
+float tile_codebook(const float *pq, int M, int dsub, float max_abs, uint16_t *out, hipStream_t s);
+// Whether the MFMA tile scan takes this search (mixed L2, dsub 8, M in
+// {8, 12, 16}, at most 128 probes, tables prepared).
+bool tile_scan_supported(const IvfSearchArgs &a, int maxp);
+// Mixed L2 candidates (kk per query, unordered (key << 32 | id) words) by the
+// MFMA tile scan; sets w.ord to the tile order. False when unsupported.
+bool launch_tile_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                      uint64_t *out_keys, IvfWork &w, hipStream_t s);
+// Queries bucketed by rank[nearest list] (rank nullable: the list itself);
+// the order within a bucket is unspecified. Sets w.ord / w.ord_ready.
+const int32_t *build_tile_order(const IvfSearchArgs &a, int maxp, const int32_t *probes,
+                                const int32_t *rank, IvfWork &w, hipStream_t s);
+// The FP32-table deep-pool stream scan (mixed L2) of the queries list[0..*count)
+// only (count is read on the device; the launch is always made).
+void launch_deep_subset(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
+                        uint64_t *out_keys, const int32_t *list, const unsigned *count, IvfWork &w,
+                        hipStream_t s);
 
 // Exact FP32 rerank of kk candidate keys per query (any order) to the top-k
 // (index_hnsw.mojo:449-496 generalised to k_factor).

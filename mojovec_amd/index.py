@@ -329,8 +329,10 @@ class IndexIVFPQ:
         _lib.check(_lib.load().mvg_ivfpq_set_prefilter(self._h, int(per_k)))
 
     def set_table_bits(self, bits: int = 32) -> None:
-        """Mixed form: 8 sums 8-bit tables in the L2 deep-pool scan that the
-        exact rerank follows (k_factor > 1); 32 keeps FP32 tables."""
+        """Mixed form, L2 scans that the exact rerank follows (k_factor > 1):
+        8 sums 8-bit tables in the deep-pool scan; 16 runs the FP16 MFMA tile
+        scan (query tiles times PQ-decoded entries; PQ with 8-dimensional
+        subspaces, M in {8, 12, 16}); 32 keeps FP32 tables."""
         _lib.check(_lib.load().mvg_ivfpq_set_table_bits(self._h, int(bits)))
 
     def set_work_queue(self, entries_per_item: int = 0) -> None:
