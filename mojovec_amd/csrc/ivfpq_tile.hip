@@ -15,18 +15,21 @@
 //
 // Selection (the k * k_factor smallest (distance, id) pairs of each query,
 // exactly, for these distances): two passes over the tile's lists.
-//   1. every distance below the query's running bound is counted in a
-//      512-bin histogram of its key (1/64 octave per bin, over a window of
-//      eight octaves placed at the query's nearest-centroid distance); the
-//      bound (the top edge of the bin holding the kk-th counted key) only
+//   1. a sample of the keys (every fourth block of 16 entries) is counted in
+//      a 512-bin histogram per query (1/64 octave per bin, over a window of
+//      eight octaves placed at the query's nearest-centroid distance) under a
+//      running bound aimed at a quarter of 1.25 kk keys; the bound only
 //      tightens, so the counts are exact at and below it;
-//   2. with the kk-th key's bin known, keys below it are written out directly
-//      and the keys inside it go to a small per-query buffer, from which a
-//      wavefront radix select takes the rest on (key, id).
-// The pass-2 counts are checked against the histogram's; a query whose
-// boundary bin exceeds the buffer, or whose counts disagree, is listed for
-// the table-based deep-pool scan, which then writes its candidates (the host
-// always enqueues it; it exits at once when the list is empty).
+//   2. every key below the bin edge pass 1 found is collected into a
+//      per-query buffer and counted in an exact histogram; if at least kk
+//      were collected (or pass 1 set no bound), the kk smallest of them are
+//      the kk smallest overall: keys below the kk-th key's bin are written
+//      out and a wavefront radix select on (key, id) takes the rest of that
+//      bin.
+// A query whose collected keys fall short of kk, or overflow the buffers, is
+// listed for the table-based deep-pool scan, which then writes its
+// candidates (the host always enqueues it; it exits at once when the list is
+// empty).
 #include <cmath>
 
 #include "dist.h"
@@ -48,7 +51,8 @@ constexpr int kTItem = 512;         // list entries per work item
 constexpr int kTGroup = 4;          // blocks of 16 entries loaded together, one group ahead
 constexpr int kTStage = 128;        // per-wave staging of pass-2 keys
 constexpr int kTRanges = 8;         // tile ranges, one per XCD
-constexpr int kTPerQuery = 13;      // per-query words in LDS
+constexpr int kTPerQuery = 8;       // per-query words in LDS
+constexpr int kTSample = 4;         // pass 1 visits every kTSample-th block of 16 entries
 constexpr int kTNT = 512;           // threads per workgroup: two waves per SIMD
 constexpr int kTNW = kTNT / 64;
 constexpr int kTSelW = 4;           // waves that select (the codebook's LDS holds their buffers)
@@ -122,9 +126,10 @@ struct TileScanArgs {
   int maxp;
   const int32_t *nprobe_q;
   int nprobe_all, kk;
+  int kk1;  // sampled keys pass 1 aims at
   uint64_t *out_keys;
-  uint64_t *bnd;  // per workgroup: kTQ boundary buffers of bufcap words
-  int bufcap, rc;
+  uint64_t *cbuf;  // per workgroup: kTQ collect buffers of cmax words
+  int cmax, bufcap, rc;
   const int32_t *order;
   int64_t ntiles;
   unsigned *ranges;
@@ -295,7 +300,8 @@ struct TileSrc {
 };
 
 // One group of kTGroup blocks of 16 list entries as one lane loads them:
-// its entry's PQ code, entry term and (pass 2) id in each block.
+// its entry's PQ code, entry term and (pass 2) id in each block; blocks
+// `stride` blocks apart (pass 1 samples every kTSample-th block).
 template <int MF>
 struct TileGrp {
   uint32_t cw[kTGroup][MF / 4];
@@ -303,11 +309,11 @@ struct TileGrp {
   int32_t id[kTGroup];
 };
 
-template <int MF, bool IDS>
+template <int MF, bool IDS, int STRIDE>
 __device__ __forceinline__ void tile_load(const TileSrc &src, uint32_t e0, TileGrp<MF> &g) {
 #pragma unroll
   for (int b = 0; b < kTGroup; b++) {
-    const uint32_t e = e0 + 16u * b;
+    const uint32_t e = e0 + 16u * STRIDE * b;
     if constexpr (MF == 16) {
       const uint4 v = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(src.codes, (int)(e * 16u), 0, 0));
       g.cw[b][0] = v.x;
@@ -328,51 +334,47 @@ __device__ __forceinline__ void tile_load(const TileSrc &src, uint32_t e0, TileG
   }
 }
 
-// Per-item registers of a lane: the list's base term and the running edges
-// of the queries whose distances the lane receives (4 * lgrp + j).
+// Per-item registers of a lane: the list's base term and the bounds of the
+// queries whose distances the lane receives (4 * lgrp + j).
 struct TileItem {
-  float base[4], E[4], L[4];
+  float base[4], E[4];
 };
 
-// Pass-2 staging of one wave: (key << 32 | id) words and (query | below << 8).
+// Pass-2 staging of one wave: (key << 32 | id) words and their query slots.
 struct TileStage {
   uint64_t *w;
   uint32_t *s;
 };
 
-// Flush of a wave's staged pass-2 keys: one slot claim per key, all lanes.
-__device__ __forceinline__ void stage_flush(const TileScanArgs &a, TileStage st, int n, const int *qid_s, int *cnt_s,
-                                            int *cnt2_s, uint64_t *bndw) {
+// Flush of a wave's staged pass-2 keys, all lanes at once: each key takes a
+// slot of its query's collect buffer and is counted in its histogram bin.
+__device__ __forceinline__ void stage_flush(const TileScanArgs &a, TileStage st, int n, int *cnt_s, const int *anc_s,
+                                            uint32_t *hist, uint64_t *cbw) {
   const int lane = threadIdx.x & 63;
   wave_sync();
   for (int i = lane; i < n; i += 64) {
     const uint64_t w = st.w[i];
-    const uint32_t x = st.s[i];
-    const int s = (int)(x & 0xFFu);
-    if (x >> 8) {
-      const int slot = atomicAdd(&cnt_s[s], 1);
-      if (slot < a.kk) a.out_keys[(int64_t)qid_s[s] * a.kk + slot] = w;
-    } else {
-      const int slot = atomicAdd(&cnt2_s[s], 1);
-      if (slot < a.bufcap) bndw[(int64_t)s * a.bufcap + slot] = w;
-    }
+    const int s = (int)st.s[i];
+    const int slot = atomicAdd(&cnt_s[s], 1);
+    if (slot < a.cmax) cbw[(int64_t)s * a.cmax + slot] = w;
+    atomicAdd(&hist[s * kTBins + key_bin((uint32_t)(w >> 32), anc_s[s])], 1u);
   }
   wave_sync();
 }
 
 // One group of blocks: decode, MFMA, and the pass's epilogue, two blocks at
 // a time (their eight codebook reads issued before either MFMA chain, the
-// two chains interleaved), since a lone wave per SIMD hides nothing else.
+// two chains interleaved).
 // MFMA layout: A = the tile's query fragments (lane: query lrow, the 8
 // dimensions of subquantizer 4 * ks + lgrp), B = the decoded entries (lane:
 // entry lrow of the block, same dimensions); the lane then holds the
 // distances of its entry to queries 4 * lgrp + j (j < 4).
-template <int MF, int PASS>
+template <int MF, int PASS, int STRIDE>
 __device__ __forceinline__ void tile_group(const TileScanArgs &a, const TileGrp<MF> &g, uint32_t e0, uint32_t ie,
                                            int nleft, const half8_t (&qf)[MF / 4], const float (&inv)[4],
                                            const int (&anc)[4], const TileItem &ti, const char *cbb,
-                                           uint32_t cb_lane, uint32_t *hist, TileStage st, int &nst,
-                                           const int *qid_s, int *cnt_s, int *cnt2_s, uint64_t *bndw) {
+                                           uint32_t cb_lane, uint32_t *hist, TileStage st, int &nst, int *cnt_s,
+                                           const int *anc_s, uint64_t *cbw) {
   constexpr int KS = MF / 4;
   const int lane = threadIdx.x & 63;
   const int lgrp = lane >> 4;
@@ -395,13 +397,13 @@ __device__ __forceinline__ void tile_group(const TileScanArgs &a, const TileGrp<
     float dist[2][4];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      const uint32_t e = e0 + 16u * (b + h);  // this lane's entry (past the group's end: +inf)
+      const uint32_t e = e0 + 16u * STRIDE * (b + h);  // this lane's entry (past the item's end: +inf)
       const float pv = (e < ie && b + h < nleft) ? g.pre[b + h] : __builtin_inff();
 #pragma unroll
       for (int j = 0; j < 4; j++) dist[h][j] = __builtin_fmaf(acc[h][j], inv[j], pv) + ti.base[j];
     }
     if constexpr (PASS == 1) {
-      // count keys under the running bound (no return: nothing waits)
+      // count sampled keys under the running bound (no return: nothing waits)
 #pragma unroll
       for (int h = 0; h < 2; h++)
 #pragma unroll
@@ -412,48 +414,41 @@ __device__ __forceinline__ void tile_group(const TileScanArgs &a, const TileGrp<
           }
         }
     } else {
-      // staged: positions from the ballots, slots claimed at the flush
+      // staged: positions from the ballots; slots and counts at the flush
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        uint64_t m[4];
-        int tot = 0;
+      for (int h = 0; h < 2; h++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          m[j] = __ballot(dist[h][j] < ti.E[j]);
-          tot += (int)__popcll(m[j]);
-        }
-        if (tot == 0) continue;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          if (m[j] == 0) continue;
+          const bool in = dist[h][j] < ti.E[j];
+          const uint64_t m = __ballot(in);
+          if (m == 0) continue;
           if (nst + 64 > kTStage) {  // a combination stages at most 64 keys
-            stage_flush(a, st, nst, qid_s, cnt_s, cnt2_s, bndw);
+            stage_flush(a, st, nst, cnt_s, anc_s, hist, cbw);
             nst = 0;
           }
-          if (dist[h][j] < ti.E[j]) {
-            const int pos = nst + lane_rank(m[j]);
+          if (in) {
+            const int pos = nst + lane_rank(m);
             st.w[pos] = ((uint64_t)f2key(dist[h][j]) << 32) | (uint32_t)g.id[b + h];
-            st.s[pos] = (uint32_t)(4 * lgrp + j) | (dist[h][j] < ti.L[j] ? 0x100u : 0u);
+            st.s[pos] = (uint32_t)(4 * lgrp + j);
           }
-          nst += (int)__popcll(m[j]);
+          nst += (int)__popcll(m);
         }
-      }
     }
   }
 }
 
 // One sweep of a workgroup's waves over the tile's work items (list chunks
-// of kTItem entries). PASS 1 counts keys below the running bound; PASS 2
-// writes keys below the lower edge to the output and keys between the edges
-// to the boundary buffer, both as (key << 32 | id).
+// of kTItem entries). PASS 1 visits every kTSample-th block of 16 entries and
+// counts the keys below the running bound (aimed at kk1 sampled keys); PASS 2
+// visits every block and collects the keys below the bound pass 1 left.
 template <int MF, int PASS>
 __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc &src, const half8_t (&qf)[MF / 4],
                                            const float (&inv)[4], const int (&anc)[4], const _Float16 *cb,
                                            uint32_t *hist, const uint64_t *run, const uint64_t *ua,
-                                           const uint64_t *ub, int U, int NI, const int *qid_s, float *E_s,
-                                           const float *Elo_s, int *cnt_s, int *cnt2_s, const int *anc_s,
-                                           float *wb, TileStage st, int *itc, uint64_t *bndw) {
+                                           const uint64_t *ub, int U, int NI, float *E_s, int *cnt_s,
+                                           const int *anc_s, float *wb, TileStage st, int *itc, uint64_t *cbw) {
   constexpr bool IDS = PASS == 2;
+  constexpr int STRIDE = PASS == 1 ? kTSample : 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lrow = lane & 15, lgrp = lane >> 4;
   float *wbw = wb + wave * kTQ;
@@ -474,10 +469,10 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
     const uint32_t lbeg = (uint32_t)(eb >> 32), llen = (uint32_t)eb;
     const uint32_t ib = lbeg + chunk * kTItem + lrow;
     const uint32_t ie = lbeg + min(llen, (chunk + 1) * (uint32_t)kTItem);
-    const int nblk = (int)((ie - (ib - lrow) + 15) >> 4);
+    const int nblk = (int)((ie - (ib - lrow) + 16 * STRIDE - 1) / (16 * STRIDE));  // blocks visited
     const int ngrp = (nblk + kTGroup - 1) / kTGroup;
     TileGrp<MF> ga, gb;
-    tile_load<MF, IDS>(src, ib, ga);
+    tile_load<MF, IDS, STRIDE>(src, ib, ga);
     // this list's base term per tile query (+inf where the query does not probe it)
     if (lane < kTQ) wbw[lane] = __builtin_inff();
     wave_sync();
@@ -492,33 +487,33 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
       const int s = 4 * lgrp + j;
       ti.base[j] = wbw[s];
       ti.E[j] = E_s[s];
-      ti.L[j] = PASS == 2 ? Elo_s[s] : 0.0f;
     }
     // two load buffers in turn: a group's loads are issued one group ahead,
     // unconditionally (a conditional load leaves the compiler unable to wait
     // for the older group alone; past the item the entries are unused and
     // past the arrays the buffer reads return zero)
+    constexpr uint32_t kStep = 16u * STRIDE * kTGroup;
     for (int gi = 0; gi < ngrp; gi += 2) {
-      tile_load<MF, IDS>(src, ib + 16u * kTGroup * (gi + 1), gb);
-      tile_group<MF, PASS>(a, ga, ib + 16u * kTGroup * gi, ie, nblk - gi * kTGroup, qf, inv, anc, ti, cbb, cb_lane,
-                           hist, st, nst, qid_s, cnt_s, cnt2_s, bndw);
+      tile_load<MF, IDS, STRIDE>(src, ib + kStep * (gi + 1), gb);
+      tile_group<MF, PASS, STRIDE>(a, ga, ib + kStep * gi, ie, nblk - gi * kTGroup, qf, inv, anc, ti, cbb, cb_lane,
+                                   hist, st, nst, cnt_s, anc_s, cbw);
       if (gi + 1 >= ngrp) break;
-      tile_load<MF, IDS>(src, ib + 16u * kTGroup * (gi + 2), ga);
-      tile_group<MF, PASS>(a, gb, ib + 16u * kTGroup * (gi + 1), ie, nblk - (gi + 1) * kTGroup, qf, inv, anc, ti,
-                           cbb, cb_lane, hist, st, nst, qid_s, cnt_s, cnt2_s, bndw);
+      tile_load<MF, IDS, STRIDE>(src, ib + kStep * (gi + 2), ga);
+      tile_group<MF, PASS, STRIDE>(a, gb, ib + kStep * (gi + 1), ie, nblk - (gi + 1) * kTGroup, qf, inv, anc, ti,
+                                   cbb, cb_lane, hist, st, nst, cnt_s, anc_s, cbw);
     }
     if constexpr (PASS == 1) {
       // tighten the bounds of this wave's queries (counts only grow, so a
       // bound read while others add is still an upper bound)
       for (int s = wave; s < kTQ; s += kTNW) {
         int bef, cnt;
-        const int bn = wave_find_bin(hist + s * kTBins, a.kk, bef, cnt);
+        const int bn = wave_find_bin(hist + s * kTBins, a.kk1, bef, cnt);
         if (lane == 0 && bn >= 0) E_s[s] = bin_edge(anc_s[s], bn + 1);
       }
     }
   }
   if constexpr (PASS == 2) {
-    if (nst > 0) stage_flush(a, st, nst, qid_s, cnt_s, cnt2_s, bndw);
+    if (nst > 0) stage_flush(a, st, nst, cnt_s, anc_s, hist, cbw);
   }
 }
 
@@ -536,22 +531,18 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
   uint64_t *ua = reinterpret_cast<uint64_t *>(smem + lay.ua);
   uint64_t *ub = reinterpret_cast<uint64_t *>(smem + lay.ub);
   int *qid_s = reinterpret_cast<int *>(smem + lay.pq);
-  float *E_s = reinterpret_cast<float *>(qid_s + T);    // pass 1 bound; pass 2 upper edge
-  float *Elo_s = E_s + T;                                // pass 2 lower edge
-  int *cnt_s = reinterpret_cast<int *>(Elo_s + T);      // keys written out
-  int *cnt2_s = cnt_s + T;                               // boundary keys
-  int *anc_s = cnt2_s + T;
+  float *E_s = reinterpret_cast<float *>(qid_s + T);    // pass-1 running bound, then the collect bound
+  int *cnt_s = reinterpret_cast<int *>(E_s + T);        // keys collected
+  int *anc_s = cnt_s + T;
   int *np_s = anc_s + T;
   unsigned *bmin_s = reinterpret_cast<unsigned *>(np_s + T);
   float *inv_s = reinterpret_cast<float *>(bmin_s + T);
   float *sq_s = inv_s + T;
-  int *bef_s = reinterpret_cast<int *>(sq_s + T);        // pass-1 count of the keys below the bin
-  int *exb_s = bef_s + T;                                // and of the keys in it
   float *wb = reinterpret_cast<float *>(smem + lay.wb);
   int *misc = reinterpret_cast<int *>(smem + lay.misc);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lgrp = lane >> 4;
-  uint64_t *bndw = a.bnd + (int64_t)blockIdx.x * T * a.bufcap;
+  uint64_t *cbw = a.cbuf + (int64_t)blockIdx.x * T * a.cmax;
   TileStage st;
   st.w = reinterpret_cast<uint64_t *>(smem + lay.stg) + wave * kTStage;
   st.s = reinterpret_cast<uint32_t *>(smem + lay.stg + kTNW * kTStage * 8) + wave * kTStage;
@@ -578,7 +569,6 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
       np_s[tid] = np;
       bmin_s[tid] = 0x7F800000u;  // +inf
       cnt_s[tid] = 0;
-      cnt2_s[tid] = 0;
       E_s[tid] = __builtin_inff();
     }
     if (tid == 0) misc[4] = 0;
@@ -694,93 +684,77 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
     for (int i = tid; i < T * kTBins; i += kTNT) hist[i] = 0u;
     __syncthreads();
 
-    // ---- pass 1: key histograms under running bounds
-    tile_sweep<MF, 1>(a, src, qf, inv, anc, cb, hist, run, ua, ub, U, NI, qid_s, E_s, Elo_s, cnt_s, cnt2_s, anc_s,
-                      wb, st, misc + 4, bndw);
+    // ---- pass 1: histograms of a sample of the keys under running bounds
+    tile_sweep<MF, 1>(a, src, qf, inv, anc, cb, hist, run, ua, ub, U, NI, E_s, cnt_s, anc_s, wb, st, misc + 4, cbw);
     __syncthreads();
-    // bin of each query's kk-th key: the pass-2 edges
+    // collect bound of each query: the top edge of the bin where its sampled
+    // keys reach kk1 (+inf when fewer were sampled: every key)
     for (int s = wave; s < T; s += kTNW) {
-      const int q = qid_s[s];
-      int bef = 0, cnt = 0, b = -1, tot = 0;
-      if (q >= 0) b = wave_find_bin(hist + s * kTBins, a.kk, bef, cnt, &tot);
-      if (lane == 0) {
-        float lo = __builtin_inff(), hi = __builtin_inff();  // fewer than kk: every key
-        bool ovf = false;
-        if (q < 0) {
-          lo = hi = -__builtin_inff();
-        } else if (b >= 0 && cnt > a.bufcap) {
-          ovf = true;
-        } else if (b >= 0) {
-          lo = b == 0 ? -__builtin_inff() : bin_edge(anc_s[s], b);
-          hi = bin_edge(anc_s[s], b + 1);
-        }
-        if (ovf) {
-          lo = hi = -__builtin_inff();
-          a.ovf_list[atomicAdd(a.ovf_count, 1u)] = q;
-        }
-        Elo_s[s] = lo;
-        E_s[s] = hi;
-        // the pass-2 counts to expect (every key below the upper edge was
-        // counted exactly); -1: listed for the deep-pool scan
-        bef_s[s] = ovf ? -1 : (b >= 0 ? bef : tot);
-        exb_s[s] = b >= 0 ? cnt : 0;
-      }
+      int bef, cnt;
+      const int b = wave_find_bin(hist + s * kTBins, a.kk1, bef, cnt);
+      if (lane == 0) E_s[s] = qid_s[s] < 0 ? -__builtin_inff() : (b >= 0 ? bin_edge(anc_s[s], b + 1) : __builtin_inff());
     }
+    __syncthreads();
+    for (int i = tid; i < T * kTBins; i += kTNT) hist[i] = 0u;
     if (tid == 0) misc[4] = 0;
     __syncthreads();
 
-    // ---- pass 2: keys below the bin out, keys in it to the boundary buffer
-    tile_sweep<MF, 2>(a, src, qf, inv, anc, cb, hist, run, ua, ub, U, NI, qid_s, E_s, Elo_s, cnt_s, cnt2_s, anc_s,
-                      wb, st, misc + 4, bndw);
+    // ---- pass 2: every key below the bound into the collect buffers, counted
+    tile_sweep<MF, 2>(a, src, qf, inv, anc, cb, hist, run, ua, ub, U, NI, E_s, cnt_s, anc_s, wb, st, misc + 4, cbw);
     __syncthreads();
 
-    // ---- per query (one wavefront each): the rest of the kk from the
-    // boundary buffer; the codebook's LDS holds it
+    // ---- per query (one wavefront each): the kk smallest of the collected
+    // keys by their exact histogram; the codebook's LDS holds the boundary
     uint64_t *wbuf = reinterpret_cast<uint64_t *>(smem + lay.cb) + wave * a.bufcap;
     for (int s = wave; s < T && wave < kTSelW; s += kTSelW) {
-      const int q = qid_s[s], bef = bef_s[s];
-      if (q < 0 || bef == -1) continue;
-      // the exact pass-2 counts must be the histogram's; they differ only
-      // where the running bounds were inconsistent, which the deep-pool scan
-      // then repairs
-      if (cnt_s[s] != bef || cnt2_s[s] != exb_s[s] || cnt_s[s] > a.kk || cnt2_s[s] > a.bufcap) {
+      const int q = qid_s[s];
+      if (q < 0) continue;
+      const int n = cnt_s[s];
+      int bef = 0, cnt = 0;
+      const int b = wave_find_bin(hist + s * kTBins, a.kk, bef, cnt);
+      // fewer keys below a finite bound than kk (the sample's margin missed),
+      // more than the buffers hold: the deep-pool scan takes the query
+      if (n > a.cmax || (b < 0 && E_s[s] != __builtin_inff()) || (b >= 0 && cnt > a.bufcap)) {
         if (lane == 0) a.ovf_list[atomicAdd(a.ovf_count, 1u)] = q;
         continue;
       }
       uint64_t *out = a.out_keys + (int64_t)q * a.kk;
-      const int nout = cnt_s[s];
-      int nb = 0;
-      if (exb_s[s] > 0) {
-        nb = cnt2_s[s];
-        const uint64_t *bsrc = bndw + (int64_t)s * a.bufcap;
-        for (int j0 = 0; j0 < nb; j0 += 4 * 64) {
-          uint64_t v[4];
+      const uint64_t *cq = cbw + (int64_t)s * a.cmax;
+      int nout = 0, nb = 0;
+      const int anc_q = anc_s[s];
+      for (int j0 = 0; j0 < n; j0 += 4 * 64) {
+        uint64_t v[4];
 #pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int j = j0 + r * 64 + lane;
-            v[r] = j < nb ? bsrc[j] : 0ull;
-          }
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int j = j0 + r * 64 + lane;
-            if (j < nb) wbuf[j] = v[r];
-          }
+        for (int r = 0; r < 4; r++) {
+          const int j = j0 + r * 64 + lane;
+          v[r] = j < n ? cq[j] : ~0ull;
         }
-        wave_sync();
-        const int need = min(a.kk - nout, nb);
-        if (need > 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int j = j0 + r * 64 + lane;
+          const int bn = key_bin((uint32_t)(v[r] >> 32), anc_q);
+          const bool lo = j < n && (b < 0 || bn < b), eq = j < n && b >= 0 && bn == b;
+          const uint64_t ml = __ballot(lo), me = __ballot(eq);
+          if (lo && nout + lane_rank(ml) < a.kk) out[nout + lane_rank(ml)] = v[r];
+          if (eq && nb + lane_rank(me) < a.bufcap) wbuf[nb + lane_rank(me)] = v[r];
+          nout += (int)__popcll(ml);
+          nb += (int)__popcll(me);
+        }
+      }
+      wave_sync();
+      int taken = 0;
+      if (b >= 0) {
+        taken = min(a.kk - nout, nb);
+        if (taken > 0) {
           // the buffer's keys share their bin; interior bins fix the key bits
           // above kTShift, the two edge bins do not
-          const uint32_t k0 = (uint32_t)(wbuf[0] >> 32);
-          const int b = key_bin(k0, anc_s[s]);
           const int shift = (b == 0 || b == kTBins - 1) ? 56 : 32 + kTShift - 8;
-          wave_select_words(wbuf, nb, need, shift, reinterpret_cast<int *>(hist + s * kTBins), out, nout, a.kk);
+          wave_select_words(wbuf, nb, taken, shift, reinterpret_cast<int *>(hist + s * kTBins), out, nout, a.kk);
         }
-        nb = need > 0 ? need : 0;
+        taken = max(taken, 0);
       }
-      for (int j = nout + nb + lane; j < a.kk; j += 64) out[j] = kSentinel64;
-      if (a.pool_writes && lane == 0)
-        atomicAdd(a.pool_writes, (unsigned long long)(cnt_s[s] + cnt2_s[s]));
+      for (int j = nout + taken + lane; j < a.kk; j += 64) out[j] = kSentinel64;
+      if (a.pool_writes && lane == 0) atomicAdd(a.pool_writes, (unsigned long long)n);
     }
   }
 }
@@ -854,14 +828,20 @@ bool launch_tile_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, i
   const int32_t *order = build_tile_order(a, maxp, probes, a.list_rank, w, s);
   const int64_t ntiles = (a.nq + T - 1) / T;
   const int bufcap = a.M * kKsub * 16 / kTSelW / 8;
+  // pass 1 aims its sample at 1.25 kk / kTSample + 16 keys: the bound then
+  // lies near the 1.25 kk-th key, six standard deviations of the sample
+  // count above the kk-th at kk = 2880; a query whose collected keys still
+  // fall short of kk goes to the deep-pool scan
+  const int kk1 = (5 * kk + 4 * kTSample - 1) / (4 * kTSample) + 16;
+  const int cmax = 2 * kk + 512;  // collect buffer per query
   int64_t grid = std::min<int64_t>((int64_t)device_cus(), ntiles);
   grid = std::max<int64_t>(kTRanges, (grid + kTRanges - 1) / kTRanges * kTRanges);
   if (w.tile_bnd.arena) {  // a fixed workspace: as many workgroups as a quarter of it holds
-    const int64_t per_wg = (int64_t)sizeof(uint64_t) * T * bufcap;
+    const int64_t per_wg = (int64_t)sizeof(uint64_t) * T * cmax;
     grid = std::min<int64_t>(grid, (int64_t)(w.tile_bnd.arena->left() / 4) / per_wg / kTRanges * kTRanges);
     if (grid < kTRanges) return false;
   }
-  w.tile_bnd.reserve(sizeof(uint64_t) * (size_t)(grid * T * bufcap));
+  w.tile_bnd.reserve(sizeof(uint64_t) * (size_t)(grid * T * cmax));
   w.tile_ctr.reserve(sizeof(unsigned) * (kTRanges + 1));
   w.tile_ovf.reserve(sizeof(int32_t) * (size_t)a.nq);
   hipLaunchKernelGGL(tile_zero_kernel, dim3(1), dim3(64), 0, s, w.tile_ctr.as<unsigned>(), kTRanges + 1);
@@ -883,7 +863,9 @@ bool launch_tile_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, i
   ta.nprobe_all = a.nprobe;
   ta.kk = kk;
   ta.out_keys = out_keys;
-  ta.bnd = w.tile_bnd.as<uint64_t>();
+  ta.kk1 = kk1;
+  ta.cbuf = w.tile_bnd.as<uint64_t>();
+  ta.cmax = cmax;
   ta.bufcap = bufcap;
   ta.rc = rc;
   ta.order = order;
