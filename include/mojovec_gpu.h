@@ -243,9 +243,20 @@ int mvg_ivfpq_set_prefilter(mvg_ivfpq *h, int per_k);
  * subquantizer's FP32 table minus its minimum, in steps of one scale per
  * query (the largest range over 255), rounded, summed exactly in integers.
  * Its candidate distances are then within M * scale / 2 of the FP32 table
- * sums; the returned distances are the rerank's. Every other form and shape
- * keeps FP32 tables. bits 32 (default): FP32 tables everywhere. */
+ * sums; the returned distances are the rerank's. bits 16 runs the mixed L2
+ * candidate scan that the exact rerank follows (k_factor > 1; PQ with
+ * 8-dimensional subspaces, M in {8, 12, 16}) on FP16 MFMA tiles of 16
+ * queries: the scaled query times the PQ-decoded entry, FP32 accumulation,
+ * plus the FP32 list and entry terms; batches below the tile minimum
+ * (mvg_ivfpq_set_tile_min_batch) take the 8-bit scan. Every other form and
+ * shape keeps FP32 tables. bits 32 (default): FP32 tables everywhere. */
 int mvg_ivfpq_set_table_bits(mvg_ivfpq *h, int bits);
+/* Tile-scan batch minimum (no reference counterpart): with table bits 16, a
+ * batch of at least min_queries queries takes the FP16 MFMA tile scan
+ * (0, the default: 16 queries per compute unit, one tile each; below that
+ * each tile's union of lists runs on one compute unit and the per-query
+ * scan is faster). Candidates of a query do not depend on the batch. */
+int mvg_ivfpq_set_tile_min_batch(mvg_ivfpq *h, int64_t min_queries);
 /* Skewed-list load balance (no reference counterpart): entries_per_item > 0
  * runs the scan as persistent workgroups over a device-built queue of
  * (query, entry-range) items of about that many list entries (at most 8 per

@@ -83,6 +83,7 @@ SIGNATURES = {
     "mvg_ivfpq_set_table_bits": (_i, [_p, _i]),
     "mvg_ivfpq_set_work_queue": (_i, [_p, _i]),
     "mvg_ivfpq_set_deep_select": (_i, [_p, _i, _i]),
+    "mvg_ivfpq_set_tile_min_batch": (_i, [_p, _i64]),
     "mvg_ivfpq_set_profiling": (_i, [_p, _i]),
     "mvg_ivfpq_read_profile": (_i, [_p, _p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "mvg_ivfpq_info": (_i, [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i), ctypes.POINTER(_i)]),

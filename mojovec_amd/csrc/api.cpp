@@ -916,7 +916,7 @@ extern "C" int mvg_ivfpq_search_device_noalloc(mvg_ivfpq *h, const float *q, int
   const bool mixed = h->precision != 0;
   if (h->dirty || h->row_of_dirty || (mixed && h->tables_dirty) ||
       (mixed && h->metric == MVG_METRIC_L2 && h->pre_dirty) ||
-      (mixed && h->metric == MVG_METRIC_L2 && h->table_bits == 8 && !h->pre8_ready) ||
+      (mixed && h->metric == MVG_METRIC_L2 && h->table_bits != 32 && !h->pre8_ready) ||
       (mixed && h->metric == MVG_METRIC_L2 && h->table_bits == 16 && !h->tile_ready) ||
       (mixed && h->prefilter > 0 && h->keep_vectors && h->vq_dirty))
     throw Error(mvg::kState,
@@ -993,7 +993,9 @@ void mvg::ivf_prepare(mvg_ivfpq *h) {
     h->pre_dirty = false;
     h->pre8_ready = false;
   }
-  if (h->precision != 0 && h->metric == MVG_METRIC_L2 && h->table_bits == 8 && !h->pre8_ready) {
+  // 8-bit terms: table bits 8, and 16 (its batches below the tile scan's
+  // minimum take the 8-bit deep-pool scan)
+  if (h->precision != 0 && h->metric == MVG_METRIC_L2 && h->table_bits != 32 && !h->pre8_ready) {
     h->entry_pre8.reserve(sizeof(float) * (size_t)std::max<int64_t>(h->ncsr, 1));
     h->table_mu.reserve(sizeof(float) * (size_t)h->d);
     mvg::precompute_centered_terms(h->coarse.as<float>(), h->nlist, h->d, h->codes.as<uint8_t>(), h->M,
@@ -1085,6 +1087,7 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
   a.ncsr = h->ncsr;
   a.deep_cap = h->top_sum.empty() ? 0 : h->top_sum[(size_t)a.nprobe];
   a.deep_min = h->deep_min;
+  a.tile_min_batch = h->tile_min_batch;
   a.deep_pool = h->deep_pool == 0 ? 1 : 0;  // ABI pool 0: keys in HBM; 1: running set in LDS
   a.kq = k_per_query;
   if (h->custom_ids && a.vectors) {
@@ -1118,7 +1121,7 @@ void mvg::ivf_search_with(mvg_ivfpq *h, const float *q, int64_t nq, int k,
         a.list_rank = h->list_rank.as<int32_t>();
       }
     }
-    if (h->metric == MVG_METRIC_L2 && h->table_bits == 8) {
+    if (h->metric == MVG_METRIC_L2 && h->table_bits != 32) {
       if (!h->pre8_ready) throw Error(mvg::kState, "8-bit table terms are not prepared");
       a.pre8 = h->entry_pre8.as<float>();
       a.mu = h->table_mu.as<float>();
@@ -1207,6 +1210,14 @@ int mvg_ivfpq_set_deep_select(mvg_ivfpq *h, int min_candidates, int pool) {
   mvg::ivf_check_unserved(h);
   h->deep_min = min_candidates;
   h->deep_pool = pool;
+  API_END
+}
+
+int mvg_ivfpq_set_tile_min_batch(mvg_ivfpq *h, int64_t min_queries) {
+  API_BEGIN
+  require(h && min_queries >= 0, "min_queries must be >= 0");
+  mvg::ivf_check_unserved(h);
+  h->tile_min_batch = min_queries;
   API_END
 }
 

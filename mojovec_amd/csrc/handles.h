@@ -13,6 +13,8 @@ struct mvg_ivfpq {
   int d = 0, nlist = 0, M = 0, metric = 0, nprobe = 1;
   bool trained = false, keep_vectors = true, dirty = true, tables_dirty = true;
   int precision = 0, split_scan = 0, wq_entries = 0, deep_min = 0, deep_pool = 0;
+  // table bits 16: fewest queries of a batch for the tile scan (0: 16 per CU)
+  int64_t tile_min_batch = 0;
   std::vector<int64_t> top_sum;  // top_sum[p]: entries in the p largest lists
   mvg::DevBuf ptab, coarse_bf16, coarse_norm, entry_pre;
   bool pre_dirty = true;  // entry_pre out of date with the lists or tables

@@ -2331,7 +2331,7 @@ template <int MF, bool IP, int MODE>
 bool launch_stream(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                    uint64_t *out_keys, IvfWork &w, hipStream_t s) {
   if constexpr (DEEP_MODE_L2(IP, MODE) && MF > 0) {
-    if (a.table_bits == 8 && a.pre8 && a.mu && a.k_factor > 1 && out_keys) {
+    if ((a.table_bits == 8 || a.table_bits == 16) && a.pre8 && a.mu && a.k_factor > 1 && out_keys) {
       // 32-bit offsets when the codes (the largest per-entry array) and a
       // pool segment stay below 4 GiB
       const int64_t cap = (std::max<int64_t>(a.deep_cap, 1) + 63) & ~int64_t(63);

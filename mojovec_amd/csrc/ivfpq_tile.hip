@@ -818,6 +818,10 @@ static void launch_tile_t(const TileScanArgs &ta, size_t sm, int64_t grid, hipSt
 bool launch_tile_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, int kk,
                       uint64_t *out_keys, IvfWork &w, hipStream_t s) {
   if (!tile_scan_supported(a, maxp) || !out_keys) return false;
+  // a batch of fewer tiles than CUs runs each tile's union on one CU: the
+  // per-query deep-pool scan is faster there
+  const int64_t min_batch = a.tile_min_batch > 0 ? a.tile_min_batch : (int64_t)kTQ * device_cus();
+  if (a.nq < min_batch) return false;
   const int T = kTQ;
   const int rc = std::max(kTNT, next_pow2(T * maxp));
   const TileLayout lay(a.M, rc);

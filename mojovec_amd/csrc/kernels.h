@@ -85,6 +85,7 @@ struct IvfSearchArgs {
   int64_t ncsr;    // list entries (CSR length of codes / ids / pre)
   int64_t deep_cap;  // sum of the nprobe largest lists: most entries one query visits
   int deep_min;      // k * k_factor from which the deep-pool scan runs (0: default)
+  int64_t tile_min_batch;  // table bits 16: fewest queries for the tile scan (0: 16 per CU)
   const float *pre;  // mixed L2: per-entry sum_m T[l][m][code_m] [ntotal] (nullable)
   int deep_pool;     // stream scan, large pools: 1 keys in HBM, 0 running set in LDS
   // mixed form: SQ8 codes of the stored rows (nullable) for the pre-rerank,

@@ -332,8 +332,14 @@ class IndexIVFPQ:
         """Mixed form, L2 scans that the exact rerank follows (k_factor > 1):
         8 sums 8-bit tables in the deep-pool scan; 16 runs the FP16 MFMA tile
         scan (query tiles times PQ-decoded entries; PQ with 8-dimensional
-        subspaces, M in {8, 12, 16}); 32 keeps FP32 tables."""
+        subspaces, M in {8, 12, 16}) on batches of at least the tile minimum
+        and the 8-bit scan below it; 32 keeps FP32 tables."""
         _lib.check(_lib.load().mvg_ivfpq_set_table_bits(self._h, int(bits)))
+
+    def set_tile_min_batch(self, min_queries: int = 0) -> None:
+        """Fewest queries of a batch for the FP16 tile scan (table bits 16);
+        0: 16 per compute unit."""
+        _lib.check(_lib.load().mvg_ivfpq_set_tile_min_batch(self._h, int(min_queries)))
 
     def set_work_queue(self, entries_per_item: int = 0) -> None:
         """Persistent (query, entry-range) work queue for skewed lists; 0 off."""

@@ -87,6 +87,7 @@ def test_null_handles_are_rejected_before_any_device_work():
         assert b"server needs an index" in lib.mvg_last_error()
     assert lib.mvg_ivfpq_set_prefilter(None, 8) != 0
     assert lib.mvg_ivfpq_set_table_bits(None, 8) != 0
+    assert lib.mvg_ivfpq_set_tile_min_batch(None, 1) != 0
     assert lib.mvg_last_error()
     # the later entry points check their handle first as well
     assert lib.mvg_ivfpq_add_with_ids(None, None, 1, None) != 0

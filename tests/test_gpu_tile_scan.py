@@ -56,11 +56,15 @@ def int_index(gpu, x, d, nlist, M, seed):
 
 
 def tile_search(gi, q, k, kf, **kw):
+    # these batches are smaller than the tile minimum (16 queries per CU):
+    # a minimum of 1 sends every batch to the tile scan
     gi.set_search_mode("mixed")
     gi.set_table_bits(16)
+    gi.set_tile_min_batch(1)
     try:
         return gi.search(q, k, k_factor=kf, **kw)
     finally:
+        gi.set_tile_min_batch(0)
         gi.set_table_bits(32)
         gi.set_search_mode("plain")
 
@@ -248,6 +252,7 @@ def test_tile_scan_graph_replays_equal_eager(float128):
     gi.nprobe = 16
     gi.set_search_mode("mixed")
     gi.set_table_bits(16)
+    gi.set_tile_min_batch(1)
     try:
         De, Ie = gi.search(q, 10, k_factor=288)
         dev = torch.device("cuda:0")
@@ -265,5 +270,23 @@ def test_tile_scan_graph_replays_equal_eager(float128):
             gi.search(q[:5], 10, k_factor=288)  # eager work between replays
         del g
     finally:
+        gi.set_tile_min_batch(0)
         gi.set_table_bits(32)
         gi.set_search_mode("plain")
+
+
+def test_tile_scan_small_batches_take_the_8bit_scan(float128):
+    # below the default minimum (16 queries per CU) table bits 16 runs the
+    # 8-bit deep-pool scan: the same results as table bits 8
+    x, q, oi, gi = float128
+    gi.nprobe = 16
+    gi.set_search_mode("mixed")
+    try:
+        gi.set_table_bits(8)
+        D8, I8 = gi.search(q, 10, k_factor=288)
+        gi.set_table_bits(16)
+        D16, I16 = gi.search(q, 10, k_factor=288)
+    finally:
+        gi.set_table_bits(32)
+        gi.set_search_mode("plain")
+    assert_same(D16, I16, D8, I8)
