@@ -444,7 +444,7 @@ __global__ __launch_bounds__(kNT) void rerank_group_kernel(const float *__restri
 
 // Mixed-form pre-rerank: the SQ8 code distance of each of a query's kk
 // candidates (the integer SQ8 distance of index_flat_sq8.mojo:80-177 over
-// codes of the stored rows), then the kk2 best of them, sorted, as the
+// codes of the stored rows), then the kk2 best of them, unordered, as the
 // candidates of the exact FP32 rerank. Eight lanes per candidate row, 16-byte
 // code chunks, eight rows per group in flight; keys are (integer distance << 32
 // | id), so equal code distances keep the lower id.
@@ -457,8 +457,8 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
     int per_k) {
   extern __shared__ __align__(16) unsigned char smem[];
   uint64_t *buf = reinterpret_cast<uint64_t *>(smem);  // kk
-  uint64_t *top = buf + ((kk + 1) & ~1);               // next_pow2(kk2)
-  SelectScratch64 *scr = reinterpret_cast<SelectScratch64 *>(top + next_pow2(kk2));
+  uint64_t *top = buf + ((kk + 1) & ~1);               // kk2
+  BucketScratch *scr = reinterpret_cast<BucketScratch *>(top + ((kk2 + 1) & ~1));
   const int64_t qi = rerank_query(nq, order);
   if (qi < 0) return;
   constexpr int G = kNT / 8, R = 8;
@@ -529,7 +529,6 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
     }
   }
   __syncthreads();
-  const int cnt = select_sort_u64(buf, kk, kk2, *scr, top);
   // per-query k: the width a search with k = kq[qi] alone would keep
   int w2 = kk2;
   if (kq) {
@@ -537,7 +536,10 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
     w2 = (int)(per_k > 0 ? min(kkq, max((int64_t)64, (int64_t)per_k * kqi)) : kkq);
     w2 = min(w2, kk2);
   }
-  for (int t = tid; t < kk2; t += kNT) out[qi * kk2 + t] = (t < cnt && t < w2) ? top[t] : kSentinel64;
+  // the w2 smallest (code distance, id) keys, unordered: the rerank selects
+  // over its candidates
+  const int cnt = bucket_select_u64(buf, kk, w2, top, *scr);
+  for (int t = tid; t < kk2; t += kNT) out[qi * kk2 + t] = t < cnt ? top[t] : kSentinel64;
 }
 
 // ---------------------------------------------------------------------------
@@ -2499,7 +2501,7 @@ void rerank_mixed(const IvfSearchArgs &a, IvfWork &w, const uint64_t *keys, int 
   w.vq_keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk2);
   sq8_encode(a.q, a.nq, a.d, ip ? 1 : 0, ip ? 0.0f : a.vq_min, 1.0f / a.vq_scale,
              w.vq_qcodes.as<uint8_t>(), w.vq_qnorms.as<uint32_t>(), nullptr, s);
-  const size_t sm = sizeof(uint64_t) * (size_t)(((kk + 1) & ~1) + next_pow2(kk2)) + sizeof(SelectScratch64);
+  const size_t sm = sizeof(uint64_t) * (size_t)(((kk + 1) & ~1) + ((kk2 + 1) & ~1)) + sizeof(BucketScratch);
   if (sm > 150 * 1024) throw Error(kInvalid, "pre-rerank workspace exceeds LDS");
   const unsigned grid =
       (unsigned)(order ? kRerankRanges * ((a.nq + kRerankRanges - 1) / kRerankRanges) : a.nq);

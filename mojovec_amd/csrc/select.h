@@ -324,4 +324,148 @@ __device__ uint64_t wave_fold_keys(const uint32_t *keys, int len, uint32_t pos0,
   return best;
 }
 
+// ---------------------------------------------------------------------------
+// Unordered workgroup selection of the w smallest u64 keys by range buckets.
+
+struct BucketScratch {
+  uint64_t bucket[64];
+  unsigned long long red[2 * kNW];
+  int hist[1024];
+  int sh[16];
+  int nout, nb, bin, before, cnt;
+};
+
+// Writes the min(w, valid) smallest keys of in[0..n) (sentinels excluded;
+// equal keys are interchangeable) to out[0..cnt) in no particular order and
+// returns cnt; out must hold w keys and must not overlap in. The keys still
+// in play lie in [lo, hi], starting at the smallest and largest valid key. A
+// pass counts them in 1024 bins of (key - lo) >> sh, sh the least shift that
+// fits hi - lo into 10 bits, writes out every key below the bin where the
+// running count reaches the number still needed, and keeps that bin in play;
+// it ends when the bin holds exactly that number (all written), at most 64
+// keys (one wavefront sorts them and writes the smallest) or one value (any
+// of them). Each pass narrows the range 1024-fold: at most seven passes.
+// All threads must call it.
+__device__ int bucket_select_u64(const uint64_t *in, int n, int w, uint64_t *out, BucketScratch &s) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint64_t mn = kSentinel64, mx = 0;
+  int valid = 0;
+  for (int i = tid; i < n; i += kNT) {
+    const uint64_t k = in[i];
+    if (k != kSentinel64) {
+      mn = k < mn ? k : mn;
+      mx = k > mx ? k : mx;
+      valid++;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a = __shfl_xor(mn, o, 64), b = __shfl_xor(mx, o, 64);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+    valid += __shfl_xor(valid, o, 64);
+  }
+  if (lane == 0) {
+    s.red[wave] = mn;
+    s.red[kNW + wave] = mx;
+    s.sh[wave] = valid;
+  }
+  if (tid == 0) s.nout = 0;
+  __syncthreads();
+  uint64_t lo = kSentinel64, hi = 0;
+  int nvalid = 0;
+#pragma unroll
+  for (int v = 0; v < kNW; v++) {
+    lo = s.red[v] < lo ? s.red[v] : lo;
+    hi = s.red[kNW + v] > hi ? s.red[kNW + v] : hi;
+    nvalid += s.sh[v];
+  }
+  __syncthreads();
+  if (nvalid <= w) {  // every valid key
+    for (int i = tid; i < n; i += kNT) {
+      const uint64_t k = in[i];
+      if (k != kSentinel64) out[atomicAdd(&s.nout, 1)] = k;
+    }
+    __syncthreads();
+    return nvalid;
+  }
+  int need = w;  // keys still to write; [lo, hi] holds at least that many
+  for (;;) {
+    const uint64_t width = hi - lo;
+    if (width == 0) {  // one value: any `need` of its keys
+      if (tid == 0) s.nb = 0;
+      __syncthreads();
+      const int base = s.nout;
+      for (int i = tid; i < n; i += kNT) {
+        if (in[i] == lo) {
+          const int slot = atomicAdd(&s.nb, 1);
+          if (slot < need) out[base + slot] = lo;
+        }
+      }
+      __syncthreads();
+      break;
+    }
+    const int sh = (width >> 10) == 0 ? 0 : 64 - __clzll((long long)width) - 10;
+    for (int b = tid; b < 1024; b += kNT) s.hist[b] = 0;
+    if (tid == 0) s.nb = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += kNT) {
+      const uint64_t k = in[i];
+      if (k != kSentinel64 && k >= lo && k <= hi) atomicAdd(&s.hist[(int)((k - lo) >> sh)], 1);
+    }
+    __syncthreads();
+    int c4[4], loc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      c4[j] = s.hist[4 * tid + j];
+      loc += c4[j];
+    }
+    int total;
+    const int ex = block_excl_scan(loc, s.sh, total);
+    if (ex < need && ex + loc >= need) {
+      int run = ex;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (run + c4[j] >= need) {
+          s.bin = 4 * tid + j;
+          s.before = run;
+          s.cnt = c4[j];
+          break;
+        }
+        run += c4[j];
+      }
+    }
+    __syncthreads();
+    const int b = s.bin, before = s.before, cb = s.cnt;
+    const uint64_t blo = lo + ((uint64_t)b << sh);
+    const uint64_t bhi = (b == 1023 || ((uint64_t)(b + 1) << sh) - 1 >= width) ? hi
+                                                                              : lo + ((uint64_t)(b + 1) << sh) - 1;
+    need -= before;
+    const bool all = cb == need, small = cb <= 64;
+    for (int i = tid; i < n; i += kNT) {
+      const uint64_t k = in[i];
+      if (k == kSentinel64 || k < lo || k > bhi) continue;
+      if (k < blo || all) out[atomicAdd(&s.nout, 1)] = k;
+      else if (small) s.bucket[atomicAdd(&s.nb, 1)] = k;
+    }
+    __syncthreads();
+    if (all) break;
+    if (small) {
+      if (wave == 0) {
+        uint64_t v = lane < s.nb ? s.bucket[lane] : kSentinel64;
+        v = wave_sort64(v);
+        const int base = s.nout;
+        if (lane < need) out[base + lane] = v;
+      }
+      __syncthreads();
+      if (tid == 0) s.nout += need;
+      __syncthreads();
+      break;
+    }
+    lo = blo;
+    hi = bhi;
+  }
+  return w;
+}
+
 }  // namespace mvg
