@@ -484,12 +484,18 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
       }
       acc[t] = 0;
     }
+    // every row's chunk loaded without a branch (an invalid candidate reads
+    // row 0 and is dropped below), so the R loads issue back to back
+#pragma unroll
+    for (int t = 0; t < R; t++) row[t] = ok[t] ? row[t] : 0;
     for (int w = gl; w < words; w += 8) {
       const uint4 a = qa[w];
+      uint4 bv[R];
+#pragma unroll
+      for (int t = 0; t < R; t++) bv[t] = reinterpret_cast<const uint4 *>(vcodes + (int64_t)row[t] * ld)[w];
 #pragma unroll
       for (int t = 0; t < R; t++) {
-        if (!ok[t]) continue;
-        const uint4 b = reinterpret_cast<const uint4 *>(vcodes + (int64_t)row[t] * ld)[w];
+        const uint4 b = bv[t];
         if (IP) {
           int v = (int)acc[t];
           v = __builtin_amdgcn_sdot4((int)a.x, (int)b.x, v, false);
@@ -505,10 +511,14 @@ __global__ __launch_bounds__(kNT) void sq8_prefilter_kernel(
         }
       }
     }
+    // sum over the row's 8 lanes (one half-row of the wavefront; a group's
+    // lanes are active together): DPP lane xor 1 and xor 2 within each quad,
+    // then the half-row mirror (lane i with lane 7 - i: the other quad)
 #pragma unroll
     for (int t = 0; t < R; t++) {
-#pragma unroll
-      for (int o = 4; o >= 1; o >>= 1) acc[t] += (uint32_t)__shfl_xor((int)acc[t], o, 8);
+      acc[t] += (uint32_t)__builtin_amdgcn_mov_dpp((int)acc[t], 0xB1, 0xF, 0xF, false);
+      acc[t] += (uint32_t)__builtin_amdgcn_mov_dpp((int)acc[t], 0x4E, 0xF, 0xF, false);
+      acc[t] += (uint32_t)__builtin_amdgcn_mov_dpp((int)acc[t], 0x141, 0xF, 0xF, false);
     }
     if (gl == 0) {
 #pragma unroll
