@@ -248,6 +248,29 @@ __device__ int select_sort_u64(const uint64_t *in, int n, int k, Scratch &s,
 // sentinel = kSentinel64): bitonic networks across the 64 lanes, no LDS and
 // no workgroup barrier.
 
+// Value of lane `lane ^ j` (j a power of two below 64, a constant after
+// unrolling). Strides 1, 2 and 8 are DPP moves (quad permutes, row rotate by
+// 8); stride 4 takes whichever of the two row rotates by 4 and 12 brings lane
+// ^ 4 (checked on the lane ids, so either rotate direction is right); 16 and
+// 32 go through the LDS permute.
+__device__ __forceinline__ uint32_t dpp_xor32(uint32_t v, int j) {
+  if (j == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+  if (j == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+  if (j == 8) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);
+  if (j == 4) {
+    const int lane = threadIdx.x & 63;
+    const int from4 = __builtin_amdgcn_mov_dpp(lane, 0x124, 0xF, 0xF, false);
+    const uint32_t a = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x124, 0xF, 0xF, false);
+    const uint32_t b = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x12C, 0xF, 0xF, false);
+    return from4 == (lane ^ 4) ? a : b;
+  }
+  return (uint32_t)__shfl_xor((int)v, j, 64);
+}
+__device__ __forceinline__ uint64_t xor_lane64(uint64_t v, int j) {
+  const uint32_t lo = dpp_xor32((uint32_t)v, j), hi = dpp_xor32((uint32_t)(v >> 32), j);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Ascending bitonic sort of the wave's 64 values.
 __device__ __forceinline__ uint64_t wave_sort64(uint64_t v) {
   const int lane = threadIdx.x & 63;
@@ -255,7 +278,7 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t v) {
   for (int k = 2; k <= 64; k <<= 1) {
 #pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
-      const uint64_t p = __shfl_xor(v, j, 64);
+      const uint64_t p = xor_lane64(v, j);
       const bool asc = (lane & k) == 0;
       const bool lower = (lane & j) == 0;
       v = (lower == asc) ? (p < v ? p : v) : (p > v ? p : v);
@@ -272,7 +295,7 @@ __device__ __forceinline__ uint64_t wave_merge64(uint64_t a, uint64_t b) {
   uint64_t v = a < r ? a : r;
 #pragma unroll
   for (int j = 32; j > 0; j >>= 1) {
-    const uint64_t p = __shfl_xor(v, j, 64);
+    const uint64_t p = xor_lane64(v, j);
     v = (lane & j) == 0 ? (p < v ? p : v) : (p > v ? p : v);
   }
   return v;
@@ -283,41 +306,69 @@ __device__ __forceinline__ uint64_t wave_merge64(uint64_t a, uint64_t b) {
 // smallest of the union end in lanes 0..k-1. Keys past `len` and sentinel
 // keys are ignored. A key can only enter the top k if it is at most the
 // current k-th best, or -- while fewer than k are held -- at most the k-th
-// smallest of the per-lane minima of this block (k lanes hold a key no larger
-// than that); the few keys under that bound are merged 64 at a time.
+// smallest of the lanes' two smallest composites (at least k keys lie at or
+// below that, and few more unless a lane holds three of the block's k
+// smallest). When the keys under the bound fit a wavefront they are compacted
+// into the block's first 64 composites (the block is overwritten: the caller's
+// wavefront owns it and reads it no more) and sorted once; otherwise they are
+// merged 64 at a time.
 template <int CHUNK>
-__device__ uint64_t wave_fold_keys(const uint32_t *keys, int len, uint32_t pos0, uint64_t best,
-                                   int k) {
+__device__ uint64_t wave_fold_keys(uint32_t *keys, int len, uint32_t pos0, uint64_t best, int k) {
   const int lane = threadIdx.x & 63;
   constexpr int PER = CHUNK / 64;
-  uint64_t theta = __shfl(best, k - 1, 64);
-  if (theta == kSentinel64) {
-    uint64_t lmin = kSentinel64;
-#pragma unroll
-    for (int j = 0; j < PER; j++) {
-      const int idx = lane + 64 * j;
-      const uint32_t key = idx < len ? keys[idx] : kSentinel;
-      if (key != kSentinel) {
-        const uint64_t c = ((uint64_t)key << 32) | (pos0 + (uint32_t)idx);
-        lmin = c < lmin ? c : lmin;
-      }
-    }
-    theta = __shfl(wave_sort64(lmin), k - 1, 64);
-  }
-  uint32_t mask = 0;
+  uint32_t kr[PER];
 #pragma unroll
   for (int j = 0; j < PER; j++) {
     const int idx = lane + 64 * j;
-    const uint32_t key = idx < len ? keys[idx] : kSentinel;
-    if (key != kSentinel && (((uint64_t)key << 32) | (pos0 + (uint32_t)idx)) <= theta) mask |= 1u << j;
+    kr[j] = idx < len ? keys[idx] : kSentinel;
+  }
+  auto comp = [&](int j) { return ((uint64_t)kr[j] << 32) | (pos0 + (uint32_t)(lane + 64 * j)); };
+  uint64_t theta = __shfl(best, k - 1, 64);
+  if (theta == kSentinel64) {
+    uint64_t m1 = kSentinel64, m2 = kSentinel64;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      if (kr[j] == kSentinel) continue;
+      const uint64_t c = comp(j);
+      if (c < m1) {
+        m2 = m1;
+        m1 = c;
+      } else if (c < m2) {
+        m2 = c;
+      }
+    }
+    theta = __shfl(wave_merge64(wave_sort64(m1), wave_sort64(m2)), k - 1, 64);
+  }
+  uint32_t mask = 0;
+#pragma unroll
+  for (int j = 0; j < PER; j++)
+    if (kr[j] != kSentinel && comp(j) <= theta) mask |= 1u << j;
+  const int mine = __popc(mask);
+  int incl = mine;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  const int total = __shfl(incl, 63, 64);
+  if (total == 0) return best;
+  if (total <= 64) {
+    // every lane's keys are in registers: the block's first 64 composites
+    // take the candidates (a wavefront's LDS operations complete in order)
+    uint64_t *slot = reinterpret_cast<uint64_t *>(keys);
+    int at = incl - mine;
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+      if ((mask >> j) & 1u) slot[at++] = comp(j);
+    const uint64_t c = lane < total ? slot[lane] : kSentinel64;
+    return wave_merge64(best, wave_sort64(c));
   }
   while (__ballot(mask != 0) != 0) {
     uint64_t c = kSentinel64;
     if (mask) {
       const int j = __builtin_ctz(mask);
       mask &= mask - 1;
-      const int idx = lane + 64 * j;
-      c = ((uint64_t)keys[idx] << 32) | (pos0 + (uint32_t)idx);
+      c = ((uint64_t)keys[lane + 64 * j] << 32) | (pos0 + (uint32_t)(lane + 64 * j));
     }
     best = wave_merge64(best, wave_sort64(c));
   }
