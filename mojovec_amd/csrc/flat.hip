@@ -223,6 +223,27 @@ __global__ __launch_bounds__(kNT) void merge_topk_kernel(
   }
 }
 
+// Results of queries whose candidate rows are already their sorted top k
+// (sentinel padded): one thread per slot.
+__global__ void sorted_topk_out_kernel(const uint64_t *__restrict__ cand, int64_t total, int negate,
+                                       float pad_dist, float *__restrict__ D, int64_t *__restrict__ I64,
+                                       int32_t *__restrict__ I32, uint64_t *__restrict__ out_keys) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= total) return;
+  const uint64_t kv = cand[o];
+  float dv = pad_dist;
+  int64_t iv = -1;
+  if (kv != kSentinel64) {
+    dv = key2f((uint32_t)(kv >> 32));
+    if (negate) dv = -dv;
+    iv = (int64_t)(uint32_t)(kv & 0xFFFFFFFFu);
+  }
+  if (D) D[o] = dv;
+  if (I64) I64[o] = iv;
+  if (I32) I32[o] = (int32_t)iv;
+  if (out_keys) out_keys[o] = kv;
+}
+
 void launch_merge(const uint64_t *cand, int64_t nq, int64_t ncand, int k, int negate,
                   float pad_dist, float *D, int64_t *I64, int32_t *I32, uint64_t *keys,
                   hipStream_t s, const int64_t *seg_off, const int32_t *kq, int kmul) {
@@ -289,6 +310,14 @@ void flat_topk(const float *base, int64_t n, int d, int metric, const float *q, 
       default: MVG_FLAT_CASE(0) break;
     }
 #undef MVG_FLAT_CASE
+  }
+  if (nchunks == 1 && k <= 64) {
+    // one chunk: the wavefront fold wrote each query's sorted top k already
+    const int64_t total = nq * (int64_t)k;
+    hipLaunchKernelGGL(sorted_topk_out_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s,
+                       part.as<uint64_t>(), total, metric == kMetricIP ? 1 : 0, pad_dist, D, I64, I32, keys);
+    MVG_LAUNCH_CHECK("sorted_topk_out_kernel");
+    return;
   }
   launch_merge(nchunks > 0 ? part.as<uint64_t>() : nullptr, nq, (int64_t)nchunks * k, k,
                metric == kMetricIP ? 1 : 0, pad_dist, D, I64, I32, keys, s);

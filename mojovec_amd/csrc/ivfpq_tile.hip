@@ -15,10 +15,10 @@
 //
 // Selection (the k * k_factor smallest (distance, id) pairs of each query,
 // exactly, for these distances): two passes over the tile's lists.
-//   1. a sample of the keys (every fourth block of 16 entries) is counted in
+//   1. a sample of the keys (every eighth block of 16 entries) is counted in
 //      a 512-bin histogram per query (1/64 octave per bin, over a window of
 //      eight octaves placed at the query's nearest-centroid distance) under a
-//      running bound aimed at a quarter of 1.25 kk keys; the bound only
+//      running bound aimed at an eighth of 1.25 kk keys; the bound only
 //      tightens, so the counts are exact at and below it;
 //   2. every key below the bin edge pass 1 found is collected into a
 //      per-query buffer and counted in an exact histogram; if at least kk
@@ -52,7 +52,7 @@ constexpr int kTGroup = 4;          // blocks of 16 entries loaded together, one
 constexpr int kTStage = 128;        // per-wave staging of pass-2 keys
 constexpr int kTRanges = 8;         // tile ranges, one per XCD
 constexpr int kTPerQuery = 8;       // per-query words in LDS
-constexpr int kTSample = 4;         // pass 1 visits every kTSample-th block of 16 entries
+constexpr int kTSample = 8;         // pass 1 visits every kTSample-th block of 16 entries
 constexpr int kTNT = 512;           // threads per workgroup: two waves per SIMD
 constexpr int kTNW = kTNT / 64;
 constexpr int kTSelW = 4;           // waves that select (the codebook's LDS holds their buffers)
@@ -833,7 +833,7 @@ bool launch_tile_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, i
   const int64_t ntiles = (a.nq + T - 1) / T;
   const int bufcap = a.M * kKsub * 16 / kTSelW / 8;
   // pass 1 aims its sample at 1.25 kk / kTSample + 16 keys: the bound then
-  // lies near the 1.25 kk-th key, six standard deviations of the sample
+  // lies near the 1.25 kk-th key, five standard deviations of the sample
   // count above the kk-th at kk = 2880; a query whose collected keys still
   // fall short of kk goes to the deep-pool scan
   const int kk1 = (5 * kk + 4 * kTSample - 1) / (4 * kTSample) + 16;
