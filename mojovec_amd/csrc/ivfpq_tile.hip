@@ -457,27 +457,48 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
   const char *cbb = reinterpret_cast<const char *>(cb);
   int nst = 0;  // staged pass-2 keys of this wave
   int u = 0;    // union cursor: items are claimed in increasing order
-  for (;;) {
+  // An item: its first entry of this lane, its end, the blocks visited, its
+  // list's base-term run; past the last item, entries from 0 with no block.
+  struct Item {
+    int it, nblk, rs, rl;
+    uint32_t ib, ie;
+  };
+  auto claim = [&]() __attribute__((always_inline)) {
     int it = 0;
     if (lane == 0) it = atomicAdd(itc, 1);
     it = __shfl(it, 0, 64);
-    if (it >= NI) break;
-    while (u + 1 < U && (int)(ua[u + 1] >> 32) <= it) u++;
-    const uint64_t ea = ua[u], eb = ub[u];
-    const uint32_t chunk = (uint32_t)(it - (int)(ea >> 32));
-    const int rs = (int)((ea >> 8) & 0xFFFFFFu), rl = (int)(ea & 0xFFu);
-    const uint32_t lbeg = (uint32_t)(eb >> 32), llen = (uint32_t)eb;
-    const uint32_t ib = lbeg + chunk * kTItem + lrow;
-    const uint32_t ie = lbeg + min(llen, (chunk + 1) * (uint32_t)kTItem);
-    const int nblk = (int)((ie - (ib - lrow) + 16 * STRIDE - 1) / (16 * STRIDE));  // blocks visited
-    const int ngrp = (nblk + kTGroup - 1) / kTGroup;
-    TileGrp<MF> ga, gb;
-    tile_load<MF, IDS, STRIDE>(src, ib, ga);
+    Item m{it, 0, 0, 0, (uint32_t)lrow, 0u};
+    if (it < NI) {
+      while (u + 1 < U && (int)(ua[u + 1] >> 32) <= it) u++;
+      const uint64_t ea = ua[u], eb = ub[u];
+      const uint32_t chunk = (uint32_t)(it - (int)(ea >> 32));
+      m.rs = (int)((ea >> 8) & 0xFFFFFFu);
+      m.rl = (int)(ea & 0xFFu);
+      const uint32_t lbeg = (uint32_t)(eb >> 32), llen = (uint32_t)eb;
+      m.ib = lbeg + chunk * kTItem + lrow;
+      m.ie = lbeg + min(llen, (chunk + 1) * (uint32_t)kTItem);
+      m.nblk = (int)((m.ie - (m.ib - lrow) + 16 * STRIDE - 1) / (16 * STRIDE));  // blocks visited
+    }
+    return m;
+  };
+  // two load buffers in turn: a group's loads are issued one group ahead,
+  // unconditionally (a conditional load leaves the compiler unable to wait
+  // for the older group alone; past the item the entries are unused and
+  // past the arrays the buffer reads return zero). An item runs an even
+  // number of groups (a last empty one costs its loads only), so the next
+  // item's first group is always loaded into the first buffer, during the
+  // current item's last group.
+  constexpr uint32_t kStep = 16u * STRIDE * kTGroup;
+  Item cur = claim();
+  TileGrp<MF> ga, gb;
+  tile_load<MF, IDS, STRIDE>(src, cur.ib, ga);
+  while (cur.it < NI) {
+    const Item nxt = claim();
     // this list's base term per tile query (+inf where the query does not probe it)
     if (lane < kTQ) wbw[lane] = __builtin_inff();
     wave_sync();
-    if (lane < rl) {
-      const uint64_t v = run[rs + lane];
+    if (lane < cur.rl) {
+      const uint64_t v = run[cur.rs + lane];
       wbw[(int)(v >> 32)] = __uint_as_float((uint32_t)v);
     }
     wave_sync();
@@ -488,19 +509,14 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
       ti.base[j] = wbw[s];
       ti.E[j] = E_s[s];
     }
-    // two load buffers in turn: a group's loads are issued one group ahead,
-    // unconditionally (a conditional load leaves the compiler unable to wait
-    // for the older group alone; past the item the entries are unused and
-    // past the arrays the buffer reads return zero)
-    constexpr uint32_t kStep = 16u * STRIDE * kTGroup;
+    const int ngrp = (((cur.nblk + kTGroup - 1) / kTGroup) + 1) & ~1;
     for (int gi = 0; gi < ngrp; gi += 2) {
-      tile_load<MF, IDS, STRIDE>(src, ib + kStep * (gi + 1), gb);
-      tile_group<MF, PASS, STRIDE>(a, ga, ib + kStep * gi, ie, nblk - gi * kTGroup, qf, inv, anc, ti, cbb, cb_lane,
-                                   hist, st, nst, cnt_s, anc_s, cbw);
-      if (gi + 1 >= ngrp) break;
-      tile_load<MF, IDS, STRIDE>(src, ib + kStep * (gi + 2), ga);
-      tile_group<MF, PASS, STRIDE>(a, gb, ib + kStep * (gi + 1), ie, nblk - (gi + 1) * kTGroup, qf, inv, anc, ti,
+      tile_load<MF, IDS, STRIDE>(src, cur.ib + kStep * (gi + 1), gb);
+      tile_group<MF, PASS, STRIDE>(a, ga, cur.ib + kStep * gi, cur.ie, cur.nblk - gi * kTGroup, qf, inv, anc, ti,
                                    cbb, cb_lane, hist, st, nst, cnt_s, anc_s, cbw);
+      tile_load<MF, IDS, STRIDE>(src, gi + 2 < ngrp ? cur.ib + kStep * (gi + 2) : nxt.ib, ga);
+      tile_group<MF, PASS, STRIDE>(a, gb, cur.ib + kStep * (gi + 1), cur.ie, cur.nblk - (gi + 1) * kTGroup, qf, inv,
+                                   anc, ti, cbb, cb_lane, hist, st, nst, cnt_s, anc_s, cbw);
     }
     if constexpr (PASS == 1) {
       // tighten the bounds of this wave's queries (counts only grow, so a
@@ -511,6 +527,7 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
         if (lane == 0 && bn >= 0) E_s[s] = bin_edge(anc_s[s], bn + 1);
       }
     }
+    cur = nxt;
   }
   if constexpr (PASS == 2) {
     if (nst > 0) stage_flush(a, st, nst, cnt_s, anc_s, hist, cbw);
