@@ -457,6 +457,7 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
   const char *cbb = reinterpret_cast<const char *>(cb);
   int nst = 0;  // staged pass-2 keys of this wave
   int u = 0;    // union cursor: items are claimed in increasing order
+  int nupd = 0;  // pass-1 items of this wave
   // An item: its first entry of this lane, its end, the blocks visited, its
   // list's base-term run; past the last item, entries from 0 with no block.
   struct Item {
@@ -519,13 +520,15 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
                                    anc, ti, cbb, cb_lane, hist, st, nst, cnt_s, anc_s, cbw);
     }
     if constexpr (PASS == 1) {
-      // tighten the bounds of this wave's queries (counts only grow, so a
-      // bound read while others add is still an upper bound)
-      for (int s = wave; s < kTQ; s += kTNW) {
-        int bef, cnt;
-        const int bn = wave_find_bin(hist + s * kTBins, a.kk1, bef, cnt);
-        if (lane == 0 && bn >= 0) E_s[s] = bin_edge(anc_s[s], bn + 1);
-      }
+      // tighten the bounds of this wave's queries after every fourth item
+      // (counts only grow, so a bound read while others add is still an
+      // upper bound; a looser one only lets more sampled keys be counted)
+      if ((++nupd & 3) == 0)
+        for (int s = wave; s < kTQ; s += kTNW) {
+          int bef, cnt;
+          const int bn = wave_find_bin(hist + s * kTBins, a.kk1, bef, cnt);
+          if (lane == 0 && bn >= 0) E_s[s] = bin_edge(anc_s[s], bn + 1);
+        }
     }
     cur = nxt;
   }
