@@ -161,33 +161,36 @@ __device__ int wave_find_bin(const uint32_t *h, int need, int &before, int &coun
   int local = 0;
 #pragma unroll
   for (int i = 0; i < 8; i++) local += hs[i];
+  // inclusive scan over the wave: DPP row shifts by 1, 2, 4, 8 within each
+  // 16-lane row (a lane with no source in its row adds 0), then the totals
+  // of the rows below (lanes 15, 31, 47 read as scalars)
   int incl = local;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += y;
-  }
-  if (total) *total = __shfl(incl, 63, 64);
+  incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xF, 0xF, false);
+  incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xF, 0xF, false);
+  incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xF, 0xF, false);
+  incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xF, 0xF, false);
+  const int r0 = __builtin_amdgcn_readlane(incl, 15), r1 = __builtin_amdgcn_readlane(incl, 31),
+            r2 = __builtin_amdgcn_readlane(incl, 47), r3 = __builtin_amdgcn_readlane(incl, 63);
+  incl += (lane >= 16 ? r0 : 0) + (lane >= 32 ? r1 : 0) + (lane >= 48 ? r2 : 0);
+  if (total) *total = r0 + r1 + r2 + r3;
   const uint64_t hit = __ballot(incl >= need);
   if (hit == 0) return -1;
   const int L = __ffsll((unsigned long long)hit) - 1;
-  int c = __shfl(incl - local, L, 64);
+  int c = __builtin_amdgcn_readlane(incl - local, L);
   int bin = -1, cnt = 0, bef = 0;
-  if (lane == L) {
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      if (bin < 0 && c + hs[i] >= need) {
-        bin = 8 * L + i;
-        cnt = hs[i];
-        bef = c;
-      }
-      c += hs[i];
+  for (int i = 0; i < 8; i++) {
+    if (bin < 0 && c + hs[i] >= need) {
+      bin = 8 * L + i;
+      cnt = hs[i];
+      bef = c;
     }
+    c += hs[i];
   }
-  bin = __shfl(bin, L, 64);
-  before = __shfl(bef, L, 64);
-  count = __shfl(cnt, L, 64);
-  return bin;
+  // lane L's findings
+  before = __builtin_amdgcn_readlane(bef, L);
+  count = __builtin_amdgcn_readlane(cnt, L);
+  return __builtin_amdgcn_readlane(bin, L);
 }
 
 // The `need` smallest of the unique words buf[0..n) (every one of them shares
