@@ -55,7 +55,7 @@ constexpr int kTPerQuery = 8;       // per-query words in LDS
 constexpr int kTSample = 8;         // pass 1 visits every kTSample-th block of 16 entries
 constexpr int kTNT = 512;           // threads per workgroup: two waves per SIMD
 constexpr int kTNW = kTNT / 64;
-constexpr int kTSelW = 4;           // waves that select (the codebook's LDS holds their buffers)
+constexpr int kTSelW = 4;           // waves of the second selection round (the codebook's LDS holds their buffers)
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -728,18 +728,26 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
 
     // ---- per query (one wavefront each): the kk smallest of the collected
     // keys by their exact histogram; the codebook's LDS holds the boundary
-    uint64_t *wbuf = reinterpret_cast<uint64_t *>(smem + lay.cb) + wave * a.bufcap;
-    for (int s = wave; s < T && wave < kTSelW; s += kTSelW) {
+    // bins. Round 1: every wavefront, bufcap / 2 keys of boundary buffer
+    // each; a query whose boundary bin holds more waits for round 2 (kTSelW
+    // wavefronts, bufcap keys each; np_s, free after the set-up, lists them).
+    if (tid == 0) misc[5] = 0;
+    __syncthreads();
+    auto select_query = [&](int s, uint64_t *wbuf, int cap, bool defer) __attribute__((always_inline)) {
       const int q = qid_s[s];
-      if (q < 0) continue;
+      if (q < 0) return;
       const int n = cnt_s[s];
       int bef = 0, cnt = 0;
       const int b = wave_find_bin(hist + s * kTBins, a.kk, bef, cnt);
+      if (defer && b >= 0 && cnt > cap && n <= a.cmax) {
+        if (lane == 0) np_s[atomicAdd(&misc[5], 1)] = s;
+        return;
+      }
       // fewer keys below a finite bound than kk (the sample's margin missed),
       // more than the buffers hold: the deep-pool scan takes the query
-      if (n > a.cmax || (b < 0 && E_s[s] != __builtin_inff()) || (b >= 0 && cnt > a.bufcap)) {
+      if (n > a.cmax || (b < 0 && E_s[s] != __builtin_inff()) || (b >= 0 && cnt > cap)) {
         if (lane == 0) a.ovf_list[atomicAdd(a.ovf_count, 1u)] = q;
-        continue;
+        return;
       }
       uint64_t *out = a.out_keys + (int64_t)q * a.kk;
       const uint64_t *cq = cbw + (int64_t)s * a.cmax;
@@ -759,7 +767,7 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
           const bool lo = j < n && (b < 0 || bn < b), eq = j < n && b >= 0 && bn == b;
           const uint64_t ml = __ballot(lo), me = __ballot(eq);
           if (lo && nout + lane_rank(ml) < a.kk) out[nout + lane_rank(ml)] = v[r];
-          if (eq && nb + lane_rank(me) < a.bufcap) wbuf[nb + lane_rank(me)] = v[r];
+          if (eq && nb + lane_rank(me) < cap) wbuf[nb + lane_rank(me)] = v[r];
           nout += (int)__popcll(ml);
           nb += (int)__popcll(me);
         }
@@ -778,6 +786,17 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
       }
       for (int j = nout + taken + lane; j < a.kk; j += 64) out[j] = kSentinel64;
       if (a.pool_writes && lane == 0) atomicAdd(a.pool_writes, (unsigned long long)n);
+    };
+    {
+      const int cap = a.bufcap / 2;
+      uint64_t *wbuf = reinterpret_cast<uint64_t *>(smem + lay.cb) + wave * cap;
+      for (int s = wave; s < T; s += kTNW) select_query(s, wbuf, cap, true);
+    }
+    __syncthreads();
+    {
+      const int nd = misc[5];
+      uint64_t *wbuf = reinterpret_cast<uint64_t *>(smem + lay.cb) + wave * a.bufcap;
+      for (int i = wave; i < nd && wave < kTSelW; i += kTSelW) select_query(np_s[i], wbuf, a.bufcap, false);
     }
   }
 }
