@@ -48,6 +48,7 @@ constexpr int kTBins = 512;         // per-query key bins (32-bit counts)
 constexpr int kTShift = 17;         // key bits below a bin: 1/64 octave
 constexpr int kTBelow = 192;        // window start: 3 octaves below the nearest centroid
 constexpr int kTItem = 512;         // list entries per work item
+constexpr int kTItem1 = 4096;       // list entries per pass-1 work item (where its prefix fits in LDS)
 constexpr int kTGroup = 4;          // blocks of 16 entries loaded together, one group ahead
 constexpr int kTStage = 128;        // per-wave staging of pass-2 keys
 constexpr int kTRanges = 8;         // tile ranges, one per XCD
@@ -96,14 +97,15 @@ __device__ __forceinline__ float bin_edge(int anc, int b) {
 }
 
 struct TileLayout {
-  int cb, hist, run, ua, ub, pq, wb, stg, misc, total;
-  __host__ __device__ TileLayout(int MF, int rc) {
+  int cb, hist, run, ua, ub, ua1, pq, wb, stg, misc, total;
+  __host__ __device__ TileLayout(int MF, int rc, bool items1) {
     cb = 0;
     hist = cb + MF * kKsub * 16;        // codebook rows of 8 halves
     run = hist + kTQ * kTBins * 4;      // (query rows during set-up)
     ua = run + rc * 8;
     ub = ua + rc * 8;
-    pq = ub + rc * 8;
+    ua1 = ub + rc * 8;                  // pass-1 item prefix (items1)
+    pq = ua1 + (items1 ? rc * 4 : 0);
     wb = pq + kTPerQuery * kTQ * 4;
     stg = wb + kTNW * kTQ * 4;
     misc = stg + kTNW * kTStage * 12;
@@ -127,6 +129,7 @@ struct TileScanArgs {
   const int32_t *nprobe_q;
   int nprobe_all, kk;
   int kk1;  // sampled keys pass 1 aims at
+  int items1;  // pass 1 runs items of kTItem1 entries (their prefix in LDS)
   uint64_t *out_keys;
   uint64_t *cbuf;  // per workgroup: kTQ collect buffers of cmax words
   int cmax, bufcap, rc;
@@ -448,7 +451,8 @@ template <int MF, int PASS>
 __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc &src, const half8_t (&qf)[MF / 4],
                                            const float (&inv)[4], const int (&anc)[4], const _Float16 *cb,
                                            uint32_t *hist, const uint64_t *run, const uint64_t *ua,
-                                           const uint64_t *ub, int U, int NI, float *E_s, int *cnt_s,
+                                           const uint64_t *ub, const uint32_t *ua1, int U, int NI, float *E_s,
+                                           int *cnt_s,
                                            const int *anc_s, float *wb, TileStage st, int *itc, uint64_t *cbw) {
   constexpr bool IDS = PASS == 2;
   constexpr int STRIDE = PASS == 1 ? kTSample : 1;
@@ -461,6 +465,7 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
   int nst = 0;  // staged pass-2 keys of this wave
   int u = 0;    // union cursor: items are claimed in increasing order
   int nupd = 0;  // pass-1 items of this wave
+  const uint32_t item = ua1 ? (uint32_t)kTItem1 : (uint32_t)kTItem;
   // An item: its first entry of this lane, its end, the blocks visited, its
   // list's base-term run; past the last item, entries from 0 with no block.
   struct Item {
@@ -473,14 +478,15 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
     it = __shfl(it, 0, 64);
     Item m{it, 0, 0, 0, (uint32_t)lrow, 0u};
     if (it < NI) {
-      while (u + 1 < U && (int)(ua[u + 1] >> 32) <= it) u++;
+      // item starts: the pass-1 prefix ua1 (items of kTItem1), else ua's
+      while (u + 1 < U && (ua1 ? (int)ua1[u + 1] : (int)(ua[u + 1] >> 32)) <= it) u++;
       const uint64_t ea = ua[u], eb = ub[u];
-      const uint32_t chunk = (uint32_t)(it - (int)(ea >> 32));
+      const uint32_t chunk = (uint32_t)(it - (ua1 ? (int)ua1[u] : (int)(ea >> 32)));
       m.rs = (int)((ea >> 8) & 0xFFFFFFu);
       m.rl = (int)(ea & 0xFFu);
       const uint32_t lbeg = (uint32_t)(eb >> 32), llen = (uint32_t)eb;
-      m.ib = lbeg + chunk * kTItem + lrow;
-      m.ie = lbeg + min(llen, (chunk + 1) * (uint32_t)kTItem);
+      m.ib = lbeg + chunk * item + lrow;
+      m.ie = lbeg + min(llen, (chunk + 1) * item);
       m.nblk = (int)((m.ie - (m.ib - lrow) + 16 * STRIDE - 1) / (16 * STRIDE));  // blocks visited
     }
     return m;
@@ -523,10 +529,11 @@ __device__ __forceinline__ void tile_sweep(const TileScanArgs &a, const TileSrc 
                                    anc, ti, cbb, cb_lane, hist, st, nst, cnt_s, anc_s, cbw);
     }
     if constexpr (PASS == 1) {
-      // tighten the bounds of this wave's queries after every fourth item
-      // (counts only grow, so a bound read while others add is still an
-      // upper bound; a looser one only lets more sampled keys be counted)
-      if ((++nupd & 3) == 0)
+      // tighten the bounds of this wave's queries after every large item or
+      // every fourth small one (counts only grow, so a bound read while
+      // others add is still an upper bound; a looser one only lets more
+      // sampled keys be counted)
+      if (ua1 || (++nupd & 3) == 0)
         for (int s = wave; s < kTQ; s += kTNW) {
           int bef, cnt;
           const int bn = wave_find_bin(hist + s * kTBins, a.kk1, bef, cnt);
@@ -546,13 +553,14 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
   constexpr int D = 8 * MF;
   constexpr int T = kTQ;
   extern __shared__ __align__(16) unsigned char smem[];
-  const TileLayout lay(MF, a.rc);
+  const TileLayout lay(MF, a.rc, a.items1 != 0);
   _Float16 *cb = reinterpret_cast<_Float16 *>(smem + lay.cb);
   uint32_t *hist = reinterpret_cast<uint32_t *>(smem + lay.hist);
   float *qv = reinterpret_cast<float *>(smem + lay.hist);
   uint64_t *run = reinterpret_cast<uint64_t *>(smem + lay.run);
   uint64_t *ua = reinterpret_cast<uint64_t *>(smem + lay.ua);
   uint64_t *ub = reinterpret_cast<uint64_t *>(smem + lay.ub);
+  uint32_t *ua1 = a.items1 ? reinterpret_cast<uint32_t *>(smem + lay.ua1) : nullptr;
   int *qid_s = reinterpret_cast<int *>(smem + lay.pq);
   float *E_s = reinterpret_cast<float *>(qid_s + T);    // pass-1 running bound, then the collect bound
   int *cnt_s = reinterpret_cast<int *>(E_s + T);        // keys collected
@@ -658,6 +666,18 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
       ua[u] = ((uint64_t)(uint32_t)ip << 32) | (e & 0xFFFFFFFFull);
       ip += (int)(((uint32_t)ub[u] + kTItem - 1) / kTItem);
     }
+    // pass-1 items of kTItem1 entries (the sampled pass's per-item costs
+    // over more blocks), or the pass-2 items where their prefix does not fit
+    int NI1 = NI;
+    if (ua1) {
+      int nit1 = 0;
+      for (int u = u0; u < u1; u++) nit1 += (int)(((uint32_t)ub[u] + kTItem1 - 1) / kTItem1);
+      int ip1 = tile_excl_scan(nit1, misc + 8, NI1);
+      for (int u = u0; u < u1; u++) {
+        ua1[u] = (uint32_t)ip1;
+        ip1 += (int)(((uint32_t)ub[u] + kTItem1 - 1) / kTItem1);
+      }
+    }
     __syncthreads();  // every list id read from run[] before it is rewritten
     // base term |q - c_l|^2 of every (query, probed list), in place of the key
     for (int i = tid; i < a.rc; i += kTNT) {
@@ -708,7 +728,8 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
     __syncthreads();
 
     // ---- pass 1: histograms of a sample of the keys under running bounds
-    tile_sweep<MF, 1>(a, src, qf, inv, anc, cb, hist, run, ua, ub, U, NI, E_s, cnt_s, anc_s, wb, st, misc + 4, cbw);
+    tile_sweep<MF, 1>(a, src, qf, inv, anc, cb, hist, run, ua, ub, ua1, U, NI1, E_s, cnt_s, anc_s, wb, st, misc + 4,
+                      cbw);
     __syncthreads();
     // collect bound of each query: the top edge of the bin where its sampled
     // keys reach kk1 (+inf when fewer were sampled: every key)
@@ -723,7 +744,8 @@ __global__ __launch_bounds__(kTNT) void ivfpq_tile_scan_kernel(TileScanArgs a) {
     __syncthreads();
 
     // ---- pass 2: every key below the bound into the collect buffers, counted
-    tile_sweep<MF, 2>(a, src, qf, inv, anc, cb, hist, run, ua, ub, U, NI, E_s, cnt_s, anc_s, wb, st, misc + 4, cbw);
+    tile_sweep<MF, 2>(a, src, qf, inv, anc, cb, hist, run, ua, ub, nullptr, U, NI, E_s, cnt_s, anc_s, wb, st,
+                      misc + 4, cbw);
     __syncthreads();
 
     // ---- per query (one wavefront each): the kk smallest of the collected
@@ -866,7 +888,8 @@ bool launch_tile_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, i
   if (a.nq < min_batch) return false;
   const int T = kTQ;
   const int rc = std::max(kTNT, next_pow2(T * maxp));
-  const TileLayout lay(a.M, rc);
+  const bool items1 = TileLayout(a.M, rc, true).total <= 160 * 1024;
+  const TileLayout lay(a.M, rc, items1);
   const size_t sm = (size_t)lay.total;
   if (sm > 160 * 1024) return false;
   // query order: by the super-cluster rank of each query's nearest list;
@@ -910,6 +933,7 @@ bool launch_tile_scan(const IvfSearchArgs &a, int maxp, const int32_t *probes, i
   ta.kk = kk;
   ta.out_keys = out_keys;
   ta.kk1 = kk1;
+  ta.items1 = items1 ? 1 : 0;
   ta.cbuf = w.tile_bnd.as<uint64_t>();
   ta.cmax = cmax;
   ta.bufcap = bufcap;
