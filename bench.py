@@ -1059,8 +1059,9 @@ def bench_batch(args):
     rows), PQ 12 x 8; 50,000 queries, each with nprobe ~ U{4..128} and
     k ~ {1, 10, 100}. The whole batch is one search with a k per query: its
     scan schedule is autotuned, then it is captured in one HIP graph and
-    replayed; k_factor 1 (ADC distances) and 4 (exact FP32 rerank of each
-    query's 4k best)."""
+    replayed; plain k_factor 1 (ADC distances) and 4 (exact FP32 rerank of
+    each query's 4k best), and the mixed form on FP16 MFMA tiles at
+    k_factor 40 (SQ8 pre-rerank, exact FP32 rerank)."""
     import torch
 
     import mojovec_amd as mv
@@ -1133,7 +1134,12 @@ def bench_batch(args):
     Dd = torch.empty((nq, kmax), dtype=torch.float32, device=dev)
     Id = torch.empty((nq, kmax), dtype=torch.int64, device=dev)
     runs = {}
-    for kf in (1, 4):
+    # plain ADC (k_factor 1), plain with the exact rerank (4), and the mixed
+    # form with the FP16 tile scan at the deepest pool that k = 100 allows
+    # (k_factor 40: 4,000 candidates per query)
+    for form, kf in (("plain", 1), ("plain", 4), ("mixed16", 40)):
+        ix.set_search_mode("mixed" if form == "mixed16" else "plain")
+        ix.set_table_bits(16 if form == "mixed16" else 32)
         sched, ms = ix.autotune(qd.data_ptr(), nq, kmax, Dd.data_ptr(), Id.data_ptr(), k_factor=kf,
                                 nprobe_ptr=npd.data_ptr(), max_nprobe=128)
         graph = ix.capture(qd.data_ptr(), nq, kmax, Dd.data_ptr(), Id.data_ptr(), k_factor=kf,
@@ -1144,9 +1150,10 @@ def bench_batch(args):
         for k, sel in sample.items():
             got = Id[torch.from_numpy(sel).to(dev)][:, :min(k, 10)]
             rec[str(k)] = round(recall_at(torch, gts[k], got), 4)
-        runs[f"k_factor_{kf}"] = {"qps": round(nq / dt, 1), "ms": round(dt * 1e3, 3), "schedule": sched,
-                                  "recall_at_min_k_10": rec}
-        log(f"batch k_factor={kf}: {runs[f'k_factor_{kf}']}")
+        name = f"k_factor_{kf}" if form == "plain" else f"{form}_k_factor_{kf}"
+        runs[name] = {"qps": round(nq / dt, 1), "ms": round(dt * 1e3, 3), "schedule": sched,
+                      "recall_at_min_k_10": rec}
+        log(f"batch {name}: {runs[name]}")
     print(json.dumps({"workload": "IVF-PQ 10M x 96, nlist 16384, PQ 12x8, 50,000 queries, nprobe "
                                   "U{4..128} and k in {1,10,100} per query, one captured graph "
                                   "per batch",
