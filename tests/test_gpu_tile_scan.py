@@ -290,3 +290,52 @@ def test_tile_scan_small_batches_take_the_8bit_scan(float128):
         gi.set_table_bits(32)
         gi.set_search_mode("plain")
     assert_same(D16, I16, D8, I8)
+
+
+@pytest.mark.parametrize("kf", [6, 40])
+def test_tile_scan_per_query_k_batch_equals_per_k_searches(int128, kf):
+    # k in {1, 10, 100} and nprobe in 4..24 per query in one batch on tiles
+    # (scanned for the largest k * k_factor, each query then cut to its own
+    # k * k_factor) equals each k group searched alone; the captured graph
+    # of the batch replays the same results
+    import torch
+
+    x, q, oi, gi = int128
+    rng = np.random.default_rng(770 + kf)
+    kq = rng.choice([1, 10, 100], q.shape[0]).astype(np.int32)
+    npq = rng.integers(4, 25, q.shape[0]).astype(np.int32)
+    kmax = int(kq.max())
+    gi.set_search_mode("mixed")
+    gi.set_table_bits(16)
+    gi.set_tile_min_batch(1)
+    try:
+        De = np.full((q.shape[0], kmax), np.float32(1e38))
+        Ie = np.full((q.shape[0], kmax), -1, np.int64)
+        for kv in np.unique(kq):
+            sel = kq == kv
+            Dk, Ik = gi.search(q[sel], int(kv), k_factor=kf, nprobe_per_query=npq[sel])
+            De[sel, :kv], Ie[sel, :kv] = Dk, Ik
+        dev = torch.device("cuda:0")
+        qd = torch.from_numpy(q).to(dev)
+        kd = torch.from_numpy(kq).to(dev)
+        nd = torch.from_numpy(npq).to(dev)
+        D = torch.empty((q.shape[0], kmax), dtype=torch.float32, device=dev)
+        I = torch.empty((q.shape[0], kmax), dtype=torch.int64, device=dev)
+        gi.search_device(qd.data_ptr(), q.shape[0], kmax, D.data_ptr(), I.data_ptr(), k_factor=kf,
+                         nprobe_ptr=nd.data_ptr(), max_nprobe=24, kq_ptr=kd.data_ptr())
+        torch.cuda.synchronize()
+        assert_same(D.cpu().numpy(), I.cpu().numpy(), De, Ie)
+        g = gi.capture(qd.data_ptr(), q.shape[0], kmax, D.data_ptr(), I.data_ptr(), k_factor=kf,
+                       nprobe_ptr=nd.data_ptr(), max_nprobe=24, kq_ptr=kd.data_ptr())
+        for rep in range(2):
+            D.fill_(0.0)
+            I.fill_(0)
+            torch.cuda.synchronize()
+            g.launch()
+            torch.cuda.synchronize()
+            assert_same(D.cpu().numpy(), I.cpu().numpy(), De, Ie)
+        del g
+    finally:
+        gi.set_tile_min_batch(0)
+        gi.set_table_bits(32)
+        gi.set_search_mode("plain")
