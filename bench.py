@@ -49,7 +49,7 @@ SCAN_TRAFFIC_PROFILED = {"at": ("mixed8", 64, 288), "bytes": 5_464_962 * 1024 * 
 # MI355X's dense FP16 MFMA peak. Its HBM bytes per launch come from the same
 # two counter passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000).
 FP16_MFMA_PEAK_TFS = 2500.0
-TILE_TRAFFIC_PROFILED = {"at": ("mixed16", 64, 288), "bytes": 547_187 * 1024 * 2 + 532_307 * 1024,
+TILE_TRAFFIC_PROFILED = {"at": ("mixed16", 64, 288), "bytes": 548_460 * 1024 * 2 + 532_474 * 1024,
                          "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, "
                                    "profiles/ivfpq_tile_scan_1Mx128.md"}
 
