@@ -275,16 +275,16 @@ int mvg_flat_search(mvg_flat *h, const float *q, int64_t nq, int k, const uint8_
     const int kf = h->k_factor;
     const int kk = k * kf;
     if (kk > mvg::kMaxCandidates) throw Error(mvg::kInvalid, "k * k_factor exceeds the device limit of 4096");
-    int nsplit = 0;
+    int64_t ncand = 0;
     mvg::gemm_topk(kind, qrows, xrows, nq, h->ntotal, h->d, h->qnorm.as<float>(), xnorm, nullptr,
-                   nullptr, nullptr, nullptr, 0.0f, kk, dx, dx ? exclude_len : 0, h->part, nsplit,
+                   nullptr, nullptr, nullptr, 0.0f, kk, dx, dx ? exclude_len : 0, h->part, ncand,
                    h->s, qinv, xinv);
     if (kf == 1) {
-      mvg::launch_merge(h->part.as<uint64_t>(), nq, (int64_t)nsplit * kk, k, ip ? 1 : 0, 0.0f,
+      mvg::launch_merge(h->part.as<uint64_t>(), nq, ncand, k, ip ? 1 : 0, 0.0f,
                         h->D.as<float>(), h->I.as<int64_t>(), nullptr, nullptr, h->s);
     } else {
       h->keys.reserve(sizeof(uint64_t) * nq * kk);
-      mvg::launch_merge(h->part.as<uint64_t>(), nq, (int64_t)nsplit * kk, kk, 0, 0.0f, nullptr,
+      mvg::launch_merge(h->part.as<uint64_t>(), nq, ncand, kk, 0, 0.0f, nullptr,
                         nullptr, nullptr, h->keys.as<uint64_t>(), h->s);
       mvg::rerank(dq, nq, h->d, h->metric, h->base.as<float>(), h->ntotal, h->keys.as<uint64_t>(), kk, k, 0.0f,
                   h->D.as<float>(), h->I.as<int64_t>(), h->s);

@@ -115,6 +115,28 @@ __device__ __forceinline__ bool xcd_tile(int64_t nq, int rows, int S, int64_t &q
 // Grid size for xcd_tile: qtiles * S rounded up to a multiple of 8.
 inline unsigned xcd_grid(int64_t qtiles, int64_t S) { return (unsigned)(8 * ((qtiles * S + 7) / 8)); }
 
+// Rows of one base split in the wide forms. P == 1: split s is the contiguous
+// range [s * split_len, (s + 1) * split_len). P > 1 (strided splits): split s
+// is residue r = s % P of chunk c = s / P, i.e. the rows r, r + P, r + 2 P, ...
+// whose local index j = (row - r) / P lies in [c * split_len, (c + 1) *
+// split_len). Local row j is base row goff + j * P.
+struct SplitRows {
+  int64_t goff, b_begin, b_end;
+  __device__ __forceinline__ SplitRows(int64_t n, int split, int P, int64_t split_len) {
+    const int r = split % P;
+    goff = r;
+    b_begin = (int64_t)(split / P) * split_len;
+    b_end = min((n - r + P - 1) / P, b_begin + split_len);
+  }
+};
+
+// A split without rows (a residue's last chunk can be one): its queries' top
+// lists are empty.
+__device__ __forceinline__ void empty_split(uint64_t *part, int64_t q0, int qcount, int S, int split, int kk) {
+  for (int i = threadIdx.x; i < qcount * kk; i += kNT)
+    part[((q0 + i / kk) * S + split) * (int64_t)kk + i % kk] = kSentinel64;
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
     const uint8_t *__restrict__ Q, const uint8_t *__restrict__ X, int64_t nq, int64_t n,
@@ -123,7 +145,7 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
     const uint32_t *__restrict__ xn_u, const uint32_t *__restrict__ xs_u, float scale_sq, int kk,
     int64_t split_len, const uint8_t *__restrict__ excl, int64_t excl_len,
     uint64_t *__restrict__ part, int S, const float *__restrict__ qinv,
-    const float *__restrict__ xinv) {
+    const float *__restrict__ xinv, const uint8_t *__restrict__ only) {
   extern __shared__ __align__(16) unsigned char smem[];
   const GemmLayout lay(row_bytes, kk);
   uint8_t *qtile = smem + lay.qtile;
@@ -142,6 +164,11 @@ __global__ __launch_bounds__(kNT) void gemm_topk_kernel(
   const int64_t b_begin = (int64_t)split * split_len;
   const int64_t b_end = min(n, b_begin + split_len);
   const int qcount = (int)min<int64_t>(kQT, nq - q0);
+  if (only) {  // only the tiles that hold a flagged query
+    bool any = false;
+    for (int qi = 0; qi < qcount; qi++) any |= only[q0 + qi] != 0;
+    if (!any) return;
+  }
 
   // Query tile -> LDS (zero rows past nq).
   const int row_words = row_bytes / 16;
@@ -332,7 +359,7 @@ __device__ __forceinline__ void wide_flush(uint64_t *top, uint64_t *buf, int *cn
 // integer kinds test candidates with an exact integer prefilter derived from
 // the running kk-th distance (a superset: the rounding of the FP32 distance
 // is covered by a margin), then the exact key on the rare pass.
-template <int KIND>
+template <int KIND, bool STR>
 struct WideSel {
   float thr_f[4], qa[4], qb[4];
   uint64_t thr_k[4];  // each query's kk-th key (the sentinel while fewer): changes at merges only
@@ -340,6 +367,9 @@ struct WideSel {
   int ithr[4];
   bool qok[4];
   int cnt_r[4];
+  // STR (strided splits): local column j of the split is row goff + j *
+  // stride of the base; otherwise the local column is the row
+  int64_t goff = 0, stride = 1;
 
   __device__ __forceinline__ void init(int64_t q0, int wq, int qcount, int lgrp, const float *qnorm,
                                        const float *qinv, const uint32_t *qn_u, const uint32_t *qs_u,
@@ -405,7 +435,7 @@ struct WideSel {
 #pragma unroll
     for (int u = 0; u < NT; u++) {
       const int t = T0 + u;
-      const int64_t col = pb0 + t * 16 + lrow;
+      const int64_t col = STR ? goff + (pb0 + t * 16 + lrow) * stride : pb0 + t * 16 + lrow;
       const bool col_ok = t * 16 + lrow <= lim && !(excl && col < excl_len && excl[col] > 0);
       float xa = 0.0f, xb = 0.0f;
       int xcorr = 0;
@@ -445,8 +475,9 @@ struct WideSel {
 #pragma unroll
       for (int j = 0; j < 4; j++) any |= pass[u][j];
       if (__ballot(any) == 0) continue;
-      const int64_t col = pb0 + t * 16 + lrow;
-      const int64_t cc = pass[u][0] || pass[u][1] || pass[u][2] || pass[u][3] ? col : b_begin;
+      const int64_t col = STR ? goff + (pb0 + t * 16 + lrow) * stride : pb0 + t * 16 + lrow;
+      const int64_t cc =
+          pass[u][0] || pass[u][1] || pass[u][2] || pass[u][3] ? col : (STR ? goff + b_begin * stride : b_begin);
       const uint64_t gmask = 0xFFFFull << (16 * lgrp);
       const uint64_t below = (1ull << lane) - 1;
 #pragma unroll
@@ -523,7 +554,7 @@ struct WideSel {
   }
 };
 
-template <int KIND>
+template <int KIND, bool STR>
 __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
     const uint8_t *__restrict__ Q, const uint8_t *__restrict__ X, int64_t nq, int64_t n,
     int row_bytes, int d_true, const float *__restrict__ qnorm, const float *__restrict__ xnorm,
@@ -531,7 +562,8 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
     const uint32_t *__restrict__ xn_u, const uint32_t *__restrict__ xs_u, float scale_sq, int kk,
     int64_t split_len, const uint8_t *__restrict__ excl, int64_t excl_len,
     uint64_t *__restrict__ part, int S, const float *__restrict__ qinv,
-    const float *__restrict__ xinv) {
+    const float *__restrict__ xinv, int P) {
+  if (!STR) P = 1;  // contiguous splits: the row arithmetic folds away
   extern __shared__ __align__(16) unsigned char smem[];
   const WideLayout lay(row_bytes, kk);
   uint8_t *atile = smem + lay.a;
@@ -547,10 +579,16 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
   int split;
   if (!xcd_tile(nq, kWQ, S, qt, split)) return;
   const int64_t q0 = qt * kWQ;
-  const int64_t b_begin = (int64_t)split * split_len;
-  const int64_t b_end = min(n, b_begin + split_len);
+  const SplitRows sr(n, split, P, split_len);
+  const int64_t b_begin = sr.b_begin, b_end = sr.b_end;
   const int qcount = (int)min<int64_t>(kWQ, nq - q0);
+  if (b_end <= b_begin) {
+    empty_split(part, q0, qcount, S, split, kk);
+    return;
+  }
   const int wq = wave * 16;  // first query row of this wavefront in the tile
+  const uint8_t *Xs = X + sr.goff * row_bytes;    // local row 0 of the split
+  const int64_t pitch = (int64_t)P * row_bytes;   // bytes between local rows
 
   const int row_words = row_bytes / 16;
   for (int i = tid; i < kWQ * row_words; i += kNT) {
@@ -565,8 +603,10 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
   for (int i = lane; i < 16 * kk; i += 64) top[i] = kSentinel64;
   if (lane < 16) cnt[lane] = 0;
 
-  WideSel<KIND> sel;
+  WideSel<KIND, STR> sel;
   sel.init(q0, wq, qcount, lgrp, qnorm, qinv, qn_u, qs_u, d_true);
+  sel.goff = sr.goff;
+  sel.stride = P;
 
   const int nks = row_bytes / 64;    // 64-byte k steps of a row
   const int nsl = (nks + 1) >> 1;    // 128-byte slices per step
@@ -578,17 +618,17 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
 #define MVG_LOAD_SLICE(B0, SL)                                                          \
   {                                                                                     \
     const int ko = min((SL) * 128 + sgrp * 16, row_bytes - 16);                          \
-    const uint8_t *src = X + ko;                                                         \
+    const uint8_t *src = Xs + ko;                                                        \
     const int64_t lastc = b_end - 1;                                                     \
     const int64_t c0 = (B0) + srow;                                                      \
-    p0 = *reinterpret_cast<const uint4 *>(src + min(c0, lastc) * row_bytes);             \
-    p1 = *reinterpret_cast<const uint4 *>(src + min(c0 + 32, lastc) * row_bytes);        \
-    p2 = *reinterpret_cast<const uint4 *>(src + min(c0 + 64, lastc) * row_bytes);        \
-    p3 = *reinterpret_cast<const uint4 *>(src + min(c0 + 96, lastc) * row_bytes);        \
-    p4 = *reinterpret_cast<const uint4 *>(src + min(c0 + 128, lastc) * row_bytes);       \
-    p5 = *reinterpret_cast<const uint4 *>(src + min(c0 + 160, lastc) * row_bytes);       \
-    p6 = *reinterpret_cast<const uint4 *>(src + min(c0 + 192, lastc) * row_bytes);       \
-    p7 = *reinterpret_cast<const uint4 *>(src + min(c0 + 224, lastc) * row_bytes);       \
+    p0 = *reinterpret_cast<const uint4 *>(src + min(c0, lastc) * pitch);                 \
+    p1 = *reinterpret_cast<const uint4 *>(src + min(c0 + 32, lastc) * pitch);            \
+    p2 = *reinterpret_cast<const uint4 *>(src + min(c0 + 64, lastc) * pitch);            \
+    p3 = *reinterpret_cast<const uint4 *>(src + min(c0 + 96, lastc) * pitch);            \
+    p4 = *reinterpret_cast<const uint4 *>(src + min(c0 + 128, lastc) * pitch);           \
+    p5 = *reinterpret_cast<const uint4 *>(src + min(c0 + 160, lastc) * pitch);           \
+    p6 = *reinterpret_cast<const uint4 *>(src + min(c0 + 192, lastc) * pitch);           \
+    p7 = *reinterpret_cast<const uint4 *>(src + min(c0 + 224, lastc) * pitch);           \
   }
 #define MVG_STORE_SLICE(SB)                                                              \
   {                                                                                      \
@@ -626,11 +666,12 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
     const int64_t c = (B0) + tid;                                                            \
     cterm = make_uint2(0, 0);                                                                \
     if (c < b_end) {                                                                         \
+      const int64_t g = sr.goff + c * P;                                                     \
       if (KIND == U8_L2)                                                                     \
-        cterm.x = (uint32_t)((int)xn_u[c] - 256 * ((int)xs_u[c] - 128 * d_true));           \
+        cterm.x = (uint32_t)((int)xn_u[g] - 256 * ((int)xs_u[g] - 128 * d_true));           \
       else                                                                                   \
-        cterm.x = __float_as_uint(xnorm[c]);                                                 \
-      if (KIND == FP8_L2 || KIND == FP8_IP) cterm.y = __float_as_uint(xinv[c]);              \
+        cterm.x = __float_as_uint(xnorm[g]);                                                 \
+      if (KIND == FP8_L2 || KIND == FP8_IP) cterm.y = __float_as_uint(xinv[g]);              \
     }                                                                                        \
   }
   int sp = 0;  // buffer of the current step's terms (0..2)
@@ -753,7 +794,7 @@ __device__ __forceinline__ void static_for(F &&f) {
   static_for_seq(f, std::make_integer_sequence<int, N>{});
 }
 
-template <int KIND, int NKS>
+template <int KIND, int NKS, bool STR>
 __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
     const uint8_t *__restrict__ Q, const uint8_t *__restrict__ X, int64_t nq, int64_t n,
     int d_true, const float *__restrict__ qnorm, const float *__restrict__ xnorm,
@@ -761,7 +802,8 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
     const uint32_t *__restrict__ xn_u, const uint32_t *__restrict__ xs_u, float scale_sq, int kk,
     int64_t split_len, const uint8_t *__restrict__ excl, int64_t excl_len,
     uint64_t *__restrict__ part, int S, const float *__restrict__ qinv,
-    const float *__restrict__ xinv) {
+    const float *__restrict__ xinv, int P) {
+  if (!STR) P = 1;  // contiguous splits: the row arithmetic folds away
   static_assert(NKS % 2 == 0, "whole 128-byte slices");
   constexpr int NSL = NKS / 2;                                    // slices per row
   constexpr bool F8 = KIND == FP8_L2 || KIND == FP8_IP;
@@ -788,10 +830,15 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
   int split;
   if (!xcd_tile(nq, kWQ, S, qt, split)) return;
   const int64_t q0 = qt * kWQ;
-  const int64_t b_begin = (int64_t)split * split_len;
-  const int64_t b_end = min(n, b_begin + split_len);
+  const SplitRows sr(n, split, P, split_len);
+  const int64_t b_begin = sr.b_begin, b_end = sr.b_end;
   const int qcount = (int)min<int64_t>(kWQ, nq - q0);
+  if (b_end <= b_begin) {
+    empty_split(part, q0, qcount, S, split, kk);
+    return;
+  }
   const int wq = wave * 16;
+  const int pitch = P * ROW;  // bytes between local rows (the host keeps a split's span below 2 GiB)
 
   // the lane's query row (wq + lrow): fragment h of slice sl at areg[2 sl + h]
   // (FP8: the 32 bytes of slice sl at a8[sl])
@@ -821,21 +868,30 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
   }
   for (int i = lane; i < 16 * kk; i += 64) top[i] = kSentinel64;
   if (lane < 16) cnt[lane] = 0;
-  WideSel<KIND> sel;
+  WideSel<KIND, STR> sel;
   sel.init(q0, wq, qcount, lgrp, qnorm, qinv, qn_u, qs_u, d_true);
+  sel.goff = sr.goff;
+  sel.stride = P;
 
   const int srow = tid >> 3, sgrp = tid & 7;  // staging: 8 columns x 16 bytes per thread
   const int sswz = wide_swz(sgrp);
   uint4 pr[DEPTH][8];
-  // the split's rows [b_begin, b_end) as one buffer (the host keeps a split
-  // below 2 GiB); flags word: raw byte addressing, as for gfx9 buffers
+  // the split's local rows [b_begin, b_end) as one buffer that ends with the
+  // last one's bytes; flags word: raw byte addressing, as for gfx9 buffers.
+  // Rows past the split's end read as zeros: strided splits put the whole
+  // offset in the vector operand, which the range check covers.
   const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
-      (void *)(X + b_begin * ROW), (short)0, (int)((b_end - b_begin) * ROW), 0x00020000);
+      (void *)(X + (sr.goff + b_begin * P) * ROW), (short)0,
+      (int)(((b_end - b_begin - 1) * P + 1) * ROW), 0x00020000);
   auto load_slice = [&](uint4(&p)[8], int64_t B0, int SL) __attribute__((always_inline)) {
-    const int vo = (int)(B0 - b_begin + srow) * ROW + SL * 128 + sgrp * 16;
+    const int vo = (int)(B0 - b_begin + srow) * pitch + SL * 128 + sgrp * 16;
 #pragma unroll
-    for (int i = 0; i < 8; i++)
-      p[i] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xrs, vo, i * 32 * ROW, 0));
+    for (int i = 0; i < 8; i++) {
+      if constexpr (STR)
+        p[i] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xrs, vo + i * 32 * pitch, 0, 0));
+      else
+        p[i] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xrs, vo, i * 32 * ROW, 0));
+    }
   };
   auto store_slice = [&](uint4(&p)[8], int SB) __attribute__((always_inline)) {
     uint8_t *plane = bst + SB * kBSlice + sgrp * kBPlane;
@@ -854,7 +910,7 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
   // the epilogue drops those columns)
   auto load_terms = [&](int64_t B0) __attribute__((always_inline)) {
     if (kColTerms) {
-      const int64_t c = min(B0 + tid, b_end - 1);
+      const int64_t c = sr.goff + min(B0 + tid, b_end - 1) * P;
       if (KIND == U8_L2)
         cterm.x = (uint32_t)((int)xn_u[c] - 256 * ((int)xs_u[c] - 128 * d_true));
       else
@@ -1092,95 +1148,112 @@ void to_bf16(const float *x, int64_t n, int d, uint16_t *out, float *norms, hipS
   MVG_LAUNCH_CHECK("to_bf16_kernel");
 }
 
-void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t n, int d,
-               const float *qnorm, const float *xnorm, const uint32_t *qn_u, const uint32_t *qs_u,
-               const uint32_t *xn_u, const uint32_t *xs_u, float scale_sq, int kk, const uint8_t *excl, int64_t excl_len, DevBuf &part,
-               int &nsplit, hipStream_t s, const float *qinv, const float *xinv) {
+namespace {
+
+// Strided splits keep each split's exact top kper (sorted, the sentinel past
+// a split's last row). Let M be the smallest kper-th key over the splits
+// whose list is full (the sentinel if none is). Every split's list holds all
+// of that split's keys up to M -- a full list ends at or above M, and a list
+// that is not full holds all its split's keys -- so the keys <= M in the
+// union are exactly the base's keys <= M. If there are at least kk of them,
+// the union's kk smallest are the base's; otherwise (a split held more than
+// kper of the query's kk nearest rows) flag[q] = 1 and the query is searched
+// again. One wavefront per query.
+__global__ void gemm_certify_kernel(const uint64_t *__restrict__ part, int64_t nq, int S, int kper, int kk,
+                                    uint8_t *__restrict__ flag) {
+  const int lane = threadIdx.x & 63;
+  const int64_t q = (int64_t)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6);
+  if (q >= nq) return;
+  const uint64_t *row = part + q * (int64_t)S * kper;
+  uint64_t m = kSentinel64;
+  for (int s = lane; s < S; s += 64) m = min(m, row[(int64_t)s * kper + kper - 1]);
+#pragma unroll
+  for (int j = 1; j < 64; j <<= 1) m = min(m, xor_lane64(m, j));
+  int c = 0;
+  if (m != kSentinel64) {
+    for (int s = lane; s < S; s += 64) {
+      const uint64_t *l = row + (int64_t)s * kper;
+      int lo = 0, hi = kper;  // first index whose key exceeds m
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (l[mid] <= m) lo = mid + 1;
+        else hi = mid;
+      }
+      c += lo;
+    }
+#pragma unroll
+    for (int j = 1; j < 64; j <<= 1) c += __shfl_xor(c, j);
+  }
+  if (lane == 0) flag[q] = m != kSentinel64 && c < kk;
+}
+
+// A flagged query's candidate row (ncand keys) becomes its exact top kk from
+// the 16-query search (redo, kk keys per query) followed by sentinels.
+__global__ void gemm_fixup_kernel(uint64_t *__restrict__ part, int64_t ncand,
+                                  const uint64_t *__restrict__ redo, int kk,
+                                  const uint8_t *__restrict__ flag) {
+  const int64_t q = blockIdx.x;
+  if (!flag[q]) return;
+  uint64_t *dst = part + q * ncand;
+  for (int64_t i = threadIdx.x; i < ncand; i += kNT) dst[i] = i < kk ? redo[q * kk + i] : kSentinel64;
+}
+
+}  // namespace
+
+GemmPlan gemm_plan(int kind, int64_t nq, int64_t n, int d, int kk) {
+  GemmPlan pl{};
   const int row_bytes = gemm_row_bytes(kind, d);
-  const WideLayout wlay(row_bytes, kk);
+  const GemmLayout glay(row_bytes, kk);
+  // Above 64 candidates the wide forms run on P strided splits (rows r, r +
+  // P, ...): each keeps its exact top kw = 64, the union is certified per
+  // query, and the 16-query form searches the rejected queries again (its
+  // layout must fit). With kk / P <= 24 of a query's kk nearest rows
+  // expected per split, more than 64 in one is far in the tail unless the
+  // rows are laid out with period P.
+  const bool strided = kk > kWMaxK;
+  const int kw = strided ? kWMaxK : kk;
+  const int P = strided ? std::max(2, ceil_div(kk, 24)) : 1;
+  const WideLayout wlay(row_bytes, kw);
   // the register-resident query tile for the 768-byte int8 / FP8 rows and the
   // 1,536-byte BF16 rows (d = 768)
   const bool reg12 = row_bytes == 768 && (kind == I8_IP || kind == U8_L2 || kind == FP8_L2 || kind == FP8_IP);
   const bool reg24 = row_bytes == 1536 && (kind == BF16_L2 || kind == BF16_IP);
-  const WideRegLayout rlay(kk);
+  const WideRegLayout rlay(kw);
   const bool reg = (reg12 || reg24) && rlay.total <= 160 * 1024;
-  if (kk <= kWMaxK && nq > kQT && (reg || wlay.total <= 160 * 1024)) {
+  if (nq > kQT && (reg || wlay.total <= 160 * 1024) &&
+      (!strided || (glay.total <= 150 * 1024 && n >= (int64_t)P * kStepCols))) {
     const int qtiles = ceil_div(nq, kWQ);
-    // Splits of the base per query tile: one workgroup per CU at a time, so a
-    // CU streams ceil(qtiles * S / cus) / S of the base; every split costs a
-    // selection warm-up and kk more candidates to merge, so take the fewest
-    // splits within 10% of the best streaming balance.
+    const int64_t nloc = ceil_div(n, P);  // local rows of the longest residue
+    // Chunks per residue (the splits, when P = 1): one workgroup per CU at a
+    // time, so a CU streams ceil(qtiles * P * C / cus) / (P * C) of the base;
+    // every split costs a selection warm-up and kw more candidates to merge,
+    // so take the fewest chunks within 10% of the best streaming balance.
     const int cus = std::max(1, device_cus());
-    const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(n, kStepCols)));
+    const int64_t cmax = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(nloc, kStepCols)));
     double best = 1e30;
-    for (int64_t c = 1; c <= smax; c++) best = std::min(best, (double)ceil_div(qtiles * c, cus) / c);
-    int64_t S = 1;
-    while (S < smax && (double)ceil_div(qtiles * S, cus) / S > 1.1 * best) S++;
+    for (int64_t c = 1; c <= cmax; c++) best = std::min(best, (double)ceil_div(qtiles * P * c, cus) / c);
+    int64_t C = 1;
+    while (C < cmax && (double)ceil_div(qtiles * P * C, cus) / C > 1.1 * best) C++;
     // the register form addresses a split's rows with 32-bit buffer offsets:
-    // at most 2 GiB per split (a step past its end included)
-    if (reg) S = std::max<int64_t>(S, ceil_div(n * (int64_t)row_bytes, (int64_t(1) << 31) - 512 * (int64_t)row_bytes));
-    int64_t split_len = ((ceil_div(n, S) + kStepCols - 1) / kStepCols) * kStepCols;
-    S = ceil_div(n, split_len);
-    nsplit = (int)S;
-    part.reserve(sizeof(uint64_t) * (size_t)nq * S * kk);
-    const dim3 grid(xcd_grid(qtiles, S));
-    if (reg) {
-      const void *rf;
-      switch (kind) {
-        case BF16_L2: rf = (const void *)gemm_wide_reg_kernel<BF16_L2, 24>; break;
-        case BF16_IP: rf = (const void *)gemm_wide_reg_kernel<BF16_IP, 24>; break;
-        case I8_IP: rf = (const void *)gemm_wide_reg_kernel<I8_IP, 12>; break;
-        case FP8_L2: rf = (const void *)gemm_wide_reg_kernel<FP8_L2, 12>; break;
-        case FP8_IP: rf = (const void *)gemm_wide_reg_kernel<FP8_IP, 12>; break;
-        default: rf = (const void *)gemm_wide_reg_kernel<U8_L2, 12>; break;
-      }
-      if (rlay.total > 64 * 1024)
-        MVG_HIP(hipFuncSetAttribute(rf, hipFuncAttributeMaxDynamicSharedMemorySize, rlay.total));
-#define MVG_WREG(K, N)                                                                             \
-  hipLaunchKernelGGL((gemm_wide_reg_kernel<K, N>), grid, dim3(kNT), rlay.total, s, Q, X, nq, n, d,   \
-                     qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len, excl, excl_len, \
-                     part.as<uint64_t>(), (int)S, qinv, xinv);
-      switch (kind) {
-        case BF16_L2: MVG_WREG(BF16_L2, 24) break;
-        case BF16_IP: MVG_WREG(BF16_IP, 24) break;
-        case I8_IP: MVG_WREG(I8_IP, 12) break;
-        case FP8_L2: MVG_WREG(FP8_L2, 12) break;
-        case FP8_IP: MVG_WREG(FP8_IP, 12) break;
-        default: MVG_WREG(U8_L2, 12) break;
-      }
-#undef MVG_WREG
-      MVG_LAUNCH_CHECK("gemm_wide_reg_kernel");
-      return;
-    }
-    const void *fn;
-    switch (kind) {
-      case BF16_L2: fn = (const void *)gemm_wide_kernel<BF16_L2>; break;
-      case BF16_IP: fn = (const void *)gemm_wide_kernel<BF16_IP>; break;
-      case I8_IP: fn = (const void *)gemm_wide_kernel<I8_IP>; break;
-      case FP8_L2: fn = (const void *)gemm_wide_kernel<FP8_L2>; break;
-      case FP8_IP: fn = (const void *)gemm_wide_kernel<FP8_IP>; break;
-      default: fn = (const void *)gemm_wide_kernel<U8_L2>; break;
-    }
-    if (wlay.total > 64 * 1024)
-      MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, wlay.total));
-#define MVG_WIDE(K)                                                                               \
-  hipLaunchKernelGGL(gemm_wide_kernel<K>, grid, dim3(kNT), wlay.total, s, Q, X, nq, n, row_bytes, d, \
-                     qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len, excl,        \
-                     excl_len, part.as<uint64_t>(), (int)S, qinv, xinv);
-    switch (kind) {
-      case BF16_L2: MVG_WIDE(BF16_L2) break;
-      case BF16_IP: MVG_WIDE(BF16_IP) break;
-      case I8_IP: MVG_WIDE(I8_IP) break;
-      case FP8_L2: MVG_WIDE(FP8_L2) break;
-      case FP8_IP: MVG_WIDE(FP8_IP) break;
-      default: MVG_WIDE(U8_L2) break;
-    }
-#undef MVG_WIDE
-    MVG_LAUNCH_CHECK("gemm_wide_kernel");
-    return;
+    // its span, a step past its end included, stays below 2 GiB
+    const int64_t prow = (int64_t)P * row_bytes;
+    if (reg) C = std::max<int64_t>(C, ceil_div(nloc * prow, (int64_t(1) << 31) - 512 * prow));
+    pl.split_len = ((ceil_div(nloc, C) + kStepCols - 1) / kStepCols) * kStepCols;
+    C = ceil_div(nloc, pl.split_len);
+    pl.wide = true;
+    pl.reg = reg;
+    pl.strided = strided;
+    pl.P = P;
+    pl.kw = kw;
+    pl.S = C * P;
+    pl.ncand = pl.S * kw;
+    // [nq][S][kw] split lists, then (strided) the 16-query form's [nq][kk]
+    // and one flag byte per query
+    pl.bytes = sizeof(uint64_t) * (size_t)nq * pl.S * kw +
+               (strided ? sizeof(uint64_t) * (size_t)nq * kk + (size_t)nq : 0);
+    return pl;
   }
-  const GemmLayout lay(row_bytes, kk);
-  if (lay.total > 150 * 1024) throw Error(kInvalid, "GEMM top-k workspace exceeds LDS");
+  if (glay.total > 150 * 1024) throw Error(kInvalid, "GEMM top-k workspace exceeds LDS");
   const int qtiles = ceil_div(nq, kQT);
   // Splits of the base per query tile. Up to 64 candidates per query, a grid
   // of ~2048 workgroups hides the row loads best. Above that every split
@@ -1190,7 +1263,7 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
   int64_t S = std::max<int64_t>(1, std::min<int64_t>(ceil_div(2048, qtiles), ceil_div(n, kStepCols)));
   if (kk > kWMaxK) {
     const int cus = std::max(1, device_cus());
-    const int per_cu = std::max(1, std::min(8, (160 * 1024) / std::max(1, lay.total)));
+    const int per_cu = std::max(1, std::min(8, (160 * 1024) / std::max(1, glay.total)));
     const int64_t slots = (int64_t)cus * per_cu;
     const int64_t smax = std::max<int64_t>(1, std::min<int64_t>(256, ceil_div(n, kStepCols)));
     double best = 1e30;
@@ -1198,37 +1271,124 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
     S = 1;
     while (S < smax && (double)ceil_div(qtiles * S, slots) / S > 1.1 * best) S++;
   }
-  int64_t split_len = ((ceil_div(n, S) + kStepCols - 1) / kStepCols) * kStepCols;
-  S = ceil_div(n, split_len);
-  nsplit = (int)S;
-  part.reserve(sizeof(uint64_t) * (size_t)nq * S * kk);
-  const void *fn;
+  pl.split_len = ((ceil_div(n, S) + kStepCols - 1) / kStepCols) * kStepCols;
+  pl.S = ceil_div(n, pl.split_len);
+  pl.P = 1;
+  pl.kw = kk;
+  pl.ncand = pl.S * kk;
+  pl.bytes = sizeof(uint64_t) * (size_t)nq * pl.S * kk;
+  return pl;
+}
+
+void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t n, int d,
+               const float *qnorm, const float *xnorm, const uint32_t *qn_u, const uint32_t *qs_u,
+               const uint32_t *xn_u, const uint32_t *xs_u, float scale_sq, int kk, const uint8_t *excl,
+               int64_t excl_len, DevBuf &part, int64_t &ncand, hipStream_t s, const float *qinv,
+               const float *xinv) {
+  const GemmPlan pl = gemm_plan(kind, nq, n, d, kk);
+  const int row_bytes = gemm_row_bytes(kind, d);
+  const GemmLayout glay(row_bytes, kk);
+  ncand = pl.ncand;
+  part.reserve(pl.bytes);
+  uint64_t *pk = part.as<uint64_t>();
+  const void *fn16;
   switch (kind) {
-    case BF16_L2: fn = (const void *)gemm_topk_kernel<BF16_L2>; break;
-    case BF16_IP: fn = (const void *)gemm_topk_kernel<BF16_IP>; break;
-    case I8_IP: fn = (const void *)gemm_topk_kernel<I8_IP>; break;
-    case FP8_L2: fn = (const void *)gemm_topk_kernel<FP8_L2>; break;
-    case FP8_IP: fn = (const void *)gemm_topk_kernel<FP8_IP>; break;
-    default: fn = (const void *)gemm_topk_kernel<U8_L2>; break;
+    case BF16_L2: fn16 = (const void *)gemm_topk_kernel<BF16_L2>; break;
+    case BF16_IP: fn16 = (const void *)gemm_topk_kernel<BF16_IP>; break;
+    case I8_IP: fn16 = (const void *)gemm_topk_kernel<I8_IP>; break;
+    case FP8_L2: fn16 = (const void *)gemm_topk_kernel<FP8_L2>; break;
+    case FP8_IP: fn16 = (const void *)gemm_topk_kernel<FP8_IP>; break;
+    default: fn16 = (const void *)gemm_topk_kernel<U8_L2>; break;
   }
-  if (lay.total > 64 * 1024)
-    MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lay.total));
-  const dim3 grid(xcd_grid(qtiles, S));
-#define MVG_GEMM(K)                                                                               \
-  hipLaunchKernelGGL(gemm_topk_kernel<K>, grid, dim3(kNT), lay.total, s, Q, X, nq, n, row_bytes, d, \
-                     qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len, excl,        \
-                     excl_len,                                                                     \
-                     part.as<uint64_t>(), (int)S, qinv, xinv);
-  switch (kind) {
-    case BF16_L2: MVG_GEMM(BF16_L2) break;
-    case BF16_IP: MVG_GEMM(BF16_IP) break;
-    case I8_IP: MVG_GEMM(I8_IP) break;
-    case FP8_L2: MVG_GEMM(FP8_L2) break;
-    case FP8_IP: MVG_GEMM(FP8_IP) break;
-    default: MVG_GEMM(U8_L2) break;
-  }
+  // the 16-query form over S16 contiguous splits into out[nq][S16][kk]
+  // (only: the tiles holding a flagged query)
+  auto run16 = [&](int64_t S16, int64_t split_len16, uint64_t *out, const uint8_t *only) {
+    if (glay.total > 64 * 1024)
+      MVG_HIP(hipFuncSetAttribute(fn16, hipFuncAttributeMaxDynamicSharedMemorySize, glay.total));
+    const dim3 grid(xcd_grid(ceil_div(nq, kQT), S16));
+#define MVG_GEMM(K)                                                                                \
+  hipLaunchKernelGGL(gemm_topk_kernel<K>, grid, dim3(kNT), glay.total, s, Q, X, nq, n, row_bytes, d, \
+                     qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kk, split_len16, excl,       \
+                     excl_len, out, (int)S16, qinv, xinv, only);
+    switch (kind) {
+      case BF16_L2: MVG_GEMM(BF16_L2) break;
+      case BF16_IP: MVG_GEMM(BF16_IP) break;
+      case I8_IP: MVG_GEMM(I8_IP) break;
+      case FP8_L2: MVG_GEMM(FP8_L2) break;
+      case FP8_IP: MVG_GEMM(FP8_IP) break;
+      default: MVG_GEMM(U8_L2) break;
+    }
 #undef MVG_GEMM
-  MVG_LAUNCH_CHECK("gemm_topk_kernel");
+    MVG_LAUNCH_CHECK("gemm_topk_kernel");
+  };
+  if (!pl.wide) {
+    run16(pl.S, pl.split_len, pk, nullptr);
+    return;
+  }
+  const int qtiles = ceil_div(nq, kWQ);
+  const int kw = pl.kw, P = pl.P;
+  const int64_t S = pl.S, split_len = pl.split_len;
+  const dim3 grid(xcd_grid(qtiles, S));
+  if (pl.reg) {
+    const WideRegLayout rlay(kw);
+#define MVG_WREG(K, N, T)                                                                          \
+  {                                                                                                \
+    const void *rf = (const void *)gemm_wide_reg_kernel<K, N, T>;                                  \
+    if (rlay.total > 64 * 1024)                                                                    \
+      MVG_HIP(hipFuncSetAttribute(rf, hipFuncAttributeMaxDynamicSharedMemorySize, rlay.total));    \
+    hipLaunchKernelGGL((gemm_wide_reg_kernel<K, N, T>), grid, dim3(kNT), rlay.total, s, Q, X, nq, n, \
+                       d, qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kw, split_len, excl,     \
+                       excl_len, pk, (int)S, qinv, xinv, P);                                       \
+  }
+#define MVG_WREG_KIND(T)                     \
+  switch (kind) {                            \
+    case BF16_L2: MVG_WREG(BF16_L2, 24, T) break; \
+    case BF16_IP: MVG_WREG(BF16_IP, 24, T) break; \
+    case I8_IP: MVG_WREG(I8_IP, 12, T) break;     \
+    case FP8_L2: MVG_WREG(FP8_L2, 12, T) break;   \
+    case FP8_IP: MVG_WREG(FP8_IP, 12, T) break;   \
+    default: MVG_WREG(U8_L2, 12, T) break;        \
+  }
+    if (pl.strided) MVG_WREG_KIND(true) else MVG_WREG_KIND(false)
+#undef MVG_WREG_KIND
+#undef MVG_WREG
+    MVG_LAUNCH_CHECK("gemm_wide_reg_kernel");
+  } else {
+    const WideLayout wlay(row_bytes, kw);
+#define MVG_WIDE(K, T)                                                                             \
+  {                                                                                                \
+    const void *fn = (const void *)gemm_wide_kernel<K, T>;                                         \
+    if (wlay.total > 64 * 1024)                                                                    \
+      MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, wlay.total));    \
+    hipLaunchKernelGGL((gemm_wide_kernel<K, T>), grid, dim3(kNT), wlay.total, s, Q, X, nq, n,      \
+                       row_bytes, d, qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kw, split_len, \
+                       excl, excl_len, pk, (int)S, qinv, xinv, P);                                 \
+  }
+#define MVG_WIDE_KIND(T)                       \
+  switch (kind) {                              \
+    case BF16_L2: MVG_WIDE(BF16_L2, T) break;  \
+    case BF16_IP: MVG_WIDE(BF16_IP, T) break;  \
+    case I8_IP: MVG_WIDE(I8_IP, T) break;      \
+    case FP8_L2: MVG_WIDE(FP8_L2, T) break;    \
+    case FP8_IP: MVG_WIDE(FP8_IP, T) break;    \
+    default: MVG_WIDE(U8_L2, T) break;         \
+  }
+    if (pl.strided) MVG_WIDE_KIND(true) else MVG_WIDE_KIND(false)
+#undef MVG_WIDE_KIND
+#undef MVG_WIDE
+    MVG_LAUNCH_CHECK("gemm_wide_kernel");
+  }
+  if (pl.strided) {
+    uint64_t *redo = pk + (size_t)nq * S * kw;
+    uint8_t *flag = reinterpret_cast<uint8_t *>(redo + (size_t)nq * kk);
+    hipLaunchKernelGGL(gemm_certify_kernel, dim3((unsigned)ceil_div(nq, kNT / 64)), dim3(kNT), 0, s, pk, nq,
+                       (int)S, kw, kk, flag);
+    MVG_LAUNCH_CHECK("gemm_certify_kernel");
+    // one split over every row: the flagged queries' top kk, sorted
+    run16(1, ((n + kStepCols - 1) / kStepCols) * kStepCols, redo, flag);
+    hipLaunchKernelGGL(gemm_fixup_kernel, dim3((unsigned)nq), dim3(kNT), 0, s, pk, ncand, redo, kk, flag);
+    MVG_LAUNCH_CHECK("gemm_fixup_kernel");
+  }
 }
 
 }  // namespace mvg

@@ -2566,12 +2566,12 @@ void ivfpq_search(const IvfSearchArgs &a, IvfWork &w, hipStream_t s) {
     w.qbf.reserve((size_t)a.nq * rb);
     w.qnorm.reserve(sizeof(float) * a.nq);
     to_bf16(a.q, a.nq, a.d, w.qbf.as<uint16_t>(), w.qnorm.as<float>(), s);
-    int nsplit = 0;
+    int64_t ncand = 0;
     gemm_topk(a.metric == kMetricIP ? kGemmBf16IP : kGemmBf16L2, w.qbf.as<uint8_t>(),
               reinterpret_cast<const uint8_t *>(a.coarse_bf16), a.nq, a.nlist, a.d, w.qnorm.as<float>(),
               a.coarse_norm, nullptr, nullptr, nullptr, nullptr, 0.0f, maxp, nullptr, 0, w.coarse_part,
-              nsplit, s);
-    launch_merge(w.coarse_part.as<uint64_t>(), a.nq, (int64_t)nsplit * maxp, maxp, 0, 0.0f, nullptr,
+              ncand, s);
+    launch_merge(w.coarse_part.as<uint64_t>(), a.nq, ncand, maxp, 0, 0.0f, nullptr,
                  nullptr, probes, nullptr, s);
   } else {
     flat_topk(a.coarse, a.nlist, a.d, a.metric, a.q, a.nq, maxp, nullptr, 0, w.coarse_part, 0.0f,

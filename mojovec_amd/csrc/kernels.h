@@ -227,12 +227,25 @@ void to_bf16(const float *x, int64_t n, int d, uint16_t *out, float *norms, hipS
 // per-row terms of the FP8 inner-product estimate).
 void to_fp8(const float *x, int64_t n, int d, uint8_t *out, float *inv, float *norms,
             hipStream_t s, bool half_diff = false);
-// Per-(query, base split) top-kk keys part[nq][nsplit][kk] of the GEMM-form
-// distance; reduce with launch_merge over nsplit * kk candidates.
+// How gemm_topk splits a search: the 64-query form (wide; reg: the register
+// tile) on S = C x P splits (P > 1: strided, each split's top kw = 64 with a
+// per-query check and the 16-query form for the queries it rejects), or the
+// 16-query form on S contiguous splits; ncand candidates per query in a
+// workspace of `bytes`.
+struct GemmPlan {
+  bool wide, reg, strided;
+  int kw, P;
+  int64_t S, split_len, ncand;
+  size_t bytes;
+};
+GemmPlan gemm_plan(int kind, int64_t nq, int64_t n, int d, int kk);
+// Candidate keys of the GEMM-form distance, part[nq][ncand]: per base split
+// the split's sorted top list (sentinel-padded), and the union holds each
+// query's exact top kk; reduce with launch_merge over ncand candidates.
 void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t n, int d,
                const float *qnorm, const float *xnorm, const uint32_t *qn_u, const uint32_t *qs_u,
                const uint32_t *xn_u, const uint32_t *xs_u, float scale_sq, int kk,
-               const uint8_t *excl, int64_t excl_len, DevBuf &part, int &nsplit, hipStream_t s,
+               const uint8_t *excl, int64_t excl_len, DevBuf &part, int64_t &ncand, hipStream_t s,
                const float *qinv = nullptr, const float *xinv = nullptr);
 
 // ---- sq8.hip ----

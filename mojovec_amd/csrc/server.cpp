@@ -107,10 +107,19 @@ int64_t batch_bytes(const mvg_server *sv, int target, int64_t nq, int k, int kf,
   if (target == 1) {
     // queries, query codes and norms, outputs, merged candidates, and the
     // per-split partial lists: at most 64 splits per query in the 64-query
-    // form, at most 2048 workgroups of 16 queries in the 16-query form
+    // form up to 64 candidates, at most 2048 workgroups of 16 queries in the
+    // 16-query form, and above 64 candidates the MFMA forms' own plan
     const int64_t ld = mvg::sq8_row_stride((int)d);
-    return nq * (d * 4 + ld + 8 + (int64_t)k * 12 + kk * 8 * 65) + (int64_t)2048 * 16 * kk * 8 +
-           (1 << 20);
+    int64_t lists = nq * kk * 8 * 64 + (int64_t)2048 * 16 * kk * 8;
+    if (sv->sq->use_mfma && sv->sq->ntotal > 0) {
+      try {
+        const int kind = sv->sq->metric == mvg::kMetricIP ? mvg::kGemmI8IP : mvg::kGemmU8L2;
+        lists = std::max<int64_t>(lists, (int64_t)mvg::gemm_plan(kind, nq, sv->sq->ntotal, (int)ld, (int)kk).bytes);
+      } catch (const mvg::Error &) {
+        // a shape the MFMA forms reject fails in the search itself
+      }
+    }
+    return nq * (d * 4 + ld + 8 + (int64_t)k * 12 + kk * 8) + lists + (1 << 20);
   }
   // queries, probes, per-query candidates (work queue: up to 8 items), outputs
   return nq * (d * 6 + (int64_t)maxp * 12 + kk * 8 * 9 + (int64_t)k * 12) + (1 << 20);
@@ -154,9 +163,10 @@ void fail_slot(mvg_server *sv, Slot &sl, const std::string &msg) {
 }
 
 // Whether a request may join a batch whose head is `head`: equal k_factor;
-// over a Flat SQ8 index also k * k_factor on the head's side of 64 (the
-// 64-query MFMA form holds up to 64 candidates per query, larger pools take
-// the 16-query form, so mixing them would slow the small ones).
+// over a Flat SQ8 index also k * k_factor on the head's side of 64 (up to 64
+// candidates per query the MFMA form keeps one top list per base split,
+// larger pools take strided splits with a per-query check, so mixing them
+// would slow the small ones).
 bool joins(const mvg_server *sv, const Request *head, const Request *r) {
   (void)sv;
   if (r->target != head->target || r->k_factor != head->k_factor) return false;

@@ -215,16 +215,16 @@ void sq8_search(const Sq8SearchArgs &a, Sq8Work &w, hipStream_t s) {
              w.qnorms.as<uint32_t>(), w.qsums.as<uint32_t>(), s);
   const float ssq = a.scale * a.scale;
   if (a.use_mfma && a.n > 0) {
-    int nsplit = 0;
+    int64_t ncand = 0;
     gemm_topk(ip ? kGemmI8IP : kGemmU8L2, w.qcodes.as<uint8_t>(), a.codes, a.nq, a.n, ld, nullptr,
               nullptr, w.qnorms.as<uint32_t>(), w.qsums.as<uint32_t>(), a.norms, a.sums, ssq, kk,
-              a.excl, a.excl_len, w.part, nsplit, s);
+              a.excl, a.excl_len, w.part, ncand, s);
     if (kf == 1) {
-      launch_merge(w.part.as<uint64_t>(), a.nq, (int64_t)nsplit * kk, a.k, ip ? 1 : 0, 0.0f, a.D, a.I,
+      launch_merge(w.part.as<uint64_t>(), a.nq, ncand, a.k, ip ? 1 : 0, 0.0f, a.D, a.I,
                    nullptr, nullptr, s, nullptr, a.kq, 1);
     } else {
       w.keys.reserve(sizeof(uint64_t) * (size_t)a.nq * kk);
-      launch_merge(w.part.as<uint64_t>(), a.nq, (int64_t)nsplit * kk, kk, 0, 0.0f, nullptr, nullptr,
+      launch_merge(w.part.as<uint64_t>(), a.nq, ncand, kk, 0, 0.0f, nullptr, nullptr,
                    nullptr, w.keys.as<uint64_t>(), s, nullptr, a.kq, kf);
       rerank(a.q, a.nq, a.d, a.metric, a.vectors, a.n, w.keys.as<uint64_t>(), kk, a.k, 0.0f, a.D, a.I, s,
              nullptr, RowMap{}, a.kq);

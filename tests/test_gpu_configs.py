@@ -4,7 +4,8 @@
   stream scan with the SQ8 pre-rerank, against the oracle's exact search over
   the probed rows;
 - SQ8 / FP8 / BF16 Flat at d = 768 (the 1M x 768 embedding layout), several
-  64-query tiles, k = 100 with k_factor 2 (the 768-key append buffer);
+  64-query tiles, k = 100 with k_factor 2 (strided splits of the 64-query
+  form, and queries the per-query check sends back to the 16-query form);
 - IVF-PQ 10M x 96 layout (d = 96, M = 12) above 4096 lists, per-query nprobe
   in 4..128 and k in {1, 10, 100}, with the autotuned schedule and a captured
   graph;
@@ -231,8 +232,9 @@ def emb768():
 def test_sq8_d768_matches_oracle(gpu, emb768, metric):
     # cosine (int8 codes, symmetric) and L2 (affine uint8 rebased to int8) on
     # int8 MFMA at d = 768 over 300 queries (five 64-query tiles), k = 10 with
-    # k_factor 1/2/4 and k = 100 with k_factor 2 (the 768-key append buffer of
-    # the 16-query form), with and without exclusions; then 1 and 17 queries
+    # k_factor 1/2/4 and k = 100 with k_factor 2 (the 64-query form on nine
+    # strided splits), with and without exclusions; then 1 query (the
+    # 16-query form with its 768-key append buffer) and 17
     x, q = emb768
     gx = gpu.IndexFlatSQ8(768, metric)
     ox = o.SQ8Index(768, metric)
@@ -275,6 +277,45 @@ def test_sq8_large_append_buffer_small_rows(gpu):
                 Dg, Ig = gx.search(q[:nq], 128, k_factor=2, exclude=exm)
                 Do, Io = ox.search(q[:nq], 128, k_factor=2, exclude=exm)
                 assert_same(Dg, Ig, Do, Io)
+
+
+def _period_layout(seed, n, d, nq):
+    """Unit rows where every 45th row (ids 0, 45, 90, ...) sits near one
+    direction and the rest around 40 others; the first half of the queries
+    sit at that direction, the rest among the other rows. Above 64 candidates
+    the 64-query form splits the rows by residue mod P (P = 5 for k x k_factor
+    = 100, 9 for 200: divisors of 45), so a near query's nearest rows all fall
+    in one split, more than the 64 its list keeps: the per-query check rejects
+    the union and the 16-query form searches that query again."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal(d)
+    a /= np.linalg.norm(a)
+    x = unit_mixture(seed + 1, n, d)
+    near = a + (0.2 / math.sqrt(d)) * rng.standard_normal((n // 45, d))
+    x[::45] = near / np.linalg.norm(near, axis=1, keepdims=True)
+    q = unit_mixture(seed + 2, nq, d)
+    qa = a + (0.2 / math.sqrt(d)) * rng.standard_normal((nq // 2, d))
+    q[: nq // 2] = qa / np.linalg.norm(qa, axis=1, keepdims=True)
+    return x.astype(np.float32), q.astype(np.float32)
+
+
+@pytest.mark.parametrize("d", [768, 48])
+@pytest.mark.parametrize("metric", [0, 1])
+def test_sq8_strided_splits_reject_and_search_again(gpu, d, metric):
+    # d = 768: the register-tile form, d = 48: the LDS form; 40 queries (three
+    # 16-query tiles for the second search, the middle one mixed), k x k_factor
+    # 200 and 100, with and without exclusions: equal to the oracle
+    x, q = _period_layout(250 + metric, 7200, d, 40)
+    ex = (np.random.default_rng(252).random(7200) < 0.2).astype(np.uint8)
+    gx = gpu.IndexFlatSQ8(d, metric)
+    ox = o.SQ8Index(d, metric)
+    gx.add(x)
+    ox.add(x)
+    for k, kf in [(100, 2), (50, 2), (100, 1)]:
+        for exm in (None, ex):
+            Dg, Ig = gx.search(q, k, k_factor=kf, exclude=exm)
+            Do, Io = ox.search(q, k, k_factor=kf, exclude=exm)
+            assert_same(Dg, Ig, Do, Io)
 
 
 def _w64_topk(x, q, k, metric):
@@ -328,7 +369,7 @@ def test_mixed_flat_d768_candidates_exact_on_small_integers(gpu, form, metric):
     ex = (rng.random(8000) < 0.2).astype(np.uint8)
     ix = gpu.IndexFlat(768, metric)
     ix.add(x)
-    for k in (1, 10, 64):
+    for k in (1, 10, 64, 100):  # 100: the 64-query form on strided splits
         ix.set_search_mode("plain")
         Dp, Ip = ix.search(q, k, exclude=ex)
         ix.set_search_mode(form)

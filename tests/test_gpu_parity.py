@@ -471,7 +471,8 @@ def test_sq8_many_query_tiles_with_exclusion(gpu, metric):
     ox = o.SQ8Index(48, metric)
     gx.add(x)
     ox.add(x)
-    # (100, 2), (50, 3): above 64 candidates the 16-query form, its split
+    # (100, 2), (50, 3): above 64 candidates the 64-query form on strided
+    # splits (one top 64 per split); 1 query: the 16-query form, its split
     # count from the CU balance and flushes by radix select
     for k, kf in [(1, 1), (10, 1), (16, 4), (64, 1), (10, 6), (100, 2), (50, 3)]:
         Dg, Ig = gx.search(q, k, k_factor=kf, exclude=ex)
