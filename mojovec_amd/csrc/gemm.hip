@@ -117,15 +117,16 @@ inline unsigned xcd_grid(int64_t qtiles, int64_t S) { return (unsigned)(8 * ((qt
 
 // Rows of one base split in the wide forms. P == 1: split s is the contiguous
 // range [s * split_len, (s + 1) * split_len). P > 1 (strided splits): split s
-// is residue r = s % P of chunk c = s / P, i.e. the rows r, r + P, r + 2 P, ...
+// is residue r = s % R of chunk c = s / R (R = P: every residue; R = 1: only
+// the rows 0, P, 2 P, ..., a 1/P sample), i.e. the rows r, r + P, r + 2 P, ...
 // whose local index j = (row - r) / P lies in [c * split_len, (c + 1) *
 // split_len). Local row j is base row goff + j * P.
 struct SplitRows {
   int64_t goff, b_begin, b_end;
-  __device__ __forceinline__ SplitRows(int64_t n, int split, int P, int64_t split_len) {
-    const int r = split % P;
+  __device__ __forceinline__ SplitRows(int64_t n, int split, int P, int R, int64_t split_len) {
+    const int r = split % R;
     goff = r;
-    b_begin = (int64_t)(split / P) * split_len;
+    b_begin = (int64_t)(split / R) * split_len;
     b_end = min((n - r + P - 1) / P, b_begin + split_len);
   }
 };
@@ -370,6 +371,9 @@ struct WideSel {
   // STR (strided splits): local column j of the split is row goff + j *
   // stride of the base; otherwise the local column is the row
   int64_t goff = 0, stride = 1;
+  // a fixed bound on each query's keys (strict; the sentinel when none):
+  // the running threshold never rises above it
+  uint64_t bnd_k[4];
 
   __device__ __forceinline__ void init(int64_t q0, int wq, int qcount, int lgrp, const float *qnorm,
                                        const float *qinv, const uint32_t *qn_u, const uint32_t *qs_u,
@@ -387,6 +391,23 @@ struct WideSel {
       qsx[j] = KIND == U8_L2 ? (int64_t)qs_u[qg] - 128 * (int64_t)d_true : 0;
       ithr[j] = KIND == I8_IP ? -(1 << 30) : -600000000;  // no bound yet: everything passes
       cnt_r[j] = 0;
+      bnd_k[j] = kSentinel64;
+    }
+  }
+
+  // Keys up to bound[q * bstride] (inclusive) pass from the start; the
+  // split's top list then holds all of its keys up to that bound while it is
+  // not full.
+  __device__ __forceinline__ void set_bound(const uint64_t *bound, int bstride, int64_t q0, int wq, int lgrp,
+                                            float scale_sq, int d_true) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (!qok[j]) continue;
+      const uint64_t b = bound[(q0 + wq + 4 * lgrp + j) * bstride];
+      if (b == kSentinel64) continue;
+      bnd_k[j] = thr_k[j] = b + 1;
+      thr_f[j] = key2f((uint32_t)(b >> 32));
+      int_thr(j, scale_sq, d_true);
     }
   }
 
@@ -515,7 +536,7 @@ struct WideSel {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           cnt_r[j] = cnt[4 * lgrp + j];
-          const uint64_t th = top[(4 * lgrp + j) * kk + kk - 1];
+          const uint64_t th = min(top[(4 * lgrp + j) * kk + kk - 1], bnd_k[j]);
           thr_k[j] = th;
           thr_f[j] = th == kSentinel64 ? INFINITY : key2f((uint32_t)(th >> 32));
           int_thr(j, scale_sq, d_true);
@@ -562,8 +583,8 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
     const uint32_t *__restrict__ xn_u, const uint32_t *__restrict__ xs_u, float scale_sq, int kk,
     int64_t split_len, const uint8_t *__restrict__ excl, int64_t excl_len,
     uint64_t *__restrict__ part, int S, const float *__restrict__ qinv,
-    const float *__restrict__ xinv, int P) {
-  if (!STR) P = 1;  // contiguous splits: the row arithmetic folds away
+    const float *__restrict__ xinv, int P, int R, const uint64_t *__restrict__ bound, int bstride) {
+  if (!STR) P = R = 1;  // contiguous splits: the row arithmetic folds away
   extern __shared__ __align__(16) unsigned char smem[];
   const WideLayout lay(row_bytes, kk);
   uint8_t *atile = smem + lay.a;
@@ -579,7 +600,7 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
   int split;
   if (!xcd_tile(nq, kWQ, S, qt, split)) return;
   const int64_t q0 = qt * kWQ;
-  const SplitRows sr(n, split, P, split_len);
+  const SplitRows sr(n, split, P, R, split_len);
   const int64_t b_begin = sr.b_begin, b_end = sr.b_end;
   const int qcount = (int)min<int64_t>(kWQ, nq - q0);
   if (b_end <= b_begin) {
@@ -607,6 +628,7 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
   sel.init(q0, wq, qcount, lgrp, qnorm, qinv, qn_u, qs_u, d_true);
   sel.goff = sr.goff;
   sel.stride = P;
+  if (STR && bound) sel.set_bound(bound, bstride, q0, wq, lgrp, scale_sq, d_true);
 
   const int nks = row_bytes / 64;    // 64-byte k steps of a row
   const int nsl = (nks + 1) >> 1;    // 128-byte slices per step
@@ -802,8 +824,8 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
     const uint32_t *__restrict__ xn_u, const uint32_t *__restrict__ xs_u, float scale_sq, int kk,
     int64_t split_len, const uint8_t *__restrict__ excl, int64_t excl_len,
     uint64_t *__restrict__ part, int S, const float *__restrict__ qinv,
-    const float *__restrict__ xinv, int P) {
-  if (!STR) P = 1;  // contiguous splits: the row arithmetic folds away
+    const float *__restrict__ xinv, int P, int R, const uint64_t *__restrict__ bound, int bstride) {
+  if (!STR) P = R = 1;  // contiguous splits: the row arithmetic folds away
   static_assert(NKS % 2 == 0, "whole 128-byte slices");
   constexpr int NSL = NKS / 2;                                    // slices per row
   constexpr bool F8 = KIND == FP8_L2 || KIND == FP8_IP;
@@ -830,7 +852,7 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
   int split;
   if (!xcd_tile(nq, kWQ, S, qt, split)) return;
   const int64_t q0 = qt * kWQ;
-  const SplitRows sr(n, split, P, split_len);
+  const SplitRows sr(n, split, P, R, split_len);
   const int64_t b_begin = sr.b_begin, b_end = sr.b_end;
   const int qcount = (int)min<int64_t>(kWQ, nq - q0);
   if (b_end <= b_begin) {
@@ -872,6 +894,7 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
   sel.init(q0, wq, qcount, lgrp, qnorm, qinv, qn_u, qs_u, d_true);
   sel.goff = sr.goff;
   sel.stride = P;
+  if (STR && bound) sel.set_bound(bound, bstride, q0, wq, lgrp, scale_sq, d_true);
 
   const int srow = tid >> 3, sgrp = tid & 7;  // staging: 8 columns x 16 bytes per thread
   const int sswz = wide_swz(sgrp);
@@ -1151,15 +1174,17 @@ void to_bf16(const float *x, int64_t n, int d, uint16_t *out, float *norms, hipS
 namespace {
 
 // Strided splits keep each split's exact top kper (sorted, the sentinel past
-// a split's last row). Let M be the smallest kper-th key over the splits
-// whose list is full (the sentinel if none is). Every split's list holds all
-// of that split's keys up to M -- a full list ends at or above M, and a list
-// that is not full holds all its split's keys -- so the keys <= M in the
-// union are exactly the base's keys <= M. If there are at least kk of them,
-// the union's kk smallest are the base's; otherwise (a split held more than
-// kper of the query's kk nearest rows) flag[q] = 1 and the query is searched
-// again. One wavefront per query.
+// a split's last row) among its keys up to the query's bound B (the sentinel
+// without one). Let M be min(B, the smallest kper-th key over the splits
+// whose list is full). Every split's list holds all of that split's keys up
+// to M -- a full list ends at or above M, and a list that is not full holds
+// all its split's keys up to B -- so the keys <= M in the union are exactly
+// the base's keys <= M. If there are at least kk of them, the union's kk
+// smallest are the base's; otherwise (a split held more than kper of the
+// query's kk nearest rows, or fewer than kk keys lie below B) flag[q] = 1 and
+// the query is searched again. One wavefront per query.
 __global__ void gemm_certify_kernel(const uint64_t *__restrict__ part, int64_t nq, int S, int kper, int kk,
+                                    const uint64_t *__restrict__ bound, int bstride,
                                     uint8_t *__restrict__ flag) {
   const int lane = threadIdx.x & 63;
   const int64_t q = (int64_t)blockIdx.x * (kNT / 64) + (threadIdx.x >> 6);
@@ -1169,6 +1194,7 @@ __global__ void gemm_certify_kernel(const uint64_t *__restrict__ part, int64_t n
   for (int s = lane; s < S; s += 64) m = min(m, row[(int64_t)s * kper + kper - 1]);
 #pragma unroll
   for (int j = 1; j < 64; j <<= 1) m = min(m, xor_lane64(m, j));
+  if (bound) m = min(m, bound[q * bstride]);  // no key above it was kept
   int c = 0;
   if (m != kSentinel64) {
     for (int s = lane; s < S; s += 64) {
@@ -1247,10 +1273,31 @@ GemmPlan gemm_plan(int kind, int64_t nq, int64_t n, int d, int kk) {
     pl.kw = kw;
     pl.S = C * P;
     pl.ncand = pl.S * kw;
-    // [nq][S][kw] split lists, then (strided) the 16-query form's [nq][kk]
-    // and one flag byte per query
+    // [nq][S][kw] split lists, then (strided) the 16-query form's [nq][kk],
+    // one flag byte per query (rounded up to 8 bytes) and the sample pass's
+    // lists and keys
     pl.bytes = sizeof(uint64_t) * (size_t)nq * pl.S * kw +
-               (strided ? sizeof(uint64_t) * (size_t)nq * kk + (size_t)nq : 0);
+               (strided ? sizeof(uint64_t) * (size_t)nq * kk + (size_t)((nq + 7) & ~int64_t(7)) : 0);
+    // A sample pass first (strided, rows 0, Ps, 2 Ps, ...): each query's
+    // kks-th key over the sample, about its (kks Ps)-th = (1.25 kk + 16 Ps)-th
+    // over the base, bounds the strided splits' lists, so each receives about
+    // (1.25 kk + 16 Ps) / P < 64 keys instead of filling and merging a top 64.
+    const int Ps = std::max(8, ceil_div(5 * kk, 4 * 48));
+    if (strided && n >= (int64_t)Ps * kStepCols) {
+      pl.Ps = Ps;
+      pl.kks = std::min(kWMaxK, ceil_div(5 * kk, 4 * Ps) + 16);
+      const int64_t ns = ceil_div(n, Ps);
+      const int64_t csmax = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(ns, kStepCols)));
+      double bs = 1e30;
+      for (int64_t c = 1; c <= csmax; c++) bs = std::min(bs, (double)ceil_div(qtiles * c, cus) / c);
+      int64_t Cs = 1;
+      while (Cs < csmax && (double)ceil_div(qtiles * Cs, cus) / Cs > 1.1 * bs) Cs++;
+      const int64_t psrow = (int64_t)Ps * row_bytes;
+      if (reg) Cs = std::max<int64_t>(Cs, ceil_div(ns * psrow, (int64_t(1) << 31) - 512 * psrow));
+      pl.split_len_s = ((ceil_div(ns, Cs) + kStepCols - 1) / kStepCols) * kStepCols;
+      pl.Ss = ceil_div(ns, pl.split_len_s);
+      pl.bytes += sizeof(uint64_t) * (size_t)nq * (pl.Ss + 1) * pl.kks;
+    }
     return pl;
   }
   if (glay.total > 150 * 1024) throw Error(kInvalid, "GEMM top-k workspace exceeds LDS");
@@ -1326,22 +1373,24 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
     return;
   }
   const int qtiles = ceil_div(nq, kWQ);
-  const int kw = pl.kw, P = pl.P;
-  const int64_t S = pl.S, split_len = pl.split_len;
-  const dim3 grid(xcd_grid(qtiles, S));
-  if (pl.reg) {
-    const WideRegLayout rlay(kw);
+  // one launch of the 64-query form: S_ splits of split_len_ local rows, top
+  // kw_ per split into out; P_ / R_ as in SplitRows; bnd: per-query bound
+  auto wide = [&](bool str, int64_t S_, int64_t split_len_, int kw_, int P_, int R_, uint64_t *out,
+                  const uint64_t *bnd, int bstride) {
+    const dim3 grid(xcd_grid(qtiles, S_));
+    if (pl.reg) {
+      const WideRegLayout rlay(kw_);
 #define MVG_WREG(K, N, T)                                                                          \
   {                                                                                                \
     const void *rf = (const void *)gemm_wide_reg_kernel<K, N, T>;                                  \
     if (rlay.total > 64 * 1024)                                                                    \
       MVG_HIP(hipFuncSetAttribute(rf, hipFuncAttributeMaxDynamicSharedMemorySize, rlay.total));    \
     hipLaunchKernelGGL((gemm_wide_reg_kernel<K, N, T>), grid, dim3(kNT), rlay.total, s, Q, X, nq, n, \
-                       d, qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kw, split_len, excl,     \
-                       excl_len, pk, (int)S, qinv, xinv, P);                                       \
+                       d, qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kw_, split_len_, excl,   \
+                       excl_len, out, (int)S_, qinv, xinv, P_, R_, bnd, bstride);                  \
   }
-#define MVG_WREG_KIND(T)                     \
-  switch (kind) {                            \
+#define MVG_WREG_KIND(T)                          \
+  switch (kind) {                                 \
     case BF16_L2: MVG_WREG(BF16_L2, 24, T) break; \
     case BF16_IP: MVG_WREG(BF16_IP, 24, T) break; \
     case I8_IP: MVG_WREG(I8_IP, 12, T) break;     \
@@ -1349,46 +1398,62 @@ void gemm_topk(int kind, const uint8_t *Q, const uint8_t *X, int64_t nq, int64_t
     case FP8_IP: MVG_WREG(FP8_IP, 12, T) break;   \
     default: MVG_WREG(U8_L2, 12, T) break;        \
   }
-    if (pl.strided) MVG_WREG_KIND(true) else MVG_WREG_KIND(false)
+      if (str) MVG_WREG_KIND(true) else MVG_WREG_KIND(false)
 #undef MVG_WREG_KIND
 #undef MVG_WREG
-    MVG_LAUNCH_CHECK("gemm_wide_reg_kernel");
-  } else {
-    const WideLayout wlay(row_bytes, kw);
+      MVG_LAUNCH_CHECK("gemm_wide_reg_kernel");
+    } else {
+      const WideLayout wlay(row_bytes, kw_);
 #define MVG_WIDE(K, T)                                                                             \
   {                                                                                                \
     const void *fn = (const void *)gemm_wide_kernel<K, T>;                                         \
     if (wlay.total > 64 * 1024)                                                                    \
       MVG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, wlay.total));    \
     hipLaunchKernelGGL((gemm_wide_kernel<K, T>), grid, dim3(kNT), wlay.total, s, Q, X, nq, n,      \
-                       row_bytes, d, qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kw, split_len, \
-                       excl, excl_len, pk, (int)S, qinv, xinv, P);                                 \
+                       row_bytes, d, qnorm, xnorm, qn_u, qs_u, xn_u, xs_u, scale_sq, kw_,           \
+                       split_len_, excl, excl_len, out, (int)S_, qinv, xinv, P_, R_, bnd, bstride); \
   }
-#define MVG_WIDE_KIND(T)                       \
-  switch (kind) {                              \
-    case BF16_L2: MVG_WIDE(BF16_L2, T) break;  \
-    case BF16_IP: MVG_WIDE(BF16_IP, T) break;  \
-    case I8_IP: MVG_WIDE(I8_IP, T) break;      \
-    case FP8_L2: MVG_WIDE(FP8_L2, T) break;    \
-    case FP8_IP: MVG_WIDE(FP8_IP, T) break;    \
-    default: MVG_WIDE(U8_L2, T) break;         \
+#define MVG_WIDE_KIND(T)                      \
+  switch (kind) {                             \
+    case BF16_L2: MVG_WIDE(BF16_L2, T) break; \
+    case BF16_IP: MVG_WIDE(BF16_IP, T) break; \
+    case I8_IP: MVG_WIDE(I8_IP, T) break;     \
+    case FP8_L2: MVG_WIDE(FP8_L2, T) break;   \
+    case FP8_IP: MVG_WIDE(FP8_IP, T) break;   \
+    default: MVG_WIDE(U8_L2, T) break;        \
   }
-    if (pl.strided) MVG_WIDE_KIND(true) else MVG_WIDE_KIND(false)
+      if (str) MVG_WIDE_KIND(true) else MVG_WIDE_KIND(false)
 #undef MVG_WIDE_KIND
 #undef MVG_WIDE
-    MVG_LAUNCH_CHECK("gemm_wide_kernel");
+      MVG_LAUNCH_CHECK("gemm_wide_kernel");
+    }
+  };
+  if (!pl.strided) {
+    wide(false, pl.S, pl.split_len, pl.kw, 1, 1, pk, nullptr, 0);
+    return;
   }
-  if (pl.strided) {
-    uint64_t *redo = pk + (size_t)nq * S * kw;
-    uint8_t *flag = reinterpret_cast<uint8_t *>(redo + (size_t)nq * kk);
-    hipLaunchKernelGGL(gemm_certify_kernel, dim3((unsigned)ceil_div(nq, kNT / 64)), dim3(kNT), 0, s, pk, nq,
-                       (int)S, kw, kk, flag);
-    MVG_LAUNCH_CHECK("gemm_certify_kernel");
-    // one split over every row: the flagged queries' top kk, sorted
-    run16(1, ((n + kStepCols - 1) / kStepCols) * kStepCols, redo, flag);
-    hipLaunchKernelGGL(gemm_fixup_kernel, dim3((unsigned)nq), dim3(kNT), 0, s, pk, ncand, redo, kk, flag);
-    MVG_LAUNCH_CHECK("gemm_fixup_kernel");
+  const int64_t S = pl.S;
+  const int kw = pl.kw;
+  uint64_t *redo = pk + (size_t)nq * S * kw;
+  uint8_t *flag = reinterpret_cast<uint8_t *>(redo + (size_t)nq * kk);
+  const uint64_t *bound = nullptr;
+  if (pl.Ps > 0) {
+    // sample pass: rows 0, Ps, 2 Ps, ... -> each query's sorted top kks,
+    // whose last key bounds the strided pass
+    uint64_t *spart = reinterpret_cast<uint64_t *>(flag + ((nq + 7) & ~int64_t(7)));
+    uint64_t *skeys = spart + (size_t)nq * pl.Ss * pl.kks;
+    wide(true, pl.Ss, pl.split_len_s, pl.kks, pl.Ps, 1, spart, nullptr, 0);
+    launch_merge(spart, nq, pl.Ss * pl.kks, pl.kks, 0, 0.0f, nullptr, nullptr, nullptr, skeys, s);
+    bound = skeys + pl.kks - 1;
   }
+  wide(true, S, pl.split_len, kw, pl.P, pl.P, pk, bound, pl.kks);
+  hipLaunchKernelGGL(gemm_certify_kernel, dim3((unsigned)ceil_div(nq, kNT / 64)), dim3(kNT), 0, s, pk, nq, (int)S,
+                     kw, kk, bound, pl.kks, flag);
+  MVG_LAUNCH_CHECK("gemm_certify_kernel");
+  // one split over every row: the rejected queries' top kk, sorted
+  run16(1, ((n + kStepCols - 1) / kStepCols) * kStepCols, redo, flag);
+  hipLaunchKernelGGL(gemm_fixup_kernel, dim3((unsigned)nq), dim3(kNT), 0, s, pk, ncand, redo, kk, flag);
+  MVG_LAUNCH_CHECK("gemm_fixup_kernel");
 }
 
 }  // namespace mvg

@@ -228,15 +228,19 @@ void to_bf16(const float *x, int64_t n, int d, uint16_t *out, float *norms, hipS
 void to_fp8(const float *x, int64_t n, int d, uint8_t *out, float *inv, float *norms,
             hipStream_t s, bool half_diff = false);
 // How gemm_topk splits a search: the 64-query form (wide; reg: the register
-// tile) on S = C x P splits (P > 1: strided, each split's top kw = 64 with a
-// per-query check and the 16-query form for the queries it rejects), or the
-// 16-query form on S contiguous splits; ncand candidates per query in a
-// workspace of `bytes`.
+// tile) on S = C x P splits (P > 1: strided, each split's top kw = 64 under a
+// per-query bound from a sample pass, with a per-query check and the 16-query
+// form for the queries it rejects), or the 16-query form on S contiguous
+// splits; ncand candidates per query in a workspace of `bytes`.
 struct GemmPlan {
   bool wide, reg, strided;
   int kw, P;
   int64_t S, split_len, ncand;
   size_t bytes;
+  // strided: the sample pass that bounds each query's keys (Ps = 0: none),
+  // rows 0, Ps, 2 Ps, ... in Ss chunks, top kks per query
+  int Ps, kks;
+  int64_t Ss, split_len_s;
 };
 GemmPlan gemm_plan(int kind, int64_t nq, int64_t n, int d, int kk);
 // Candidate keys of the GEMM-form distance, part[nq][ncand]: per base split

@@ -279,33 +279,38 @@ def test_sq8_large_append_buffer_small_rows(gpu):
                 assert_same(Dg, Ig, Do, Io)
 
 
-def _period_layout(seed, n, d, nq):
-    """Unit rows where every 45th row (ids 0, 45, 90, ...) sits near one
-    direction and the rest around 40 others; the first half of the queries
-    sit at that direction, the rest among the other rows. Above 64 candidates
-    the 64-query form splits the rows by residue mod P (P = 5 for k x k_factor
-    = 100, 9 for 200: divisors of 45), so a near query's nearest rows all fall
-    in one split, more than the 64 its list keeps: the per-query check rejects
-    the union and the 16-query form searches that query again."""
+def _period_layout(seed, n, d, nq, period):
+    """Unit rows where every period-th row (ids 0, period, 2 period, ...) sits
+    near one direction and the rest around 40 others; the first half of the
+    queries sit at that direction, the rest among the other rows. Above 64
+    candidates the 64-query form first bounds each query's keys by a sample
+    pass over the rows 0, 8, 16, ..., then splits the rows by residue mod P
+    (P = 5 for k x k_factor = 100, 9 for 200) under that bound. Period 45 (P
+    divides it): a near query's nearest rows all fall in one split, more than
+    the 64 its list keeps. Period 8: they all lie in the sample, so its bound
+    is that query's 32nd / 48th nearest row and fewer than k x k_factor keys
+    pass. Either way the per-query check rejects the union and the 16-query
+    form searches that query again."""
     rng = np.random.default_rng(seed)
     a = rng.standard_normal(d)
     a /= np.linalg.norm(a)
     x = unit_mixture(seed + 1, n, d)
-    near = a + (0.2 / math.sqrt(d)) * rng.standard_normal((n // 45, d))
-    x[::45] = near / np.linalg.norm(near, axis=1, keepdims=True)
+    near = a + (0.2 / math.sqrt(d)) * rng.standard_normal((len(range(0, n, period)), d))
+    x[::period] = near / np.linalg.norm(near, axis=1, keepdims=True)
     q = unit_mixture(seed + 2, nq, d)
     qa = a + (0.2 / math.sqrt(d)) * rng.standard_normal((nq // 2, d))
     q[: nq // 2] = qa / np.linalg.norm(qa, axis=1, keepdims=True)
     return x.astype(np.float32), q.astype(np.float32)
 
 
+@pytest.mark.parametrize("period", [45, 8])
 @pytest.mark.parametrize("d", [768, 48])
 @pytest.mark.parametrize("metric", [0, 1])
-def test_sq8_strided_splits_reject_and_search_again(gpu, d, metric):
+def test_sq8_strided_splits_reject_and_search_again(gpu, d, metric, period):
     # d = 768: the register-tile form, d = 48: the LDS form; 40 queries (three
     # 16-query tiles for the second search, the middle one mixed), k x k_factor
     # 200 and 100, with and without exclusions: equal to the oracle
-    x, q = _period_layout(250 + metric, 7200, d, 40)
+    x, q = _period_layout(250 + metric, 7200, d, 40, period)
     ex = (np.random.default_rng(252).random(7200) < 0.2).astype(np.uint8)
     gx = gpu.IndexFlatSQ8(d, metric)
     ox = o.SQ8Index(d, metric)
