@@ -22,6 +22,7 @@
 // (K = 32 bf16 or 64 int8 per instruction), and each query keeps its running
 // top-kk in LDS behind a threshold, so only candidates that beat the current
 // kk-th key are appended and merged.
+#include <algorithm>
 #include <utility>
 
 #include "dist.h"
@@ -1233,12 +1234,20 @@ GemmPlan gemm_plan(int kind, int64_t nq, int64_t n, int d, int kk) {
   // Above 64 candidates the wide forms run on P strided splits (rows r, r +
   // P, ...): each keeps its exact top kw = 64, the union is certified per
   // query, and the 16-query form searches the rejected queries again (its
-  // layout must fit). With kk / P <= 24 of a query's kk nearest rows
-  // expected per split, more than 64 in one is far in the tail unless the
-  // rows are laid out with period P.
+  // layout must fit). A sample pass first (rows 0, Ps, 2 Ps, ...) takes each
+  // query's kks-th key over the sample, about its (kks Ps)-th = (1.25 kk +
+  // 16 Ps)-th over the base; at least kk base keys lie below it unless more
+  // than kks of the kk nearest rows fall in the sample (for kk = 200: 32
+  // against 12.5 expected, 5.7 standard deviations). That key bounds the
+  // strided lists: with P >= kks Ps / 40 each split receives about 40 keys
+  // instead of filling and merging a top 64, and with kk / P <= 24 of the kk
+  // nearest rows per split, more than 64 in one is far in the tail unless
+  // the rows are laid out with period P.
   const bool strided = kk > kWMaxK;
   const int kw = strided ? kWMaxK : kk;
-  const int P = strided ? std::max(2, ceil_div(kk, 24)) : 1;
+  const int Ps = std::max(16, ceil_div(5 * kk, 4 * 48));
+  const int kks = std::min(kWMaxK, ceil_div(5 * kk, 4 * Ps) + 16);
+  const int P = strided ? std::max({2, ceil_div(kk, 24), ceil_div(kks * Ps, 40)}) : 1;
   const WideLayout wlay(row_bytes, kw);
   // the register-resident query tile for the 768-byte int8 / FP8 rows and the
   // 1,536-byte BF16 rows (d = 768)
@@ -1278,14 +1287,10 @@ GemmPlan gemm_plan(int kind, int64_t nq, int64_t n, int d, int kk) {
     // lists and keys
     pl.bytes = sizeof(uint64_t) * (size_t)nq * pl.S * kw +
                (strided ? sizeof(uint64_t) * (size_t)nq * kk + (size_t)((nq + 7) & ~int64_t(7)) : 0);
-    // A sample pass first (strided, rows 0, Ps, 2 Ps, ...): each query's
-    // kks-th key over the sample, about its (kks Ps)-th = (1.25 kk + 16 Ps)-th
-    // over the base, bounds the strided splits' lists, so each receives about
-    // (1.25 kk + 16 Ps) / P < 64 keys instead of filling and merging a top 64.
-    const int Ps = std::max(8, ceil_div(5 * kk, 4 * 48));
+    // the sample pass (without enough rows for one, the lists start unbounded)
     if (strided && n >= (int64_t)Ps * kStepCols) {
       pl.Ps = Ps;
-      pl.kks = std::min(kWMaxK, ceil_div(5 * kk, 4 * Ps) + 16);
+      pl.kks = kks;
       const int64_t ns = ceil_div(n, Ps);
       const int64_t csmax = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(ns, kStepCols)));
       double bs = 1e30;

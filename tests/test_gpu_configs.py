@@ -284,13 +284,14 @@ def _period_layout(seed, n, d, nq, period):
     near one direction and the rest around 40 others; the first half of the
     queries sit at that direction, the rest among the other rows. Above 64
     candidates the 64-query form first bounds each query's keys by a sample
-    pass over the rows 0, 8, 16, ..., then splits the rows by residue mod P
-    (P = 5 for k x k_factor = 100, 9 for 200) under that bound. Period 45 (P
-    divides it): a near query's nearest rows all fall in one split, more than
-    the 64 its list keeps. Period 8: they all lie in the sample, so its bound
-    is that query's 32nd / 48th nearest row and fewer than k x k_factor keys
-    pass. Either way the per-query check rejects the union and the 16-query
-    form searches that query again."""
+    pass over the rows 0, 16, 32, ..., then splits the rows by residue mod P
+    (P = 10 for k x k_factor = 100, 13 for 200) under that bound. Period 130
+    (both P divide it): a near query's nearest rows all fall in one split,
+    more than the 64 its list keeps. Period 8: half of them lie in the
+    sample, so its bound (the sample's 24th / 32nd key) is about that query's
+    48th / 64th nearest row and fewer than k x k_factor keys pass. Either way
+    the per-query check rejects the union and the 16-query form searches that
+    query again."""
     rng = np.random.default_rng(seed)
     a = rng.standard_normal(d)
     a /= np.linalg.norm(a)
@@ -303,15 +304,16 @@ def _period_layout(seed, n, d, nq, period):
     return x.astype(np.float32), q.astype(np.float32)
 
 
-@pytest.mark.parametrize("period", [45, 8])
+@pytest.mark.parametrize("period", [130, 8])
 @pytest.mark.parametrize("d", [768, 48])
 @pytest.mark.parametrize("metric", [0, 1])
 def test_sq8_strided_splits_reject_and_search_again(gpu, d, metric, period):
     # d = 768: the register-tile form, d = 48: the LDS form; 40 queries (three
     # 16-query tiles for the second search, the middle one mixed), k x k_factor
     # 200 and 100, with and without exclusions: equal to the oracle
-    x, q = _period_layout(250 + metric, 7200, d, 40, period)
-    ex = (np.random.default_rng(252).random(7200) < 0.2).astype(np.uint8)
+    n = 14040  # 108 rows at multiples of 130
+    x, q = _period_layout(250 + metric, n, d, 40, period)
+    ex = (np.random.default_rng(252).random(n) < 0.2).astype(np.uint8)
     gx = gpu.IndexFlatSQ8(d, metric)
     ox = o.SQ8Index(d, metric)
     gx.add(x)
