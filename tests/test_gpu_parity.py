@@ -78,6 +78,13 @@ def test_flat_mixed_bf16_mfma_is_exact_on_integer_data(gpu, metric):
     ix.set_search_mode("mixed", k_factor=4)
     Dr, Ir = ix.search(q, 10)
     assert np.array_equal(Ir, Ip) and np.array_equal(Dr, Dp)
+    # k = 100: 100 candidates per query, the 64-query form's LDS layout on
+    # strided splits under a sampled bound (exact here, so equal as well)
+    ix.set_search_mode("plain")
+    Dp, Ip = ix.search(q, 100)
+    ix.set_search_mode("mixed")
+    Dm, Im = ix.search(q, 100)
+    assert np.array_equal(Im, Ip) and np.array_equal(Dm, Dp)
 
 
 def test_flat_mixed_rerank_on_float_data(gpu):
@@ -424,6 +431,12 @@ def test_ivfpq_mixed_mfma_coarse_large_nlist(gpu):
     overlap = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(Im.tolist(), Io.tolist())])
     assert overlap > 0.9
     assert np.allclose(np.sort(Dm, 1)[:, :5], np.sort(Do, 1)[:, :5], rtol=1e-3, atol=1.0)
+    # 100 probes: the coarse top-100 on strided splits of the centroids
+    gi.nprobe = 100
+    Dm, Im = gi.search(q, 10)
+    Do, Io = oi.search(q, 10, 100)
+    overlap = np.mean([len(set(a) & set(b)) / 10 for a, b in zip(Im.tolist(), Io.tolist())])
+    assert overlap > 0.9
     gi.set_search_mode("plain")
 
 
