@@ -441,13 +441,30 @@ struct WideSel {
   // take no branch while nothing passes (straight-line code the scheduler
   // can interleave with MFMAs); passing keys go to the queries' append
   // buffers, and a buffer near full is merged into its query's top kk and
-  // the bounds are refreshed.
+  // the bounds are refreshed. EX: an exclusion mask may be given; without
+  // one the column test is a single compare, with no branch per tile (int8
+  // and FP8 kinds at d = 768: 7-11% less time per search).
   template <int T0, int T1>
   __device__ __forceinline__ void tiles(const floatx4 (&accf)[16], const intx4 (&acci)[16], int64_t pb0,
                                         const uint2 *cv, int lim, int64_t b_begin, const uint8_t *excl,
                                         int64_t excl_len, const uint32_t *xn_u, const uint32_t *xs_u,
                                         float scale_sq, int d_true, uint64_t *top, uint64_t *buf, int *cnt,
                                         int kk) {
+    // the uint8 and BF16 kinds keep the general test: their kernels ran
+    // slower with the mask-free one (register allocation)
+    if (excl || KIND == U8_L2 || KIND == BF16_L2 || KIND == BF16_IP)
+      tiles_x<T0, T1, true>(accf, acci, pb0, cv, lim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true,
+                            top, buf, cnt, kk);
+    else
+      tiles_x<T0, T1, false>(accf, acci, pb0, cv, lim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true,
+                             top, buf, cnt, kk);
+  }
+  template <int T0, int T1, bool EX>
+  __device__ __forceinline__ void tiles_x(const floatx4 (&accf)[16], const intx4 (&acci)[16], int64_t pb0,
+                                          const uint2 *cv, int lim, int64_t b_begin, const uint8_t *excl,
+                                          int64_t excl_len, const uint32_t *xn_u, const uint32_t *xs_u,
+                                          float scale_sq, int d_true, uint64_t *top, uint64_t *buf, int *cnt,
+                                          int kk) {
     constexpr bool kColTerms = KIND == BF16_L2 || KIND == FP8_L2 || KIND == FP8_IP || KIND == U8_L2;
     constexpr int NT = T1 - T0;
     const int lane = threadIdx.x & 63, lrow = lane & 15, lgrp = lane >> 4;
@@ -458,7 +475,7 @@ struct WideSel {
     for (int u = 0; u < NT; u++) {
       const int t = T0 + u;
       const int64_t col = STR ? goff + (pb0 + t * 16 + lrow) * stride : pb0 + t * 16 + lrow;
-      const bool col_ok = t * 16 + lrow <= lim && !(excl && col < excl_len && excl[col] > 0);
+      const bool col_ok = t * 16 + lrow <= lim && !(EX && col < excl_len && excl[col] > 0);
       float xa = 0.0f, xb = 0.0f;
       int xcorr = 0;
       if (kColTerms) {
