@@ -470,6 +470,7 @@ struct WideSel {
     const int lane = threadIdx.x & 63, lrow = lane & 15, lgrp = lane >> 4;
     float dist[NT][4];
     bool pass[NT][4];
+    int xc[NT];  // U8_L2: the column's xn - 256 sx, exact in int32 (d <= 1264)
     bool any_chunk = false;
 #pragma unroll
     for (int u = 0; u < NT; u++) {
@@ -484,6 +485,7 @@ struct WideSel {
         xb = __uint_as_float(v.y);
         xcorr = (int)v.x;
       }
+      xc[u] = xcorr;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (KIND == I8_IP) {
@@ -515,8 +517,6 @@ struct WideSel {
       for (int j = 0; j < 4; j++) any |= pass[u][j];
       if (__ballot(any) == 0) continue;
       const int64_t col = STR ? goff + (pb0 + t * 16 + lrow) * stride : pb0 + t * 16 + lrow;
-      const int64_t cc =
-          pass[u][0] || pass[u][1] || pass[u][2] || pass[u][3] ? col : (STR ? goff + b_begin * stride : b_begin);
       const uint64_t gmask = 0xFFFFull << (16 * lgrp);
       const uint64_t below = (1ull << lane) - 1;
 #pragma unroll
@@ -524,9 +524,10 @@ struct WideSel {
         if (KIND == I8_IP) {
           dist[u][j] = __fmul_rn(-(float)acci[t][j], scale_sq);
         } else if (KIND == U8_L2) {
-          const int64_t sx = (int64_t)xs_u[cc] - 128 * (int64_t)d_true;
-          const int64_t dot = (int64_t)acci[t][j] + 128 * (qsx[j] + sx) + 16384 * (int64_t)d_true;
-          const int64_t v = qc[j] + (int64_t)xn_u[cc] - 2 * dot;
+          // qn + xn - 2 (acc + 128 (sq + sx) + 16384 d), from the staged
+          // column term xn - 256 sx (no reload of the column's norm and sum)
+          const int64_t v = qc[j] - 256 * qsx[j] - 32768 * (int64_t)d_true + (int64_t)xc[u] -
+                            2 * (int64_t)acci[t][j];
           dist[u][j] = __fmul_rn((float)(uint32_t)(v > 0 ? v : 0), scale_sq);
         }
         const uint64_t key = pack_key(dist[u][j], (uint32_t)col);
