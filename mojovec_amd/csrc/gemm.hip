@@ -442,17 +442,17 @@ struct WideSel {
   // can interleave with MFMAs); passing keys go to the queries' append
   // buffers, and a buffer near full is merged into its query's top kk and
   // the bounds are refreshed. EX: an exclusion mask may be given; without
-  // one the column test is a single compare, with no branch per tile (int8
-  // and FP8 kinds at d = 768: 7-11% less time per search).
+  // one the column test is a single compare, with no branch per tile (int8,
+  // uint8 and FP8 kinds at d = 768: 5-11% less time per search).
   template <int T0, int T1>
   __device__ __forceinline__ void tiles(const floatx4 (&accf)[16], const intx4 (&acci)[16], int64_t pb0,
                                         const uint2 *cv, int lim, int64_t b_begin, const uint8_t *excl,
                                         int64_t excl_len, const uint32_t *xn_u, const uint32_t *xs_u,
                                         float scale_sq, int d_true, uint64_t *top, uint64_t *buf, int *cnt,
                                         int kk) {
-    // the uint8 and BF16 kinds keep the general test: their kernels ran
-    // slower with the mask-free one (register allocation)
-    if (excl || KIND == U8_L2 || KIND == BF16_L2 || KIND == BF16_IP)
+    // the BF16 kinds keep the general test: their kernels ran slower with the
+    // mask-free one (register allocation)
+    if (excl || KIND == BF16_L2 || KIND == BF16_IP)
       tiles_x<T0, T1, true>(accf, acci, pb0, cv, lim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true,
                             top, buf, cnt, kk);
     else
