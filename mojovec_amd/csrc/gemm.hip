@@ -361,7 +361,7 @@ __device__ __forceinline__ void wide_flush(uint64_t *top, uint64_t *buf, int *cn
 // integer kinds test candidates with an exact integer prefilter derived from
 // the running kk-th distance (a superset: the rounding of the FP32 distance
 // is covered by a margin), then the exact key on the rare pass.
-template <int KIND, bool STR>
+template <int KIND, bool STR, bool REG>
 struct WideSel {
   float thr_f[4], qa[4], qb[4];
   uint64_t thr_k[4];  // each query's kk-th key (the sentinel while fewer): changes at merges only
@@ -443,16 +443,17 @@ struct WideSel {
   // buffers, and a buffer near full is merged into its query's top kk and
   // the bounds are refreshed. EX: an exclusion mask may be given; without
   // one the column test is a single compare, with no branch per tile (int8,
-  // uint8 and FP8 kinds at d = 768: 5-11% less time per search).
+  // uint8 and FP8 kinds at d = 768, BF16 at d = 128: 5-11% less time per
+  // search). REG: the register-tile kernel's selection.
   template <int T0, int T1>
   __device__ __forceinline__ void tiles(const floatx4 (&accf)[16], const intx4 (&acci)[16], int64_t pb0,
                                         const uint2 *cv, int lim, int64_t b_begin, const uint8_t *excl,
                                         int64_t excl_len, const uint32_t *xn_u, const uint32_t *xs_u,
                                         float scale_sq, int d_true, uint64_t *top, uint64_t *buf, int *cnt,
                                         int kk) {
-    // the BF16 kinds keep the general test: their kernels ran slower with the
-    // mask-free one (register allocation)
-    if (excl || KIND == BF16_L2 || KIND == BF16_IP)
+    // the BF16 register-tile kernel keeps the general test: it ran slower
+    // with the mask-free one (register allocation)
+    if (excl || (REG && (KIND == BF16_L2 || KIND == BF16_IP)))
       tiles_x<T0, T1, true>(accf, acci, pb0, cv, lim, b_begin, excl, excl_len, xn_u, xs_u, scale_sq, d_true,
                             top, buf, cnt, kk);
     else
@@ -643,7 +644,7 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_kernel(
   for (int i = lane; i < 16 * kk; i += 64) top[i] = kSentinel64;
   if (lane < 16) cnt[lane] = 0;
 
-  WideSel<KIND, STR> sel;
+  WideSel<KIND, STR, false> sel;
   sel.init(q0, wq, qcount, lgrp, qnorm, qinv, qn_u, qs_u, d_true);
   sel.goff = sr.goff;
   sel.stride = P;
@@ -909,7 +910,7 @@ __global__ __launch_bounds__(kNT, 1) void gemm_wide_reg_kernel(
   }
   for (int i = lane; i < 16 * kk; i += 64) top[i] = kSentinel64;
   if (lane < 16) cnt[lane] = 0;
-  WideSel<KIND, STR> sel;
+  WideSel<KIND, STR, true> sel;
   sel.init(q0, wq, qcount, lgrp, qnorm, qinv, qn_u, qs_u, d_true);
   sel.goff = sr.goff;
   sel.stride = P;
